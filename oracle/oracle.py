@@ -1,0 +1,198 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, as the checker.  It loads oracle/_build/liboracle.so (parity build,
+no FMA contraction) or liboracle_omp.so (OpenMP build with the reference's
+release flags, used for CPU timing).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+OK, E_ZERO_NODE, E_ZERO_PATTERN, E_ARG, E_NOT_CONVERGED, E_NOMEM, E_CAPACITY = range(7)
+ZERO_THROW, ZERO_ALLOW = 0, 1
+RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST = 0, 1, 2
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"oracle {what} failed with code {code}")
+        self.code = code
+
+
+class Mesh(C.Structure):
+    _fields_ = [("orig", C.c_double * 3), ("e", (C.c_double * 3) * 3), ("n", C.c_int32 * 3)]
+
+
+class Local(C.Structure):
+    _fields_ = [("nfacet", C.c_int32), ("nvert", C.c_int32), ("area", C.c_double),
+                ("mass", C.c_double * 16), ("lb", C.c_double * 16), ("stab", C.c_double * 16),
+                ("rhs", C.c_double * 4), ("hat_sq", C.c_double * 4)]
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+_libs = {}
+
+
+def lib(kind="parity"):
+    name = "liboracle.so" if kind == "parity" else "liboracle_omp.so"
+    if name in _libs:
+        return _libs[name]
+    path = os.path.join(HERE, "_build", name)
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+    L = C.CDLL(path)
+    D, I32, I64 = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    M = C.POINTER(Mesh)
+    L.orc_classify.argtypes = [M, D, C.c_int, C.c_int64, I64, I64]
+    L.orc_number_p1if.argtypes = [M, D, C.c_double, C.c_int, I32, I64]
+    L.orc_tet_local.argtypes = [M, D, C.c_int64, D, C.c_int, D, C.c_int, C.POINTER(Local)]
+    L.orc_assemble_p1.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64,
+                                  I64, I32, D, D, D, D]
+    L.orc_surface_area.argtypes = [M, D, C.c_int, D]
+    L.orc_spmv.argtypes = [C.c_int64, I64, I32, D, D, D]
+    L.orc_kahan_dot.argtypes = [C.c_int64, D, D]
+    L.orc_kahan_dot.restype = C.c_double
+    L.orc_pcg_jacobi.argtypes = [C.c_int64, I64, I32, D, D, D, I32, D, C.c_int32]
+    L.orc_gmres_jacobi.argtypes = [C.c_int64, I64, I32, D, D, D, C.c_int32, I32, D, C.c_int32,
+                                   C.c_int32, C.c_int32]
+    L.orc_lattice_tables.argtypes = [I32, I32]
+    L.orc_refpatch_table.argtypes = [I32, I32, I32]
+    _libs[name] = L
+    return L
+
+
+def make_mesh(orig, e, n):
+    m = Mesh()
+    for c in range(3):
+        m.orig[c] = float(orig[c])
+        m.n[c] = int(n[c])
+        for k in range(3):
+            m.e[k][c] = float(e[k][c])
+    return m
+
+
+def _check(code, what):
+    if code != OK:
+        raise OracleError(code, what)
+
+
+def lattice_tables():
+    b = np.zeros((10, 4), np.int32)
+    t = np.zeros((8, 4), np.int32)
+    lib().orc_lattice_tables(_ptr(b, C.c_int32), _ptr(t, C.c_int32))
+    return b, t
+
+
+def refpatch_table():
+    size = np.zeros(81, np.int32)
+    tri = np.zeros((81, 2, 3), np.int32)
+    bnd = np.zeros(81, np.int32)
+    lib().orc_refpatch_table(_ptr(size, C.c_int32), _ptr(tri, C.c_int32), _ptr(bnd, C.c_int32))
+    return size, tri, bnd
+
+
+def classify(mesh, phi, zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    n = C.c_int64()
+    _check(lib().orc_classify(C.byref(mesh), _ptr(phi, C.c_double), zero_policy, 0, C.byref(n), None),
+           "classify")
+    out = np.zeros(max(n.value, 1), np.int64)
+    _check(lib().orc_classify(C.byref(mesh), _ptr(phi, C.c_double), zero_policy, n.value, C.byref(n),
+                              _ptr(out, C.c_int64)), "classify")
+    return out[: n.value]
+
+
+def number_p1if(mesh, phi, omit_bound=-1.0, zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    nv = (mesh.n[0] + 1) * (mesh.n[1] + 1) * (mesh.n[2] + 1)
+    dof = np.empty(nv, np.int32)
+    nd = C.c_int64()
+    _check(lib().orc_number_p1if(C.byref(mesh), _ptr(phi, C.c_double), omit_bound, zero_policy,
+                                 _ptr(dof, C.c_int32), C.byref(nd)), "number_p1if")
+    return dof, nd.value
+
+
+def tet_local(mesh, phi, tet, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+              zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    out = Local()
+    _check(lib().orc_tet_local(C.byref(mesh), _ptr(phi, C.c_double), int(tet), _ptr(co, C.c_double),
+                               rhs_kind, _ptr(rp, C.c_double), zero_policy, C.byref(out)), "tet_local")
+    return out
+
+
+def assemble_p1(mesh, phi, dof_of_vertex, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE,
+                rhs_param=None, zero_policy=ZERO_THROW, kind="parity"):
+    """Returns dict(rowptr, colind, mass, lb, stab, rhs) -- CSR with int64 row pointers."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    L = lib(kind)
+    ncut = classify(mesh, phi, zero_policy).size
+    cap = 16 * max(ncut, 1)
+    rowptr = np.zeros(ndof + 1, np.int64)
+    col = np.zeros(cap, np.int32)
+    vm, vl, vs = (np.zeros(cap) for _ in range(3))
+    rhs = np.zeros(max(ndof, 1))
+    nnz = C.c_int64()
+    _check(L.orc_assemble_p1(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
+                             _ptr(co, C.c_double), rhs_kind, _ptr(rp, C.c_double), zero_policy, cap,
+                             C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
+                             _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(vs, C.c_double),
+                             _ptr(rhs, C.c_double)), "assemble_p1")
+    k = nnz.value
+    return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
+                stab=vs[:k].copy(), rhs=rhs[:ndof].copy())
+
+
+def surface_area(mesh, phi, zero_policy=ZERO_ALLOW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    a = C.c_double()
+    _check(lib().orc_surface_area(C.byref(mesh), _ptr(phi, C.c_double), zero_policy, C.byref(a)),
+           "surface_area")
+    return a.value
+
+
+def spmv(rowptr, col, val, x):
+    y = np.zeros(rowptr.size - 1)
+    lib().orc_spmv(rowptr.size - 1, _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                   _ptr(np.ascontiguousarray(x, np.float64), C.c_double), _ptr(y, C.c_double))
+    return y
+
+
+def kahan_dot(x, y):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().orc_kahan_dot(x.size, _ptr(x, C.c_double), _ptr(y, C.c_double))
+
+
+def pcg_jacobi(rowptr, col, val, b, x0=None, max_iter=10000, tol=1e-10, rel=True, kind="parity"):
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    it = C.c_int32(max_iter)
+    t = C.c_double(tol)
+    code = lib(kind).orc_pcg_jacobi(n, _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                    _ptr(np.ascontiguousarray(b, np.float64), C.c_double), _ptr(x, C.c_double),
+                                    C.byref(it), C.byref(t), int(rel))
+    return x, it.value, t.value, code == OK
+
+
+def gmres_jacobi(rowptr, col, val, b, x0=None, restart=50, max_iter=10000, tol=1e-10, rel=True,
+                 left_pc=False, use_pc=True):
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    it = C.c_int32(max_iter)
+    t = C.c_double(tol)
+    code = lib().orc_gmres_jacobi(n, _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                  _ptr(np.ascontiguousarray(b, np.float64), C.c_double), _ptr(x, C.c_double),
+                                  restart, C.byref(it), C.byref(t), int(rel), int(left_pc), int(use_pc))
+    return x, it.value, t.value, code == OK

@@ -1,0 +1,1104 @@
+/*
+ * tracefem_oracle.c -- CPU oracle (TEST INFRASTRUCTURE ONLY) for the P1
+ * trace-FEM assembly over level-set-cut Kuhn tetrahedra.
+ *
+ * Restates, in plain C and in the reference's own operation order, the DROPS
+ * routines cited at each function.  It deliberately keeps the reference's
+ * data structures (explicit vertex lists, the edge-merge map of
+ * MergeCutPolicyCL, per-row builder maps) so that the GPU library, which is
+ * designed differently, is checked against an independent restatement.
+ *
+ * Compile with -ffp-contract=off for parity runs (the reference evaluates
+ * every product and sum separately in the order used here).
+ */
+#include "tracefem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Tables                                                                    */
+/* ------------------------------------------------------------------------ */
+
+/* Vertices of the 6 edges of the reference tet, geom/topo.cpp:45-46. */
+static const int VOE[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
+
+/* Corner offsets (d1,d2,d3) of the 6 Kuhn tets of a brick cell, in the vertex
+ * order of geom/builder.cpp:155-175. */
+static const int KUHN[6][4][3] = {
+    {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}},
+    {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}},
+    {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}}};
+
+static int LAT_B2[10][4];    /* lattice-2 vertices, barycentric x 2 */
+static double LAT_B[10][4];  /* lattice-2 vertices, barycentric */
+static int LAT_T[8][4];      /* lattice-2 sub-tets */
+static int LAT_OF_P2[10];    /* lattice vertex of P2 dof (vertices 0-3, edges 0-5) */
+
+static int RP_SIZE[81], RP_TRI[81][2][3], RP_BND[81];
+
+static double Q2_N[4][3], Q2_W[4];     /* Quad2_2DDataCL, num/discretize.cpp:234-238 */
+static double Q5_N[7][3], Q5_W[7];     /* Quad5_2DDataCL, num/discretize.cpp:263-296 */
+static double V5_N[15][4];             /* Quad5DataCL nodes, num/discretize.cpp:181-203 */
+static double V5_C1, V5_C2;            /* Quad5CL::quad weights, num/discretize.tpp:754-762 */
+
+static int g_inited = 0;
+
+static int lattice_vertex_index(int i, int j, int k)
+{
+    /* PrincipalLatticeCL::vertex_index, geom/principallattice.h:62-64 */
+    return (i > 0 ? ((i + 2) * (i + 1) * i) / 6 : 0) + (j > 0 ? ((j + 1) * j) / 2 : 0) + k;
+}
+
+static int in_ref_tetra(const int v[3], int n) { return n >= v[0] && v[0] >= v[1] && v[1] >= v[2]; }
+
+static void init_tables(void)
+{
+    if (g_inited)
+        return;
+    /* PrincipalLatticeCL ctor, geom/principallattice.cpp:33-44 (n = 2) */
+    int idx = 0;
+    for (int x = 0; x <= 2; ++x)
+        for (int y = 0; y <= x; ++y)
+            for (int z = 0; z <= y; ++z) {
+                LAT_B2[idx][0] = 2 - x;
+                LAT_B2[idx][1] = x - y;
+                LAT_B2[idx][2] = y - z;
+                LAT_B2[idx][3] = z;
+                for (int c = 0; c < 4; ++c)
+                    LAT_B[idx][c] = (double)LAT_B2[idx][c] / 2.0;
+                ++idx;
+            }
+    /* create_tetras(0, 2), geom/principallattice.cpp:46-80 */
+    static const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int nt = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k <= j; ++k) {
+                int t[4][3];
+                t[0][0] = i; t[0][1] = j; t[0][2] = k;
+                for (int l = 0; l < 6; ++l) {
+                    const int *p = perm[l];
+                    memcpy(t[1], t[0], sizeof t[0]);
+                    ++t[1][p[0]];
+                    if (!in_ref_tetra(t[1], 2)) continue;
+                    memcpy(t[2], t[1], sizeof t[0]);
+                    ++t[2][p[1]];
+                    if (!in_ref_tetra(t[2], 2)) continue;
+                    memcpy(t[3], t[2], sizeof t[0]);
+                    ++t[3][p[2]];
+                    if (!in_ref_tetra(t[3], 2)) continue;
+                    for (int q = 0; q < 4; ++q)
+                        LAT_T[nt][q] = lattice_vertex_index(t[q][0], t[q][1], t[q][2]);
+                    ++nt;
+                }
+            }
+    /* P2 dof -> lattice vertex (p2_dof_on_lattice_2, geom/principallattice.cpp:103-106),
+     * derived from the barycentric coordinates */
+    for (int l = 0; l < 10; ++l) {
+        int nz = 0, a = -1, b = -1;
+        for (int c = 0; c < 4; ++c)
+            if (LAT_B2[l][c]) {
+                if (a < 0) a = c; else b = c;
+                ++nz;
+            }
+        if (nz == 1)
+            LAT_OF_P2[a] = l;
+        else
+            for (int e = 0; e < 6; ++e)
+                if (VOE[e][0] == a && VOE[e][1] == b)
+                    LAT_OF_P2[4 + e] = l;
+    }
+    /* RefPatchBuilderCL<3>::StaticInit/assign, geom/reftetracut.cpp:40-60,
+     * with SignTraitsCL<3>::assign/compute_cuts, geom/signtraits.h:112-139 */
+    for (int i = 0; i < 81; ++i) {
+        RP_SIZE[i] = 0;
+        RP_BND[i] = 0;
+    }
+    int ls[4];
+    for (ls[0] = -1; ls[0] < 2; ++ls[0])
+        for (ls[1] = -1; ls[1] < 2; ++ls[1])
+            for (ls[2] = -1; ls[2] < 2; ++ls[2])
+                for (ls[3] = -1; ls[3] < 2; ++ls[3]) {
+                    if (!ls[0] && !ls[1] && !ls[2] && !ls[3])
+                        continue;
+                    const int pid = ls[0] + 3 * ls[1] + 9 * ls[2] + 27 * ls[3] + 40;
+                    int nrv = 0, nr = 0, rep[4] = {0, 0, 0, 0};
+                    const int sum = ls[0] + ls[1] + ls[2] + ls[3];
+                    if (abs(sum) != 4) {
+                        for (int v = 0; v < 4; ++v)
+                            if (ls[v] == 0) rep[nrv++] = v;
+                        nr = nrv;
+                        for (int e = 0; e < 6; ++e)
+                            if (ls[VOE[e][0]] * ls[VOE[e][1]] == -1) rep[nr++] = e + 4;
+                    }
+                    const int ntri = nr >= 3 ? nr - 2 : 0;
+                    RP_SIZE[pid] = ntri;
+                    for (int s = 0; s < ntri; ++s)
+                        for (int j = 0; j < 3; ++j)
+                            RP_TRI[pid][s][j] = rep[s + j];
+                    RP_BND[pid] = (nrv == 3);
+                }
+    /* Quad2_2DDataCL */
+    memset(Q2_N, 0, sizeof Q2_N);
+    Q2_N[0][0] = 1.; Q2_N[1][1] = 1.; Q2_N[2][2] = 1.;
+    Q2_N[3][0] = Q2_N[3][1] = Q2_N[3][2] = 1. / 3.;
+    Q2_W[0] = Q2_W[1] = Q2_W[2] = 1. / 24.;
+    Q2_W[3] = 0.375;
+    /* Quad5_2DDataCL */
+    const double s15 = sqrt(15.0);
+    Q5_N[0][0] = Q5_N[0][1] = Q5_N[0][2] = 1. / 3.;
+    {
+        const double A1 = (6.0 - s15) / 21.0, B1 = (9.0 + 2.0 * s15) / 21.0;
+        const double A2 = (6.0 + s15) / 21.0, B2 = (9.0 - 2.0 * s15) / 21.0;
+        const double n1[3][3] = {{A1, A1, B1}, {A1, B1, A1}, {B1, A1, A1}};
+        const double n2[3][3] = {{A2, A2, B2}, {A2, B2, A2}, {B2, A2, A2}};
+        memcpy(Q5_N[1], n1, sizeof n1);
+        memcpy(Q5_N[4], n2, sizeof n2);
+    }
+    Q5_W[0] = 9. / 80.;
+    Q5_W[1] = Q5_W[2] = Q5_W[3] = (155. - s15) / 2400.;
+    Q5_W[4] = Q5_W[5] = Q5_W[6] = (155. + s15) / 2400.;
+    /* Quad5DataCL nodes */
+    for (int c = 0; c < 4; ++c) V5_N[0][c] = 0.25;
+    {
+        const double A1 = (7.0 - s15) / 34.0, B1 = (13.0 + 3.0 * s15) / 34.0;
+        const double A2 = (7.0 + s15) / 34.0, B2 = (13.0 - 3.0 * s15) / 34.0;
+        const double A3 = (10.0 - 2.0 * s15) / 40.0, B3 = (10.0 + 2.0 * s15) / 40.0;
+        for (int q = 0; q < 4; ++q)
+            for (int c = 0; c < 4; ++c) {
+                V5_N[1 + q][c] = (c == 3 - q) ? B1 : A1;
+                V5_N[5 + q][c] = (c == 3 - q) ? B2 : A2;
+            }
+        const double n3[6][4] = {{A3, A3, B3, B3}, {A3, B3, A3, B3}, {A3, B3, B3, A3},
+                                 {B3, A3, A3, B3}, {B3, A3, B3, A3}, {B3, B3, A3, A3}};
+        memcpy(V5_N[9], n3, sizeof n3);
+    }
+    V5_C1 = (2665.0 + 14.0 * s15) / 226800.0;
+    V5_C2 = (2665.0 - 14.0 * s15) / 226800.0;
+    g_inited = 1;
+}
+
+void orc_lattice_tables(int32_t bary2[10][4], int32_t tets[8][4])
+{
+    init_tables();
+    for (int i = 0; i < 10; ++i)
+        for (int c = 0; c < 4; ++c) bary2[i][c] = LAT_B2[i][c];
+    for (int i = 0; i < 8; ++i)
+        for (int c = 0; c < 4; ++c) tets[i][c] = LAT_T[i][c];
+}
+
+void orc_refpatch_table(int32_t size[81], int32_t tri[81][2][3], int32_t bnd[81])
+{
+    init_tables();
+    for (int i = 0; i < 81; ++i) {
+        size[i] = RP_SIZE[i];
+        bnd[i] = RP_BND[i];
+        for (int s = 0; s < 2; ++s)
+            for (int j = 0; j < 3; ++j) tri[i][s][j] = s < RP_SIZE[i] ? RP_TRI[i][s][j] : -1;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh                                                                      */
+/* ------------------------------------------------------------------------ */
+
+void orc_vertex_coord(const orc_mesh_t *m, int64_t i1, int64_t i2, int64_t i3, double x[3])
+{
+    /* geom/builder.cpp:108-120: off_k = 1/n_k * e_k; x = orig + i1 off1 + i2 off2 + i3 off3 */
+    for (int c = 0; c < 3; ++c) {
+        const double o1 = 1.0 / (double)m->n[0] * m->e[0][c];
+        const double o2 = 1.0 / (double)m->n[1] * m->e[1][c];
+        const double o3 = 1.0 / (double)m->n[2] * m->e[2][c];
+        x[c] = ((m->orig[c] + (double)i1 * o1) + (double)i2 * o2) + (double)i3 * o3;
+    }
+}
+
+void orc_halfgrid_coord(const orc_mesh_t *m, int64_t j1, int64_t j2, int64_t j3, double x[3])
+{
+    /* P2 node = vertex or barycentre of a Kuhn edge (BaryCenter, misc/container.h:395-400) */
+    double a[3], b[3];
+    orc_vertex_coord(m, j1 / 2, j2 / 2, j3 / 2, a);
+    if (!(j1 & 1) && !(j2 & 1) && !(j3 & 1)) {
+        memcpy(x, a, sizeof a);
+        return;
+    }
+    orc_vertex_coord(m, (j1 + 1) / 2, (j2 + 1) / 2, (j3 + 1) / 2, b);
+    for (int c = 0; c < 3; ++c) x[c] = .5 * (a[c] + b[c]);
+}
+
+typedef struct {
+    int64_t i[3];   /* cell index */
+    int k;          /* Kuhn tet type */
+    double X[4][3]; /* vertex coordinates */
+    int64_t vid[4]; /* global vertex ids */
+    double ls[10];  /* level set on lattice-2 vertices */
+} tet_t;
+
+static void load_tet(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T)
+{
+    const int64_t n1 = m->n[0], n2 = m->n[1];
+    const int64_t cell = t / 6;
+    T->k = (int)(t % 6);
+    T->i[0] = cell % n1;
+    T->i[1] = (cell / n1) % n2;
+    T->i[2] = cell / (n1 * n2);
+    for (int q = 0; q < 4; ++q) {
+        const int *d = KUHN[T->k][q];
+        const int64_t v1 = T->i[0] + d[0], v2 = T->i[1] + d[1], v3 = T->i[2] + d[2];
+        orc_vertex_coord(m, v1, v2, v3, T->X[q]);
+        T->vid[q] = (v3 * (n2 + 1) + v2) * (n1 + 1) + v1;
+    }
+    const int64_t h1 = 2 * n1 + 1, h2 = 2 * n2 + 1;
+    for (int l = 0; l < 10; ++l) {
+        int64_t j[3];
+        for (int c = 0; c < 3; ++c) {
+            j[c] = 2 * T->i[c];
+            for (int q = 0; q < 4; ++q) j[c] += LAT_B2[l][q] * KUHN[T->k][q][c];
+        }
+        /* evaluate_on_vertexes(LocalP2CL, lattice-2) = P2 nodal values, num/lattice-eval.h:41-62 */
+        T->ls[l] = phi[(j[2] * h2 + j[1]) * h1 + j[0]];
+    }
+}
+
+static int sgn(double d) { return d > 0. ? 1 : (d < 0. ? -1 : 0); } /* misc/utils.h:810-813 */
+
+/* equalSigns, geom/subtriangulation.tpp:32-36 */
+static int equal_signs(const double *f, int n)
+{
+    int sum = 0;
+    for (int i = 0; i < n; ++i) sum += sgn(f[i]);
+    return abs(sum) == n;
+}
+
+static int has_zero(const double *f, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (f[i] == 0.) return 1;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Interface patch (SPatchCL<3>::make_patch<MergeCutPolicyCL>)               */
+/* ------------------------------------------------------------------------ */
+
+#define MAXV 40
+#define MAXF 16
+
+typedef struct {
+    int nf, nv;
+    int facet[MAXF][3];
+    int bnd[MAXF];
+    double vb[MAXV][4];
+    double vw[MAXV][3];
+    double absdet[MAXF];
+    double normal[MAXF][3];
+    int deficient[MAXF];
+} patch_t;
+
+/* geom/subtriangulation.tpp:194-231 (make_patch), :149-177 (make_sub_facet),
+ * geom/subtriangulation.h:514-527 (MergeCutPolicyCL), misc/container.h:403-410 (ConvexComb) */
+static int make_patch(const double ls[10], patch_t *p)
+{
+    int ekey[MAXV][2];
+    double cutb[MAXV][4];
+    int ncut = 0;
+    int copied[10];
+    double zerob[10][4];
+    int nzero = 0;
+    int zf[3 * MAXF], zj[3 * MAXF], nzu = 0;
+    for (int i = 0; i < 10; ++i) copied[i] = -1;
+    p->nf = 0;
+    for (int lt = 0; lt < 8; ++lt) {
+        const int *v = LAT_T[lt];
+        double lls[4];
+        int s[4];
+        for (int q = 0; q < 4; ++q) {
+            lls[q] = ls[v[q]];
+            s[q] = sgn(lls[q]);
+        }
+        if (!s[0] && !s[1] && !s[2] && !s[3])
+            return ORC_E_ZERO_PATTERN; /* RefPatchCL::instance, geom/reftetracut.h:148-155 */
+        const int pid = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;
+        for (int tr = 0; tr < RP_SIZE[pid]; ++tr) {
+            int out[3];
+            for (int j = 0; j < 3; ++j) {
+                const int rep = RP_TRI[pid][tr][j];
+                if (rep < 4) {
+                    const int lv = v[rep];
+                    if (copied[lv] < 0) {
+                        copied[lv] = nzero;
+                        memcpy(zerob[nzero], LAT_B[lv], sizeof zerob[0]);
+                        ++nzero;
+                    }
+                    out[j] = copied[lv];
+                    zf[nzu] = p->nf;
+                    zj[nzu] = j;
+                    ++nzu;
+                } else {
+                    const int e = rep - 4, v0 = VOE[e][0], v1 = VOE[e][1];
+                    const int a = v[v0], b = v[v1];
+                    const int ka = a < b ? a : b, kb = a < b ? b : a;
+                    int found = -1;
+                    for (int c = 0; c < ncut; ++c)
+                        if (ekey[c][0] == ka && ekey[c][1] == kb) { found = c; break; }
+                    if (found < 0) {
+                        const double alpha = lls[v0] / (lls[v0] - lls[v1]);
+                        for (int c = 0; c < 4; ++c)
+                            cutb[ncut][c] = (1.0 - alpha) * LAT_B[a][c] + alpha * LAT_B[b][c];
+                        ekey[ncut][0] = ka;
+                        ekey[ncut][1] = kb;
+                        found = ncut++;
+                    }
+                    out[j] = found;
+                }
+            }
+            memcpy(p->facet[p->nf], out, sizeof out);
+            p->bnd[p->nf] = RP_BND[pid];
+            ++p->nf;
+        }
+    }
+    for (int u = 0; u < nzu; ++u) p->facet[zf[u]][zj[u]] += ncut;
+    p->nv = ncut + nzero;
+    memcpy(p->vb, cutb, sizeof(double) * 4 * ncut);
+    memcpy(p->vb + ncut, zerob, sizeof(double) * 4 * nzero);
+    return ORC_OK;
+}
+
+/* Bary2WorldCoordCL: SMatrixCL<3,4> * b, geom/simplex.h:867-881, misc/container.h:966-973 */
+static void bary_to_world(const double X[4][3], const double b[4], double w[3])
+{
+    for (int r = 0; r < 3; ++r) {
+        double s = 0.0;
+        for (int i = 0; i < 4; ++i) s += X[i][r] * b[i];
+        w[r] = s;
+    }
+}
+
+/* SPatchCL::compute_normals, geom/subtriangulation.cpp:140-170 with
+ * QRDecompCL<3,2>::prepare_solve / apply_Q / Determinant_R, misc/container.h:1210-1290 */
+static void patch_geometry(const double X[4][3], patch_t *p)
+{
+    for (int v = 0; v < p->nv; ++v) bary_to_world(X, p->vb[v], p->vw[v]);
+    for (int f = 0; f < p->nf; ++f) {
+        double a[3][2], d[2], beta[2];
+        const double *w0 = p->vw[p->facet[f][0]];
+        for (int i = 1; i < 3; ++i) {
+            const double *wi = p->vw[p->facet[f][i]];
+            for (int r = 0; r < 3; ++r) a[r][i - 1] = wi[r] - w0[r];
+        }
+        int deficient = 0;
+        for (int j = 0; j < 2 && !deficient; ++j) {
+            double sigma = 0.;
+            for (int i = j; i < 3; ++i) sigma += a[i][j] * a[i][j];
+            if (sigma == 0.) { deficient = 1; break; }
+            d[j] = (a[j][j] < 0 ? 1. : -1.) * sqrt(sigma);
+            beta[j] = 1. / (d[j] * a[j][j] - sigma);
+            a[j][j] -= d[j];
+            for (int k = j + 1; k < 2; ++k) {
+                double sp = 0.;
+                for (int i = j; i < 3; ++i) sp += a[i][j] * a[i][k];
+                sp *= beta[j];
+                for (int i = j; i < 3; ++i) a[i][k] += a[i][j] * sp;
+            }
+        }
+        p->deficient[f] = deficient;
+        if (deficient) {
+            p->absdet[f] = 0.;
+            p->normal[f][0] = p->normal[f][1] = p->normal[f][2] = 0.;
+            continue;
+        }
+        double det = 1.0;
+        det *= d[0];
+        det *= d[1];
+        p->absdet[f] = fabs(det);
+        double t[3] = {0., 0., 1.};
+        for (int j = 1; j >= 0; --j) {
+            double sp = 0.;
+            for (int i = j; i < 3; ++i) sp += a[i][j] * t[i];
+            sp *= beta[j];
+            for (int i = j; i < 3; ++i) t[i] += a[i][j] * sp;
+        }
+        memcpy(p->normal[f], t, sizeof t);
+    }
+}
+
+/* make_CompositeQuadDomainCodim1<QuadData, Codim1Absdet, 3>, num/quadrature.tpp:265-312:
+ * node = sum_j vertex(facet[j]) * Node[i][j]; weight = (0.5 if boundary facet) * absdet * W[i] */
+static int composite_nodes(const patch_t *p, int nn, const double N[][3], const double *W,
+                           double (*node)[4], double *weight)
+{
+    int k = 0;
+    for (int f = 0; f < p->nf; ++f) {
+        const double *v0 = p->vb[p->facet[f][0]], *v1 = p->vb[p->facet[f][1]], *v2 = p->vb[p->facet[f][2]];
+        double a = p->bnd[f] ? 0.5 : 1.;
+        a *= p->absdet[f];
+        for (int i = 0; i < nn; ++i, ++k) {
+            for (int c = 0; c < 4; ++c) {
+                double s = v0[c] * N[i][0];
+                s += v1[c] * N[i][1];
+                s += v2[c] * N[i][2];
+                node[k][c] = s;
+            }
+            weight[k] = a * W[i];
+        }
+    }
+    return k;
+}
+
+/* P1DiscCL::GetGradients(Point3DCL[4], double&, const TetraCL&), num/discretize.h:1724-1746 */
+static double p1_gradients(const double X[4][3], double H[4][3])
+{
+    double M[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[j][i] = X[i + 1][j] - X[0][j];
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1])
+                     - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0])
+                     + M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    H[1][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    H[1][1] = (M[2][1] * M[0][2] - M[0][1] * M[2][2]) / det;
+    H[1][2] = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) / det;
+    H[2][0] = (M[2][0] * M[1][2] - M[1][0] * M[2][2]) / det;
+    H[2][1] = (M[0][0] * M[2][2] - M[2][0] * M[0][2]) / det;
+    H[2][2] = (M[1][0] * M[0][2] - M[0][0] * M[1][2]) / det;
+    H[3][0] = (M[1][0] * M[2][1] - M[2][0] * M[1][1]) / det;
+    H[3][1] = (M[2][0] * M[0][1] - M[0][0] * M[2][1]) / det;
+    H[3][2] = (M[0][0] * M[1][1] - M[1][0] * M[0][1]) / det;
+    for (int c = 0; c < 3; ++c) H[0][c] = ((-H[1][c]) - H[2][c]) - H[3][c];
+    return det;
+}
+
+/* TetraCL::GetVolume, geom/simplex.cpp:522-531 */
+static double tet_volume(const double X[4][3])
+{
+    double v[3][3];
+    for (int i = 1; i < 4; ++i)
+        for (int c = 0; c < 3; ++c) v[i - 1][c] = X[i][c] - X[0][c];
+    return fabs(v[0][0] * (v[1][1] * v[2][2] - v[1][2] * v[2][1])
+              - v[0][1] * (v[1][0] * v[2][2] - v[1][2] * v[2][0])
+              + v[0][2] * (v[1][0] * v[2][1] - v[1][1] * v[2][0])) / 6.0;
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* FE_P2CL::H0..H9 at barycentric p, num/fe.h:287-296, summed as in
+ * LinearCombinationCL::do_it, num/fe.h:44-53 */
+static double p2_eval(const double c[10], const double p[4])
+{
+    const double h[10] = {p[0] * (2. * p[0] - 1.), p[1] * (2. * p[1] - 1.), p[2] * (2. * p[2] - 1.),
+                          p[3] * (2. * p[3] - 1.), 4. * p[0] * p[1],         4. * p[0] * p[2],
+                          4. * p[1] * p[2],         4. * p[0] * p[3],         4. * p[1] * p[3],
+                          4. * p[2] * p[3]};
+    double s = h[0] * c[0];
+    for (int d = 1; d < 10; ++d) s += h[d] * c[d];
+    return s;
+}
+
+/* Normalised gradient of the P2 level set at barycentric point b.
+ * grad phi_h = sum_k G_k grad(lambda_k) with G_k = v_k (4 b_k - 1) + 4 sum_{e=(k,l)} v_e b_l
+ * (the normal used by LocalAssembler::build3DNormal, surfactant/ifacetransp.h:1073-1076). */
+static void p2_normal(const double v[10], const double b[4], const double G1[4][3], double n[3])
+{
+    double G[4];
+    for (int k = 0; k < 4; ++k) G[k] = v[k] * (4.0 * b[k] - 1.0);
+    for (int e = 0; e < 6; ++e) {
+        const int a = VOE[e][0], c = VOE[e][1];
+        G[a] += 4.0 * (v[4 + e] * b[c]);
+        G[c] += 4.0 * (v[4 + e] * b[a]);
+    }
+    double g[3];
+    for (int d = 0; d < 3; ++d) g[d] = ((G[0] * G1[0][d] + G[1] * G1[1][d]) + G[2] * G1[2][d]) + G[3] * G1[3][d];
+    const double nrm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    for (int d = 0; d < 3; ++d) n[d] = g[d] / nrm;
+}
+
+static double rhs_fun(int kind, const double *param, const double x[3])
+{
+    switch (kind) {
+    case ORC_RHS_3XY_MINUS_X3: return 3.0 * x[0] * x[1] - x[0] * x[0] * x[0];
+    case ORC_RHS_CONST: return param ? param[0] : 0.;
+    default: return 0.;
+    }
+}
+
+/* All per-tet local quantities of one cut tet with a non-empty patch. */
+static void local_quantities(const tet_t *T, const patch_t *p, const double coef[3], int rhs_kind,
+                             const double *rhs_param, orc_local_t *o)
+{
+    double node[MAXF * 7][4], w[MAXF * 7];
+    o->nfacet = p->nf;
+    o->nvert = p->nv;
+    /* ---- LocalInterfaceMassP1CL::setup, surfactant/ifacetransp.h:893-903 (Quad2 composite) ---- */
+    int nq = composite_nodes(p, 4, Q2_N, Q2_W, node, w);
+    for (int i = 0; i < 4; ++i) {
+        double s = 0.;
+        for (int k = 0; k < nq; ++k) s += w[k] * (node[k][i] * node[k][i]);
+        o->mass[i * 4 + i] = s;
+        for (int j = 0; j < i; ++j) {
+            double t = 0.;
+            for (int k = 0; k < nq; ++k) t += w[k] * (node[k][j] * node[k][i]);
+            o->mass[i * 4 + j] = o->mass[j * 4 + i] = coef[0] * t;
+        }
+    }
+    /* interface numbering integral of phi_i^2, misc/problem.cpp:452-456, 487-492 */
+    for (int i = 0; i < 4; ++i) {
+        double c[10] = {0};
+        c[i] = 1.;
+        for (int e = 0; e < 6; ++e)
+            if (VOE[e][0] == i || VOE[e][1] == i) c[4 + e] = 0.25;
+        double s = 0.;
+        for (int k = 0; k < nq; ++k) s += w[k] * p2_eval(c, node[k]);
+        o->hat_sq[i] = s;
+    }
+    /* area: quad_2D(1) on the Quad5 composite rule, as tests/principallattice.cpp:318-323 */
+    {
+        double n5[MAXF * 7][4], w5[MAXF * 7];
+        const int n5q = composite_nodes(p, 7, Q5_N, Q5_W, n5, w5);
+        double s = 0.;
+        for (int k = 0; k < n5q; ++k) s += w5[k] * 1.;
+        o->area = s;
+    }
+    /* ---- LocalLaplaceBeltramiP1CL::setup, surfactant/ifacetransp.h:1671-1686 ---- */
+    double G1[4][3];
+    const double det = p1_gradients(T->X, G1);
+    {
+        double q[4][MAXF][3];
+        for (int i = 0; i < 4; ++i)
+            for (int f = 0; f < p->nf; ++f) {
+                const double d = dot3(G1[i], p->normal[f]);
+                for (int c = 0; c < 3; ++c) q[i][f][c] = G1[i][c] - d * p->normal[f][c];
+            }
+        const double fac = coef[1] * 0.5;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double s = 0.;
+                for (int f = 0; f < p->nf; ++f) s += dot3(q[i][f], q[j][f]) * p->absdet[f];
+                o->lb[i * 4 + j] = o->lb[j * 4 + i] = fac * s;
+            }
+    }
+    /* ---- LocalNormalLaplaceBulkP1CL::setup(tet, InterfaceCommonDataP1CL),
+     *      surfactant/ifacetransp.h:2032-2051: D * Quad5(U_i U_j, |det|/6), U_i = n . grad(phi_i) ---- */
+    {
+        double v[10];
+        for (int d = 0; d < 10; ++d) v[d] = T->ls[LAT_OF_P2[d]];
+        double U[4][15];
+        for (int q = 0; q < 15; ++q) {
+            double n[3];
+            p2_normal(v, V5_N[q], G1, n);
+            for (int i = 0; i < 4; ++i) U[i][q] = dot3(n, G1[i]);
+        }
+        const double ad = fabs(det) / 6;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double f[15];
+                for (int q = 0; q < 15; ++q) f[q] = U[i][q] * U[j][q];
+                const double val = ad * (((8. / 405. * f[0] + V5_C1 * (((f[1] + f[2]) + f[3]) + f[4]))
+                                          + V5_C2 * (((f[5] + f[6]) + f[7]) + f[8]))
+                                         + 5. / 567. * (((((f[9] + f[10]) + f[11]) + f[12]) + f[13]) + f[14]));
+                o->stab[i * 4 + j] = o->stab[j * 4 + i] = coef[2] * val;
+            }
+    }
+    /* ---- LocalVectorP1CL::setup, surfactant/ifacetransp.h:785-797 (Quad5 composite) ---- */
+    if (rhs_kind != ORC_RHS_NONE) {
+        nq = composite_nodes(p, 7, Q5_N, Q5_W, node, w);
+        double qf[MAXF * 7];
+        for (int k = 0; k < nq; ++k) {
+            double x[3];
+            bary_to_world(T->X, node[k], x);
+            qf[k] = rhs_fun(rhs_kind, rhs_param, x);
+        }
+        for (int i = 0; i < 4; ++i) {
+            double s = 0.;
+            for (int k = 0; k < nq; ++k) s += w[k] * (qf[k] * node[k][i]);
+            o->rhs[i] = s;
+        }
+    } else {
+        o->rhs[0] = o->rhs[1] = o->rhs[2] = o->rhs[3] = 0.;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Public entry points                                                       */
+/* ------------------------------------------------------------------------ */
+
+static int check_mesh(const orc_mesh_t *m)
+{
+    return m && m->n[0] > 0 && m->n[1] > 0 && m->n[2] > 0;
+}
+
+int orc_classify(const orc_mesh_t *m, const double *phi, int zero_policy,
+                 int64_t cap, int64_t *ncut, int64_t *cut_tets)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !ncut) return ORC_E_ARG;
+    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    int64_t c = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        tet_t T;
+        load_tet(m, phi, t, &T);
+        if (equal_signs(T.ls, 10)) continue;
+        if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+        if (cut_tets) {
+            if (c >= cap) return ORC_E_CAPACITY;
+            cut_tets[c] = t;
+        }
+        ++c;
+    }
+    *ncut = c;
+    return ORC_OK;
+}
+
+int orc_tet_local(const orc_mesh_t *m, const double *phi, int64_t tet, const double coef[3],
+                  int rhs_kind, const double *rhs_param, int zero_policy, orc_local_t *out)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !out) return ORC_E_ARG;
+    memset(out, 0, sizeof *out);
+    tet_t T;
+    load_tet(m, phi, tet, &T);
+    if (equal_signs(T.ls, 10)) return ORC_OK;
+    if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+    patch_t p;
+    int err = make_patch(T.ls, &p);
+    if (err) return err;
+    if (p.nf == 0) return ORC_OK;
+    patch_geometry(T.X, &p);
+    local_quantities(&T, &p, coef, rhs_kind, rhs_param, out);
+    return ORC_OK;
+}
+
+/* IdxDescCL::CreateNumbOnInterface -> CreateNumbOnInterfaceVertex, misc/problem.cpp:443-503 */
+int orc_number_p1if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
+                    int32_t *dof_of_vertex, int64_t *ndof)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !ndof) return ORC_E_ARG;
+    const int64_t nv = (int64_t)(m->n[0] + 1) * (m->n[1] + 1) * (m->n[2] + 1);
+    for (int64_t v = 0; v < nv; ++v) dof_of_vertex[v] = -1;
+    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    int64_t counter = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        tet_t T;
+        load_tet(m, phi, t, &T);
+        if (equal_signs(T.ls, 10)) continue; /* distance() != 0 */
+        if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+        const double limit = omit_bound < 0. ? 0. : omit_bound * pow(tet_volume(T.X) * 6, 4. / 3.);
+        patch_t p;
+        int err = make_patch(T.ls, &p);
+        if (err) return err;
+        if (p.nf == 0) continue;
+        patch_geometry(T.X, &p);
+        double node[MAXF * 4][4], w[MAXF * 4];
+        const int nq = composite_nodes(&p, 4, Q2_N, Q2_W, node, w);
+        for (int i = 0; i < 4; ++i) {
+            if (dof_of_vertex[T.vid[i]] >= 0) continue;
+            double c[10] = {0};
+            c[i] = 1.;
+            for (int e = 0; e < 6; ++e)
+                if (VOE[e][0] == i || VOE[e][1] == i) c[4 + e] = 0.25;
+            double s = 0.;
+            for (int k = 0; k < nq; ++k) s += w[k] * p2_eval(c, node[k]);
+            if (s > limit) dof_of_vertex[T.vid[i]] = (int32_t)counter++;
+        }
+    }
+    *ndof = counter;
+    return ORC_OK;
+}
+
+/* ---- row builder with the semantics of SparseMatBuilderCL (num/spmat.h:551-677):
+ *      every touched (i,j) is stored (explicit zeros kept), values are summed in
+ *      insertion order starting from 0.0, rows are sorted by column on Build() ---- */
+#define ROWCAP 32
+typedef struct {
+    int32_t n;
+    int32_t col[ROWCAP];
+    double v[3][ROWCAP];
+} brow_t;
+
+static int builder_add(brow_t *r, int32_t col, const double val[3])
+{
+    int k = 0;
+    for (; k < r->n; ++k)
+        if (r->col[k] == col) break;
+    if (k == r->n) {
+        if (r->n == ROWCAP) return ORC_E_CAPACITY;
+        r->col[k] = col;
+        r->v[0][k] = r->v[1][k] = r->v[2][k] = 0.0;
+        ++r->n;
+    }
+    for (int o = 0; o < 3; ++o) r->v[o][k] += val[o];
+    return ORC_OK;
+}
+
+typedef struct {
+    int64_t t;
+    int err;
+    orc_local_t loc;
+    int64_t vid[4];
+    int nonempty;
+} cutrec_t;
+
+int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                    const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                    int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                    double *val_mass, double *val_lb, double *val_stab, double *rhs)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !nnz || !rowptr || !coef) return ORC_E_ARG;
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    cutrec_t *rec = (cutrec_t *)malloc(sizeof(cutrec_t) * (ncut > 0 ? ncut : 1));
+    brow_t *rows = (brow_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow_t));
+    if (!cut || !rec || !rows) {
+        free(cut); free(rec); free(rows);
+        return ORC_E_NOMEM;
+    }
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
+    /* local matrices: independent per tet (the reference runs them per colour class in parallel) */
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t c = 0; c < ncut; ++c) {
+        tet_t T;
+        load_tet(m, phi, cut[c], &T);
+        patch_t p;
+        rec[c].t = cut[c];
+        rec[c].nonempty = 0;
+        rec[c].err = make_patch(T.ls, &p);
+        for (int i = 0; i < 4; ++i) rec[c].vid[i] = T.vid[i];
+        if (rec[c].err || p.nf == 0) continue;
+        rec[c].nonempty = 1;
+        patch_geometry(T.X, &p);
+        local_quantities(&T, &p, coef, rhs_kind, rhs_param, &rec[c].loc);
+    }
+    if (rhs)
+        for (int64_t i = 0; i < ndof; ++i) rhs[i] = 0.;
+    /* scatter in mesh order: update_global_matrix, surfactant/ifacetransp.tpp:31-43,
+     * InterfaceVectorAccuCL::visit, surfactant/ifacetransp.h:630-637 */
+    for (int64_t c = 0; c < ncut && !err; ++c) {
+        if (rec[c].err) { err = rec[c].err; break; }
+        if (!rec[c].nonempty) continue; /* cdata.empty() */
+        int32_t numb[4];
+        for (int i = 0; i < 4; ++i) numb[i] = dof_of_vertex[rec[c].vid[i]];
+        const orc_local_t *L = &rec[c].loc;
+        for (int i = 0; i < 4 && !err; ++i) {
+            if (numb[i] < 0) continue;
+            for (int j = 0; j < 4; ++j) {
+                if (numb[j] < 0) continue;
+                const double v3[3] = {L->mass[i * 4 + j], L->lb[i * 4 + j], L->stab[i * 4 + j]};
+                err = builder_add(&rows[numb[i]], numb[j], v3);
+                if (err) break;
+            }
+        }
+        if (rhs)
+            for (int i = 0; i < 4; ++i)
+                if (numb[i] >= 0) rhs[numb[i]] += L->rhs[i];
+    }
+    /* Build(): row sizes, prefix sum, per-row sort by column */
+    if (!err) {
+        rowptr[0] = 0;
+        for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
+        *nnz = rowptr[ndof];
+        if (*nnz > cap_nnz) err = ORC_E_CAPACITY;
+    }
+    if (!err && colind) {
+        for (int64_t r = 0; r < ndof; ++r) {
+            brow_t *R = &rows[r];
+            int32_t ord[ROWCAP];
+            for (int k = 0; k < R->n; ++k) ord[k] = k;
+            for (int a = 1; a < R->n; ++a) {
+                int32_t x = ord[a];
+                int b = a - 1;
+                while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
+                ord[b + 1] = x;
+            }
+            for (int k = 0; k < R->n; ++k) {
+                const int64_t dst = rowptr[r] + k;
+                colind[dst] = R->col[ord[k]];
+                if (val_mass) val_mass[dst] = R->v[0][ord[k]];
+                if (val_lb) val_lb[dst] = R->v[1][ord[k]];
+                if (val_stab) val_stab[dst] = R->v[2][ord[k]];
+            }
+        }
+    }
+    free(cut);
+    free(rec);
+    free(rows);
+    return err;
+}
+
+/* tests/principallattice.cpp:296-330: patch + make_CompositeQuad5Domain2D + quad_2D(1) over every tet */
+int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, double *area)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !area) return ORC_E_ARG;
+    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    double surf = 0.;
+    for (int64_t t = 0; t < nt; ++t) {
+        tet_t T;
+        load_tet(m, phi, t, &T);
+        if (equal_signs(T.ls, 10)) continue;
+        if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+        patch_t p;
+        int err = make_patch(T.ls, &p);
+        if (err) return err;
+        if (p.nf == 0) continue;
+        patch_geometry(T.X, &p);
+        double node[MAXF * 7][4], w[MAXF * 7];
+        const int nq = composite_nodes(&p, 7, Q5_N, Q5_W, node, w);
+        double s = 0.;
+        for (int k = 0; k < nq; ++k) s += w[k] * 1.;
+        surf += s;
+    }
+    *area = surf;
+    return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse linear algebra                                                     */
+/* ------------------------------------------------------------------------ */
+
+/* y_Ax, num/spmat.h:1683-1711 */
+void orc_spmv(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+              const double *x, double *y)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double sum = 0.0;
+        for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) sum += val[k] * x[colind[k]];
+        y[i] = sum;
+    }
+}
+
+/* KahanInnerProd, num/spmat.h:318-335 (compiled without associative math) */
+#if defined(__GNUC__) && !defined(__clang__)
+__attribute__((optimize("no-associative-math")))
+#endif
+double orc_kahan_dot(int64_t n, const double *x, const double *y)
+{
+    volatile double sum = 0., c = 0.;
+    for (int64_t i = 0; i < n; ++i) {
+        const double yy = x[i] * y[i] - c;
+        const double t = sum + yy;
+        c = (t - sum) - yy;
+        sum = t;
+    }
+    return sum;
+}
+
+/* SolveGSstep<PB_JAC0> with omega = 1, num/precond.h:155-170: diagonal found by a row scan */
+static void jacobi_apply(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                         const double *b, double *x)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t nz = rowptr[i];
+        for (const int64_t end = rowptr[i + 1]; nz < end && colind[nz] != i; ++nz) ;
+        x[i] = 1.0 * b[i] / val[nz];
+    }
+}
+
+/* PCG, num/krylovsolver.h:107-196, with DummyExchangeCL dots (parallel/exchange.h:58-84) */
+int orc_pcg_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                   const double *b, double *x, int32_t *max_iter, double *tol, int32_t rel)
+{
+    double *p = (double *)calloc(n, sizeof(double)), *z = (double *)malloc(n * sizeof(double)),
+           *q = (double *)malloc(n * sizeof(double)), *r = (double *)malloc(n * sizeof(double));
+    if (!p || !z || !q || !r) { free(p); free(z); free(q); free(r); return ORC_E_NOMEM; }
+    orc_spmv(n, rowptr, colind, val, x, q);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    double resid = sqrt(orc_kahan_dot(n, r, r));
+    double normb = sqrt(orc_kahan_dot(n, b, b));
+    if (normb == 0.0 || rel == 0) normb = 1.0;
+    resid = resid / normb;
+    int ret = ORC_E_NOT_CONVERGED;
+    if (resid <= *tol) {
+        *tol = resid;
+        *max_iter = 0;
+        ret = ORC_OK;
+        goto done;
+    }
+    jacobi_apply(n, rowptr, colind, val, r, z);
+    double rho = orc_kahan_dot(n, z, r), rho_1;
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i];
+    for (int it = 1; it <= *max_iter; ++it) {
+        orc_spmv(n, rowptr, colind, val, p, q);
+        const double lambda = orc_kahan_dot(n, q, p);
+        const double alpha = rho / lambda;
+        for (int64_t i = 0; i < n; ++i) x[i] += alpha * p[i];
+        for (int64_t i = 0; i < n; ++i) r[i] += (-alpha) * q[i];
+        resid = sqrt(orc_kahan_dot(n, r, r)) / normb;
+        if (resid <= *tol) {
+            *tol = resid;
+            *max_iter = it;
+            ret = ORC_OK;
+            goto done;
+        }
+        jacobi_apply(n, rowptr, colind, val, r, z);
+        rho_1 = rho;
+        rho = orc_kahan_dot(n, z, r);
+        const double beta = rho / rho_1;
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    *tol = resid;
+done:
+    free(p); free(z); free(q); free(r);
+    return ret;
+}
+
+static double plain_dot(int64_t n, const double *x, const double *y)
+{
+    double s = 0.; /* std::inner_product, num/spmat.h:164-170 */
+    for (int64_t i = 0; i < n; ++i) s = s + x[i] * y[i];
+    return s;
+}
+
+/* GMRES_GeneratePlaneRotation / ApplyPlaneRotation, num/krylovsolver.h:258-280 */
+static void gen_rot(double dx, double dy, double *cs, double *sn)
+{
+    if (dy == 0.0) { *cs = 1.0; *sn = 0.0; }
+    else { const double r = sqrt(dx * dx + dy * dy); *cs = dx / r; *sn = dy / r; }
+}
+static void apply_rot(double *dx, double *dy, double cs, double sn)
+{
+    const double tmp = cs * *dx + sn * *dy;
+    *dy = -sn * *dx + cs * *dy;
+    *dx = tmp;
+}
+
+/* GMRES_Update, num/krylovsolver.h:283-296 (H stored column-major, ldh = m) */
+static void gmres_update(int64_t n, double *x, int k, const double *H, int m, const double *s, double *const *v)
+{
+    double *y = (double *)malloc(sizeof(double) * m);
+    for (int i = 0; i < m; ++i) y[i] = s[i];
+    for (int i = k; i >= 0; --i) {
+        y[i] /= H[i + (size_t)i * m];
+        for (int j = i - 1; j >= 0; --j) y[j] -= H[j + (size_t)i * m] * y[i];
+    }
+    for (int i = 0; i <= k; ++i)
+        for (int64_t l = 0; l < n; ++l) x[l] += y[i] * v[i][l];
+    free(y);
+}
+
+/* serial GMRES, num/krylovsolver.h:320-463 (restart m performs m-1 Arnoldi steps per cycle) */
+int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                     const double *b, double *x, int32_t restart, int32_t *max_iter, double *tol,
+                     int32_t rel, int32_t left_pc, int32_t use_pc)
+{
+    int m = restart <= *max_iter ? restart : *max_iter;
+    if (m < 2) m = 2;
+    double *H = (double *)calloc((size_t)m * m, sizeof(double));
+    double *s = (double *)calloc(m, sizeof(double)), *cs = (double *)calloc(m, sizeof(double)),
+           *sn = (double *)calloc(m, sizeof(double));
+    double *w = (double *)malloc(n * sizeof(double)), *r = (double *)malloc(n * sizeof(double));
+    double *z = (double *)malloc(n * sizeof(double)), *t = (double *)malloc(n * sizeof(double));
+    double **v = (double **)malloc(sizeof(double *) * m);
+    for (int i = 0; i < m; ++i) v[i] = (double *)malloc(n * sizeof(double));
+    double beta, normb, resid;
+    int ret = ORC_E_NOT_CONVERGED;
+#define PC_APPLY(dst, src) \
+    do { if (use_pc) jacobi_apply(n, rowptr, colind, val, (src), (dst)); \
+         else memcpy((dst), (src), n * sizeof(double)); } while (0)
+    if (!left_pc) {
+        orc_spmv(n, rowptr, colind, val, x, t);
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - t[i];
+        beta = sqrt(plain_dot(n, r, r));
+        normb = sqrt(plain_dot(n, b, b));
+    } else {
+        orc_spmv(n, rowptr, colind, val, x, t);
+        for (int64_t i = 0; i < n; ++i) t[i] = b[i] - t[i];
+        PC_APPLY(r, t);
+        beta = sqrt(plain_dot(n, r, r));
+        PC_APPLY(w, b);
+        normb = sqrt(plain_dot(n, w, w));
+    }
+    if (normb == 0.0 || rel == 0) normb = 1.0;
+    resid = beta / normb;
+    if (resid <= *tol) {
+        *tol = resid;
+        *max_iter = 0;
+        ret = ORC_OK;
+        goto out;
+    }
+    int j = 1;
+    while (j <= *max_iter) {
+        for (int64_t l = 0; l < n; ++l) v[0][l] = r[l] * (1.0 / beta);
+        for (int i = 0; i < m; ++i) s[i] = 0.0;
+        s[0] = beta;
+        int i;
+        for (i = 0; i < m - 1 && j <= *max_iter; ++i, ++j) {
+            if (!left_pc) {
+                PC_APPLY(t, v[i]);
+                orc_spmv(n, rowptr, colind, val, t, w);
+            } else {
+                orc_spmv(n, rowptr, colind, val, v[i], t);
+                PC_APPLY(w, t);
+            }
+            for (int k = 0; k <= i; ++k) {
+                H[k + (size_t)i * m] = plain_dot(n, w, v[k]);
+                const double hk = H[k + (size_t)i * m];
+                for (int64_t l = 0; l < n; ++l) w[l] -= hk * v[k][l];
+            }
+            H[(i + 1) + (size_t)i * m] = sqrt(plain_dot(n, w, w));
+            {
+                const double inv = 1.0 / H[(i + 1) + (size_t)i * m];
+                for (int64_t l = 0; l < n; ++l) v[i + 1][l] = w[l] * inv;
+            }
+            for (int k = 0; k < i; ++k)
+                apply_rot(&H[k + (size_t)i * m], &H[k + 1 + (size_t)i * m], cs[k], sn[k]);
+            gen_rot(H[i + (size_t)i * m], H[i + 1 + (size_t)i * m], &cs[i], &sn[i]);
+            apply_rot(&H[i + (size_t)i * m], &H[i + 1 + (size_t)i * m], cs[i], sn[i]);
+            apply_rot(&s[i], &s[i + 1], cs[i], sn[i]);
+            resid = fabs(s[i + 1]) / normb;
+            if (resid <= *tol) {
+                if (!left_pc) {
+                    for (int64_t l = 0; l < n; ++l) z[l] = 0.;
+                    gmres_update(n, z, i, H, m, s, v);
+                    PC_APPLY(t, z);
+                    for (int64_t l = 0; l < n; ++l) x[l] += t[l];
+                } else {
+                    gmres_update(n, x, i, H, m, s, v);
+                }
+                *tol = resid;
+                *max_iter = j;
+                ret = ORC_OK;
+                goto out;
+            }
+        }
+        if (!left_pc) {
+            for (int64_t l = 0; l < n; ++l) z[l] = 0.;
+            gmres_update(n, z, i - 1, H, m, s, v);
+            PC_APPLY(t, z);
+            for (int64_t l = 0; l < n; ++l) x[l] += t[l];
+            orc_spmv(n, rowptr, colind, val, x, t);
+            for (int64_t l = 0; l < n; ++l) r[l] = b[l] - t[l];
+        } else {
+            gmres_update(n, x, i - 1, H, m, s, v);
+            orc_spmv(n, rowptr, colind, val, x, t);
+            for (int64_t l = 0; l < n; ++l) t[l] = b[l] - t[l];
+            PC_APPLY(r, t);
+        }
+        beta = sqrt(plain_dot(n, r, r));
+        resid = beta / normb;
+        if (resid <= *tol) {
+            *tol = resid;
+            *max_iter = j;
+            ret = ORC_OK;
+            goto out;
+        }
+    }
+    *tol = resid;
+#undef PC_APPLY
+out:
+    for (int i = 0; i < m; ++i) free(v[i]);
+    free(v); free(H); free(s); free(cs); free(sn); free(w); free(r); free(z); free(t);
+    return ret;
+}

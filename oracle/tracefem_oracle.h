@@ -1,0 +1,120 @@
+/*
+ * tracefem_oracle.h -- CPU oracle for the trace-FEM assembly + Krylov path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the DROPS
+ * (56th/drops) algorithms, used by tests/, by __graft_entry__.smoke() and by
+ * the cpu_baseline leg of bench.py as the checker.  No product code path
+ * links, loads or calls it.
+ *
+ * Every function names the reference file:line it restates (paths relative
+ * to the reference's src/ directory).  Parity pins (see tests/test_oracle_*):
+ *   - lattice / sign / patch tables: brute force from their definitions
+ *   - P1 interface numbering: tests/specifications/serial/TestinterfaceP1FE.ref
+ *     (0, 4, 8, 4, 1156)
+ *   - surface quadrature + patch: Testprincipallattice.ref (area 3.0909)
+ *   - Quad2/Quad5 2D rules: exactness on polynomials (TestQuad5_2D.ref idea)
+ *   - mass matrix: two independent routes (per-facet Quad5_2D vs composite
+ *     Quad2), as tests/interfaceP1FE.cpp:140-230 does
+ *   - Laplace-Beltrami: closed form on planar patches + the O(h^2)
+ *     convergence to the continuous sphere solution
+ *   - normal stabilisation: closed form for planar level sets
+ *   - CSR builder / SpMV / Kahan dot: dense brute force
+ *   - PCG / GMRES: dense direct solve on small systems
+ */
+#ifndef TRACEFEM_ORACLE_H
+#define TRACEFEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_OK = 0,
+    ORC_E_ZERO_NODE = 1,     /* std::invalid_argument, geom/subtriangulation.tpp:44 */
+    ORC_E_ZERO_PATTERN = 2,  /* DROPSErrCL, geom/reftetracut.h:152-153 */
+    ORC_E_ARG = 3,
+    ORC_E_NOT_CONVERGED = 4,
+    ORC_E_NOMEM = 5,
+    ORC_E_CAPACITY = 6
+};
+
+/* zero_policy: 0 = reference behaviour (a cut tet with an exact-zero P2 node
+ * raises ORC_E_ZERO_NODE, misc/problem.cpp:472 via distance()); 1 = allow
+ * (historical behaviour, used to pin zero-vertex / boundary-facet handling). */
+enum { ORC_ZERO_THROW = 0, ORC_ZERO_ALLOW = 1 };
+
+/* Kuhn brick mesh: BrickBuilderCL(orig, e1, e2, e3, n1, n2, n3),
+ * geom/builder.cpp:99-178.  e[k] is the k-th edge vector of the brick. */
+typedef struct {
+    double orig[3];
+    double e[3][3];
+    int32_t n[3];
+} orc_mesh_t;
+
+/* Operator selection bits for the local matrices. */
+enum { ORC_OP_MASS = 1, ORC_OP_LB = 2, ORC_OP_STAB = 4 };
+
+/* right-hand-side functions evaluated at surface quadrature points */
+enum { ORC_RHS_NONE = 0, ORC_RHS_3XY_MINUS_X3 = 1, ORC_RHS_CONST = 2 };
+
+/* ---- tables (exposed for the table tests) ---- */
+void orc_lattice_tables(int32_t bary2[10][4], int32_t tets[8][4]);
+/* 81 patterns indexed by pattern_idx+40: size, 2 triangles x 3, boundary flag */
+void orc_refpatch_table(int32_t size[81], int32_t tri[81][2][3], int32_t bnd[81]);
+
+/* ---- mesh / level set ---- */
+void orc_vertex_coord(const orc_mesh_t *m, int64_t i1, int64_t i2, int64_t i3, double x[3]);
+/* world coordinates of half-grid point (j1,j2,j3): vertex or edge midpoint */
+void orc_halfgrid_coord(const orc_mesh_t *m, int64_t j1, int64_t j2, int64_t j3, double x[3]);
+
+/* ---- classification: all tets whose 10 P2 values do not share one strict sign ---- */
+int orc_classify(const orc_mesh_t *m, const double *phi, int zero_policy,
+                 int64_t cap, int64_t *ncut, int64_t *cut_tets);
+
+/* ---- P1 interface numbering (misc/problem.cpp:443-503) ---- */
+int orc_number_p1if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
+                    int32_t *dof_of_vertex, int64_t *ndof);
+
+/* ---- per-tet local data (debug / fine-grained parity) ---- */
+typedef struct {
+    int32_t nfacet;
+    int32_t nvert;
+    double area;          /* sum over facets of absdet/2 * (0.5 on boundary facets) */
+    double mass[16];
+    double lb[16];
+    double stab[16];
+    double rhs[4];
+    double hat_sq[4];     /* Quad2 composite integral of phi_i^2 (numbering test) */
+} orc_local_t;
+
+int orc_tet_local(const orc_mesh_t *m, const double *phi, int64_t tet, const double coef[3],
+                  int rhs_kind, const double *rhs_param, int zero_policy, orc_local_t *out);
+
+/* ---- assembly (accumulate() with InterfaceCommonDataP1CL + matrix/vector
+ *      accumulators, num/accumulator.h:215-228, surfactant/ifacetransp.h:533-641) ---- */
+int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                    const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                    int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                    double *val_mass, double *val_lb, double *val_stab, double *rhs);
+
+/* surface integral of 1 over the patch (tests/principallattice.cpp:296-330) */
+int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, double *area);
+
+/* ---- sparse linear algebra (num/spmat.h, num/krylovsolver.h) ---- */
+void orc_spmv(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+              const double *x, double *y);
+double orc_kahan_dot(int64_t n, const double *x, const double *y);
+/* PCG with Jacobi (num/krylovsolver.h:107-196 + num/precond.h:155-170) */
+int orc_pcg_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                   const double *b, double *x, int32_t *max_iter, double *tol, int32_t rel);
+/* serial restarted GMRES, right/left Jacobi preconditioning (num/krylovsolver.h:320-463) */
+int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                     const double *b, double *x, int32_t restart, int32_t *max_iter, double *tol,
+                     int32_t rel, int32_t left_pc, int32_t use_pc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
