@@ -1,0 +1,1022 @@
+// dhip.hip -- libdhip: MI355X (gfx950) trace-FEM assembly + Krylov core behind
+// the C ABI of include/dhip.h.  Single translation unit; built with
+//   hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fPIC -shared
+#include "dhip_common.hpp"
+#include "dhip_assembly.hpp"
+#include "dhip_krylov.hpp"
+
+#include <algorithm>
+#include <mutex>
+
+namespace dhip {
+
+__constant__ Tables c_tab;
+
+// ---------------------------------------------------------------------------
+// host-side derivation of the device tables
+// ---------------------------------------------------------------------------
+static Tables g_host_tab;
+static std::once_flag g_tab_once;
+static std::mutex g_dev_mutex;
+static std::vector<int> g_dev_uploaded;
+
+static const int EDGE_V[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};  // geom/topo.cpp:45-46
+
+static void derive_tables(Tables &T) {
+    std::memset(&T, 0, sizeof T);
+    // Kuhn tets of a cell, vertex order of BrickBuilderCL::build_ser_impl (geom/builder.cpp:155-175)
+    const int kuhn[6][4][3] = {{{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}},
+                               {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {1, 1, 1}},
+                               {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}},
+                               {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+                               {{0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}},
+                               {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}}};
+    for (int k = 0; k < 6; ++k)
+        for (int q = 0; q < 4; ++q)
+            for (int c = 0; c < 3; ++c) T.kuhn[k][q][c] = (int8_t)kuhn[k][q][c];
+    // order-2 principal lattice: points (x,y,z), 2 >= x >= y >= z >= 0, in rlex order,
+    // barycentric (2-x, x-y, y-z, z)/2 (geom/principallattice.cpp:37-43)
+    int pts[10][3], np = 0;
+    for (int x = 0; x <= 2; ++x)
+        for (int y = 0; y <= x; ++y)
+            for (int z = 0; z <= y; ++z) {
+                pts[np][0] = x; pts[np][1] = y; pts[np][2] = z;
+                T.lat_b2[np][0] = (int8_t)(2 - x);
+                T.lat_b2[np][1] = (int8_t)(x - y);
+                T.lat_b2[np][2] = (int8_t)(y - z);
+                T.lat_b2[np][3] = (int8_t)z;
+                for (int c = 0; c < 4; ++c) T.lat_b[np][c] = (double)T.lat_b2[np][c] / 2.0;
+                ++np;
+            }
+    auto pindex = [&](int x, int y, int z) {
+        for (int i = 0; i < np; ++i)
+            if (pts[i][0] == x && pts[i][1] == y && pts[i][2] == z) return i;
+        return -1;
+    };
+    // sub-tets: unit steps along a permutation of the axes from every start point
+    // that stays inside the lattice (geom/principallattice.cpp:46-80)
+    const int perm[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    int nt = 0;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j <= i; ++j)
+            for (int k = 0; k <= j; ++k)
+                for (int l = 0; l < 6; ++l) {
+                    int p[3] = {i, j, k}, v[4];
+                    v[0] = pindex(p[0], p[1], p[2]);
+                    bool inside = true;
+                    for (int s = 0; s < 3 && inside; ++s) {
+                        ++p[perm[l][s]];
+                        inside = 2 >= p[0] && p[0] >= p[1] && p[1] >= p[2];
+                        if (inside) v[s + 1] = pindex(p[0], p[1], p[2]);
+                    }
+                    if (!inside) continue;
+                    for (int q = 0; q < 4; ++q) T.lat_t[nt][q] = (int8_t)v[q];
+                    ++nt;
+                }
+    // lattice edges numbered at first appearance (sub-tet order, local edge order);
+    // the first sub-tet containing a cut edge is the one that creates its cut vertex
+    int ne = 0;
+    for (int t = 0; t < 8; ++t)
+        for (int e = 0; e < 6; ++e) {
+            const int a = T.lat_t[t][EDGE_V[e][0]], b = T.lat_t[t][EDGE_V[e][1]];
+            int id = -1;
+            for (int f = 0; f < ne; ++f)
+                if ((T.edge_ab[f][0] == a && T.edge_ab[f][1] == b) || (T.edge_ab[f][0] == b && T.edge_ab[f][1] == a))
+                    id = f;
+            if (id < 0) {
+                id = ne++;
+                T.edge_ab[id][0] = (int8_t)a;
+                T.edge_ab[id][1] = (int8_t)b;
+            }
+            T.sub_edge[t][e] = (int8_t)id;
+        }
+    // sign pattern -> triangles: roots = zero vertices, then sign-changing edges, in
+    // increasing number; triangles are consecutive root triples
+    // (SignTraitsCL<3>::compute_cuts, RefPatchBuilderCL<3>::assign)
+    for (int pid = 0; pid < 81; ++pid) {
+        int s[4], rem = pid;
+        for (int q = 0; q < 4; ++q) {
+            s[q] = rem % 3 - 1;
+            rem /= 3;
+        }
+        // pid = (s0+1) + 3(s1+1) + 9(s2+1) + 27(s3+1) = s0 + 3 s1 + 9 s2 + 27 s3 + 40
+        int roots[4], nz = 0, nr = 0;
+        if (std::abs(s[0] + s[1] + s[2] + s[3]) != 4) {
+            for (int q = 0; q < 4; ++q)
+                if (s[q] == 0) roots[nr++] = q;
+            nz = nr;
+            for (int e = 0; e < 6; ++e)
+                if (s[EDGE_V[e][0]] * s[EDGE_V[e][1]] == -1) roots[nr++] = 4 + e;
+        }
+        const int ntri = nr >= 3 ? nr - 2 : 0;
+        T.rp_size[pid] = (int8_t)ntri;
+        for (int tr = 0; tr < 2; ++tr)
+            for (int j = 0; j < 3; ++j) T.rp_tri[pid][tr][j] = (int8_t)(tr < ntri ? roots[tr + j] : 0);
+        T.rp_bnd[pid] = (int8_t)(nz == 3);
+    }
+    // P2 dof (vertices 0-3, edges 0-5) -> lattice vertex
+    for (int l = 0; l < 10; ++l) {
+        int a = -1, b = -1;
+        for (int c = 0; c < 4; ++c)
+            if (T.lat_b2[l][c]) (a < 0 ? a : b) = c;
+        if (b < 0) T.lat_of_p2[a] = (int8_t)l;
+        else
+            for (int e = 0; e < 6; ++e)
+                if (EDGE_V[e][0] == a && EDGE_V[e][1] == b) T.lat_of_p2[4 + e] = (int8_t)l;
+    }
+    // cell-node index (3x3x3 half-grid) of each lattice vertex of each Kuhn tet
+    for (int k = 0; k < 6; ++k) {
+        uint32_t m = 0;
+        for (int l = 0; l < 10; ++l) {
+            int o[3] = {0, 0, 0};
+            for (int q = 0; q < 4; ++q)
+                for (int c = 0; c < 3; ++c) o[c] += T.lat_b2[l][q] * kuhn[k][q][c];
+            T.tet_node27[k][l] = (int8_t)(o[2] * 9 + o[1] * 3 + o[0]);
+            m |= 1u << T.tet_node27[k][l];
+        }
+        T.tetmask27[k] = m;
+    }
+    // incidence: vertex = cell + delta, delta = (d&1, d>>1&1, d>>2&1)
+    for (int d = 0; d < 8; ++d)
+        for (int k = 0; k < 6; ++k) {
+            T.inc[d][k] = -1;
+            for (int q = 0; q < 4; ++q)
+                if (kuhn[k][q][0] == (d & 1) && kuhn[k][q][1] == ((d >> 1) & 1) && kuhn[k][q][2] == ((d >> 2) & 1))
+                    T.inc[d][k] = (int8_t)q;
+        }
+    // neighbour slots: the 15 vertex offsets of a Kuhn-mesh row (self + 7 edge directions x 2)
+    int ns = 0;
+    auto slot_of = [&](int a, int b, int c) {
+        for (int s = 0; s < ns; ++s)
+            if (T.slot_delta[s][0] == a && T.slot_delta[s][1] == b && T.slot_delta[s][2] == c) return s;
+        T.slot_delta[ns][0] = (int8_t)a;
+        T.slot_delta[ns][1] = (int8_t)b;
+        T.slot_delta[ns][2] = (int8_t)c;
+        return ns++;
+    };
+    slot_of(0, 0, 0);
+    for (int k = 0; k < 6; ++k)
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                T.slot[k][i][j] = (int8_t)slot_of(kuhn[k][j][0] - kuhn[k][i][0], kuhn[k][j][1] - kuhn[k][i][1],
+                                                  kuhn[k][j][2] - kuhn[k][i][2]);
+    // quadrature rules (num/discretize.cpp:153-211, 234-238, 263-296; num/discretize.tpp:754-762)
+    const double s15 = std::sqrt(15.0);
+    const double q2n[4][3] = {{1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}, {1. / 3., 1. / 3., 1. / 3.}};
+    std::memcpy(T.q2n, q2n, sizeof q2n);
+    T.q2w[0] = T.q2w[1] = T.q2w[2] = 1. / 24.;
+    T.q2w[3] = 0.375;
+    {
+        const double A1 = (6.0 - s15) / 21.0, B1 = (9.0 + 2.0 * s15) / 21.0;
+        const double A2 = (6.0 + s15) / 21.0, B2 = (9.0 - 2.0 * s15) / 21.0;
+        const double q5n[7][3] = {{1. / 3., 1. / 3., 1. / 3.}, {A1, A1, B1}, {A1, B1, A1}, {B1, A1, A1},
+                                  {A2, A2, B2}, {A2, B2, A2}, {B2, A2, A2}};
+        std::memcpy(T.q5n, q5n, sizeof q5n);
+        T.q5w[0] = 9. / 80.;
+        T.q5w[1] = T.q5w[2] = T.q5w[3] = (155. - s15) / 2400.;
+        T.q5w[4] = T.q5w[5] = T.q5w[6] = (155. + s15) / 2400.;
+    }
+    {
+        const double A1 = (7.0 - s15) / 34.0, B1 = (13.0 + 3.0 * s15) / 34.0;
+        const double A2 = (7.0 + s15) / 34.0, B2 = (13.0 - 3.0 * s15) / 34.0;
+        const double A3 = (10.0 - 2.0 * s15) / 40.0, B3 = (10.0 + 2.0 * s15) / 40.0;
+        const double v5[15][4] = {{0.25, 0.25, 0.25, 0.25},
+                                  {A1, A1, A1, B1}, {A1, A1, B1, A1}, {A1, B1, A1, A1}, {B1, A1, A1, A1},
+                                  {A2, A2, A2, B2}, {A2, A2, B2, A2}, {A2, B2, A2, A2}, {B2, A2, A2, A2},
+                                  {A3, A3, B3, B3}, {A3, B3, A3, B3}, {A3, B3, B3, A3},
+                                  {B3, A3, A3, B3}, {B3, A3, B3, A3}, {B3, B3, A3, A3}};
+        std::memcpy(T.v5n, v5, sizeof v5);
+        T.v5c1 = (2665.0 + 14.0 * s15) / 226800.0;
+        T.v5c2 = (2665.0 - 14.0 * s15) / 226800.0;
+    }
+}
+
+const Tables &host_tables() {
+    std::call_once(g_tab_once, [] { derive_tables(g_host_tab); });
+    return g_host_tab;
+}
+
+static hipError_t upload_tables(int device) {
+    std::lock_guard<std::mutex> lk(g_dev_mutex);
+    if (std::find(g_dev_uploaded.begin(), g_dev_uploaded.end(), device) != g_dev_uploaded.end())
+        return hipSuccess;
+    const Tables &T = host_tables();
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_tab), &T, sizeof T, 0, hipMemcpyHostToDevice);
+    if (e == hipSuccess) g_dev_uploaded.push_back(device);
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// errors, profiling
+// ---------------------------------------------------------------------------
+int set_error(dhip_ctx *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        va_list ap;
+        va_start(ap, fmt);
+        std::vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+hipError_t take_event(dhip_ctx *ctx, hipEvent_t *ev) {
+    if (!ctx->event_pool.empty()) {
+        *ev = ctx->event_pool.back();
+        ctx->event_pool.pop_back();
+        return hipSuccess;
+    }
+    return hipEventCreate(ev);
+}
+
+ProfScope::ProfScope(dhip_ctx *c, const char *n) : ctx(c), name(n) {
+    if (!ctx->prof) return;
+    if (take_event(ctx, &a) != hipSuccess || take_event(ctx, &b) != hipSuccess) {
+        a = b = nullptr;
+        return;
+    }
+    (void)hipEventRecord(a, ctx->stream);
+}
+
+ProfScope::~ProfScope() {
+    if (!ctx->prof || !a) return;
+    (void)hipEventRecord(b, ctx->stream);
+    ctx->pending.push_back({name, a, b});
+}
+
+int flush_profile(dhip_ctx *ctx) {
+    for (auto &p : ctx->pending) {
+        float ms = 0.f;
+        DHIP_HIPCHK(ctx, hipEventSynchronize(p.b));
+        DHIP_HIPCHK(ctx, hipEventElapsedTime(&ms, p.a, p.b));
+        auto &acc = ctx->prof_acc[p.name];
+        acc.first += ms;
+        acc.second += 1;
+        ctx->event_pool.push_back(p.a);
+        ctx->event_pool.push_back(p.b);
+    }
+    ctx->pending.clear();
+    return DHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// device-wide exclusive scan of int32
+// ---------------------------------------------------------------------------
+constexpr int SCAN_ITEMS = 4, SCAN_TILE = 256 * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const int32_t *in, int64_t n, int32_t *bsum) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int64_t j = base + (int64_t)i * 256 + threadIdx.x;
+        if (j < n) s += in[j];
+    }
+    int tot;
+    block_excl_scan<256>(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan_top(int32_t *bsum, int64_t nb, int32_t *total) {
+    int carry = 0;
+    for (int64_t base = 0; base < nb; base += 256) {
+        const int64_t j = base + threadIdx.x;
+        const int v = j < nb ? bsum[j] : 0;
+        int tot;
+        const int ex = block_excl_scan<256>(v, &tot);
+        if (j < nb) bsum[j] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(256) k_scan_final(const int32_t *in, int32_t *out, int64_t n,
+                                                     const int32_t *bsum) {
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS], s = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        v[i] = base + i < n ? in[base + i] : 0;
+        s += v[i];
+    }
+    int off = block_excl_scan<256>(s, nullptr) + bsum[blockIdx.x];
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        if (base + i < n) out[base + i] = off;
+        off += v[i];
+    }
+}
+
+int scan_exclusive_i32(dhip_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total_host) {
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    DHIP_HIPCHK(ctx, ctx->scan_tmp.ensure(sizeof(int32_t) * (nb + 2)));
+    int32_t *bsum = ctx->scan_tmp.as<int32_t>();
+    int32_t *tot = bsum + nb + 1;
+    if (nb > 0) {
+        DHIP_LAUNCH(ctx, "scan_reduce", k_scan_reduce, (unsigned)nb, 256, 0, in, n, bsum);
+    }
+    DHIP_LAUNCH(ctx, "scan_top", k_scan_top, 1, 256, 0, bsum, nb, tot);
+    if (nb > 0) {
+        DHIP_LAUNCH(ctx, "scan_final", k_scan_final, (unsigned)nb, 256, 0, in, out, n, bsum);
+    }
+    if (total_host) {
+        int32_t t = 0;
+        DHIP_HIPCHK(ctx, hipMemcpyAsync(&t, tot, sizeof t, hipMemcpyDeviceToHost, ctx->stream));
+        DHIP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        *total_host = t;
+    }
+    return DHIP_OK;
+}
+
+static int check_err_flag(dhip_ctx *ctx) {
+    int32_t f = 0;
+    DHIP_HIPCHK(ctx, hipMemcpyAsync(&f, ctx->errflag.p, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
+    DHIP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (f & 2)
+        return set_error(ctx, DHIP_E_ZERO_PATTERN,
+                         "found a full body in the zero level set, grid is too coarse");
+    if (f & 1)
+        return set_error(ctx, DHIP_E_ZERO_NODE, "levelset is exactly zero at node(s) of a cut tetra");
+    return DHIP_OK;
+}
+
+}  // namespace dhip
+
+using namespace dhip;
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int dhip_create(dhip_ctx_t **out, int32_t device) {
+    if (!out) return DHIP_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DHIP_E_HIP;
+    if (device < 0 || device >= ndev) return DHIP_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return DHIP_E_HIP;
+    if (upload_tables(device) != hipSuccess) return DHIP_E_HIP;
+    dhip_ctx *c = new dhip_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return DHIP_E_HIP;
+    }
+    c->cu_count = prop.multiProcessorCount;
+    if (c->errflag.ensure(256) != hipSuccess || c->bar.ensure(256) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return DHIP_E_NOMEM;
+    }
+    *out = c;
+    return DHIP_OK;
+}
+
+void dhip_destroy(dhip_ctx_t *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) {
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    DevBuf *bufs[] = {&c->phi_own, &c->cellmask, &c->blockcnt, &c->cellinfo, &c->cut, &c->errflag,
+                      &c->loc, &c->flags, &c->first, &c->isown, &c->ownrank, &c->dofs, &c->rowowner,
+                      &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
+                      &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &b : c->kv) b.release();
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *dhip_last_error(const dhip_ctx_t *c) { return c ? c->err : "null context"; }
+
+int dhip_profile_enable(dhip_ctx_t *c, int32_t on) {
+    if (!c) return DHIP_E_ARG;
+    if (!on) {
+        int rc = flush_profile(c);
+        if (rc) return rc;
+    }
+    c->prof = on != 0;
+    return DHIP_OK;
+}
+
+int dhip_profile_get(dhip_ctx_t *c, const char *name, double *total_ms, int64_t *count) {
+    if (!c || !name) return DHIP_E_ARG;
+    int rc = flush_profile(c);
+    if (rc) return rc;
+    auto it = c->prof_acc.find(name);
+    if (total_ms) *total_ms = it == c->prof_acc.end() ? 0.0 : it->second.first;
+    if (count) *count = it == c->prof_acc.end() ? 0 : it->second.second;
+    return DHIP_OK;
+}
+
+int dhip_profile_reset(dhip_ctx_t *c) {
+    if (!c) return DHIP_E_ARG;
+    int rc = flush_profile(c);
+    c->prof_acc.clear();
+    return rc;
+}
+
+int dhip_malloc(dhip_ctx_t *c, void **dptr, int64_t bytes) {
+    if (!c || !dptr || bytes < 0) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DHIP_HIPCHK(c, hipMalloc(dptr, bytes > 0 ? (size_t)bytes : 1));
+    return DHIP_OK;
+}
+
+int dhip_free(dhip_ctx_t *c, void *dptr) {
+    if (!c) return DHIP_E_ARG;
+    if (dptr) DHIP_HIPCHK(c, hipFree(dptr));
+    return DHIP_OK;
+}
+
+int dhip_memcpy(dhip_ctx_t *c, void *dst, const void *src, int64_t bytes, int32_t kind) {
+    if (!c || bytes < 0 || kind < 0 || kind > 2) return DHIP_E_ARG;
+    if (bytes == 0) return DHIP_OK;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                         : hipMemcpyDeviceToDevice;
+    DHIP_HIPCHK(c, hipMemcpyAsync(dst, src, (size_t)bytes, k, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_synchronize(dhip_ctx_t *c) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_mesh_kuhn(dhip_ctx_t *c, const double orig[3], const double e[9], const int32_t n[3]) {
+    if (!c || !orig || !e || !n) return DHIP_E_ARG;
+    for (int k = 0; k < 3; ++k)
+        if (n[k] <= 0) return set_error(c, DHIP_E_ARG, "cell counts must be positive");
+    const int64_t ncell = (int64_t)n[0] * n[1] * n[2];
+    // cut indices are packed as (index << 6 | mask) in 32 bits; half-grid indices fit int64
+    if (ncell * 6 > (int64_t)1 << 31)
+        return set_error(c, DHIP_E_CAPACITY, "mesh too large: %lld tets", (long long)(ncell * 6));
+    MeshP &m = c->mesh;
+    for (int cc = 0; cc < 3; ++cc) {
+        m.orig[cc] = orig[cc];
+        for (int k = 0; k < 3; ++k) {
+            c->e[k][cc] = e[k * 3 + cc];
+            m.off[k][cc] = 1.0 / (double)n[k] * e[k * 3 + cc];  // geom/builder.cpp:108-110
+        }
+        m.n[cc] = n[cc];
+    }
+    m.ncell = ncell;
+    c->nvert = (int64_t)(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+    c->has_mesh = true;
+    c->stage = 0;
+    c->phi = nullptr;
+    c->phi_count = 0;
+    return DHIP_OK;
+}
+
+int dhip_levelset_p2(dhip_ctx_t *c, const double *phi, int64_t count, int32_t where) {
+    if (!c || !phi) return DHIP_E_ARG;
+    if (!c->has_mesh) return set_error(c, DHIP_E_STATE, "dhip_mesh_kuhn must precede dhip_levelset_p2");
+    const int64_t need = (int64_t)(2 * c->mesh.n[0] + 1) * (2 * c->mesh.n[1] + 1) * (2 * c->mesh.n[2] + 1);
+    if (count != need)
+        return set_error(c, DHIP_E_ARG, "level set has %lld values, the half-grid needs %lld",
+                         (long long)count, (long long)need);
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    if (where == DHIP_DEVICE) {
+        c->phi = phi;
+    } else {
+        DHIP_HIPCHK(c, c->phi_own.ensure(sizeof(double) * need));
+        DHIP_HIPCHK(c, hipMemcpyAsync(c->phi_own.p, phi, sizeof(double) * need, hipMemcpyHostToDevice, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->phi = c->phi_own.as<double>();
+    }
+    c->phi_count = need;
+    c->stage = 0;
+    return DHIP_OK;
+}
+
+static int classify_impl(dhip_ctx *c, int32_t zero_policy, int64_t *ncut, bool sync_count) {
+    if (!c->has_mesh || !c->phi) return set_error(c, DHIP_E_STATE, "mesh and level set required");
+    const int64_t ncell = c->mesh.ncell;
+    const unsigned nb = blocks_for(ncell, CLS_BLOCK);
+    DHIP_HIPCHK(c, c->cellmask.ensure(ncell));
+    DHIP_HIPCHK(c, c->blockcnt.ensure(sizeof(int32_t) * (nb + 1)));
+    DHIP_HIPCHK(c, c->cellinfo.ensure(sizeof(uint32_t) * ncell));
+    DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+    ClassArgs ca{c->mesh, c->phi, c->cellmask.as<uint8_t>(), c->blockcnt.as<int32_t>(), c->errflag.as<int32_t>(),
+                 zero_policy == DHIP_ZERO_THROW};
+    DHIP_LAUNCH(c, "classify", k_classify, nb, CLS_BLOCK, 0, ca);
+    int64_t total = 0;
+    int rc = scan_exclusive_i32(c, c->blockcnt.as<int32_t>(), c->blockcnt.as<int32_t>(), nb, &total);
+    if (rc) return rc;
+    (void)sync_count;
+    if (total >= ((int64_t)1 << 26))
+        return set_error(c, DHIP_E_CAPACITY, "%lld cut tets exceed the 2^26 supported", (long long)total);
+    rc = check_err_flag(c);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, c->cut.ensure(sizeof(int64_t) * (total + 1)));
+    CompactArgs pa{ncell, c->cellmask.as<uint8_t>(), c->blockcnt.as<int32_t>(), c->cellinfo.as<uint32_t>(),
+                   c->cut.as<int64_t>()};
+    DHIP_LAUNCH(c, "compact", k_compact, nb, CLS_BLOCK, 0, pa);
+    c->ncut = total;
+    c->zero_policy = zero_policy;
+    c->stage = 1;
+    if (ncut) *ncut = total;
+    return DHIP_OK;
+}
+
+int dhip_classify(dhip_ctx_t *c, int32_t zero_policy, int64_t *ncut) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = classify_impl(c, zero_policy, ncut, true);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_local_p1");
+    const int64_t N = c->ncut;
+    DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC_N * (N + 1)));
+    DHIP_HIPCHK(c, c->flags.ensure(N + 1));
+    for (int k = 0; k < 3; ++k) c->coef[k] = coef ? coef[k] : 1.0;
+    LocalArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->coef[0], c->coef[1], c->coef[2], rhs_kind,
+                 rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
+                 c->errflag.as<int32_t>()};
+    if (N > 0) DHIP_LAUNCH(c, "local_p1", k_local_p1, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la);
+    int rc = check_err_flag(c);
+    if (rc) return rc;
+    c->stage = 2;
+    return DHIP_OK;
+}
+
+int dhip_local_p1(dhip_ctx_t *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = local_impl(c, coef, rhs_kind, rhs_param);
+    if (rc) return rc;
+    return flush_profile(c);
+}
+
+static int number_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
+    if (c->stage < 2) return set_error(c, DHIP_E_STATE, "dhip_local_p1 must precede dhip_number_p1if");
+    const int64_t N = c->ncut, S = 4 * N;
+    DHIP_HIPCHK(c, c->first.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->isown.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->ownrank.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->dofs.ensure(sizeof(int32_t) * (S + 1)));
+    if (N > 0) {
+        QualArgs qa{c->mesh, c->cut.as<int64_t>(), N, c->loc.as<double>(), c->flags.as<uint8_t>(), omit_bound};
+        DHIP_LAUNCH(c, "qualify", k_qualify, blocks_for(N, 256), 256, 0, qa);
+        FirstArgs fa{c->mesh, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->flags.as<uint8_t>(),
+                     c->first.as<int32_t>(), c->isown.as<int32_t>()};
+        DHIP_LAUNCH(c, "first_owner", k_first, blocks_for(S, 256), 256, 0, fa);
+    }
+    int64_t nd = 0;
+    int rc = scan_exclusive_i32(c, c->isown.as<int32_t>(), c->ownrank.as<int32_t>(), S, &nd);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, c->rowowner.ensure(sizeof(int32_t) * (nd + 1)));
+    if (N > 0) {
+        DofArgs da{S, c->first.as<int32_t>(), c->ownrank.as<int32_t>(), c->dofs.as<int32_t>(),
+                   c->rowowner.as<int32_t>()};
+        DHIP_LAUNCH(c, "dofs", k_dofs, blocks_for(S, 256), 256, 0, da);
+    }
+    c->ndof = nd;
+    c->stage = 3;
+    if (ndof) *ndof = nd;
+    return DHIP_OK;
+}
+
+int dhip_number_p1if(dhip_ctx_t *c, double omit_bound, int64_t *ndof) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = number_impl(c, omit_bound, ndof);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+static int build_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
+    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "dhip_number_p1if must precede dhip_build_p1");
+    const int64_t R = c->ndof;
+    DHIP_HIPCHK(c, c->rowlen.ensure(sizeof(int32_t) * (R + 1)));
+    DHIP_HIPCHK(c, c->rowptr.ensure(sizeof(int32_t) * (R + 1)));
+    DHIP_HIPCHK(c, c->rhs.ensure(sizeof(double) * (R + 1)));
+    DHIP_HIPCHK(c, c->diag.ensure(sizeof(double) * (R + 1)));
+    RowArgs ra{c->mesh, c->cut.as<int64_t>(), c->ncut, c->cellinfo.as<uint32_t>(), c->flags.as<uint8_t>(),
+               c->dofs.as<int32_t>(), c->rowowner.as<int32_t>(), R, c->loc.as<double>(), c->rowlen.as<int32_t>(),
+               c->rowptr.as<int32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr, c->rhs.as<double>(),
+               c->diag.as<double>(), lc ? lc[0] : 1.0, lc ? lc[1] : 1.0, lc ? lc[2] : 1.0};
+    if (R > 0) DHIP_LAUNCH(c, "rowcount", k_rowcount, blocks_for(R, 128), 128, 0, ra);
+    int64_t total = 0;
+    int rc = scan_exclusive_i32(c, c->rowlen.as<int32_t>(), c->rowptr.as<int32_t>(), R, &total);
+    if (rc) return rc;
+    {
+        int32_t t32 = (int32_t)total;  // rowptr[R] = nnz
+        DHIP_HIPCHK(c, hipMemcpyAsync(c->rowptr.as<int32_t>() + R, &t32, sizeof t32, hipMemcpyHostToDevice,
+                                      c->stream));
+    }
+    DHIP_HIPCHK(c, c->col.ensure(sizeof(int32_t) * (total + 1)));
+    for (int k = 0; k < 4; ++k) DHIP_HIPCHK(c, c->val[k].ensure(sizeof(double) * (total + 1)));
+    ra.col = c->col.as<int32_t>();
+    ra.vm = c->val[0].as<double>();
+    ra.vl = c->val[1].as<double>();
+    ra.vs = c->val[2].as<double>();
+    ra.vc = c->val[3].as<double>();
+    if (R > 0) DHIP_LAUNCH(c, "rowfill", k_rowfill, blocks_for(R, 128), 128, 0, ra);
+    c->nnz = total;
+    c->stage = 4;
+    if (nnz) *nnz = total;
+    return DHIP_OK;
+}
+
+int dhip_build_p1(dhip_ctx_t *c, const double lc[3], int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = build_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_assemble_p1(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                     const double *rhs_param, double omit_bound, const double lc[3], int64_t *ncut,
+                     int64_t *ndof, int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = classify_impl(c, zero_policy, ncut, false);
+    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param);
+    if (!rc) rc = number_impl(c, omit_bound, ndof);
+    if (!rc) rc = build_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+static int d2h(dhip_ctx *c, void *dst, const void *src, size_t bytes) {
+    if (!dst || bytes == 0) return DHIP_OK;
+    DHIP_HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    return DHIP_OK;
+}
+
+int dhip_get_cut_tets(dhip_ctx_t *c, int64_t *tets) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "no classification");
+    int rc = d2h(c, tets, c->cut.p, sizeof(int64_t) * c->ncut);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_get_local(dhip_ctx_t *c, double *local57, uint8_t *flags) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 2) return set_error(c, DHIP_E_STATE, "no local matrices");
+    const int64_t N = c->ncut;
+    if (local57 && N > 0) {
+        std::vector<double> soa((size_t)LOC_N * N);
+        int rc = d2h(c, soa.data(), c->loc.p, sizeof(double) * LOC_N * N);
+        if (rc) return rc;
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < N; ++i)
+            for (int e = 0; e < LOC_N; ++e) local57[i * LOC_N + e] = soa[(size_t)e * N + i];
+    }
+    int rc = d2h(c, flags, c->flags.p, N);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_get_tet_dofs(dhip_ctx_t *c, int32_t *dofs) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "no numbering");
+    int rc = d2h(c, dofs, c->dofs.p, sizeof(int32_t) * 4 * c->ncut);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_get_dof_vertices(dhip_ctx_t *c, int64_t *vertex_of_dof) {
+    if (!c || !vertex_of_dof) return DHIP_E_ARG;
+    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "no numbering");
+    std::vector<int32_t> owner(c->ndof);
+    std::vector<int64_t> cut(c->ncut);
+    int rc = d2h(c, owner.data(), c->rowowner.p, sizeof(int32_t) * c->ndof);
+    if (!rc) rc = d2h(c, cut.data(), c->cut.p, sizeof(int64_t) * c->ncut);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Tables &T = host_tables();
+    const MeshP &m = c->mesh;
+    for (int64_t r = 0; r < c->ndof; ++r) {
+        const int64_t s = owner[r], t = cut[s >> 2];
+        const int k = (int)(t % 6), i = (int)(s & 3);
+        const int64_t cell = t / 6;
+        const int64_t i1 = cell % m.n[0] + T.kuhn[k][i][0];
+        const int64_t i2 = (cell / m.n[0]) % m.n[1] + T.kuhn[k][i][1];
+        const int64_t i3 = cell / ((int64_t)m.n[0] * m.n[1]) + T.kuhn[k][i][2];
+        vertex_of_dof[r] = (i3 * (m.n[1] + 1) + i2) * (m.n[0] + 1) + i1;
+    }
+    return DHIP_OK;
+}
+
+int dhip_get_csr(dhip_ctx_t *c, int32_t *rowptr, int32_t *colind, double *mass, double *lb, double *stab,
+                 double *comb, double *rhs) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 4) return set_error(c, DHIP_E_STATE, "no assembled matrix");
+    const size_t nz = (size_t)c->nnz, R = (size_t)c->ndof;
+    int rc = 0;
+    rc = rc ? rc : d2h(c, rowptr, c->rowptr.p, sizeof(int32_t) * (R + 1));
+    rc = rc ? rc : d2h(c, colind, c->col.p, sizeof(int32_t) * nz);
+    rc = rc ? rc : d2h(c, mass, c->val[0].p, sizeof(double) * nz);
+    rc = rc ? rc : d2h(c, lb, c->val[1].p, sizeof(double) * nz);
+    rc = rc ? rc : d2h(c, stab, c->val[2].p, sizeof(double) * nz);
+    rc = rc ? rc : d2h(c, comb, c->val[3].p, sizeof(double) * nz);
+    rc = rc ? rc : d2h(c, rhs, c->rhs.p, sizeof(double) * R);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_device_arrays(dhip_ctx_t *c, const int32_t **rowptr, const int32_t **colind, const double **comb,
+                       const double **rhs) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 4) return set_error(c, DHIP_E_STATE, "no assembled matrix");
+    if (rowptr) *rowptr = c->rowptr.as<int32_t>();
+    if (colind) *colind = c->col.as<int32_t>();
+    if (comb) *comb = c->val[3].as<double>();
+    if (rhs) *rhs = c->rhs.as<double>();
+    return DHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR objects
+// ---------------------------------------------------------------------------
+static int csr_finish(dhip_ctx *c, dhip_csr *A) {
+    DHIP_HIPCHK(c, A->diag.ensure(sizeof(double) * (A->n + 1)));
+    DHIP_HIPCHK(c, A->diag32.ensure(sizeof(float) * (A->n + 1)));
+    DHIP_HIPCHK(c, A->val32.ensure(sizeof(float) * (A->nnz + 1)));
+    if (A->n > 0)
+        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(A->n, KB), KB, 0, A->n, A->rowptr, A->col, A->val,
+                    A->diag.as<double>(), A->diag32.as<float>());
+    if (A->nnz > 0)
+        DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, A->val32.as<float>());
+    A->has32 = true;
+    return DHIP_OK;
+}
+
+int dhip_csr_create(dhip_ctx_t *c, int64_t nrows, int64_t nnz, const int32_t *rowptr, const int32_t *colind,
+                    const double *val, int32_t where, dhip_csr_t **out) {
+    if (!c || !out || nrows < 0 || nnz < 0 || !rowptr || (nnz > 0 && (!colind || !val))) return DHIP_E_ARG;
+    if (nnz >= ((int64_t)1 << 31)) return set_error(c, DHIP_E_CAPACITY, "nnz beyond int32");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    dhip_csr *A = new dhip_csr();
+    A->ctx = c;
+    A->n = nrows;
+    A->nnz = nnz;
+    auto fail = [&](int rc) { delete A; return rc; };
+    if (A->own_rowptr.ensure(sizeof(int32_t) * (nrows + 1)) || A->own_col.ensure(sizeof(int32_t) * (nnz + 1)) ||
+        A->own_val.ensure(sizeof(double) * (nnz + 1)))
+        return fail(set_error(c, DHIP_E_NOMEM, "csr allocation failed"));
+    const hipMemcpyKind k = where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (hipMemcpyAsync(A->own_rowptr.p, rowptr, sizeof(int32_t) * (nrows + 1), k, c->stream) ||
+        (nnz > 0 && (hipMemcpyAsync(A->own_col.p, colind, sizeof(int32_t) * nnz, k, c->stream) ||
+                     hipMemcpyAsync(A->own_val.p, val, sizeof(double) * nnz, k, c->stream))))
+        return fail(set_error(c, DHIP_E_HIP, "csr upload failed"));
+    A->rowptr = A->own_rowptr.as<int32_t>();
+    A->col = A->own_col.as<int32_t>();
+    A->val = A->own_val.as<double>();
+    int rc = csr_finish(c, A);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_error(c, DHIP_E_HIP, "sync failed");
+    if (rc) return fail(rc);
+    *out = A;
+    return flush_profile(c);
+}
+
+int dhip_csr_from_assembly(dhip_ctx_t *c, int32_t op, dhip_csr_t **out) {
+    if (!c || !out || op < 0 || op > 3) return DHIP_E_ARG;
+    if (c->stage < 4) return set_error(c, DHIP_E_STATE, "no assembled matrix");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    dhip_csr *A = new dhip_csr();
+    A->ctx = c;
+    A->n = c->ndof;
+    A->nnz = c->nnz;
+    A->rowptr = c->rowptr.as<int32_t>();
+    A->col = c->col.as<int32_t>();
+    A->val = c->val[op].as<double>();
+    int rc = csr_finish(c, A);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = set_error(c, DHIP_E_HIP, "sync failed");
+    if (rc) {
+        delete A;
+        return rc;
+    }
+    *out = A;
+    return flush_profile(c);
+}
+
+void dhip_csr_destroy(dhip_csr_t *A) {
+    if (!A) return;
+    (void)hipSetDevice(A->ctx->device);
+    (void)hipStreamSynchronize(A->ctx->stream);
+    A->own_rowptr.release();
+    A->own_col.release();
+    A->own_val.release();
+    A->diag.release();
+    A->diag32.release();
+    A->val32.release();
+    delete A;
+}
+
+int dhip_csr_info(const dhip_csr_t *A, int64_t *nrows, int64_t *nnz) {
+    if (!A) return DHIP_E_ARG;
+    if (nrows) *nrows = A->n;
+    if (nnz) *nnz = A->nnz;
+    return DHIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// vector core
+// ---------------------------------------------------------------------------
+// Stage caller vectors on the device (host pointers) or use them directly.
+struct VecStage {
+    dhip_ctx *c;
+    DevBuf buf;
+    ~VecStage() { buf.release(); }
+};
+
+static int stage_in(dhip_ctx *c, DevBuf &buf, const double *src, int64_t n, int32_t where, const double **dst) {
+    if (where == DHIP_DEVICE) {
+        *dst = src;
+        return DHIP_OK;
+    }
+    DHIP_HIPCHK(c, buf.ensure(sizeof(double) * (n + 1)));
+    if (n > 0) DHIP_HIPCHK(c, hipMemcpyAsync(buf.p, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    *dst = buf.as<double>();
+    return DHIP_OK;
+}
+
+int dhip_spmv(dhip_ctx_t *c, const dhip_csr_t *A, const double *x, double *y, int32_t where) {
+    if (!c || !A || !x || !y) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DevBuf bx, by;
+    const double *dx;
+    int rc = stage_in(c, bx, x, A->n, where, &dx);
+    if (rc) return rc;
+    double *dy = y;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, by.ensure(sizeof(double) * (A->n + 1)));
+        dy = by.as<double>();
+    }
+    if (A->n > 0)
+        DHIP_LAUNCH(c, "spmv", k_spmv, blocks_for(A->n * SPMV_G, KB), KB, 0, A->n, A->rowptr, A->col, A->val, dx, dy);
+    if (where != DHIP_DEVICE && A->n > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * A->n, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    bx.release();
+    by.release();
+    return flush_profile(c);
+}
+
+int dhip_dot(dhip_ctx_t *c, int64_t n, const double *x, const double *y, double *result, int32_t where) {
+    if (!c || !x || !y || !result || n < 0) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DevBuf bx, by;
+    const double *dx, *dy;
+    int rc = stage_in(c, bx, x, n, where, &dx);
+    if (!rc) rc = stage_in(c, by, y, n, where, &dy);
+    if (rc) return rc;
+    const unsigned nb = std::max(1u, std::min(1024u, blocks_for(n, KB)));
+    DHIP_HIPCHK(c, c->red_tmp.ensure(sizeof(double) * (nb + 2)));
+    double *part = c->red_tmp.as<double>();
+    DHIP_LAUNCH(c, "dot_partial", k_dot_partial, nb, KB, 0, n, dx, dy, part);
+    DHIP_LAUNCH(c, "dot_reduce", k_reduce, 1, KB, 0, part, (int)nb, part + nb);
+    DHIP_HIPCHK(c, hipMemcpyAsync(result, part + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    bx.release();
+    by.release();
+    return flush_profile(c);
+}
+
+int dhip_axpy(dhip_ctx_t *c, int64_t n, double a, const double *x, double *y, int32_t where) {
+    if (!c || !x || !y || n < 0) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DevBuf bx, by;
+    const double *dx;
+    int rc = stage_in(c, bx, x, n, where, &dx);
+    if (rc) return rc;
+    double *dy = y;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, by.ensure(sizeof(double) * (n + 1)));
+        if (n > 0) DHIP_HIPCHK(c, hipMemcpyAsync(by.p, y, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        dy = by.as<double>();
+    }
+    if (n > 0) DHIP_LAUNCH(c, "axpy", k_axpy, blocks_for(n, KB), KB, 0, n, a, dx, dy);
+    if (where != DHIP_DEVICE && n > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    bx.release();
+    by.release();
+    return flush_profile(c);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov solvers
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+template <class K>
+static int coop_blocks(dhip_ctx *c, K kernel, int64_t n, unsigned *nb) {
+    int per_cu = 0;
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, KB, 0));
+    if (per_cu < 1) return set_error(c, DHIP_E_HIP, "persistent kernel cannot be resident");
+    const int64_t want = std::max<int64_t>(1, (n + 127) / 128);
+    const int64_t cap = std::min<int64_t>((int64_t)per_cu * c->cu_count, c->cu_count);
+    *nb = (unsigned)std::min(want, cap);
+    return DHIP_OK;
+}
+
+extern "C" {
+
+int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
+             int32_t *max_iter, double *tol, int32_t rel, int32_t precision) {
+    if (!c || !A || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
+    if (pc != DHIP_PC_NONE && pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "unknown preconditioner");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    DevBuf bb, bx;
+    const double *db;
+    int rc = stage_in(c, bb, b, n, where, &db);
+    if (rc) return rc;
+    double *dx = x;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, bx.ensure(sizeof(double) * (n + 1)));
+        if (n > 0) DHIP_HIPCHK(c, hipMemcpyAsync(bx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        dx = bx.as<double>();
+    }
+    for (int k = 0; k < 6; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(double) * (n + 1)));
+    DHIP_HIPCHK(c, c->red_tmp.ensure(sizeof(double) * (4 * 1024 + 16)));
+    DHIP_HIPCHK(c, c->host_stage.ensure(64));
+    int32_t *outs = c->host_stage.as<int32_t>();
+    double *outd = reinterpret_cast<double *>(outs + 4);
+    DHIP_LAUNCH(c, "zero_bar", k_zero_u32, 1, 64, 0, c->bar.as<uint32_t>(), 16);
+    int h_iter = 0, h_status = 1, h_outer = 0;
+    double h_resid = 0.0;
+    if (precision == DHIP_FP64) {
+        unsigned nb = 1;
+        rc = coop_blocks(c, k_pcg64, n, &nb);
+        if (rc) return rc;
+        PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? A->diag.as<double>() : nullptr, db, dx,
+                   c->kv[0].as<double>(), c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(),
+                   c->kv[4].as<double>(), c->red_tmp.as<double>(), c->bar.as<GridBar>(), *max_iter, *tol, rel,
+                   outs, outd, outs + 1};
+        void *args[] = {&pa};
+        {
+            ProfScope ps(c, "pcg64");
+            DHIP_HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_pcg64, dim3(nb), dim3(KB), args, 0, c->stream));
+        }
+    } else if (precision == DHIP_FP32_IR) {
+        if (pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "FP32 refinement path uses the Jacobi preconditioner");
+        for (int k = 6; k < 12; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(float) * (n + 1)));
+        unsigned nb = 1;
+        rc = coop_blocks(c, k_pcg_mixed, n, &nb);
+        if (rc) return rc;
+        PcgMixArgs pa{n, A->rowptr, A->col, A->val, A->val32.as<float>(), A->diag32.as<float>(), db, dx,
+                      c->kv[0].as<double>(), c->kv[6].as<float>(), c->kv[7].as<float>(), c->kv[8].as<float>(),
+                      c->kv[9].as<float>(), c->kv[10].as<float>(), c->kv[11].as<float>(), c->red_tmp.as<double>(),
+                      c->bar.as<GridBar>(), *max_iter, *tol, rel, 1e-4, 400, outs, outd, outs + 1, outs + 2};
+        void *args[] = {&pa};
+        {
+            ProfScope ps(c, "pcg_mixed");
+            DHIP_HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_pcg_mixed, dim3(nb), dim3(KB), args, 0,
+                                                      c->stream));
+        }
+    } else {
+        return set_error(c, DHIP_E_ARG, "unknown precision");
+    }
+    int32_t hs[4];
+    DHIP_HIPCHK(c, hipMemcpyAsync(hs, outs, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipMemcpyAsync(&h_resid, outd, sizeof h_resid, hipMemcpyDeviceToHost, c->stream));
+    if (where != DHIP_DEVICE && n > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    h_iter = hs[0];
+    h_status = hs[1];
+    h_outer = hs[2];
+    (void)h_outer;
+    bb.release();
+    bx.release();
+    rc = flush_profile(c);
+    if (rc) return rc;
+    *max_iter = h_iter;
+    *tol = h_resid;
+    if (h_status == 2) return set_error(c, DHIP_E_HIP, "grid barrier timed out in the persistent solver");
+    if (h_status != 0) return set_error(c, DHIP_E_NOT_CONVERGED, "PCG did not converge: resid %g after %d", h_resid, h_iter);
+    return DHIP_OK;
+}
+
+int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
+               int32_t restart, int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
+    (void)A; (void)pc; (void)b; (void)x; (void)where; (void)restart; (void)max_iter; (void)tol; (void)rel;
+    (void)left_pc;
+    if (!c) return DHIP_E_ARG;
+    return set_error(c, DHIP_E_STATE, "GMRES not available in this build");
+}
+
+}  // extern "C"

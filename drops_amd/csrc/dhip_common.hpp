@@ -1,0 +1,230 @@
+// dhip_common.hpp -- context, device tables, launch/profiling helpers and
+// device-wide scans shared by the assembly and Krylov parts of libdhip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dhip.h"
+
+namespace dhip {
+
+// ---------------------------------------------------------------------------
+// Device tables (filled once per process from host-side derivations)
+// ---------------------------------------------------------------------------
+struct Tables {
+    int8_t kuhn[6][4][3];      // corner offsets of the Kuhn tets (d1,d2,d3)
+    int8_t lat_b2[10][4];      // lattice-2 vertices, barycentric x 2
+    int8_t lat_t[8][4];        // lattice-2 sub-tets
+    int8_t sub_edge[8][6];     // lattice edge id of sub-tet local edge
+    int8_t edge_ab[25][2];     // lattice edge endpoints in first-appearance orientation
+    int8_t rp_size[81];        // triangles per sign pattern
+    int8_t rp_tri[81][2][3];   // triangle roots: 0-3 zero vertex, 4-9 cut edge + 4
+    int8_t rp_bnd[81];         // facet lies in a face of the sub-tet
+    int8_t lat_of_p2[10];      // lattice vertex of P2 dof (vertices, edges)
+    int8_t inc[8][6];          // local index of corner delta in Kuhn tet k (-1 if absent)
+    int8_t slot[6][4][4];      // neighbour slot 0..14 of (k, row vertex, col vertex)
+    int8_t tet_node27[6][10];  // 3x3x3 cell-node index of lattice vertex l of tet k
+    uint32_t tetmask27[6];     // the 10 cell nodes of tet k as a 27-bit mask
+    int8_t slot_delta[15][3];  // neighbour offset of each slot
+    double lat_b[10][4];
+    double q2n[4][3], q2w[4];  // Quad2_2DDataCL
+    double q5n[7][3], q5w[7];  // Quad5_2DDataCL
+    double v5n[15][4];         // Quad5DataCL nodes
+    double v5c1, v5c2;         // Quad5CL::quad weights of node groups 1 and 2
+};
+
+extern __constant__ Tables c_tab;
+void init_tables_once(int device);  // derives the tables on the host and uploads them
+const Tables &host_tables();
+
+// ---------------------------------------------------------------------------
+// Mesh parameters passed by value to kernels
+// ---------------------------------------------------------------------------
+struct MeshP {
+    double orig[3];
+    double off[3][3];  // off[k][c] = 1/n_k * e_k[c]
+    int32_t n[3];
+    int64_t ncell;
+};
+
+// ---------------------------------------------------------------------------
+// Device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b) {
+        if (b <= bytes && p) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t nb = b < 256 ? 256 : b;
+        hipError_t e = hipMalloc(&p, nb);
+        if (e == hipSuccess) bytes = nb;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+}  // namespace dhip
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct dhip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char err[512] = {0};
+    int cu_count = 0;
+
+    // profiling: per-kernel accumulated milliseconds / launches
+    bool prof = false;
+    struct Pending { std::string name; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<double, int64_t>> prof_acc;
+
+    // mesh
+    bool has_mesh = false;
+    dhip::MeshP mesh{};
+    double e[3][3] = {{0}};
+    int64_t nvert = 0;
+
+    // level set
+    const double *phi = nullptr;
+    int64_t phi_count = 0;
+    dhip::DevBuf phi_own;
+
+    // assembly state: 0 none, 1 classified, 2 local, 3 numbered, 4 built
+    int stage = 0;
+    int zero_policy = DHIP_ZERO_THROW;
+    int64_t ncut = 0, ndof = 0, nnz = 0;
+    double coef[3] = {1, 1, 1};
+    dhip::DevBuf cellmask, blockcnt, cellinfo, cut, errflag;
+    dhip::DevBuf loc, flags;
+    dhip::DevBuf first, isown, ownrank, dofs, rowowner;
+    dhip::DevBuf rowlen, rowptr, col, val[4], rhs, diag;
+    dhip::DevBuf scan_tmp, red_tmp, bar;
+    dhip::DevBuf host_stage;  // pinned staging for scalars
+    // Krylov work vectors
+    dhip::DevBuf kv[12];
+};
+
+struct dhip_csr {
+    dhip_ctx *ctx = nullptr;
+    int64_t n = 0, nnz = 0;
+    const int32_t *rowptr = nullptr, *col = nullptr;
+    const double *val = nullptr;
+    dhip::DevBuf own_rowptr, own_col, own_val;
+    dhip::DevBuf diag;    // diagonal found by a row scan (SolveGSstep<PB_JAC0>)
+    dhip::DevBuf val32;   // FP32 copy of the values (mixed precision)
+    dhip::DevBuf diag32;
+    bool has32 = false;
+};
+
+namespace dhip {
+
+int set_error(dhip_ctx *ctx, int code, const char *fmt, ...);
+
+#define DHIP_HIPCHK(ctx, expr)                                                                     \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return ::dhip::set_error((ctx), DHIP_E_HIP, "%s failed: %s (%s:%d)", #expr,            \
+                                     hipGetErrorString(_e), __FILE__, __LINE__);                   \
+    } while (0)
+
+// Launch with optional HIP-event timing on the context stream.
+struct ProfScope {
+    dhip_ctx *ctx;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(dhip_ctx *c, const char *n);
+    ~ProfScope();
+};
+hipError_t take_event(dhip_ctx *ctx, hipEvent_t *ev);
+int flush_profile(dhip_ctx *ctx);
+
+#define DHIP_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                                    \
+    do {                                                                                           \
+        ::dhip::ProfScope _ps((ctx), (name));                                                      \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (shmem), (ctx)->stream, __VA_ARGS__);  \
+        DHIP_HIPCHK((ctx), hipGetLastError());                                                     \
+    } while (0)
+
+inline unsigned blocks_for(int64_t n, int per_block) {
+    return (unsigned)((n + per_block - 1) / per_block);
+}
+
+// ---------------------------------------------------------------------------
+// Wave / block primitives (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+__device__ inline int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+template <int BLOCK>
+__device__ inline int block_excl_scan(int v, int *total) {
+    __shared__ int wsum[BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        if (i < w) off += wsum[i];
+        tot += wsum[i];
+    }
+    __syncthreads();
+    if (total) *total = tot;
+    return off + inc - v;
+}
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int BLOCK>
+__device__ inline double block_sum(double v) {
+    __shared__ double ws[BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) s += ws[i];
+    __syncthreads();
+    return s;
+}
+
+// Exclusive scan of n int32 values (out may alias in); *total_host receives the sum.
+int scan_exclusive_i32(dhip_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total_host);
+
+}  // namespace dhip

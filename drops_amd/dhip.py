@@ -1,0 +1,338 @@
+"""ctypes binding of libdhip (include/dhip.h).
+
+The GPU library is the only compute path: if libdhip.so is missing or cannot
+be loaded, every entry point raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdhip.so")
+
+OK, E_ZERO_NODE, E_ZERO_PATTERN, E_ARG, E_NOT_CONVERGED, E_NOMEM, E_HIP, E_STATE, E_CAPACITY = range(9)
+HOST, DEVICE = 0, 1
+ZERO_THROW, ZERO_ALLOW = 0, 1
+OP_MASS, OP_LB, OP_STAB, OP_COMB = range(4)
+RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST = range(3)
+PC_NONE, PC_JACOBI = 0, 1
+FP64, FP32_IR = 0, 1
+LOC_N = 57
+
+# every symbol declared in include/dhip.h
+EXPORTS = [
+    "dhip_create", "dhip_destroy", "dhip_last_error", "dhip_profile_enable", "dhip_profile_get",
+    "dhip_profile_reset", "dhip_malloc", "dhip_free", "dhip_memcpy", "dhip_synchronize",
+    "dhip_mesh_kuhn", "dhip_levelset_p2", "dhip_classify", "dhip_local_p1", "dhip_number_p1if",
+    "dhip_build_p1", "dhip_assemble_p1", "dhip_get_cut_tets", "dhip_get_local", "dhip_get_tet_dofs",
+    "dhip_get_dof_vertices", "dhip_get_csr", "dhip_device_arrays", "dhip_csr_create",
+    "dhip_csr_from_assembly", "dhip_csr_destroy", "dhip_csr_info", "dhip_spmv", "dhip_dot",
+    "dhip_axpy", "dhip_pcg", "dhip_gmres",
+]
+
+
+class DhipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dhip error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    P, D, I32, I64, VP = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.c_void_p
+    L.dhip_create.argtypes = [C.POINTER(P), C.c_int32]
+    L.dhip_destroy.argtypes = [P]
+    L.dhip_destroy.restype = None
+    L.dhip_last_error.argtypes = [P]
+    L.dhip_last_error.restype = C.c_char_p
+    L.dhip_profile_enable.argtypes = [P, C.c_int32]
+    L.dhip_profile_get.argtypes = [P, C.c_char_p, D, I64]
+    L.dhip_profile_reset.argtypes = [P]
+    L.dhip_malloc.argtypes = [P, C.POINTER(VP), C.c_int64]
+    L.dhip_free.argtypes = [P, VP]
+    L.dhip_memcpy.argtypes = [P, VP, VP, C.c_int64, C.c_int32]
+    L.dhip_synchronize.argtypes = [P]
+    L.dhip_mesh_kuhn.argtypes = [P, D, D, I32]
+    L.dhip_levelset_p2.argtypes = [P, VP, C.c_int64, C.c_int32]
+    L.dhip_classify.argtypes = [P, C.c_int32, I64]
+    L.dhip_local_p1.argtypes = [P, D, C.c_int32, D]
+    L.dhip_number_p1if.argtypes = [P, C.c_double, I64]
+    L.dhip_build_p1.argtypes = [P, D, I64]
+    L.dhip_assemble_p1.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
+    L.dhip_get_cut_tets.argtypes = [P, I64]
+    L.dhip_get_local.argtypes = [P, D, C.POINTER(C.c_uint8)]
+    L.dhip_get_tet_dofs.argtypes = [P, I32]
+    L.dhip_get_dof_vertices.argtypes = [P, I64]
+    L.dhip_get_csr.argtypes = [P, I32, I32, D, D, D, D, D]
+    L.dhip_device_arrays.argtypes = [P, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(VP)]
+    L.dhip_csr_create.argtypes = [P, C.c_int64, C.c_int64, VP, VP, VP, C.c_int32, C.POINTER(P)]
+    L.dhip_csr_from_assembly.argtypes = [P, C.c_int32, C.POINTER(P)]
+    L.dhip_csr_destroy.argtypes = [P]
+    L.dhip_csr_destroy.restype = None
+    L.dhip_csr_info.argtypes = [P, I64, I64]
+    L.dhip_spmv.argtypes = [P, P, VP, VP, C.c_int32]
+    L.dhip_dot.argtypes = [P, C.c_int64, VP, VP, D, C.c_int32]
+    L.dhip_axpy.argtypes = [P, C.c_int64, C.c_double, VP, VP, C.c_int32]
+    L.dhip_pcg.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    L.dhip_gmres.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    _lib = L
+    return L
+
+
+def _p(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+def _vp(a):
+    if a is None:
+        return None
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    return C.c_void_p(a.ctypes.data)
+
+
+class Csr:
+    """Device CSR matrix handle (MatrixCL mirror)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self.h = ctx, handle
+
+    @property
+    def shape(self):
+        n, z = C.c_int64(), C.c_int64()
+        self.ctx.L.dhip_csr_info(self.h, C.byref(n), C.byref(z))
+        return n.value, z.value
+
+    def close(self):
+        if self.h:
+            self.ctx.L.dhip_csr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    def __init__(self, device=0):
+        self.L = load()
+        self.h = C.c_void_p()
+        rc = self.L.dhip_create(C.byref(self.h), device)
+        if rc != OK:
+            raise DhipError(rc, "dhip_create failed (no usable GPU?)")
+
+    def check(self, rc):
+        if rc != OK:
+            raise DhipError(rc, self.L.dhip_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if self.h:
+            self.L.dhip_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- profiling / memory ----
+    def profile(self, on):
+        self.check(self.L.dhip_profile_enable(self.h, int(on)))
+
+    def profile_get(self, name):
+        ms, cnt = C.c_double(), C.c_int64()
+        self.check(self.L.dhip_profile_get(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def profile_reset(self):
+        self.check(self.L.dhip_profile_reset(self.h))
+
+    def malloc(self, nbytes):
+        p = C.c_void_p()
+        self.check(self.L.dhip_malloc(self.h, C.byref(p), int(nbytes)))
+        return p.value
+
+    def free(self, p):
+        self.check(self.L.dhip_free(self.h, C.c_void_p(p)))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.malloc(arr.nbytes)
+        self.check(self.L.dhip_memcpy(self.h, C.c_void_p(p), _vp(arr), arr.nbytes, 0))
+        return p
+
+    def to_host(self, p, shape, dtype):
+        out = np.empty(shape, dtype)
+        self.check(self.L.dhip_memcpy(self.h, _vp(out), C.c_void_p(p), out.nbytes, 1))
+        return out
+
+    def copy_d2d(self, dst, src, nbytes):
+        self.check(self.L.dhip_memcpy(self.h, C.c_void_p(dst), C.c_void_p(src), int(nbytes), 2))
+
+    def synchronize(self):
+        self.check(self.L.dhip_synchronize(self.h))
+
+    # ---- mesh / level set ----
+    def mesh(self, mesh):
+        orig = np.asarray(mesh.orig, np.float64)
+        e = np.asarray(mesh.e, np.float64).reshape(9)
+        n = np.asarray(mesh.n, np.int32)
+        self.check(self.L.dhip_mesh_kuhn(self.h, _p(orig, C.c_double), _p(e, C.c_double), _p(n, C.c_int32)))
+        self._mesh = mesh
+
+    def levelset(self, phi=None, device_ptr=None, count=None):
+        if device_ptr is not None:
+            self.check(self.L.dhip_levelset_p2(self.h, C.c_void_p(device_ptr), int(count), DEVICE))
+        else:
+            phi = np.ascontiguousarray(phi, np.float64)
+            self.check(self.L.dhip_levelset_p2(self.h, _vp(phi), phi.size, HOST))
+
+    # ---- assembly ----
+    def classify(self, zero_policy=ZERO_THROW):
+        n = C.c_int64()
+        self.check(self.L.dhip_classify(self.h, zero_policy, C.byref(n)))
+        self.ncut = n.value
+        return n.value
+
+    def local_p1(self, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None):
+        co = np.asarray(coef, np.float64)
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        self.check(self.L.dhip_local_p1(self.h, _p(co, C.c_double), rhs_kind, _p(rp, C.c_double)))
+
+    def number_p1if(self, omit_bound=-1.0):
+        n = C.c_int64()
+        self.check(self.L.dhip_number_p1if(self.h, omit_bound, C.byref(n)))
+        self.ndof = n.value
+        return n.value
+
+    def build_p1(self, lc=(1.0, 1.0, 1.0)):
+        lcv = np.asarray(lc, np.float64)
+        n = C.c_int64()
+        self.check(self.L.dhip_build_p1(self.h, _p(lcv, C.c_double), C.byref(n)))
+        self.nnz = n.value
+        return n.value
+
+    def assemble_p1(self, zero_policy=ZERO_THROW, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+                    omit_bound=-1.0, lc=(1.0, 1.0, 1.0)):
+        co = np.asarray(coef, np.float64)
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        lcv = np.asarray(lc, np.float64)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(self.L.dhip_assemble_p1(self.h, zero_policy, _p(co, C.c_double), rhs_kind, _p(rp, C.c_double),
+                                           omit_bound, _p(lcv, C.c_double), C.byref(a), C.byref(b), C.byref(c)))
+        self.ncut, self.ndof, self.nnz = a.value, b.value, c.value
+        return a.value, b.value, c.value
+
+    def cut_tets(self):
+        out = np.zeros(max(self.ncut, 1), np.int64)
+        self.check(self.L.dhip_get_cut_tets(self.h, _p(out, C.c_int64)))
+        return out[: self.ncut]
+
+    def local(self):
+        loc = np.zeros((max(self.ncut, 1), LOC_N))
+        fl = np.zeros(max(self.ncut, 1), np.uint8)
+        self.check(self.L.dhip_get_local(self.h, _p(loc, C.c_double), _p(fl, C.c_uint8)))
+        return loc[: self.ncut], fl[: self.ncut]
+
+    def tet_dofs(self):
+        out = np.zeros((max(self.ncut, 1), 4), np.int32)
+        self.check(self.L.dhip_get_tet_dofs(self.h, _p(out, C.c_int32)))
+        return out[: self.ncut]
+
+    def dof_vertices(self):
+        out = np.zeros(max(self.ndof, 1), np.int64)
+        self.check(self.L.dhip_get_dof_vertices(self.h, _p(out, C.c_int64)))
+        return out[: self.ndof]
+
+    def csr(self):
+        n, z = self.ndof, self.nnz
+        rp = np.zeros(n + 1, np.int32)
+        col = np.zeros(max(z, 1), np.int32)
+        vals = [np.zeros(max(z, 1)) for _ in range(4)]
+        rhs = np.zeros(max(n, 1))
+        self.check(self.L.dhip_get_csr(self.h, _p(rp, C.c_int32), _p(col, C.c_int32),
+                                       *[_p(v, C.c_double) for v in vals], _p(rhs, C.c_double)))
+        return dict(rowptr=rp, colind=col[:z], mass=vals[0][:z], lb=vals[1][:z], stab=vals[2][:z],
+                    comb=vals[3][:z], rhs=rhs[:n])
+
+    def device_arrays(self):
+        ps = [C.c_void_p() for _ in range(4)]
+        self.check(self.L.dhip_device_arrays(self.h, *[C.byref(p) for p in ps]))
+        return [p.value for p in ps]
+
+    # ---- matrices / vectors ----
+    def csr_create(self, rowptr, colind, val):
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        colind = np.ascontiguousarray(colind, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        h = C.c_void_p()
+        self.check(self.L.dhip_csr_create(self.h, rowptr.size - 1, colind.size, _vp(rowptr), _vp(colind), _vp(val),
+                                          HOST, C.byref(h)))
+        return Csr(self, h)
+
+    def csr_from_assembly(self, op=OP_COMB):
+        h = C.c_void_p()
+        self.check(self.L.dhip_csr_from_assembly(self.h, op, C.byref(h)))
+        return Csr(self, h)
+
+    def spmv(self, A, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(A.shape[0])
+        self.check(self.L.dhip_spmv(self.h, A.h, _vp(x), _vp(y), HOST))
+        return y
+
+    def dot(self, x, y):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y, np.float64)
+        r = C.c_double()
+        self.check(self.L.dhip_dot(self.h, x.size, _vp(x), _vp(y), C.byref(r), HOST))
+        return r.value
+
+    def axpy(self, a, x, y):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.array(y, np.float64)
+        self.check(self.L.dhip_axpy(self.h, x.size, a, _vp(x), _vp(y), HOST))
+        return y
+
+    def pcg(self, A, b, x0=None, pc=PC_JACOBI, max_iter=10000, tol=1e-10, rel=True, precision=FP64,
+            where=HOST, raise_on_fail=True):
+        """Returns (x, iterations, residual, converged).  With where=DEVICE, b and x0 are
+        device pointers and x0 is overwritten with the solution."""
+        it = C.c_int32(max_iter)
+        t = C.c_double(tol)
+        if where == DEVICE:
+            rc = self.L.dhip_pcg(self.h, A.h, pc, C.c_void_p(b), C.c_void_p(x0), DEVICE, C.byref(it), C.byref(t),
+                                 int(rel), precision)
+            x = None
+        else:
+            b = np.ascontiguousarray(b, np.float64)
+            x = np.zeros(b.size) if x0 is None else np.array(x0, np.float64)
+            rc = self.L.dhip_pcg(self.h, A.h, pc, _vp(b), _vp(x), HOST, C.byref(it), C.byref(t), int(rel), precision)
+        if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
+            self.check(rc)
+        return x, it.value, t.value, rc == OK
+
+    def gmres(self, A, b, x0=None, pc=PC_JACOBI, restart=50, max_iter=10000, tol=1e-10, rel=True, left_pc=False,
+              raise_on_fail=True):
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(b.size) if x0 is None else np.array(x0, np.float64)
+        it = C.c_int32(max_iter)
+        t = C.c_double(tol)
+        rc = self.L.dhip_gmres(self.h, A.h, pc, _vp(b), _vp(x), HOST, restart, C.byref(it), C.byref(t), int(rel),
+                               int(left_pc))
+        if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
+            self.check(rc)
+        return x, it.value, t.value, rc == OK

@@ -1,0 +1,180 @@
+/*
+ * dhip.h -- C ABI of the MI355X (gfx950) trace-FEM assembly + Krylov library.
+ *
+ * The library replaces, for one GPU, the DROPS (56th/drops) hot path
+ *   accumulate( {InterfaceCommonDataP1CL, InterfaceMatrixAccuCL<Local*P1CL>,
+ *                InterfaceVectorAccuCL<LocalVectorP1CL>} )      (assembly)
+ *   PCGSolverCL<JACPcCL>::Solve / GMResSolverCL::Solve           (Krylov core)
+ * behind plain C entry points: pointers, sizes and integer error codes, no
+ * C++ or torch types.  The reference is header-template C++ with no FFI of
+ * its own; each entry point names the reference interface it stands in for.
+ *
+ * Conventions
+ *   - Every call returns DHIP_OK (0) or a DHIP_E_* code; dhip_last_error()
+ *     gives the message of the last failure on that context.
+ *   - A context owns one HIP stream and all device buffers it allocates; calls
+ *     on one context are stream ordered and synchronous on return.  A context
+ *     must not be used by two host threads at once.
+ *   - `where` arguments say whether caller pointers are host (DHIP_HOST) or
+ *     device (DHIP_DEVICE) memory.
+ *   - Level sets are P2 nodal vectors on a Kuhn brick stored as a dense
+ *     float64 array over the half-spaced grid, shape (2n3+1, 2n2+1, 2n1+1),
+ *     x fastest: vertices at even indices, Kuhn edge midpoints at odd ones.
+ *   - Tets are numbered t = 6*((i3*n2 + i2)*n1 + i1) + k (geom/builder.cpp:150-176),
+ *     vertices v = (i3*(n2+1) + i2)*(n1+1) + i1.
+ *   - CSR matrices use int32 row pointers and column indices, columns sorted
+ *     ascending within each row, explicit zeros kept (SparseMatBuilderCL).
+ */
+#ifndef DHIP_H
+#define DHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define DHIP_OK 0
+#define DHIP_E_ZERO_NODE 1       /* std::invalid_argument "levelset is exactly zero at node(s)",
+                                    geom/subtriangulation.tpp:44 */
+#define DHIP_E_ZERO_PATTERN 2    /* DROPSErrCL "found a full body in the zero level set",
+                                    geom/reftetracut.h:152-153 */
+#define DHIP_E_ARG 3
+#define DHIP_E_NOT_CONVERGED 4   /* solver returned false, num/krylovsolver.h:176-178 */
+#define DHIP_E_NOMEM 5
+#define DHIP_E_HIP 6
+#define DHIP_E_STATE 7           /* call out of order (e.g. number before classify) */
+#define DHIP_E_CAPACITY 8        /* size beyond what the library supports */
+
+#define DHIP_HOST 0
+#define DHIP_DEVICE 1
+
+/* zero-node policy: DHIP_ZERO_THROW reproduces the reference (a cut tet with an
+ * exact-zero P2 node is an error); DHIP_ZERO_ALLOW treats zeros as roots
+ * (zero vertices and boundary facets of the patch, geom/reftetracut.cpp:53-60). */
+#define DHIP_ZERO_THROW 0
+#define DHIP_ZERO_ALLOW 1
+
+/* operators of the P1 interface assembly */
+#define DHIP_OP_MASS 0   /* LocalInterfaceMassP1CL, surfactant/ifacetransp.h:881-906 */
+#define DHIP_OP_LB 1     /* LocalLaplaceBeltramiP1CL, surfactant/ifacetransp.h:1657-1691 */
+#define DHIP_OP_STAB 2   /* LocalNormalLaplaceBulkP1CL, surfactant/ifacetransp.h:2032-2051 */
+#define DHIP_OP_COMB 3   /* LinComb of the three, num/spmat.h:819-825 + 1458-1560 */
+
+/* right-hand sides (LocalVectorP1CL, surfactant/ifacetransp.h:768-798) */
+#define DHIP_RHS_NONE 0
+#define DHIP_RHS_3XY_MINUS_X3 1  /* f(x) = 3 x y - x^3 */
+#define DHIP_RHS_CONST 2         /* f(x) = param[0] */
+
+/* preconditioners */
+#define DHIP_PC_NONE 0           /* DummyPcCL, num/precond.h:1032-1065 */
+#define DHIP_PC_JACOBI 1         /* JACPcCL, num/precond.h:874 */
+
+/* Krylov precision */
+#define DHIP_FP64 0
+#define DHIP_FP32_IR 1           /* FP32 inner iterations, FP64 residual/refinement */
+
+typedef struct dhip_ctx dhip_ctx_t;
+typedef struct dhip_csr dhip_csr_t;
+
+/* ---- context ---- */
+int dhip_create(dhip_ctx_t **ctx, int32_t device);
+void dhip_destroy(dhip_ctx_t *ctx);
+const char *dhip_last_error(const dhip_ctx_t *ctx);
+/* Per-kernel HIP-event timing on the context's stream (0 = off).  While on,
+ * every library kernel launch is bracketed by events; dhip_profile_get returns
+ * the accumulated milliseconds and launch count of kernel `name`. */
+int dhip_profile_enable(dhip_ctx_t *ctx, int32_t on);
+int dhip_profile_get(dhip_ctx_t *ctx, const char *name, double *total_ms, int64_t *count);
+int dhip_profile_reset(dhip_ctx_t *ctx);
+/* Device buffers for callers without their own allocator (bench, tests). */
+int dhip_malloc(dhip_ctx_t *ctx, void **dptr, int64_t bytes);
+int dhip_free(dhip_ctx_t *ctx, void *dptr);
+int dhip_memcpy(dhip_ctx_t *ctx, void *dst, const void *src, int64_t bytes, int32_t kind);
+/* kind: 0 host->device, 1 device->host, 2 device->device */
+int dhip_synchronize(dhip_ctx_t *ctx);
+
+/* ---- mesh and level set ----
+ * BrickBuilderCL(orig, e1, e2, e3, n1, n2, n3) (geom/builder.cpp:99-178);
+ * e is row-major {e1, e2, e3}. */
+int dhip_mesh_kuhn(dhip_ctx_t *ctx, const double orig[3], const double e[9], const int32_t n[3]);
+/* The P2 level set (LevelsetP2CL::Phi, levelset/levelset.h:69) on the half-grid.
+ * DHIP_DEVICE: the library keeps the pointer (no copy); it must stay valid. */
+int dhip_levelset_p2(dhip_ctx_t *ctx, const double *phi, int64_t count, int32_t where);
+
+/* ---- assembly (replaces accumulate() over InterfaceCommonDataP1CL and the
+ *      P1 interface matrix/vector accumulators, num/accumulator.h:215-268) ---- */
+/* Sign classification of every tet on its 10 P2 values (equalSigns,
+ * geom/subtriangulation.tpp:32-36) and stream compaction of the cut tets in
+ * mesh order.  Returns DHIP_E_ZERO_NODE under DHIP_ZERO_THROW when a cut tet
+ * has an exact-zero node (distance(), geom/subtriangulation.tpp:39-46). */
+int dhip_classify(dhip_ctx_t *ctx, int32_t zero_policy, int64_t *ncut);
+/* Interface patch (SPatchCL::make_patch<MergeCutPolicyCL>) and all local
+ * quantities of every cut tet: 4x4 mass (coef[0] = alpha), Laplace-Beltrami
+ * (coef[1] = D), normal-gradient stabilisation (coef[2] = D), load vector,
+ * the interface-numbering integrals of phi_i^2 and the patch area. */
+int dhip_local_p1(dhip_ctx_t *ctx, const double coef[3], int32_t rhs_kind, const double *rhs_param);
+/* First-encounter P1 interface numbering, IdxDescCL::CreateNumbering ->
+ * CreateNumbOnInterfaceVertex (misc/problem.cpp:443-503, 725-745).
+ * omit_bound < 0 numbers every vertex with a positive phi_i^2 integral. */
+int dhip_number_p1if(dhip_ctx_t *ctx, double omit_bound, int64_t *ndof);
+/* CSR pattern + values of the three operators, their linear combination
+ * comb = 1*(lc[0]*MASS + lc[1]*LB) + lc[2]*STAB (LinComb(lc0,M,lc1,A,lc2,C)),
+ * and the load vector (update_global_matrix, surfactant/ifacetransp.tpp:31-43;
+ * SparseMatBuilderCL::Build, num/spmat.h:621-677). */
+int dhip_build_p1(dhip_ctx_t *ctx, const double lc[3], int64_t *nnz);
+/* classify + local + number + build in one call. */
+int dhip_assemble_p1(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                     const double *rhs_param, double omit_bound, const double lc[3],
+                     int64_t *ncut, int64_t *ndof, int64_t *nnz);
+
+/* ---- results of the assembly (host copies; any pointer may be NULL) ---- */
+int dhip_get_cut_tets(dhip_ctx_t *ctx, int64_t *tets);
+/* per cut tet: 16 mass, 16 LB, 16 stab, 4 rhs, 4 phi_i^2 integrals, area (57 doubles),
+ * and flags (bit0 non-empty patch, bits1-4 vertex qualifies for numbering) */
+int dhip_get_local(dhip_ctx_t *ctx, double *local57, uint8_t *flags);
+/* DOF of each (cut tet, local vertex) pair, -1 where the vertex is not numbered */
+int dhip_get_tet_dofs(dhip_ctx_t *ctx, int32_t *dofs);
+/* vertex id of each DOF */
+int dhip_get_dof_vertices(dhip_ctx_t *ctx, int64_t *vertex_of_dof);
+int dhip_get_csr(dhip_ctx_t *ctx, int32_t *rowptr, int32_t *colind, double *mass, double *lb,
+                 double *stab, double *comb, double *rhs);
+/* Device pointers of the assembled arrays (valid until the next assembly). */
+int dhip_device_arrays(dhip_ctx_t *ctx, const int32_t **rowptr, const int32_t **colind,
+                       const double **comb, const double **rhs);
+
+/* ---- sparse matrices (MatrixCL = SparseMatBaseCL<double>, num/spmat.h:683-878) ---- */
+int dhip_csr_create(dhip_ctx_t *ctx, int64_t nrows, int64_t nnz, const int32_t *rowptr,
+                    const int32_t *colind, const double *val, int32_t where, dhip_csr_t **A);
+/* wraps the assembled operator `op` (DHIP_OP_*) without copying the pattern */
+int dhip_csr_from_assembly(dhip_ctx_t *ctx, int32_t op, dhip_csr_t **A);
+void dhip_csr_destroy(dhip_csr_t *A);
+int dhip_csr_info(const dhip_csr_t *A, int64_t *nrows, int64_t *nnz);
+
+/* ---- vector core (num/spmat.h:164-235, 1683-1722) ---- */
+int dhip_spmv(dhip_ctx_t *ctx, const dhip_csr_t *A, const double *x, double *y, int32_t where);
+int dhip_dot(dhip_ctx_t *ctx, int64_t n, const double *x, const double *y, double *result,
+             int32_t where);
+int dhip_axpy(dhip_ctx_t *ctx, int64_t n, double a, const double *x, double *y, int32_t where);
+
+/* ---- Krylov solvers ----
+ * PCG(A, x, b, ex, M, max_iter, tol, rel), num/krylovsolver.h:107-196 and
+ * PCGSolverCL<PC>::Solve, :1805-1835.  x holds the start vector on entry and
+ * the solution on return; *max_iter / *tol are IN limits and OUT used
+ * iterations / final residual (relative if rel != 0).  Returns DHIP_OK on
+ * convergence, DHIP_E_NOT_CONVERGED otherwise (the reference's `false`).
+ * precision DHIP_FP32_IR: FP32 inner PCG sweeps inside an FP64 iterative
+ * refinement loop; *max_iter then counts inner iterations. */
+int dhip_pcg(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
+             int32_t where, int32_t *max_iter, double *tol, int32_t rel, int32_t precision);
+/* GMRES(A, x, b, ex, M, m, max_iter, tol, rel, calc2norm=false, method),
+ * num/krylovsolver.h:320-463 (serial variant: m-1 Arnoldi steps per cycle). */
+int dhip_gmres(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
+               int32_t where, int32_t restart, int32_t *max_iter, double *tol, int32_t rel,
+               int32_t left_pc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DHIP_H */
