@@ -33,6 +33,12 @@ class Local(C.Structure):
                 ("rhs", C.c_double * 4), ("hat_sq", C.c_double * 4)]
 
 
+class Patch(C.Structure):
+    _fields_ = [("nfacet", C.c_int32), ("bnd", C.c_int32 * 16), ("world", ((C.c_double * 3) * 3) * 16),
+                ("bary", ((C.c_double * 4) * 3) * 16), ("absdet", C.c_double * 16),
+                ("normal", (C.c_double * 3) * 16), ("ls", C.c_double * 10)]
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
 
@@ -56,6 +62,7 @@ def lib(kind="parity"):
     L.orc_assemble_p1.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64,
                                   I64, I32, D, D, D, D]
     L.orc_surface_area.argtypes = [M, D, C.c_int, D]
+    L.orc_tet_patch.argtypes = [M, D, C.c_int64, C.c_int, C.POINTER(Patch)]
     L.orc_spmv.argtypes = [C.c_int64, I64, I32, D, D, D]
     L.orc_kahan_dot.argtypes = [C.c_int64, D, D]
     L.orc_kahan_dot.restype = C.c_double
@@ -128,6 +135,19 @@ def tet_local(mesh, phi, tet, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param
     _check(lib().orc_tet_local(C.byref(mesh), _ptr(phi, C.c_double), int(tet), _ptr(co, C.c_double),
                                rhs_kind, _ptr(rp, C.c_double), zero_policy, C.byref(out)), "tet_local")
     return out
+
+
+def tet_patch(mesh, phi, tet, zero_policy=ZERO_THROW):
+    """dict(nfacet, bnd, world (F,3,3), bary (F,3,4), absdet (F,), normal (F,3), ls (10,))"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    p = Patch()
+    _check(lib().orc_tet_patch(C.byref(mesh), _ptr(phi, C.c_double), int(tet), zero_policy, C.byref(p)),
+           "tet_patch")
+    f = p.nfacet
+    return dict(nfacet=f, bnd=np.ctypeslib.as_array(p.bnd)[:f].copy(),
+                world=np.ctypeslib.as_array(p.world)[:f].copy(), bary=np.ctypeslib.as_array(p.bary)[:f].copy(),
+                absdet=np.ctypeslib.as_array(p.absdet)[:f].copy(), normal=np.ctypeslib.as_array(p.normal)[:f].copy(),
+                ls=np.ctypeslib.as_array(p.ls).copy())
 
 
 def assemble_p1(mesh, phi, dof_of_vertex, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE,

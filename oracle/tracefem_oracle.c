@@ -675,6 +675,33 @@ int orc_tet_local(const orc_mesh_t *m, const double *phi, int64_t tet, const dou
     return ORC_OK;
 }
 
+int orc_tet_patch(const orc_mesh_t *m, const double *phi, int64_t tet, int zero_policy, orc_patch_t *out)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !out) return ORC_E_ARG;
+    memset(out, 0, sizeof *out);
+    tet_t T;
+    load_tet(m, phi, tet, &T);
+    memcpy(out->ls, T.ls, sizeof T.ls);
+    if (equal_signs(T.ls, 10)) return ORC_OK;
+    if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+    patch_t p;
+    int err = make_patch(T.ls, &p);
+    if (err) return err;
+    patch_geometry(T.X, &p);
+    out->nfacet = p.nf;
+    for (int f = 0; f < p.nf; ++f) {
+        out->bnd[f] = p.bnd[f];
+        out->absdet[f] = p.absdet[f];
+        for (int c = 0; c < 3; ++c) out->normal[f][c] = p.normal[f][c];
+        for (int j = 0; j < 3; ++j) {
+            for (int c = 0; c < 3; ++c) out->world[f][j][c] = p.vw[p.facet[f][j]][c];
+            for (int c = 0; c < 4; ++c) out->bary[f][j][c] = p.vb[p.facet[f][j]][c];
+        }
+    }
+    return ORC_OK;
+}
+
 /* IdxDescCL::CreateNumbOnInterface -> CreateNumbOnInterfaceVertex, misc/problem.cpp:443-503 */
 int orc_number_p1if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
                     int32_t *dof_of_vertex, int64_t *ndof)

@@ -92,6 +92,19 @@ typedef struct {
 int orc_tet_local(const orc_mesh_t *m, const double *phi, int64_t tet, const double coef[3],
                   int rhs_kind, const double *rhs_param, int zero_policy, orc_local_t *out);
 
+/* patch of one tet: facets as world/barycentric triangles (debug / geometry pins) */
+typedef struct {
+    int32_t nfacet;
+    int32_t bnd[16];
+    double world[16][3][3];
+    double bary[16][3][4];
+    double absdet[16];
+    double normal[16][3];
+    double ls[10];        /* level set on the lattice-2 vertices */
+} orc_patch_t;
+
+int orc_tet_patch(const orc_mesh_t *m, const double *phi, int64_t tet, int zero_policy, orc_patch_t *out);
+
 /* ---- assembly (accumulate() with InterfaceCommonDataP1CL + matrix/vector
  *      accumulators, num/accumulator.h:215-228, surfactant/ifacetransp.h:533-641) ---- */
 int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
