@@ -750,15 +750,8 @@ int dhip_device_arrays(dhip_ctx_t *c, const int32_t **rowptr, const int32_t **co
 // CSR objects
 // ---------------------------------------------------------------------------
 static int csr_finish(dhip_ctx *c, dhip_csr *A) {
-    DHIP_HIPCHK(c, A->diag.ensure(sizeof(double) * (A->n + 1)));
-    DHIP_HIPCHK(c, A->diag32.ensure(sizeof(float) * (A->n + 1)));
-    DHIP_HIPCHK(c, A->val32.ensure(sizeof(float) * (A->nnz + 1)));
-    if (A->n > 0)
-        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(A->n, KB), KB, 0, A->n, A->rowptr, A->col, A->val,
-                    A->diag.as<double>(), A->diag32.as<float>());
-    if (A->nnz > 0)
-        DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, A->val32.as<float>());
-    A->has32 = true;
+    (void)c;
+    (void)A;
     return DHIP_OK;
 }
 
@@ -818,9 +811,6 @@ void dhip_csr_destroy(dhip_csr_t *A) {
     A->own_rowptr.release();
     A->own_col.release();
     A->own_val.release();
-    A->diag.release();
-    A->diag32.release();
-    A->val32.release();
     delete A;
 }
 
@@ -955,6 +945,20 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     DHIP_HIPCHK(c, c->host_stage.ensure(64));
     int32_t *outs = c->host_stage.as<int32_t>();
     double *outd = reinterpret_cast<double *>(outs + 4);
+    // Jacobi diagonal by a row scan on every solve (JACPcCL finds a_ii per Apply,
+    // num/precond.h:163-168); FP32 copies for the refinement path
+    DHIP_HIPCHK(c, c->kv[12].ensure(sizeof(double) * (n + 1)));
+    DHIP_HIPCHK(c, c->kv[13].ensure(sizeof(float) * (n + 1)));
+    const bool mixed = precision == DHIP_FP32_IR;
+    if (mixed) DHIP_HIPCHK(c, c->kv[14].ensure(sizeof(float) * (A->nnz + 1)));
+    double *diag = c->kv[12].as<double>();
+    float *diag32 = c->kv[13].as<float>();
+    float *val32 = c->kv[14].as<float>();
+    if (n > 0)
+        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, diag,
+                    mixed ? diag32 : nullptr);
+    if (mixed && A->nnz > 0)
+        DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, val32);
     DHIP_LAUNCH(c, "zero_bar", k_zero_u32, 1, 64, 0, c->bar.as<uint32_t>(), 16);
     int h_iter = 0, h_status = 1, h_outer = 0;
     double h_resid = 0.0;
@@ -962,7 +966,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         unsigned nb = 1;
         rc = coop_blocks(c, k_pcg64, n, &nb);
         if (rc) return rc;
-        PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? A->diag.as<double>() : nullptr, db, dx,
+        PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
                    c->kv[0].as<double>(), c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(),
                    c->kv[4].as<double>(), c->red_tmp.as<double>(), c->bar.as<GridBar>(), *max_iter, *tol, rel,
                    outs, outd, outs + 1};
@@ -977,7 +981,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         unsigned nb = 1;
         rc = coop_blocks(c, k_pcg_mixed, n, &nb);
         if (rc) return rc;
-        PcgMixArgs pa{n, A->rowptr, A->col, A->val, A->val32.as<float>(), A->diag32.as<float>(), db, dx,
+        PcgMixArgs pa{n, A->rowptr, A->col, A->val, val32, diag32, db, dx,
                       c->kv[0].as<double>(), c->kv[6].as<float>(), c->kv[7].as<float>(), c->kv[8].as<float>(),
                       c->kv[9].as<float>(), c->kv[10].as<float>(), c->kv[11].as<float>(), c->red_tmp.as<double>(),
                       c->bar.as<GridBar>(), *max_iter, *tol, rel, 1e-4, 400, outs, outd, outs + 1, outs + 2};

@@ -123,8 +123,8 @@ struct dhip_ctx {
     dhip::DevBuf rowlen, rowptr, col, val[4], rhs, diag;
     dhip::DevBuf scan_tmp, red_tmp, bar;
     dhip::DevBuf host_stage;  // pinned staging for scalars
-    // Krylov work vectors
-    dhip::DevBuf kv[12];
+    // Krylov work vectors (grow-only: no device allocation once warmed up)
+    dhip::DevBuf kv[16];
 };
 
 struct dhip_csr {
@@ -133,10 +133,6 @@ struct dhip_csr {
     const int32_t *rowptr = nullptr, *col = nullptr;
     const double *val = nullptr;
     dhip::DevBuf own_rowptr, own_col, own_val;
-    dhip::DevBuf diag;    // diagonal found by a row scan (SolveGSstep<PB_JAC0>)
-    dhip::DevBuf val32;   // FP32 copy of the values (mixed precision)
-    dhip::DevBuf diag32;
-    bool has32 = false;
 };
 
 namespace dhip {

@@ -116,12 +116,12 @@ def classify(mesh, phi, zero_policy=ZERO_THROW):
     return out[: n.value]
 
 
-def number_p1if(mesh, phi, omit_bound=-1.0, zero_policy=ZERO_THROW):
+def number_p1if(mesh, phi, omit_bound=-1.0, zero_policy=ZERO_THROW, kind="parity"):
     phi = np.ascontiguousarray(phi, np.float64)
     nv = (mesh.n[0] + 1) * (mesh.n[1] + 1) * (mesh.n[2] + 1)
     dof = np.empty(nv, np.int32)
     nd = C.c_int64()
-    _check(lib().orc_number_p1if(C.byref(mesh), _ptr(phi, C.c_double), omit_bound, zero_policy,
+    _check(lib(kind).orc_number_p1if(C.byref(mesh), _ptr(phi, C.c_double), omit_bound, zero_policy,
                                  _ptr(dof, C.c_int32), C.byref(nd)), "number_p1if")
     return dof, nd.value
 

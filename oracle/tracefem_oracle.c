@@ -243,7 +243,8 @@ typedef struct {
     double ls[10];  /* level set on lattice-2 vertices */
 } tet_t;
 
-static void load_tet(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T)
+/* level-set values only (enough to classify) */
+static void load_tet_ls(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T)
 {
     const int64_t n1 = m->n[0], n2 = m->n[1];
     const int64_t cell = t / 6;
@@ -251,12 +252,6 @@ static void load_tet(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T
     T->i[0] = cell % n1;
     T->i[1] = (cell / n1) % n2;
     T->i[2] = cell / (n1 * n2);
-    for (int q = 0; q < 4; ++q) {
-        const int *d = KUHN[T->k][q];
-        const int64_t v1 = T->i[0] + d[0], v2 = T->i[1] + d[1], v3 = T->i[2] + d[2];
-        orc_vertex_coord(m, v1, v2, v3, T->X[q]);
-        T->vid[q] = (v3 * (n2 + 1) + v2) * (n1 + 1) + v1;
-    }
     const int64_t h1 = 2 * n1 + 1, h2 = 2 * n2 + 1;
     for (int l = 0; l < 10; ++l) {
         int64_t j[3];
@@ -266,6 +261,19 @@ static void load_tet(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T
         }
         /* evaluate_on_vertexes(LocalP2CL, lattice-2) = P2 nodal values, num/lattice-eval.h:41-62 */
         T->ls[l] = phi[(j[2] * h2 + j[1]) * h1 + j[0]];
+    }
+}
+
+/* level-set values, vertex coordinates and vertex ids */
+static void load_tet(const orc_mesh_t *m, const double *phi, int64_t t, tet_t *T)
+{
+    const int64_t n1 = m->n[0], n2 = m->n[1];
+    load_tet_ls(m, phi, t, T);
+    for (int q = 0; q < 4; ++q) {
+        const int *d = KUHN[T->k][q];
+        const int64_t v1 = T->i[0] + d[0], v2 = T->i[1] + d[1], v3 = T->i[2] + d[2];
+        orc_vertex_coord(m, v1, v2, v3, T->X[q]);
+        T->vid[q] = (v3 * (n2 + 1) + v2) * (n1 + 1) + v1;
     }
 }
 
@@ -634,24 +642,69 @@ static int check_mesh(const orc_mesh_t *m)
     return m && m->n[0] > 0 && m->n[1] > 0 && m->n[2] > 0;
 }
 
+/* Per-cell classification of the 6 Kuhn tets (equalSigns on the 10 lattice values,
+ * geom/subtriangulation.tpp:32-36); bit k of mask[cell] = tet k is cut, bit 6 = a cut
+ * tet has an exact-zero node.  Independent per cell (OpenMP in the baseline build). */
+static void cell_masks(const orc_mesh_t *m, const double *phi, uint8_t *mask)
+{
+    const int64_t n1 = m->n[0], n2 = m->n[1], n3 = m->n[2];
+    const int64_t h1 = 2 * n1 + 1, h2 = 2 * n2 + 1;
+    int64_t off[6][10];
+    for (int k = 0; k < 6; ++k)
+        for (int l = 0; l < 10; ++l) {
+            int64_t o[3] = {0, 0, 0};
+            for (int c = 0; c < 3; ++c)
+                for (int q = 0; q < 4; ++q) o[c] += LAT_B2[l][q] * KUHN[k][q][c];
+            off[k][l] = (o[2] * h2 + o[1]) * h1 + o[0];
+        }
+    const int64_t ncell = n1 * n2 * n3;
+#pragma omp parallel for schedule(static)
+    for (int64_t cell = 0; cell < ncell; ++cell) {
+        const int64_t i1 = cell % n1, i2 = (cell / n1) % n2, i3 = cell / (n1 * n2);
+        const double *base = phi + ((2 * i3) * h2 + 2 * i2) * h1 + 2 * i1;
+        uint8_t mk = 0;
+        for (int k = 0; k < 6; ++k) {
+            int sum = 0, zero = 0;
+            for (int l = 0; l < 10; ++l) {
+                const double v = base[off[k][l]];
+                sum += sgn(v);
+                zero |= v == 0.;
+            }
+            if (abs(sum) != 10) {
+                mk |= (uint8_t)(1u << k);
+                if (zero) mk |= 64u;
+            }
+        }
+        mask[cell] = mk;
+    }
+}
+
 int orc_classify(const orc_mesh_t *m, const double *phi, int zero_policy,
                  int64_t cap, int64_t *ncut, int64_t *cut_tets)
 {
     init_tables();
     if (!check_mesh(m) || !phi || !ncut) return ORC_E_ARG;
-    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    const int64_t ncell = (int64_t)m->n[0] * m->n[1] * m->n[2];
+    uint8_t *mask = (uint8_t *)malloc(ncell > 0 ? ncell : 1);
+    if (!mask) return ORC_E_NOMEM;
+    cell_masks(m, phi, mask);
     int64_t c = 0;
-    for (int64_t t = 0; t < nt; ++t) {
-        tet_t T;
-        load_tet(m, phi, t, &T);
-        if (equal_signs(T.ls, 10)) continue;
-        if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
-        if (cut_tets) {
-            if (c >= cap) return ORC_E_CAPACITY;
-            cut_tets[c] = t;
+    int err = ORC_OK;
+    for (int64_t cell = 0; cell < ncell && !err; ++cell) {
+        const uint8_t mk = mask[cell];
+        if (!(mk & 63)) continue;
+        if (zero_policy == ORC_ZERO_THROW && (mk & 64)) { err = ORC_E_ZERO_NODE; break; }
+        for (int k = 0; k < 6; ++k) {
+            if (!(mk & (1u << k))) continue;
+            if (cut_tets) {
+                if (c >= cap) { err = ORC_E_CAPACITY; break; }
+                cut_tets[c] = cell * 6 + k;
+            }
+            ++c;
         }
-        ++c;
     }
+    free(mask);
+    if (err) return err;
     *ncut = c;
     return ORC_OK;
 }
@@ -702,7 +755,9 @@ int orc_tet_patch(const orc_mesh_t *m, const double *phi, int64_t tet, int zero_
     return ORC_OK;
 }
 
-/* IdxDescCL::CreateNumbOnInterface -> CreateNumbOnInterfaceVertex, misc/problem.cpp:443-503 */
+/* IdxDescCL::CreateNumbOnInterface -> CreateNumbOnInterfaceVertex, misc/problem.cpp:443-503.
+ * Tets whose 10 values share one strict sign are skipped (distance() != 0), so the serial
+ * first-encounter loop runs over the cut tets in mesh order. */
 int orc_number_p1if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
                     int32_t *dof_of_vertex, int64_t *ndof)
 {
@@ -710,32 +765,37 @@ int orc_number_p1if(const orc_mesh_t *m, const double *phi, double omit_bound, i
     if (!check_mesh(m) || !phi || !dof_of_vertex || !ndof) return ORC_E_ARG;
     const int64_t nv = (int64_t)(m->n[0] + 1) * (m->n[1] + 1) * (m->n[2] + 1);
     for (int64_t v = 0; v < nv; ++v) dof_of_vertex[v] = -1;
-    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    if (!cut) return ORC_E_NOMEM;
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
     int64_t counter = 0;
-    for (int64_t t = 0; t < nt; ++t) {
+    for (int64_t c = 0; c < ncut && !err; ++c) {
         tet_t T;
-        load_tet(m, phi, t, &T);
-        if (equal_signs(T.ls, 10)) continue; /* distance() != 0 */
-        if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+        load_tet(m, phi, cut[c], &T);
         const double limit = omit_bound < 0. ? 0. : omit_bound * pow(tet_volume(T.X) * 6, 4. / 3.);
         patch_t p;
-        int err = make_patch(T.ls, &p);
-        if (err) return err;
+        err = make_patch(T.ls, &p);
+        if (err) break;
         if (p.nf == 0) continue;
         patch_geometry(T.X, &p);
         double node[MAXF * 4][4], w[MAXF * 4];
         const int nq = composite_nodes(&p, 4, Q2_N, Q2_W, node, w);
         for (int i = 0; i < 4; ++i) {
             if (dof_of_vertex[T.vid[i]] >= 0) continue;
-            double c[10] = {0};
-            c[i] = 1.;
+            double cc[10] = {0};
+            cc[i] = 1.;
             for (int e = 0; e < 6; ++e)
-                if (VOE[e][0] == i || VOE[e][1] == i) c[4 + e] = 0.25;
+                if (VOE[e][0] == i || VOE[e][1] == i) cc[4 + e] = 0.25;
             double s = 0.;
-            for (int k = 0; k < nq; ++k) s += w[k] * p2_eval(c, node[k]);
+            for (int k = 0; k < nq; ++k) s += w[k] * p2_eval(cc, node[k]);
             if (s > limit) dof_of_vertex[T.vid[i]] = (int32_t)counter++;
         }
     }
+    free(cut);
+    if (err) return err;
     *ndof = counter;
     return ORC_OK;
 }
@@ -871,9 +931,10 @@ int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, do
     double surf = 0.;
     for (int64_t t = 0; t < nt; ++t) {
         tet_t T;
-        load_tet(m, phi, t, &T);
+        load_tet_ls(m, phi, t, &T);
         if (equal_signs(T.ls, 10)) continue;
         if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+        load_tet(m, phi, t, &T);
         patch_t p;
         int err = make_patch(T.ls, &p);
         if (err) return err;
