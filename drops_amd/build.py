@@ -37,6 +37,27 @@ def build_lib(verbose=False):
     return LIB
 
 
+HOST_EXAMPLES = {"tracefem_p1": os.path.join(HERE, "host", "examples", "tracefem_p1.cpp")}
+
+
+def build_host_examples(verbose=False):
+    """C++ drivers written against the host mirror (drops_amd/host/dropship.hpp), linked to libdhip."""
+    out = {}
+    for name, src in HOST_EXAMPLES.items():
+        exe = os.path.join(HERE, "lib", name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", exe + ".tmp", src, "-L" + os.path.dirname(LIB), "-ldhip",
+               "-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError(f"g++ failed for {name}")
+        os.replace(exe + ".tmp", exe)
+        out[name] = exe
+    return out
+
+
 def build_oracle(verbose=False):
     """The oracle is test infrastructure; it is compiled so that tests and the CPU
     baseline can run, never linked into libdhip."""
@@ -51,4 +72,5 @@ def build_oracle(verbose=False):
 
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    build_host_examples(verbose=True)
     build_oracle(verbose=True)

@@ -911,14 +911,32 @@ int dhip_axpy(dhip_ctx_t *c, int64_t n, double a, const double *x, double *y, in
 // ---------------------------------------------------------------------------
 }  // extern "C"
 
-template <class K>
-static int coop_blocks(dhip_ctx *c, K kernel, int64_t n, unsigned *nb) {
+// Launch geometry of the persistent solvers: one row per thread when the rows
+// fit 512 x (CUs) threads, else RPT rows per thread; one block per CU at most.
+static int persistent_geometry(dhip_ctx *c, int64_t n, unsigned *nb, int *rpt) {
+    const int64_t cap_blocks = c->cu_count;
+    int r = 1;
+    while ((int64_t)r * PB * cap_blocks < n && r < 16) r *= 2;
+    if ((int64_t)r * PB * cap_blocks < n)
+        return set_error(c, DHIP_E_CAPACITY, "%lld rows exceed the persistent solver (max %lld)", (long long)n,
+                         (long long)(16LL * PB * cap_blocks));
+    *rpt = r;
+    *nb = (unsigned)std::max<int64_t>(1, (n + (int64_t)r * PB - 1) / ((int64_t)r * PB));
+    return DHIP_OK;
+}
+
+template <class ArgT>
+static int launch_persistent(dhip_ctx *c, const char *name, const void *const kernels[5], int rpt, unsigned nb,
+                             ArgT &args) {
+    const int idx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
+    const void *k = kernels[idx];
     int per_cu = 0;
-    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, KB, 0));
-    if (per_cu < 1) return set_error(c, DHIP_E_HIP, "persistent kernel cannot be resident");
-    const int64_t want = std::max<int64_t>(1, (n + 127) / 128);
-    const int64_t cap = std::min<int64_t>((int64_t)per_cu * c->cu_count, c->cu_count);
-    *nb = (unsigned)std::min(want, cap);
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, LDS_BYTES));
+    if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
+        return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
+    void *argv[] = {&args};
+    ProfScope ps(c, name);
+    DHIP_HIPCHK(c, hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
     return DHIP_OK;
 }
 
@@ -951,6 +969,11 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     DHIP_HIPCHK(c, c->kv[13].ensure(sizeof(float) * (n + 1)));
     const bool mixed = precision == DHIP_FP32_IR;
     if (mixed) DHIP_HIPCHK(c, c->kv[14].ensure(sizeof(float) * (A->nnz + 1)));
+    if (n == 0) {
+        *max_iter = 0;
+        *tol = 0.0;
+        return DHIP_OK;
+    }
     double *diag = c->kv[12].as<double>();
     float *diag32 = c->kv[13].as<float>();
     float *val32 = c->kv[14].as<float>();
@@ -959,38 +982,33 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
                     mixed ? diag32 : nullptr);
     if (mixed && A->nnz > 0)
         DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, val32);
-    DHIP_LAUNCH(c, "zero_bar", k_zero_u32, 1, 64, 0, c->bar.as<uint32_t>(), 16);
+    unsigned nb = 1;
+    int rpt = 1;
+    rc = persistent_geometry(c, n, &nb, &rpt);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, c->bar.ensure(sizeof(SyncSlot) * (nb + 1)));
+    DHIP_LAUNCH(c, "zero_slots", k_zero_u32, blocks_for((int64_t)nb * sizeof(SyncSlot) / 4, 256), 256, 0,
+                c->bar.as<uint32_t>(), (int)(nb * sizeof(SyncSlot) / 4));
     int h_iter = 0, h_status = 1, h_outer = 0;
     double h_resid = 0.0;
     if (precision == DHIP_FP64) {
-        unsigned nb = 1;
-        rc = coop_blocks(c, k_pcg64, n, &nb);
-        if (rc) return rc;
         PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
-                   c->kv[0].as<double>(), c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(),
-                   c->kv[4].as<double>(), c->red_tmp.as<double>(), c->bar.as<GridBar>(), *max_iter, *tol, rel,
-                   outs, outd, outs + 1};
-        void *args[] = {&pa};
-        {
-            ProfScope ps(c, "pcg64");
-            DHIP_HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_pcg64, dim3(nb), dim3(KB), args, 0, c->stream));
-        }
+                   c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(), c->bar.as<SyncSlot>(),
+                   *max_iter, *tol, rel, outs, outd, outs + 1};
+        const void *ks[5] = {(const void *)k_pcg64<1>, (const void *)k_pcg64<2>, (const void *)k_pcg64<4>,
+                             (const void *)k_pcg64<8>, (const void *)k_pcg64<16>};
+        rc = launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+        if (rc) return rc;
     } else if (precision == DHIP_FP32_IR) {
         if (pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "FP32 refinement path uses the Jacobi preconditioner");
-        for (int k = 6; k < 12; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(float) * (n + 1)));
-        unsigned nb = 1;
-        rc = coop_blocks(c, k_pcg_mixed, n, &nb);
+        for (int k = 6; k < 9; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(float) * (n + 1)));
+        PcgMixArgs pa{n, A->rowptr, A->col, A->val, diag, val32, db, dx, c->kv[6].as<float>(), c->kv[7].as<float>(),
+                      c->kv[8].as<float>(), c->bar.as<SyncSlot>(), *max_iter, *tol, rel, 1e-4, 400, outs, outd,
+                      outs + 1, outs + 2};
+        const void *ks[5] = {(const void *)k_pcg_mixed<1>, (const void *)k_pcg_mixed<2>, (const void *)k_pcg_mixed<4>,
+                             (const void *)k_pcg_mixed<8>, (const void *)k_pcg_mixed<16>};
+        rc = launch_persistent(c, "pcg_mixed", ks, rpt, nb, pa);
         if (rc) return rc;
-        PcgMixArgs pa{n, A->rowptr, A->col, A->val, val32, diag32, db, dx,
-                      c->kv[0].as<double>(), c->kv[6].as<float>(), c->kv[7].as<float>(), c->kv[8].as<float>(),
-                      c->kv[9].as<float>(), c->kv[10].as<float>(), c->kv[11].as<float>(), c->red_tmp.as<double>(),
-                      c->bar.as<GridBar>(), *max_iter, *tol, rel, 1e-4, 400, outs, outd, outs + 1, outs + 2};
-        void *args[] = {&pa};
-        {
-            ProfScope ps(c, "pcg_mixed");
-            DHIP_HIPCHK(c, hipLaunchCooperativeKernel((const void *)k_pcg_mixed, dim3(nb), dim3(KB), args, 0,
-                                                      c->stream));
-        }
     } else {
         return set_error(c, DHIP_E_ARG, "unknown precision");
     }

@@ -89,194 +89,278 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// grid barrier for cooperative launches (all blocks co-resident)
+// Grid synchronisation for cooperative (co-resident) launches.
+//
+// Every block owns one 128-byte slot {generation, partial sums[2][3]}.  To
+// synchronise, a block (1) waits until its own global stores have reached L2
+// (s_waitcnt vmcnt(0) in every wave -- s_barrier alone does not wait for them),
+// (2) writes its partial sums, (3) one agent-scope release (L2 write-back, so
+// other XCDs see the block's data) and (4) stores its generation.  One wave
+// then polls the generation of every slot, acquires, and sums the partials of
+// all blocks in lane order -- the same order in every block, so every block
+// holds bit-identical totals and takes identical decisions.  No atomic RMW:
+// arrival cost does not grow with a serialised counter.  The partial sums are
+// double buffered by generation parity because a fast block may publish
+// generation g+1 before a slow one has read generation g.
 // ---------------------------------------------------------------------------
-struct GridBar {
-    uint32_t count;
+struct alignas(128) SyncSlot {
     uint32_t gen;
     uint32_t timeout;
-    uint32_t pad;
+    double v[2][3];
 };
 
-constexpr uint32_t SPIN_LIMIT = 1u << 22;  // ~seconds; a healthy barrier takes microseconds
+constexpr uint32_t SPIN_LIMIT = 1u << 22;  // polls; a healthy barrier takes microseconds
 
-// Returns false (in every thread of the block) if the barrier timed out.
-__device__ inline bool grid_sync(GridBar *bar, uint32_t &gen) {
-    __shared__ int ok_s;
-    __threadfence();
+__device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
+// Returns false in every thread of every block that saw a timeout.
+template <int BLOCK>
+__device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, double a1, double a2, double out[3]) {
+    __shared__ double sh[3];
+    __shared__ int ok_sh;
+    wait_own_stores();
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x, nb = gridDim.x;
+        if (lane == 0) {
+            SyncSlot *s = slots + blockIdx.x;
+            s->v[g & 1][0] = a0;
+            s->v[g & 1][1] = a1;
+            s->v[g & 1][2] = a2;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&s->gen, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         int ok = 1;
-        const uint32_t g = gen;
-        const uint32_t arrived = atomicAdd(&bar->count, 1u);
-        if (arrived == gridDim.x - 1) {
-            atomicExch(&bar->count, 0u);
-            __threadfence();
-            __hip_atomic_fetch_add(&bar->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
-                if (++spins > SPIN_LIMIT) {
-                    ok = 0;
-                    atomicOr(&bar->timeout, 1u);
-                    break;
+        for (int base = 0; base < nb; base += 64) {
+            const int l = base + lane;
+            if (l < nb) {
+                uint32_t spins = 0;
+                while (__hip_atomic_load(&slots[l].gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g) {
+                    if (++spins > SPIN_LIMIT) {
+                        ok = 0;
+                        break;
+                    }
                 }
-                __builtin_amdgcn_s_sleep(1);
             }
         }
-        __threadfence();
-        gen = g + 1;
-        ok_s = ok;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+        for (int base = 0; base < nb; base += 64) {
+            const int l = base + lane;
+            if (l < nb) {
+                v0 += slots[l].v[g & 1][0];
+                v1 += slots[l].v[g & 1][1];
+                v2 += slots[l].v[g & 1][2];
+            }
+        }
+        v0 = wave_sum(v0);
+        v1 = wave_sum(v1);
+        v2 = wave_sum(v2);
+        const bool all_ok = __ballot(ok) == __ballot(1);
+        if (lane == 0) {
+            sh[0] = v0;
+            sh[1] = v1;
+            sh[2] = v2;
+            ok_sh = all_ok ? 1 : 0;
+            if (!all_ok) slots[blockIdx.x].timeout = 1;
+        }
     }
     __syncthreads();
-    return ok_s != 0;
-}
-
-// Sum of the per-block partials part[0..gridDim.x) in a fixed order; the same
-// value in every block and thread.
-__device__ inline double grid_total(const double *part) {
-    __shared__ double tot_s;
-    if (threadIdx.x < 64) {
-        double s = 0.0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += 64) s += part[i];
-        s = wave_sum(s);
-        if (threadIdx.x == 0) tot_s = s;
-    }
+    out[0] = sh[0];
+    out[1] = sh[1];
+    out[2] = sh[2];
+    const bool ok = ok_sh != 0;
     __syncthreads();
-    const double t = tot_s;
-    __syncthreads();
-    return t;
+    return ok;
 }
 
 // ---------------------------------------------------------------------------
 // persistent PCG (FP64), num/krylovsolver.h:107-196 semantics
+//
+// Block b owns rows [r0, r1) (one or RPT rows per thread, values in registers)
+// and keeps its CSR slice in LDS when it fits.  Only z and p are exchanged
+// through global memory; the p-update of iteration i is formed on the fly
+// inside the SpMV of iteration i+1 (p double buffered), so an iteration costs
+// two grid synchronisations: after q = A p (lambda) and after the vector
+// update (|r|^2, z.r).
 // ---------------------------------------------------------------------------
+constexpr int PB = 512;                   // threads per block of the persistent solvers
+constexpr int LDS_BYTES = 64 * 1024;      // dynamic LDS per block for the CSR slice
+
 struct PcgArgs {
     int64_t n;
     const int32_t *rp, *col;
     const double *val, *diag;   // diag == nullptr: no preconditioner
     const double *b;
-    double *x, *r, *z, *p0, *p1, *q;
-    double *part;               // [4][gridDim.x]
-    GridBar *bar;
+    double *x, *z, *p0, *p1;
+    SyncSlot *slots;
     int max_iter;
     double tol;
     int rel;
     int *out_iter;
     double *out_resid;
-    int *out_status;            // 0 converged, 1 not converged, 2 barrier timeout
+    int *out_status;            // 0 converged, 1 not converged, 2 synchronisation timeout
 };
 
-__global__ void __launch_bounds__(KB) k_pcg64(PcgArgs a) {
+// CSR slice of a block: LDS copy when it fits, else the global arrays.
+template <class VT>
+struct Slice {
+    const int32_t *rp;   // local row pointers (LDS) or global rp
+    const int32_t *col;
+    const VT *val;
+    int32_t off;         // subtract from rp entries when reading the global arrays
+    int64_t r0;
+};
+
+template <class VT>
+__device__ inline Slice<VT> load_slice(unsigned char *lds, const int32_t *rp, const int32_t *col, const VT *val,
+                                       int64_t r0, int64_t r1) {
+    Slice<VT> S;
+    S.r0 = r0;
+    const int64_t nr = r1 > r0 ? r1 - r0 : 0;
+    const int32_t nz0 = nr ? rp[r0] : 0;
+    const int64_t nz = nr ? (int64_t)rp[r1] - nz0 : 0;
+    const size_t need = (size_t)(nr + 1) * 4 + (size_t)nz * (sizeof(VT) + 4) + 16;
+    if (nr && need <= (size_t)LDS_BYTES) {
+        int32_t *lrp = reinterpret_cast<int32_t *>(lds);
+        VT *lval = reinterpret_cast<VT *>(lds + (((nr + 1) * 4 + 15) & ~(int64_t)15));
+        int32_t *lcol = reinterpret_cast<int32_t *>(lval + nz);
+        for (int64_t i = threadIdx.x; i <= nr; i += blockDim.x) lrp[i] = rp[r0 + i] - nz0;
+        for (int64_t i = threadIdx.x; i < nz; i += blockDim.x) {
+            lval[i] = val[nz0 + i];
+            lcol[i] = col[nz0 + i];
+        }
+        __syncthreads();
+        S.rp = lrp;
+        S.col = lcol;
+        S.val = lval;
+        S.off = 0;
+    } else {
+        S.rp = rp + r0;
+        S.col = col + nz0;
+        S.val = val + nz0;
+        S.off = nz0;
+    }
+    return S;
+}
+
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)blockIdx.x * per;
     const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
-    const int NB = gridDim.x;
-    double *P0 = a.part, *P1 = a.part + NB, *P2 = a.part + 2 * NB, *P3 = a.part + 3 * NB;
-    const int lane = threadIdx.x % SPMV_G, grp = threadIdx.x / SPMV_G;
-    constexpr int GPB = KB / SPMV_G;
+    const Slice<double> S = load_slice<double>(lds, a.rp, a.col, a.val, r0, r1);
     uint32_t gen = 0;
+    double red[3];
     int status = 1, it_done = a.max_iter;
     double resid = 0.0;
 
-    // r = b - A x ; |r|, |b|
-    double rr = 0.0, bb = 0.0;
-    for (int64_t r = r0 + grp; r < r1; r += GPB) {
-        const double s = row_dot<SPMV_G>(a.rp, a.col, a.val, a.x, r, lane);
-        if (lane == 0) {
-            const double rv = a.b[r] - s;
-            a.r[r] = rv;
-            rr += rv * rv;
-            bb += a.b[r] * a.b[r];
-        }
+    double xr[RPT], rr[RPT], zr[RPT], pr[RPT], dr[RPT];
+    bool own[RPT];
+    int64_t lr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = r0 + lr[k] < r1;
     }
-    rr = block_sum<KB>(rr);
-    bb = block_sum<KB>(bb);
-    if (threadIdx.x == 0) {
-        P1[blockIdx.x] = rr;
-        P3[blockIdx.x] = bb;
+    auto row = [&](int64_t l, auto &&xv) {
+        double s = 0.0;
+        const int32_t e = S.rp[l + 1] - S.off;
+        for (int32_t q = S.rp[l] - S.off; q < e; ++q) s += S.val[q] * xv(S.col[q]);
+        return s;
+    };
+
+    // r = b - A x, |r|^2, |b|^2
+    double p_rr = 0.0, p_bb = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        if (!own[k]) continue;
+        const int64_t r = r0 + lr[k];
+        const double ax = row(lr[k], [&](int32_t c) { return a.x[c]; });
+        const double bv = a.b[r];
+        xr[k] = a.x[r];
+        rr[k] = bv - ax;
+        dr[k] = a.diag ? a.diag[r] : 1.0;
+        p_rr += rr[k] * rr[k];
+        p_bb += bv * bv;
     }
-    if (!grid_sync(a.bar, gen)) { status = 2; goto done; }
+    p_rr = block_sum<PB>(p_rr);
+    p_bb = block_sum<PB>(p_bb);
+    if (!sync_reduce<PB>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; goto done; }
     {
-        resid = sqrt(grid_total(P1));
-        double normb = sqrt(grid_total(P3));
+        double normb = sqrt(red[1]);
         if (normb == 0.0 || !a.rel) normb = 1.0;
-        resid = resid / normb;
+        resid = sqrt(red[0]) / normb;
         if (resid <= a.tol) {
             status = 0;
             it_done = 0;
             goto done;
         }
-        // z = M r, rho = z.r, p_old = 0
-        double zr = 0.0;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += KB) {
-            const double rv = a.r[r];
-            const double zv = a.diag ? 1.0 * rv / a.diag[r] : rv;
-            a.z[r] = zv;
+        double p_zr = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            zr[k] = a.diag ? 1.0 * rr[k] / dr[k] : rr[k];
+            pr[k] = 0.0;
+            a.z[r] = zr[k];
             a.p0[r] = 0.0;
-            zr += zv * rv;
+            p_zr += zr[k] * rr[k];
         }
-        zr = block_sum<KB>(zr);
-        if (threadIdx.x == 0) P2[blockIdx.x] = zr;
-        if (!grid_sync(a.bar, gen)) { status = 2; goto done; }
-        double rho = grid_total(P2), beta = 0.0;
+        p_zr = block_sum<PB>(p_zr);
+        if (!sync_reduce<PB>(a.slots, ++gen, p_zr, 0.0, 0.0, red)) { status = 2; goto done; }
+        double rho = red[0], beta = 0.0;
 
         for (int it = 1; it <= a.max_iter; ++it) {
             double *pn = (it & 1) ? a.p1 : a.p0;
             const double *po = (it & 1) ? a.p0 : a.p1;
-            // q = A p with p = z + beta p_old formed on the fly; lambda = q.p
-            double lam = 0.0;
-            for (int64_t r = r0 + grp; r < r1; r += GPB) {
-                double s = 0.0;
-                const int32_t e = a.rp[r + 1];
-                for (int32_t k = a.rp[r] + lane; k < e; k += SPMV_G) {
-                    const int32_t cc = a.col[k];
-                    s += a.val[k] * (a.z[cc] + beta * po[cc]);
-                }
+            const double *zg = a.z;
+            double qv[RPT], p_lam = 0.0;
 #pragma unroll
-                for (int o = SPMV_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-                if (lane == 0) {
-                    const double pv = a.z[r] + beta * po[r];
-                    pn[r] = pv;
-                    a.q[r] = s;
-                    lam += s * pv;
-                }
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                pr[k] = zr[k] + beta * pr[k];
+                pn[r] = pr[k];
+                qv[k] = row(lr[k], [&](int32_t c) { return zg[c] + beta * po[c]; });
+                p_lam += qv[k] * pr[k];
             }
-            lam = block_sum<KB>(lam);
-            if (threadIdx.x == 0) P0[blockIdx.x] = lam;
-            if (!grid_sync(a.bar, gen)) { status = 2; goto done; }
-            const double alpha = rho / grid_total(P0);
-            double rr2 = 0.0, zr2 = 0.0;
-            for (int64_t r = r0 + threadIdx.x; r < r1; r += KB) {
-                const double pv = pn[r];
-                a.x[r] += alpha * pv;
-                double rv = a.r[r];
-                rv += (-alpha) * a.q[r];
-                a.r[r] = rv;
-                rr2 += rv * rv;
-                const double zv = a.diag ? 1.0 * rv / a.diag[r] : rv;
-                a.z[r] = zv;
-                zr2 += zv * rv;
+            p_lam = block_sum<PB>(p_lam);
+            if (!sync_reduce<PB>(a.slots, ++gen, p_lam, 0.0, 0.0, red)) { status = 2; goto done; }
+            const double alpha = rho / red[0];
+            double p_rr2 = 0.0, p_zr2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                xr[k] += alpha * pr[k];
+                rr[k] += (-alpha) * qv[k];
+                p_rr2 += rr[k] * rr[k];
+                zr[k] = a.diag ? 1.0 * rr[k] / dr[k] : rr[k];
+                a.z[r] = zr[k];
+                p_zr2 += zr[k] * rr[k];
             }
-            rr2 = block_sum<KB>(rr2);
-            zr2 = block_sum<KB>(zr2);
-            if (threadIdx.x == 0) {
-                P1[blockIdx.x] = rr2;
-                P2[blockIdx.x] = zr2;
-            }
-            if (!grid_sync(a.bar, gen)) { status = 2; goto done; }
-            resid = sqrt(grid_total(P1)) / normb;
+            p_rr2 = block_sum<PB>(p_rr2);
+            p_zr2 = block_sum<PB>(p_zr2);
+            if (!sync_reduce<PB>(a.slots, ++gen, p_rr2, p_zr2, 0.0, red)) { status = 2; goto done; }
+            resid = sqrt(red[0]) / normb;
             if (resid <= a.tol) {
                 status = 0;
                 it_done = it;
                 break;
             }
             const double rho1 = rho;
-            rho = grid_total(P2);
+            rho = red[1];
             beta = rho / rho1;
         }
     }
 done:
+    if (status != 2) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k] && it_done > 0) a.x[r0 + lr[k]] = xr[k];
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *a.out_iter = it_done;
         *a.out_resid = resid;
@@ -286,21 +370,20 @@ done:
 
 // ---------------------------------------------------------------------------
 // persistent PCG with FP32 inner iterations and FP64 iterative refinement
-//   outer (FP64): r = b - A x (FP64 values), resid = |r|/|b|, stop at tol
-//   inner (FP32 storage, FP64 dot accumulation): PCG on A d = r / |r| from d = 0
-//         until |r_in| <= inner_tol or inner_max iterations
+//   outer (FP64): r = b - A x with the FP64 values, resid = |r|/|b|, stop at tol
+//   inner (FP32 values and vectors, FP64 dot accumulation): PCG on A d = r/|r|
+//         from d = 0 until |r_in| <= inner_tol or inner_max iterations
 //   update (FP64): x += |r| d
 // ---------------------------------------------------------------------------
 struct PcgMixArgs {
     int64_t n;
     const int32_t *rp, *col;
-    const double *val;          // FP64 values (residual)
-    const float *val32, *diag32;
+    const double *val, *diag;   // FP64 values (outer residual) and diagonal
+    const float *val32;
     const double *b;
-    double *x, *r;              // FP64
-    float *d, *rin, *z, *p0, *p1, *q;
-    double *part;
-    GridBar *bar;
+    double *x;                  // FP64 solution (exchanged for the outer residual)
+    float *z, *p0, *p1;         // FP32 exchange vectors
+    SyncSlot *slots;
     int max_iter;               // total inner iterations allowed
     double tol;
     int rel;
@@ -312,41 +395,53 @@ struct PcgMixArgs {
     int *out_outer;
 };
 
-__global__ void __launch_bounds__(KB) k_pcg_mixed(PcgMixArgs a) {
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)blockIdx.x * per;
     const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
-    const int NB = gridDim.x;
-    double *P0 = a.part, *P1 = a.part + NB, *P2 = a.part + 2 * NB, *P3 = a.part + 3 * NB;
-    const int lane = threadIdx.x % SPMV_G, grp = threadIdx.x / SPMV_G;
-    constexpr int GPB = KB / SPMV_G;
+    const Slice<float> S = load_slice<float>(lds, a.rp, a.col, a.val32, r0, r1);
     uint32_t gen = 0;
+    double red[3];
     int status = 1, total_it = 0, outer = 0;
     double resid = 0.0, normb = 1.0;
-    bool first = true;
 
+    double xr[RPT], dd[RPT];
+    float df[RPT], dv[RPT], rin[RPT], zv[RPT], pv[RPT];
+    bool own[RPT];
+    int64_t lr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = r0 + lr[k] < r1;
+        if (own[k]) {
+            xr[k] = a.x[r0 + lr[k]];
+            dd[k] = a.diag[r0 + lr[k]];
+            df[k] = (float)dd[k];
+        }
+    }
+    bool first = true;
     for (;;) {
         // ---- FP64 residual ----
-        double rr = 0.0, bb = 0.0;
-        for (int64_t r = r0 + grp; r < r1; r += GPB) {
-            const double s = row_dot<SPMV_G>(a.rp, a.col, a.val, a.x, r, lane);
-            if (lane == 0) {
-                const double rv = a.b[r] - s;
-                a.r[r] = rv;
-                rr += rv * rv;
-                if (first) bb += a.b[r] * a.b[r];
-            }
+        double p_rr = 0.0, p_bb = 0.0, r64[RPT];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            double s = 0.0;
+            for (int32_t q = a.rp[r]; q < a.rp[r + 1]; ++q) s += a.val[q] * a.x[a.col[q]];
+            const double bv = a.b[r];
+            r64[k] = bv - s;
+            p_rr += r64[k] * r64[k];
+            p_bb += bv * bv;
         }
-        rr = block_sum<KB>(rr);
-        if (first) bb = block_sum<KB>(bb);
-        if (threadIdx.x == 0) {
-            P1[blockIdx.x] = rr;
-            if (first) P3[blockIdx.x] = bb;
-        }
-        if (!grid_sync(a.bar, gen)) { status = 2; break; }
-        const double rnorm = sqrt(grid_total(P1));
+        p_rr = block_sum<PB>(p_rr);
+        if (first) p_bb = block_sum<PB>(p_bb);
+        if (!sync_reduce<PB>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; break; }
+        const double rnorm = sqrt(red[0]);
         if (first) {
-            normb = sqrt(grid_total(P3));
+            normb = sqrt(red[1]);
             if (normb == 0.0 || !a.rel) normb = 1.0;
             first = false;
         }
@@ -356,74 +451,78 @@ __global__ void __launch_bounds__(KB) k_pcg_mixed(PcgMixArgs a) {
         ++outer;
         // ---- FP32 inner PCG on A d = r/|r| ----
         const double scale = 1.0 / rnorm;
-        double zr = 0.0;
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += KB) {
-            const float rv = (float)(a.r[r] * scale);
-            a.rin[r] = rv;
-            a.d[r] = 0.0f;
+        double p_zr = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            rin[k] = (float)(r64[k] * scale);
+            dv[k] = 0.0f;
+            pv[k] = 0.0f;
+            zv[k] = rin[k] / df[k];
+            a.z[r] = zv[k];
             a.p0[r] = 0.0f;
-            const float zv = rv / a.diag32[r];
-            a.z[r] = zv;
-            zr += (double)zv * (double)rv;
+            p_zr += (double)zv[k] * (double)rin[k];
         }
-        zr = block_sum<KB>(zr);
-        if (threadIdx.x == 0) P2[blockIdx.x] = zr;
-        if (!grid_sync(a.bar, gen)) { status = 2; break; }
-        double rho = grid_total(P2);
+        p_zr = block_sum<PB>(p_zr);
+        if (!sync_reduce<PB>(a.slots, ++gen, p_zr, 0.0, 0.0, red)) { status = 2; break; }
+        double rho = red[0];
         float beta = 0.0f;
         bool fail = false;
         for (int it = 1; it <= a.inner_max && total_it < a.max_iter; ++it) {
             ++total_it;
             float *pn = (it & 1) ? a.p1 : a.p0;
             const float *po = (it & 1) ? a.p0 : a.p1;
-            double lam = 0.0;
-            for (int64_t r = r0 + grp; r < r1; r += GPB) {
-                float s = 0.0f;
-                const int32_t e = a.rp[r + 1];
-                for (int32_t k = a.rp[r] + lane; k < e; k += SPMV_G) {
-                    const int32_t cc = a.col[k];
-                    s += a.val32[k] * (a.z[cc] + beta * po[cc]);
-                }
+            const float *zg = a.z;
+            float qv[RPT];
+            double p_lam = 0.0;
 #pragma unroll
-                for (int o = SPMV_G / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-                if (lane == 0) {
-                    const float pv = a.z[r] + beta * po[r];
-                    pn[r] = pv;
-                    a.q[r] = s;
-                    lam += (double)s * (double)pv;
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                pv[k] = zv[k] + beta * pv[k];
+                pn[r] = pv[k];
+                float s = 0.0f;
+                const int32_t e = S.rp[lr[k] + 1] - S.off;
+                for (int32_t q = S.rp[lr[k]] - S.off; q < e; ++q) {
+                    const int32_t c = S.col[q];
+                    s += S.val[q] * (zg[c] + beta * po[c]);
                 }
+                qv[k] = s;
+                p_lam += (double)s * (double)pv[k];
             }
-            lam = block_sum<KB>(lam);
-            if (threadIdx.x == 0) P0[blockIdx.x] = lam;
-            if (!grid_sync(a.bar, gen)) { fail = true; break; }
-            const float alpha = (float)(rho / grid_total(P0));
-            double rr2 = 0.0, zr2 = 0.0;
-            for (int64_t r = r0 + threadIdx.x; r < r1; r += KB) {
-                a.d[r] += alpha * pn[r];
-                float rv = a.rin[r];
-                rv -= alpha * a.q[r];
-                a.rin[r] = rv;
-                rr2 += (double)rv * (double)rv;
-                const float zv = rv / a.diag32[r];
-                a.z[r] = zv;
-                zr2 += (double)zv * (double)rv;
+            p_lam = block_sum<PB>(p_lam);
+            if (!sync_reduce<PB>(a.slots, ++gen, p_lam, 0.0, 0.0, red)) { fail = true; break; }
+            const float alpha = (float)(rho / red[0]);
+            double p_rr2 = 0.0, p_zr2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                dv[k] += alpha * pv[k];
+                rin[k] -= alpha * qv[k];
+                p_rr2 += (double)rin[k] * (double)rin[k];
+                zv[k] = rin[k] / df[k];
+                a.z[r] = zv[k];
+                p_zr2 += (double)zv[k] * (double)rin[k];
             }
-            rr2 = block_sum<KB>(rr2);
-            zr2 = block_sum<KB>(zr2);
-            if (threadIdx.x == 0) {
-                P1[blockIdx.x] = rr2;
-                P2[blockIdx.x] = zr2;
-            }
-            if (!grid_sync(a.bar, gen)) { fail = true; break; }
-            if (sqrt(grid_total(P1)) <= a.inner_tol) break;
+            p_rr2 = block_sum<PB>(p_rr2);
+            p_zr2 = block_sum<PB>(p_zr2);
+            if (!sync_reduce<PB>(a.slots, ++gen, p_rr2, p_zr2, 0.0, red)) { fail = true; break; }
+            if (sqrt(red[0]) <= a.inner_tol) break;
             const double rho1 = rho;
-            rho = grid_total(P2);
+            rho = red[1];
             beta = (float)(rho / rho1);
         }
         if (fail) { status = 2; break; }
-        // ---- FP64 refinement update ----
-        for (int64_t r = r0 + threadIdx.x; r < r1; r += KB) a.x[r] += rnorm * (double)a.d[r];
-        if (!grid_sync(a.bar, gen)) { status = 2; break; }
+        // ---- FP64 refinement update, published for the next outer residual ----
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            xr[k] += rnorm * (double)dv[k];
+            a.x[r0 + lr[k]] = xr[k];
+        }
+        if (!sync_reduce<PB>(a.slots, ++gen, 0.0, 0.0, 0.0, red)) { status = 2; break; }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *a.out_iter = total_it;

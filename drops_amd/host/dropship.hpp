@@ -1,0 +1,682 @@
+// dropship.hpp -- C++ host mirror of the DROPS (56th/drops) interfaces on the
+// P1 trace-FEM hot path, backed by libdhip (include/dhip.h) on one MI355X.
+//
+// A DROPS user switches the includes of num/spmat.h, num/krylovsolver.h,
+// num/precond.h, num/accumulator.h and the P1 parts of surfactant/ifacetransp.h
+// to this header.  Class names, constructor arguments and the in/out semantics
+// of the solver calls follow the reference:
+//   VectorCL / MatrixCL / MatrixBuilderCL ............ num/spmat.h:83-147, 551-878
+//   DummyExchangeCL .................................. parallel/exchange.h:45-85
+//   JACPcCL, DummyPcCL ............................... num/precond.h:874, 1032-1065
+//   SolverBaseCL, PCGSolverCL, GMResSolverCL, PCG() .. num/solverbase.h:30-58,
+//                                                      num/krylovsolver.h:107, 1805, 1924
+//   BrickBuilderCL / MultiGridCL (Kuhn bricks only) .. geom/builder.cpp:67-178
+//   LevelsetP2CL::Phi (P2 level set) ................. levelset/levelset.h:69
+//   IdxDescCL(P1IF_FE)::CreateNumbering .............. misc/problem.cpp:725-745
+//   MatDescCL, VecDescCL ............................. misc/problem.h:650-768
+//   InterfaceCommonDataP1CL, InterfaceMatrixAccuCL,
+//   InterfaceVectorAccuCL, Local*P1CL, accumulate() .. surfactant/ifacetransp.h:103-156,
+//                                                      533-641, 768-906, 1657-1691, 1985-2057,
+//                                                      num/accumulator.h:80-268
+// Differences (documented at each class): meshes are Kuhn bricks; level sets are
+// P2 nodal vectors on the half-spaced grid; right-hand sides and the
+// stabilisation normal are device-evaluable (closed-form kinds / the normalised
+// gradient of the P2 level set) instead of std::function; errors are thrown as
+// DROPSErrCL / std::invalid_argument with the library's message.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <valarray>
+#include <vector>
+#include <algorithm>
+
+#include "../../include/dhip.h"
+
+namespace DROPS {
+
+typedef unsigned long IdxT;
+const IdxT NoIdx = static_cast<IdxT>(-1);
+
+class DROPSErrCL : public std::runtime_error {
+  public:
+    explicit DROPSErrCL(const std::string &m) : std::runtime_error(m) {}
+};
+
+// ---------------------------------------------------------------------------
+// device context: one libdhip context per GPU, shared by all objects
+// ---------------------------------------------------------------------------
+class DeviceCL {
+    dhip_ctx_t *ctx_ = nullptr;
+    explicit DeviceCL(int device) {
+        if (dhip_create(&ctx_, device) != DHIP_OK) throw DROPSErrCL("DeviceCL: no usable gfx950 device");
+    }
+
+  public:
+    ~DeviceCL() { dhip_destroy(ctx_); }
+    static DeviceCL &instance(int device = 0) {
+        static std::map<int, std::unique_ptr<DeviceCL>> devs;
+        auto &p = devs[device];
+        if (!p) p.reset(new DeviceCL(device));
+        return *p;
+    }
+    dhip_ctx_t *ctx() const { return ctx_; }
+    // maps library codes to the reference's exception types
+    void check(int rc) const {
+        if (rc == DHIP_OK) return;
+        const std::string msg = dhip_last_error(ctx_);
+        if (rc == DHIP_E_ZERO_NODE) throw std::invalid_argument("distance: " + msg);
+        throw DROPSErrCL(msg);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// vectors and CSR matrices
+// ---------------------------------------------------------------------------
+template <class T>
+class VectorBaseCL : public std::valarray<T> {
+  public:
+    typedef std::valarray<T> base_type;
+    VectorBaseCL() {}
+    explicit VectorBaseCL(size_t s) : base_type(T(), s) {}
+    VectorBaseCL(T c, size_t s) : base_type(c, s) {}
+    VectorBaseCL(const base_type &v) : base_type(v) {}
+    VectorBaseCL &operator=(const base_type &v) {
+        if (this->size() != v.size()) this->resize(v.size());
+        base_type::operator=(v);
+        return *this;
+    }
+    VectorBaseCL &operator=(T c) {
+        base_type::operator=(c);
+        return *this;
+    }
+    T *raw() { return this->size() ? &(*this)[0] : nullptr; }
+    const T *raw() const { return this->size() ? &const_cast<VectorBaseCL &>(*this)[0] : nullptr; }
+};
+typedef VectorBaseCL<double> VectorCL;
+
+template <class T>
+inline T dot(const VectorBaseCL<T> &v, const VectorBaseCL<T> &w) {
+    T s = T();
+    for (size_t i = 0; i < v.size(); ++i) s = s + v[i] * w[i];
+    return s;
+}
+template <class T>
+inline T norm(const VectorBaseCL<T> &v) { return std::sqrt(dot(v, v)); }
+template <class T>
+inline void axpy(T a, const VectorBaseCL<T> &x, VectorBaseCL<T> &y) { y += a * x; }
+
+template <class T>
+class SparseMatBaseCL;
+typedef SparseMatBaseCL<double> MatrixCL;
+
+// CSR with size_t row pointers / column indices as the reference (num/spmat.h:683-878);
+// a device mirror (int32 indices) is created lazily and refreshed when Version() changes.
+template <class T>
+class SparseMatBaseCL {
+    size_t rows_ = 0, cols_ = 0;
+    std::valarray<size_t> rowbeg_{size_t(0), 1}, colind_;
+    std::valarray<T> val_;
+    size_t version_ = 1;
+    mutable dhip_csr_t *dev_ = nullptr;
+    mutable size_t dev_version_ = 0;
+
+    void drop_mirror() const {
+        if (dev_) dhip_csr_destroy(dev_);
+        dev_ = nullptr;
+        dev_version_ = 0;
+    }
+
+  public:
+    SparseMatBaseCL() {}
+    SparseMatBaseCL(const SparseMatBaseCL &o)
+        : rows_(o.rows_), cols_(o.cols_), rowbeg_(o.rowbeg_), colind_(o.colind_), val_(o.val_), version_(1) {}
+    SparseMatBaseCL &operator=(const SparseMatBaseCL &o) {
+        if (this != &o) {
+            rows_ = o.rows_;
+            cols_ = o.cols_;
+            rowbeg_.resize(o.rowbeg_.size());
+            rowbeg_ = o.rowbeg_;
+            colind_.resize(o.colind_.size());
+            colind_ = o.colind_;
+            val_.resize(o.val_.size());
+            val_ = o.val_;
+            IncrementVersion();
+        }
+        return *this;
+    }
+    ~SparseMatBaseCL() { drop_mirror(); }
+
+    size_t num_rows() const { return rows_; }
+    size_t num_cols() const { return cols_; }
+    size_t num_nonzeros() const { return val_.size(); }
+    size_t row_beg(size_t i) const { return rowbeg_[i]; }
+    size_t col_ind(size_t k) const { return colind_[k]; }
+    T val(size_t k) const { return val_[k]; }
+    const size_t *raw_row() const { return &rowbeg_[0]; }
+    const size_t *raw_col() const { return colind_.size() ? &colind_[0] : nullptr; }
+    const T *raw_val() const { return val_.size() ? &val_[0] : nullptr; }
+    size_t *raw_row() { return &rowbeg_[0]; }
+    size_t *raw_col() { return colind_.size() ? &colind_[0] : nullptr; }
+    T *raw_val() { return val_.size() ? &val_[0] : nullptr; }
+    size_t Version() const { return version_; }
+    void IncrementVersion() { ++version_; }
+
+    void resize(size_t rows, size_t cols, size_t nnz) {
+        rows_ = rows;
+        cols_ = cols;
+        rowbeg_.resize(rows + 1);
+        colind_.resize(nnz);
+        val_.resize(nnz);
+        IncrementVersion();
+    }
+    void clear() { resize(0, 0, 0); rowbeg_[0] = 0; }
+
+    // Device mirror, uploaded when the host data changed (int32 indices, overflow checked).
+    const dhip_csr_t *device(DeviceCL &dev = DeviceCL::instance()) const {
+        if (dev_ && dev_version_ == version_) return dev_;
+        drop_mirror();
+        if (val_.size() >= (size_t(1) << 31) || rows_ >= (size_t(1) << 31))
+            throw DROPSErrCL("MatrixCL::device: matrix too large for int32 indices");
+        std::vector<int32_t> rp(rows_ + 1), ci(val_.size());
+        for (size_t i = 0; i <= rows_; ++i) rp[i] = (int32_t)rowbeg_[i];
+        for (size_t k = 0; k < val_.size(); ++k) ci[k] = (int32_t)colind_[k];
+        dhip_csr_t *A = nullptr;
+        dev.check(dhip_csr_create(dev.ctx(), (int64_t)rows_, (int64_t)val_.size(), rp.data(), ci.data(), raw_val(),
+                                  DHIP_HOST, &A));
+        dev_ = A;
+        dev_version_ = version_;
+        return dev_;
+    }
+    // LinComb(a, A, b, B): pattern merge of the two sorted rows (num/spmat.h:1458-1560)
+    SparseMatBaseCL &LinComb(double a, const SparseMatBaseCL &A, double b, const SparseMatBaseCL &B) {
+        if (A.num_rows() != B.num_rows() || A.num_cols() != B.num_cols())
+            throw DROPSErrCL("LinComb: incompatible dimensions");
+        std::vector<size_t> rb(A.rows_ + 1, 0), ci;
+        std::vector<T> v;
+        for (size_t r = 0; r < A.rows_; ++r) {
+            size_t iA = A.rowbeg_[r], iB = B.rowbeg_[r];
+            const size_t eA = A.rowbeg_[r + 1], eB = B.rowbeg_[r + 1];
+            while (iA < eA && iB < eB) {
+                if (B.colind_[iB] < A.colind_[iA]) { v.push_back(b * B.val_[iB]); ci.push_back(B.colind_[iB++]); }
+                else if (A.colind_[iA] < B.colind_[iB]) { v.push_back(a * A.val_[iA]); ci.push_back(A.colind_[iA++]); }
+                else { v.push_back(a * A.val_[iA++] + b * B.val_[iB]); ci.push_back(B.colind_[iB++]); }
+            }
+            for (; iA < eA; ++iA) { v.push_back(a * A.val_[iA]); ci.push_back(A.colind_[iA]); }
+            for (; iB < eB; ++iB) { v.push_back(b * B.val_[iB]); ci.push_back(B.colind_[iB]); }
+            rb[r + 1] = ci.size();
+        }
+        resize(A.rows_, A.cols_, ci.size());
+        for (size_t i = 0; i <= rows_; ++i) rowbeg_[i] = rb[i];
+        for (size_t k = 0; k < ci.size(); ++k) {
+            colind_[k] = ci[k];
+            val_[k] = v[k];
+        }
+        return *this;
+    }
+    template <class... Args>
+    SparseMatBaseCL &LinComb(double a, const SparseMatBaseCL &A, double b, const SparseMatBaseCL &B, Args... args) {
+        SparseMatBaseCL tmp;
+        tmp.LinComb(a, A, b, B);
+        return LinComb(1., tmp, args...);
+    }
+
+    template <class U>
+    friend class SparseMatBuilderCL;
+};
+
+// y = A x on the device (y_Ax, num/spmat.h:1683-1722)
+inline VectorCL operator*(const MatrixCL &A, const VectorCL &x) {
+    DeviceCL &dev = DeviceCL::instance();
+    VectorCL y(A.num_rows());
+    if (A.num_rows() == 0) return y;
+    dev.check(dhip_spmv(dev.ctx(), A.device(dev), x.raw(), y.raw(), DHIP_HOST));
+    return y;
+}
+
+// MatrixBuilderCL semantics: every touched (i,j) is stored, rows sorted on Build() (num/spmat.h:551-677)
+template <class T>
+class SparseMatBuilderCL {
+    SparseMatBaseCL<T> *mat_;
+    size_t rows_, cols_;
+    std::vector<std::unordered_map<size_t, T>> coupl_;
+
+  public:
+    SparseMatBuilderCL(SparseMatBaseCL<T> *mat, size_t rows, size_t cols)
+        : mat_(mat), rows_(rows), cols_(cols), coupl_(rows) {
+        mat->IncrementVersion();
+    }
+    T &operator()(size_t i, size_t j) { return coupl_[i][j]; }
+    void Build() {
+        size_t nnz = 0;
+        for (auto &r : coupl_) nnz += r.size();
+        mat_->resize(rows_, cols_, nnz);
+        size_t k = 0;
+        mat_->rowbeg_[0] = 0;
+        for (size_t i = 0; i < rows_; ++i) {
+            std::vector<std::pair<size_t, T>> pv(coupl_[i].begin(), coupl_[i].end());
+            std::sort(pv.begin(), pv.end(), [](const std::pair<size_t, T> &a, const std::pair<size_t, T> &b) {
+                return a.first < b.first;
+            });
+            for (auto &e : pv) {
+                mat_->colind_[k] = e.first;
+                mat_->val_[k++] = e.second;
+            }
+            mat_->rowbeg_[i + 1] = k;
+        }
+        coupl_.clear();
+    }
+};
+typedef SparseMatBuilderCL<double> MatrixBuilderCL;
+
+// ---------------------------------------------------------------------------
+// exchange, preconditioners, solvers
+// ---------------------------------------------------------------------------
+// Serial stand-in of the MPI exchange (parallel/exchange.h:45-85): dots and norms
+// of the device solvers are reduced on the GPU.
+class DummyExchangeCL {
+  public:
+    double ParDot(const VectorCL &x, bool, const VectorCL &y, bool, VectorCL * = 0, VectorCL * = 0) const {
+        DeviceCL &dev = DeviceCL::instance();
+        double r = 0.0;
+        dev.check(dhip_dot(dev.ctx(), (int64_t)x.size(), x.raw(), y.raw(), &r, DHIP_HOST));
+        return r;
+    }
+    double Norm(const VectorCL &x, bool, VectorCL * = 0) const { return std::sqrt(ParDot(x, true, x, true)); }
+};
+
+// JACPcCL = PreGSCL<P_JAC0> (num/precond.h:874): x_i = omega b_i / a_ii.  The device
+// solvers implement omega = 1 (the default and the only value the reference's
+// callers on this path use).
+class JACPcCL {
+    double omega_;
+
+  public:
+    explicit JACPcCL(double omega = 1.0) : omega_(omega) {}
+    double GetOmega() const { return omega_; }
+    static int device_kind() { return DHIP_PC_JACOBI; }
+};
+class DummyPcCL {
+  public:
+    static int device_kind() { return DHIP_PC_NONE; }
+    double GetOmega() const { return 1.0; }
+};
+
+class SolverBaseCL {
+  protected:
+    int maxiter_;
+    mutable int iter_ = -1;
+    double tol_;
+    mutable double res_ = -1.;
+    bool rel_;
+    std::ostream *output_;
+    SolverBaseCL(int maxiter, double tol, bool rel = false, std::ostream *output = 0)
+        : maxiter_(maxiter), tol_(tol), rel_(rel), output_(output) {}
+    virtual ~SolverBaseCL() {}
+
+  public:
+    virtual void SetTol(double tol) { tol_ = tol; }
+    virtual void SetMaxIter(int iter) { maxiter_ = iter; }
+    virtual void SetRelError(bool rel) { rel_ = rel; }
+    virtual double GetTol() const { return tol_; }
+    virtual int GetMaxIter() const { return maxiter_; }
+    virtual double GetResid() const { return res_; }
+    virtual int GetIter() const { return iter_; }
+    virtual bool GetRelError() const { return rel_; }
+    virtual void SetOutput(std::ostream *os) { output_ = os; }
+};
+
+// PCG(A, x, b, ex, M, max_iter, tol, rel) (num/krylovsolver.h:107-196): returns
+// convergence; max_iter/tol are IN limits, OUT iterations/residual.
+template <class PC>
+inline bool PCG(const MatrixCL &A, VectorCL &x, const VectorCL &b, const DummyExchangeCL &, PC &M, int &max_iter,
+                double &tol, bool measure_relative_tol = false, std::ostream * = 0, int precision = DHIP_FP64) {
+    if (M.GetOmega() != 1.0) throw DROPSErrCL("PCG: the device Jacobi preconditioner implements omega = 1");
+    DeviceCL &dev = DeviceCL::instance();
+    if (x.size() != b.size()) x.resize(b.size(), 0.0);
+    int32_t it = max_iter;
+    double t = tol;
+    const int rc = dhip_pcg(dev.ctx(), A.device(dev), PC::device_kind(), b.raw(), x.raw(), DHIP_HOST, &it, &t,
+                            measure_relative_tol ? 1 : 0, precision);
+    if (rc != DHIP_OK && rc != DHIP_E_NOT_CONVERGED) dev.check(rc);
+    max_iter = it;
+    tol = t;
+    return rc == DHIP_OK;
+}
+
+template <class PC>
+class PCGSolverCL : public SolverBaseCL {
+    PC &pc_;
+    int precision_ = DHIP_FP64;
+
+  public:
+    PCGSolverCL(PC &pc, int maxiter, double tol, bool rel = false, std::ostream *output = 0)
+        : SolverBaseCL(maxiter, tol, rel, output), pc_(pc) {}
+    PC &GetPc() { return pc_; }
+    const PC &GetPc() const { return pc_; }
+    // not in the reference: DHIP_FP32_IR runs FP32 inner sweeps inside FP64 refinement
+    void SetPrecision(int p) { precision_ = p; }
+    template <class Mat, class Vec, class ExT>
+    void Solve(const Mat &A, Vec &x, const Vec &b, const ExT &ex) {
+        res_ = tol_;
+        iter_ = maxiter_;
+        PCG(A, x, b, ex, pc_, iter_, res_, rel_, output_, precision_);
+    }
+    template <class Mat, class Vec, class ExT>
+    void Solve(const Mat &A, Vec &x, const Vec &b, const ExT &ex, int &numIter, double &resid) const {
+        resid = tol_;
+        numIter = maxiter_;
+        PCG(A, x, b, ex, pc_, numIter, resid, rel_, output_, precision_);
+    }
+};
+
+enum PreMethGMRES { RightPreconditioning, LeftPreconditioning };
+
+template <class PC>
+inline bool GMRES(const MatrixCL &A, VectorCL &x, const VectorCL &b, const DummyExchangeCL &, PC &M, int m,
+                  int &max_iter, double &tol, bool measure_relative_tol = true, bool = false,
+                  PreMethGMRES method = RightPreconditioning, std::ostream * = 0) {
+    if (M.GetOmega() != 1.0) throw DROPSErrCL("GMRES: the device Jacobi preconditioner implements omega = 1");
+    DeviceCL &dev = DeviceCL::instance();
+    if (x.size() != b.size()) x.resize(b.size(), 0.0);
+    int32_t it = max_iter;
+    double t = tol;
+    const int rc = dhip_gmres(dev.ctx(), A.device(dev), PC::device_kind(), b.raw(), x.raw(), DHIP_HOST, m, &it, &t,
+                              measure_relative_tol ? 1 : 0, method == LeftPreconditioning ? 1 : 0);
+    if (rc != DHIP_OK && rc != DHIP_E_NOT_CONVERGED) dev.check(rc);
+    max_iter = it;
+    tol = t;
+    return rc == DHIP_OK;
+}
+
+template <class PC>
+class GMResSolverCL : public SolverBaseCL {
+    PC &pc_;
+    int restart_;
+    PreMethGMRES method_;
+
+  public:
+    GMResSolverCL(PC &pc, int restart, int maxiter, double tol, bool relative = true, bool = false,
+                  PreMethGMRES method = LeftPreconditioning, bool = true, bool = false, std::ostream *output = 0)
+        : SolverBaseCL(maxiter, tol, relative, output), pc_(pc), restart_(restart), method_(method) {}
+    PC &GetPc() { return pc_; }
+    int GetRestart() const { return restart_; }
+    template <class Mat, class Vec, class ExT>
+    void Solve(const Mat &A, Vec &x, const Vec &b, const ExT &ex) {
+        res_ = tol_;
+        iter_ = maxiter_;
+        GMRES(A, x, b, ex, pc_, restart_, iter_, res_, rel_, false, method_);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// mesh, level set, numbering
+// ---------------------------------------------------------------------------
+typedef double Point3DCL[3];
+
+// BrickBuilderCL(orig, e1, e2, e3, n1, n2, n3): Kuhn brick (geom/builder.cpp:67-178)
+struct BrickBuilderCL {
+    double orig[3], e[3][3];
+    int n[3];
+    BrickBuilderCL(const double o[3], const double e1[3], const double e2[3], const double e3[3], int n1, int n2,
+                   int n3) {
+        for (int c = 0; c < 3; ++c) {
+            orig[c] = o[c];
+            e[0][c] = e1[c];
+            e[1][c] = e2[c];
+            e[2][c] = e3[c];
+        }
+        n[0] = n1;
+        n[1] = n2;
+        n[2] = n3;
+    }
+};
+
+class MultiGridCL {
+    BrickBuilderCL brick_;
+
+  public:
+    explicit MultiGridCL(const BrickBuilderCL &b) : brick_(b) {}
+    const BrickBuilderCL &brick() const { return brick_; }
+    size_t GetNumTriangTetra() const { return 6ul * brick_.n[0] * brick_.n[1] * brick_.n[2]; }
+    // world coordinates of half-grid point j (vertex for even j, Kuhn edge barycentre for odd j)
+    void halfgrid_coord(long j1, long j2, long j3, double x[3]) const {
+        auto vc = [&](long i1, long i2, long i3, double *v) {
+            for (int c = 0; c < 3; ++c) {
+                const double o1 = 1.0 / (double)brick_.n[0] * brick_.e[0][c];
+                const double o2 = 1.0 / (double)brick_.n[1] * brick_.e[1][c];
+                const double o3 = 1.0 / (double)brick_.n[2] * brick_.e[2][c];
+                v[c] = ((brick_.orig[c] + (double)i1 * o1) + (double)i2 * o2) + (double)i3 * o3;
+            }
+        };
+        double a[3], b[3];
+        vc(j1 / 2, j2 / 2, j3 / 2, a);
+        vc((j1 + 1) / 2, (j2 + 1) / 2, (j3 + 1) / 2, b);
+        for (int c = 0; c < 3; ++c) x[c] = 0.5 * (a[c] + b[c]);
+    }
+};
+
+enum FiniteElementT { P1IF_FE = 1 };
+
+// P2 level set on the half-spaced grid (LevelsetP2CL::Phi)
+class VecDescCL;
+class IdxDescCL;
+
+class VecDescCL {
+  public:
+    VectorCL Data;
+    IdxDescCL *RowIdx = nullptr;
+    double t = 0.;
+    VecDescCL() {}
+    explicit VecDescCL(IdxDescCL *idx);
+    void SetIdx(IdxDescCL *idx);
+};
+
+struct LevelsetP2CL {
+    VecDescCL Phi;
+    const MultiGridCL *mg = nullptr;
+    // VecDescCL::Interpolate of f at the P2 nodes
+    void Init(const MultiGridCL &m, const std::function<double(const double *, double)> &f) {
+        mg = &m;
+        const auto &B = m.brick();
+        const long h1 = 2 * B.n[0] + 1, h2 = 2 * B.n[1] + 1, h3 = 2 * B.n[2] + 1;
+        Phi.Data.resize(h1 * h2 * h3);
+        for (long j3 = 0; j3 < h3; ++j3)
+            for (long j2 = 0; j2 < h2; ++j2)
+                for (long j1 = 0; j1 < h1; ++j1) {
+                    double x[3];
+                    m.halfgrid_coord(j1, j2, j3, x);
+                    Phi.Data[(j3 * h2 + j2) * h1 + j1] = f(x, 0.);
+                }
+    }
+};
+struct BndDataCL {};  // level-set boundary data is not used on Kuhn bricks
+
+class IdxDescCL {
+    FiniteElementT fe_;
+    double omit_bound_;
+    size_t num_ = 0;
+    std::vector<int64_t> vertex_of_dof_;
+
+  public:
+    explicit IdxDescCL(FiniteElementT fe = P1IF_FE, double omit_bound = -99) : fe_(fe), omit_bound_(omit_bound) {}
+    FiniteElementT GetFE() const { return fe_; }
+    size_t NumUnknowns() const { return num_; }
+    const std::vector<int64_t> &vertex_of_dof() const { return vertex_of_dof_; }
+    // CreateNumbering (misc/problem.cpp:725-745) -> CreateNumbOnInterfaceVertex (:443-503), on the GPU
+    size_t CreateNumbering(unsigned, const MultiGridCL &mg, const VecDescCL *lsetp, const BndDataCL * = 0) {
+        if (!lsetp) throw DROPSErrCL("IdxDescCL::CreateNumbering: no level set function for interface numbering given");
+        DeviceCL &dev = DeviceCL::instance();
+        const auto &B = mg.brick();
+        const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                             B.e[2][0], B.e[2][1], B.e[2][2]};
+        const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+        dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+        dev.check(dhip_levelset_p2(dev.ctx(), lsetp->Data.raw(), (int64_t)lsetp->Data.size(), DHIP_HOST));
+        int64_t ncut = 0, nd = 0;
+        const double coef[3] = {1., 1., 1.};
+        dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &ncut));
+        dev.check(dhip_local_p1(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
+        dev.check(dhip_number_p1if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
+        num_ = (size_t)nd;
+        vertex_of_dof_.assign(num_, 0);
+        if (num_) dev.check(dhip_get_dof_vertices(dev.ctx(), vertex_of_dof_.data()));
+        return (size_t)ncut;
+    }
+};
+
+inline VecDescCL::VecDescCL(IdxDescCL *idx) { SetIdx(idx); }
+inline void VecDescCL::SetIdx(IdxDescCL *idx) {
+    RowIdx = idx;
+    Data.resize(idx ? idx->NumUnknowns() : 0, 0.0);
+}
+
+class MatDescCL {
+  public:
+    MatrixCL Data;
+    IdxDescCL *RowIdx = nullptr, *ColIdx = nullptr;
+    MatDescCL() {}
+    MatDescCL(IdxDescCL *r, IdxDescCL *c) : RowIdx(r), ColIdx(c) {}
+    void SetIdx(IdxDescCL *r, IdxDescCL *c) {
+        RowIdx = r;
+        ColIdx = c;
+        Data.clear();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// accumulators (the assembly plug-in interface)
+// ---------------------------------------------------------------------------
+class TetraAccumulatorCL {
+  public:
+    virtual ~TetraAccumulatorCL() {}
+};
+
+class InterfaceCommonDataP1CL : public TetraAccumulatorCL {
+  public:
+    const VecDescCL *ls;
+    InterfaceCommonDataP1CL(const VecDescCL &ls_arg, const BndDataCL &) : ls(&ls_arg) {}
+};
+
+// local P1 interface operators: coefficient carriers for the device assembly
+struct LocalInterfaceMassP1CL {       // surfactant/ifacetransp.h:881-906
+    static const int device_op = DHIP_OP_MASS;
+    double coef;
+    explicit LocalInterfaceMassP1CL(double alpha = 1.) : coef(alpha) {}
+};
+struct LocalLaplaceBeltramiP1CL {     // surfactant/ifacetransp.h:1657-1691
+    static const int device_op = DHIP_OP_LB;
+    double coef;
+    explicit LocalLaplaceBeltramiP1CL(double D) : coef(D) {}
+};
+// surfactant/ifacetransp.h:1985-2057: D * Quad5((n.grad phi_i)(n.grad phi_j), |det|/6); the normal
+// is the normalised gradient of the P2 level set (the reference takes it as a user function).
+struct LocalNormalLaplaceBulkP1CL {
+    static const int device_op = DHIP_OP_STAB;
+    double coef;
+    LocalNormalLaplaceBulkP1CL(double D, double = 0., std::nullptr_t = nullptr, double = 0.) : coef(D) {}
+};
+// load vector of a device-evaluable right-hand side (surfactant/ifacetransp.h:768-798)
+struct DeviceFunctionCL {
+    int kind = DHIP_RHS_NONE;
+    double param = 0.;
+};
+struct LocalVectorP1CL {
+    DeviceFunctionCL f;
+    LocalVectorP1CL(DeviceFunctionCL fn, double = 0.) : f(fn) {}
+};
+
+template <class LocalMatrixT, class CommonT>
+class InterfaceMatrixAccuCL : public TetraAccumulatorCL {
+  public:
+    MatDescCL *mat;
+    LocalMatrixT local;
+    const CommonT *cdata;
+    std::string name;
+    InterfaceMatrixAccuCL(MatDescCL *m, const LocalMatrixT &l, const CommonT &c, std::string n = std::string())
+        : mat(m), local(l), cdata(&c), name(n) {}
+};
+
+template <class LocalVectorT, class CommonT>
+class InterfaceVectorAccuCL : public TetraAccumulatorCL {
+  public:
+    VecDescCL *vec;
+    LocalVectorT local;
+    const CommonT *cdata;
+    std::string name;
+    InterfaceVectorAccuCL(VecDescCL *v, const LocalVectorT &l, const CommonT &c, std::string n = std::string())
+        : vec(v), local(l), cdata(&c), name(n) {}
+};
+
+class TetraAccumulatorTupleCL {
+  public:
+    const InterfaceCommonDataP1CL *common = nullptr;
+    struct MatEntry { MatDescCL *mat; int op; double coef; };
+    std::vector<MatEntry> mats;
+    VecDescCL *rhs = nullptr;
+    DeviceFunctionCL rhs_fn;
+
+    void push_back(InterfaceCommonDataP1CL *c) { common = c; }
+    template <class L>
+    void push_back(InterfaceMatrixAccuCL<L, InterfaceCommonDataP1CL> *a) {
+        for (auto &m : mats)
+            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P1 interface operator");
+        mats.push_back({a->mat, L::device_op, a->local.coef});
+    }
+    void push_back(InterfaceVectorAccuCL<LocalVectorP1CL, InterfaceCommonDataP1CL> *a) {
+        rhs = a->vec;
+        rhs_fn = a->local.f;
+    }
+};
+
+// accumulate(tuple, mg, level, bnd) (num/accumulator.h:265-268): one device pass
+// computes the patches and local matrices of all cut tets and scatters every
+// operator of the tuple; results are copied into the caller's MatDescCL/VecDescCL.
+inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsigned = 0, const BndDataCL * = 0) {
+    if (!t.common) throw DROPSErrCL("accumulate: InterfaceCommonDataP1CL missing from the tuple");
+    DeviceCL &dev = DeviceCL::instance();
+    const auto &B = mg.brick();
+    const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                         B.e[2][0], B.e[2][1], B.e[2][2]};
+    const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+    double coef[3] = {1., 1., 1.};
+    for (auto &m : t.mats) coef[m.op] = m.coef;
+    const VectorCL &phi = t.common->ls->Data;
+    dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+    dev.check(dhip_levelset_p2(dev.ctx(), phi.raw(), (int64_t)phi.size(), DHIP_HOST));
+    int64_t ncut = 0, nd = 0, nnz = 0;
+    const double param = t.rhs_fn.param;
+    const double lc[3] = {1., 1., 1.};
+    dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param, -1.0,
+                               lc, &ncut, &nd, &nnz));
+    std::vector<int32_t> rp(nd + 1), ci(nnz);
+    std::vector<double> v[3], rhs(nd);
+    for (auto &x : v) x.resize(nnz);
+    dev.check(dhip_get_csr(dev.ctx(), rp.data(), ci.data(), v[0].data(), v[1].data(), v[2].data(), nullptr,
+                           rhs.data()));
+    for (auto &m : t.mats) {
+        if (m.mat->RowIdx && m.mat->RowIdx->NumUnknowns() != (size_t)nd)
+            throw DROPSErrCL("accumulate: numbering of the matrix does not match the level set");
+        MatrixCL &A = m.mat->Data;
+        A.resize(nd, nd, nnz);
+        for (int64_t i = 0; i <= nd; ++i) A.raw_row()[i] = (size_t)rp[i];
+        for (int64_t k = 0; k < nnz; ++k) {
+            A.raw_col()[k] = (size_t)ci[k];
+            A.raw_val()[k] = v[m.op][k];
+        }
+    }
+    if (t.rhs) {
+        // InterfaceVectorAccuCL adds into the vector (surfactant/ifacetransp.h:635-637)
+        if (t.rhs->Data.size() != (size_t)nd) t.rhs->Data.resize(nd, 0.0);
+        for (int64_t i = 0; i < nd; ++i) t.rhs->Data[i] += rhs[i];
+    }
+}
+
+}  // namespace DROPS

@@ -1,0 +1,109 @@
+// tracefem_p1.cpp -- a DROPS-style driver of the P1 trace-FEM path written against
+// the host mirror (dropship.hpp), in the shape of SetupLBP1 / the interfaceP1FE test:
+// Kuhn brick, P2 level set, interface numbering, accumulator tuple, LinComb, PCG.
+//
+//   tracefem_p1 sphere20                    -> the 1156-DOF known answer (TestinterfaceP1FE.ref)
+//   tracefem_p1 c1 <n> <outdir>             -> configuration-1 problem, writes CSR/rhs/solution
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../dropship.hpp"
+
+using namespace DROPS;
+
+static void write_bin(const std::string &path, const void *p, size_t bytes) {
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) throw DROPSErrCL("cannot write " + path);
+    std::fwrite(p, 1, bytes, f);
+    std::fclose(f);
+}
+
+static int sphere20() {
+    const double o[3] = {0, 0, 0}, e1[3] = {1, 0, 0}, e2[3] = {0, 1, 0}, e3[3] = {0, 0, 1};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, 20, 20, 20));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) {
+        const double x = p[0] - 0.5, y = p[1] - 0.5, z = p[2] - 0.5;
+        return std::sqrt(x * x + y * y + z * z) - 0.31;
+    });
+    IdxDescCL ifaceidx(P1IF_FE);
+    ifaceidx.CreateNumbering(0, mg, &lset.Phi);
+    std::printf("(sphere) NumUnknowns: %zu\n", ifaceidx.NumUnknowns());
+    return 0;
+}
+
+static int c1(int n, const std::string &out) {
+    // [-2,2]^3 moved by -s, s = 1e-3 h (1,2,3)/sqrt(14) (mesh transform p -> p - s)
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL lsbnd;
+
+    IdxDescCL ifaceidx(P1IF_FE);
+    ifaceidx.CreateNumbering(0, mg, &lset.Phi, &lsbnd);
+    std::printf("NumUnknowns: %zu\n", ifaceidx.NumUnknowns());
+
+    MatDescCL M(&ifaceidx, &ifaceidx), A(&ifaceidx, &ifaceidx), C(&ifaceidx, &ifaceidx);
+    VecDescCL b(&ifaceidx);
+    TetraAccumulatorTupleCL accus;
+    InterfaceCommonDataP1CL cdata(lset.Phi, lsbnd);
+    accus.push_back(&cdata);
+    InterfaceMatrixAccuCL<LocalInterfaceMassP1CL, InterfaceCommonDataP1CL> mass(&M, LocalInterfaceMassP1CL(), cdata, "mass");
+    InterfaceMatrixAccuCL<LocalLaplaceBeltramiP1CL, InterfaceCommonDataP1CL> lb(&A, LocalLaplaceBeltramiP1CL(1.), cdata, "LB");
+    InterfaceMatrixAccuCL<LocalNormalLaplaceBulkP1CL, InterfaceCommonDataP1CL> stab(&C, LocalNormalLaplaceBulkP1CL(1.), cdata, "stab");
+    DeviceFunctionCL f;
+    f.kind = DHIP_RHS_3XY_MINUS_X3;
+    InterfaceVectorAccuCL<LocalVectorP1CL, InterfaceCommonDataP1CL> rhs(&b, LocalVectorP1CL(f), cdata, "rhs");
+    accus.push_back(&mass);
+    accus.push_back(&lb);
+    accus.push_back(&stab);
+    accus.push_back(&rhs);
+    accumulate(accus, mg, 0, &lsbnd);
+    std::printf("nnz: %zu\n", M.Data.num_nonzeros());
+
+    MatrixCL L;
+    L.LinComb(1.0, M.Data, 1.0, A.Data, h, C.Data);
+    VectorCL x(L.num_rows());
+    JACPcCL pc;
+    PCGSolverCL<JACPcCL> solver(pc, 10000, 1e-10, true);
+    solver.Solve(L, x, b.Data, DummyExchangeCL());
+    std::printf("PCG iterations: %d\nPCG residual: %.6e\n", solver.GetIter(), solver.GetResid());
+    // true residual on the device
+    const VectorCL Lx = L * x;
+    VectorCL r(b.Data);
+    r -= Lx;
+    std::printf("true relative residual: %.6e\n", norm(r) / norm(b.Data));
+
+    if (!out.empty()) {
+        std::vector<int64_t> rp(L.num_rows() + 1), ci(L.num_nonzeros());
+        for (size_t i = 0; i <= L.num_rows(); ++i) rp[i] = (int64_t)L.raw_row()[i];
+        for (size_t k = 0; k < L.num_nonzeros(); ++k) ci[k] = (int64_t)L.raw_col()[k];
+        write_bin(out + "/rowptr.bin", rp.data(), rp.size() * 8);
+        write_bin(out + "/colind.bin", ci.data(), ci.size() * 8);
+        write_bin(out + "/mass.bin", M.Data.raw_val(), M.Data.num_nonzeros() * 8);
+        write_bin(out + "/lb.bin", A.Data.raw_val(), A.Data.num_nonzeros() * 8);
+        write_bin(out + "/stab.bin", C.Data.raw_val(), C.Data.num_nonzeros() * 8);
+        write_bin(out + "/comb.bin", L.raw_val(), L.num_nonzeros() * 8);
+        write_bin(out + "/rhs.bin", b.Data.raw(), b.Data.size() * 8);
+        write_bin(out + "/x.bin", x.raw(), x.size() * 8);
+        write_bin(out + "/vertex_of_dof.bin", ifaceidx.vertex_of_dof().data(), ifaceidx.vertex_of_dof().size() * 8);
+    }
+    return 0;
+}
+
+int main(int argc, char **argv) {
+    try {
+        if (argc >= 2 && !std::strcmp(argv[1], "sphere20")) return sphere20();
+        if (argc >= 3 && !std::strcmp(argv[1], "c1")) return c1(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
+        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir]\n", argv[0]);
+        return 2;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+}

@@ -147,7 +147,8 @@ int dhip_device_arrays(dhip_ctx_t *ctx, const int32_t **rowptr, const int32_t **
 /* ---- sparse matrices (MatrixCL = SparseMatBaseCL<double>, num/spmat.h:683-878) ---- */
 int dhip_csr_create(dhip_ctx_t *ctx, int64_t nrows, int64_t nnz, const int32_t *rowptr,
                     const int32_t *colind, const double *val, int32_t where, dhip_csr_t **A);
-/* wraps the assembled operator `op` (DHIP_OP_*) without copying the pattern */
+/* wraps the assembled operator `op` (DHIP_OP_*) without copying; valid until the
+ * next assembly on the context */
 int dhip_csr_from_assembly(dhip_ctx_t *ctx, int32_t op, dhip_csr_t **A);
 void dhip_csr_destroy(dhip_csr_t *A);
 int dhip_csr_info(const dhip_csr_t *A, int64_t *nrows, int64_t *nnz);

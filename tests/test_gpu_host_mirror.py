@@ -1,0 +1,56 @@
+"""The C++ host mirror (drops_amd/host/dropship.hpp) driven like a DROPS application:
+the driver binary assembles through accumulate() and solves with
+PCGSolverCL<JACPcCL>; its matrices must equal the oracle's bit for bit and its
+solution must meet the tolerance."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "drops_amd", "lib", "tracefem_p1")
+
+
+def run(*args):
+    print(f"[host mirror] {' '.join(args)}", flush=True)
+    r = subprocess.run([EXE, *args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_known_answer_through_host_mirror():
+    out = run("sphere20")
+    assert re.search(r"\(sphere\) NumUnknowns: 1156\b", out)
+
+
+def test_c1_through_host_mirror(tmp_path):
+    n = 32
+    out = run("c1", str(n), str(tmp_path))
+    it = int(re.search(r"PCG iterations: (\d+)", out).group(1))
+    true_res = float(re.search(r"true relative residual: (\S+)", out).group(1))
+    assert true_res < 2e-10
+    ld = lambda name, dt: np.fromfile(tmp_path / f"{name}.bin", dtype=dt)
+    rp, col = ld("rowptr", np.int64), ld("colind", np.int64)
+    mesh, phi = cases.c1_sphere(n)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    dof, nd = O.number_p1if(om, phi)
+    ref = O.assemble_p1(om, phi, dof, nd, rhs_kind=O.RHS_3XY_MINUS_X3)
+    assert np.array_equal(rp, ref["rowptr"]) and np.array_equal(col, ref["colind"].astype(np.int64))
+    for k in ("mass", "lb", "stab", "rhs"):
+        assert np.array_equal(ld(k, np.float64), ref[k]), k
+    vod = ld("vertex_of_dof", np.int64)
+    assert np.array_equal(dof[vod], np.arange(nd))
+    # LinComb(1, M, 1, A, h, C) = 1*(M + A) + h*C, as the reference's variadic LinComb
+    comb = (1.0 * (ref["mass"] + ref["lb"])) + mesh.h * ref["stab"]
+    assert np.array_equal(ld("comb", np.float64), comb)
+    xs, its, _, _ = O.pcg_jacobi(ref["rowptr"], ref["colind"], comb, ref["rhs"], tol=1e-13)
+    x = ld("x", np.float64)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+    _, itr, _, _ = O.pcg_jacobi(ref["rowptr"], ref["colind"], comb, ref["rhs"], tol=1e-10)
+    assert abs(it - itr) <= max(2, 0.02 * itr)
