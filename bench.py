@@ -143,7 +143,7 @@ def main():
     log(f"[bench] ncut={ncut} ndof={ndof} nnz={nnz}")
 
     results = {}
-    for name, prec in (("fp64", D.FP64), ("fp32ir", D.FP32_IR)):
+    for name, prec in (("fp64", D.FP64), ("fp64_classic", D.FP64_CLASSIC), ("fp32ir", D.FP32_IR)):
         for _ in range(args.warmup):
             solve_once(ctx, mesh, x_dev, zero_dev, nbytes, prec, coef, lc)
         ctx.synchronize()
@@ -159,7 +159,8 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         kern = {}
         for k in ("classify", "compact", "local_p1", "qualify", "first_owner", "dofs", "rowcount", "rowfill",
-                  "scan_reduce", "scan_top", "scan_final", "csr_diag", "csr_to_f32", "pcg64", "pcg_mixed"):
+                  "scan_reduce", "scan_top", "scan_final", "csr_diag", "csr_to_f32", "pcg64", "pcg64_classic",
+                  "pcg_mixed"):
             ms, cnt = ctx.profile_get(k)
             if cnt:
                 kern[k] = dict(avg_us=1e3 * ms / cnt, per_step=cnt / args.steps)
@@ -168,7 +169,7 @@ def main():
         results[name] = dict(s_per_solve=dt, iters=int(np.median(its)), kernels=kern, x=x)
         log(f"[bench] {name}: {dt * 1e3:.3f} ms/solve, iters {its[0]}")
 
-    # dominant kernel = the persistent FP64 PCG kernel
+    # dominant kernel = the persistent FP64 PCG kernel (single-reduction form)
     pk = results["fp64"]["kernels"]["pcg64"]
     iters = results["fp64"]["iters"]
     bytes_per_iter = 12 * nnz + 156 * ndof
@@ -211,6 +212,8 @@ def main():
                    "cells": n, "cut_tets": ncut, "dofs": ndof, "nnz": nnz,
                    "pcg_iters_fp64": results["fp64"]["iters"], "pcg_iters_fp32ir_inner": results["fp32ir"]["iters"]},
         "fp32ir_s_per_solve": results["fp32ir"]["s_per_solve"],
+        "fp64_classic_s_per_solve": results["fp64_classic"]["s_per_solve"],
+        "pcg_iters_fp64_classic": results["fp64_classic"]["iters"],
         "err_vs_fp64_oracle": err or None,
         "spmv_hbm_gbs": spmv["gbs"],
         "spmv": {k: spmv[k] for k in ("nnz", "rows", "csr_mb", "avg_us", "copies")},

@@ -915,13 +915,15 @@ int dhip_axpy(dhip_ctx_t *c, int64_t n, double a, const double *x, double *y, in
 // fit 512 x (CUs) threads, else RPT rows per thread; one block per CU at most.
 static int persistent_geometry(dhip_ctx *c, int64_t n, unsigned *nb, int *rpt) {
     const int64_t cap_blocks = c->cu_count;
+    int64_t rows = c->pcg_rows_per_block > 0 ? c->pcg_rows_per_block : PB;
+    rows = std::max<int64_t>(rows, (n + cap_blocks - 1) / cap_blocks);  // at most one block per CU
     int r = 1;
-    while ((int64_t)r * PB * cap_blocks < n && r < 16) r *= 2;
-    if ((int64_t)r * PB * cap_blocks < n)
+    while ((int64_t)r * PB < rows && r < 16) r *= 2;
+    if ((int64_t)r * PB < rows)
         return set_error(c, DHIP_E_CAPACITY, "%lld rows exceed the persistent solver (max %lld)", (long long)n,
                          (long long)(16LL * PB * cap_blocks));
     *rpt = r;
-    *nb = (unsigned)std::max<int64_t>(1, (n + (int64_t)r * PB - 1) / ((int64_t)r * PB));
+    *nb = (unsigned)std::max<int64_t>(1, (n + rows - 1) / rows);
     return DHIP_OK;
 }
 
@@ -992,12 +994,27 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     int h_iter = 0, h_status = 1, h_outer = 0;
     double h_resid = 0.0;
     if (precision == DHIP_FP64) {
+        for (int k = 0; k < 6; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(double) * (n + 1)));
+        PcgSrArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
+                     {c->kv[0].as<double>(), c->kv[1].as<double>()}, {c->kv[2].as<double>(), c->kv[3].as<double>()},
+                     {c->kv[4].as<double>(), c->kv[5].as<double>()}, c->bar.as<SyncSlot>(), *max_iter, *tol, rel,
+                     outs, outd, outs + 1, nullptr};
+        if (c->probe_on) {
+            DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
+            pa.probe = c->probe.as<uint64_t>();
+            c->probe_blocks = nb;
+        }
+        const void *ks[5] = {(const void *)k_pcg64_sr<1>, (const void *)k_pcg64_sr<2>, (const void *)k_pcg64_sr<4>,
+                             (const void *)k_pcg64_sr<8>, (const void *)k_pcg64_sr<16>};
+        rc = launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+        if (rc) return rc;
+    } else if (precision == DHIP_FP64_CLASSIC) {
         PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
                    c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(), c->bar.as<SyncSlot>(),
                    *max_iter, *tol, rel, outs, outd, outs + 1};
         const void *ks[5] = {(const void *)k_pcg64<1>, (const void *)k_pcg64<2>, (const void *)k_pcg64<4>,
                              (const void *)k_pcg64<8>, (const void *)k_pcg64<16>};
-        rc = launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+        rc = launch_persistent(c, "pcg64_classic", ks, rpt, nb, pa);
         if (rc) return rc;
     } else if (precision == DHIP_FP32_IR) {
         if (pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "FP32 refinement path uses the Jacobi preconditioner");
@@ -1030,6 +1047,28 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     *tol = h_resid;
     if (h_status == 2) return set_error(c, DHIP_E_HIP, "grid barrier timed out in the persistent solver");
     if (h_status != 0) return set_error(c, DHIP_E_NOT_CONVERGED, "PCG did not converge: resid %g after %d", h_resid, h_iter);
+    return DHIP_OK;
+}
+
+int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
+    if (!c || !key) return DHIP_E_ARG;
+    if (!std::strcmp(key, "pcg_rows_per_block")) {
+        if (value < 0 || value > 16 * PB) return set_error(c, DHIP_E_ARG, "pcg_rows_per_block out of range");
+        c->pcg_rows_per_block = value;
+        return DHIP_OK;
+    }
+    return set_error(c, DHIP_E_ARG, "unknown tuning key %s", key);
+}
+
+int dhip_solver_probe(dhip_ctx_t *c, int32_t on, uint64_t *ticks, int32_t *nblocks) {
+    if (!c) return DHIP_E_ARG;
+    if (ticks && c->probe_blocks > 0) {
+        DHIP_HIPCHK(c, hipMemcpyAsync(ticks, c->probe.p, sizeof(uint64_t) * 4 * c->probe_blocks,
+                                      hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (nblocks) *nblocks = c->probe_blocks;
+    c->probe_on = on != 0;
     return DHIP_OK;
 }
 

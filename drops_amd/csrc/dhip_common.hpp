@@ -125,6 +125,11 @@ struct dhip_ctx {
     dhip::DevBuf host_stage;  // pinned staging for scalars
     // Krylov work vectors (grow-only: no device allocation once warmed up)
     dhip::DevBuf kv[16];
+    // optional per-block wall-clock probe of the persistent solver
+    bool probe_on = false;
+    int64_t pcg_rows_per_block = 0;   // 0: one row per thread of a 512-thread block
+    int probe_blocks = 0;
+    dhip::DevBuf probe;
 };
 
 struct dhip_csr {

@@ -91,76 +91,86 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
 // ---------------------------------------------------------------------------
 // Grid synchronisation for cooperative (co-resident) launches.
 //
-// Every block owns one 128-byte slot {generation, partial sums[2][3]}.  To
-// synchronise, a block (1) waits until its own global stores have reached L2
-// (s_waitcnt vmcnt(0) in every wave -- s_barrier alone does not wait for them),
-// (2) writes its partial sums, (3) one agent-scope release (L2 write-back, so
-// other XCDs see the block's data) and (4) stores its generation.  One wave
-// then polls the generation of every slot, acquires, and sums the partials of
-// all blocks in lane order -- the same order in every block, so every block
-// holds bit-identical totals and takes identical decisions.  No atomic RMW:
-// arrival cost does not grow with a serialised counter.  The partial sums are
-// double buffered by generation parity because a fast block may publish
-// generation g+1 before a slow one has read generation g.
+// Every block owns one slot of 64-bit words.  A block (1) waits until its own
+// global stores have reached L2 (s_waitcnt vmcnt(0) in every wave -- s_barrier
+// alone does not wait for them), (2) one agent-scope release (L2 write-back, so
+// other XCDs see the block's data), (3) publishes each partial sum as two
+// 64-bit words {generation:32 | half of the double:32}.  64-bit atomic accesses
+// are single-copy atomic, so a reader that sees the generation in both words
+// holds the matching value: one polling round trip returns arrival and data.
+// One wave per block polls every slot, then acquires (for the vector data) and
+// sums the partials in lane order -- the same order in every block, so every
+// block holds bit-identical totals and takes identical decisions.  No atomic
+// read-modify-write, so arrival cost does not grow with a serialised counter.
+// Words are double buffered by generation parity: a block can run at most one
+// generation ahead of the slowest reader.
 // ---------------------------------------------------------------------------
+constexpr int SYNC_VALS = 3;
+
 struct alignas(128) SyncSlot {
-    uint32_t gen;
+    uint64_t w[2][2 * SYNC_VALS];
     uint32_t timeout;
-    double v[2][3];
 };
 
 constexpr uint32_t SPIN_LIMIT = 1u << 22;  // polls; a healthy barrier takes microseconds
 
 __device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Returns false in every thread of every block that saw a timeout.
+// Returns false in every thread of a block that saw a timeout.
 template <int BLOCK>
 __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, double a1, double a2, double out[3]) {
-    __shared__ double sh[3];
+    __shared__ double sh[SYNC_VALS];
     __shared__ int ok_sh;
     wait_own_stores();
     __syncthreads();
     if (threadIdx.x < 64) {
-        const int lane = threadIdx.x, nb = gridDim.x;
+        const int lane = threadIdx.x, nb = gridDim.x, par = g & 1;
+        const uint64_t tag = (uint64_t)g << 32;
         if (lane == 0) {
-            SyncSlot *s = slots + blockIdx.x;
-            s->v[g & 1][0] = a0;
-            s->v[g & 1][1] = a1;
-            s->v[g & 1][2] = a2;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_store(&s->gen, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t *w = slots[blockIdx.x].w[par];
+            const double in[SYNC_VALS] = {a0, a1, a2};
+#pragma unroll
+            for (int i = 0; i < SYNC_VALS; ++i) {
+                const uint64_t bits = __double_as_longlong(in[i]);
+                __hip_atomic_store(&w[2 * i], tag | (bits & 0xffffffffull), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&w[2 * i + 1], tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
         int ok = 1;
+        double v[SYNC_VALS] = {0.0, 0.0, 0.0};
         for (int base = 0; base < nb; base += 64) {
             const int l = base + lane;
             if (l < nb) {
+                uint64_t *w = slots[l].w[par];
+                uint64_t x[2 * SYNC_VALS];
                 uint32_t spins = 0;
-                while (__hip_atomic_load(&slots[l].gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < g) {
+                for (;;) {
+                    bool ready = true;
+#pragma unroll
+                    for (int i = 0; i < 2 * SYNC_VALS; ++i) {
+                        x[i] = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ready &= (uint32_t)(x[i] >> 32) == g;
+                    }
+                    if (ready) break;
                     if (++spins > SPIN_LIMIT) {
                         ok = 0;
                         break;
                     }
                 }
+#pragma unroll
+                for (int i = 0; i < SYNC_VALS; ++i)
+                    v[i] += __longlong_as_double((long long)((x[2 * i] & 0xffffffffull) | (x[2 * i + 1] << 32)));
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-        for (int base = 0; base < nb; base += 64) {
-            const int l = base + lane;
-            if (l < nb) {
-                v0 += slots[l].v[g & 1][0];
-                v1 += slots[l].v[g & 1][1];
-                v2 += slots[l].v[g & 1][2];
-            }
-        }
-        v0 = wave_sum(v0);
-        v1 = wave_sum(v1);
-        v2 = wave_sum(v2);
+#pragma unroll
+        for (int i = 0; i < SYNC_VALS; ++i) v[i] = wave_sum(v[i]);
         const bool all_ok = __ballot(ok) == __ballot(1);
         if (lane == 0) {
-            sh[0] = v0;
-            sh[1] = v1;
-            sh[2] = v2;
+#pragma unroll
+            for (int i = 0; i < SYNC_VALS; ++i) sh[i] = v[i];
             ok_sh = all_ok ? 1 : 0;
             if (!all_ok) slots[blockIdx.x].timeout = 1;
         }
@@ -244,6 +254,41 @@ __device__ inline Slice<VT> load_slice(unsigned char *lds, const int32_t *rp, co
     return S;
 }
 
+// Row product sum_k val_k * xv(col_k) for local row l of a slice.  Rows of at
+// most MAXR entries (P1 trace-FEM rows hold <= 15) are processed as one batch:
+// indices and values first, then all gathers issued together, then the sum in
+// column order -- the same rounding as the sequential loop, one memory latency
+// instead of one per entry.
+constexpr int MAXR = 16;
+
+template <class ACC, class VT, class F>
+__device__ __forceinline__ ACC row_product(const int32_t *rp, const int32_t *col, const VT *val, int32_t off,
+                                           int64_t l, F &&xv) {
+    const int32_t b = rp[l] - off, e = rp[l + 1] - off;
+    const int len = e - b;
+    ACC s = ACC(0);
+    if (len <= MAXR) {
+        int32_t cc[MAXR];
+        VT vv[MAXR];
+        ACC xx[MAXR];
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+            if (j < len) {
+                cc[j] = col[b + j];
+                vv[j] = val[b + j];
+            }
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+            if (j < len) xx[j] = xv(cc[j]);
+#pragma unroll
+        for (int j = 0; j < MAXR; ++j)
+            if (j < len) s += (ACC)vv[j] * xx[j];
+    } else {
+        for (int32_t q = b; q < e; ++q) s += (ACC)val[q] * xv(col[q]);
+    }
+    return s;
+}
+
 template <int RPT>
 __global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -264,12 +309,7 @@ __global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a) {
         lr[k] = threadIdx.x + (int64_t)k * PB;
         own[k] = r0 + lr[k] < r1;
     }
-    auto row = [&](int64_t l, auto &&xv) {
-        double s = 0.0;
-        const int32_t e = S.rp[l + 1] - S.off;
-        for (int32_t q = S.rp[l] - S.off; q < e; ++q) s += S.val[q] * xv(S.col[q]);
-        return s;
-    };
+    auto row = [&](int64_t l, auto &&xv) { return row_product<double>(S.rp, S.col, S.val, S.off, l, xv); };
 
     // r = b - A x, |r|^2, |b|^2
     double p_rr = 0.0, p_bb = 0.0;
@@ -369,6 +409,211 @@ done:
 }
 
 // ---------------------------------------------------------------------------
+// persistent single-reduction PCG (FP64): the Chronopoulos-Gear arrangement of
+// the same recurrences -- one fused reduction of (r,u), (w,u), (r,r) per
+// iteration instead of two dependent ones:
+//   p = u + beta p,  s = w + beta s,  x += alpha p,  r -= alpha s,  u = M r,  w = A u
+//   gamma = (r,u), delta = (w,u), beta' = gamma/gamma_old,
+//   alpha' = gamma / (delta - beta' gamma / alpha)
+// with u = M r the Jacobi step.  w = A u is formed from the neighbours' values
+// of the previous synchronisation: u_c = (r_c - alpha (w_c + beta s_c)) / d_c,
+// using the column-scaled slice a_rc / d_c kept in LDS.  r, w and s are double
+// buffered by iteration parity.
+// ---------------------------------------------------------------------------
+struct PcgSrArgs {
+    int64_t n;
+    const int32_t *rp, *col;
+    const double *val, *diag;   // diag == nullptr: no preconditioner
+    const double *b;
+    double *x;
+    double *R[2], *W[2], *S[2];
+    SyncSlot *slots;
+    int max_iter;
+    double tol;
+    int rel;
+    int *out_iter;
+    double *out_resid;
+    int *out_status;
+    uint64_t *probe;            // optional [gridDim.x][4] wall-clock ticks: setup, compute, sync, total
+};
+
+// LDS copy of the slice with values divided by the diagonal of their column.
+__device__ inline Slice<double> load_slice_scaled(unsigned char *lds, const int32_t *rp, const int32_t *col,
+                                                  const double *val, const double *diag, int64_t r0, int64_t r1,
+                                                  bool &scaled) {
+    Slice<double> S;
+    S.r0 = r0;
+    const int64_t nr = r1 > r0 ? r1 - r0 : 0;
+    const int32_t nz0 = nr ? rp[r0] : 0;
+    const int64_t nz = nr ? (int64_t)rp[r1] - nz0 : 0;
+    const size_t need = (size_t)(nr + 1) * 4 + (size_t)nz * 12 + 16;
+    scaled = false;
+    if (nr && need <= (size_t)LDS_BYTES) {
+        int32_t *lrp = reinterpret_cast<int32_t *>(lds);
+        double *lval = reinterpret_cast<double *>(lds + (((nr + 1) * 4 + 15) & ~(int64_t)15));
+        int32_t *lcol = reinterpret_cast<int32_t *>(lval + nz);
+        for (int64_t i = threadIdx.x; i <= nr; i += blockDim.x) lrp[i] = rp[r0 + i] - nz0;
+        for (int64_t i = threadIdx.x; i < nz; i += blockDim.x) {
+            const int32_t c = col[nz0 + i];
+            lval[i] = diag ? val[nz0 + i] / diag[c] : val[nz0 + i];
+            lcol[i] = c;
+        }
+        __syncthreads();
+        S.rp = lrp;
+        S.col = lcol;
+        S.val = lval;
+        S.off = 0;
+        scaled = true;
+    } else {
+        S.rp = rp + r0;
+        S.col = col + nz0;
+        S.val = val + nz0;
+        S.off = nz0;
+    }
+    return S;
+}
+
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_compute = 0, t_sync = 0, t_mark = 0;
+    const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
+    bool scaled;
+    const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, a.val, a.diag, r0, r1, scaled);
+    const double *dg = a.diag;
+    // sum_c (a_rc / d_c) * y_c over local row l, y gathered by yv
+    auto scaled_row = [&](int64_t l, auto &&yv) {
+        if (scaled || !dg) return row_product<double>(S.rp, S.col, S.val, S.off, l, yv);
+        return row_product<double>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv(c) / dg[c]; });
+    };
+    uint32_t gen = 0;
+    double red[3];
+    int status = 1, it_done = a.max_iter;
+    double resid = 0.0;
+    double xr[RPT], rr[RPT], ur[RPT], wr[RPT], pr[RPT], sr[RPT], dr[RPT];
+    bool own[RPT];
+    int64_t lr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = r0 + lr[k] < r1;
+    }
+    // r0 = b - A x0 (unscaled values from global memory)
+    double p_rr = 0.0, p_bb = 0.0;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        if (!own[k]) continue;
+        const int64_t r = r0 + lr[k];
+        const double ax = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
+        const double bv = a.b[r];
+        xr[k] = a.x[r];
+        rr[k] = bv - ax;
+        dr[k] = dg ? dg[r] : 1.0;
+        a.R[0][r] = rr[k];
+        p_rr += rr[k] * rr[k];
+        p_bb += bv * bv;
+    }
+    p_rr = block_sum<PB>(p_rr);
+    p_bb = block_sum<PB>(p_bb);
+    if (!sync_reduce<PB>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; goto done; }
+    {
+        double normb = sqrt(red[1]);
+        if (normb == 0.0 || !a.rel) normb = 1.0;
+        resid = sqrt(red[0]) / normb;
+        if (resid <= a.tol) {
+            status = 0;
+            it_done = 0;
+            goto done;
+        }
+        // w0 = A M r0, gamma0 = (r0, u0), delta0 = (w0, u0)
+        double p_g = 0.0, p_d = 0.0;
+        const double *R0 = a.R[0];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
+            wr[k] = scaled_row(lr[k], [&](int32_t c) { return R0[c]; });
+            sr[k] = 0.0;
+            pr[k] = 0.0;
+            a.W[0][r] = wr[k];
+            a.S[0][r] = 0.0;
+            p_g += rr[k] * ur[k];
+            p_d += wr[k] * ur[k];
+        }
+        p_g = block_sum<PB>(p_g);
+        p_d = block_sum<PB>(p_d);
+        if (!sync_reduce<PB>(a.slots, ++gen, p_g, p_d, 0.0, red)) { status = 2; goto done; }
+        double gamma = red[0], alpha = red[0] / red[1], beta = 0.0;
+        const uint64_t t_setup = wall_clock64() - t_start;
+        if (a.probe && threadIdx.x == 0) a.probe[blockIdx.x * 4 + 0] = t_setup;
+
+        for (int it = 0; it < a.max_iter; ++it) {
+            if (a.probe) t_mark = wall_clock64();
+            const int cur = it & 1, nxt = cur ^ 1;
+            const double *Rc = a.R[cur], *Wc = a.W[cur], *Sc = a.S[cur];
+            double p_g2 = 0.0, p_d2 = 0.0, p_rr2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                pr[k] = ur[k] + beta * pr[k];
+                sr[k] = wr[k] + beta * sr[k];
+                xr[k] += alpha * pr[k];
+                rr[k] = rr[k] - alpha * sr[k];
+                ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
+                wr[k] = scaled_row(lr[k], [&](int32_t c) { return Rc[c] - alpha * (Wc[c] + beta * Sc[c]); });
+                a.R[nxt][r] = rr[k];
+                a.W[nxt][r] = wr[k];
+                a.S[nxt][r] = sr[k];
+                p_g2 += rr[k] * ur[k];
+                p_d2 += wr[k] * ur[k];
+                p_rr2 += rr[k] * rr[k];
+            }
+            p_g2 = block_sum<PB>(p_g2);
+            p_d2 = block_sum<PB>(p_d2);
+            p_rr2 = block_sum<PB>(p_rr2);
+            if (a.probe) {
+                const uint64_t t1 = wall_clock64();
+                t_compute += t1 - t_mark;
+                t_mark = t1;
+            }
+            if (!sync_reduce<PB>(a.slots, ++gen, p_g2, p_d2, p_rr2, red)) { status = 2; goto done; }
+            if (a.probe) t_sync += wall_clock64() - t_mark;
+            resid = sqrt(red[2]) / normb;
+            if (resid <= a.tol) {
+                status = 0;
+                it_done = it + 1;
+                break;
+            }
+            const double gamma_new = red[0];
+            beta = gamma_new / gamma;
+            alpha = gamma_new / (red[1] - beta * gamma_new / alpha);
+            gamma = gamma_new;
+        }
+    }
+done:
+    if (status != 2) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k] && it_done > 0) a.x[r0 + lr[k]] = xr[k];
+    }
+    if (a.probe && threadIdx.x == 0) {
+        a.probe[blockIdx.x * 4 + 1] = t_compute;
+        a.probe[blockIdx.x * 4 + 2] = t_sync;
+        a.probe[blockIdx.x * 4 + 3] = wall_clock64() - t_start;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.out_iter = it_done;
+        *a.out_resid = resid;
+        *a.out_status = status;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // persistent PCG with FP32 inner iterations and FP64 iterative refinement
 //   outer (FP64): r = b - A x with the FP64 values, resid = |r|/|b|, stop at tol
 //   inner (FP32 values and vectors, FP64 dot accumulation): PCG on A d = r/|r|
@@ -429,8 +674,7 @@ __global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
         for (int k = 0; k < RPT; ++k) {
             if (!own[k]) continue;
             const int64_t r = r0 + lr[k];
-            double s = 0.0;
-            for (int32_t q = a.rp[r]; q < a.rp[r + 1]; ++q) s += a.val[q] * a.x[a.col[q]];
+            const double s = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
             const double bv = a.b[r];
             r64[k] = bv - s;
             p_rr += r64[k] * r64[k];
@@ -482,12 +726,8 @@ __global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
                 const int64_t r = r0 + lr[k];
                 pv[k] = zv[k] + beta * pv[k];
                 pn[r] = pv[k];
-                float s = 0.0f;
-                const int32_t e = S.rp[lr[k] + 1] - S.off;
-                for (int32_t q = S.rp[lr[k]] - S.off; q < e; ++q) {
-                    const int32_t c = S.col[q];
-                    s += S.val[q] * (zg[c] + beta * po[c]);
-                }
+                const float s = row_product<float>(S.rp, S.col, S.val, S.off, lr[k],
+                                                   [&](int32_t c) { return zg[c] + beta * po[c]; });
                 qv[k] = s;
                 p_lam += (double)s * (double)pv[k];
             }

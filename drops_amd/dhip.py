@@ -17,13 +17,13 @@ ZERO_THROW, ZERO_ALLOW = 0, 1
 OP_MASS, OP_LB, OP_STAB, OP_COMB = range(4)
 RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST = range(3)
 PC_NONE, PC_JACOBI = 0, 1
-FP64, FP32_IR = 0, 1
+FP64, FP32_IR, FP64_CLASSIC = 0, 1, 2
 LOC_N = 57
 
 # every symbol declared in include/dhip.h
 EXPORTS = [
     "dhip_create", "dhip_destroy", "dhip_last_error", "dhip_profile_enable", "dhip_profile_get",
-    "dhip_profile_reset", "dhip_malloc", "dhip_free", "dhip_memcpy", "dhip_synchronize",
+    "dhip_profile_reset", "dhip_solver_probe", "dhip_tune", "dhip_malloc", "dhip_free", "dhip_memcpy", "dhip_synchronize",
     "dhip_mesh_kuhn", "dhip_levelset_p2", "dhip_classify", "dhip_local_p1", "dhip_number_p1if",
     "dhip_build_p1", "dhip_assemble_p1", "dhip_get_cut_tets", "dhip_get_local", "dhip_get_tet_dofs",
     "dhip_get_dof_vertices", "dhip_get_csr", "dhip_device_arrays", "dhip_csr_create",
@@ -57,6 +57,8 @@ def load(path=LIB_PATH):
     L.dhip_profile_enable.argtypes = [P, C.c_int32]
     L.dhip_profile_get.argtypes = [P, C.c_char_p, D, I64]
     L.dhip_profile_reset.argtypes = [P]
+    L.dhip_solver_probe.argtypes = [P, C.c_int32, C.POINTER(C.c_uint64), I32]
+    L.dhip_tune.argtypes = [P, C.c_char_p, C.c_int64]
     L.dhip_malloc.argtypes = [P, C.POINTER(VP), C.c_int64]
     L.dhip_free.argtypes = [P, VP]
     L.dhip_memcpy.argtypes = [P, VP, VP, C.c_int64, C.c_int32]
@@ -159,6 +161,19 @@ class Context:
 
     def profile_reset(self):
         self.check(self.L.dhip_profile_reset(self.h))
+
+    def tune(self, key, value):
+        self.check(self.L.dhip_tune(self.h, key.encode(), int(value)))
+
+    def solver_probe(self, on):
+        """Arm/disarm the solver probe; returns the last solve's ticks as (nblocks, 4) or None."""
+        nb = C.c_int32()
+        self.check(self.L.dhip_solver_probe(self.h, int(on), None, C.byref(nb)))
+        if nb.value <= 0:
+            return None
+        out = np.zeros((nb.value, 4), np.uint64)
+        self.check(self.L.dhip_solver_probe(self.h, int(on), out.ctypes.data_as(C.POINTER(C.c_uint64)), None))
+        return out
 
     def malloc(self, nbytes):
         p = C.c_void_p()

@@ -71,9 +71,12 @@ extern "C" {
 #define DHIP_PC_NONE 0           /* DummyPcCL, num/precond.h:1032-1065 */
 #define DHIP_PC_JACOBI 1         /* JACPcCL, num/precond.h:874 */
 
-/* Krylov precision */
-#define DHIP_FP64 0
+/* Krylov variants (the `precision` argument of dhip_pcg) */
+#define DHIP_FP64 0              /* FP64, one fused reduction per iteration (Chronopoulos-Gear
+                                    arrangement of the PCG recurrences) */
 #define DHIP_FP32_IR 1           /* FP32 inner iterations, FP64 residual/refinement */
+#define DHIP_FP64_CLASSIC 2      /* FP64, the reference's operation order: two reductions per
+                                    iteration (num/krylovsolver.h:150-191) */
 
 typedef struct dhip_ctx dhip_ctx_t;
 typedef struct dhip_csr dhip_csr_t;
@@ -88,6 +91,14 @@ const char *dhip_last_error(const dhip_ctx_t *ctx);
 int dhip_profile_enable(dhip_ctx_t *ctx, int32_t on);
 int dhip_profile_get(dhip_ctx_t *ctx, const char *name, double *total_ms, int64_t *count);
 int dhip_profile_reset(dhip_ctx_t *ctx);
+/* Wall-clock probe of the persistent FP64 solver (100 MHz ticks per block:
+ * setup, compute phases, synchronisations, total).  dhip_solver_probe(ctx, 1, ...)
+ * arms it for the following solves; `ticks` (4 per block, may be NULL) receives
+ * the last solve's values and `nblocks` its block count. */
+int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
+/* Tuning parameters (chosen by drops_amd/tune.py): "pcg_rows_per_block" -- rows owned
+ * by one block of the persistent solvers (0 = 512, one per thread). */
+int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);
 /* Device buffers for callers without their own allocator (bench, tests). */
 int dhip_malloc(dhip_ctx_t *ctx, void **dptr, int64_t bytes);
 int dhip_free(dhip_ctx_t *ctx, void *dptr);

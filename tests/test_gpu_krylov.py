@@ -45,12 +45,15 @@ def test_spmv_dot_axpy(ctx):
     A.close()
 
 
+@pytest.mark.parametrize("variant", [D.FP64, D.FP64_CLASSIC])
 @pytest.mark.parametrize("n", [16, 32, 64])
-def test_pcg_fp64_matches_oracle(ctx, n):
+def test_pcg_fp64_matches_oracle(ctx, n, variant):
+    """Both FP64 forms: the single-reduction arrangement (default) and the reference's
+    two-reduction operation order."""
     g, A = assembled_system(ctx, n)
     rp = g["rowptr"].astype(np.int64)
     b = g["rhs"]
-    x, it, res, ok = ctx.pcg(A, b, tol=1e-10)
+    x, it, res, ok = ctx.pcg(A, b, tol=1e-10, precision=variant)
     xr, itr, resr, okr = O.pcg_jacobi(rp, g["colind"], g["comb"], b, tol=1e-10)
     assert ok and okr and res <= 1e-10
     assert abs(it - itr) <= max(2, 0.02 * itr), (it, itr)
@@ -73,16 +76,17 @@ def test_pcg_mixed_precision(ctx):
     A.close()
 
 
-def test_pcg_not_converged_and_start_vector(ctx):
+@pytest.mark.parametrize("variant", [D.FP64, D.FP64_CLASSIC, D.FP32_IR])
+def test_pcg_not_converged_and_start_vector(ctx, variant):
     g, A = assembled_system(ctx, 16)
     b = g["rhs"]
-    x, it, res, ok = ctx.pcg(A, b, tol=1e-10, max_iter=3, raise_on_fail=False)
+    x, it, res, ok = ctx.pcg(A, b, tol=1e-10, max_iter=3, raise_on_fail=False, precision=variant)
     assert not ok and it == 3 and res > 1e-10
     # restart from the partial solution converges (x holds the start vector on entry)
-    x2, it2, res2, ok2 = ctx.pcg(A, b, x0=x, tol=1e-10)
+    x2, it2, res2, ok2 = ctx.pcg(A, b, x0=x, tol=1e-10, precision=variant)
     assert ok2 and res2 <= 1e-10
     # zero right-hand side: converged at iteration 0
-    x3, it3, res3, ok3 = ctx.pcg(A, np.zeros_like(b), tol=1e-10, rel=False)
+    x3, it3, res3, ok3 = ctx.pcg(A, np.zeros_like(b), tol=1e-10, rel=False, precision=variant)
     assert ok3 and it3 == 0 and not np.any(x3)
     A.close()
 
@@ -97,8 +101,8 @@ def test_pcg_on_uploaded_matrix(ctx):
     M.sort_indices()
     b = rng.normal(size=n)
     A = ctx.csr_create(M.indptr, M.indices, M.data)
-    for pc in (D.PC_NONE, D.PC_JACOBI):
-        x, it, res, ok = ctx.pcg(A, b, pc=pc, tol=1e-12)
+    for pc, variant in [(pc, v) for pc in (D.PC_NONE, D.PC_JACOBI) for v in (D.FP64, D.FP64_CLASSIC)]:
+        x, it, res, ok = ctx.pcg(A, b, pc=pc, tol=1e-12, precision=variant)
         assert ok
         xd = np.linalg.solve(M.toarray(), b)
         assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-10
