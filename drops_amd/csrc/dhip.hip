@@ -1074,10 +1074,66 @@ int dhip_solver_probe(dhip_ctx_t *c, int32_t on, uint64_t *ticks, int32_t *nbloc
 
 int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
                int32_t restart, int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
-    (void)A; (void)pc; (void)b; (void)x; (void)where; (void)restart; (void)max_iter; (void)tol; (void)rel;
-    (void)left_pc;
-    if (!c) return DHIP_E_ARG;
-    return set_error(c, DHIP_E_STATE, "GMRES not available in this build");
+    if (!c || !A || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
+    if (pc != DHIP_PC_NONE && pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "unknown preconditioner");
+    // m = min(restart, max_iter) as the reference; at least one Arnoldi step per cycle
+    const int m = std::min<int>(restart, *max_iter);
+    if (m < 2 || m > KMAX) return set_error(c, DHIP_E_ARG, "GMRES restart must be in [2, %d]", KMAX);
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    if (n == 0) {
+        *max_iter = 0;
+        *tol = 0.0;
+        return DHIP_OK;
+    }
+    DevBuf bb, bx;
+    const double *db;
+    int rc = stage_in(c, bb, b, n, where, &db);
+    if (rc) return rc;
+    double *dx = x;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, bx.ensure(sizeof(double) * (n + 1)));
+        DHIP_HIPCHK(c, hipMemcpyAsync(bx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        dx = bx.as<double>();
+    }
+    DHIP_HIPCHK(c, c->kv[12].ensure(sizeof(double) * (n + 1)));
+    DHIP_HIPCHK(c, c->kv[15].ensure(sizeof(double) * (size_t)m * n + 8));
+    double *diag = c->kv[12].as<double>();
+    DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, diag, nullptr);
+    unsigned nb = 1;
+    int rpt = 1;
+    rc = persistent_geometry(c, n, &nb, &rpt);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, c->bar.ensure(sizeof(SyncSlot) * (nb + 1)));
+    DHIP_HIPCHK(c, c->red_tmp.ensure(sizeof(double) * 2 * (size_t)nb * KMAX + 64));
+    DHIP_HIPCHK(c, c->host_stage.ensure(64));
+    DHIP_LAUNCH(c, "zero_slots", k_zero_u32, blocks_for((int64_t)nb * sizeof(SyncSlot) / 4, 256), 256, 0,
+                c->bar.as<uint32_t>(), (int)(nb * sizeof(SyncSlot) / 4));
+    int32_t *outs = c->host_stage.as<int32_t>();
+    double *outd = reinterpret_cast<double *>(outs + 4);
+    GmresArgs ga{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx, c->kv[15].as<double>(),
+                 c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
+                 outs, outd, outs + 1};
+    const void *ks[5] = {(const void *)k_gmres<1>, (const void *)k_gmres<2>, (const void *)k_gmres<4>,
+                         (const void *)k_gmres<8>, (const void *)k_gmres<16>};
+    rc = launch_persistent(c, "gmres", ks, rpt, nb, ga);
+    if (rc) return rc;
+    int32_t hs[4];
+    double h_resid = 0.0;
+    DHIP_HIPCHK(c, hipMemcpyAsync(hs, outs, sizeof hs, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipMemcpyAsync(&h_resid, outd, sizeof h_resid, hipMemcpyDeviceToHost, c->stream));
+    if (where != DHIP_DEVICE)
+        DHIP_HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    bb.release();
+    bx.release();
+    rc = flush_profile(c);
+    if (rc) return rc;
+    *max_iter = hs[0];
+    *tol = h_resid;
+    if (hs[1] == 2) return set_error(c, DHIP_E_HIP, "grid synchronisation timed out in the persistent solver");
+    if (hs[1] != 0) return set_error(c, DHIP_E_NOT_CONVERGED, "GMRES did not converge: resid %g", h_resid);
+    return DHIP_OK;
 }
 
 }  // extern "C"

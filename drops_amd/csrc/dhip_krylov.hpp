@@ -614,6 +614,333 @@ done:
 }
 
 // ---------------------------------------------------------------------------
+// Vector-payload synchronisation: up to KMAX partial sums per block.  The
+// payload is written with plain stores before the release; the generation is
+// one 64-bit tagged word per slot.  After all generations are seen (acquire),
+// thread k sums partial k over the blocks in block order -- the same order in
+// every block.
+// ---------------------------------------------------------------------------
+constexpr int KMAX = 64;
+
+template <int BLOCK>
+__device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t g, const double *mine, int K,
+                                       double *out_sh) {
+    __shared__ int ok_sh;
+    const int nb = gridDim.x, par = g & 1;
+    double *P = parts + (size_t)par * nb * KMAX;
+    if (threadIdx.x < K) P[(size_t)blockIdx.x * KMAX + threadIdx.x] = mine[threadIdx.x];
+    wait_own_stores();
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const uint64_t tag = (uint64_t)g << 32;
+        if (lane == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(&slots[blockIdx.x].w[par][0], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        int ok = 1;
+        for (int base = 0; base < nb; base += 64) {
+            const int l = base + lane;
+            if (l < nb) {
+                uint32_t spins = 0;
+                while ((uint32_t)(__hip_atomic_load(&slots[l].w[par][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >>
+                                  32) != g) {
+                    if (++spins > SPIN_LIMIT) {
+                        ok = 0;
+                        break;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        const bool all_ok = __ballot(ok) == __ballot(1);
+        if (lane == 0) {
+            ok_sh = all_ok ? 1 : 0;
+            if (!all_ok) slots[blockIdx.x].timeout = 1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0.0;
+        for (int b = 0; b < nb; ++b) s += P[(size_t)b * KMAX + threadIdx.x];
+        out_sh[threadIdx.x] = s;
+    }
+    __syncthreads();
+    return ok_sh != 0;
+}
+
+// ---------------------------------------------------------------------------
+// persistent restarted GMRES(m), num/krylovsolver.h:320-463 semantics
+// (serial variant: m-1 Arnoldi steps per cycle, Givens rotations, right or left
+// Jacobi preconditioning, relative or absolute tolerance).  Orthogonalisation
+// is classical Gram-Schmidt with one re-orthogonalisation (CGS2): the i+1
+// dependent reductions of modified Gram-Schmidt become two fused ones; the norm
+// of the twice-projected vector is |w1|^2 - |h2|^2.  The small dense part
+// (Hessenberg column, rotations, back substitution) is computed redundantly and
+// identically by every block from the identical reduced values.
+// ---------------------------------------------------------------------------
+struct GmresArgs {
+    int64_t n;
+    const int32_t *rp, *col;
+    const double *val, *diag;   // diag == nullptr: no preconditioner
+    const double *b;
+    double *x;                  // start vector in, solution out (also exchanged for residuals)
+    double *V;                  // [m][n] Krylov basis
+    SyncSlot *slots;
+    double *parts;              // [2][gridDim.x][KMAX]
+    int m;
+    int max_iter;
+    double tol;
+    int rel;
+    int left;
+    int *out_iter;
+    double *out_resid;
+    int *out_status;
+};
+
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ double Hc[KMAX + 1];          // current Hessenberg column
+    __shared__ double Hs[KMAX * KMAX / 2 + KMAX];  // rotated upper triangle, packed by columns
+    __shared__ double cs[KMAX], sn[KMAX], sv[KMAX + 1], yv[KMAX];
+    __shared__ double red_sh[KMAX + 2];
+    __shared__ double msh[KMAX + 2];         // this block's partial sums
+    const int64_t n = a.n;
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r1 = r0 + per < n ? r0 + per : n;
+    const bool right = !a.left;
+    bool scaled;
+    // right preconditioning: the slice holds a_rc / d_c (A M v); left: plain values (M A v)
+    const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, a.val, right ? a.diag : nullptr, r0, r1, scaled);
+    const double *dg = a.diag;
+    auto spmv_row = [&](int64_t l, auto &&yv_) {
+        if (scaled || !right || !dg) return row_product<double>(S.rp, S.col, S.val, S.off, l, yv_);
+        return row_product<double>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
+    };
+    auto col_of = [](int k) { return k * (k + 1) / 2; };  // start of packed column k (rows 0..k)
+    const int m = a.m;
+    uint32_t gen = 0;
+    int status = 1, j = 1;
+    double resid = 0.0, normb = 1.0, beta = 0.0;
+    bool own[RPT];
+    int64_t lr[RPT];
+    double xr[RPT], wr[RPT], dr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = r0 + lr[k] < r1;
+        if (own[k]) {
+            xr[k] = a.x[r0 + lr[k]];
+            dr[k] = dg ? dg[r0 + lr[k]] : 1.0;
+        }
+    }
+    auto prec = [&](int k, double v) { return dg ? 1.0 * v / dr[k] : v; };
+    // residual r = b - A x (left: M r) into wr; returns block partials |r|^2 (and |b|^2 or |M b|^2)
+    auto residual = [&](bool with_b, double &p_rr, double &p_bb) {
+        p_rr = 0.0;
+        p_bb = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            const double ax = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
+            double rv = a.b[r] - ax;
+            double bv = a.b[r];
+            if (!right) {
+                rv = prec(k, rv);
+                bv = prec(k, bv);
+            }
+            wr[k] = rv;
+            p_rr += rv * rv;
+            if (with_b) p_bb += bv * bv;
+        }
+        p_rr = block_sum<PB>(p_rr);
+        if (with_b) p_bb = block_sum<PB>(p_bb);
+    };
+    // x += (M) V y over columns 0..kk of the basis
+    auto update_x = [&](int kk) {
+        if (threadIdx.x == 0) {
+            for (int i = kk; i >= 0; --i) yv[i] = sv[i];
+            for (int i = kk; i >= 0; --i) {
+                yv[i] /= Hs[col_of(i) + i];
+                for (int jj = i - 1; jj >= 0; --jj) yv[jj] -= Hs[col_of(i) + jj] * yv[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            double z = 0.0;
+            for (int i = 0; i <= kk; ++i) z += yv[i] * a.V[(size_t)i * n + r];
+            xr[k] += right ? prec(k, z) : z;
+        }
+    };
+
+    {
+        double p_rr, p_bb;
+        residual(true, p_rr, p_bb);
+        if (threadIdx.x == 0) {
+            msh[0] = p_rr;
+            msh[1] = p_bb;
+        }
+        if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 2, red_sh)) { status = 2; goto done; }
+        beta = sqrt(red_sh[0]);
+        normb = sqrt(red_sh[1]);
+        if (normb == 0.0 || !a.rel) normb = 1.0;
+        resid = beta / normb;
+        if (resid <= a.tol) {
+            status = 0;
+            j = 0;
+            goto done;
+        }
+    }
+    while (j <= a.max_iter) {
+        // v0 = r / beta
+        {
+            const double inv = 1.0 / beta;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (own[k]) a.V[r0 + lr[k]] = wr[k] * inv;
+            if (threadIdx.x < KMAX + 1) sv[threadIdx.x] = 0.0;
+            if (threadIdx.x == 0) sv[0] = beta;
+            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+        }
+        int i;
+        bool converged = false;
+        for (i = 0; i < m - 1 && j <= a.max_iter; ++i, ++j) {
+            const double *vi = a.V + (size_t)i * n;
+            // w = A M v_i (right) or M A v_i (left); h1 = V^T w
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                double w = spmv_row(lr[k], [&](int32_t c) { return vi[c]; });
+                if (!right) w = prec(k, w);
+                wr[k] = w;
+            }
+            for (int kk = 0; kk <= i; ++kk) {
+                double pk = 0.0;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k)
+                    if (own[k]) pk += wr[k] * a.V[(size_t)kk * n + r0 + lr[k]];
+                pk = block_sum<PB>(pk);
+                if (threadIdx.x == 0) msh[kk] = pk;
+            }
+            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 1, red_sh)) { status = 2; goto done; }
+            if (threadIdx.x <= i) Hc[threadIdx.x] = red_sh[threadIdx.x];
+            __syncthreads();
+            // w1 = w - V h1; h2 = V^T w1; |w1|^2
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                double w = wr[k];
+                for (int kk = 0; kk <= i; ++kk) w -= Hc[kk] * a.V[(size_t)kk * n + r];
+                wr[k] = w;
+            }
+            for (int kk = 0; kk <= i + 1; ++kk) {
+                double pk = 0.0;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k)
+                    if (own[k]) pk += wr[k] * (kk <= i ? a.V[(size_t)kk * n + r0 + lr[k]] : wr[k]);
+                pk = block_sum<PB>(pk);
+                if (threadIdx.x == 0) msh[kk] = pk;
+            }
+            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 2, red_sh)) { status = 2; goto done; }
+            double hn2 = red_sh[i + 1];
+            for (int kk = 0; kk <= i; ++kk) hn2 -= red_sh[kk] * red_sh[kk];
+            const double hnorm = sqrt(hn2 > 0.0 ? hn2 : 0.0);
+            // v_{i+1} = (w1 - V h2) / |.|
+            {
+                const double inv = 1.0 / hnorm;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) {
+                    if (!own[k]) continue;
+                    const int64_t r = r0 + lr[k];
+                    double w = wr[k];
+                    for (int kk = 0; kk <= i; ++kk) w -= red_sh[kk] * a.V[(size_t)kk * n + r];
+                    a.V[(size_t)(i + 1) * n + r] = w * inv;
+                }
+            }
+            // Hessenberg column i: h1 + h2, then the Givens rotations (num/krylovsolver.h:258-280, 392-399)
+            if (threadIdx.x == 0) {
+                for (int kk = 0; kk <= i; ++kk) Hc[kk] += red_sh[kk];
+                Hc[i + 1] = hnorm;
+                for (int kk = 0; kk < i; ++kk) {
+                    const double t = cs[kk] * Hc[kk] + sn[kk] * Hc[kk + 1];
+                    Hc[kk + 1] = -sn[kk] * Hc[kk] + cs[kk] * Hc[kk + 1];
+                    Hc[kk] = t;
+                }
+                const double dx = Hc[i], dy = Hc[i + 1];
+                if (dy == 0.0) {
+                    cs[i] = 1.0;
+                    sn[i] = 0.0;
+                } else {
+                    const double rr = sqrt(dx * dx + dy * dy);
+                    cs[i] = dx / rr;
+                    sn[i] = dy / rr;
+                }
+                {
+                    const double t = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
+                    Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
+                    Hc[i] = t;
+                }
+                {
+                    const double t = cs[i] * sv[i] + sn[i] * sv[i + 1];
+                    sv[i + 1] = -sn[i] * sv[i] + cs[i] * sv[i + 1];
+                    sv[i] = t;
+                }
+                for (int kk = 0; kk <= i; ++kk) Hs[col_of(i) + kk] = Hc[kk];
+            }
+            __syncthreads();
+            resid = fabs(sv[i + 1]) / normb;
+            if (resid <= a.tol) {
+                update_x(i);
+                converged = true;
+                break;
+            }
+            // v_{i+1} visible to all blocks before the next SpMV
+            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+        }
+        if (converged) {
+            status = 0;
+            break;
+        }
+        update_x(i - 1);
+        // restart: publish x, r = b - A x (left: M r)
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k]) a.x[r0 + lr[k]] = xr[k];
+        if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+        {
+            double p_rr, p_bb;
+            residual(false, p_rr, p_bb);
+            if (threadIdx.x == 0) msh[0] = p_rr;
+            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 1, red_sh)) { status = 2; goto done; }
+            beta = sqrt(red_sh[0]);
+            resid = beta / normb;
+            if (resid <= a.tol) {
+                status = 0;
+                break;
+            }
+        }
+    }
+done:
+    if (status != 2) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k]) a.x[r0 + lr[k]] = xr[k];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.out_iter = status == 0 ? j : a.max_iter;
+        *a.out_resid = resid;
+        *a.out_status = status;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // persistent PCG with FP32 inner iterations and FP64 iterative refinement
 //   outer (FP64): r = b - A x with the FP64 values, resid = |r|/|b|, stop at tol
 //   inner (FP32 values and vectors, FP64 dot accumulation): PCG on A d = r/|r|

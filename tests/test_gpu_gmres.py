@@ -1,0 +1,78 @@
+"""GPU GMRES(m) against the oracle's serial GMRES (num/krylovsolver.h:320-463) and
+direct solves.  Bar: solutions within 1e-9 relative of the direct solution at
+tol 1e-12; iteration counts within 5% (+2) of the oracle's (the device
+orthogonalises by CGS2, the reference by MGS); failure reported as `false`
+with max_iter unchanged, as the reference."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import cases
+import oracle as O
+from drops_amd import dhip as D
+
+pytestmark = pytest.mark.gpu
+
+
+def nonsym(n, seed, density=0.03):
+    M = sp.random(n, n, density=density, random_state=seed)
+    M = (M + sp.eye(n) * 6.0).tocsr()
+    M.sort_indices()
+    return M
+
+
+@pytest.mark.parametrize("left", [False, True])
+@pytest.mark.parametrize("pc", [D.PC_NONE, D.PC_JACOBI])
+def test_gmres_nonsymmetric(ctx, left, pc):
+    print(f"[gmres] left={left} pc={pc}", flush=True)
+    M = nonsym(400, 11)
+    b = np.random.default_rng(3).normal(size=400)
+    A = ctx.csr_create(M.indptr, M.indices, M.data)
+    x, it, res, ok = ctx.gmres(A, b, pc=pc, restart=20, tol=1e-12, left_pc=left)
+    assert ok and res <= 1e-12
+    xd = np.linalg.solve(M.toarray(), b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    xo, ito, reso, oko = O.gmres_jacobi(M.indptr.astype(np.int64), M.indices.astype(np.int32), M.data, b,
+                                        restart=20, tol=1e-12, left_pc=left, use_pc=pc == D.PC_JACOBI)
+    assert oko and abs(it - ito) <= 0.05 * ito + 2, (it, ito)
+    A.close()
+
+
+def test_gmres_tracefem_system_with_convection(ctx):
+    """The assembled configuration-1 operator plus a skew-symmetric perturbation on its pattern
+    (a convection-like nonsymmetric system), restart 50 and 10."""
+    mesh, phi = cases.c1_sphere(32)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    ctx.assemble_p1(coef=(1.0, 1.0, 1.0), rhs_kind=D.RHS_3XY_MINUS_X3, lc=(1.0, 1.0, mesh.h))
+    g = ctx.csr()
+    n = g["rowptr"].size - 1
+    rp, col = g["rowptr"].astype(np.int64), g["colind"]
+    L = sp.csr_matrix((g["comb"], col, rp), shape=(n, n))
+    K = sp.csr_matrix((np.random.default_rng(1).normal(size=col.size) * 0.05 * np.abs(g["comb"]).max(), col, rp),
+                      shape=(n, n))
+    Ms = (L + (K - K.T)).tocsr()
+    Ms.sort_indices()
+    b = g["rhs"]
+    A = ctx.csr_create(Ms.indptr, Ms.indices, Ms.data)
+    for m in (50, 10):
+        x, it, res, ok = ctx.gmres(A, b, pc=D.PC_JACOBI, restart=m, tol=1e-10, max_iter=20000)
+        assert ok
+        assert np.linalg.norm(b - Ms @ x) / np.linalg.norm(b) < 1e-9
+        xo, ito, _, oko = O.gmres_jacobi(Ms.indptr.astype(np.int64), Ms.indices.astype(np.int32), Ms.data, b,
+                                         restart=m, tol=1e-10, max_iter=20000)
+        assert oko and abs(it - ito) <= 0.05 * ito + 2, (m, it, ito)
+    A.close()
+
+
+def test_gmres_not_converged_and_restart_two(ctx):
+    M = nonsym(200, 5)
+    b = np.ones(200)
+    A = ctx.csr_create(M.indptr, M.indices, M.data)
+    x, it, res, ok = ctx.gmres(A, b, restart=10, max_iter=3, tol=1e-14, raise_on_fail=False)
+    assert not ok and it == 3 and res > 1e-14
+    x, it, res, ok = ctx.gmres(A, b, restart=2, max_iter=5000, tol=1e-10)
+    assert ok
+    with pytest.raises(D.DhipError):
+        ctx.gmres(A, b, restart=1, tol=1e-10)
+    A.close()
