@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from drops_amd import dhip as D  # noqa: E402
+from drops_amd import tune  # noqa: E402
 from drops_amd.mesh import KuhnMesh, config_shift, sphere  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -111,6 +112,26 @@ def cpu_baseline(mesh, phi, coef, lc, budget_s=30.0):
                 threads=threads, reps=len(times), iters=out[1]), out
 
 
+def probe_sweep(ctx, ndof, x_dev, zero_dev):
+    A = ctx.csr_from_assembly(D.OP_COMB)
+    rhs = ctx.device_arrays()[3]
+    for rows in (128, 256, 512, 1024, 2048, 4096):
+        ctx.tune("pcg_rows_per_block", rows)
+        ctx.solver_probe(True)
+        ts = []
+        for _ in range(4):
+            ctx.copy_d2d(x_dev, zero_dev, 8 * ndof)
+            t0 = time.perf_counter()
+            _, it, res, ok = ctx.pcg(A, rhs, x_dev, tol=TOL, where=D.DEVICE)
+            ts.append(time.perf_counter() - t0)
+        pr = ctx.solver_probe(False).astype(np.float64) / 100.0  # 100 MHz ticks -> us
+        print(json.dumps(dict(rows_per_block=rows, blocks=int(pr.shape[0]), iters=it, ms_per_solve=1e3 * min(ts),
+                              compute_us_per_iter=float((pr[:, 1] / it).mean()),
+                              sync_us_per_iter=float((pr[:, 2] / it).mean()),
+                              setup_us=float(pr[:, 0].mean()))), flush=True)
+    A.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +140,9 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-copies", type=int, default=80)
+    ap.add_argument("--probe-sweep", action="store_true",
+                    help="print the per-iteration compute/sync breakdown of the persistent PCG for a sweep of "
+                         "rows per block and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; the multi-GPU sharding is "
@@ -141,6 +165,15 @@ def main():
     nbytes = 8 * ndof
     x_dev, zero_dev = ctx.malloc(nbytes), ctx.to_device(np.zeros(ndof))
     log(f"[bench] ncut={ncut} ndof={ndof} nnz={nnz}")
+    if args.probe_sweep:
+        probe_sweep(ctx, ndof, x_dev, zero_dev)
+        return
+    # block geometry of the persistent solver, tuned on this matrix outside the timed region
+    A0 = ctx.csr_from_assembly(D.OP_COMB)
+    rhs0 = ctx.device_arrays()[3]
+    rows_best, tune_t = tune.tune_pcg(ctx, A0, rhs0, x_dev, zero_dev, ndof, tol=TOL)
+    A0.close()
+    log(f"[bench] tuned pcg_rows_per_block={rows_best} {tune_t}")
 
     results = {}
     for name, prec in (("fp64", D.FP64), ("fp64_classic", D.FP64_CLASSIC), ("fp32ir", D.FP32_IR)):
@@ -210,7 +243,8 @@ def main():
         "config": {"workload": "P1 TraceFEM LB+mass+h*normal-stab, sphere |x|^2-1 on [-2,2]^3, "
                                f"{n}^3 Kuhn cells, assembly + Jacobi-PCG to rel. residual 1e-8",
                    "cells": n, "cut_tets": ncut, "dofs": ndof, "nnz": nnz,
-                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_iters_fp32ir_inner": results["fp32ir"]["iters"]},
+                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_iters_fp32ir_inner": results["fp32ir"]["iters"],
+                   "pcg_rows_per_block": rows_best},
         "fp32ir_s_per_solve": results["fp32ir"]["s_per_solve"],
         "fp64_classic_s_per_solve": results["fp64_classic"]["s_per_solve"],
         "pcg_iters_fp64_classic": results["fp64_classic"]["iters"],

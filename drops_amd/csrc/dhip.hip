@@ -915,7 +915,9 @@ int dhip_axpy(dhip_ctx_t *c, int64_t n, double a, const double *x, double *y, in
 // fit 512 x (CUs) threads, else RPT rows per thread; one block per CU at most.
 static int persistent_geometry(dhip_ctx *c, int64_t n, unsigned *nb, int *rpt) {
     const int64_t cap_blocks = c->cu_count;
-    int64_t rows = c->pcg_rows_per_block > 0 ? c->pcg_rows_per_block : PB;
+    // default 256 rows per block: measured optimum of compute (gathers) vs synchronisation
+    // (block count) for 30k-row systems; drops_amd/tune.py re-tunes per matrix
+    int64_t rows = c->pcg_rows_per_block > 0 ? c->pcg_rows_per_block : 256;
     rows = std::max<int64_t>(rows, (n + cap_blocks - 1) / cap_blocks);  // at most one block per CU
     int r = 1;
     while ((int64_t)r * PB < rows && r < 16) r *= 2;

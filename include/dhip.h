@@ -97,7 +97,7 @@ int dhip_profile_reset(dhip_ctx_t *ctx);
  * the last solve's values and `nblocks` its block count. */
 int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
 /* Tuning parameters (chosen by drops_amd/tune.py): "pcg_rows_per_block" -- rows owned
- * by one block of the persistent solvers (0 = 512, one per thread). */
+ * by one block of the persistent solvers (0 = library default, 256). */
 int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);
 /* Device buffers for callers without their own allocator (bench, tests). */
 int dhip_malloc(dhip_ctx_t *ctx, void **dptr, int64_t bytes);
