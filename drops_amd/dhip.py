@@ -5,6 +5,7 @@ be loaded, every entry point raises -- there is no CPU fallback.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -103,10 +104,12 @@ def _vp(a):
 
 
 class Csr:
-    """Device CSR matrix handle (MatrixCL mirror)."""
+    """Device CSR matrix handle (MatrixCL mirror).  Owned by its Context: closing the
+    context closes its matrices first (a handle must not outlive its context)."""
 
     def __init__(self, ctx, handle):
         self.ctx, self.h = ctx, handle
+        ctx._csrs.add(self)
 
     @property
     def shape(self):
@@ -115,9 +118,10 @@ class Csr:
         return n.value, z.value
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             self.ctx.L.dhip_csr_destroy(self.h)
-            self.h = None
+        self.h = None
+        self.ctx._csrs.discard(self)
 
     def __del__(self):
         try:
@@ -129,6 +133,7 @@ class Csr:
 class Context:
     def __init__(self, device=0):
         self.L = load()
+        self._csrs = weakref.WeakSet()
         self.h = C.c_void_p()
         rc = self.L.dhip_create(C.byref(self.h), device)
         if rc != OK:
@@ -140,6 +145,8 @@ class Context:
         return rc
 
     def close(self):
+        for A in list(self._csrs):
+            A.close()
         if self.h:
             self.L.dhip_destroy(self.h)
             self.h = None

@@ -161,6 +161,7 @@ int dhip_csr_create(dhip_ctx_t *ctx, int64_t nrows, int64_t nnz, const int32_t *
 /* wraps the assembled operator `op` (DHIP_OP_*) without copying; valid until the
  * next assembly on the context */
 int dhip_csr_from_assembly(dhip_ctx_t *ctx, int32_t op, dhip_csr_t **A);
+/* Handles must be destroyed before the context that created them. */
 void dhip_csr_destroy(dhip_csr_t *A);
 int dhip_csr_info(const dhip_csr_t *A, int64_t *nrows, int64_t *nnz);
 

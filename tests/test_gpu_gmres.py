@@ -40,7 +40,8 @@ def test_gmres_nonsymmetric(ctx, left, pc):
 
 def test_gmres_tracefem_system_with_convection(ctx):
     """The assembled configuration-1 operator plus a skew-symmetric perturbation on its pattern
-    (a convection-like nonsymmetric system), restart 50 and 10."""
+    (a convection-like nonsymmetric system), restart 50 and 30 (the oracle needs 563 and 916
+    iterations; restart 10 stagnates there as well)."""
     mesh, phi = cases.c1_sphere(32)
     ctx.mesh(mesh)
     ctx.levelset(phi)
@@ -55,7 +56,7 @@ def test_gmres_tracefem_system_with_convection(ctx):
     Ms.sort_indices()
     b = g["rhs"]
     A = ctx.csr_create(Ms.indptr, Ms.indices, Ms.data)
-    for m in (50, 10):
+    for m in (50, 30):
         x, it, res, ok = ctx.gmres(A, b, pc=D.PC_JACOBI, restart=m, tol=1e-10, max_iter=20000)
         assert ok
         assert np.linalg.norm(b - Ms @ x) / np.linalg.norm(b) < 1e-9
