@@ -940,7 +940,16 @@ static int launch_persistent(dhip_ctx *c, const char *name, const void *const ke
         return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
     void *argv[] = {&args};
     ProfScope ps(c, name);
-    DHIP_HIPCHK(c, hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
+    if (!c->coop_refused) {
+        const hipError_t e = hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream);
+        if (e == hipSuccess) return DHIP_OK;
+        // The cooperative-launch resource is per device and can be held by another process.
+        // The grid is already capped at one block per CU (and checked against occupancy
+        // above) and every wait is bounded, so launch the same grid normally instead.
+        (void)hipGetLastError();
+        c->coop_refused = true;
+    }
+    DHIP_HIPCHK(c, hipLaunchKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
     return DHIP_OK;
 }
 
