@@ -112,6 +112,19 @@ def cpu_baseline(mesh, phi, coef, lc, budget_s=30.0):
                 threads=threads, reps=len(times), iters=out[1]), out
 
 
+def measured_traffic(kernel, rows, ndof):
+    """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
+    each scaled by a known-size copy), when it was measured at this block geometry and system size."""
+    path = os.path.join(ROOT, "profiles", "pcg64_c2_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 5 and f[0] == "traffic" and f[1] == kernel and int(f[2]) == rows and int(f[3]) == ndof:
+            return float(f[4])
+    return None
+
+
 def probe_sweep(ctx, ndof, x_dev, zero_dev):
     A = ctx.csr_from_assembly(D.OP_COMB)
     rhs = ctx.device_arrays()[3]
@@ -140,6 +153,13 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-copies", type=int, default=80)
+    ap.add_argument("--rows-per-block", type=int, default=0,
+                    help="fix the persistent solver's rows per block instead of tuning it (profiling runs)")
+    ap.add_argument("--variants", default="fp64,fp64_classic,fp32ir",
+                    help="comma-separated solver forms to time; the first must be fp64")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (the known-size kernel that "
+                         "scales the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--probe-sweep", action="store_true",
                     help="print the per-iteration compute/sync breakdown of the persistent PCG for a sweep of "
                          "rows per block and exit")
@@ -168,15 +188,32 @@ def main():
     if args.probe_sweep:
         probe_sweep(ctx, ndof, x_dev, zero_dev)
         return
-    # block geometry of the persistent solver, tuned on this matrix outside the timed region
-    A0 = ctx.csr_from_assembly(D.OP_COMB)
-    rhs0 = ctx.device_arrays()[3]
-    rows_best, tune_t = tune.tune_pcg(ctx, A0, rhs0, x_dev, zero_dev, ndof, tol=TOL)
-    A0.close()
-    log(f"[bench] tuned pcg_rows_per_block={rows_best} {tune_t}")
+    if args.copy_mib:
+        cb = args.copy_mib << 20
+        src, dst = ctx.malloc(cb), ctx.malloc(cb)
+        ctx.copy_d2d(dst, src, cb)
+        ctx.synchronize()
+        ctx.free(src)
+        ctx.free(dst)
+        log(f"[bench] copy kernel: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+    if args.rows_per_block:
+        rows_best = args.rows_per_block
+        ctx.tune("pcg_rows_per_block", rows_best)
+    else:
+        # block geometry of the persistent solver, tuned on this matrix outside the timed region
+        A0 = ctx.csr_from_assembly(D.OP_COMB)
+        rhs0 = ctx.device_arrays()[3]
+        rows_best, tune_t = tune.tune_pcg(ctx, A0, rhs0, x_dev, zero_dev, ndof, tol=TOL)
+        A0.close()
+        log(f"[bench] tuned pcg_rows_per_block={rows_best} {tune_t}")
 
+    forms = {"fp64": D.FP64, "fp64_classic": D.FP64_CLASSIC, "fp32ir": D.FP32_IR}
+    variants = [v for v in args.variants.split(",") if v]
+    if not variants or variants[0] != "fp64" or any(v not in forms for v in variants):
+        sys.exit(f"--variants must start with fp64 and name only {sorted(forms)}")
     results = {}
-    for name, prec in (("fp64", D.FP64), ("fp64_classic", D.FP64_CLASSIC), ("fp32ir", D.FP32_IR)):
+    for name in variants:
+        prec = forms[name]
         for _ in range(args.warmup):
             solve_once(ctx, mesh, x_dev, zero_dev, nbytes, prec, coef, lc)
         ctx.synchronize()
@@ -209,7 +246,7 @@ def main():
     achieved = bytes_per_iter * iters / (pk["avg_us"] * 1e-6) / 1e9
     cls = results["fp64"]["kernels"]["classify"]
     cls_bytes = 8 * phi.size + mesh.n[0] * mesh.n[1] * mesh.n[2] * (1 + 0)  # level set read + cell mask
-    spmv = spmv_bandwidth(ctx, g, args.spmv_copies)
+    spmv = spmv_bandwidth(ctx, g, args.spmv_copies) if args.spmv_copies else None
     log(f"[bench] spmv {spmv}")
 
     cpu = None
@@ -243,18 +280,15 @@ def main():
         "config": {"workload": "P1 TraceFEM LB+mass+h*normal-stab, sphere |x|^2-1 on [-2,2]^3, "
                                f"{n}^3 Kuhn cells, assembly + Jacobi-PCG to rel. residual 1e-8",
                    "cells": n, "cut_tets": ncut, "dofs": ndof, "nnz": nnz,
-                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_iters_fp32ir_inner": results["fp32ir"]["iters"],
-                   "pcg_rows_per_block": rows_best},
-        "fp32ir_s_per_solve": results["fp32ir"]["s_per_solve"],
-        "fp64_classic_s_per_solve": results["fp64_classic"]["s_per_solve"],
-        "pcg_iters_fp64_classic": results["fp64_classic"]["iters"],
+                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_rows_per_block": rows_best},
+        "variants": {k: {"s_per_solve": r["s_per_solve"], "iters": r["iters"]} for k, r in results.items()},
         "err_vs_fp64_oracle": err or None,
-        "spmv_hbm_gbs": spmv["gbs"],
-        "spmv": {k: spmv[k] for k in ("nnz", "rows", "csr_mb", "avg_us", "copies")},
+        "spmv_hbm_gbs": spmv["gbs"] if spmv else None,
+        "spmv": {k: spmv[k] for k in ("nnz", "rows", "csr_mb", "avg_us", "copies")} if spmv else None,
         "kernels_us": {k: round(v2["avg_us"], 2) for k, v2 in results["fp64"]["kernels"].items()},
         "classify_gbs": cls_bytes / (cls["avg_us"] * 1e-6) / 1e9,
         "roofline": {"kernel": "pcg64", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pcg64", rows_best, ndof),
                      "bytes_per_iter": bytes_per_iter, "iters": iters, "avg_launch_us": pk["avg_us"]},
         "learned": None,
     }

@@ -4,6 +4,7 @@
 #include "dhip_common.hpp"
 #include "dhip_assembly.hpp"
 #include "dhip_krylov.hpp"
+#include "dhip_ch.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -1083,37 +1084,16 @@ int dhip_solver_probe(dhip_ctx_t *c, int32_t on, uint64_t *ticks, int32_t *nbloc
     return DHIP_OK;
 }
 
-int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
-               int32_t restart, int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
-    if (!c || !A || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
-    if (pc != DHIP_PC_NONE && pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "unknown preconditioner");
-    // m = min(restart, max_iter) as the reference; at least one Arnoldi step per cycle
-    const int m = std::min<int>(restart, *max_iter);
-    if (m < 2 || m > KMAX) return set_error(c, DHIP_E_ARG, "GMRES restart must be in [2, %d]", KMAX);
-    DHIP_HIPCHK(c, hipSetDevice(c->device));
+}  // extern "C"
+
+// Persistent GMRES(m) on device vectors: diag == nullptr means no preconditioner.
+static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const double *db, double *dx, int m,
+                      int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
     const int64_t n = A->n;
-    if (n == 0) {
-        *max_iter = 0;
-        *tol = 0.0;
-        return DHIP_OK;
-    }
-    DevBuf bb, bx;
-    const double *db;
-    int rc = stage_in(c, bb, b, n, where, &db);
-    if (rc) return rc;
-    double *dx = x;
-    if (where != DHIP_DEVICE) {
-        DHIP_HIPCHK(c, bx.ensure(sizeof(double) * (n + 1)));
-        DHIP_HIPCHK(c, hipMemcpyAsync(bx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-        dx = bx.as<double>();
-    }
-    DHIP_HIPCHK(c, c->kv[12].ensure(sizeof(double) * (n + 1)));
     DHIP_HIPCHK(c, c->kv[15].ensure(sizeof(double) * (size_t)m * n + 8));
-    double *diag = c->kv[12].as<double>();
-    DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, diag, nullptr);
     unsigned nb = 1;
     int rpt = 1;
-    rc = persistent_geometry(c, n, &nb, &rpt);
+    int rc = persistent_geometry(c, n, &nb, &rpt);
     if (rc) return rc;
     DHIP_HIPCHK(c, c->bar.ensure(sizeof(SyncSlot) * (nb + 1)));
     DHIP_HIPCHK(c, c->red_tmp.ensure(sizeof(double) * 2 * (size_t)nb * KMAX + 64));
@@ -1122,7 +1102,7 @@ int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, 
                 c->bar.as<uint32_t>(), (int)(nb * sizeof(SyncSlot) / 4));
     int32_t *outs = c->host_stage.as<int32_t>();
     double *outd = reinterpret_cast<double *>(outs + 4);
-    GmresArgs ga{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx, c->kv[15].as<double>(),
+    GmresArgs ga{n, A->rowptr, A->col, A->val, diag, db, dx, c->kv[15].as<double>(),
                  c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
                  outs, outd, outs + 1};
     const void *ks[5] = {(const void *)k_gmres<1>, (const void *)k_gmres<2>, (const void *)k_gmres<4>,
@@ -1133,18 +1113,311 @@ int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, 
     double h_resid = 0.0;
     DHIP_HIPCHK(c, hipMemcpyAsync(hs, outs, sizeof hs, hipMemcpyDeviceToHost, c->stream));
     DHIP_HIPCHK(c, hipMemcpyAsync(&h_resid, outd, sizeof h_resid, hipMemcpyDeviceToHost, c->stream));
-    if (where != DHIP_DEVICE)
-        DHIP_HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
-    bb.release();
-    bx.release();
-    rc = flush_profile(c);
-    if (rc) return rc;
     *max_iter = hs[0];
     *tol = h_resid;
     if (hs[1] == 2) return set_error(c, DHIP_E_HIP, "grid synchronisation timed out in the persistent solver");
     if (hs[1] != 0) return set_error(c, DHIP_E_NOT_CONVERGED, "GMRES did not converge: resid %g", h_resid);
     return DHIP_OK;
+}
+
+static int gmres_restart(dhip_ctx *c, int32_t restart, int32_t max_iter, int *m) {
+    // m = min(restart, max_iter) as the reference; at least one Arnoldi step per cycle
+    *m = std::min<int>(restart, max_iter);
+    if (*m < 2 || *m > KMAX) return set_error(c, DHIP_E_ARG, "GMRES restart must be in [2, %d]", KMAX);
+    return DHIP_OK;
+}
+
+// Shared host wrapper: stage vectors, run, copy the solution back.
+static int gmres_entry(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double *pcdiag, const double *b, double *x,
+                       int32_t where, int32_t restart, int32_t *max_iter, double *tol, int32_t rel,
+                       int32_t left_pc) {
+    int m = 0;
+    int rc = gmres_restart(c, restart, *max_iter, &m);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    if (n == 0) {
+        *max_iter = 0;
+        *tol = 0.0;
+        return DHIP_OK;
+    }
+    DevBuf bb, bx, bd;
+    const double *db;
+    rc = stage_in(c, bb, b, n, where, &db);
+    if (rc) return rc;
+    double *dx = x;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, bx.ensure(sizeof(double) * (n + 1)));
+        DHIP_HIPCHK(c, hipMemcpyAsync(bx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+        dx = bx.as<double>();
+    }
+    const double *diag = nullptr;
+    if (pcdiag) {
+        rc = stage_in(c, bd, pcdiag, n, where, &diag);
+        if (rc) return rc;
+    } else if (pc == DHIP_PC_JACOBI) {
+        DHIP_HIPCHK(c, c->kv[12].ensure(sizeof(double) * (n + 1)));
+        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val,
+                    c->kv[12].as<double>(), nullptr);
+        diag = c->kv[12].as<double>();
+    }
+    rc = gmres_core(c, A, diag, db, dx, m, max_iter, tol, rel, left_pc);
+    if ((rc == DHIP_OK || rc == DHIP_E_NOT_CONVERGED) && where != DHIP_DEVICE) {
+        if (hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            rc = set_error(c, DHIP_E_HIP, "copy of the GMRES solution failed");
+    }
+    bb.release();
+    bx.release();
+    bd.release();
+    const int rp = flush_profile(c);
+    return rc ? rc : rp;
+}
+
+extern "C" {
+
+int dhip_gmres(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
+               int32_t restart, int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
+    if (!c || !A || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
+    if (pc != DHIP_PC_NONE && pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "unknown preconditioner");
+    return gmres_entry(c, A, pc, nullptr, b, x, where, restart, max_iter, tol, rel, left_pc);
+}
+
+int dhip_gmres_diag(dhip_ctx_t *c, const dhip_csr_t *A, const double *pcdiag, const double *b, double *x,
+                    int32_t where, int32_t restart, int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
+    if (!c || !A || !pcdiag || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
+    return gmres_entry(c, A, DHIP_PC_JACOBI, pcdiag, b, x, where, restart, max_iter, tol, rel, left_pc);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard time stepping (CahnHilliardcGP1CL, theta = 1, resting interface)
+// ---------------------------------------------------------------------------
+struct dhip_ch {
+    dhip_ctx *ctx = nullptr;
+    dhip_ch_params_t p{};
+    ChCoef k{};
+    MeshP mesh{};
+    int64_t ncut = 0, ndof = 0, nnz = 0;
+    // snapshot of the assembly the system was built from
+    DevBuf phi, cut, cellinfo, flags, dofs, rowowner, massloc, rowptr, col, mass, lb, stab;
+    // step system and state [mu; c]
+    DevBuf tet, brp, bcol, bval, pcdiag, rhs, x;
+    dhip_csr K;
+    bool prepared = false;
+};
+
+static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
+    DHIP_HIPCHK(c, dst.ensure(bytes + 8));
+    if (bytes) DHIP_HIPCHK(c, hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return DHIP_OK;
+}
+
+static void ch_release(dhip_ch *h) {
+    DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
+                      &h->col, &h->mass, &h->lb, &h->stab, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
+                      &h->rhs, &h->x};
+    for (DevBuf *b : bufs) b->release();
+}
+
+static int ch_prepare_impl(dhip_ch *h) {
+    dhip_ctx *c = h->ctx;
+    const int64_t N = h->ncut, R = h->ndof;
+    ChTetArgs ta{h->mesh, h->phi.as<double>(), h->cut.as<int64_t>(), N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(),
+                 h->massloc.as<double>(), h->x.as<double>() + R, h->tet.as<double>()};
+    if (N > 0) DHIP_LAUNCH(c, "ch_tet", k_ch_tet, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, ta);
+    ChRowArgs ra{h->mesh, h->cut.as<int64_t>(), h->cellinfo.as<uint32_t>(), h->flags.as<uint8_t>(),
+                 h->dofs.as<int32_t>(), h->rowowner.as<int32_t>(), R, N, h->tet.as<double>(),
+                 h->rowptr.as<int32_t>(), h->mass.as<double>(), h->stab.as<double>(), h->bval.as<double>(),
+                 h->pcdiag.as<double>(), h->rhs.as<double>(), h->k};
+    if (R > 0) DHIP_LAUNCH(c, "ch_rows", k_ch_rows, blocks_for(R, 128), 128, 0, ra);
+    h->prepared = true;
+    return DHIP_OK;
+}
+
+static int ch_solve_impl(dhip_ch *h, int32_t *iters, double *resid) {
+    dhip_ctx *c = h->ctx;
+    int m = 0;
+    int rc = gmres_restart(c, h->p.restart, h->p.max_iter, &m);
+    if (rc) return rc;
+    int32_t it = h->p.max_iter;
+    double tol = h->p.tol;
+    rc = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), h->x.as<double>(), m, &it, &tol,
+                    h->p.rel, 0);
+    h->prepared = false;
+    if (iters) *iters = it;
+    if (resid) *resid = tol;
+    return rc;
+}
+
+extern "C" {
+
+int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p, dhip_ch_t **out, int64_t *ndof) {
+    if (!c || !p || !out) return DHIP_E_ARG;
+    if (!(p->dt > 0.0) || p->restart < 2 || p->restart > KMAX || p->max_iter < 1 || !(p->tol > 0.0))
+        return set_error(c, DHIP_E_ARG, "invalid Cahn-Hilliard parameters");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const double one3[3] = {1.0, 1.0, 1.0};
+    int rc = classify_impl(c, zero_policy, nullptr, false);
+    if (!rc) rc = local_impl(c, one3, DHIP_RHS_NONE, nullptr);
+    if (!rc) rc = number_impl(c, -1.0, nullptr);
+    if (!rc) rc = build_impl(c, one3, nullptr);
+    if (rc) return rc;
+    dhip_ch *h = new dhip_ch();
+    h->ctx = c;
+    h->p = *p;
+    h->mesh = c->mesh;
+    h->ncut = c->ncut;
+    h->ndof = c->ndof;
+    h->nnz = c->nnz;
+    const int64_t N = h->ncut, R = h->ndof, Z = h->nnz;
+    const double eps = p->epsilon, dt = p->dt, rho = p->rho;
+    h->k.sdt = p->sigma * dt;
+    h->k.rdt = rho * dt;
+    h->k.dtr = dt * rho;
+    h->k.dt = dt;
+    h->k.S = p->S;
+    h->k.e2 = eps * eps;
+    h->k.re2 = rho * eps * eps;
+    h->k.e2r = eps * eps * rho;
+    h->k.meps = -eps;
+    auto fail = [&](int code) {
+        ch_release(h);
+        delete h;
+        return code;
+    };
+    if ((rc = ch_copy(c, h->phi, c->phi, sizeof(double) * c->phi_count)) ||
+        (rc = ch_copy(c, h->cut, c->cut.p, sizeof(int64_t) * N)) ||
+        (rc = ch_copy(c, h->cellinfo, c->cellinfo.p, sizeof(uint32_t) * c->mesh.ncell)) ||
+        (rc = ch_copy(c, h->flags, c->flags.p, N)) || (rc = ch_copy(c, h->dofs, c->dofs.p, sizeof(int32_t) * 4 * N)) ||
+        (rc = ch_copy(c, h->rowowner, c->rowowner.p, sizeof(int32_t) * R)) ||
+        (rc = ch_copy(c, h->massloc, c->loc.p, sizeof(double) * 16 * N)) ||
+        (rc = ch_copy(c, h->rowptr, c->rowptr.p, sizeof(int32_t) * (R + 1))) ||
+        (rc = ch_copy(c, h->col, c->col.p, sizeof(int32_t) * Z)) ||
+        (rc = ch_copy(c, h->mass, c->val[0].p, sizeof(double) * Z)) ||
+        (rc = ch_copy(c, h->lb, c->val[1].p, sizeof(double) * Z)) ||
+        (rc = ch_copy(c, h->stab, c->val[2].p, sizeof(double) * Z)))
+        return fail(rc);
+    if (h->tet.ensure(sizeof(double) * CHT_N * (N + 1)) || h->brp.ensure(sizeof(int32_t) * (2 * R + 1)) ||
+        h->bcol.ensure(sizeof(int32_t) * (4 * Z + 1)) || h->bval.ensure(sizeof(double) * (4 * Z + 1)) ||
+        h->pcdiag.ensure(sizeof(double) * (2 * R + 1)) || h->rhs.ensure(sizeof(double) * (2 * R + 1)) ||
+        h->x.ensure(sizeof(double) * (2 * R + 1)))
+        return fail(set_error(c, DHIP_E_NOMEM, "Cahn-Hilliard buffers"));
+    if (hipMemsetAsync(h->x.p, 0, sizeof(double) * 2 * R, c->stream) != hipSuccess)
+        return fail(set_error(c, DHIP_E_HIP, "state clear failed"));
+    ChBlockArgs ba{R, Z, h->rowptr.as<int32_t>(), h->col.as<int32_t>(), h->mass.as<double>(), h->lb.as<double>(),
+                   h->stab.as<double>(), h->brp.as<int32_t>(), h->bcol.as<int32_t>(), h->bval.as<double>(),
+                   h->pcdiag.as<double>(), h->k};
+    {
+        ProfScope ps(c, "ch_block");
+        hipLaunchKernelGGL(k_ch_block, dim3(blocks_for(R + 1, 128)), dim3(128), 0, c->stream, ba);
+        if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_block launch failed"));
+    }
+    h->K.ctx = c;
+    h->K.n = 2 * R;
+    h->K.nnz = 4 * Z;
+    h->K.rowptr = h->brp.as<int32_t>();
+    h->K.col = h->bcol.as<int32_t>();
+    h->K.val = h->bval.as<double>();
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "sync failed"));
+    *out = h;
+    if (ndof) *ndof = R;
+    return flush_profile(c);
+}
+
+void dhip_ch_destroy(dhip_ch_t *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->ctx->device);
+    (void)hipStreamSynchronize(h->ctx->stream);
+    ch_release(h);
+    delete h;
+}
+
+int dhip_ch_set_state(dhip_ch_t *h, const double *c_in, const double *mu_in, int32_t where) {
+    if (!h || !c_in || !mu_in) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const hipMemcpyKind kd = where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    const int64_t R = h->ndof;
+    if (R > 0) {
+        DHIP_HIPCHK(c, hipMemcpyAsync(h->x.as<double>(), mu_in, sizeof(double) * R, kd, c->stream));
+        DHIP_HIPCHK(c, hipMemcpyAsync(h->x.as<double>() + R, c_in, sizeof(double) * R, kd, c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    h->prepared = false;
+    return DHIP_OK;
+}
+
+int dhip_ch_get_state(dhip_ch_t *h, double *c_out, double *mu_out, int32_t where) {
+    if (!h) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const hipMemcpyKind kd = where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    const int64_t R = h->ndof;
+    if (R > 0 && mu_out) DHIP_HIPCHK(c, hipMemcpyAsync(mu_out, h->x.as<double>(), sizeof(double) * R, kd, c->stream));
+    if (R > 0 && c_out) DHIP_HIPCHK(c, hipMemcpyAsync(c_out, h->x.as<double>() + R, sizeof(double) * R, kd, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_ch_prepare(dhip_ch_t *h) {
+    if (!h) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = ch_prepare_impl(h);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_ch_system(dhip_ch_t *h, const int32_t **rowptr, const int32_t **col, const double **val, const double **rhs,
+                   const double **pcdiag, double **state, int64_t *n2, int64_t *nnz2) {
+    if (!h) return DHIP_E_ARG;
+    if (rowptr) *rowptr = h->brp.as<int32_t>();
+    if (col) *col = h->bcol.as<int32_t>();
+    if (val) *val = h->bval.as<double>();
+    if (rhs) *rhs = h->rhs.as<double>();
+    if (pcdiag) *pcdiag = h->pcdiag.as<double>();
+    if (state) *state = h->x.as<double>();
+    if (n2) *n2 = 2 * h->ndof;
+    if (nnz2) *nnz2 = 4 * h->nnz;
+    return DHIP_OK;
+}
+
+int dhip_ch_solve(dhip_ch_t *h, const double *x0, int32_t where, int32_t *iters, double *resid) {
+    if (!h) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_solve");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    if (x0 && h->ndof > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(h->x.p, x0, sizeof(double) * 2 * h->ndof,
+                                      where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                      c->stream));
+    int rc = h->ndof > 0 ? ch_solve_impl(h, iters, resid) : DHIP_OK;
+    h->prepared = false;
+    const int rp = flush_profile(c);
+    return rc ? rc : rp;
+}
+
+int dhip_ch_step(dhip_ch_t *h, int32_t nsteps, int32_t *iters, double *resid) {
+    if (!h || nsteps < 0) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    for (int32_t s = 0; s < nsteps; ++s) {
+        int rc = ch_prepare_impl(h);
+        int32_t it = 0;
+        double res = 0.0;
+        if (!rc && h->ndof > 0) rc = ch_solve_impl(h, &it, &res);
+        if (iters) iters[s] = it;
+        if (resid) resid[s] = res;
+        if (rc) {
+            flush_profile(c);
+            return rc;
+        }
+    }
+    return flush_profile(c);
 }
 
 }  // extern "C"

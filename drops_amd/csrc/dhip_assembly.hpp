@@ -193,6 +193,117 @@ __global__ void __launch_bounds__(CLS_BLOCK) k_compact(CompactArgs a) {
 // ---------------------------------------------------------------------------
 // per cut tet: patch + local matrices
 // ---------------------------------------------------------------------------
+// Vertex coordinates and the 10 P2 level-set values (lattice-2 order) of tet t.
+__device__ inline void load_tet_geom(const MeshP &m, const double *phi, int64_t t, double X[4][3], double ls[10]) {
+    const int k = (int)(t % 6);
+    int64_t i1, i2, i3;
+    cell_of(m, t / 6, i1, i2, i3);
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        vertex_coord(m, i1 + c_tab.kuhn[k][q][0], i2 + c_tab.kuhn[k][q][1], i3 + c_tab.kuhn[k][q][2], X[q]);
+    const int64_t h1 = 2 * (int64_t)m.n[0] + 1, h2 = 2 * (int64_t)m.n[1] + 1;
+    const double *base = phi + ((2 * i3) * h2 + 2 * i2) * h1 + 2 * i1;
+#pragma unroll
+    for (int l = 0; l < 10; ++l) {
+        const int nd = c_tab.tet_node27[k][l];
+        ls[l] = base[((nd / 9) * h2 + (nd / 3) % 3) * h1 + nd % 3];
+    }
+}
+
+// Barycentric vertices of triangle `tr` of sub-tet `lt` with sign pattern `pid`: zero
+// vertices or edge cuts (1-a) v_a + a v_b, a = ls_a / (ls_a - ls_b) (ConvexComb,
+// misc/container.h:403-410), merged per lattice edge.
+__device__ inline void facet_vertices(const double ls[10], int lt, int pid, int tr, double vb[3][4]) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int rep = c_tab.rp_tri[pid][tr][j];
+        if (rep < 4) {
+            const int lv = c_tab.lat_t[lt][rep];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) vb[j][cc] = c_tab.lat_b[lv][cc];
+        } else {
+            const int E = c_tab.sub_edge[lt][rep - 4];
+            const int la = c_tab.edge_ab[E][0], lb = c_tab.edge_ab[E][1];
+            const double alpha = ls[la] / (ls[la] - ls[lb]);
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc)
+                vb[j][cc] = (1.0 - alpha) * c_tab.lat_b[la][cc] + alpha * c_tab.lat_b[lb][cc];
+        }
+    }
+}
+
+// |det R| and unit normal Q e3 of the Householder QR of [w1-w0, w2-w0]
+// (QRDecompCL<3,2>, misc/container.h:1270-1296); rank deficient -> 0.
+__device__ inline void facet_qr(const double X[4][3], const double vb[3][4], double &absdet, double nrm[3]) {
+    double w[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) b2w(X, vb[j], w[j]);
+    double A[3][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        A[r][0] = w[1][r] - w[0][r];
+        A[r][1] = w[2][r] - w[0][r];
+    }
+    absdet = 0.;
+    nrm[0] = nrm[1] = nrm[2] = 0.;
+    double d0, d1, b0, b1;
+    double sig = 0.;
+    for (int i = 0; i < 3; ++i) sig += A[i][0] * A[i][0];
+    bool def = sig == 0.;
+    if (def) return;
+    d0 = (A[0][0] < 0 ? 1. : -1.) * sqrt(sig);
+    b0 = 1. / (d0 * A[0][0] - sig);
+    A[0][0] -= d0;
+    double sp = 0.;
+    for (int i = 0; i < 3; ++i) sp += A[i][0] * A[i][1];
+    sp *= b0;
+    for (int i = 0; i < 3; ++i) A[i][1] += A[i][0] * sp;
+    sig = 0.;
+    for (int i = 1; i < 3; ++i) sig += A[i][1] * A[i][1];
+    def = sig == 0.;
+    if (def) return;
+    d1 = (A[1][1] < 0 ? 1. : -1.) * sqrt(sig);
+    b1 = 1. / (d1 * A[1][1] - sig);
+    A[1][1] -= d1;
+    double dd = 1.0;
+    dd *= d0;
+    dd *= d1;
+    absdet = fabs(dd);
+    double tq[3] = {0., 0., 1.};
+    sp = 0.;
+    for (int i = 1; i < 3; ++i) sp += A[i][1] * tq[i];
+    sp *= b1;
+    for (int i = 1; i < 3; ++i) tq[i] += A[i][1] * sp;
+    sp = 0.;
+    for (int i = 0; i < 3; ++i) sp += A[i][0] * tq[i];
+    sp *= b0;
+    for (int i = 0; i < 3; ++i) tq[i] += A[i][0] * sp;
+    nrm[0] = tq[0];
+    nrm[1] = tq[1];
+    nrm[2] = tq[2];
+}
+
+// Normalised gradient of the P2 level set (P2 dof order v[10]) at barycentric b:
+// grad phi_h = sum_k G_k grad(lambda_k), G_k = v_k (4 b_k - 1) + 4 sum_{e=(k,l)} v_e b_l.
+__device__ inline void p2_unit_normal(const double v[10], const double b[4], const double G1[4][3], double n3[3]) {
+    double G[4];
+    const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) G[kk] = v[kk] * (4.0 * b[kk] - 1.0);
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        G[ea[e]] += 4.0 * (v[4 + e] * b[eb[e]]);
+        G[eb[e]] += 4.0 * (v[4 + e] * b[ea[e]]);
+    }
+    double g[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) g[d] = ((G[0] * G1[0][d] + G[1] * G1[1][d]) + G[2] * G1[2][d]) + G[3] * G1[3][d];
+    const double nn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    n3[0] = g[0] / nn;
+    n3[1] = g[1] / nn;
+    n3[2] = g[2] / nn;
+}
+
 constexpr int LOC_N = 57;  // 16 mass, 16 lb, 16 stab, 4 rhs, 4 hat_sq, 1 area
 constexpr int LOC_MASS = 0, LOC_LB = 16, LOC_STAB = 32, LOC_RHS = 48, LOC_HAT = 52, LOC_AREA = 56;
 
@@ -214,25 +325,8 @@ constexpr int LOCAL_BLOCK = 64;
 __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
     const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
     if (c >= a.ncut) return;
-    const int64_t t = a.cut[c];
-    const int k = (int)(t % 6);
-    int64_t i1, i2, i3;
-    cell_of(a.m, t / 6, i1, i2, i3);
-    double X[4][3];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        vertex_coord(a.m, i1 + c_tab.kuhn[k][q][0], i2 + c_tab.kuhn[k][q][1], i3 + c_tab.kuhn[k][q][2],
-                     X[q]);
-    double ls[10];
-    {
-        const int64_t h1 = 2 * (int64_t)a.m.n[0] + 1, h2 = 2 * (int64_t)a.m.n[1] + 1;
-        const double *base = a.phi + ((2 * i3) * h2 + 2 * i2) * h1 + 2 * i1;
-#pragma unroll
-        for (int l = 0; l < 10; ++l) {
-            const int nd = c_tab.tet_node27[k][l];
-            ls[l] = base[((nd / 9) * h2 + (nd / 3) % 3) * h1 + nd % 3];
-        }
-    }
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
     double G1[4][3];
     const double det = p1_grad(X, G1);
 
@@ -255,73 +349,11 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
         const int ntri = c_tab.rp_size[pid];
         const bool bnd = c_tab.rp_bnd[pid] != 0;
         for (int tr = 0; tr < ntri; ++tr) {
-            // facet vertices (barycentric): zero vertex or merged edge cut
             double vb[3][4];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int rep = c_tab.rp_tri[pid][tr][j];
-                if (rep < 4) {
-                    const int lv = c_tab.lat_t[lt][rep];
-#pragma unroll
-                    for (int cc = 0; cc < 4; ++cc) vb[j][cc] = c_tab.lat_b[lv][cc];
-                } else {
-                    const int E = c_tab.sub_edge[lt][rep - 4];
-                    const int la = c_tab.edge_ab[E][0], lb = c_tab.edge_ab[E][1];
-                    const double alpha = ls[la] / (ls[la] - ls[lb]);
-#pragma unroll
-                    for (int cc = 0; cc < 4; ++cc)
-                        vb[j][cc] = (1.0 - alpha) * c_tab.lat_b[la][cc] + alpha * c_tab.lat_b[lb][cc];
-                }
-            }
+            facet_vertices(ls, lt, pid, tr, vb);
             ++nfacet;
-            // QR of [w1-w0, w2-w0] (QRDecompCL<3,2>, misc/container.h:1270-1296)
-            double w[3][3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) b2w(X, vb[j], w[j]);
-            double A[3][2];
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                A[r][0] = w[1][r] - w[0][r];
-                A[r][1] = w[2][r] - w[0][r];
-            }
-            double absdet = 0., nrm[3] = {0., 0., 0.};
-            {
-                double d0, d1, b0, b1;
-                double sig = 0.;
-                for (int i = 0; i < 3; ++i) sig += A[i][0] * A[i][0];
-                bool def = sig == 0.;
-                if (!def) {
-                    d0 = (A[0][0] < 0 ? 1. : -1.) * sqrt(sig);
-                    b0 = 1. / (d0 * A[0][0] - sig);
-                    A[0][0] -= d0;
-                    double sp = 0.;
-                    for (int i = 0; i < 3; ++i) sp += A[i][0] * A[i][1];
-                    sp *= b0;
-                    for (int i = 0; i < 3; ++i) A[i][1] += A[i][0] * sp;
-                    sig = 0.;
-                    for (int i = 1; i < 3; ++i) sig += A[i][1] * A[i][1];
-                    def = sig == 0.;
-                    if (!def) {
-                        d1 = (A[1][1] < 0 ? 1. : -1.) * sqrt(sig);
-                        b1 = 1. / (d1 * A[1][1] - sig);
-                        A[1][1] -= d1;
-                        double dd = 1.0;
-                        dd *= d0;
-                        dd *= d1;
-                        absdet = fabs(dd);
-                        double tq[3] = {0., 0., 1.};
-                        sp = 0.;
-                        for (int i = 1; i < 3; ++i) sp += A[i][1] * tq[i];
-                        sp *= b1;
-                        for (int i = 1; i < 3; ++i) tq[i] += A[i][1] * sp;
-                        sp = 0.;
-                        for (int i = 0; i < 3; ++i) sp += A[i][0] * tq[i];
-                        sp *= b0;
-                        for (int i = 0; i < 3; ++i) tq[i] += A[i][0] * sp;
-                        nrm[0] = tq[0]; nrm[1] = tq[1]; nrm[2] = tq[2];
-                    }
-                }
-            }
+            double absdet, nrm[3];
+            facet_qr(X, vb, absdet, nrm);
             double af = bnd ? 0.5 : 1.;
             af *= absdet;
             // Quad2 composite: mass + numbering integrals
@@ -432,23 +464,9 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
 #pragma unroll
         for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
         double R[10], S[10];
-        const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
         for (int qn = 0; qn < 15; ++qn) {
-            const double *b = c_tab.v5n[qn];
-            double G[4];
-#pragma unroll
-            for (int kk = 0; kk < 4; ++kk) G[kk] = v[kk] * (4.0 * b[kk] - 1.0);
-#pragma unroll
-            for (int e = 0; e < 6; ++e) {
-                G[ea[e]] += 4.0 * (v[4 + e] * b[eb[e]]);
-                G[eb[e]] += 4.0 * (v[4 + e] * b[ea[e]]);
-            }
-            double g[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d)
-                g[d] = ((G[0] * G1[0][d] + G[1] * G1[1][d]) + G[2] * G1[2][d]) + G[3] * G1[3][d];
-            const double nn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
-            double n3[3] = {g[0] / nn, g[1] / nn, g[2] / nn};
+            double n3[3];
+            p2_unit_normal(v, c_tab.v5n[qn], G1, n3);
             double U[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) U[i] = dot3(n3, G1[i]);

@@ -1,0 +1,267 @@
+// dhip_ch.hpp -- one implicit-Euler step of the P1 trace-FEM Cahn-Hilliard system on gfx950.
+//
+// CahnHilliardcGP1CL (surfactant/ifacetransp.cpp:1526-1869) with theta = 1 on a resting
+// interface (no wind: the convection and mass-divergence matrices vanish) and no sources.
+// Unknowns [mu; c] on the P1 interface space of the context's assembly; per step
+//   A_ = sigma dt LM(c_old) + rho dt C         B_ = M
+//   C_ = -eps M                               D_ = eps^2 L + S M + rho eps^2 C
+//   rhs3 = M c_old                            rhs4 = -M f'(c_old) + S M c_old
+// LM = LocalLaplaceMobilityP1CL (surfactant/ifacetransp.h:1731-1766), the local products
+// M c and M f'(c) are LocalMatVecP1CL accumulations (:800-828).  The block matrix has the
+// layout of the SparseMatBaseCL(A, B, C, D) constructor (num/spmat.h:719-751).
+//
+//   k_ch_tet    one lane per cut tet: patch, Quad5 composite rule, mobility interpolant and
+//               projected gradients at every quadrature point (reference order), the 4x4
+//               local mobility matrix and the local products M c, M f'(c).  SoA output.
+//   k_ch_rows   one lane per DOF row: sums the local rows of the incident cut tets in
+//               ascending tet id (the order of the reference's builder +=), writes the A_
+//               block, the Precond3 diagonal and both right-hand sides.
+//   k_ch_block  once per system: block CSR pattern, the constant B_, C_, D_ blocks and the
+//               Precond4 diagonal.
+#pragma once
+
+#include "dhip_assembly.hpp"
+
+namespace dhip {
+
+// Mobility_function, surfactant/ifacetransp.cpp:3334-3339
+__device__ __forceinline__ double mobility_fun(double x) {
+    const double v = (1. - x) * x;
+    return v > 0. ? v : 0.;
+}
+
+// Potential_prime_function, surfactant/ifacetransp.cpp:3360-3364
+__device__ __forceinline__ double potential_prime(double x) { return (x - 1.) * x * (x - 0.5); }
+
+constexpr int CHT_N = 24;  // 16 mobility, 4 M c, 4 M f'(c)
+constexpr int CHT_MOB = 0, CHT_TD = 16, CHT_WL = 20;
+
+struct ChTetArgs {
+    MeshP m;
+    const double *phi;
+    const int64_t *cut;
+    int64_t ncut;
+    const uint8_t *flags;
+    const int32_t *dofs;     // [ncut][4], -1 where the vertex has no DOF
+    const double *loc;       // local mass (alpha = 1) from k_local_p1
+    const double *conc;      // c on the DOFs
+    double *out;             // SoA [CHT_N][ncut]
+};
+
+__global__ void __launch_bounds__(LOCAL_BLOCK) k_ch_tet(ChTetArgs a) {
+    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+    if (c >= a.ncut) return;
+    const int64_t N = a.ncut;
+    double *O = a.out;
+    if (!(a.flags[c] & 1)) {
+        for (int e = 0; e < CHT_N; ++e) O[e * N + c] = 0.;
+        return;
+    }
+    double xc[4], mv[4];
+    int32_t nb[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        nb[i] = a.dofs[c * 4 + i];
+        xc[i] = nb[i] >= 0 ? a.conc[nb[i]] : 0.;
+        mv[i] = mobility_fun(xc[i]);
+    }
+    // LocalMatVecP1CL<LocalInterfaceMassP1CL> for c and f'(c)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        double vt = 0., vw = 0.;
+        if (nb[i] >= 0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (nb[j] < 0) continue;
+                const double mij = a.loc[(int64_t)(LOC_MASS + i * 4 + j) * N + c];
+                vt += mij * xc[j];
+                vw += mij * potential_prime(xc[j]);
+            }
+        }
+        O[(CHT_TD + i) * N + c] = vt;
+        O[(CHT_WL + i) * N + c] = vw;
+    }
+    // LocalLaplaceMobilityP1CL on the Quad5 composite rule
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+    double G1[4][3];
+    p1_grad(X, G1);
+    double v[10];
+#pragma unroll
+    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+    double acc[10];
+#pragma unroll
+    for (int o = 0; o < 10; ++o) acc[o] = 0.;
+    for (int lt = 0; lt < 8; ++lt) {
+        int s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = sgn(ls[c_tab.lat_t[lt][q]]);
+        const int pid = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;
+        const int ntri = c_tab.rp_size[pid];
+        const bool bnd = c_tab.rp_bnd[pid] != 0;
+        for (int tr = 0; tr < ntri; ++tr) {
+            double vb[3][4];
+            facet_vertices(ls, lt, pid, tr, vb);
+            double absdet, nrm[3];
+            facet_qr(X, vb, absdet, nrm);
+            double af = bnd ? 0.5 : 1.;
+            af *= absdet;
+            for (int qn = 0; qn < 7; ++qn) {
+                double b[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    double sv = vb[0][cc] * c_tab.q5n[qn][0];
+                    sv += vb[1][cc] * c_tab.q5n[qn][1];
+                    sv += vb[2][cc] * c_tab.q5n[qn][2];
+                    b[cc] = sv;
+                }
+                const double wq = af * c_tab.q5w[qn];
+                double n3[3];
+                p2_unit_normal(v, b, G1, n3);
+                double q[4][3];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const double d = dot3(G1[i], n3);
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) q[i][cc] = G1[i][cc] - d * n3[cc];
+                }
+                const double qm = b[0] * mv[0] + b[1] * mv[1] + b[2] * mv[2] + b[3] * mv[3];
+                int o = 0;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j <= i; ++j) acc[o++] += wq * (qm * dot3(q[i], q[j]));
+            }
+        }
+    }
+    int o = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j, ++o) {
+            O[(CHT_MOB + i * 4 + j) * N + c] = acc[o];
+            O[(CHT_MOB + j * 4 + i) * N + c] = acc[o];
+        }
+}
+
+struct ChCoef {
+    double sdt;        // sigma * dt
+    double rdt;        // rho * dt
+    double dtr;        // dt * rho
+    double dt;
+    double S;
+    double e2;         // eps * eps
+    double re2;        // rho * eps * eps
+    double e2r;        // eps * eps * rho
+    double meps;       // -eps
+};
+
+struct ChRowArgs {
+    MeshP m;
+    const int64_t *cut;
+    const uint32_t *cellinfo;
+    const uint8_t *flags;
+    const int32_t *dofs;
+    const int32_t *rowowner;
+    int64_t ndof, ncut;
+    const double *tet;      // k_ch_tet output
+    const int32_t *rowptr;  // P1 pattern
+    const double *mass, *stab;
+    double *bval;           // block values
+    double *pcdiag;         // [2 ndof]
+    double *rhs;            // [2 ndof]
+    ChCoef k;
+};
+
+__global__ void k_ch_rows(ChRowArgs a) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.ndof) return;
+    int64_t x1, x2, x3;
+    slot_vertex(a.m, a.cut, a.rowowner[r], x1, x2, x3);
+    uint32_t present = 0;
+    int32_t cols[15];
+    double lm[15];
+#pragma unroll
+    for (int s = 0; s < 15; ++s) {
+        cols[s] = 0;
+        lm[s] = 0.0;
+    }
+    double td = 0.0, wl = 0.0;
+    const int64_t N = a.ncut;
+    for_incident_cut_tets(a.m, a.cellinfo, x1, x2, x3, [&](int64_t ci, int k, int li) {
+        if (!(a.flags[ci] & 1)) return false;
+        for (int j = 0; j < 4; ++j) {
+            const int32_t dj = a.dofs[ci * 4 + j];
+            if (dj < 0) continue;
+            const int sl = c_tab.slot[k][li][j];
+            const double v = a.tet[(int64_t)(CHT_MOB + li * 4 + j) * N + ci];
+#pragma unroll
+            for (int s = 0; s < 15; ++s)
+                if (s == sl) {
+                    cols[s] = dj;
+                    lm[s] += v;
+                }
+            present |= 1u << sl;
+        }
+        td += a.tet[(int64_t)(CHT_TD + li) * N + ci];
+        wl += a.tet[(int64_t)(CHT_WL + li) * N + ci];
+        return false;
+    });
+    const int64_t e0 = a.rowptr[r];
+#pragma unroll
+    for (int s = 0; s < 15; ++s) {
+        if (!((present >> s) & 1u)) continue;
+        int rank = 0;
+#pragma unroll
+        for (int u = 0; u < 15; ++u)
+            rank += (((present >> u) & 1u) && cols[u] < cols[s]) ? 1 : 0;
+        const int64_t e = e0 + rank;
+        const double st = a.stab[e];
+        // A_.LinComb(sigma dt, LaplaceM, rho dt, Volume_stab)
+        a.bval[2 * e0 + rank] = a.k.sdt * lm[s] + a.k.rdt * st;
+        // Precond3.LinComb(1, Mass, sigma dt, LaplaceM, dt rho, Volume_stab)
+        if (cols[s] == r) a.pcdiag[r] = 1.0 * (1.0 * a.mass[e] + a.k.sdt * lm[s]) + a.k.dtr * st;
+    }
+    // rhs3 = theta (timeder + dt load3), rhs4 = -well + S timeder + load4 (no sources)
+    a.rhs[r] = 1.0 * (td + a.k.dt * 0.0);
+    a.rhs[a.ndof + r] = ((-1.0) * wl + a.k.S * td) + 0.0;
+}
+
+struct ChBlockArgs {
+    int64_t ndof, nnz;
+    const int32_t *rowptr, *col;
+    const double *mass, *lb, *stab;
+    int32_t *brp, *bcol;
+    double *bval, *pcdiag;
+    ChCoef k;
+};
+
+__global__ void k_ch_block(ChBlockArgs a) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > a.ndof) return;
+    const int64_t n = a.ndof, nnz = a.nnz;
+    if (r == n) {
+        a.brp[2 * n] = (int32_t)(4 * nnz);
+        return;
+    }
+    const int64_t e0 = a.rowptr[r], len = a.rowptr[r + 1] - e0;
+    const int64_t top = 2 * e0, bot = 2 * nnz + 2 * e0;
+    a.brp[r] = (int32_t)top;
+    a.brp[n + r] = (int32_t)bot;
+    for (int64_t k = 0; k < len; ++k) {
+        const int64_t e = e0 + k;
+        const int32_t cj = a.col[e];
+        const double m = a.mass[e], l = a.lb[e], s = a.stab[e];
+        a.bcol[top + k] = cj;                               // A_ (values per step)
+        a.bcol[top + len + k] = (int32_t)(cj + n);          // B_.LinComb(1, M, 0 dt, Massd, dt, Conv)
+        a.bval[top + len + k] = 1.0 * (1.0 * m + 0.0 * 0.0) + a.k.dt * 0.0;
+        a.bcol[bot + k] = cj;                               // C_.LinComb(0, L, -eps, M)
+        a.bval[bot + k] = 0.0 * l + a.k.meps * m;
+        a.bcol[bot + len + k] = (int32_t)(cj + n);          // D_.LinComb(eps^2, L, S, M, rho eps^2, C)
+        a.bval[bot + len + k] = 1.0 * (a.k.e2 * l + a.k.S * m) + a.k.re2 * s;
+        // Precond4.LinComb(1, Mass, eps^2, Laplace, eps^2 rho, Volume_stab)
+        if (cj == r) a.pcdiag[n + r] = 1.0 * (1.0 * m + a.k.e2 * l) + a.k.e2r * s;
+    }
+}
+
+}  // namespace dhip

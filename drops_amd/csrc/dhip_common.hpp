@@ -127,8 +127,8 @@ struct dhip_ctx {
     dhip::DevBuf kv[16];
     // optional per-block wall-clock probe of the persistent solver
     bool probe_on = false;
-    int64_t pcg_rows_per_block = 0;
-    bool coop_refused = false;        // cooperative launch unavailable: same grid, regular launch   // 0: one row per thread of a 512-thread block
+    int64_t pcg_rows_per_block = 0;   // 0: one row per thread of a 512-thread block
+    bool coop_refused = false;        // cooperative launch unavailable: same grid, regular launch
     int probe_blocks = 0;
     dhip::DevBuf probe;
 };

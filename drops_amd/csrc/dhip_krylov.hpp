@@ -265,17 +265,19 @@ template <class ACC, class VT, class F>
 __device__ __forceinline__ ACC row_product(const int32_t *rp, const int32_t *col, const VT *val, int32_t off,
                                            int64_t l, F &&xv) {
     const int32_t b = rp[l] - off, e = rp[l + 1] - off;
-    const int len = e - b;
     ACC s = ACC(0);
-    if (len <= MAXR) {
+    // chunks of MAXR entries: all gathers of a chunk are issued before the first use,
+    // the products are summed in column order (the order of a serial row loop)
+    for (int32_t q0 = b; q0 < e; q0 += MAXR) {
+        const int len = e - q0 < MAXR ? e - q0 : MAXR;
         int32_t cc[MAXR];
         VT vv[MAXR];
         ACC xx[MAXR];
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
             if (j < len) {
-                cc[j] = col[b + j];
-                vv[j] = val[b + j];
+                cc[j] = col[q0 + j];
+                vv[j] = val[q0 + j];
             }
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
@@ -283,8 +285,6 @@ __device__ __forceinline__ ACC row_product(const int32_t *rp, const int32_t *col
 #pragma unroll
         for (int j = 0; j < MAXR; ++j)
             if (j < len) s += (ACC)vv[j] * xx[j];
-    } else {
-        for (int32_t q = b; q < e; ++q) s += (ACC)val[q] * xv(col[q]);
     }
     return s;
 }

@@ -29,8 +29,16 @@ EXPORTS = [
     "dhip_build_p1", "dhip_assemble_p1", "dhip_get_cut_tets", "dhip_get_local", "dhip_get_tet_dofs",
     "dhip_get_dof_vertices", "dhip_get_csr", "dhip_device_arrays", "dhip_csr_create",
     "dhip_csr_from_assembly", "dhip_csr_destroy", "dhip_csr_info", "dhip_spmv", "dhip_dot",
-    "dhip_axpy", "dhip_pcg", "dhip_gmres",
+    "dhip_axpy", "dhip_pcg", "dhip_gmres", "dhip_gmres_diag", "dhip_ch_create", "dhip_ch_destroy",
+    "dhip_ch_set_state", "dhip_ch_get_state", "dhip_ch_prepare", "dhip_ch_system", "dhip_ch_solve", "dhip_ch_step",
 ]
+
+
+class ChParams(C.Structure):
+    """dhip_ch_params_t"""
+    _fields_ = [("dt", C.c_double), ("epsilon", C.c_double), ("rho", C.c_double), ("sigma", C.c_double),
+                ("S", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
+                ("rel", C.c_int32)]
 
 
 class DhipError(RuntimeError):
@@ -87,6 +95,17 @@ def load(path=LIB_PATH):
     L.dhip_axpy.argtypes = [P, C.c_int64, C.c_double, VP, VP, C.c_int32]
     L.dhip_pcg.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.dhip_gmres.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    L.dhip_gmres_diag.argtypes = [P, P, VP, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    L.dhip_ch_create.argtypes = [P, C.c_int32, C.POINTER(ChParams), C.POINTER(P), I64]
+    L.dhip_ch_destroy.argtypes = [P]
+    L.dhip_ch_destroy.restype = None
+    L.dhip_ch_set_state.argtypes = [P, VP, VP, C.c_int32]
+    L.dhip_ch_get_state.argtypes = [P, VP, VP, C.c_int32]
+    L.dhip_ch_prepare.argtypes = [P]
+    L.dhip_ch_system.argtypes = [P, C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(VP), C.POINTER(VP),
+                                 C.POINTER(VP), I64, I64]
+    L.dhip_ch_solve.argtypes = [P, VP, C.c_int32, I32, D]
+    L.dhip_ch_step.argtypes = [P, C.c_int32, I32, D]
     _lib = L
     return L
 
@@ -120,6 +139,87 @@ class Csr:
     def close(self):
         if self.h and self.ctx.h:
             self.ctx.L.dhip_csr_destroy(self.h)
+        self.h = None
+        self.ctx._csrs.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CahnHilliard:
+    """Device Cahn-Hilliard stepper (CahnHilliardcGP1CL mirror, theta = 1, resting interface).
+    Unknowns [mu; c] on the P1 interface space of the context's mesh and level set."""
+
+    def __init__(self, ctx, dt, epsilon, rho, sigma=1.0, S=1.0, restart=50, max_iter=20000, tol=1e-8, rel=True,
+                 zero_policy=ZERO_THROW):
+        self.ctx, self.L = ctx, ctx.L
+        self.params = ChParams(dt, epsilon, rho, sigma, S, restart, max_iter, tol, int(rel))
+        self.h = C.c_void_p()
+        n = C.c_int64()
+        ctx.check(self.L.dhip_ch_create(ctx.h, zero_policy, C.byref(self.params), C.byref(self.h), C.byref(n)))
+        self.n = n.value
+        ctx._csrs.add(self)
+
+    def set_state(self, c, mu):
+        c = np.ascontiguousarray(c, np.float64)
+        mu = np.ascontiguousarray(mu, np.float64)
+        assert c.size == self.n and mu.size == self.n
+        self.ctx.check(self.L.dhip_ch_set_state(self.h, _vp(c), _vp(mu), HOST))
+
+    def state(self):
+        c, mu = np.zeros(self.n), np.zeros(self.n)
+        self.ctx.check(self.L.dhip_ch_get_state(self.h, _vp(c), _vp(mu), HOST))
+        return c, mu
+
+    def prepare(self):
+        self.ctx.check(self.L.dhip_ch_prepare(self.h))
+
+    def system(self):
+        """Device pointers of the step system and the state vector."""
+        ptr = [C.c_void_p() for _ in range(6)]
+        n2, z2 = C.c_int64(), C.c_int64()
+        self.ctx.check(self.L.dhip_ch_system(self.h, *[C.byref(p) for p in ptr], C.byref(n2), C.byref(z2)))
+        keys = ("rowptr", "col", "val", "rhs", "pcdiag", "state")
+        out = {k: p.value for k, p in zip(keys, ptr)}
+        out.update(n2=n2.value, nnz2=z2.value)
+        return out
+
+    def system_host(self):
+        """Host copies of the step system (block CSR, rhs, preconditioner diagonal, state)."""
+        s = self.system()
+        ctx = self.ctx
+        return dict(rowptr=ctx.to_host(s["rowptr"], s["n2"] + 1, np.int32), col=ctx.to_host(s["col"], s["nnz2"], np.int32),
+                    val=ctx.to_host(s["val"], s["nnz2"], np.float64), rhs=ctx.to_host(s["rhs"], s["n2"], np.float64),
+                    pcdiag=ctx.to_host(s["pcdiag"], s["n2"], np.float64),
+                    state=ctx.to_host(s["state"], s["n2"], np.float64))
+
+    def solve(self, x0=None, where=HOST, raise_on_fail=True):
+        """x0: [mu; c] start (host array, or device pointer with where=DEVICE); None = previous step."""
+        it, t = C.c_int32(), C.c_double()
+        if x0 is None:
+            rc = self.L.dhip_ch_solve(self.h, None, HOST, C.byref(it), C.byref(t))
+        elif where == DEVICE:
+            rc = self.L.dhip_ch_solve(self.h, C.c_void_p(x0), DEVICE, C.byref(it), C.byref(t))
+        else:
+            x0 = np.ascontiguousarray(x0, np.float64)
+            assert x0.size == 2 * self.n
+            rc = self.L.dhip_ch_solve(self.h, _vp(x0), HOST, C.byref(it), C.byref(t))
+        if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
+            self.ctx.check(rc)
+        return it.value, t.value, rc == OK
+
+    def step(self, nsteps=1):
+        its = np.zeros(nsteps, np.int32)
+        res = np.zeros(nsteps)
+        self.ctx.check(self.L.dhip_ch_step(self.h, nsteps, _p(its, C.c_int32), _p(res, C.c_double)))
+        return its, res
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.L.dhip_ch_destroy(self.h)
         self.h = None
         self.ctx._csrs.discard(self)
 
@@ -346,6 +446,23 @@ class Context:
         if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
             self.check(rc)
         return x, it.value, t.value, rc == OK
+
+    def gmres_diag(self, A, pcdiag, b, x0=None, restart=50, max_iter=10000, tol=1e-10, rel=True, left_pc=False,
+                   raise_on_fail=True):
+        """GMRES preconditioned by z = r / pcdiag (block Jacobi with a given diagonal)."""
+        b = np.ascontiguousarray(b, np.float64)
+        d = np.ascontiguousarray(pcdiag, np.float64)
+        x = np.zeros(b.size) if x0 is None else np.array(x0, np.float64)
+        it = C.c_int32(max_iter)
+        t = C.c_double(tol)
+        rc = self.L.dhip_gmres_diag(self.h, A.h, _vp(d), _vp(b), _vp(x), HOST, restart, C.byref(it), C.byref(t),
+                                    int(rel), int(left_pc))
+        if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
+            self.check(rc)
+        return x, it.value, t.value, rc == OK
+
+    def cahn_hilliard(self, dt, epsilon, rho, **kw):
+        return CahnHilliard(self, dt, epsilon, rho, **kw)
 
     def gmres(self, A, b, x0=None, pc=PC_JACOBI, restart=50, max_iter=10000, tol=1e-10, rel=True, left_pc=False,
               raise_on_fail=True):

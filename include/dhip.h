@@ -187,6 +187,58 @@ int dhip_gmres(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b
                int32_t where, int32_t restart, int32_t *max_iter, double *tol, int32_t rel,
                int32_t left_pc);
 
+/* GMRES as dhip_gmres, preconditioned by z_i = r_i / pcdiag_i with a caller-given diagonal
+ * (JACPcCL applied blockwise: DiagBlockPreCL, num/oseenprecond.h:1051-1073, with a Jacobi
+ * sweep of each block's preconditioning matrix in place of its inner solve). */
+int dhip_gmres_diag(dhip_ctx_t *ctx, const dhip_csr_t *A, const double *pcdiag, const double *b, double *x,
+                    int32_t where, int32_t restart, int32_t *max_iter, double *tol, int32_t rel,
+                    int32_t left_pc);
+
+/* ---- Cahn-Hilliard time stepping on the P1 interface space ----
+ * CahnHilliardcGP1CL (surfactant/ifacetransp.h:2947-2990, surfactant/ifacetransp.cpp:1526-1869)
+ * with theta = 1 (implicit Euler) on a resting interface: no wind (the convection and
+ * mass-divergence matrices vanish) and no source terms.  Unknowns [mu; c].  Per step
+ *   [ sigma dt LM(c_old) + rho dt C    M                    ] [mu]   [ M c_old                    ]
+ *   [ -eps M                           eps^2 L + S M + rho eps^2 C ] [c ] = [ -M f'(c_old) + S M c_old ]
+ * with LM the mobility matrix (LocalLaplaceMobilityP1CL, mobility (1-c)c clipped at 0),
+ * f'(x) = (x-1) x (x-1/2) (Potential_prime_function, :3360-3364), solved by right-preconditioned
+ * GMRES(restart) with the block Jacobi preconditioner built from the diagonals of the reference's
+ * Precond3 = M + sigma dt LM + dt rho C and Precond4 = M + eps^2 L + eps^2 rho C (:1567-1570).
+ * The normal of the mobility and stabilisation terms is the normalised P2 level-set gradient.
+ * The block matrix has the layout of SparseMatBaseCL(A, B, C, D) (num/spmat.h:719-751). */
+typedef struct dhip_ch dhip_ch_t;
+typedef struct {
+    double dt;        /* time step */
+    double epsilon;   /* interface width eps */
+    double rho;       /* volume stabilisation factor (Scaling * h^Power) */
+    double sigma;     /* mobility scaling ("Mobility") */
+    double S;         /* time-derivative stabilisation ("S = beta_s") */
+    int32_t restart;  /* GMRES restart m (m - 1 Arnoldi steps per cycle) */
+    int32_t max_iter;
+    double tol;
+    int32_t rel;      /* residual relative to |b| */
+} dhip_ch_params_t;
+/* Assembles the P1 interface space of the context's mesh + level set (as dhip_assemble_p1 with
+ * coefficients 1, no load vector) and keeps its own copy of everything a step needs.  The state
+ * starts at zero. */
+int dhip_ch_create(dhip_ctx_t *ctx, int32_t zero_policy, const dhip_ch_params_t *params, dhip_ch_t **ch,
+                   int64_t *ndof);
+void dhip_ch_destroy(dhip_ch_t *ch);
+int dhip_ch_set_state(dhip_ch_t *ch, const double *c, const double *mu, int32_t where);
+int dhip_ch_get_state(dhip_ch_t *ch, double *c, double *mu, int32_t where);
+/* InitStep + Update: the mobility matrix, block A_, Precond3 diagonal and right-hand sides from
+ * the current concentration. */
+int dhip_ch_prepare(dhip_ch_t *ch);
+/* Device pointers of the step system: block CSR (2n rows, 4 nnz), rhs, preconditioner diagonal,
+ * and the state vector [mu; c] (valid until dhip_ch_destroy). */
+int dhip_ch_system(dhip_ch_t *ch, const int32_t **rowptr, const int32_t **col, const double **val,
+                   const double **rhs, const double **pcdiag, double **state, int64_t *n2, int64_t *nnz2);
+/* DoStep's solve (BlockMatrixSolverCL::Solve, num/oseensolver.h:223-239): start from x0 = [mu; c]
+ * (NULL: the current state, i.e. the previous step) and commit the solution as the new state. */
+int dhip_ch_solve(dhip_ch_t *ch, const double *x0, int32_t where, int32_t *iters, double *resid);
+/* nsteps x (prepare + solve from the previous step); iters/resid (may be NULL) get one entry per step. */
+int dhip_ch_step(dhip_ch_t *ch, int32_t nsteps, int32_t *iters, double *resid);
+
 #ifdef __cplusplus
 }
 #endif

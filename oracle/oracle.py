@@ -69,6 +69,9 @@ def lib(kind="parity"):
     L.orc_pcg_jacobi.argtypes = [C.c_int64, I64, I32, D, D, D, I32, D, C.c_int32]
     L.orc_gmres_jacobi.argtypes = [C.c_int64, I64, I32, D, D, D, C.c_int32, I32, D, C.c_int32,
                                    C.c_int32, C.c_int32]
+    L.orc_gmres_diag.argtypes = [C.c_int64, I64, I32, D, D, D, D, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    L.orc_tet_mobility.argtypes = [M, D, C.c_int64, D, C.c_int, D]
+    L.orc_ch_assemble.argtypes = [M, D, I32, C.c_int64, I64, I32, D, C.c_int, D, D, D]
     L.orc_lattice_tables.argtypes = [I32, I32]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
     _libs[name] = L
@@ -216,3 +219,86 @@ def gmres_jacobi(rowptr, col, val, b, x0=None, restart=50, max_iter=10000, tol=1
                                   _ptr(np.ascontiguousarray(b, np.float64), C.c_double), _ptr(x, C.c_double),
                                   restart, C.byref(it), C.byref(t), int(rel), int(left_pc), int(use_pc))
     return x, it.value, t.value, code == OK
+
+
+def gmres_diag(rowptr, col, val, pcdiag, b, x0=None, restart=50, max_iter=10000, tol=1e-10, rel=True,
+               left_pc=False):
+    """Serial GMRES with z = r / pcdiag (block-Jacobi with the given diagonal)."""
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    it = C.c_int32(max_iter)
+    t = C.c_double(tol)
+    d = np.ascontiguousarray(pcdiag, np.float64)
+    code = lib().orc_gmres_diag(n, _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32), _ptr(val, C.c_double),
+                                _ptr(d, C.c_double), _ptr(np.ascontiguousarray(b, np.float64), C.c_double),
+                                _ptr(x, C.c_double), restart, C.byref(it), C.byref(t), int(rel), int(left_pc))
+    return x, it.value, t.value, code == OK
+
+
+def tet_mobility(mesh, phi, tet, cv, zero_policy=ZERO_THROW):
+    out = np.zeros(16)
+    cv = np.ascontiguousarray(cv, np.float64)
+    _check(lib().orc_tet_mobility(C.byref(mesh), _ptr(phi, C.c_double), int(tet), _ptr(cv, C.c_double),
+                                  zero_policy, _ptr(out, C.c_double)), "tet_mobility")
+    return out.reshape(4, 4)
+
+
+def ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, zero_policy=ZERO_THROW, kind="parity"):
+    """Mobility matrix values on the pattern, M c and M f'(c) (LocalMatVecP1CL accumulations)."""
+    lm = np.zeros(col.size)
+    td, wl = np.zeros(ndof), np.zeros(ndof)
+    c = np.ascontiguousarray(c, np.float64)
+    _check(lib(kind).orc_ch_assemble(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32),
+                                     ndof, _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
+                                     _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), _ptr(c, C.c_double),
+                                     zero_policy, _ptr(lm, C.c_double), _ptr(td, C.c_double),
+                                     _ptr(wl, C.c_double)), "ch_assemble")
+    return lm, td, wl
+
+
+def block2x2(rowptr, col, A, B, Cb, D):
+    """SparseMatBaseCL(A, B, C, D) block constructor (num/spmat.h:719-751) for four blocks that
+    share one n x n pattern: row i = [A row | B row + n], row n+i = [C row | D row + n]."""
+    n = rowptr.size - 1
+    rp = np.asarray(rowptr, np.int64)
+    ln = np.diff(rp)
+    brp = np.zeros(2 * n + 1, np.int64)
+    brp[1:] = np.cumsum(np.concatenate([2 * ln, 2 * ln]))
+    bcol = np.empty(4 * col.size, np.int32)
+    bval = np.empty(4 * col.size)
+    for i in range(n):
+        s, e = rp[i], rp[i + 1]
+        for I, (L, R) in ((i, (A, B)), (n + i, (Cb, D))):
+            o = brp[I]
+            k = e - s
+            bcol[o:o + k] = col[s:e]
+            bcol[o + k:o + 2 * k] = col[s:e] + n
+            bval[o:o + k] = L[s:e]
+            bval[o + k:o + 2 * k] = R[s:e]
+    return brp, bcol, bval
+
+
+def ch_step(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, mu, dt, eps, rho,
+            sigma=1.0, S=1.0, restart=50, tol=1e-8, max_iter=20000, kind="parity"):
+    """One step of CahnHilliardcGP1CL::DoStep (surfactant/ifacetransp.cpp:1823-1856) with theta = 1,
+    resting interface (no wind: Conv = Massd = 0) and no sources, solved by right-preconditioned
+    GMRES(restart) with block Jacobi: the diagonals of Precond3 = M + sigma dt LM + dt rho C and
+    Precond4 = M + eps^2 L + eps^2 rho C (:1567-1570).  Unknowns [mu; c], start = previous step."""
+    lm, timeder, well = ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, kind=kind)
+    A = sigma * dt * lm + rho * dt * stab
+    B = 1.0 * (1.0 * mass + (0.0 * dt) * np.zeros_like(mass)) + dt * np.zeros_like(mass)
+    Cb = 0.0 * lb + (-eps) * mass
+    D = 1.0 * (eps * eps * lb + S * mass) + (rho * eps * eps) * stab
+    P3 = 1.0 * (1.0 * mass + (sigma * dt) * lm) + (dt * rho) * stab
+    P4 = 1.0 * (1.0 * mass + (eps * eps) * lb) + (eps * eps * rho) * stab
+    rhs3 = 1.0 * (timeder + dt * np.zeros(ndof))
+    rhs4 = ((-1.0) * well + S * timeder) + np.zeros(ndof)
+    rp = np.asarray(rowptr, np.int64)
+    diag = lambda v: np.array([v[rp[i] + np.flatnonzero(col[rp[i]:rp[i + 1]] == i)[0]] for i in range(ndof)])
+    pcd = np.concatenate([diag(P3), diag(P4)])
+    brp, bcol, bval = block2x2(rp, col, A, B, Cb, D)
+    b = np.concatenate([rhs3, rhs4])
+    x0 = np.concatenate([mu, c])
+    x, it, res, ok = gmres_diag(brp, bcol, bval, pcd, b, x0, restart=restart, max_iter=max_iter, tol=tol)
+    return dict(mu=x[:ndof], c=x[ndof:], iters=it, res=res, ok=ok, lm=lm, timeder=timeder, well=well,
+                rhs=b, pcdiag=pcd, rowptr=brp, col=bcol, val=bval, x0=x0)

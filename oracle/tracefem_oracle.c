@@ -951,6 +951,145 @@ int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, do
 }
 
 /* ------------------------------------------------------------------------ */
+/* Cahn-Hilliard step pieces (CahnHilliardcGP1CL, resting interface)          */
+/* ------------------------------------------------------------------------ */
+
+/* Mobility_function, surfactant/ifacetransp.cpp:3334-3339 */
+static double mobility_fun(double x)
+{
+    const double v = (1. - x) * x;
+    return v > 0. ? v : 0.;
+}
+
+/* Potential_prime_function, surfactant/ifacetransp.cpp:3360-3364 */
+static double potential_prime(double x) { return (x - 1.) * x * (x - 0.5); }
+
+/* LocalLaplaceMobilityP1CL::setup, surfactant/ifacetransp.h:1731-1766: Quad5 composite rule,
+ * mobility = P1 interpolant of Mobility_function at the vertices (LocalP1CL, evaluated as
+ * FE_P1CL::val, num/fe.h:150-153), projected P1 gradients with the normal at each quadrature
+ * point; the normal is the normalised gradient of the P2 level set. */
+static void local_mobility(const tet_t *T, const patch_t *p, const double cv[4], double out[16])
+{
+    double node[MAXF * 7][4], w[MAXF * 7];
+    const int nq = composite_nodes(p, 7, Q5_N, Q5_W, node, w);
+    double G1[4][3];
+    p1_gradients(T->X, G1);
+    double v[10];
+    for (int d = 0; d < 10; ++d) v[d] = T->ls[LAT_OF_P2[d]];
+    double mv[4];
+    for (int i = 0; i < 4; ++i) mv[i] = mobility_fun(cv[i]);
+    double qq[4][MAXF * 7][3], qm[MAXF * 7];
+    for (int k = 0; k < nq; ++k) {
+        double nrm[3];
+        p2_normal(v, node[k], G1, nrm);
+        for (int j = 0; j < 4; ++j) {
+            const double d = dot3(G1[j], nrm);
+            for (int c = 0; c < 3; ++c) qq[j][k][c] = G1[j][c] - d * nrm[c];
+        }
+        const double *b = node[k];
+        qm[k] = b[0] * mv[0] + b[1] * mv[1] + b[2] * mv[2] + b[3] * mv[3];
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            double s = 0.;
+            for (int k = 0; k < nq; ++k) s += w[k] * (qm[k] * dot3(qq[i][k], qq[j][k]));
+            out[i * 4 + j] = s;
+        }
+}
+
+int orc_tet_mobility(const orc_mesh_t *m, const double *phi, int64_t tet, const double cv[4], int zero_policy,
+                     double out[16])
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !cv || !out) return ORC_E_ARG;
+    memset(out, 0, 16 * sizeof(double));
+    tet_t T;
+    load_tet(m, phi, tet, &T);
+    if (equal_signs(T.ls, 10)) return ORC_OK;
+    if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+    patch_t p;
+    int err = make_patch(T.ls, &p);
+    if (err) return err;
+    if (p.nf == 0) return ORC_OK;
+    patch_geometry(T.X, &p);
+    local_mobility(&T, &p, cv, out);
+    return ORC_OK;
+}
+
+/* CahnHilliardcGP1CL::Update (LaplaceM, surfactant/ifacetransp.cpp:1560-1561) and InitStep with
+ * theta = 1 (:1610-1641): the mobility matrix from concentration c on the given pattern, and the
+ * vectors timeder = LocalMatVecP1CL<LocalInterfaceMassP1CL>(c) and
+ * well = LocalMatVecP1CL<LocalInterfaceMassP1CL>(f'(c)) (surfactant/ifacetransp.h:800-828), each
+ * accumulated in mesh order. */
+int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                    const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                    double *lm, double *timeder, double *well)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !rowptr || !colind || !c || !lm || !timeder || !well)
+        return ORC_E_ARG;
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    typedef struct { int err, nonempty; int32_t numb[4]; double mass[16], mob[16]; } chrec_t;
+    chrec_t *rec = (chrec_t *)malloc(sizeof(chrec_t) * (ncut > 0 ? ncut : 1));
+    if (!cut || !rec) { free(cut); free(rec); return ORC_E_NOMEM; }
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
+    const double one3[3] = {1., 1., 1.};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t q = 0; q < ncut; ++q) {
+        tet_t T;
+        load_tet(m, phi, cut[q], &T);
+        patch_t p;
+        rec[q].nonempty = 0;
+        rec[q].err = make_patch(T.ls, &p);
+        if (rec[q].err || p.nf == 0) continue;
+        rec[q].nonempty = 1;
+        patch_geometry(T.X, &p);
+        double cv[4];
+        for (int i = 0; i < 4; ++i) {
+            rec[q].numb[i] = dof_of_vertex[T.vid[i]];
+            cv[i] = rec[q].numb[i] >= 0 ? c[rec[q].numb[i]] : 0.;
+        }
+        orc_local_t L;
+        local_quantities(&T, &p, one3, ORC_RHS_NONE, NULL, &L);
+        memcpy(rec[q].mass, L.mass, sizeof L.mass);
+        local_mobility(&T, &p, cv, rec[q].mob);
+    }
+    for (int64_t e = 0; e < rowptr[ndof]; ++e) lm[e] = 0.;
+    for (int64_t i = 0; i < ndof; ++i) timeder[i] = well[i] = 0.;
+    for (int64_t q = 0; q < ncut && !err; ++q) {
+        if (rec[q].err) { err = rec[q].err; break; }
+        if (!rec[q].nonempty) continue;
+        const int32_t *nb = rec[q].numb;
+        double xc[4], xf[4];
+        for (int j = 0; j < 4; ++j) {
+            xc[j] = nb[j] >= 0 ? c[nb[j]] : 0.;
+            xf[j] = potential_prime(xc[j]);
+        }
+        for (int i = 0; i < 4; ++i) {
+            if (nb[i] < 0) continue;
+            double vt = 0., vw = 0.;
+            for (int j = 0; j < 4; ++j) {
+                if (nb[j] < 0) continue;
+                vt += rec[q].mass[i * 4 + j] * xc[j];
+                vw += rec[q].mass[i * 4 + j] * xf[j];
+                int64_t e = rowptr[nb[i]];
+                while (e < rowptr[nb[i] + 1] && colind[e] != nb[j]) ++e;
+                if (e == rowptr[nb[i] + 1]) { err = ORC_E_ARG; break; }
+                lm[e] += rec[q].mob[i * 4 + j];
+            }
+            timeder[nb[i]] += vt;
+            well[nb[i]] += vw;
+        }
+    }
+    free(cut);
+    free(rec);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Sparse linear algebra                                                     */
 /* ------------------------------------------------------------------------ */
 
@@ -1077,9 +1216,36 @@ static void gmres_update(int64_t n, double *x, int k, const double *H, int m, co
 }
 
 /* serial GMRES, num/krylovsolver.h:320-463 (restart m performs m-1 Arnoldi steps per cycle) */
+/* z_i = 1.0 * b_i / d_i: JACPcCL applied with a given diagonal (the diagonal of each block's
+ * preconditioning matrix for the block-Jacobi form of DiagBlockPreCL, num/oseenprecond.h:1051-1073) */
+static void diag_apply(int64_t n, const double *d, const double *b, double *x)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) x[i] = 1.0 * b[i] / d[i];
+}
+
+static int gmres_core(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                      const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,
+                      double *tol, int32_t rel, int32_t left_pc, int32_t use_pc);
+
 int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
                      const double *b, double *x, int32_t restart, int32_t *max_iter, double *tol,
                      int32_t rel, int32_t left_pc, int32_t use_pc)
+{
+    return gmres_core(n, rowptr, colind, val, NULL, b, x, restart, max_iter, tol, rel, left_pc, use_pc);
+}
+
+int orc_gmres_diag(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                   const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,
+                   double *tol, int32_t rel, int32_t left_pc)
+{
+    if (!pcdiag) return ORC_E_ARG;
+    return gmres_core(n, rowptr, colind, val, pcdiag, b, x, restart, max_iter, tol, rel, left_pc, 1);
+}
+
+static int gmres_core(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                      const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,
+                      double *tol, int32_t rel, int32_t left_pc, int32_t use_pc)
 {
     int m = restart <= *max_iter ? restart : *max_iter;
     if (m < 2) m = 2;
@@ -1093,7 +1259,8 @@ int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, co
     double beta, normb, resid;
     int ret = ORC_E_NOT_CONVERGED;
 #define PC_APPLY(dst, src) \
-    do { if (use_pc) jacobi_apply(n, rowptr, colind, val, (src), (dst)); \
+    do { if (use_pc && pcdiag) diag_apply(n, pcdiag, (src), (dst)); \
+         else if (use_pc) jacobi_apply(n, rowptr, colind, val, (src), (dst)); \
          else memcpy((dst), (src), n * sizeof(double)); } while (0)
     if (!left_pc) {
         orc_spmv(n, rowptr, colind, val, x, t);

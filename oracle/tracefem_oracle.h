@@ -127,6 +127,20 @@ int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, co
                      const double *b, double *x, int32_t restart, int32_t *max_iter, double *tol,
                      int32_t rel, int32_t left_pc, int32_t use_pc);
 
+/* GMRES as above, preconditioned by z = b / d with a given diagonal d (block Jacobi) */
+int orc_gmres_diag(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
+                   const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,
+                   double *tol, int32_t rel, int32_t left_pc);
+
+/* ---- Cahn-Hilliard step pieces (CahnHilliardcGP1CL, surfactant/ifacetransp.cpp:1526-1689) ---- */
+/* LocalLaplaceMobilityP1CL on one tet, cv = concentration at its 4 vertices */
+int orc_tet_mobility(const orc_mesh_t *m, const double *phi, int64_t tet, const double cv[4], int zero_policy,
+                     double out[16]);
+/* mobility matrix on the given pattern, M c and M f'(c) as LocalMatVecP1CL accumulations */
+int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                    const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                    double *lm, double *timeder, double *well);
+
 #ifdef __cplusplus
 }
 #endif
