@@ -5,6 +5,7 @@
 #include "dhip_assembly.hpp"
 #include "dhip_krylov.hpp"
 #include "dhip_ch.hpp"
+#include "dhip_mlp.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -1105,8 +1106,10 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     GmresArgs ga{n, A->rowptr, A->col, A->val, diag, db, dx, c->kv[15].as<double>(),
                  c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
                  outs, outd, outs + 1};
-    const void *ks[5] = {(const void *)k_gmres<1>, (const void *)k_gmres<2>, (const void *)k_gmres<4>,
-                         (const void *)k_gmres<8>, (const void *)k_gmres<16>};
+    const bool regv = rpt == 1 && m <= KREG;
+    const void *ks[5] = {regv ? (const void *)k_gmres<1, KREG> : (const void *)k_gmres<1, 0>,
+                         (const void *)k_gmres<2, 0>, (const void *)k_gmres<4, 0>, (const void *)k_gmres<8, 0>,
+                         (const void *)k_gmres<16, 0>};
     rc = launch_persistent(c, "gmres", ks, rpt, nb, ga);
     if (rc) return rc;
     int32_t hs[4];
@@ -1205,6 +1208,10 @@ struct dhip_ch {
     DevBuf phi, cut, cellinfo, flags, dofs, rowowner, massloc, rowptr, col, mass, lb, stab;
     // step system and state [mu; c]
     DevBuf tet, brp, bcol, bval, pcdiag, rhs, x;
+    // previous time levels for the learned start: hist[k] = state k+1 levels back (k < nhist)
+    DevBuf hist[3];
+    int nhist = 0;
+    DevBuf feat, pred;
     dhip_csr K;
     bool prepared = false;
 };
@@ -1218,7 +1225,7 @@ static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
 static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
                       &h->col, &h->mass, &h->lb, &h->stab, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
-                      &h->rhs, &h->x};
+                      &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -1234,6 +1241,20 @@ static int ch_prepare_impl(dhip_ch *h) {
                  h->pcdiag.as<double>(), h->rhs.as<double>(), h->k};
     if (R > 0) DHIP_LAUNCH(c, "ch_rows", k_ch_rows, blocks_for(R, 128), 128, 0, ra);
     h->prepared = true;
+    return DHIP_OK;
+}
+
+// The current state becomes the newest history level (the ring keeps three).
+static int ch_push_history(dhip_ch *h) {
+    dhip_ctx *c = h->ctx;
+    const size_t bytes = sizeof(double) * 2 * h->ndof;
+    DevBuf last = h->hist[2];
+    h->hist[2] = h->hist[1];
+    h->hist[1] = h->hist[0];
+    h->hist[0] = last;
+    DHIP_HIPCHK(c, h->hist[0].ensure(bytes + 8));
+    DHIP_HIPCHK(c, hipMemcpyAsync(h->hist[0].p, h->x.p, bytes, hipMemcpyDeviceToDevice, c->stream));
+    h->nhist = std::min(h->nhist + 1, 3);
     return DHIP_OK;
 }
 
@@ -1347,6 +1368,7 @@ int dhip_ch_set_state(dhip_ch_t *h, const double *c_in, const double *mu_in, int
     }
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     h->prepared = false;
+    h->nhist = 0;
     return DHIP_OK;
 }
 
@@ -1391,11 +1413,13 @@ int dhip_ch_solve(dhip_ch_t *h, const double *x0, int32_t where, int32_t *iters,
     dhip_ctx *c = h->ctx;
     if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_solve");
     DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = h->ndof > 0 ? ch_push_history(h) : DHIP_OK;
+    if (rc) return rc;
     if (x0 && h->ndof > 0)
         DHIP_HIPCHK(c, hipMemcpyAsync(h->x.p, x0, sizeof(double) * 2 * h->ndof,
                                       where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                       c->stream));
-    int rc = h->ndof > 0 ? ch_solve_impl(h, iters, resid) : DHIP_OK;
+    rc = h->ndof > 0 ? ch_solve_impl(h, iters, resid) : DHIP_OK;
     h->prepared = false;
     const int rp = flush_profile(c);
     return rc ? rc : rp;
@@ -1409,6 +1433,7 @@ int dhip_ch_step(dhip_ch_t *h, int32_t nsteps, int32_t *iters, double *resid) {
         int rc = ch_prepare_impl(h);
         int32_t it = 0;
         double res = 0.0;
+        if (!rc && h->ndof > 0) rc = ch_push_history(h);
         if (!rc && h->ndof > 0) rc = ch_solve_impl(h, &it, &res);
         if (iters) iters[s] = it;
         if (resid) resid[s] = res;
@@ -1418,6 +1443,231 @@ int dhip_ch_step(dhip_ch_t *h, int32_t nsteps, int32_t *iters, double *resid) {
         }
     }
     return flush_profile(c);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned warm start
+// ---------------------------------------------------------------------------
+struct dhip_mlp {
+    dhip_ctx *ctx = nullptr;
+    MlpNorm norm{};
+    int adam_step = 0;
+    DevBuf theta, m, v, packed, grad, part;
+    bool packed_ok = false;
+};
+
+static int mlp_pack(dhip_mlp *q) {
+    if (q->packed_ok) return DHIP_OK;
+    dhip_ctx *c = q->ctx;
+    DHIP_LAUNCH(c, "mlp_pack", k_mlp_pack, blocks_for(MLP_NPACK, 256), 256, 0, q->theta.as<float>(),
+                q->packed.as<uint16_t>());
+    q->packed_ok = true;
+    return DHIP_OK;
+}
+
+static int mlp_predict_impl(dhip_mlp *q, const float *X, int64_t S, float *Y) {
+    dhip_ctx *c = q->ctx;
+    int rc = mlp_pack(q);
+    if (rc) return rc;
+    if (S > 0)
+        DHIP_LAUNCH(c, "mlp_infer", k_mlp_infer, blocks_for(S, 16 * INFER_WAVES), 64 * INFER_WAVES, 0, X, S,
+                    q->packed.as<uint16_t>(), q->theta.as<float>(), Y);
+    return DHIP_OK;
+}
+
+static int mlp_features_impl(dhip_mlp *q, const double *const hist[4], int32_t nhist, const double *rhs,
+                             const double *pcdiag, int64_t n, float *X) {
+    if (nhist < 1 || nhist > 4) return set_error(q->ctx, DHIP_E_ARG, "1 to 4 history levels");
+    const double *h[4];
+    for (int k = 0; k < 4; ++k) h[k] = hist[k < nhist ? k : nhist - 1];
+    FeatArgs fa{h[0], h[1], h[2], h[3], rhs, pcdiag, n, q->norm, X};
+    if (n > 0) DHIP_LAUNCH(q->ctx, "mlp_features", k_mlp_features, blocks_for(n, 256), 256, 0, fa);
+    return DHIP_OK;
+}
+
+// gradient (double, q->grad) and loss of a minibatch; returns the tile count
+static int mlp_grad_impl(dhip_mlp *q, const float *X, const float *T, const int32_t *idx, int64_t B) {
+    dhip_ctx *c = q->ctx;
+    const int64_t tiles = (B + TS - 1) / TS;
+    DHIP_HIPCHK(c, q->part.ensure(sizeof(float) * (size_t)(tiles + 1) * (MLP_NPARAM + 1)));
+    TrainArgs ta{X, T, idx, B, q->theta.as<float>(), q->part.as<float>()};
+    DHIP_LAUNCH(c, "mlp_train", k_mlp_train, (unsigned)tiles, TRAIN_T, 0, ta);
+    DHIP_LAUNCH(c, "mlp_reduce", k_mlp_reduce, blocks_for(MLP_NPARAM + 1, 256), 256, 0, q->part.as<float>(),
+                (int)tiles, q->grad.as<double>());
+    return DHIP_OK;
+}
+
+extern "C" {
+
+int dhip_mlp_create(dhip_ctx_t *c, dhip_mlp_t **out) {
+    if (!c || !out) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    dhip_mlp *q = new dhip_mlp();
+    q->ctx = c;
+    for (int k = 0; k < MLP_NF; ++k) q->norm.in_scale[k] = 1.0;
+    for (int k = 0; k < MLP_NO; ++k) q->norm.out_scale[k] = 1.0;
+    if (q->theta.ensure(sizeof(float) * MLP_NPARAM) || q->m.ensure(sizeof(float) * MLP_NPARAM) ||
+        q->v.ensure(sizeof(float) * MLP_NPARAM) || q->packed.ensure(sizeof(uint16_t) * MLP_NPACK) ||
+        q->grad.ensure(sizeof(double) * (MLP_NPARAM + 1)) ||
+        hipMemsetAsync(q->theta.p, 0, sizeof(float) * MLP_NPARAM, c->stream) ||
+        hipMemsetAsync(q->m.p, 0, sizeof(float) * MLP_NPARAM, c->stream) ||
+        hipMemsetAsync(q->v.p, 0, sizeof(float) * MLP_NPARAM, c->stream) || hipStreamSynchronize(c->stream)) {
+        DevBuf *b[] = {&q->theta, &q->m, &q->v, &q->packed, &q->grad};
+        for (DevBuf *x : b) x->release();
+        delete q;
+        return set_error(c, DHIP_E_NOMEM, "network buffers");
+    }
+    *out = q;
+    return DHIP_OK;
+}
+
+void dhip_mlp_destroy(dhip_mlp_t *q) {
+    if (!q) return;
+    (void)hipSetDevice(q->ctx->device);
+    (void)hipStreamSynchronize(q->ctx->stream);
+    DevBuf *b[] = {&q->theta, &q->m, &q->v, &q->packed, &q->grad, &q->part};
+    for (DevBuf *x : b) x->release();
+    delete q;
+}
+
+int dhip_mlp_params(dhip_mlp_t *q, const float *theta_in, float *theta_out) {
+    if (!q) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    if (theta_in) {
+        DHIP_HIPCHK(c, hipMemcpyAsync(q->theta.p, theta_in, sizeof(float) * MLP_NPARAM, hipMemcpyHostToDevice,
+                                      c->stream));
+        DHIP_HIPCHK(c, hipMemsetAsync(q->m.p, 0, sizeof(float) * MLP_NPARAM, c->stream));
+        DHIP_HIPCHK(c, hipMemsetAsync(q->v.p, 0, sizeof(float) * MLP_NPARAM, c->stream));
+        q->adam_step = 0;
+        q->packed_ok = false;
+    }
+    if (theta_out)
+        DHIP_HIPCHK(c, hipMemcpyAsync(theta_out, q->theta.p, sizeof(float) * MLP_NPARAM, hipMemcpyDeviceToHost,
+                                      c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_mlp_normalization(dhip_mlp_t *q, const double in_mean[12], const double in_scale[12],
+                           const double out_mean[2], const double out_scale[2]) {
+    if (!q || !in_mean || !in_scale || !out_mean || !out_scale) return DHIP_E_ARG;
+    for (int k = 0; k < MLP_NF; ++k) {
+        q->norm.in_mean[k] = in_mean[k];
+        q->norm.in_scale[k] = in_scale[k];
+    }
+    for (int k = 0; k < MLP_NO; ++k) {
+        if (!(out_scale[k] != 0.0)) return set_error(q->ctx, DHIP_E_ARG, "output scale must be nonzero");
+        q->norm.out_mean[k] = out_mean[k];
+        q->norm.out_scale[k] = out_scale[k];
+    }
+    return DHIP_OK;
+}
+
+int dhip_mlp_features(dhip_mlp_t *q, const double *const hist[4], int32_t nhist, const double *rhs,
+                      const double *pcdiag, int64_t n, float *X) {
+    if (!q || !hist || !rhs || !pcdiag || !X || n < 0) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = mlp_features_impl(q, hist, nhist, rhs, pcdiag, n, X);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_mlp_targets(dhip_mlp_t *q, const double *x_next, const double *x_cur, int64_t n, float *T) {
+    if (!q || !x_next || !x_cur || !T || n < 0) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    TargetArgs ta{x_next, x_cur, n, q->norm, T};
+    if (n > 0) DHIP_LAUNCH(c, "mlp_targets", k_mlp_targets, blocks_for(n, 256), 256, 0, ta);
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_mlp_predict(dhip_mlp_t *q, const float *X, int64_t S, float *Y) {
+    if (!q || !X || !Y || S < 0) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = mlp_predict_impl(q, X, S, Y);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_mlp_grad(dhip_mlp_t *q, const float *X, const float *T, const int32_t *idx, int64_t B, double *grad,
+                  double *loss) {
+    if (!q || !X || !T || !idx || B <= 0) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = mlp_grad_impl(q, X, T, idx, B);
+    if (rc) return rc;
+    if (grad)
+        DHIP_HIPCHK(c, hipMemcpyAsync(grad, q->grad.p, sizeof(double) * MLP_NPARAM, hipMemcpyDeviceToHost, c->stream));
+    double l = 0.0;
+    DHIP_HIPCHK(c, hipMemcpyAsync(&l, q->grad.as<double>() + MLP_NPARAM, sizeof l, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (loss) *loss = l / (double)B;
+    return flush_profile(c);
+}
+
+int dhip_mlp_train_step(dhip_mlp_t *q, const float *X, const float *T, const int32_t *idx, int64_t B, float lr,
+                        double *loss) {
+    if (!q || !X || !T || !idx || B <= 0 || !(lr > 0.f)) return DHIP_E_ARG;
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = mlp_grad_impl(q, X, T, idx, B);
+    if (rc) return rc;
+    AdamArgs aa{q->theta.as<float>(), q->m.as<float>(), q->v.as<float>(), q->grad.as<double>(), lr, 0.9f, 0.999f,
+                1e-8f, ++q->adam_step};
+    DHIP_LAUNCH(c, "mlp_adam", k_mlp_adam, blocks_for(MLP_NPARAM, 256), 256, 0, aa);
+    q->packed_ok = false;
+    if (loss) {
+        double l = 0.0;
+        DHIP_HIPCHK(c, hipMemcpyAsync(&l, q->grad.as<double>() + MLP_NPARAM, sizeof l, hipMemcpyDeviceToHost,
+                                      c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        *loss = l / (double)B;
+    }
+    return flush_profile(c);
+}
+
+int dhip_ch_history(dhip_ch_t *h, int32_t level, const double **state) {
+    if (!h || !state || level < 0 || level > 3) return DHIP_E_ARG;
+    // missing levels repeat the oldest one held
+    const int k = std::min<int>(level, h->nhist);
+    *state = k == 0 ? h->x.as<double>() : h->hist[k - 1].as<double>();
+    return DHIP_OK;
+}
+
+int dhip_ch_solve_learned(dhip_ch_t *h, dhip_mlp_t *q, int32_t *iters, double *resid) {
+    if (!h || !q) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    if (q->ctx != c) return set_error(c, DHIP_E_ARG, "network and stepper belong to different contexts");
+    if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_solve_learned");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = h->ndof;
+    int rc = DHIP_OK;
+    if (n > 0) {
+        DHIP_HIPCHK(c, h->feat.ensure(sizeof(float) * MLP_NF * (n + 1)));
+        DHIP_HIPCHK(c, h->pred.ensure(sizeof(float) * MLP_NO * (n + 1)));
+        const double *hist[4] = {h->x.as<double>(), h->hist[0].as<double>(), h->hist[1].as<double>(),
+                                 h->hist[2].as<double>()};
+        rc = mlp_features_impl(q, hist, 1 + h->nhist, h->rhs.as<double>(), h->pcdiag.as<double>(), n,
+                               h->feat.as<float>());
+        if (!rc) rc = mlp_predict_impl(q, h->feat.as<float>(), n, h->pred.as<float>());
+        if (!rc) rc = ch_push_history(h);
+        if (!rc) {
+            CombineArgs ca{h->hist[0].as<double>(), h->pred.as<float>(), n, q->norm, h->x.as<double>()};
+            DHIP_LAUNCH(c, "mlp_combine", k_mlp_combine, blocks_for(n, 256), 256, 0, ca);
+        }
+        if (!rc) rc = ch_solve_impl(h, iters, resid);
+    }
+    h->prepared = false;
+    const int rp = flush_profile(c);
+    return rc ? rc : rp;
 }
 
 }  // extern "C"

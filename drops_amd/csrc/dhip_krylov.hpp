@@ -261,29 +261,29 @@ __device__ inline Slice<VT> load_slice(unsigned char *lds, const int32_t *rp, co
 // instead of one per entry.
 constexpr int MAXR = 16;
 
-template <class ACC, class VT, class F>
+template <class ACC, int BATCH = MAXR, class VT, class F>
 __device__ __forceinline__ ACC row_product(const int32_t *rp, const int32_t *col, const VT *val, int32_t off,
                                            int64_t l, F &&xv) {
     const int32_t b = rp[l] - off, e = rp[l + 1] - off;
     ACC s = ACC(0);
-    // chunks of MAXR entries: all gathers of a chunk are issued before the first use,
+    // chunks of BATCH entries: all gathers of a chunk are issued before the first use,
     // the products are summed in column order (the order of a serial row loop)
-    for (int32_t q0 = b; q0 < e; q0 += MAXR) {
-        const int len = e - q0 < MAXR ? e - q0 : MAXR;
-        int32_t cc[MAXR];
-        VT vv[MAXR];
-        ACC xx[MAXR];
+    for (int32_t q0 = b; q0 < e; q0 += BATCH) {
+        const int len = e - q0 < BATCH ? e - q0 : BATCH;
+        int32_t cc[BATCH];
+        VT vv[BATCH];
+        ACC xx[BATCH];
 #pragma unroll
-        for (int j = 0; j < MAXR; ++j)
+        for (int j = 0; j < BATCH; ++j)
             if (j < len) {
                 cc[j] = col[q0 + j];
                 vv[j] = val[q0 + j];
             }
 #pragma unroll
-        for (int j = 0; j < MAXR; ++j)
+        for (int j = 0; j < BATCH; ++j)
             if (j < len) xx[j] = xv(cc[j]);
 #pragma unroll
-        for (int j = 0; j < MAXR; ++j)
+        for (int j = 0; j < BATCH; ++j)
             if (j < len) s += (ACC)vv[j] * xx[j];
     }
     return s;
@@ -625,7 +625,9 @@ constexpr int KMAX = 64;
 template <int BLOCK>
 __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t g, const double *mine, int K,
                                        double *out_sh) {
+    static_assert(BLOCK == 512, "the chunked reduction maps 8 waves to 8 chunks of blocks");
     __shared__ int ok_sh;
+    __shared__ double chunk_sh[BLOCK / 64][KMAX];
     const int nb = gridDim.x, par = g & 1;
     double *P = parts + (size_t)par * nb * KMAX;
     if (threadIdx.x < K) P[(size_t)blockIdx.x * KMAX + threadIdx.x] = mine[threadIdx.x];
@@ -660,13 +662,61 @@ __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t 
         }
     }
     __syncthreads();
-    if (threadIdx.x < K) {
-        double s = 0.0;
-        for (int b = 0; b < nb; ++b) s += P[(size_t)b * KMAX + threadIdx.x];
-        out_sh[threadIdx.x] = s;
+    if (K > 0) {
+        // wave w sums blocks w, w+8, w+16, ... of entry (lane); the gathers of a batch are all
+        // issued before the first add; then the 8 chunk sums are added in chunk order.  Every
+        // block performs the identical sequence, so all blocks hold bit-identical results.
+        const int kk = threadIdx.x & 63, ch = threadIdx.x >> 6;
+        constexpr int NCH = BLOCK / 64, BATCH = 32;
+        if (kk < K) {
+            double s = 0.0;
+            for (int q0 = 0; ch + NCH * q0 < nb; q0 += BATCH) {
+                double v[BATCH];
+#pragma unroll
+                for (int q = 0; q < BATCH; ++q) {
+                    const int b = ch + NCH * (q0 + q);
+                    v[q] = b < nb ? P[(size_t)b * KMAX + kk] : 0.0;
+                }
+#pragma unroll
+                for (int q = 0; q < BATCH; ++q)
+                    if (ch + NCH * (q0 + q) < nb) s += v[q];
+            }
+            chunk_sh[ch][kk] = s;
+        }
+        __syncthreads();
+        if (threadIdx.x < K) {
+            double s = 0.0;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) s += chunk_sh[c][threadIdx.x];
+            out_sh[threadIdx.x] = s;
+        }
+        __syncthreads();
     }
-    __syncthreads();
     return ok_sh != 0;
+}
+
+// Eight wave sums at once by a butterfly transpose (10 shuffles instead of 48): on return lane l
+// holds the wave sum of p[l >> 3], bit-identical in the 8 lanes of its group.
+__device__ inline double wave_sum8(const double p[8]) {
+    const int lane = threadIdx.x & 63;
+    const bool h = (lane >> 5) & 1, g = (lane >> 4) & 1, f = (lane >> 3) & 1;
+    double q[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const double keep = h ? p[4 + t] : p[t], send = h ? p[t] : p[4 + t];
+        q[t] = keep + __shfl_xor(send, 32, 64);
+    }
+    double r2[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const double keep = g ? q[2 + u] : q[u], send = g ? q[u] : q[2 + u];
+        r2[u] = keep + __shfl_xor(send, 16, 64);
+    }
+    double v = (f ? r2[1] : r2[0]) + __shfl_xor(f ? r2[0] : r2[1], 8, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 1, 64);
+    return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -698,7 +748,9 @@ struct GmresArgs {
     int *out_status;
 };
 
-template <int RPT>
+constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG, one row per thread)
+
+template <int RPT, int KR>
 __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ double Hc[KMAX + 1];          // current Hessenberg column
@@ -715,9 +767,11 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     // right preconditioning: the slice holds a_rc / d_c (A M v); left: plain values (M A v)
     const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, a.val, right ? a.diag : nullptr, r0, r1, scaled);
     const double *dg = a.diag;
+    // gathers in batches of 8 when the basis rows occupy registers
+    constexpr int GB = KR > 0 ? 8 : MAXR;
     auto spmv_row = [&](int64_t l, auto &&yv_) {
-        if (scaled || !right || !dg) return row_product<double>(S.rp, S.col, S.val, S.off, l, yv_);
-        return row_product<double>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
+        if (scaled || !right || !dg) return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, yv_);
+        return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
     };
     auto col_of = [](int k) { return k * (k + 1) / 2; };  // start of packed column k (rows 0..k)
     const int m = a.m;
@@ -737,6 +791,75 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         }
     }
     auto prec = [&](int k, double v) { return dg ? 1.0 * v / dr[k] : v; };
+    // One row per thread: the thread's entries of the Krylov basis also live in registers, so
+    // the dot products and basis combinations read no memory; the global basis holds the
+    // vectors the SpMV gathers from other blocks.
+    static_assert(KR == 0 || RPT == 1, "register basis needs one row per thread");
+    constexpr bool REGV = KR > 0;
+    // Register basis: vector q of the current cycle sits in slot KR - nvec + q; a new vector
+    // shifts the slots down by one and enters at the top, so every register index is static.
+    double vreg[REGV ? KR : 1];
+    int nvec = 0;
+    auto vstore = [&](int q, int64_t r, double v, bool mine) {
+        if (mine) a.V[(size_t)q * n + r] = v;
+        if constexpr (REGV) {
+#pragma unroll
+            for (int t = 0; t < KR - 1; ++t) vreg[t] = vreg[t + 1];
+            vreg[KR - 1] = v;
+        }
+    };
+    // block sums of w . V[q] (q < K) and, with `self`, w . w as entry K
+    __shared__ double wave_part[PB / 64][KMAX + 2];
+    auto dots = [&](int K, bool self, double *out) {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int KT = K + (self ? 1 : 0);
+        if constexpr (REGV) {
+            const int off = KR - nvec;  // slot of vector 0
+#pragma unroll
+            for (int c0 = 0; c0 < KR; c0 += 8) {
+                if (c0 + 8 > off) {
+                    double p[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int t = c0 + u, q = t - off;
+                        p[u] = (t < KR && q >= 0 && q < K && own[0]) ? wr[0] * vreg[t < KR ? t : KR - 1] : 0.0;
+                    }
+                    const double v = wave_sum8(p);
+                    const int q = c0 + (lane >> 3) - off;
+                    if ((lane & 7) == 0 && q >= 0 && q < K) wave_part[wv][q] = v;
+                }
+            }
+            if (self) {
+                const double v = wave_sum(own[0] ? wr[0] * wr[0] : 0.0);
+                if (lane == 0) wave_part[wv][K] = v;
+            }
+        } else {
+            for (int c0 = 0; c0 < KT; c0 += 8) {
+                double p[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int q = c0 + u;
+                    double sacc = 0.0;
+                    if (q < KT) {
+#pragma unroll
+                        for (int k = 0; k < RPT; ++k)
+                            if (own[k]) sacc += wr[k] * (q < K ? a.V[(size_t)q * n + r0 + lr[k]] : wr[k]);
+                    }
+                    p[u] = sacc;
+                }
+                const double v = wave_sum8(p);
+                if ((lane & 7) == 0 && c0 + (lane >> 3) < KT) wave_part[wv][c0 + (lane >> 3)] = v;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < KT) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int ww = 0; ww < PB / 64; ++ww) sacc += wave_part[ww][threadIdx.x];
+            out[threadIdx.x] = sacc;
+        }
+        __syncthreads();
+    };
     // residual r = b - A x (left: M r) into wr; returns block partials |r|^2 (and |b|^2 or |M b|^2)
     auto residual = [&](bool with_b, double &p_rr, double &p_bb) {
         p_rr = 0.0;
@@ -760,6 +883,28 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         if (with_b) p_bb = block_sum<PB>(p_bb);
     };
     // x += (M) V y over columns 0..kk of the basis
+    // w - sign * sum_{q < cnt} h_q V[q][r], subtracting in q order (registers, or gathers of 8
+    // basis entries issued before their first use)
+    auto combine_V = [&](double w, int64_t r, int cnt, const double *h, double sign) {
+        if constexpr (REGV) {
+            const int off = KR - nvec;
+#pragma unroll
+            for (int t = 0; t < KR; ++t) {
+                const int q = t - off;
+                if (q >= 0 && q < cnt) w -= sign * (h[q] * vreg[t]);
+            }
+        } else {
+            for (int c0 = 0; c0 < cnt; c0 += 8) {
+                double vv[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) vv[q] = c0 + q < cnt ? a.V[(size_t)(c0 + q) * n + r] : 0.0;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (c0 + q < cnt) w -= sign * (h[c0 + q] * vv[q]);
+            }
+        }
+        return w;
+    };
     auto update_x = [&](int kk) {
         if (threadIdx.x == 0) {
             for (int i = kk; i >= 0; --i) yv[i] = sv[i];
@@ -773,8 +918,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         for (int k = 0; k < RPT; ++k) {
             if (!own[k]) continue;
             const int64_t r = r0 + lr[k];
-            double z = 0.0;
-            for (int i = 0; i <= kk; ++i) z += yv[i] * a.V[(size_t)i * n + r];
+            const double z = combine_V(0.0, r, kk + 1, yv, -1.0);
             xr[k] += right ? prec(k, z) : z;
         }
     };
@@ -801,9 +945,11 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         // v0 = r / beta
         {
             const double inv = 1.0 / beta;
+            nvec = 0;
 #pragma unroll
             for (int k = 0; k < RPT; ++k)
-                if (own[k]) a.V[r0 + lr[k]] = wr[k] * inv;
+                vstore(0, r0 + lr[k], own[k] ? wr[k] * inv : 0.0, own[k]);
+            nvec = 1;
             if (threadIdx.x < KMAX + 1) sv[threadIdx.x] = 0.0;
             if (threadIdx.x == 0) sv[0] = beta;
             if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
@@ -820,14 +966,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 if (!right) w = prec(k, w);
                 wr[k] = w;
             }
-            for (int kk = 0; kk <= i; ++kk) {
-                double pk = 0.0;
-#pragma unroll
-                for (int k = 0; k < RPT; ++k)
-                    if (own[k]) pk += wr[k] * a.V[(size_t)kk * n + r0 + lr[k]];
-                pk = block_sum<PB>(pk);
-                if (threadIdx.x == 0) msh[kk] = pk;
-            }
+            dots(i + 1, false, msh);
             if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 1, red_sh)) { status = 2; goto done; }
             if (threadIdx.x <= i) Hc[threadIdx.x] = red_sh[threadIdx.x];
             __syncthreads();
@@ -836,18 +975,9 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
             for (int k = 0; k < RPT; ++k) {
                 if (!own[k]) continue;
                 const int64_t r = r0 + lr[k];
-                double w = wr[k];
-                for (int kk = 0; kk <= i; ++kk) w -= Hc[kk] * a.V[(size_t)kk * n + r];
-                wr[k] = w;
+                wr[k] = combine_V(wr[k], r, i + 1, Hc, 1.0);
             }
-            for (int kk = 0; kk <= i + 1; ++kk) {
-                double pk = 0.0;
-#pragma unroll
-                for (int k = 0; k < RPT; ++k)
-                    if (own[k]) pk += wr[k] * (kk <= i ? a.V[(size_t)kk * n + r0 + lr[k]] : wr[k]);
-                pk = block_sum<PB>(pk);
-                if (threadIdx.x == 0) msh[kk] = pk;
-            }
+            dots(i + 1, true, msh);
             if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 2, red_sh)) { status = 2; goto done; }
             double hn2 = red_sh[i + 1];
             for (int kk = 0; kk <= i; ++kk) hn2 -= red_sh[kk] * red_sh[kk];
@@ -855,14 +985,13 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
             // v_{i+1} = (w1 - V h2) / |.|
             {
                 const double inv = 1.0 / hnorm;
+                double vnew[RPT];
 #pragma unroll
-                for (int k = 0; k < RPT; ++k) {
-                    if (!own[k]) continue;
-                    const int64_t r = r0 + lr[k];
-                    double w = wr[k];
-                    for (int kk = 0; kk <= i; ++kk) w -= red_sh[kk] * a.V[(size_t)kk * n + r];
-                    a.V[(size_t)(i + 1) * n + r] = w * inv;
-                }
+                for (int k = 0; k < RPT; ++k)
+                    vnew[k] = own[k] ? combine_V(wr[k], r0 + lr[k], i + 1, red_sh, 1.0) * inv : 0.0;
+#pragma unroll
+                for (int k = 0; k < RPT; ++k) vstore(i + 1, r0 + lr[k], vnew[k], own[k]);
+                nvec = i + 2;
             }
             // Hessenberg column i: h1 + h2, then the Givens rotations (num/krylovsolver.h:258-280, 392-399)
             if (threadIdx.x == 0) {

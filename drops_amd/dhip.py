@@ -31,7 +31,11 @@ EXPORTS = [
     "dhip_csr_from_assembly", "dhip_csr_destroy", "dhip_csr_info", "dhip_spmv", "dhip_dot",
     "dhip_axpy", "dhip_pcg", "dhip_gmres", "dhip_gmres_diag", "dhip_ch_create", "dhip_ch_destroy",
     "dhip_ch_set_state", "dhip_ch_get_state", "dhip_ch_prepare", "dhip_ch_system", "dhip_ch_solve", "dhip_ch_step",
+    "dhip_mlp_create", "dhip_mlp_destroy", "dhip_mlp_params", "dhip_mlp_normalization", "dhip_mlp_features",
+    "dhip_mlp_targets", "dhip_mlp_predict", "dhip_mlp_grad", "dhip_mlp_train_step", "dhip_ch_history",
+    "dhip_ch_solve_learned",
 ]
+MLP_NPARAM, MLP_NF = 9282, 12
 
 
 class ChParams(C.Structure):
@@ -106,6 +110,19 @@ def load(path=LIB_PATH):
                                  C.POINTER(VP), I64, I64]
     L.dhip_ch_solve.argtypes = [P, VP, C.c_int32, I32, D]
     L.dhip_ch_step.argtypes = [P, C.c_int32, I32, D]
+    F = C.POINTER(C.c_float)
+    L.dhip_mlp_create.argtypes = [P, C.POINTER(P)]
+    L.dhip_mlp_destroy.argtypes = [P]
+    L.dhip_mlp_destroy.restype = None
+    L.dhip_mlp_params.argtypes = [P, F, F]
+    L.dhip_mlp_normalization.argtypes = [P, D, D, D, D]
+    L.dhip_mlp_features.argtypes = [P, C.POINTER(VP), C.c_int32, VP, VP, C.c_int64, VP]
+    L.dhip_mlp_targets.argtypes = [P, VP, VP, C.c_int64, VP]
+    L.dhip_mlp_predict.argtypes = [P, VP, C.c_int64, VP]
+    L.dhip_mlp_grad.argtypes = [P, VP, VP, VP, C.c_int64, D, D]
+    L.dhip_mlp_train_step.argtypes = [P, VP, VP, VP, C.c_int64, C.c_float, D]
+    L.dhip_ch_history.argtypes = [P, C.c_int32, C.POINTER(VP)]
+    L.dhip_ch_solve_learned.argtypes = [P, P, I32, D]
     _lib = L
     return L
 
@@ -211,6 +228,19 @@ class CahnHilliard:
             self.ctx.check(rc)
         return it.value, t.value, rc == OK
 
+    def history(self, level):
+        """Device pointer of the state `level` steps back (0 = current)."""
+        p = C.c_void_p()
+        self.ctx.check(self.L.dhip_ch_history(self.h, level, C.byref(p)))
+        return p.value
+
+    def solve_learned(self, mlp, raise_on_fail=True):
+        it, t = C.c_int32(), C.c_double()
+        rc = self.L.dhip_ch_solve_learned(self.h, mlp.h, C.byref(it), C.byref(t))
+        if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
+            self.ctx.check(rc)
+        return it.value, t.value, rc == OK
+
     def step(self, nsteps=1):
         its = np.zeros(nsteps, np.int32)
         res = np.zeros(nsteps)
@@ -220,6 +250,71 @@ class CahnHilliard:
     def close(self):
         if self.h and self.ctx.h:
             self.L.dhip_ch_destroy(self.h)
+        self.h = None
+        self.ctx._csrs.discard(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mlp:
+    """The learned warm-start network (dhip_mlp_t); float32 parameters in the layout of
+    oracle/mlp_oracle.py, device training and matrix-core inference."""
+
+    def __init__(self, ctx, theta=None):
+        self.ctx, self.L = ctx, ctx.L
+        self.h = C.c_void_p()
+        ctx.check(self.L.dhip_mlp_create(ctx.h, C.byref(self.h)))
+        ctx._csrs.add(self)
+        if theta is not None:
+            self.set_params(theta)
+
+    def set_params(self, theta):
+        th = np.ascontiguousarray(theta, np.float32)
+        assert th.size == MLP_NPARAM
+        self.ctx.check(self.L.dhip_mlp_params(self.h, _p(th, C.c_float), None))
+
+    def params(self):
+        th = np.zeros(MLP_NPARAM, np.float32)
+        self.ctx.check(self.L.dhip_mlp_params(self.h, None, _p(th, C.c_float)))
+        return th
+
+    def set_normalization(self, in_mean, in_scale, out_mean, out_scale):
+        arr = [np.ascontiguousarray(v, np.float64) for v in (in_mean, in_scale, out_mean, out_scale)]
+        self.norm = arr
+        self.ctx.check(self.L.dhip_mlp_normalization(self.h, *[_p(v, C.c_double) for v in arr]))
+
+    def features(self, hist_ptrs, rhs_ptr, pcdiag_ptr, n, X_ptr):
+        hp = (C.c_void_p * 4)(*[C.c_void_p(p) for p in hist_ptrs] + [None] * (4 - len(hist_ptrs)))
+        self.ctx.check(self.L.dhip_mlp_features(self.h, hp, len(hist_ptrs), C.c_void_p(rhs_ptr),
+                                                C.c_void_p(pcdiag_ptr), n, C.c_void_p(X_ptr)))
+
+    def targets(self, xnext_ptr, xcur_ptr, n, T_ptr):
+        self.ctx.check(self.L.dhip_mlp_targets(self.h, C.c_void_p(xnext_ptr), C.c_void_p(xcur_ptr), n,
+                                               C.c_void_p(T_ptr)))
+
+    def predict(self, X_ptr, S, Y_ptr):
+        self.ctx.check(self.L.dhip_mlp_predict(self.h, C.c_void_p(X_ptr), S, C.c_void_p(Y_ptr)))
+
+    def grad(self, X_ptr, T_ptr, idx_ptr, B):
+        g = np.zeros(MLP_NPARAM)
+        loss = C.c_double()
+        self.ctx.check(self.L.dhip_mlp_grad(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), C.c_void_p(idx_ptr), B,
+                                            _p(g, C.c_double), C.byref(loss)))
+        return loss.value, g
+
+    def train_step(self, X_ptr, T_ptr, idx_ptr, B, lr, want_loss=False):
+        loss = C.c_double()
+        self.ctx.check(self.L.dhip_mlp_train_step(self.h, C.c_void_p(X_ptr), C.c_void_p(T_ptr), C.c_void_p(idx_ptr),
+                                                  B, lr, C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.L.dhip_mlp_destroy(self.h)
         self.h = None
         self.ctx._csrs.discard(self)
 
@@ -463,6 +558,9 @@ class Context:
 
     def cahn_hilliard(self, dt, epsilon, rho, **kw):
         return CahnHilliard(self, dt, epsilon, rho, **kw)
+
+    def mlp(self, theta=None):
+        return Mlp(self, theta)
 
     def gmres(self, A, b, x0=None, pc=PC_JACOBI, restart=50, max_iter=10000, tol=1e-10, rel=True, left_pc=False,
               raise_on_fail=True):

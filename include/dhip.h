@@ -239,6 +239,43 @@ int dhip_ch_solve(dhip_ch_t *ch, const double *x0, int32_t where, int32_t *iters
 /* nsteps x (prepare + solve from the previous step); iters/resid (may be NULL) get one entry per step. */
 int dhip_ch_step(dhip_ch_t *ch, int32_t nsteps, int32_t *iters, double *resid);
 
+/* ---- learned warm start (no reference counterpart: an addition of this library) ----
+ * A per-DOF network 12 -> 64 -> 64 -> 64 -> 2 (tanh hidden layers, bf16 weights and layer inputs,
+ * float32 accumulation on the matrix cores) predicts the change of [mu; c] over one step from
+ * the last four time levels, the step's right-hand side and the preconditioner diagonal.  GMRES
+ * still runs to the same tolerance from the predicted start.  Parameter layout (float32, 9282):
+ * W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2] (weights [out][in]).
+ * Features per DOF i (float64, then (f - in_mean) * in_scale, stored float32 [n][12]):
+ *   c0, c0-c1, c1-c2, c2-c3, mu0, mu0-mu1, mu1-mu2, mu2-mu3, rhs_i/d_i, rhs_{n+i}/d_{n+i}, d_i, d_{n+i}
+ * (index 0 = newest level; missing levels repeat the oldest).  Prediction: change = y * out_scale
+ * + out_mean, in the order (mu, c). */
+#define DHIP_MLP_NPARAM 9282
+#define DHIP_MLP_NF 12
+typedef struct dhip_mlp dhip_mlp_t;
+int dhip_mlp_create(dhip_ctx_t *ctx, dhip_mlp_t **mlp);
+void dhip_mlp_destroy(dhip_mlp_t *mlp);
+/* set (theta_in, host, resets the optimiser) and/or read (theta_out, host) the parameters */
+int dhip_mlp_params(dhip_mlp_t *mlp, const float *theta_in, float *theta_out);
+int dhip_mlp_normalization(dhip_mlp_t *mlp, const double in_mean[12], const double in_scale[12],
+                           const double out_mean[2], const double out_scale[2]);
+/* device pointers: hist[0..nhist) newest first, rhs and pcdiag of length 2n, X [n][12] */
+int dhip_mlp_features(dhip_mlp_t *mlp, const double *const hist[4], int32_t nhist, const double *rhs,
+                      const double *pcdiag, int64_t n, float *X);
+/* device pointers: T[n][2] = ((x_next - x_cur) - out_mean) / out_scale */
+int dhip_mlp_targets(dhip_mlp_t *mlp, const double *x_next, const double *x_cur, int64_t n, float *T);
+/* device pointers: Y[S][2] (normalised changes) */
+int dhip_mlp_predict(dhip_mlp_t *mlp, const float *X, int64_t S, float *Y);
+/* minibatch of B samples idx (device) from X, T (device): gradient (host, may be NULL) and loss
+ * 1/2 mean |y - t|^2; dhip_mlp_train_step applies one Adam step (beta 0.9, 0.999). */
+int dhip_mlp_grad(dhip_mlp_t *mlp, const float *X, const float *T, const int32_t *idx, int64_t B,
+                  double *grad, double *loss);
+int dhip_mlp_train_step(dhip_mlp_t *mlp, const float *X, const float *T, const int32_t *idx, int64_t B,
+                        float lr, double *loss);
+/* device pointer of the state `level` steps back (0 = current) as the stepper holds them */
+int dhip_ch_history(dhip_ch_t *ch, int32_t level, const double **state);
+/* dhip_ch_solve from the network's predicted start */
+int dhip_ch_solve_learned(dhip_ch_t *ch, dhip_mlp_t *mlp, int32_t *iters, double *resid);
+
 #ifdef __cplusplus
 }
 #endif
