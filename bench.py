@@ -1,4 +1,5 @@
-"""Benchmark of the trace-FEM surface solve on one MI355X.
+"""Benchmark of the trace-FEM surface solve on one MI355X (and, beside it, the Cahn-Hilliard
+trajectory with and without the learned warm start).
 
 Workload (configuration 2): P1 TraceFEM Laplace-Beltrami + mass + h-scaled
 normal-derivative stabilisation for phi = |x|^2 - 1 on [-2,2]^3 with 128^3 Kuhn
@@ -112,6 +113,61 @@ def cpu_baseline(mesh, phi, coef, lc, budget_s=30.0):
                 threads=threads, reps=len(times), iters=out[1]), out
 
 
+def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=3, with_cpu=True):
+    """Cahn-Hilliard trajectory on the torus (implicit Euler, GMRES(50) + block Jacobi to 1e-8):
+    the previous-step start against the learned start (trained weights from
+    drops_amd/data/ch_warmstart_torus<cells>.npz, produced by `python -m drops_amd.learn`),
+    on a seed the network was not trained on; the CPU port times the first steps."""
+    from drops_amd import ch_run, learn
+    path = os.path.join(learn.DATA_DIR, f"ch_warmstart_torus{cells}.npz")
+    m, phi = ch_run.torus_mesh(cells)
+    out = {"workload": f"P1 trace-FEM Cahn-Hilliard (CahnHilliardcGP1CL, theta 1), torus R=1 r=0.3 on "
+                       f"[-2,2]^3, {cells}^3 Kuhn cells, {steps} implicit Euler steps dt={ch_run.DT} "
+                       f"eps={ch_run.EPS}, GMRES(50) right block-Jacobi to rel. residual 1e-8",
+           "seed": seed}
+    with D.Context(0) as ctx:
+        ch = ch_run.make_stepper(ctx, m, phi)
+        out["dofs"] = 2 * ch.n
+        if os.path.exists(path):
+            mlp, meta = learn.load(ctx, path)
+            out["trained_on_seeds"] = meta.get("train_seeds")
+            ev = learn.evaluate(ch, mlp, seed, steps)
+            mlp.close()
+        else:
+            log(f"[bench] {path} missing: previous-step start only")
+            ev = learn.evaluate(ch, None, seed, steps, variants=("previous",))
+        for v, r in ev.items():
+            out[v] = {"s_total": float(r["wall"].sum()), "s_per_step": float(r["wall"].mean()),
+                      "gmres_iters": int(r["iters"].sum()), "iters_first_steps": r["iters"][:5].tolist(),
+                      "iters_last_steps": r["iters"][-3:].tolist(), "s_first_3_steps": float(r["wall"][:3].sum())}
+            if v != "previous":
+                out[v]["starts"] = r["starts"]
+            log(f"[bench] ch {v}: {out[v]['s_total']:.3f} s for {steps} steps, {out[v]['gmres_iters']} GMRES iterations")
+        if "learned" in ev:
+            out["speedup_learned_vs_previous"] = out["previous"]["s_total"] / out["learned"]["s_total"]
+            out["speedup_extrapolation_vs_previous"] = out["previous"]["s_total"] / out["extrapolation"]["s_total"]
+            cp, cl = ev["previous"]["state"][0], ev["learned"]["state"][0]
+            out["c_rel_diff_learned_vs_previous"] = float(np.linalg.norm(cl - cp) / np.linalg.norm(cp))
+        ch.close()
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.make_mesh(m.orig, m.e, m.n)
+        dof, nd = O.number_p1if(om, phi, kind="omp")
+        r = O.assemble_p1(om, phi, dof, nd, coef=(1.0, 1.0, 1.0), kind="omp")
+        c0, mu0 = ch_run.initial_state(nd, seed)
+        x = np.concatenate([mu0, c0])
+        t0 = time.perf_counter()
+        for s in range(cpu_steps):
+            O.ch_step_c(om, phi, dof, nd, r["rowptr"], r["colind"], r["mass"], r["lb"], r["stab"], x, ch_run.DT,
+                        ch_run.EPS, m.h, kind="omp")
+        cpu_s = time.perf_counter() - t0
+        out["cpu_port"] = {"s_first_steps": cpu_s, "steps": cpu_steps,
+                           "sample": f"first {cpu_steps} steps of the same seed (the costliest: most GMRES "
+                                     f"iterations), oracle OpenMP build"}
+    return out
+
+
 def measured_traffic(kernel, rows, ndof):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
     each scaled by a known-size copy), when it was measured at this block geometry and system size."""
@@ -160,6 +216,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--no-ch", action="store_true", help="skip the Cahn-Hilliard trajectory leg")
+    ap.add_argument("--ch-cells", type=int, default=96)
+    ap.add_argument("--ch-steps", type=int, default=200)
     ap.add_argument("--probe-sweep", action="store_true",
                     help="print the per-iteration compute/sync breakdown of the persistent PCG for a sweep of "
                          "rows per block and exit")
@@ -263,6 +322,8 @@ def main():
         err["cpu"] = float(np.linalg.norm(xc - xs) / np.linalg.norm(xs))
     ctx.close()
 
+    ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu)
+
     v = results["fp64"]["s_per_solve"]
     line = {
         "metric": "time-to-1e-8 residual per surface solve (s) + SpMV HBM GB/s, learned vs plain",
@@ -290,7 +351,7 @@ def main():
         "roofline": {"kernel": "pcg64", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pcg64", rows_best, ndof),
                      "bytes_per_iter": bytes_per_iter, "iters": iters, "avg_launch_us": pk["avg_us"]},
-        "learned": None,
+        "learned": ch,
     }
     if cpu is not None:
         line["cpu_baseline"] = {"value": cpu["value"], "unit": "s per solve", "cores": cpu["threads"],

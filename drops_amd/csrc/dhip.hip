@@ -1211,7 +1211,7 @@ struct dhip_ch {
     // previous time levels for the learned start: hist[k] = state k+1 levels back (k < nhist)
     DevBuf hist[3];
     int nhist = 0;
-    DevBuf feat, pred;
+    DevBuf feat, pred, xstart;
     dhip_csr K;
     bool prepared = false;
 };
@@ -1225,7 +1225,7 @@ static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
 static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
                       &h->col, &h->mass, &h->lb, &h->stab, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
-                      &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred};
+                      &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred, &h->xstart};
     for (DevBuf *b : bufs) b->release();
 }
 
@@ -1455,8 +1455,22 @@ struct dhip_mlp {
     MlpNorm norm{};
     int adam_step = 0;
     DevBuf theta, m, v, packed, grad, part;
+    DevBuf scale, spart;  // step scale (s_mu, s_c) and its partial sums
     bool packed_ok = false;
 };
+
+static constexpr int SCALE_BLOCKS = 128;
+
+// s_mu, s_c from x_n and x_{n-1} (both 2n, device); 1 where there is no history
+static int mlp_scale_impl(dhip_mlp *q, const double *x0, const double *x1, int64_t n) {
+    dhip_ctx *c = q->ctx;
+    DHIP_HIPCHK(c, q->scale.ensure(sizeof(double) * 2));
+    DHIP_HIPCHK(c, q->spart.ensure(sizeof(double) * 2 * SCALE_BLOCKS));
+    DHIP_LAUNCH(c, "mlp_scale_part", k_mlp_scale_part, SCALE_BLOCKS, SCALE_B, 0, x0, x1, n, q->spart.as<double>());
+    DHIP_LAUNCH(c, "mlp_scale_final", k_mlp_scale_final, 1, 64, 0, q->spart.as<double>(), SCALE_BLOCKS, n,
+                q->scale.as<double>());
+    return DHIP_OK;
+}
 
 static int mlp_pack(dhip_mlp *q) {
     if (q->packed_ok) return DHIP_OK;
@@ -1477,13 +1491,16 @@ static int mlp_predict_impl(dhip_mlp *q, const float *X, int64_t S, float *Y) {
     return DHIP_OK;
 }
 
-static int mlp_features_impl(dhip_mlp *q, const double *const hist[4], int32_t nhist, const double *rhs,
+static int mlp_features_impl(dhip_mlp *q, const double *const hist[4], int32_t nhist, const double *res,
                              const double *pcdiag, int64_t n, float *X) {
     if (nhist < 1 || nhist > 4) return set_error(q->ctx, DHIP_E_ARG, "1 to 4 history levels");
     const double *h[4];
     for (int k = 0; k < 4; ++k) h[k] = hist[k < nhist ? k : nhist - 1];
-    FeatArgs fa{h[0], h[1], h[2], h[3], rhs, pcdiag, n, q->norm, X};
-    if (n > 0) DHIP_LAUNCH(q->ctx, "mlp_features", k_mlp_features, blocks_for(n, 256), 256, 0, fa);
+    if (n == 0) return DHIP_OK;
+    int rc = mlp_scale_impl(q, h[0], h[1], n);
+    if (rc) return rc;
+    FeatArgs fa{h[0], h[1], h[2], h[3], res, pcdiag, q->scale.as<double>(), n, q->norm, X};
+    DHIP_LAUNCH(q->ctx, "mlp_features", k_mlp_features, blocks_for(n, 256), 256, 0, fa);
     return DHIP_OK;
 }
 
@@ -1527,7 +1544,7 @@ void dhip_mlp_destroy(dhip_mlp_t *q) {
     if (!q) return;
     (void)hipSetDevice(q->ctx->device);
     (void)hipStreamSynchronize(q->ctx->stream);
-    DevBuf *b[] = {&q->theta, &q->m, &q->v, &q->packed, &q->grad, &q->part};
+    DevBuf *b[] = {&q->theta, &q->m, &q->v, &q->packed, &q->grad, &q->part, &q->scale, &q->spart};
     for (DevBuf *x : b) x->release();
     delete q;
 }
@@ -1566,23 +1583,30 @@ int dhip_mlp_normalization(dhip_mlp_t *q, const double in_mean[12], const double
     return DHIP_OK;
 }
 
-int dhip_mlp_features(dhip_mlp_t *q, const double *const hist[4], int32_t nhist, const double *rhs,
+int dhip_mlp_features(dhip_mlp_t *q, const double *const hist[4], int32_t nhist, const double *res,
                       const double *pcdiag, int64_t n, float *X) {
-    if (!q || !hist || !rhs || !pcdiag || !X || n < 0) return DHIP_E_ARG;
+    if (!q || !hist || !res || !pcdiag || !X || n < 0) return DHIP_E_ARG;
     dhip_ctx *c = q->ctx;
     DHIP_HIPCHK(c, hipSetDevice(c->device));
-    int rc = mlp_features_impl(q, hist, nhist, rhs, pcdiag, n, X);
+    int rc = mlp_features_impl(q, hist, nhist, res, pcdiag, n, X);
     if (rc) return rc;
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return flush_profile(c);
 }
 
-int dhip_mlp_targets(dhip_mlp_t *q, const double *x_next, const double *x_cur, int64_t n, float *T) {
-    if (!q || !x_next || !x_cur || !T || n < 0) return DHIP_E_ARG;
+int dhip_mlp_targets(dhip_mlp_t *q, const double *x_next, const double *const hist[3], int32_t nhist, int64_t n,
+                     float *T) {
+    if (!q || !x_next || !hist || !T || n < 0 || nhist < 1 || nhist > 3) return DHIP_E_ARG;
     dhip_ctx *c = q->ctx;
     DHIP_HIPCHK(c, hipSetDevice(c->device));
-    TargetArgs ta{x_next, x_cur, n, q->norm, T};
-    if (n > 0) DHIP_LAUNCH(c, "mlp_targets", k_mlp_targets, blocks_for(n, 256), 256, 0, ta);
+    const double *h[3];
+    for (int k = 0; k < 3; ++k) h[k] = hist[k < nhist ? k : nhist - 1];
+    if (n > 0) {
+        int rc = mlp_scale_impl(q, h[0], h[1], n);
+        if (rc) return rc;
+        TargetArgs ta{x_next, h[0], h[1], h[2], q->scale.as<double>(), n, q->norm, T};
+        DHIP_LAUNCH(c, "mlp_targets", k_mlp_targets, blocks_for(n, 256), 256, 0, ta);
+    }
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return flush_profile(c);
 }
@@ -1642,32 +1666,98 @@ int dhip_ch_history(dhip_ch_t *h, int32_t level, const double **state) {
     return DHIP_OK;
 }
 
-int dhip_ch_solve_learned(dhip_ch_t *h, dhip_mlp_t *q, int32_t *iters, double *resid) {
+int dhip_ch_residual(dhip_ch_t *h, double *r) {
+    if (!h || !r) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_residual");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n2 = 2 * h->ndof;
+    if (n2 > 0)
+        DHIP_LAUNCH(c, "ch_residual", k_ch_residual, blocks_for(n2, 256), 256, 0, n2, h->K.rowptr, h->K.col, h->K.val,
+                    h->rhs.as<double>(), h->x.as<double>(), r);
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+// Candidate starts for the prepared step from the current state x_n and the history (nothing is
+// modified): the network's, the plain quadratic extrapolation and x_n; the one with the smallest
+// residual is written to x0 (device) and its index to flag (device).
+static int ch_learned_start(dhip_ch *h, dhip_mlp *q, double *x0, int32_t **flag_out) {
+    dhip_ctx *c = h->ctx;
+    const int64_t n = h->ndof, n2 = 2 * n;
+    DHIP_HIPCHK(c, h->feat.ensure(sizeof(float) * MLP_NF * (n + 1)));
+    DHIP_HIPCHK(c, h->pred.ensure(sizeof(float) * MLP_NO * (n + 1)));
+    DHIP_HIPCHK(c, c->kv[11].ensure(sizeof(double) * (n2 + 1)));
+    DHIP_HIPCHK(c, c->kv[10].ensure(sizeof(double) * 4 * SCALE_BLOCKS + sizeof(int32_t) * 4));
+    DHIP_HIPCHK(c, c->kv[9].ensure(sizeof(double) * (n2 + 1)));
+    double *res = c->kv[11].as<double>();
+    double *part = c->kv[10].as<double>();
+    double *xq = c->kv[9].as<double>();
+    int32_t *flag = reinterpret_cast<int32_t *>(part + 4 * SCALE_BLOCKS);
+    const double *xn = h->x.as<double>();
+    // residual of the new system at the current state, features, prediction
+    DHIP_LAUNCH(c, "ch_residual", k_ch_residual, blocks_for(n2, 256), 256, 0, n2, h->K.rowptr, h->K.col, h->K.val,
+                h->rhs.as<double>(), xn, res);
+    const double *hist[4] = {xn, h->hist[0].as<double>(), h->hist[1].as<double>(), h->hist[2].as<double>()};
+    const int levels = 1 + h->nhist;
+    int rc = mlp_features_impl(q, hist, levels, res, h->pcdiag.as<double>(), n, h->feat.as<float>());
+    if (!rc) rc = mlp_predict_impl(q, h->feat.as<float>(), n, h->pred.as<float>());
+    if (rc) return rc;
+    const double *h1 = hist[std::min(1, levels - 1)], *h2 = hist[std::min(2, levels - 1)];
+    CombineArgs ca{xn, h1, h2, h->pred.as<float>(), q->scale.as<double>(), n, q->norm, x0, xq};
+    DHIP_LAUNCH(c, "mlp_combine", k_mlp_combine, blocks_for(n, 256), 256, 0, ca);
+    DHIP_LAUNCH(c, "ch_three_resid", k_ch_three_resid, SCALE_BLOCKS, RES3_B, 0, n2, h->K.rowptr, h->K.col, h->K.val,
+                h->rhs.as<double>(), xn, x0, xq, part);
+    // the network's candidate is trusted while the extrapolation's residual exceeds 100 x tolerance
+    const double gate = 100.0 * h->p.tol;
+    DHIP_LAUNCH(c, "ch_pick", k_ch_pick, 1, 64, 0, part, SCALE_BLOCKS, gate, flag);
+    DHIP_LAUNCH(c, "ch_keep", k_ch_keep, blocks_for(n2, 256), 256, 0, n2, flag, xn, xq, x0);
+    *flag_out = flag;
+    return DHIP_OK;
+}
+
+int dhip_ch_predict_start(dhip_ch_t *h, dhip_mlp_t *q, double *x0, int32_t where, int32_t *start) {
+    if (!h || !q || !x0) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    if (q->ctx != c) return set_error(c, DHIP_E_ARG, "network and stepper belong to different contexts");
+    if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_predict_start");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n2 = 2 * h->ndof;
+    int32_t st = 0;
+    if (h->nhist == 0 || n2 == 0) {
+        if (n2) DHIP_HIPCHK(c, hipMemcpyAsync(x0, h->x.p, sizeof(double) * n2,
+                                              where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                              c->stream));
+    } else {
+        DHIP_HIPCHK(c, h->xstart.ensure(sizeof(double) * (n2 + 1)));
+        int32_t *flag = nullptr;
+        int rc = ch_learned_start(h, q, h->xstart.as<double>(), &flag);
+        if (rc) return rc;
+        DHIP_HIPCHK(c, hipMemcpyAsync(&st, flag, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipMemcpyAsync(x0, h->xstart.p, sizeof(double) * n2,
+                                      where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                      c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (start) *start = st;
+    return flush_profile(c);
+}
+
+int dhip_ch_solve_learned(dhip_ch_t *h, dhip_mlp_t *q, int32_t *iters, double *resid, int32_t *start) {
     if (!h || !q) return DHIP_E_ARG;
     dhip_ctx *c = h->ctx;
     if (q->ctx != c) return set_error(c, DHIP_E_ARG, "network and stepper belong to different contexts");
     if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_solve_learned");
     DHIP_HIPCHK(c, hipSetDevice(c->device));
-    const int64_t n = h->ndof;
-    int rc = DHIP_OK;
-    if (n > 0) {
-        DHIP_HIPCHK(c, h->feat.ensure(sizeof(float) * MLP_NF * (n + 1)));
-        DHIP_HIPCHK(c, h->pred.ensure(sizeof(float) * MLP_NO * (n + 1)));
-        const double *hist[4] = {h->x.as<double>(), h->hist[0].as<double>(), h->hist[1].as<double>(),
-                                 h->hist[2].as<double>()};
-        rc = mlp_features_impl(q, hist, 1 + h->nhist, h->rhs.as<double>(), h->pcdiag.as<double>(), n,
-                               h->feat.as<float>());
-        if (!rc) rc = mlp_predict_impl(q, h->feat.as<float>(), n, h->pred.as<float>());
-        if (!rc) rc = ch_push_history(h);
-        if (!rc) {
-            CombineArgs ca{h->hist[0].as<double>(), h->pred.as<float>(), n, q->norm, h->x.as<double>()};
-            DHIP_LAUNCH(c, "mlp_combine", k_mlp_combine, blocks_for(n, 256), 256, 0, ca);
-        }
-        if (!rc) rc = ch_solve_impl(h, iters, resid);
-    }
-    h->prepared = false;
-    const int rp = flush_profile(c);
-    return rc ? rc : rp;
+    if (start) *start = 0;
+    if (h->nhist == 0 || h->ndof == 0) return dhip_ch_solve(h, nullptr, DHIP_DEVICE, iters, resid);
+    DHIP_HIPCHK(c, h->xstart.ensure(sizeof(double) * (2 * h->ndof + 1)));
+    int32_t *flag = nullptr;
+    int rc = ch_learned_start(h, q, h->xstart.as<double>(), &flag);
+    if (rc) return rc;
+    if (start) DHIP_HIPCHK(c, hipMemcpyAsync(start, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    // the verified stepper path: x_n into the history, the start into the state, GMRES
+    return dhip_ch_solve(h, h->xstart.as<double>(), DHIP_DEVICE, iters, resid);
 }
 
 }  // extern "C"

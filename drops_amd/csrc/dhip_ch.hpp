@@ -264,4 +264,70 @@ __global__ void k_ch_block(ChBlockArgs a) {
     }
 }
 
+// r = b - K x (one lane per row of the block system)
+__global__ void k_ch_residual(int64_t n2, const int32_t *rp, const int32_t *col, const double *val, const double *b,
+                              const double *x, double *r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    double s = 0.0;
+    for (int32_t e = rp[i]; e < rp[i + 1]; ++e) s += val[e] * x[col[e]];
+    r[i] = b[i] - s;
+}
+
+// |b - K x_k|^2 of three candidate starts and |b|^2 per block (block-order sums later)
+constexpr int RES3_B = 256;
+__global__ void __launch_bounds__(RES3_B) k_ch_three_resid(int64_t n2, const int32_t *rp, const int32_t *col,
+                                                           const double *val, const double *b, const double *x0,
+                                                           const double *x1, const double *x2, double *part) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, sb = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * RES3_B + threadIdx.x; i < n2; i += (int64_t)gridDim.x * RES3_B) {
+        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
+        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
+            const double v = val[e];
+            const int32_t cj = col[e];
+            p0 += v * x0[cj];
+            p1 += v * x1[cj];
+            p2 += v * x2[cj];
+        }
+        const double r0 = b[i] - p0, r1 = b[i] - p1, r2 = b[i] - p2;
+        s0 += r0 * r0;
+        s1 += r1 * r1;
+        s2 += r2 * r2;
+        sb += b[i] * b[i];
+    }
+    s0 = block_sum<RES3_B>(s0);
+    s1 = block_sum<RES3_B>(s1);
+    s2 = block_sum<RES3_B>(s2);
+    sb = block_sum<RES3_B>(sb);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x] = s0;
+        part[4 * blockIdx.x + 1] = s1;
+        part[4 * blockIdx.x + 2] = s2;
+        part[4 * blockIdx.x + 3] = sb;
+    }
+}
+
+// Candidates 0 = x_n, 1 = network, 2 = linear extrapolation.  The smaller residual of x_n and the
+// extrapolation wins; the network's candidate replaces it when its residual is smaller still and
+// the extrapolation is far from converged (residual above gate * |b|, the transient), where the
+// residual is dominated by the change of the step and not by the solver's tolerance-level error.
+__global__ void k_ch_pick(const double *part, int nb, double gate, int32_t *flag) {
+    if (threadIdx.x != 0) return;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nb; ++k)
+        for (int q = 0; q < 4; ++q) s[q] += part[4 * k + q];
+    int best = s[2] < s[0] ? 2 : 0;
+    if (s[1] < s[best] && s[2] > gate * gate * s[3]) best = 1;
+    *flag = best;
+}
+
+// x (holding candidate 1) becomes candidate `flag`
+__global__ void k_ch_keep(int64_t n2, const int32_t *flag, const double *x0, const double *x2, double *x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n2) return;
+    const int f = *flag;
+    if (f == 0) x[i] = x0[i];
+    else if (f == 2) x[i] = x2[i];
+}
+
 }  // namespace dhip

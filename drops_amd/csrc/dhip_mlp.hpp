@@ -38,44 +38,90 @@ __device__ __forceinline__ float bfround(float f) { return __uint_as_float((uint
 
 struct MlpNorm {
     double in_mean[MLP_NF], in_scale[MLP_NF];   // feature -> (feature - mean) * scale
-    double out_mean[MLP_NO], out_scale[MLP_NO]; // change = y * scale + mean
+    double out_mean[MLP_NO], out_scale[MLP_NO]; // correction = y * scale + mean
 };
 
+// Step scale: s_mu, s_c = RMS over the DOFs of the last change of mu and c (1 where it is 0).
+// Two passes: per-block partial sums of squares, then one block adds them in block order.
+constexpr int SCALE_B = 256;
+
+__global__ void __launch_bounds__(SCALE_B) k_mlp_scale_part(const double *x0, const double *x1, int64_t n,
+                                                            double *part) {
+    double sm = 0.0, sc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * SCALE_B + threadIdx.x; i < n; i += (int64_t)gridDim.x * SCALE_B) {
+        const double dm = x0[i] - x1[i], dc = x0[n + i] - x1[n + i];
+        sm += dm * dm;
+        sc += dc * dc;
+    }
+    sm = block_sum<SCALE_B>(sm);
+    sc = block_sum<SCALE_B>(sc);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sm;
+        part[2 * blockIdx.x + 1] = sc;
+    }
+}
+
+__global__ void k_mlp_scale_final(const double *part, int nb, int64_t n, double *scale) {
+    if (threadIdx.x != 0) return;
+    double sm = 0.0, sc = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        sm += part[2 * b];
+        sc += part[2 * b + 1];
+    }
+    const double rm = sqrt(sm / (double)n), rc = sqrt(sc / (double)n);
+    scale[0] = rm > 0.0 ? rm : 1.0;
+    scale[1] = rc > 0.0 ? rc : 1.0;
+}
+
 // ---------------------------------------------------------------------------------------------
+// Features of DOF i for the step n -> n+1 (h0 = x_n, newest first; s = step scale):
+//   c_n, dc_n/s_c, dc_{n-1}/s_c, dc_{n-2}/s_c, mu_n, dmu_n/s_mu, dmu_{n-1}/s_mu, dmu_{n-2}/s_mu,
+//   (r_i/d_i)/s_mu, (r_{n+i}/d_{n+i})/s_c, d_i, d_{n+i}
+// with dv_k = v_k - v_{k-1}, r = b - K x_n the residual of the new system at the current state
+// (the local right-hand side of the correction), d the preconditioner diagonal.
 struct FeatArgs {
-    const double *h0, *h1, *h2, *h3;  // states [mu; c], newest first (missing levels repeat)
-    const double *rhs, *pcdiag;       // step system (2n)
+    const double *h0, *h1, *h2, *h3;
+    const double *res, *pcdiag;
+    const double *scale;
     int64_t n;
     MlpNorm norm;
-    float *X;                          // [n][12]
+    float *X;  // [n][12]
 };
 
 __global__ void k_mlp_features(FeatArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const int64_t n = a.n;
+    const double sm = a.scale[0], sc = a.scale[1];
     double f[MLP_NF];
     const double c0 = a.h0[n + i], c1 = a.h1[n + i], c2 = a.h2[n + i], c3 = a.h3[n + i];
     const double m0 = a.h0[i], m1 = a.h1[i], m2 = a.h2[i], m3 = a.h3[i];
     f[0] = c0;
-    f[1] = c0 - c1;
-    f[2] = c1 - c2;
-    f[3] = c2 - c3;
+    f[1] = (c0 - c1) / sc;
+    f[2] = (c1 - c2) / sc;
+    f[3] = (c2 - c3) / sc;
     f[4] = m0;
-    f[5] = m0 - m1;
-    f[6] = m1 - m2;
-    f[7] = m2 - m3;
-    f[8] = a.rhs[i] / a.pcdiag[i];
-    f[9] = a.rhs[n + i] / a.pcdiag[n + i];
+    f[5] = (m0 - m1) / sm;
+    f[6] = (m1 - m2) / sm;
+    f[7] = (m2 - m3) / sm;
+    f[8] = (a.res[i] / a.pcdiag[i]) / sm;
+    f[9] = (a.res[n + i] / a.pcdiag[n + i]) / sc;
     f[10] = a.pcdiag[i];
     f[11] = a.pcdiag[n + i];
 #pragma unroll
     for (int k = 0; k < MLP_NF; ++k) a.X[i * MLP_NF + k] = (float)((f[k] - a.norm.in_mean[k]) * a.norm.in_scale[k]);
 }
 
-// targets: ((x_next - x_cur) - mean) * (1 / scale), per DOF [mu, c]
+// The prediction is a correction of linear extrapolation in scaled units:
+//   x_{n+1} = x_n + s * (dx_n/s + y * out_scale + out_mean).
+// (Linear, not quadratic: extrapolating the solver's own tolerance-level errors with weights
+// 3, -3, 1 lets them grow from step to step in the slowly converging modes of GMRES.)
+__device__ __forceinline__ double extrap_scaled(double v0, double v1, double, double s) { return (v0 - v1) / s; }
+
+// targets: ((x_next - x_n)/s - extrapolation - mean) / out_scale, per DOF [mu, c]
 struct TargetArgs {
-    const double *xn, *xc;
+    const double *xn1, *h0, *h1, *h2;
+    const double *scale;
     int64_t n;
     MlpNorm norm;
     float *T;  // [n][2]
@@ -84,25 +130,40 @@ struct TargetArgs {
 __global__ void k_mlp_targets(TargetArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const double dm = a.xn[i] - a.xc[i], dc = a.xn[a.n + i] - a.xc[a.n + i];
-    a.T[2 * i] = (float)((dm - a.norm.out_mean[0]) / a.norm.out_scale[0]);
-    a.T[2 * i + 1] = (float)((dc - a.norm.out_mean[1]) / a.norm.out_scale[1]);
+    const int64_t n = a.n;
+#pragma unroll
+    for (int o = 0; o < MLP_NO; ++o) {
+        const int64_t j = o == 0 ? i : n + i;  // o = 0: mu, o = 1: c
+        const double s = a.scale[o];
+        const double corr = (a.xn1[j] - a.h0[j]) / s - extrap_scaled(a.h0[j], a.h1[j], a.h2[j], s);
+        a.T[2 * i + o] = (float)((corr - a.norm.out_mean[o]) / a.norm.out_scale[o]);
+    }
 }
 
-// start vector x0 = x_cur + (y * scale + mean)
+// candidate starts: x0 = x_n + s * (extrapolation + y * out_scale + out_mean) (the network's) and
+// xq = x_n + s * extrapolation (linear extrapolation alone)
 struct CombineArgs {
-    const double *xc;
+    const double *h0, *h1, *h2;
     const float *Y;
+    const double *scale;
     int64_t n;
     MlpNorm norm;
-    double *x0;
+    double *x0, *xq;
 };
 
 __global__ void k_mlp_combine(CombineArgs a) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    a.x0[i] = a.xc[i] + ((double)a.Y[2 * i] * a.norm.out_scale[0] + a.norm.out_mean[0]);
-    a.x0[a.n + i] = a.xc[a.n + i] + ((double)a.Y[2 * i + 1] * a.norm.out_scale[1] + a.norm.out_mean[1]);
+    const int64_t n = a.n;
+#pragma unroll
+    for (int o = 0; o < MLP_NO; ++o) {
+        const int64_t j = o == 0 ? i : n + i;
+        const double s = a.scale[o];
+        const double ex = extrap_scaled(a.h0[j], a.h1[j], a.h2[j], s);
+        const double corr = (double)a.Y[2 * i + o] * a.norm.out_scale[o] + a.norm.out_mean[o];
+        a.x0[j] = a.h0[j] + s * (ex + corr);
+        a.xq[j] = a.h0[j] + s * ex;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------

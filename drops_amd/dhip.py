@@ -33,7 +33,7 @@ EXPORTS = [
     "dhip_ch_set_state", "dhip_ch_get_state", "dhip_ch_prepare", "dhip_ch_system", "dhip_ch_solve", "dhip_ch_step",
     "dhip_mlp_create", "dhip_mlp_destroy", "dhip_mlp_params", "dhip_mlp_normalization", "dhip_mlp_features",
     "dhip_mlp_targets", "dhip_mlp_predict", "dhip_mlp_grad", "dhip_mlp_train_step", "dhip_ch_history",
-    "dhip_ch_solve_learned",
+    "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -117,12 +117,14 @@ def load(path=LIB_PATH):
     L.dhip_mlp_params.argtypes = [P, F, F]
     L.dhip_mlp_normalization.argtypes = [P, D, D, D, D]
     L.dhip_mlp_features.argtypes = [P, C.POINTER(VP), C.c_int32, VP, VP, C.c_int64, VP]
-    L.dhip_mlp_targets.argtypes = [P, VP, VP, C.c_int64, VP]
+    L.dhip_mlp_targets.argtypes = [P, VP, C.POINTER(VP), C.c_int32, C.c_int64, VP]
     L.dhip_mlp_predict.argtypes = [P, VP, C.c_int64, VP]
     L.dhip_mlp_grad.argtypes = [P, VP, VP, VP, C.c_int64, D, D]
     L.dhip_mlp_train_step.argtypes = [P, VP, VP, VP, C.c_int64, C.c_float, D]
     L.dhip_ch_history.argtypes = [P, C.c_int32, C.POINTER(VP)]
-    L.dhip_ch_solve_learned.argtypes = [P, P, I32, D]
+    L.dhip_ch_residual.argtypes = [P, VP]
+    L.dhip_ch_predict_start.argtypes = [P, P, VP, C.c_int32, I32]
+    L.dhip_ch_solve_learned.argtypes = [P, P, I32, D, I32]
     _lib = L
     return L
 
@@ -234,12 +236,25 @@ class CahnHilliard:
         self.ctx.check(self.L.dhip_ch_history(self.h, level, C.byref(p)))
         return p.value
 
+    def residual(self, r_ptr):
+        """r = b - K x of the prepared step at the current state into device memory r_ptr."""
+        self.ctx.check(self.L.dhip_ch_residual(self.h, C.c_void_p(r_ptr)))
+
+    def predict_start(self, mlp):
+        """(start vector [mu; c] on the host, start kind) for the prepared step."""
+        x0 = np.zeros(2 * self.n)
+        st = C.c_int32()
+        self.ctx.check(self.L.dhip_ch_predict_start(self.h, mlp.h, _vp(x0), HOST, C.byref(st)))
+        return x0, st.value
+
     def solve_learned(self, mlp, raise_on_fail=True):
-        it, t = C.c_int32(), C.c_double()
-        rc = self.L.dhip_ch_solve_learned(self.h, mlp.h, C.byref(it), C.byref(t))
+        """Returns (iterations, residual, converged, start) with start 0 = previous step,
+        1 = network prediction, 2 = plain quadratic extrapolation."""
+        it, t, st = C.c_int32(), C.c_double(), C.c_int32()
+        rc = self.L.dhip_ch_solve_learned(self.h, mlp.h, C.byref(it), C.byref(t), C.byref(st))
         if rc not in (OK, E_NOT_CONVERGED) or (raise_on_fail and rc != OK):
             self.ctx.check(rc)
-        return it.value, t.value, rc == OK
+        return it.value, t.value, rc == OK, st.value
 
     def step(self, nsteps=1):
         its = np.zeros(nsteps, np.int32)
@@ -292,8 +307,9 @@ class Mlp:
         self.ctx.check(self.L.dhip_mlp_features(self.h, hp, len(hist_ptrs), C.c_void_p(rhs_ptr),
                                                 C.c_void_p(pcdiag_ptr), n, C.c_void_p(X_ptr)))
 
-    def targets(self, xnext_ptr, xcur_ptr, n, T_ptr):
-        self.ctx.check(self.L.dhip_mlp_targets(self.h, C.c_void_p(xnext_ptr), C.c_void_p(xcur_ptr), n,
+    def targets(self, xnext_ptr, hist_ptrs, n, T_ptr):
+        hp = (C.c_void_p * 3)(*[C.c_void_p(p) for p in hist_ptrs] + [None] * (3 - len(hist_ptrs)))
+        self.ctx.check(self.L.dhip_mlp_targets(self.h, C.c_void_p(xnext_ptr), hp, len(hist_ptrs), n,
                                                C.c_void_p(T_ptr)))
 
     def predict(self, X_ptr, S, Y_ptr):

@@ -241,14 +241,19 @@ int dhip_ch_step(dhip_ch_t *ch, int32_t nsteps, int32_t *iters, double *resid);
 
 /* ---- learned warm start (no reference counterpart: an addition of this library) ----
  * A per-DOF network 12 -> 64 -> 64 -> 64 -> 2 (tanh hidden layers, bf16 weights and layer inputs,
- * float32 accumulation on the matrix cores) predicts the change of [mu; c] over one step from
- * the last four time levels, the step's right-hand side and the preconditioner diagonal.  GMRES
- * still runs to the same tolerance from the predicted start.  Parameter layout (float32, 9282):
+ * float32 accumulation on the matrix cores) predicts, for the step n -> n+1, a correction of the
+ * linear extrapolation of [mu; c] in units of the step scale s = (RMS of mu_n - mu_{n-1},
+ * RMS of c_n - c_{n-1}):
+ *     x_{n+1} ~ x_n + s * (dx_n/s + y * out_scale + out_mean),  dx_k = x_k - x_{k-1}.
+ * GMRES still runs to the same tolerance.  The start is x_n or the plain extrapolation, whichever
+ * has the smaller residual, replaced by the network's candidate where that has a smaller residual
+ * still and the extrapolation's residual exceeds 100 x tol x |b| (the transient).  Parameter layout (float32, 9282):
  * W1[64][12] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[2][64] b4[2] (weights [out][in]).
  * Features per DOF i (float64, then (f - in_mean) * in_scale, stored float32 [n][12]):
- *   c0, c0-c1, c1-c2, c2-c3, mu0, mu0-mu1, mu1-mu2, mu2-mu3, rhs_i/d_i, rhs_{n+i}/d_{n+i}, d_i, d_{n+i}
- * (index 0 = newest level; missing levels repeat the oldest).  Prediction: change = y * out_scale
- * + out_mean, in the order (mu, c). */
+ *   c_n, dc_n/s_c, dc_{n-1}/s_c, dc_{n-2}/s_c, mu_n, dmu_n/s_mu, dmu_{n-1}/s_mu, dmu_{n-2}/s_mu,
+ *   (r_i/d_i)/s_mu, (r_{n+i}/d_{n+i})/s_c, d_i, d_{n+i}
+ * with r = b - K x_n the residual of the new step's system at the current state and d its
+ * preconditioner diagonal; missing history levels repeat the oldest one. */
 #define DHIP_MLP_NPARAM 9282
 #define DHIP_MLP_NF 12
 typedef struct dhip_mlp dhip_mlp_t;
@@ -258,12 +263,14 @@ void dhip_mlp_destroy(dhip_mlp_t *mlp);
 int dhip_mlp_params(dhip_mlp_t *mlp, const float *theta_in, float *theta_out);
 int dhip_mlp_normalization(dhip_mlp_t *mlp, const double in_mean[12], const double in_scale[12],
                            const double out_mean[2], const double out_scale[2]);
-/* device pointers: hist[0..nhist) newest first, rhs and pcdiag of length 2n, X [n][12] */
-int dhip_mlp_features(dhip_mlp_t *mlp, const double *const hist[4], int32_t nhist, const double *rhs,
+/* device pointers: hist[0..nhist) states newest first (2n), residual and pcdiag (2n), X [n][12] */
+int dhip_mlp_features(dhip_mlp_t *mlp, const double *const hist[4], int32_t nhist, const double *res,
                       const double *pcdiag, int64_t n, float *X);
-/* device pointers: T[n][2] = ((x_next - x_cur) - out_mean) / out_scale */
-int dhip_mlp_targets(dhip_mlp_t *mlp, const double *x_next, const double *x_cur, int64_t n, float *T);
-/* device pointers: Y[S][2] (normalised changes) */
+/* device pointers: training targets T[n][2] = ((x_next - x_n)/s - dx_n/s - out_mean) / out_scale
+ * from hist[0..nhist) = x_n, x_{n-1}(, x_{n-2}) */
+int dhip_mlp_targets(dhip_mlp_t *mlp, const double *x_next, const double *const hist[3], int32_t nhist,
+                     int64_t n, float *T);
+/* device pointers: Y[S][2] (normalised corrections) */
 int dhip_mlp_predict(dhip_mlp_t *mlp, const float *X, int64_t S, float *Y);
 /* minibatch of B samples idx (device) from X, T (device): gradient (host, may be NULL) and loss
  * 1/2 mean |y - t|^2; dhip_mlp_train_step applies one Adam step (beta 0.9, 0.999). */
@@ -273,8 +280,14 @@ int dhip_mlp_train_step(dhip_mlp_t *mlp, const float *X, const float *T, const i
                         float lr, double *loss);
 /* device pointer of the state `level` steps back (0 = current) as the stepper holds them */
 int dhip_ch_history(dhip_ch_t *ch, int32_t level, const double **state);
-/* dhip_ch_solve from the network's predicted start */
-int dhip_ch_solve_learned(dhip_ch_t *ch, dhip_mlp_t *mlp, int32_t *iters, double *resid);
+/* r = b - K x for the prepared step at the current state (device pointer, 2n) */
+int dhip_ch_residual(dhip_ch_t *ch, double *r);
+/* The start dhip_ch_solve_learned would use for the prepared step, into x0 (2n, host or device per
+ * `where`); state and history are not modified.  *start as below. */
+int dhip_ch_predict_start(dhip_ch_t *ch, dhip_mlp_t *mlp, double *x0, int32_t where, int32_t *start);
+/* dhip_ch_solve from the start chosen as above (x_n on the first step after dhip_ch_set_state);
+ * *start (may be NULL) = 0 x_n, 1 network, 2 linear extrapolation */
+int dhip_ch_solve_learned(dhip_ch_t *ch, dhip_mlp_t *mlp, int32_t *iters, double *resid, int32_t *start);
 
 #ifdef __cplusplus
 }

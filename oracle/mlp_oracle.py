@@ -99,24 +99,56 @@ def loss_and_grad(theta, X, T, rnd=bf16):
     return loss, np.concatenate([g[name].ravel() for name, _ in SHAPES])
 
 
-def raw_features(hist, rhs, pcdiag, n):
+def step_scale(h0, h1, n):
+    """RMS over the DOFs of mu_n - mu_{n-1} and c_n - c_{n-1} (1 where it is 0)."""
+    dm, dc = h0[:n] - h1[:n], h0[n:] - h1[n:]
+    sm, sc = np.sqrt(np.sum(dm * dm) / n), np.sqrt(np.sum(dc * dc) / n)
+    return (sm if sm > 0 else 1.0), (sc if sc > 0 else 1.0)
+
+
+def _levels(hist, k):
+    return list(hist) + [hist[-1]] * (k - len(hist))
+
+
+def raw_features(hist, res, pcdiag, n):
     """hist: newest first, up to 4 states [mu; c] (2n each; missing levels repeat the oldest);
-    rhs, pcdiag: the step system's right-hand side and preconditioner diagonal (2n).
-    Returns the 12 unnormalised features per DOF (float64)."""
-    h = list(hist) + [hist[-1]] * (4 - len(hist))
+    res: residual b - K x_n of the new step's system, pcdiag its preconditioner diagonal (2n).
+    Returns the 12 unnormalised features per DOF (float64) and the step scale."""
+    h = _levels(hist, 4)
+    sm, sc = step_scale(h[0], h[1], n)
     c = [v[n:] for v in h]
     mu = [v[:n] for v in h]
     F = np.empty((n, NF))
     F[:, 0] = c[0]
-    F[:, 1] = c[0] - c[1]
-    F[:, 2] = c[1] - c[2]
-    F[:, 3] = c[2] - c[3]
+    F[:, 1] = (c[0] - c[1]) / sc
+    F[:, 2] = (c[1] - c[2]) / sc
+    F[:, 3] = (c[2] - c[3]) / sc
     F[:, 4] = mu[0]
-    F[:, 5] = mu[0] - mu[1]
-    F[:, 6] = mu[1] - mu[2]
-    F[:, 7] = mu[2] - mu[3]
-    F[:, 8] = rhs[:n] / pcdiag[:n]
-    F[:, 9] = rhs[n:] / pcdiag[n:]
+    F[:, 5] = (mu[0] - mu[1]) / sm
+    F[:, 6] = (mu[1] - mu[2]) / sm
+    F[:, 7] = (mu[2] - mu[3]) / sm
+    F[:, 8] = (res[:n] / pcdiag[:n]) / sm
+    F[:, 9] = (res[n:] / pcdiag[n:]) / sc
     F[:, 10] = pcdiag[:n]
     F[:, 11] = pcdiag[n:]
-    return F
+    return F, (sm, sc)
+
+
+def extrapolation(hist, n, scale):
+    """dx_n/s per DOF and component (mu, c), float64: linear extrapolation in scaled units."""
+    h = _levels(hist, 3)
+    out = np.empty((n, NO))
+    for o, (sl, s) in enumerate(((slice(0, n), scale[0]), (slice(n, 2 * n), scale[1]))):
+        out[:, o] = (h[0][sl] - h[1][sl]) / s
+    return out
+
+
+def targets(x_next, hist, n, out_mean, out_scale):
+    h = _levels(hist, 3)
+    scale = step_scale(h[0], h[1], n)
+    ex = extrapolation(h, n, scale)
+    T = np.empty((n, NO))
+    for o, (sl, s) in enumerate(((slice(0, n), scale[0]), (slice(n, 2 * n), scale[1]))):
+        corr = (x_next[sl] - h[0][sl]) / s - ex[:, o]
+        T[:, o] = (corr - out_mean[o]) / out_scale[o]
+    return T

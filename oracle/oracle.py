@@ -72,6 +72,10 @@ def lib(kind="parity"):
     L.orc_gmres_diag.argtypes = [C.c_int64, I64, I32, D, D, D, D, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.orc_tet_mobility.argtypes = [M, D, C.c_int64, D, C.c_int, D]
     L.orc_ch_assemble.argtypes = [M, D, I32, C.c_int64, I64, I32, D, C.c_int, D, D, D]
+    L.orc_ch_system.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_double, I64, I32, D, D, D]
+    L.orc_ch_step.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
     L.orc_lattice_tables.argtypes = [I32, I32]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
     _libs[name] = L
@@ -278,26 +282,48 @@ def block2x2(rowptr, col, A, B, Cb, D):
     return brp, bcol, bval
 
 
+def ch_system(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, dt, eps, rho, sigma=1.0, S=1.0,
+              kind="parity"):
+    """Block system of one step (orc_ch_system): block CSR, rhs and block-Jacobi diagonal."""
+    nnz = col.size
+    brp = np.zeros(2 * ndof + 1, np.int64)
+    bcol = np.zeros(4 * nnz, np.int32)
+    bval, rhs, pcd = np.zeros(4 * nnz), np.zeros(2 * ndof), np.zeros(2 * ndof)
+    f64 = lambda a: _ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+    _check(lib(kind).orc_ch_system(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
+                                   _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
+                                   _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), f64(mass), f64(lb), f64(stab),
+                                   f64(c), dt, eps, rho, sigma, S, _ptr(brp, C.c_int64), _ptr(bcol, C.c_int32),
+                                   _ptr(bval, C.c_double), _ptr(rhs, C.c_double), _ptr(pcd, C.c_double)), "ch_system")
+    return brp, bcol, bval, rhs, pcd
+
+
+def ch_step_c(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, x, dt, eps, rho, sigma=1.0, S=1.0,
+              restart=50, tol=1e-8, max_iter=20000, rel=True, kind="parity"):
+    """One full step in C (orc_ch_step); x = [mu; c] updated in place.  Returns (iters, resid, ok)."""
+    it, t = C.c_int32(max_iter), C.c_double(tol)
+    f64 = lambda a: _ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+    code = lib(kind).orc_ch_step(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
+                                 _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
+                                 _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), f64(mass), f64(lb), f64(stab),
+                                 _ptr(x, C.c_double), dt, eps, rho, sigma, S, restart, C.byref(it), C.byref(t),
+                                 int(rel))
+    if code not in (OK, E_NOT_CONVERGED):
+        _check(code, "ch_step")
+    return it.value, t.value, code == OK
+
+
 def ch_step(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, mu, dt, eps, rho,
             sigma=1.0, S=1.0, restart=50, tol=1e-8, max_iter=20000, kind="parity"):
     """One step of CahnHilliardcGP1CL::DoStep (surfactant/ifacetransp.cpp:1823-1856) with theta = 1,
     resting interface (no wind: Conv = Massd = 0) and no sources, solved by right-preconditioned
     GMRES(restart) with block Jacobi: the diagonals of Precond3 = M + sigma dt LM + dt rho C and
-    Precond4 = M + eps^2 L + eps^2 rho C (:1567-1570).  Unknowns [mu; c], start = previous step."""
+    Precond4 = M + eps^2 L + eps^2 rho C (:1567-1570).  Unknowns [mu; c], start = previous step.
+    The system comes from orc_ch_system; the mobility matrix and LocalMatVec vectors are returned
+    too (orc_ch_assemble)."""
     lm, timeder, well = ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, kind=kind)
-    A = sigma * dt * lm + rho * dt * stab
-    B = 1.0 * (1.0 * mass + (0.0 * dt) * np.zeros_like(mass)) + dt * np.zeros_like(mass)
-    Cb = 0.0 * lb + (-eps) * mass
-    D = 1.0 * (eps * eps * lb + S * mass) + (rho * eps * eps) * stab
-    P3 = 1.0 * (1.0 * mass + (sigma * dt) * lm) + (dt * rho) * stab
-    P4 = 1.0 * (1.0 * mass + (eps * eps) * lb) + (eps * eps * rho) * stab
-    rhs3 = 1.0 * (timeder + dt * np.zeros(ndof))
-    rhs4 = ((-1.0) * well + S * timeder) + np.zeros(ndof)
-    rp = np.asarray(rowptr, np.int64)
-    diag = lambda v: np.array([v[rp[i] + np.flatnonzero(col[rp[i]:rp[i + 1]] == i)[0]] for i in range(ndof)])
-    pcd = np.concatenate([diag(P3), diag(P4)])
-    brp, bcol, bval = block2x2(rp, col, A, B, Cb, D)
-    b = np.concatenate([rhs3, rhs4])
+    brp, bcol, bval, b, pcd = ch_system(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, dt, eps,
+                                        rho, sigma, S, kind=kind)
     x0 = np.concatenate([mu, c])
     x, it, res, ok = gmres_diag(brp, bcol, bval, pcd, b, x0, restart=restart, max_iter=max_iter, tol=tol)
     return dict(mu=x[:ndof], c=x[ndof:], iters=it, res=res, ok=ok, lm=lm, timeder=timeder, well=well,

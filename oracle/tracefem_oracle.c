@@ -1089,6 +1089,76 @@ int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
     return err;
 }
 
+/* The step system of CahnHilliardcGP1CL::DoStep (surfactant/ifacetransp.cpp:1823-1856), theta = 1,
+ * no wind, no sources: block CSR (SparseMatBaseCL(A, B, C, D), num/spmat.h:719-751), right-hand
+ * side [M c; -M f'(c) + S M c] and the block-Jacobi diagonal [diag Precond3; diag Precond4]
+ * (:1567-1570), each LinComb evaluated in the reference's order. */
+int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                  const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                  const double *stab, const double *c, double dt, double eps, double rho, double sigma, double S,
+                  int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
+{
+    const int64_t nnz = rowptr[ndof];
+    double *lm = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
+    double *td = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
+    double *wl = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
+    if (!lm || !td || !wl) { free(lm); free(td); free(wl); return ORC_E_NOMEM; }
+    int err = orc_ch_assemble(m, phi, dof_of_vertex, ndof, rowptr, colind, c, ORC_ZERO_THROW, lm, td, wl);
+    if (!err) {
+        const double sdt = sigma * dt, rdt = rho * dt, dtr = dt * rho, e2 = eps * eps, re2 = rho * eps * eps,
+                     e2r = eps * eps * rho, meps = -eps;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < ndof; ++i) {
+            const int64_t e0 = rowptr[i], len = rowptr[i + 1] - e0;
+            const int64_t top = 2 * e0, bot = 2 * nnz + 2 * e0;
+            brp[i] = top;
+            brp[ndof + i] = bot;
+            for (int64_t k = 0; k < len; ++k) {
+                const int64_t e = e0 + k;
+                const int32_t cj = colind[e];
+                bcol[top + k] = cj;
+                bval[top + k] = sdt * lm[e] + rdt * stab[e];
+                bcol[top + len + k] = (int32_t)(cj + ndof);
+                bval[top + len + k] = 1.0 * (1.0 * mass[e] + 0.0 * 0.0) + dt * 0.0;
+                bcol[bot + k] = cj;
+                bval[bot + k] = 0.0 * lb[e] + meps * mass[e];
+                bcol[bot + len + k] = (int32_t)(cj + ndof);
+                bval[bot + len + k] = 1.0 * (e2 * lb[e] + S * mass[e]) + re2 * stab[e];
+                if (cj == i) {
+                    pcdiag[i] = 1.0 * (1.0 * mass[e] + sdt * lm[e]) + dtr * stab[e];
+                    pcdiag[ndof + i] = 1.0 * (1.0 * mass[e] + e2 * lb[e]) + e2r * stab[e];
+                }
+            }
+            rhs[i] = 1.0 * (td[i] + dt * 0.0);
+            rhs[ndof + i] = ((-1.0) * wl[i] + S * td[i]) + 0.0;
+        }
+        brp[2 * ndof] = 4 * nnz;
+    }
+    free(lm); free(td); free(wl);
+    return err;
+}
+
+/* one step: system + right-preconditioned GMRES(restart) from x = [mu; c] (in/out) */
+int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                int32_t restart, int32_t *max_iter, double *tol, int32_t rel)
+{
+    const int64_t nnz = rowptr[ndof], n2 = 2 * ndof;
+    int64_t *brp = (int64_t *)malloc(sizeof(int64_t) * (n2 + 1));
+    int32_t *bcol = (int32_t *)malloc(sizeof(int32_t) * (4 * nnz + 1));
+    double *bval = (double *)malloc(sizeof(double) * (4 * nnz + 1));
+    double *rhs = (double *)malloc(sizeof(double) * (n2 + 1));
+    double *pcd = (double *)malloc(sizeof(double) * (n2 + 1));
+    int err = (!brp || !bcol || !bval || !rhs || !pcd) ? ORC_E_NOMEM : ORC_OK;
+    if (!err)
+        err = orc_ch_system(m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, x + ndof, dt, eps, rho,
+                            sigma, S, brp, bcol, bval, rhs, pcd);
+    if (!err) err = orc_gmres_diag(n2, brp, bcol, bval, pcd, rhs, x, restart, max_iter, tol, rel, 0);
+    free(brp); free(bcol); free(bval); free(rhs); free(pcd);
+    return err;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Sparse linear algebra                                                     */
 /* ------------------------------------------------------------------------ */

@@ -141,6 +141,16 @@ int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
                     const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
                     double *lm, double *timeder, double *well);
 
+/* the block system of one step (block CSR, rhs, block-Jacobi diagonal) and one full step */
+int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                  const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                  const double *stab, const double *c, double dt, double eps, double rho, double sigma, double S,
+                  int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag);
+int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                int32_t restart, int32_t *max_iter, double *tol, int32_t rel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,24 +17,35 @@ def upload(ctx, a):
     return ctx.to_device(np.ascontiguousarray(a))
 
 
-def test_features_bit_exact(ctx):
+def test_features_match_oracle(ctx):
+    """Features within float32 rounding of the oracle's (the device's step scale sums its squares
+    in block order, so the scale may differ from the oracle's in the last bits); targets likewise."""
     print("[gpu mlp] features", flush=True)
     rng = np.random.default_rng(0)
     n = 777
     hist = [rng.normal(size=2 * n) for _ in range(3)]
-    rhs = rng.normal(size=2 * n)
+    xn = rng.normal(size=2 * n)
+    res = rng.normal(size=2 * n)
     pcd = rng.uniform(0.5, 2.0, 2 * n)
     mean, scale = rng.normal(size=12), rng.uniform(0.5, 3.0, 12)
+    omean, oscale = rng.normal(size=2), rng.uniform(0.5, 2.0, 2)
     mlp = ctx.mlp()
-    mlp.set_normalization(mean, scale, [0.0, 0.0], [1.0, 1.0])
+    mlp.set_normalization(mean, scale, omean, oscale)
     hp = [upload(ctx, h) for h in hist]
-    rp, pp = upload(ctx, rhs), upload(ctx, pcd)
+    rp, pp, xp = upload(ctx, res), upload(ctx, pcd), upload(ctx, xn)
     Xd = ctx.malloc(4 * 12 * n)
-    mlp.features(hp, rp, pp, n, Xd)
-    X = ctx.to_host(Xd, (n, 12), np.float32)
-    ref = ((M.raw_features(hist, rhs, pcd, n) - mean) * scale).astype(np.float32)
-    assert np.array_equal(X, ref)
-    for p in hp + [rp, pp, Xd]:
+    Td = ctx.malloc(4 * 2 * n)
+    for nh in (1, 2, 3):
+        mlp.features(hp[:nh], rp, pp, n, Xd)
+        X = ctx.to_host(Xd, (n, 12), np.float32)
+        F, _ = M.raw_features(hist[:nh], res, pcd, n)
+        ref = ((F - mean) * scale).astype(np.float32)
+        assert np.all(np.abs(X - ref) <= 2.0 ** -21 * np.maximum(np.abs(ref), 1.0)), nh
+        mlp.targets(xp, hp[:nh], n, Td)
+        T = ctx.to_host(Td, (n, 2), np.float32)
+        Tr = M.targets(xn, hist[:nh], n, omean, oscale).astype(np.float32)
+        assert np.all(np.abs(T - Tr) <= 1e-5 * np.maximum(np.abs(Tr), 1.0)), nh
+    for p in hp + [rp, pp, xp, Xd, Td]:
         ctx.free(p)
     mlp.close()
 
@@ -87,8 +98,9 @@ def test_training_lowers_loss(ctx):
         pass
     data = Data()
     data.X, data.T, data.S = Xd, Td, S
-    hist = learn.train(ctx, mlp, data, iters=300, batch=4096, lr=3e-3, lr_end=3e-4, log_every=50)
-    assert hist[-1][1] < 0.1 * hist[0][1], hist
+    hist = learn.train(ctx, mlp, data, iters=400, batch=4096, lr=3e-3, lr_end=3e-4, log_every=50)
+    assert hist[-1][1] < 0.25 * hist[0][1], hist
+    assert all(b[1] < a[1] * 1.05 for a, b in zip(hist, hist[1:])), hist
     # the device parameters are the ones the oracle evaluates to the same loss
     th = mlp.params().astype(np.float64)
     lo, _ = M.loss_and_grad(th, X.astype(np.float32).astype(np.float64), T.astype(np.float64))
@@ -116,11 +128,14 @@ def test_learned_start_cuts_iterations(ctx):
     learn.train(ctx, mlp, data, iters=600, batch=8192, lr=2e-3, lr_end=1e-4)
     res = learn.evaluate(ch, mlp, seed=100, steps=steps)
     prev, lrn = res["previous"]["iters"], res["learned"]["iters"]
-    print(f"[gpu mlp] iterations previous-step {prev.sum()} learned {lrn.sum()}", flush=True)
-    assert lrn.sum() < 0.9 * prev.sum()
-    # same discrete solutions either way (both solved to 1e-8)
+    print(f"[gpu mlp] iterations previous-step {prev.sum()} learned {lrn.sum()} "
+          f"(starts {res['learned']['starts']})", flush=True)
+    assert lrn.sum() < 0.7 * prev.sum()
+    assert res["learned"]["starts"]["network"] > 0
+    # the same discrete trajectory either way: every step solved to 1e-8, 40 steps accumulate
+    # differences at the level of the tolerance times the conditioning
     cp, cl = res["previous"]["state"][0], res["learned"]["state"][0]
-    assert np.linalg.norm(cp - cl) <= 1e-6 * np.linalg.norm(cp)
+    assert np.linalg.norm(cp - cl) <= 1e-5 * np.linalg.norm(cp)
     data.free()
     for r in recs:
         r.free()

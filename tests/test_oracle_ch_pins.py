@@ -118,3 +118,31 @@ def test_ch_step_direct_solve_and_conservation():
         assert abs(np.ones(nd) @ (M @ c) - total0) <= 1e-8 * abs(total0)
     # three short steps keep the concentration inside the two phases' range
     assert 0.0 < c.min() and c.max() < 1.0
+
+
+def test_c_system_matches_numpy_composition():
+    """orc_ch_system against the blocks composed here from the assembled matrices and the
+    mobility matrix: LinComb orders as in CahnHilliardcGP1CL::DoStep/Update, bit for bit."""
+    m, phi, om, dof, nd, r = setup_torus(10)
+    c = 0.5 + np.random.default_rng(5).uniform(-0.05, 0.05, nd)
+    dt, eps, rho, sigma, S = 1e-4, 0.05, m.h, 1.0, 1.0
+    lm, td, wl = O.ch_assemble(om, phi, dof, nd, r["rowptr"], r["colind"], c)
+    mass, lb, stab = r["mass"], r["lb"], r["stab"]
+    A = (sigma * dt) * lm + (rho * dt) * stab
+    B = 1.0 * (1.0 * mass + 0.0 * 0.0) + dt * 0.0
+    Cb = 0.0 * lb + (-eps) * mass
+    Dm = 1.0 * ((eps * eps) * lb + S * mass) + (rho * eps * eps) * stab
+    brp, bcol, bval = O.block2x2(r["rowptr"], r["colind"], A, B, Cb, Dm)
+    b2, c2, v2, rhs, pcd = O.ch_system(om, phi, dof, nd, r["rowptr"], r["colind"], mass, lb, stab, c, dt, eps, rho)
+    assert np.array_equal(brp, b2) and np.array_equal(bcol, c2) and np.array_equal(bval, v2)
+    assert np.array_equal(rhs, np.concatenate([1.0 * (td + dt * 0.0), ((-1.0) * wl + S * td) + 0.0]))
+    rp = r["rowptr"]
+    di = np.array([rp[i] + np.flatnonzero(r["colind"][rp[i]:rp[i + 1]] == i)[0] for i in range(nd)])
+    P3 = 1.0 * (1.0 * mass + (sigma * dt) * lm) + (dt * rho) * stab
+    P4 = 1.0 * (1.0 * mass + (eps * eps) * lb) + (eps * eps * rho) * stab
+    assert np.array_equal(pcd, np.concatenate([P3[di], P4[di]]))
+    # the full C step equals system + GMRES
+    x = np.concatenate([np.zeros(nd), c])
+    it, res, ok = O.ch_step_c(om, phi, dof, nd, r["rowptr"], r["colind"], mass, lb, stab, x, dt, eps, rho)
+    xo, ito, reso, oko = O.gmres_diag(b2, c2, v2, pcd, rhs, np.concatenate([np.zeros(nd), c]), tol=1e-8)
+    assert ok and oko and it == ito and np.array_equal(x, xo)

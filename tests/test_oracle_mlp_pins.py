@@ -2,6 +2,7 @@
 differences of the exact (unrounded) network, the bfloat16 rounding against its definition,
 and the features against their closed form."""
 import numpy as np
+import pytest
 
 import mlp_oracle as M
 
@@ -48,10 +49,30 @@ def test_features_closed_form():
     n = 5
     rng = np.random.default_rng(3)
     hist = [rng.normal(size=2 * n) for _ in range(2)]
-    rhs = rng.normal(size=2 * n)
+    res = rng.normal(size=2 * n)
     d = rng.uniform(1, 2, 2 * n)
-    F = M.raw_features(hist, rhs, d, n)
+    F, (sm, sc) = M.raw_features(hist, res, d, n)
     c0, c1 = hist[0][n:], hist[1][n:]
-    assert np.array_equal(F[:, 0], c0) and np.array_equal(F[:, 1], c0 - c1)
+    assert sc == np.sqrt(np.mean((c0 - c1) ** 2)) or abs(sc / np.sqrt(np.mean((c0 - c1) ** 2)) - 1) < 1e-15
+    assert np.array_equal(F[:, 0], c0) and np.allclose(F[:, 1], (c0 - c1) / sc, rtol=0, atol=0)
+    assert np.sqrt(np.mean(F[:, 1] ** 2)) == pytest.approx(1.0, rel=1e-14)   # unit step scale
     assert not F[:, 2].any() and not F[:, 3].any()   # missing levels repeat the oldest
-    assert np.array_equal(F[:, 8], rhs[:n] / d[:n]) and np.array_equal(F[:, 11], d[n:])
+    assert np.array_equal(F[:, 9], (res[n:] / d[n:]) / sc) and np.array_equal(F[:, 11], d[n:])
+
+
+def test_targets_invert_extrapolation():
+    """A trajectory that is exactly linear in time has zero correction targets; the target
+    of a general next state reproduces it through the extrapolation formula."""
+    n = 7
+    rng = np.random.default_rng(4)
+    a, b = (rng.normal(size=2 * n) for _ in range(2))
+    x = [a + b * t for t in range(4)]        # x_0 .. x_3
+    hist = [x[2], x[1], x[0]]
+    T = M.targets(x[3], hist, n, np.zeros(2), np.ones(2))
+    assert np.abs(T).max() < 1e-10
+    xn = rng.normal(size=2 * n)
+    T = M.targets(xn, hist, n, np.zeros(2), np.ones(2))
+    scale = M.step_scale(hist[0], hist[1], n)
+    ex = M.extrapolation(hist, n, scale)
+    back = np.concatenate([hist[0][:n] + scale[0] * (ex[:, 0] + T[:, 0]), hist[0][n:] + scale[1] * (ex[:, 1] + T[:, 1])])
+    assert np.allclose(back, xn, rtol=1e-12, atol=1e-12)
