@@ -1105,7 +1105,12 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     double *outd = reinterpret_cast<double *>(outs + 4);
     GmresArgs ga{n, A->rowptr, A->col, A->val, diag, db, dx, c->kv[15].as<double>(),
                  c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
-                 outs, outd, outs + 1};
+                 outs, outd, outs + 1, nullptr};
+    if (c->probe_on) {
+        DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
+        ga.probe = c->probe.as<uint64_t>();
+        c->probe_blocks = nb;
+    }
     const bool regv = rpt == 1 && m <= KREG;
     const void *ks[5] = {regv ? (const void *)k_gmres<1, KREG> : (const void *)k_gmres<1, 0>,
                          (const void *)k_gmres<2, 0>, (const void *)k_gmres<4, 0>, (const void *)k_gmres<8, 0>,
@@ -1207,7 +1212,7 @@ struct dhip_ch {
     // snapshot of the assembly the system was built from
     DevBuf phi, cut, cellinfo, flags, dofs, rowowner, massloc, rowptr, col, mass, lb, stab;
     // step system and state [mu; c]
-    DevBuf tet, brp, bcol, bval, pcdiag, rhs, x;
+    DevBuf geom, tet, brp, bcol, bval, pcdiag, rhs, x;
     // previous time levels for the learned start: hist[k] = state k+1 levels back (k < nhist)
     DevBuf hist[3];
     int nhist = 0;
@@ -1224,7 +1229,7 @@ static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
 
 static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
-                      &h->col, &h->mass, &h->lb, &h->stab, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
+                      &h->col, &h->mass, &h->lb, &h->stab, &h->geom, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
                       &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred, &h->xstart};
     for (DevBuf *b : bufs) b->release();
 }
@@ -1232,9 +1237,9 @@ static void ch_release(dhip_ch *h) {
 static int ch_prepare_impl(dhip_ch *h) {
     dhip_ctx *c = h->ctx;
     const int64_t N = h->ncut, R = h->ndof;
-    ChTetArgs ta{h->mesh, h->phi.as<double>(), h->cut.as<int64_t>(), N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(),
-                 h->massloc.as<double>(), h->x.as<double>() + R, h->tet.as<double>()};
-    if (N > 0) DHIP_LAUNCH(c, "ch_tet", k_ch_tet, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, ta);
+    ChTetArgs ta{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->massloc.as<double>(), h->geom.as<double>(),
+                 h->x.as<double>() + R, h->tet.as<double>()};
+    if (N > 0) DHIP_LAUNCH(c, "ch_tet", k_ch_tet, blocks_for(N, 256), 256, 0, ta);
     ChRowArgs ra{h->mesh, h->cut.as<int64_t>(), h->cellinfo.as<uint32_t>(), h->flags.as<uint8_t>(),
                  h->dofs.as<int32_t>(), h->rowowner.as<int32_t>(), R, N, h->tet.as<double>(),
                  h->rowptr.as<int32_t>(), h->mass.as<double>(), h->stab.as<double>(), h->bval.as<double>(),
@@ -1321,7 +1326,8 @@ int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p
         (rc = ch_copy(c, h->lb, c->val[1].p, sizeof(double) * Z)) ||
         (rc = ch_copy(c, h->stab, c->val[2].p, sizeof(double) * Z)))
         return fail(rc);
-    if (h->tet.ensure(sizeof(double) * CHT_N * (N + 1)) || h->brp.ensure(sizeof(int32_t) * (2 * R + 1)) ||
+    if (h->geom.ensure(sizeof(double) * CHG_N * (N + 1)) || h->tet.ensure(sizeof(double) * CHT_N * (N + 1)) ||
+        h->brp.ensure(sizeof(int32_t) * (2 * R + 1)) ||
         h->bcol.ensure(sizeof(int32_t) * (4 * Z + 1)) || h->bval.ensure(sizeof(double) * (4 * Z + 1)) ||
         h->pcdiag.ensure(sizeof(double) * (2 * R + 1)) || h->rhs.ensure(sizeof(double) * (2 * R + 1)) ||
         h->x.ensure(sizeof(double) * (2 * R + 1)))
@@ -1335,6 +1341,13 @@ int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p
         ProfScope ps(c, "ch_block");
         hipLaunchKernelGGL(k_ch_block, dim3(blocks_for(R + 1, 128)), dim3(128), 0, c->stream, ba);
         if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_block launch failed"));
+    }
+    if (N > 0) {
+        ChGeomArgs ga{h->mesh, h->phi.as<double>(), h->cut.as<int64_t>(), N, h->flags.as<uint8_t>(),
+                      h->geom.as<double>()};
+        ProfScope ps(c, "ch_geom");
+        hipLaunchKernelGGL(k_ch_geom, dim3(blocks_for(N, LOCAL_BLOCK)), dim3(LOCAL_BLOCK), 0, c->stream, ga);
+        if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_geom launch failed"));
     }
     h->K.ctx = c;
     h->K.n = 2 * R;
@@ -1486,8 +1499,8 @@ static int mlp_predict_impl(dhip_mlp *q, const float *X, int64_t S, float *Y) {
     int rc = mlp_pack(q);
     if (rc) return rc;
     if (S > 0)
-        DHIP_LAUNCH(c, "mlp_infer", k_mlp_infer, blocks_for(S, 16 * INFER_WAVES), 64 * INFER_WAVES, 0, X, S,
-                    q->packed.as<uint16_t>(), q->theta.as<float>(), Y);
+        DHIP_LAUNCH(c, "mlp_infer", k_mlp_infer, std::min<unsigned>(blocks_for(S, 16 * INFER_WAVES), 2 * c->cu_count),
+                    64 * INFER_WAVES, 0, X, S, q->packed.as<uint16_t>(), q->theta.as<float>(), Y);
     return DHIP_OK;
 }
 

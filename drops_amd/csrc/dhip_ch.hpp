@@ -10,9 +10,11 @@
 // M c and M f'(c) are LocalMatVecP1CL accumulations (:800-828).  The block matrix has the
 // layout of the SparseMatBaseCL(A, B, C, D) constructor (num/spmat.h:719-751).
 //
-//   k_ch_tet    one lane per cut tet: patch, Quad5 composite rule, mobility interpolant and
-//               projected gradients at every quadrature point (reference order), the 4x4
-//               local mobility matrix and the local products M c, M f'(c).  SoA output.
+//   k_ch_geom   once per system, one lane per cut tet: patch, Quad5 composite rule, projected
+//               gradients at every quadrature point -> the 40 geometric coefficients of the
+//               mobility matrix (linear in the vertex mobilities).
+//   k_ch_tet    per step, one lane per cut tet: the 4x4 local mobility matrix from the vertex
+//               concentrations and the local products M c, M f'(c).  SoA output.
 //   k_ch_rows   one lane per DOF row: sums the local rows of the incident cut tets in
 //               ascending tet id (the order of the reference's builder +=), writes the A_
 //               block, the Precond3 diagonal and both right-hand sides.
@@ -21,6 +23,7 @@
 #pragma once
 
 #include "dhip_assembly.hpp"
+#include "dhip_krylov.hpp"
 
 namespace dhip {
 
@@ -35,21 +38,101 @@ __device__ __forceinline__ double potential_prime(double x) { return (x - 1.) * 
 
 constexpr int CHT_N = 24;  // 16 mobility, 4 M c, 4 M f'(c)
 constexpr int CHT_MOB = 0, CHT_TD = 16, CHT_WL = 20;
+constexpr int CHG_N = 40;  // T[k][ij]: 4 vertices x 10 lower-triangular (i, j) pairs
 
-struct ChTetArgs {
+struct ChGeomArgs {
     MeshP m;
     const double *phi;
     const int64_t *cut;
     int64_t ncut;
     const uint8_t *flags;
+    double *T;               // SoA [CHG_N][ncut]
+};
+
+// Geometry of LocalLaplaceMobilityP1CL (surfactant/ifacetransp.h:1731-1766), computed once: the
+// mobility enters through its P1 interpolant sum_k Mob(c_k) lambda_k, so
+//   LM_ij = sum_k Mob(c_k) T_k,ij,   T_k,ij = sum_q w_q lambda_k(x_q) (P grad phi_i . P grad phi_j)(x_q)
+// over the Quad5 composite rule of the patch, P = I - n n^T with n the normalised gradient of the
+// P2 level set at x_q.  Per step only the contraction with the four vertex mobilities remains.
+__global__ void __launch_bounds__(LOCAL_BLOCK) k_ch_geom(ChGeomArgs a) {
+    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+    if (c >= a.ncut) return;
+    const int64_t N = a.ncut;
+    double acc[CHG_N];
+#pragma unroll
+    for (int o = 0; o < CHG_N; ++o) acc[o] = 0.;
+    if (a.flags[c] & 1) {
+        double X[4][3], ls[10];
+        load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+        double G1[4][3];
+        p1_grad(X, G1);
+        double v[10];
+#pragma unroll
+        for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+        for (int lt = 0; lt < 8; ++lt) {
+            int sg[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sg[q] = sgn(ls[c_tab.lat_t[lt][q]]);
+            const int pid = sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40;
+            const int ntri = c_tab.rp_size[pid];
+            const bool bnd = c_tab.rp_bnd[pid] != 0;
+            for (int tr = 0; tr < ntri; ++tr) {
+                double vb[3][4];
+                facet_vertices(ls, lt, pid, tr, vb);
+                double absdet, nrm[3];
+                facet_qr(X, vb, absdet, nrm);
+                double af = bnd ? 0.5 : 1.;
+                af *= absdet;
+                for (int qn = 0; qn < 7; ++qn) {
+                    double b[4];
+#pragma unroll
+                    for (int cc = 0; cc < 4; ++cc) {
+                        double sv = vb[0][cc] * c_tab.q5n[qn][0];
+                        sv += vb[1][cc] * c_tab.q5n[qn][1];
+                        sv += vb[2][cc] * c_tab.q5n[qn][2];
+                        b[cc] = sv;
+                    }
+                    const double wq = af * c_tab.q5w[qn];
+                    double n3[3];
+                    p2_unit_normal(v, b, G1, n3);
+                    double q[4][3];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double d = dot3(G1[i], n3);
+#pragma unroll
+                        for (int cc = 0; cc < 3; ++cc) q[i][cc] = G1[i][cc] - d * n3[cc];
+                    }
+                    int o = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j <= i; ++j, ++o) {
+                            const double g = wq * dot3(q[i], q[j]);
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) acc[k * 10 + o] += b[k] * g;
+                        }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = 0; o < CHG_N; ++o) a.T[(int64_t)o * N + c] = acc[o];
+}
+
+struct ChTetArgs {
+    int64_t ncut;
+    const uint8_t *flags;
     const int32_t *dofs;     // [ncut][4], -1 where the vertex has no DOF
     const double *loc;       // local mass (alpha = 1) from k_local_p1
+    const double *T;         // k_ch_geom
     const double *conc;      // c on the DOFs
     double *out;             // SoA [CHT_N][ncut]
 };
 
-__global__ void __launch_bounds__(LOCAL_BLOCK) k_ch_tet(ChTetArgs a) {
-    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+// Per step: the local mobility matrix from the vertex concentrations and the local products
+// M c, M f'(c) (LocalMatVecP1CL<LocalInterfaceMassP1CL>, surfactant/ifacetransp.h:800-828).
+__global__ void k_ch_tet(ChTetArgs a) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.ncut) return;
     const int64_t N = a.ncut;
     double *O = a.out;
@@ -65,7 +148,6 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_ch_tet(ChTetArgs a) {
         xc[i] = nb[i] >= 0 ? a.conc[nb[i]] : 0.;
         mv[i] = mobility_fun(xc[i]);
     }
-    // LocalMatVecP1CL<LocalInterfaceMassP1CL> for c and f'(c)
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         double vt = 0., vw = 0.;
@@ -81,66 +163,16 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_ch_tet(ChTetArgs a) {
         O[(CHT_TD + i) * N + c] = vt;
         O[(CHT_WL + i) * N + c] = vw;
     }
-    // LocalLaplaceMobilityP1CL on the Quad5 composite rule
-    double X[4][3], ls[10];
-    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
-    double G1[4][3];
-    p1_grad(X, G1);
-    double v[10];
-#pragma unroll
-    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
-    double acc[10];
-#pragma unroll
-    for (int o = 0; o < 10; ++o) acc[o] = 0.;
-    for (int lt = 0; lt < 8; ++lt) {
-        int s[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] = sgn(ls[c_tab.lat_t[lt][q]]);
-        const int pid = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;
-        const int ntri = c_tab.rp_size[pid];
-        const bool bnd = c_tab.rp_bnd[pid] != 0;
-        for (int tr = 0; tr < ntri; ++tr) {
-            double vb[3][4];
-            facet_vertices(ls, lt, pid, tr, vb);
-            double absdet, nrm[3];
-            facet_qr(X, vb, absdet, nrm);
-            double af = bnd ? 0.5 : 1.;
-            af *= absdet;
-            for (int qn = 0; qn < 7; ++qn) {
-                double b[4];
-#pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    double sv = vb[0][cc] * c_tab.q5n[qn][0];
-                    sv += vb[1][cc] * c_tab.q5n[qn][1];
-                    sv += vb[2][cc] * c_tab.q5n[qn][2];
-                    b[cc] = sv;
-                }
-                const double wq = af * c_tab.q5w[qn];
-                double n3[3];
-                p2_unit_normal(v, b, G1, n3);
-                double q[4][3];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const double d = dot3(G1[i], n3);
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) q[i][cc] = G1[i][cc] - d * n3[cc];
-                }
-                const double qm = b[0] * mv[0] + b[1] * mv[1] + b[2] * mv[2] + b[3] * mv[3];
-                int o = 0;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j <= i; ++j) acc[o++] += wq * (qm * dot3(q[i], q[j]));
-            }
-        }
-    }
     int o = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j <= i; ++j, ++o) {
-            O[(CHT_MOB + i * 4 + j) * N + c] = acc[o];
-            O[(CHT_MOB + j * 4 + i) * N + c] = acc[o];
+            double lm = 0.;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lm += mv[k] * a.T[(int64_t)(k * 10 + o) * N + c];
+            O[(CHT_MOB + i * 4 + j) * N + c] = lm;
+            O[(CHT_MOB + j * 4 + i) * N + c] = lm;
         }
 }
 
@@ -264,14 +296,12 @@ __global__ void k_ch_block(ChBlockArgs a) {
     }
 }
 
-// r = b - K x (one lane per row of the block system)
+// r = b - K x (one lane per row of the block system, row gathers batched)
 __global__ void k_ch_residual(int64_t n2, const int32_t *rp, const int32_t *col, const double *val, const double *b,
                               const double *x, double *r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n2) return;
-    double s = 0.0;
-    for (int32_t e = rp[i]; e < rp[i + 1]; ++e) s += val[e] * x[col[e]];
-    r[i] = b[i] - s;
+    r[i] = b[i] - row_product<double>(rp, col, val, 0, i, [&](int32_t c) { return x[c]; });
 }
 
 // |b - K x_k|^2 of three candidate starts and |b|^2 per block (block-order sums later)
@@ -282,12 +312,31 @@ __global__ void __launch_bounds__(RES3_B) k_ch_three_resid(int64_t n2, const int
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, sb = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * RES3_B + threadIdx.x; i < n2; i += (int64_t)gridDim.x * RES3_B) {
         double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-        for (int32_t e = rp[i]; e < rp[i + 1]; ++e) {
-            const double v = val[e];
-            const int32_t cj = col[e];
-            p0 += v * x0[cj];
-            p1 += v * x1[cj];
-            p2 += v * x2[cj];
+        const int32_t e0 = rp[i], e1 = rp[i + 1];
+        for (int32_t q0 = e0; q0 < e1; q0 += 8) {
+            const int len = e1 - q0 < 8 ? e1 - q0 : 8;
+            int32_t cc[8];
+            double vv[8], a0[8], a1[8], a2[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < len) {
+                    cc[j] = col[q0 + j];
+                    vv[j] = val[q0 + j];
+                }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < len) {
+                    a0[j] = x0[cc[j]];
+                    a1[j] = x1[cc[j]];
+                    a2[j] = x2[cc[j]];
+                }
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                if (j < len) {
+                    p0 += vv[j] * a0[j];
+                    p1 += vv[j] * a1[j];
+                    p2 += vv[j] * a2[j];
+                }
         }
         const double r0 = b[i] - p0, r1 = b[i] - p1, r2 = b[i] - p2;
         s0 += r0 * r0;
@@ -307,15 +356,23 @@ __global__ void __launch_bounds__(RES3_B) k_ch_three_resid(int64_t n2, const int
     }
 }
 
+// sum of the `stride`-strided partials k of nb blocks by one wave (lane-strided loads, then the
+// wave's butterfly sum: the same order in every call)
+__device__ inline double wave_sum_parts(const double *part, int nb, int stride, int k) {
+    double v = 0.0;
+    for (int b = threadIdx.x & 63; b < nb; b += 64) v += part[stride * b + k];
+    return wave_sum(v);
+}
+
 // Candidates 0 = x_n, 1 = network, 2 = linear extrapolation.  The smaller residual of x_n and the
 // extrapolation wins; the network's candidate replaces it when its residual is smaller still and
 // the extrapolation is far from converged (residual above gate * |b|, the transient), where the
 // residual is dominated by the change of the step and not by the solver's tolerance-level error.
 __global__ void k_ch_pick(const double *part, int nb, double gate, int32_t *flag) {
+    double s[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = wave_sum_parts(part, nb, 4, q);
     if (threadIdx.x != 0) return;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < nb; ++k)
-        for (int q = 0; q < 4; ++q) s[q] += part[4 * k + q];
     int best = s[2] < s[0] ? 2 : 0;
     if (s[1] < s[best] && s[2] > gate * gate * s[3]) best = 1;
     *flag = best;

@@ -746,6 +746,7 @@ struct GmresArgs {
     int *out_iter;
     double *out_resid;
     int *out_status;
+    uint64_t *probe;            // optional [gridDim.x][4] wall-clock ticks: setup, compute, sync, total
 };
 
 constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG, one row per thread)
@@ -753,6 +754,8 @@ constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG,
 template <int RPT, int KR>
 __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
+    const uint64_t t_start = a.probe ? wall_clock64() : 0;
+    uint64_t t_sync = 0;
     __shared__ double Hc[KMAX + 1];          // current Hessenberg column
     __shared__ double Hs[KMAX * KMAX / 2 + KMAX];  // rotated upper triangle, packed by columns
     __shared__ double cs[KMAX], sn[KMAX], sv[KMAX + 1], yv[KMAX];
@@ -776,6 +779,12 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     auto col_of = [](int k) { return k * (k + 1) / 2; };  // start of packed column k (rows 0..k)
     const int m = a.m;
     uint32_t gen = 0;
+    auto syncv = [&](int K, double *out) {
+        const uint64_t t0 = a.probe ? wall_clock64() : 0;
+        const bool ok = sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, K, out);
+        if (a.probe) t_sync += wall_clock64() - t0;
+        return ok;
+    };
     int status = 1, j = 1;
     double resid = 0.0, normb = 1.0, beta = 0.0;
     bool own[RPT];
@@ -930,7 +939,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
             msh[0] = p_rr;
             msh[1] = p_bb;
         }
-        if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 2, red_sh)) { status = 2; goto done; }
+        if (!syncv(2, red_sh)) { status = 2; goto done; }
         beta = sqrt(red_sh[0]);
         normb = sqrt(red_sh[1]);
         if (normb == 0.0 || !a.rel) normb = 1.0;
@@ -952,7 +961,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
             nvec = 1;
             if (threadIdx.x < KMAX + 1) sv[threadIdx.x] = 0.0;
             if (threadIdx.x == 0) sv[0] = beta;
-            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+            if (!syncv(0, red_sh)) { status = 2; goto done; }
         }
         int i;
         bool converged = false;
@@ -967,7 +976,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 wr[k] = w;
             }
             dots(i + 1, false, msh);
-            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 1, red_sh)) { status = 2; goto done; }
+            if (!syncv(i + 1, red_sh)) { status = 2; goto done; }
             if (threadIdx.x <= i) Hc[threadIdx.x] = red_sh[threadIdx.x];
             __syncthreads();
             // w1 = w - V h1; h2 = V^T w1; |w1|^2
@@ -978,7 +987,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 wr[k] = combine_V(wr[k], r, i + 1, Hc, 1.0);
             }
             dots(i + 1, true, msh);
-            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, i + 2, red_sh)) { status = 2; goto done; }
+            if (!syncv(i + 2, red_sh)) { status = 2; goto done; }
             double hn2 = red_sh[i + 1];
             for (int kk = 0; kk <= i; ++kk) hn2 -= red_sh[kk] * red_sh[kk];
             const double hnorm = sqrt(hn2 > 0.0 ? hn2 : 0.0);
@@ -993,16 +1002,18 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 for (int k = 0; k < RPT; ++k) vstore(i + 1, r0 + lr[k], vnew[k], own[k]);
                 nvec = i + 2;
             }
-            // Hessenberg column i: h1 + h2, then the Givens rotations (num/krylovsolver.h:258-280, 392-399)
+            // Hessenberg column i: h1 + h2 (one thread per entry), then the Givens rotations by one
+            // thread with the rotated entry carried in a register (num/krylovsolver.h:258-280, 392-399)
+            if (threadIdx.x <= (unsigned)i) Hc[threadIdx.x] += red_sh[threadIdx.x];
+            __syncthreads();
             if (threadIdx.x == 0) {
-                for (int kk = 0; kk <= i; ++kk) Hc[kk] += red_sh[kk];
-                Hc[i + 1] = hnorm;
+                double x = Hc[0];
                 for (int kk = 0; kk < i; ++kk) {
-                    const double t = cs[kk] * Hc[kk] + sn[kk] * Hc[kk + 1];
-                    Hc[kk + 1] = -sn[kk] * Hc[kk] + cs[kk] * Hc[kk + 1];
-                    Hc[kk] = t;
+                    const double y = Hc[kk + 1], ck = cs[kk], sk = sn[kk];
+                    Hs[col_of(i) + kk] = ck * x + sk * y;
+                    x = -sk * x + ck * y;
                 }
-                const double dx = Hc[i], dy = Hc[i + 1];
+                const double dx = x, dy = hnorm;
                 if (dy == 0.0) {
                     cs[i] = 1.0;
                     sn[i] = 0.0;
@@ -1011,17 +1022,12 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                     cs[i] = dx / rr;
                     sn[i] = dy / rr;
                 }
-                {
-                    const double t = cs[i] * Hc[i] + sn[i] * Hc[i + 1];
-                    Hc[i + 1] = -sn[i] * Hc[i] + cs[i] * Hc[i + 1];
-                    Hc[i] = t;
-                }
+                Hs[col_of(i) + i] = cs[i] * dx + sn[i] * dy;
                 {
                     const double t = cs[i] * sv[i] + sn[i] * sv[i + 1];
                     sv[i + 1] = -sn[i] * sv[i] + cs[i] * sv[i + 1];
                     sv[i] = t;
                 }
-                for (int kk = 0; kk <= i; ++kk) Hs[col_of(i) + kk] = Hc[kk];
             }
             __syncthreads();
             resid = fabs(sv[i + 1]) / normb;
@@ -1031,7 +1037,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 break;
             }
             // v_{i+1} visible to all blocks before the next SpMV
-            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+            if (!syncv(0, red_sh)) { status = 2; goto done; }
         }
         if (converged) {
             status = 0;
@@ -1042,12 +1048,12 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
 #pragma unroll
         for (int k = 0; k < RPT; ++k)
             if (own[k]) a.x[r0 + lr[k]] = xr[k];
-        if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 0, red_sh)) { status = 2; goto done; }
+        if (!syncv(0, red_sh)) { status = 2; goto done; }
         {
             double p_rr, p_bb;
             residual(false, p_rr, p_bb);
             if (threadIdx.x == 0) msh[0] = p_rr;
-            if (!sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, 1, red_sh)) { status = 2; goto done; }
+            if (!syncv(1, red_sh)) { status = 2; goto done; }
             beta = sqrt(red_sh[0]);
             resid = beta / normb;
             if (resid <= a.tol) {
@@ -1061,6 +1067,13 @@ done:
 #pragma unroll
         for (int k = 0; k < RPT; ++k)
             if (own[k]) a.x[r0 + lr[k]] = xr[k];
+    }
+    if (a.probe && threadIdx.x == 0) {
+        const uint64_t total = wall_clock64() - t_start;
+        a.probe[blockIdx.x * 4 + 0] = 0;
+        a.probe[blockIdx.x * 4 + 1] = total - t_sync;
+        a.probe[blockIdx.x * 4 + 2] = t_sync;
+        a.probe[blockIdx.x * 4 + 3] = total;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *a.out_iter = status == 0 ? j : a.max_iter;

@@ -61,13 +61,16 @@ __global__ void __launch_bounds__(SCALE_B) k_mlp_scale_part(const double *x0, co
     }
 }
 
+// one wave: lane-strided partials, then the wave's butterfly sum (the same order in every call)
 __global__ void k_mlp_scale_final(const double *part, int nb, int64_t n, double *scale) {
-    if (threadIdx.x != 0) return;
     double sm = 0.0, sc = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = threadIdx.x & 63; b < nb; b += 64) {
         sm += part[2 * b];
         sc += part[2 * b + 1];
     }
+    sm = wave_sum(sm);
+    sc = wave_sum(sc);
+    if (threadIdx.x != 0) return;
     const double rm = sqrt(sm / (double)n), rc = sqrt(sc / (double)n);
     scale[0] = rm > 0.0 ? rm : 1.0;
     scale[1] = rc > 0.0 ? rc : 1.0;
@@ -208,8 +211,11 @@ __global__ void __launch_bounds__(64 * INFER_WAVES) k_mlp_infer(const float *X, 
     if (threadIdx.x < MLP_NO) bias[3 * MLP_H + threadIdx.x] = theta[MLP_B4 + threadIdx.x];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int64_t m0 = ((int64_t)blockIdx.x * INFER_WAVES + wv) * 16;
     const int row = lane & 15, kq = lane >> 4;
+    // each block stages the weights once and walks its share of 64-DOF tiles
+    const int64_t ntiles = (S + 16 * INFER_WAVES - 1) / (16 * INFER_WAVES);
+    for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t m0 = (tile * INFER_WAVES + wv) * 16;
     // layer 1 input: features 8kq .. 8kq+7 of DOF m0+row (12 real features, zero padding)
     bf16x8_t a1;
     {
@@ -272,6 +278,8 @@ __global__ void __launch_bounds__(64 * INFER_WAVES) k_mlp_infer(const float *X, 
                 if (s < S) Y[s * MLP_NO + row] = d[i] + bias[3 * MLP_H + row];
             }
         }
+    }
+    __syncthreads();  // the next tile overwrites this wave's activations
     }
 }
 

@@ -91,7 +91,7 @@ const char *dhip_last_error(const dhip_ctx_t *ctx);
 int dhip_profile_enable(dhip_ctx_t *ctx, int32_t on);
 int dhip_profile_get(dhip_ctx_t *ctx, const char *name, double *total_ms, int64_t *count);
 int dhip_profile_reset(dhip_ctx_t *ctx);
-/* Wall-clock probe of the persistent FP64 solver (100 MHz ticks per block:
+/* Wall-clock probe of the persistent FP64 PCG and of GMRES (100 MHz ticks per block:
  * setup, compute phases, synchronisations, total).  dhip_solver_probe(ctx, 1, ...)
  * arms it for the following solves; `ticks` (4 per block, may be NULL) receives
  * the last solve's values and `nblocks` its block count. */
