@@ -201,6 +201,28 @@ def probe_sweep(ctx, ndof, x_dev, zero_dev):
     A.close()
 
 
+def gmres_probe_sweep(cells=96):
+    """Per-iteration compute/synchronisation split of the persistent GMRES on the first step of the
+    Cahn-Hilliard trajectory (torus, `cells`^3, seed 0) for a sweep of rows per block."""
+    from drops_amd import ch_run
+    m, phi = ch_run.torus_mesh(cells)
+    with D.Context(0) as ctx:
+        ch = ch_run.make_stepper(ctx, m, phi)
+        for rows in (128, 192, 256, 384, 512):
+            ctx.tune("pcg_rows_per_block", rows)
+            c0, mu0 = ch_run.initial_state(ch.n, 0)
+            ch.set_state(c0, mu0)
+            ch.prepare()
+            ctx.solver_probe(True)
+            it, res, ok = ch.solve()
+            pr = ctx.solver_probe(False).astype(np.float64) / 100.0  # 100 MHz ticks -> us
+            print(json.dumps(dict(rows_per_block=rows, blocks=int(pr.shape[0]), iters=it,
+                                  us_per_iter=float(pr[:, 3].mean() / it),
+                                  compute_us_per_iter=float(pr[:, 1].mean() / it),
+                                  sync_us_per_iter=float(pr[:, 2].mean() / it))), flush=True)
+        ch.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +241,8 @@ def main():
     ap.add_argument("--no-ch", action="store_true", help="skip the Cahn-Hilliard trajectory leg")
     ap.add_argument("--ch-cells", type=int, default=96)
     ap.add_argument("--ch-steps", type=int, default=200)
+    ap.add_argument("--gmres-probe", action="store_true",
+                    help="print the compute/sync split of the persistent GMRES on the Cahn-Hilliard system and exit")
     ap.add_argument("--probe-sweep", action="store_true",
                     help="print the per-iteration compute/sync breakdown of the persistent PCG for a sweep of "
                          "rows per block and exit")
@@ -228,6 +252,9 @@ def main():
                                    "described in DESIGN.md and not launched", "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.gmres_probe:
+        gmres_probe_sweep(args.ch_cells)
+        return
     n = args.cells
     mesh = KuhnMesh.box(-2.0, 2.0, n, shift=config_shift(4.0 / n))
     log(f"[bench] sampling the P2 level set on {mesh.halfgrid_shape} half-grid points")
