@@ -33,6 +33,11 @@ class Local(C.Structure):
                 ("rhs", C.c_double * 4), ("hat_sq", C.c_double * 4)]
 
 
+class LocalP2(C.Structure):
+    _fields_ = [("nfacet", C.c_int32), ("mass", C.c_double * 100), ("lb", C.c_double * 100),
+                ("stab", C.c_double * 100), ("rhs", C.c_double * 10), ("hat_sq", C.c_double * 10)]
+
+
 class Patch(C.Structure):
     _fields_ = [("nfacet", C.c_int32), ("bnd", C.c_int32 * 16), ("world", ((C.c_double * 3) * 3) * 16),
                 ("bary", ((C.c_double * 4) * 3) * 16), ("absdet", C.c_double * 16),
@@ -60,6 +65,10 @@ def lib(kind="parity"):
     L.orc_number_p1if.argtypes = [M, D, C.c_double, C.c_int, I32, I64]
     L.orc_tet_local.argtypes = [M, D, C.c_int64, D, C.c_int, D, C.c_int, C.POINTER(Local)]
     L.orc_assemble_p1.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64,
+                                  I64, I32, D, D, D, D]
+    L.orc_number_p2if.argtypes = [M, D, C.c_double, C.c_int, I32, I64]
+    L.orc_tet_local_p2.argtypes = [M, D, C.c_int64, D, C.c_int, D, C.c_int, C.POINTER(LocalP2)]
+    L.orc_assemble_p2.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64,
                                   I64, I32, D, D, D, D]
     L.orc_surface_area.argtypes = [M, D, C.c_int, D]
     L.orc_tet_patch.argtypes = [M, D, C.c_int64, C.c_int, C.POINTER(Patch)]
@@ -176,6 +185,52 @@ def assemble_p1(mesh, phi, dof_of_vertex, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=R
                              C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
                              _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(vs, C.c_double),
                              _ptr(rhs, C.c_double)), "assemble_p1")
+    k = nnz.value
+    return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
+                stab=vs[:k].copy(), rhs=rhs[:ndof].copy())
+
+
+def number_p2if(mesh, phi, omit_bound=-1.0, zero_policy=ZERO_THROW, kind="parity"):
+    """dof number of every half-grid node (-1: not an interface DOF) and the DOF count"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    nn = (2 * mesh.n[0] + 1) * (2 * mesh.n[1] + 1) * (2 * mesh.n[2] + 1)
+    dof = np.empty(nn, np.int32)
+    nd = C.c_int64()
+    _check(lib(kind).orc_number_p2if(C.byref(mesh), _ptr(phi, C.c_double), omit_bound, zero_policy,
+                                     _ptr(dof, C.c_int32), C.byref(nd)), "number_p2if")
+    return dof, nd.value
+
+
+def tet_local_p2(mesh, phi, tet, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+                 zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    out = LocalP2()
+    _check(lib().orc_tet_local_p2(C.byref(mesh), _ptr(phi, C.c_double), int(tet), _ptr(co, C.c_double),
+                                  rhs_kind, _ptr(rp, C.c_double), zero_policy, C.byref(out)), "tet_local_p2")
+    return out
+
+
+def assemble_p2(mesh, phi, dof_of_node, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE,
+                rhs_param=None, zero_policy=ZERO_THROW, kind="parity"):
+    """P2 interface matrices: dict(rowptr, colind, mass, lb, stab, rhs), int64 row pointers."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    L = lib(kind)
+    ncut = classify(mesh, phi, zero_policy).size
+    cap = 100 * max(ncut, 1)
+    rowptr = np.zeros(ndof + 1, np.int64)
+    col = np.zeros(cap, np.int32)
+    vm, vl, vs = (np.zeros(cap) for _ in range(3))
+    rhs = np.zeros(max(ndof, 1))
+    nnz = C.c_int64()
+    _check(L.orc_assemble_p2(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_node, C.c_int32), ndof,
+                             _ptr(co, C.c_double), rhs_kind, _ptr(rp, C.c_double), zero_policy, cap,
+                             C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
+                             _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(vs, C.c_double),
+                             _ptr(rhs, C.c_double)), "assemble_p2")
     k = nnz.value
     return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
                 stab=vs[:k].copy(), rhs=rhs[:ndof].copy())

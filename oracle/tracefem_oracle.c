@@ -951,6 +951,277 @@ int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, do
 }
 
 /* ------------------------------------------------------------------------ */
+/* P2 interface space on the planar patch                                    */
+/* ------------------------------------------------------------------------ */
+
+/* FE_P2CL::H0..H9 at barycentric p (num/fe.h:287-296): vertices l(2l-1), edges 4 l_a l_b */
+static void p2_basis(const double p[4], double h[10])
+{
+    for (int k = 0; k < 4; ++k) h[k] = p[k] * (2. * p[k] - 1.);
+    for (int e = 0; e < 6; ++e) h[4 + e] = 4. * p[VOE[e][0]] * p[VOE[e][1]];
+}
+
+/* gradients of the P2 basis at p from the barycentric derivatives:
+ * grad h_k = (4 l_k - 1) grad l_k, grad h_e = 4 l_b grad l_a + 4 l_a grad l_b */
+static void p2_gradients(const double p[4], const double G1[4][3], double g[10][3])
+{
+    for (int k = 0; k < 4; ++k) {
+        const double f = 4.0 * p[k] - 1.0;
+        for (int d = 0; d < 3; ++d) g[k][d] = f * G1[k][d];
+    }
+    for (int e = 0; e < 6; ++e) {
+        const int a = VOE[e][0], b = VOE[e][1];
+        for (int d = 0; d < 3; ++d) g[4 + e][d] = 4.0 * p[b] * G1[a][d] + 4.0 * p[a] * G1[b][d];
+    }
+}
+
+static int64_t p2_node_index(const orc_mesh_t *m, const tet_t *T, int d)
+{
+    const int64_t h1 = 2LL * m->n[0] + 1, h2 = 2LL * m->n[1] + 1;
+    const int l = LAT_OF_P2[d];
+    int64_t j[3];
+    for (int c = 0; c < 3; ++c) {
+        j[c] = 2 * T->i[c];
+        for (int q = 0; q < 4; ++q) j[c] += LAT_B2[l][q] * KUHN[T->k][q][c];
+    }
+    return (j[2] * h2 + j[1]) * h1 + j[0];
+}
+
+/* CreateNumbOnInterfaceP2 (misc/problem.cpp:571-636): first-encounter in tet order over the 10
+ * P2 nodes (vertices, then edges (0,1),(0,2),(1,2),(0,3),(1,3),(2,3)); a node gets the next number
+ * iff the Quad5 composite integral of its squared P2 basis function exceeds the limit. */
+int orc_number_p2if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
+                    int32_t *dof_of_node, int64_t *ndof)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_node || !ndof) return ORC_E_ARG;
+    const int64_t nn = (2LL * m->n[0] + 1) * (2LL * m->n[1] + 1) * (2LL * m->n[2] + 1);
+    for (int64_t v = 0; v < nn; ++v) dof_of_node[v] = -1;
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    if (!cut) return ORC_E_NOMEM;
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
+    int64_t counter = 0;
+    for (int64_t c = 0; c < ncut && !err; ++c) {
+        tet_t T;
+        load_tet(m, phi, cut[c], &T);
+        const double limit = omit_bound < 0. ? 0. : omit_bound * pow(tet_volume(T.X) * 6, 4. / 3.);
+        patch_t p;
+        err = make_patch(T.ls, &p);
+        if (err) break;
+        if (p.nf == 0) continue;
+        patch_geometry(T.X, &p);
+        double node[MAXF * 7][4], w[MAXF * 7];
+        const int nq = composite_nodes(&p, 7, Q5_N, Q5_W, node, w);
+        for (int i = 0; i < 10; ++i) {
+            const int64_t g = p2_node_index(m, &T, i);
+            if (dof_of_node[g] >= 0) continue;
+            double s = 0.;
+            for (int k = 0; k < nq; ++k) {
+                double h[10];
+                p2_basis(node[k], h);
+                s += w[k] * (h[i] * h[i]);
+            }
+            if (s > limit) dof_of_node[g] = (int32_t)counter++;
+        }
+    }
+    free(cut);
+    if (err) return err;
+    *ndof = counter;
+    return ORC_OK;
+}
+
+/* Local P2 quantities of one cut tet with a non-empty patch (Quad5 composite rule, normal = the
+ * normalised P2 level-set gradient at each point, as LocalAssembler::buildNormal,
+ * surfactant/ifacetransp.h:1062-1068):
+ *   mass  M_ij = sum_q w_q h_j h_i                         (LocalAssembler::M_P2P2, :1579-1588)
+ *   lb    A_ij = D sum_q w_q (P grad h_j).(P grad h_i)      (the P2 form of A_P1P1 with
+ *                                                           buildSurfGradP2, :1049-1053)
+ *   stab  C_ij = D |det| sum_q5 w (n.grad h_i)(n.grad h_j) (the volume integral, as C_n_P1P1,
+ *                                                           :1545-1555, 15-point Quad5 rule)
+ *   rhs   b_i = sum_q w_q f(x_q) h_i;  hat_sq_i = sum_q w_q h_i h_i (the numbering integrals) */
+static void local_quantities_p2(const tet_t *T, const patch_t *p, const double coef[3], int rhs_kind,
+                                const double *rhs_param, orc_local_p2_t *o)
+{
+    double node[MAXF * 7][4], w[MAXF * 7];
+    const int nq = composite_nodes(p, 7, Q5_N, Q5_W, node, w);
+    double G1[4][3];
+    const double det = p1_gradients(T->X, G1);
+    double v[10];
+    for (int d = 0; d < 10; ++d) v[d] = T->ls[LAT_OF_P2[d]];
+    memset(o, 0, sizeof *o);
+    o->nfacet = p->nf;
+    for (int k = 0; k < nq; ++k) {
+        double h[10], g[10][3], sg[10][3], nrm[3];
+        p2_basis(node[k], h);
+        p2_gradients(node[k], G1, g);
+        p2_normal(v, node[k], G1, nrm);
+        for (int i = 0; i < 10; ++i) {
+            const double d = dot3(g[i], nrm);
+            for (int c = 0; c < 3; ++c) sg[i][c] = g[i][c] - d * nrm[c];
+        }
+        double f = 0.;
+        if (rhs_kind != ORC_RHS_NONE) {
+            double x[3];
+            bary_to_world(T->X, node[k], x);
+            f = rhs_fun(rhs_kind, rhs_param, x);
+        }
+        for (int i = 0; i < 10; ++i) {
+            o->hat_sq[i] += w[k] * (h[i] * h[i]);
+            if (rhs_kind != ORC_RHS_NONE) o->rhs[i] += w[k] * (f * h[i]);
+            for (int j = 0; j <= i; ++j) {
+                o->mass[i * 10 + j] += w[k] * (h[j] * h[i]);
+                o->lb[i * 10 + j] += w[k] * dot3(sg[j], sg[i]);
+            }
+        }
+    }
+    for (int i = 0; i < 10; ++i)
+        for (int j = 0; j <= i; ++j) {
+            o->lb[i * 10 + j] = coef[1] * o->lb[i * 10 + j];
+            o->mass[i * 10 + j] = coef[0] * o->mass[i * 10 + j];
+            o->mass[j * 10 + i] = o->mass[i * 10 + j];
+            o->lb[j * 10 + i] = o->lb[i * 10 + j];
+        }
+    /* volume stabilisation on the 15-point Quad5 rule (weights of Quad5CL::quad) */
+    double U[10][15];
+    for (int q = 0; q < 15; ++q) {
+        double g[10][3], nrm[3];
+        p2_gradients(V5_N[q], G1, g);
+        p2_normal(v, V5_N[q], G1, nrm);
+        for (int i = 0; i < 10; ++i) U[i][q] = dot3(nrm, g[i]);
+    }
+    const double ad = fabs(det);
+    for (int i = 0; i < 10; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double f[15];
+            for (int q = 0; q < 15; ++q) f[q] = U[i][q] * U[j][q];
+            const double val = ad * (((8. / 405. * f[0] + V5_C1 * (((f[1] + f[2]) + f[3]) + f[4]))
+                                      + V5_C2 * (((f[5] + f[6]) + f[7]) + f[8]))
+                                     + 5. / 567. * (((((f[9] + f[10]) + f[11]) + f[12]) + f[13]) + f[14]));
+            o->stab[i * 10 + j] = o->stab[j * 10 + i] = coef[2] * val;
+        }
+}
+
+int orc_tet_local_p2(const orc_mesh_t *m, const double *phi, int64_t tet, const double coef[3], int rhs_kind,
+                     const double *rhs_param, int zero_policy, orc_local_p2_t *out)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !out || !coef) return ORC_E_ARG;
+    memset(out, 0, sizeof *out);
+    tet_t T;
+    load_tet(m, phi, tet, &T);
+    if (equal_signs(T.ls, 10)) return ORC_OK;
+    if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+    patch_t p;
+    int err = make_patch(T.ls, &p);
+    if (err) return err;
+    if (p.nf == 0) return ORC_OK;
+    patch_geometry(T.X, &p);
+    local_quantities_p2(&T, &p, coef, rhs_kind, rhs_param, out);
+    return ORC_OK;
+}
+
+#define ROWCAP2 160
+typedef struct {
+    int32_t n;
+    int32_t col[ROWCAP2];
+    double v[3][ROWCAP2];
+} brow2_t;
+
+/* accumulate() with the P2 forms and SparseMatBuilderCL semantics (num/spmat.h:551-677) */
+int orc_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                    const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                    int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                    double *val_mass, double *val_lb, double *val_stab, double *rhs)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_node || !nnz || !rowptr || !coef) return ORC_E_ARG;
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    orc_local_p2_t *loc = (orc_local_p2_t *)malloc(sizeof(orc_local_p2_t) * (ncut > 0 ? ncut : 1));
+    int32_t (*numb)[10] = (int32_t (*)[10])malloc(sizeof(int32_t) * 10 * (ncut > 0 ? ncut : 1));
+    int *st = (int *)calloc(ncut > 0 ? ncut : 1, sizeof(int));
+    brow2_t *rows = (brow2_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow2_t));
+    if (!cut || !loc || !numb || !st || !rows) {
+        free(cut); free(loc); free(numb); free(st); free(rows);
+        return ORC_E_NOMEM;
+    }
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t c = 0; c < ncut; ++c) {
+        tet_t T;
+        load_tet(m, phi, cut[c], &T);
+        patch_t p;
+        int e = make_patch(T.ls, &p);
+        if (e) { st[c] = e; continue; }
+        if (p.nf == 0) { st[c] = -1; continue; }
+        patch_geometry(T.X, &p);
+        local_quantities_p2(&T, &p, coef, rhs_kind, rhs_param, &loc[c]);
+        for (int i = 0; i < 10; ++i) numb[c][i] = dof_of_node[p2_node_index(m, &T, i)];
+    }
+    if (rhs)
+        for (int64_t i = 0; i < ndof; ++i) rhs[i] = 0.;
+    for (int64_t c = 0; c < ncut && !err; ++c) {
+        if (st[c] > 0) { err = st[c]; break; }
+        if (st[c] < 0) continue;
+        for (int i = 0; i < 10 && !err; ++i) {
+            const int32_t r = numb[c][i];
+            if (r < 0) continue;
+            brow2_t *R = &rows[r];
+            for (int j = 0; j < 10; ++j) {
+                const int32_t cj = numb[c][j];
+                if (cj < 0) continue;
+                int k = 0;
+                for (; k < R->n; ++k)
+                    if (R->col[k] == cj) break;
+                if (k == R->n) {
+                    if (R->n == ROWCAP2) { err = ORC_E_CAPACITY; break; }
+                    R->col[k] = cj;
+                    R->v[0][k] = R->v[1][k] = R->v[2][k] = 0.0;
+                    ++R->n;
+                }
+                R->v[0][k] += loc[c].mass[i * 10 + j];
+                R->v[1][k] += loc[c].lb[i * 10 + j];
+                R->v[2][k] += loc[c].stab[i * 10 + j];
+            }
+            if (rhs) rhs[r] += loc[c].rhs[i];
+        }
+    }
+    if (!err) {
+        rowptr[0] = 0;
+        for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
+        *nnz = rowptr[ndof];
+        if (*nnz > cap_nnz) err = ORC_E_CAPACITY;
+    }
+    if (!err && colind) {
+        for (int64_t r = 0; r < ndof; ++r) {
+            brow2_t *R = &rows[r];
+            int32_t ord[ROWCAP2];
+            for (int k = 0; k < R->n; ++k) ord[k] = k;
+            for (int a = 1; a < R->n; ++a) {
+                int32_t x = ord[a];
+                int b = a - 1;
+                while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
+                ord[b + 1] = x;
+            }
+            for (int k = 0; k < R->n; ++k) {
+                const int64_t dst = rowptr[r] + k;
+                colind[dst] = R->col[ord[k]];
+                if (val_mass) val_mass[dst] = R->v[0][ord[k]];
+                if (val_lb) val_lb[dst] = R->v[1][ord[k]];
+                if (val_stab) val_stab[dst] = R->v[2][ord[k]];
+            }
+        }
+    }
+    free(cut); free(loc); free(numb); free(st); free(rows);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Cahn-Hilliard step pieces (CahnHilliardcGP1CL, resting interface)          */
 /* ------------------------------------------------------------------------ */
 

@@ -127,6 +127,22 @@ int orc_gmres_jacobi(int64_t n, const int64_t *rowptr, const int32_t *colind, co
                      const double *b, double *x, int32_t restart, int32_t *max_iter, double *tol,
                      int32_t rel, int32_t left_pc, int32_t use_pc);
 
+/* ---- P2 interface space on the planar patch ---- */
+typedef struct {
+    int32_t nfacet;
+    double mass[100], lb[100], stab[100];   /* 10 x 10, P2 dof order: vertices, edges (0,1),(0,2),(1,2),(0,3),(1,3),(2,3) */
+    double rhs[10], hat_sq[10];
+} orc_local_p2_t;
+/* CreateNumbOnInterfaceP2 (misc/problem.cpp:571-636); dof_of_node over the half-grid nodes */
+int orc_number_p2if(const orc_mesh_t *m, const double *phi, double omit_bound, int zero_policy,
+                    int32_t *dof_of_node, int64_t *ndof);
+int orc_tet_local_p2(const orc_mesh_t *m, const double *phi, int64_t tet, const double coef[3], int rhs_kind,
+                     const double *rhs_param, int zero_policy, orc_local_p2_t *out);
+int orc_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                    const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                    int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                    double *val_mass, double *val_lb, double *val_stab, double *rhs);
+
 /* GMRES as above, preconditioned by z = b / d with a given diagonal d (block Jacobi) */
 int orc_gmres_diag(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
                    const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,

@@ -54,3 +54,27 @@ PARITY_CASES = {
     "skew_brick": skew_case,
     "sphere20": sphere20,
 }
+
+
+KUHN = np.array([[[0, 0, 0], [1, 0, 0], [1, 1, 0], [1, 1, 1]],
+                 [[0, 0, 0], [1, 0, 0], [1, 0, 1], [1, 1, 1]],
+                 [[0, 0, 0], [0, 1, 0], [1, 1, 0], [1, 1, 1]],
+                 [[0, 0, 0], [0, 1, 0], [0, 1, 1], [1, 1, 1]],
+                 [[0, 0, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1]],
+                 [[0, 0, 0], [0, 0, 1], [0, 1, 1], [1, 1, 1]]])
+P2_EDGES = ((0, 1), (0, 2), (1, 2), (0, 3), (1, 3), (2, 3))
+
+
+def tet_p2_nodes(mesh, tets):
+    """Half-grid indices (T, 10) of the P2 nodes of tets, in the order vertices 0..3, then
+    edges (0,1), (0,2), (1,2), (0,3), (1,3), (2,3); an edge node sits at doubled coordinates
+    v_a + v_b."""
+    tets = np.asarray(tets, np.int64)
+    n1, n2, _ = mesh.n
+    h1, h2 = 2 * n1 + 1, 2 * n2 + 1
+    cell, k = tets // 6, tets % 6
+    base = np.stack([cell % n1, (cell // n1) % n2, cell // (n1 * n2)], axis=-1) * 2  # (T,3)
+    off = KUHN[k]  # (T,4,3)
+    rel = [2 * off[:, a] for a in range(4)] + [off[:, a] + off[:, b] for a, b in P2_EDGES]
+    j = base[:, None, :] + np.stack(rel, axis=1)  # (T,10,3)
+    return (j[..., 2] * h2 + j[..., 1]) * h1 + j[..., 0]
