@@ -3,6 +3,7 @@
 //   hipcc --offload-arch=gfx950 -O3 -ffp-contract=off -fPIC -shared
 #include "dhip_common.hpp"
 #include "dhip_assembly.hpp"
+#include "dhip_p2.hpp"
 #include "dhip_krylov.hpp"
 #include "dhip_ch.hpp"
 #include "dhip_mlp.hpp"
@@ -137,6 +138,11 @@ static void derive_tables(Tables &T) {
             m |= 1u << T.tet_node27[k][l];
         }
         T.tetmask27[k] = m;
+        for (int i = 0; i < 27; ++i) T.p2_of27[k][i] = -1;
+        for (int d = 0; d < 10; ++d) {
+            T.p2_node27[k][d] = T.tet_node27[k][T.lat_of_p2[d]];
+            T.p2_of27[k][T.p2_node27[k][d]] = (int8_t)d;
+        }
     }
     // incidence: vertex = cell + delta, delta = (d&1, d>>1&1, d>>2&1)
     for (int d = 0; d < 8; ++d)
@@ -548,6 +554,7 @@ static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const
     int rc = check_err_flag(c);
     if (rc) return rc;
     c->stage = 2;
+    c->space = 1;
     return DHIP_OK;
 }
 
@@ -560,7 +567,8 @@ int dhip_local_p1(dhip_ctx_t *c, const double coef[3], int32_t rhs_kind, const d
 }
 
 static int number_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
-    if (c->stage < 2) return set_error(c, DHIP_E_STATE, "dhip_local_p1 must precede dhip_number_p1if");
+    if (c->stage < 2 || c->space != 1)
+        return set_error(c, DHIP_E_STATE, "dhip_local_p1 must precede dhip_number_p1if");
     const int64_t N = c->ncut, S = 4 * N;
     DHIP_HIPCHK(c, c->first.ensure(sizeof(int32_t) * (S + 1)));
     DHIP_HIPCHK(c, c->isown.ensure(sizeof(int32_t) * (S + 1)));
@@ -598,7 +606,8 @@ int dhip_number_p1if(dhip_ctx_t *c, double omit_bound, int64_t *ndof) {
 }
 
 static int build_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
-    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "dhip_number_p1if must precede dhip_build_p1");
+    if (c->stage < 3 || c->space != 1)
+        return set_error(c, DHIP_E_STATE, "dhip_number_p1if must precede dhip_build_p1");
     const int64_t R = c->ndof;
     DHIP_HIPCHK(c, c->rowlen.ensure(sizeof(int32_t) * (R + 1)));
     DHIP_HIPCHK(c, c->rowptr.ensure(sizeof(int32_t) * (R + 1)));
@@ -654,6 +663,136 @@ int dhip_assemble_p1(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], i
     return flush_profile(c);
 }
 
+// ---------------------------------------------------------------------------
+// P2 interface space
+// ---------------------------------------------------------------------------
+static int local_p2_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_local_p2");
+    const int64_t N = c->ncut;
+    DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC2_N * (N + 1)));
+    DHIP_HIPCHK(c, c->flags.ensure(N + 1));
+    for (int k = 0; k < 3; ++k) c->coef[k] = coef ? coef[k] : 1.0;
+    LocalP2Args la{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->coef[0], c->coef[1], c->coef[2], rhs_kind,
+                   rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
+                   c->errflag.as<int32_t>()};
+    if (N > 0) DHIP_LAUNCH(c, "local_p2", k_local_p2, (unsigned)N, 64, 0, la);
+    int rc = check_err_flag(c);
+    if (rc) return rc;
+    c->stage = 2;
+    c->space = 2;
+    return DHIP_OK;
+}
+
+static int number_p2_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
+    if (c->stage < 2 || c->space != 2)
+        return set_error(c, DHIP_E_STATE, "dhip_local_p2 must precede dhip_number_p2if");
+    const int64_t N = c->ncut, S = 10 * N;
+    DHIP_HIPCHK(c, c->qual.ensure(sizeof(uint16_t) * (N + 1)));
+    DHIP_HIPCHK(c, c->first.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->isown.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->ownrank.ensure(sizeof(int32_t) * (S + 1)));
+    DHIP_HIPCHK(c, c->dofs.ensure(sizeof(int32_t) * (S + 1)));
+    if (N > 0) {
+        QualP2Args qa{c->mesh, c->cut.as<int64_t>(), N, c->loc.as<double>(), c->flags.as<uint8_t>(),
+                      c->qual.as<uint16_t>(), omit_bound};
+        DHIP_LAUNCH(c, "qualify_p2", k_qualify_p2, blocks_for(N, 256), 256, 0, qa);
+        FirstP2Args fa{c->mesh, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->flags.as<uint8_t>(),
+                       c->qual.as<uint16_t>(), c->first.as<int32_t>(), c->isown.as<int32_t>()};
+        DHIP_LAUNCH(c, "first_owner_p2", k_first_p2, blocks_for(S, 256), 256, 0, fa);
+    }
+    int64_t nd = 0;
+    int rc = scan_exclusive_i32(c, c->isown.as<int32_t>(), c->ownrank.as<int32_t>(), S, &nd);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, c->rowowner.ensure(sizeof(int32_t) * (nd + 1)));
+    if (N > 0) {
+        DofArgs da{S, c->first.as<int32_t>(), c->ownrank.as<int32_t>(), c->dofs.as<int32_t>(),
+                   c->rowowner.as<int32_t>()};
+        DHIP_LAUNCH(c, "dofs_p2", k_dofs, blocks_for(S, 256), 256, 0, da);
+    }
+    c->ndof = nd;
+    c->stage = 3;
+    if (ndof) *ndof = nd;
+    return DHIP_OK;
+}
+
+static int build_p2_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
+    if (c->stage < 3 || c->space != 2)
+        return set_error(c, DHIP_E_STATE, "dhip_number_p2if must precede dhip_build_p2");
+    const int64_t R = c->ndof;
+    DHIP_HIPCHK(c, c->rowlen.ensure(sizeof(int32_t) * (R + 1)));
+    DHIP_HIPCHK(c, c->rowptr.ensure(sizeof(int32_t) * (R + 1)));
+    DHIP_HIPCHK(c, c->rhs.ensure(sizeof(double) * (R + 1)));
+    DHIP_HIPCHK(c, c->diag.ensure(sizeof(double) * (R + 1)));
+    RowP2Args ra{c->mesh, c->cut.as<int64_t>(), c->ncut, c->cellinfo.as<uint32_t>(), c->flags.as<uint8_t>(),
+                 c->dofs.as<int32_t>(), c->rowowner.as<int32_t>(), R, c->loc.as<double>(), c->rowlen.as<int32_t>(),
+                 c->rowptr.as<int32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr, c->rhs.as<double>(),
+                 c->diag.as<double>(), lc ? lc[0] : 1.0, lc ? lc[1] : 1.0, lc ? lc[2] : 1.0};
+    const unsigned nb = blocks_for(R, ROWP2_WAVES);
+    if (R > 0) DHIP_LAUNCH(c, "rowcount_p2", k_rows_p2<false>, nb, 64 * ROWP2_WAVES, 0, ra);
+    int64_t total = 0;
+    int rc = scan_exclusive_i32(c, c->rowlen.as<int32_t>(), c->rowptr.as<int32_t>(), R, &total);
+    if (rc) return rc;
+    if (total >= ((int64_t)1 << 31) - 1)
+        return set_error(c, DHIP_E_CAPACITY, "%lld nonzeros exceed int32 row pointers", (long long)total);
+    {
+        int32_t t32 = (int32_t)total;
+        DHIP_HIPCHK(c, hipMemcpyAsync(c->rowptr.as<int32_t>() + R, &t32, sizeof t32, hipMemcpyHostToDevice,
+                                      c->stream));
+    }
+    DHIP_HIPCHK(c, c->col.ensure(sizeof(int32_t) * (total + 1)));
+    for (int k = 0; k < 4; ++k) DHIP_HIPCHK(c, c->val[k].ensure(sizeof(double) * (total + 1)));
+    ra.col = c->col.as<int32_t>();
+    ra.vm = c->val[0].as<double>();
+    ra.vl = c->val[1].as<double>();
+    ra.vs = c->val[2].as<double>();
+    ra.vc = c->val[3].as<double>();
+    if (R > 0) DHIP_LAUNCH(c, "rowfill_p2", k_rows_p2<true>, nb, 64 * ROWP2_WAVES, 0, ra);
+    c->nnz = total;
+    c->stage = 4;
+    if (nnz) *nnz = total;
+    return DHIP_OK;
+}
+
+int dhip_local_p2(dhip_ctx_t *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = local_p2_impl(c, coef, rhs_kind, rhs_param);
+    if (rc) return rc;
+    return flush_profile(c);
+}
+
+int dhip_number_p2if(dhip_ctx_t *c, double omit_bound, int64_t *ndof) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = number_p2_impl(c, omit_bound, ndof);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_build_p2(dhip_ctx_t *c, const double lc[3], int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = build_p2_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_assemble_p2(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                     const double *rhs_param, double omit_bound, const double lc[3], int64_t *ncut,
+                     int64_t *ndof, int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = classify_impl(c, zero_policy, ncut, false);
+    if (!rc) rc = local_p2_impl(c, coef, rhs_kind, rhs_param);
+    if (!rc) rc = number_p2_impl(c, omit_bound, ndof);
+    if (!rc) rc = build_p2_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
 static int d2h(dhip_ctx *c, void *dst, const void *src, size_t bytes) {
     if (!dst || bytes == 0) return DHIP_OK;
     DHIP_HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -671,7 +810,7 @@ int dhip_get_cut_tets(dhip_ctx_t *c, int64_t *tets) {
 
 int dhip_get_local(dhip_ctx_t *c, double *local57, uint8_t *flags) {
     if (!c) return DHIP_E_ARG;
-    if (c->stage < 2) return set_error(c, DHIP_E_STATE, "no local matrices");
+    if (c->stage < 2 || c->space != 1) return set_error(c, DHIP_E_STATE, "no P1 local matrices");
     const int64_t N = c->ncut;
     if (local57 && N > 0) {
         std::vector<double> soa((size_t)LOC_N * N);
@@ -687,10 +826,54 @@ int dhip_get_local(dhip_ctx_t *c, double *local57, uint8_t *flags) {
     return DHIP_OK;
 }
 
+int dhip_get_local_p2(dhip_ctx_t *c, double *local320, uint8_t *flags) {
+    if (!c) return DHIP_E_ARG;
+    if (c->stage < 2 || c->space != 2) return set_error(c, DHIP_E_STATE, "no P2 local matrices");
+    const int64_t N = c->ncut;
+    if (local320 && N > 0) {
+        std::vector<double> soa((size_t)LOC2_N * N);
+        int rc = d2h(c, soa.data(), c->loc.p, sizeof(double) * LOC2_N * N);
+        if (rc) return rc;
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int64_t i = 0; i < N; ++i)
+            for (int e = 0; e < LOC2_N; ++e) local320[i * LOC2_N + e] = soa[(size_t)e * N + i];
+    }
+    int rc = d2h(c, flags, c->flags.p, N);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_get_dof_nodes(dhip_ctx_t *c, int64_t *node_of_dof) {
+    if (!c || !node_of_dof) return DHIP_E_ARG;
+    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "no numbering");
+    const int per = c->space == 2 ? 10 : 4;
+    std::vector<int32_t> owner(c->ndof);
+    std::vector<int64_t> cut(c->ncut);
+    int rc = d2h(c, owner.data(), c->rowowner.p, sizeof(int32_t) * c->ndof);
+    if (!rc) rc = d2h(c, cut.data(), c->cut.p, sizeof(int64_t) * c->ncut);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    const Tables &T = host_tables();
+    const MeshP &m = c->mesh;
+    const int64_t h1 = 2 * (int64_t)m.n[0] + 1, h2 = 2 * (int64_t)m.n[1] + 1;
+    for (int64_t r = 0; r < c->ndof; ++r) {
+        const int64_t s = owner[r], t = cut[s / per];
+        const int k = (int)(t % 6), d = (int)(s % per);
+        const int n27 = per == 10 ? T.p2_node27[k][d] : T.tet_node27[k][T.lat_of_p2[d]];
+        const int64_t cell = t / 6;
+        const int64_t j1 = 2 * (cell % m.n[0]) + n27 % 3;
+        const int64_t j2 = 2 * ((cell / m.n[0]) % m.n[1]) + (n27 / 3) % 3;
+        const int64_t j3 = 2 * (cell / ((int64_t)m.n[0] * m.n[1])) + n27 / 9;
+        node_of_dof[r] = (j3 * h2 + j2) * h1 + j1;
+    }
+    return DHIP_OK;
+}
+
 int dhip_get_tet_dofs(dhip_ctx_t *c, int32_t *dofs) {
     if (!c) return DHIP_E_ARG;
     if (c->stage < 3) return set_error(c, DHIP_E_STATE, "no numbering");
-    int rc = d2h(c, dofs, c->dofs.p, sizeof(int32_t) * 4 * c->ncut);
+    int rc = d2h(c, dofs, c->dofs.p, sizeof(int32_t) * (c->space == 2 ? 10 : 4) * c->ncut);
     if (rc) return rc;
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return DHIP_OK;
@@ -698,7 +881,7 @@ int dhip_get_tet_dofs(dhip_ctx_t *c, int32_t *dofs) {
 
 int dhip_get_dof_vertices(dhip_ctx_t *c, int64_t *vertex_of_dof) {
     if (!c || !vertex_of_dof) return DHIP_E_ARG;
-    if (c->stage < 3) return set_error(c, DHIP_E_STATE, "no numbering");
+    if (c->stage < 3 || c->space != 1) return set_error(c, DHIP_E_STATE, "no P1 numbering");
     std::vector<int32_t> owner(c->ndof);
     std::vector<int64_t> cut(c->ncut);
     int rc = d2h(c, owner.data(), c->rowowner.p, sizeof(int32_t) * c->ndof);

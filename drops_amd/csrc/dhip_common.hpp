@@ -34,6 +34,8 @@ struct Tables {
     int8_t slot[6][4][4];      // neighbour slot 0..14 of (k, row vertex, col vertex)
     int8_t tet_node27[6][10];  // 3x3x3 cell-node index of lattice vertex l of tet k
     uint32_t tetmask27[6];     // the 10 cell nodes of tet k as a 27-bit mask
+    int8_t p2_node27[6][10];   // 3x3x3 cell-node index of P2 dof d of tet k
+    int8_t p2_of27[6][27];     // P2 dof of tet k at cell node (-1 if the tet has no node there)
     int8_t slot_delta[15][3];  // neighbour offset of each slot
     double lat_b[10][4];
     double q2n[4][3], q2w[4];  // Quad2_2DDataCL
@@ -114,11 +116,12 @@ struct dhip_ctx {
 
     // assembly state: 0 none, 1 classified, 2 local, 3 numbered, 4 built
     int stage = 0;
+    int space = 1;   // 1: P1 interface space, 2: P2 interface space (set by the local step)
     int zero_policy = DHIP_ZERO_THROW;
     int64_t ncut = 0, ndof = 0, nnz = 0;
     double coef[3] = {1, 1, 1};
     dhip::DevBuf cellmask, blockcnt, cellinfo, cut, errflag;
-    dhip::DevBuf loc, flags;
+    dhip::DevBuf loc, flags, qual;
     dhip::DevBuf first, isown, ownrank, dofs, rowowner;
     dhip::DevBuf rowlen, rowptr, col, val[4], rhs, diag;
     dhip::DevBuf scan_tmp, red_tmp, bar;

@@ -20,6 +20,7 @@ RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST = range(3)
 PC_NONE, PC_JACOBI = 0, 1
 FP64, FP32_IR, FP64_CLASSIC = 0, 1, 2
 LOC_N = 57
+LOC2_N = 320
 
 # every symbol declared in include/dhip.h
 EXPORTS = [
@@ -33,7 +34,8 @@ EXPORTS = [
     "dhip_ch_set_state", "dhip_ch_get_state", "dhip_ch_prepare", "dhip_ch_system", "dhip_ch_solve", "dhip_ch_step",
     "dhip_mlp_create", "dhip_mlp_destroy", "dhip_mlp_params", "dhip_mlp_normalization", "dhip_mlp_features",
     "dhip_mlp_targets", "dhip_mlp_predict", "dhip_mlp_grad", "dhip_mlp_train_step", "dhip_ch_history",
-    "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned",
+    "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned", "dhip_local_p2", "dhip_number_p2if",
+    "dhip_build_p2", "dhip_assemble_p2", "dhip_get_local_p2", "dhip_get_dof_nodes",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -85,6 +87,12 @@ def load(path=LIB_PATH):
     L.dhip_assemble_p1.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
     L.dhip_get_cut_tets.argtypes = [P, I64]
     L.dhip_get_local.argtypes = [P, D, C.POINTER(C.c_uint8)]
+    L.dhip_local_p2.argtypes = [P, D, C.c_int32, D]
+    L.dhip_number_p2if.argtypes = [P, C.c_double, I64]
+    L.dhip_build_p2.argtypes = [P, D, I64]
+    L.dhip_assemble_p2.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
+    L.dhip_get_local_p2.argtypes = [P, D, C.POINTER(C.c_uint8)]
+    L.dhip_get_dof_nodes.argtypes = [P, I64]
     L.dhip_get_tet_dofs.argtypes = [P, I32]
     L.dhip_get_dof_vertices.argtypes = [P, I64]
     L.dhip_get_csr.argtypes = [P, I32, I32, D, D, D, D, D]
@@ -466,8 +474,31 @@ class Context:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         self.check(self.L.dhip_assemble_p1(self.h, zero_policy, _p(co, C.c_double), rhs_kind, _p(rp, C.c_double),
                                            omit_bound, _p(lcv, C.c_double), C.byref(a), C.byref(b), C.byref(c)))
-        self.ncut, self.ndof, self.nnz = a.value, b.value, c.value
+        self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 1
         return a.value, b.value, c.value
+
+    def assemble_p2(self, zero_policy=ZERO_THROW, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+                    omit_bound=-1.0, lc=(1.0, 1.0, 1.0)):
+        """P2 interface space: (ncut, ndof, nnz); results through csr(), tet_dofs(), dof_nodes()."""
+        co = np.asarray(coef, np.float64)
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        lcv = np.asarray(lc, np.float64)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(self.L.dhip_assemble_p2(self.h, zero_policy, _p(co, C.c_double), rhs_kind, _p(rp, C.c_double),
+                                           omit_bound, _p(lcv, C.c_double), C.byref(a), C.byref(b), C.byref(c)))
+        self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 2
+        return a.value, b.value, c.value
+
+    def local_p2(self):
+        loc = np.zeros((max(self.ncut, 1), LOC2_N))
+        fl = np.zeros(max(self.ncut, 1), np.uint8)
+        self.check(self.L.dhip_get_local_p2(self.h, _p(loc, C.c_double), _p(fl, C.c_uint8)))
+        return loc[: self.ncut], fl[: self.ncut]
+
+    def dof_nodes(self):
+        out = np.zeros(max(self.ndof, 1), np.int64)
+        self.check(self.L.dhip_get_dof_nodes(self.h, _p(out, C.c_int64)))
+        return out[: self.ndof]
 
     def cut_tets(self):
         out = np.zeros(max(self.ncut, 1), np.int64)
@@ -481,7 +512,7 @@ class Context:
         return loc[: self.ncut], fl[: self.ncut]
 
     def tet_dofs(self):
-        out = np.zeros((max(self.ncut, 1), 4), np.int32)
+        out = np.zeros((max(self.ncut, 1), 10 if getattr(self, "space", 1) == 2 else 4), np.int32)
         self.check(self.L.dhip_get_tet_dofs(self.h, _p(out, C.c_int32)))
         return out[: self.ncut]
 

@@ -140,15 +140,40 @@ int dhip_assemble_p1(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3],
                      const double *rhs_param, double omit_bound, const double lc[3],
                      int64_t *ncut, int64_t *ndof, int64_t *nnz);
 
+/* ---- P2 interface space on the same cut tets and planar patch ----
+ * Local forms of LocalAssembler (surfactant/ifacetransp.h:914-1619) with the P2 basis
+ * FE_P2CL (num/fe.h:287-296): mass M_P2P2 (:1579-1588, times coef[0]), Laplace-Beltrami
+ * with the projected P2 gradients of buildSurfGradP2 (:1049-1053, times coef[1]), normal
+ * = normalised P2 level-set gradient (build3DNormal, :1062-1076), the normal-gradient
+ * stabilisation as the true volume integral of C_n_P1P1 (:1545-1555) on the 15-point
+ * Quad5 rule (times coef[2]), load vector; composite Quad5 surface rule throughout.
+ * The P2 dof order is vertices 0-3, then edges (0,1),(0,2),(1,2),(0,3),(1,3),(2,3). */
+int dhip_local_p2(dhip_ctx_t *ctx, const double coef[3], int32_t rhs_kind, const double *rhs_param);
+/* First-encounter P2 interface numbering, CreateNumbOnInterfaceP2 (misc/problem.cpp:571-636):
+ * a half-grid node gets the next number on its first (tet, local dof) in mesh order whose
+ * composite-Quad5 integral of the squared basis function exceeds the limit. */
+int dhip_number_p2if(dhip_ctx_t *ctx, double omit_bound, int64_t *ndof);
+/* CSR of the P2 operators and load vector, as dhip_build_p1 (same result accessors). */
+int dhip_build_p2(dhip_ctx_t *ctx, const double lc[3], int64_t *nnz);
+/* classify + local_p2 + number_p2if + build_p2 in one call. */
+int dhip_assemble_p2(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                     const double *rhs_param, double omit_bound, const double lc[3],
+                     int64_t *ncut, int64_t *ndof, int64_t *nnz);
+
 /* ---- results of the assembly (host copies; any pointer may be NULL) ---- */
 int dhip_get_cut_tets(dhip_ctx_t *ctx, int64_t *tets);
 /* per cut tet: 16 mass, 16 LB, 16 stab, 4 rhs, 4 phi_i^2 integrals, area (57 doubles),
  * and flags (bit0 non-empty patch, bits1-4 vertex qualifies for numbering) */
 int dhip_get_local(dhip_ctx_t *ctx, double *local57, uint8_t *flags);
-/* DOF of each (cut tet, local vertex) pair, -1 where the vertex is not numbered */
+/* P2 space: per cut tet 100 mass, 100 LB, 100 stab, 10 rhs, 10 squared-basis integrals
+ * (320 doubles, row-major 10x10 in P2 dof order) and flags (bit0 non-empty patch) */
+int dhip_get_local_p2(dhip_ctx_t *ctx, double *local320, uint8_t *flags);
+/* DOF of each (cut tet, local dof) pair (4 per tet for P1, 10 for P2), -1 where not numbered */
 int dhip_get_tet_dofs(dhip_ctx_t *ctx, int32_t *dofs);
-/* vertex id of each DOF */
+/* vertex id of each DOF (P1 space) */
 int dhip_get_dof_vertices(dhip_ctx_t *ctx, int64_t *vertex_of_dof);
+/* half-grid node index (j3 (2 n2 + 1) + j2) (2 n1 + 1) + j1 of each DOF (either space) */
+int dhip_get_dof_nodes(dhip_ctx_t *ctx, int64_t *node_of_dof);
 int dhip_get_csr(dhip_ctx_t *ctx, int32_t *rowptr, int32_t *colind, double *mass, double *lb,
                  double *stab, double *comb, double *rhs);
 /* Device pointers of the assembled arrays (valid until the next assembly). */
