@@ -80,10 +80,16 @@ def lib(kind="parity"):
                                    C.c_int32, C.c_int32]
     L.orc_gmres_diag.argtypes = [C.c_int64, I64, I32, D, D, D, D, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.orc_tet_mobility.argtypes = [M, D, C.c_int64, D, C.c_int, D]
+    L.orc_tet_mobility_p2.argtypes = [M, D, C.c_int64, D, C.c_int, D]
     L.orc_ch_assemble.argtypes = [M, D, I32, C.c_int64, I64, I32, D, C.c_int, D, D, D]
+    L.orc_ch_assemble_p2.argtypes = [M, D, I32, C.c_int64, I64, I32, D, C.c_int, D, D, D]
     L.orc_ch_system.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_double, I64, I32, D, D, D]
+    L.orc_ch_system_p2.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_double, I64, I32, D, D, D]
     L.orc_ch_step.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
+    L.orc_ch_step_p2.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
     L.orc_lattice_tables.argtypes = [I32, I32]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
@@ -302,12 +308,25 @@ def tet_mobility(mesh, phi, tet, cv, zero_policy=ZERO_THROW):
     return out.reshape(4, 4)
 
 
-def ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, zero_policy=ZERO_THROW, kind="parity"):
-    """Mobility matrix values on the pattern, M c and M f'(c) (LocalMatVecP1CL accumulations)."""
+def tet_mobility_p2(mesh, phi, tet, cv, zero_policy=ZERO_THROW):
+    out = np.zeros(100)
+    cv = np.ascontiguousarray(cv, np.float64)
+    _check(lib().orc_tet_mobility_p2(C.byref(mesh), _ptr(phi, C.c_double), int(tet), _ptr(cv, C.c_double),
+                                     zero_policy, _ptr(out, C.c_double)), "tet_mobility_p2")
+    return out.reshape(10, 10)
+
+
+def _sfx(space):
+    return "_p2" if space == 2 else ""
+
+
+def ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, zero_policy=ZERO_THROW, kind="parity", space=1):
+    """Mobility matrix values on the pattern, M c and M f'(c) (LocalMatVecP1CL accumulations).
+    space=2: the P2 interface space, dof_of_vertex is then the half-grid node numbering."""
     lm = np.zeros(col.size)
     td, wl = np.zeros(ndof), np.zeros(ndof)
     c = np.ascontiguousarray(c, np.float64)
-    _check(lib(kind).orc_ch_assemble(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32),
+    _check(getattr(lib(kind), "orc_ch_assemble" + _sfx(space))(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32),
                                      ndof, _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
                                      _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), _ptr(c, C.c_double),
                                      zero_policy, _ptr(lm, C.c_double), _ptr(td, C.c_double),
@@ -338,14 +357,14 @@ def block2x2(rowptr, col, A, B, Cb, D):
 
 
 def ch_system(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, dt, eps, rho, sigma=1.0, S=1.0,
-              kind="parity"):
+              kind="parity", space=1):
     """Block system of one step (orc_ch_system): block CSR, rhs and block-Jacobi diagonal."""
     nnz = col.size
     brp = np.zeros(2 * ndof + 1, np.int64)
     bcol = np.zeros(4 * nnz, np.int32)
     bval, rhs, pcd = np.zeros(4 * nnz), np.zeros(2 * ndof), np.zeros(2 * ndof)
     f64 = lambda a: _ptr(np.ascontiguousarray(a, np.float64), C.c_double)
-    _check(lib(kind).orc_ch_system(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
+    _check(getattr(lib(kind), "orc_ch_system" + _sfx(space))(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
                                    _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
                                    _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), f64(mass), f64(lb), f64(stab),
                                    f64(c), dt, eps, rho, sigma, S, _ptr(brp, C.c_int64), _ptr(bcol, C.c_int32),
@@ -354,11 +373,11 @@ def ch_system(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, dt
 
 
 def ch_step_c(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, x, dt, eps, rho, sigma=1.0, S=1.0,
-              restart=50, tol=1e-8, max_iter=20000, rel=True, kind="parity"):
+              restart=50, tol=1e-8, max_iter=20000, rel=True, kind="parity", space=1):
     """One full step in C (orc_ch_step); x = [mu; c] updated in place.  Returns (iters, resid, ok)."""
     it, t = C.c_int32(max_iter), C.c_double(tol)
     f64 = lambda a: _ptr(np.ascontiguousarray(a, np.float64), C.c_double)
-    code = lib(kind).orc_ch_step(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
+    code = getattr(lib(kind), "orc_ch_step" + _sfx(space))(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32), ndof,
                                  _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
                                  _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), f64(mass), f64(lb), f64(stab),
                                  _ptr(x, C.c_double), dt, eps, rho, sigma, S, restart, C.byref(it), C.byref(t),
@@ -369,16 +388,16 @@ def ch_step_c(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, x, dt
 
 
 def ch_step(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, mu, dt, eps, rho,
-            sigma=1.0, S=1.0, restart=50, tol=1e-8, max_iter=20000, kind="parity"):
+            sigma=1.0, S=1.0, restart=50, tol=1e-8, max_iter=20000, kind="parity", space=1):
     """One step of CahnHilliardcGP1CL::DoStep (surfactant/ifacetransp.cpp:1823-1856) with theta = 1,
     resting interface (no wind: Conv = Massd = 0) and no sources, solved by right-preconditioned
     GMRES(restart) with block Jacobi: the diagonals of Precond3 = M + sigma dt LM + dt rho C and
     Precond4 = M + eps^2 L + eps^2 rho C (:1567-1570).  Unknowns [mu; c], start = previous step.
     The system comes from orc_ch_system; the mobility matrix and LocalMatVec vectors are returned
     too (orc_ch_assemble)."""
-    lm, timeder, well = ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, kind=kind)
+    lm, timeder, well = ch_assemble(mesh, phi, dof_of_vertex, ndof, rowptr, col, c, kind=kind, space=space)
     brp, bcol, bval, b, pcd = ch_system(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, dt, eps,
-                                        rho, sigma, S, kind=kind)
+                                        rho, sigma, S, kind=kind, space=space)
     x0 = np.concatenate([mu, c])
     x, it, res, ok = gmres_diag(brp, bcol, bval, pcd, b, x0, restart=restart, max_iter=max_iter, tol=tol)
     return dict(mu=x[:ndof], c=x[ndof:], iters=it, res=res, ok=ok, lm=lm, timeder=timeder, well=well,

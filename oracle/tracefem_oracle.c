@@ -1268,6 +1268,56 @@ static void local_mobility(const tet_t *T, const patch_t *p, const double cv[4],
         }
 }
 
+/* The P2 counterpart: mobility = P2 interpolant sum_d Mob(c_d) h_d of the nodal mobilities
+ * (LinearCombinationCL order, num/fe.h:44-53), projected P2 gradients (buildSurfGradP2,
+ * surfactant/ifacetransp.h:1049-1053) with the P2 level-set normal, Quad5 composite rule. */
+static void local_mobility_p2(const tet_t *T, const patch_t *p, const double cv[10], double out[100])
+{
+    double node[MAXF * 7][4], w[MAXF * 7];
+    const int nq = composite_nodes(p, 7, Q5_N, Q5_W, node, w);
+    double G1[4][3];
+    p1_gradients(T->X, G1);
+    double v[10], mv[10];
+    for (int d = 0; d < 10; ++d) {
+        v[d] = T->ls[LAT_OF_P2[d]];
+        mv[d] = mobility_fun(cv[d]);
+    }
+    for (int e = 0; e < 100; ++e) out[e] = 0.;
+    for (int k = 0; k < nq; ++k) {
+        double h[10], g[10][3], sg[10][3], nrm[3];
+        p2_basis(node[k], h);
+        p2_gradients(node[k], G1, g);
+        p2_normal(v, node[k], G1, nrm);
+        for (int i = 0; i < 10; ++i) {
+            const double d = dot3(g[i], nrm);
+            for (int c = 0; c < 3; ++c) sg[i][c] = g[i][c] - d * nrm[c];
+        }
+        double qm = h[0] * mv[0];
+        for (int d = 1; d < 10; ++d) qm += h[d] * mv[d];
+        for (int i = 0; i < 10; ++i)
+            for (int j = 0; j < 10; ++j) out[i * 10 + j] += w[k] * (qm * dot3(sg[i], sg[j]));
+    }
+}
+
+int orc_tet_mobility_p2(const orc_mesh_t *m, const double *phi, int64_t tet, const double cv[10], int zero_policy,
+                        double out[100])
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !cv || !out) return ORC_E_ARG;
+    memset(out, 0, 100 * sizeof(double));
+    tet_t T;
+    load_tet(m, phi, tet, &T);
+    if (equal_signs(T.ls, 10)) return ORC_OK;
+    if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) return ORC_E_ZERO_NODE;
+    patch_t p;
+    int err = make_patch(T.ls, &p);
+    if (err) return err;
+    if (p.nf == 0) return ORC_OK;
+    patch_geometry(T.X, &p);
+    local_mobility_p2(&T, &p, cv, out);
+    return ORC_OK;
+}
+
 int orc_tet_mobility(const orc_mesh_t *m, const double *phi, int64_t tet, const double cv[4], int zero_policy,
                      double out[16])
 {
@@ -1292,18 +1342,19 @@ int orc_tet_mobility(const orc_mesh_t *m, const double *phi, int64_t tet, const 
  * vectors timeder = LocalMatVecP1CL<LocalInterfaceMassP1CL>(c) and
  * well = LocalMatVecP1CL<LocalInterfaceMassP1CL>(f'(c)) (surfactant/ifacetransp.h:800-828), each
  * accumulated in mesh order. */
-int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
-                    const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
-                    double *lm, double *timeder, double *well)
+static int ch_assemble_sp(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                          const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                          double *lm, double *timeder, double *well)
 {
     init_tables();
-    if (!check_mesh(m) || !phi || !dof_of_vertex || !rowptr || !colind || !c || !lm || !timeder || !well)
+    if (!check_mesh(m) || !phi || !dof_of || !rowptr || !colind || !c || !lm || !timeder || !well)
         return ORC_E_ARG;
+    const int nl = space == 2 ? 10 : 4;
     int64_t ncut = 0;
     int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
     if (err) return err;
     int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
-    typedef struct { int err, nonempty; int32_t numb[4]; double mass[16], mob[16]; } chrec_t;
+    typedef struct { int err, nonempty; int32_t numb[10]; double mass[100], mob[100]; } chrec_t;
     chrec_t *rec = (chrec_t *)malloc(sizeof(chrec_t) * (ncut > 0 ? ncut : 1));
     if (!cut || !rec) { free(cut); free(rec); return ORC_E_NOMEM; }
     orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
@@ -1318,15 +1369,22 @@ int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
         if (rec[q].err || p.nf == 0) continue;
         rec[q].nonempty = 1;
         patch_geometry(T.X, &p);
-        double cv[4];
-        for (int i = 0; i < 4; ++i) {
-            rec[q].numb[i] = dof_of_vertex[T.vid[i]];
+        double cv[10];
+        for (int i = 0; i < nl; ++i) {
+            rec[q].numb[i] = space == 2 ? dof_of[p2_node_index(m, &T, i)] : dof_of[T.vid[i]];
             cv[i] = rec[q].numb[i] >= 0 ? c[rec[q].numb[i]] : 0.;
         }
-        orc_local_t L;
-        local_quantities(&T, &p, one3, ORC_RHS_NONE, NULL, &L);
-        memcpy(rec[q].mass, L.mass, sizeof L.mass);
-        local_mobility(&T, &p, cv, rec[q].mob);
+        if (space == 2) {
+            orc_local_p2_t L;
+            local_quantities_p2(&T, &p, one3, ORC_RHS_NONE, NULL, &L);
+            memcpy(rec[q].mass, L.mass, sizeof L.mass);
+            local_mobility_p2(&T, &p, cv, rec[q].mob);
+        } else {
+            orc_local_t L;
+            local_quantities(&T, &p, one3, ORC_RHS_NONE, NULL, &L);
+            memcpy(rec[q].mass, L.mass, sizeof L.mass);
+            local_mobility(&T, &p, cv, rec[q].mob);
+        }
     }
     for (int64_t e = 0; e < rowptr[ndof]; ++e) lm[e] = 0.;
     for (int64_t i = 0; i < ndof; ++i) timeder[i] = well[i] = 0.;
@@ -1334,22 +1392,22 @@ int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
         if (rec[q].err) { err = rec[q].err; break; }
         if (!rec[q].nonempty) continue;
         const int32_t *nb = rec[q].numb;
-        double xc[4], xf[4];
-        for (int j = 0; j < 4; ++j) {
+        double xc[10], xf[10];
+        for (int j = 0; j < nl; ++j) {
             xc[j] = nb[j] >= 0 ? c[nb[j]] : 0.;
             xf[j] = potential_prime(xc[j]);
         }
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < nl; ++i) {
             if (nb[i] < 0) continue;
             double vt = 0., vw = 0.;
-            for (int j = 0; j < 4; ++j) {
+            for (int j = 0; j < nl; ++j) {
                 if (nb[j] < 0) continue;
-                vt += rec[q].mass[i * 4 + j] * xc[j];
-                vw += rec[q].mass[i * 4 + j] * xf[j];
+                vt += rec[q].mass[i * nl + j] * xc[j];
+                vw += rec[q].mass[i * nl + j] * xf[j];
                 int64_t e = rowptr[nb[i]];
                 while (e < rowptr[nb[i] + 1] && colind[e] != nb[j]) ++e;
                 if (e == rowptr[nb[i] + 1]) { err = ORC_E_ARG; break; }
-                lm[e] += rec[q].mob[i * 4 + j];
+                lm[e] += rec[q].mob[i * nl + j];
             }
             timeder[nb[i]] += vt;
             well[nb[i]] += vw;
@@ -1360,21 +1418,37 @@ int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
     return err;
 }
 
+int orc_ch_assemble(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                    const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                    double *lm, double *timeder, double *well)
+{
+    return ch_assemble_sp(1, m, phi, dof_of_vertex, ndof, rowptr, colind, c, zero_policy, lm, timeder, well);
+}
+
+/* The same on the P2 interface space: dof_of_node over the half-grid nodes, P2 local mass and
+ * P2 mobility matrix (local_mobility_p2), 10 local dofs per cut tet. */
+int orc_ch_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                       const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                       double *lm, double *timeder, double *well)
+{
+    return ch_assemble_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, c, zero_policy, lm, timeder, well);
+}
+
 /* The step system of CahnHilliardcGP1CL::DoStep (surfactant/ifacetransp.cpp:1823-1856), theta = 1,
  * no wind, no sources: block CSR (SparseMatBaseCL(A, B, C, D), num/spmat.h:719-751), right-hand
  * side [M c; -M f'(c) + S M c] and the block-Jacobi diagonal [diag Precond3; diag Precond4]
  * (:1567-1570), each LinComb evaluated in the reference's order. */
-int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
-                  const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
-                  const double *stab, const double *c, double dt, double eps, double rho, double sigma, double S,
-                  int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
+static int ch_system_sp(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                        const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                        const double *stab, const double *c, double dt, double eps, double rho, double sigma,
+                        double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
 {
     const int64_t nnz = rowptr[ndof];
     double *lm = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
     double *td = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
     double *wl = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
     if (!lm || !td || !wl) { free(lm); free(td); free(wl); return ORC_E_NOMEM; }
-    int err = orc_ch_assemble(m, phi, dof_of_vertex, ndof, rowptr, colind, c, ORC_ZERO_THROW, lm, td, wl);
+    int err = ch_assemble_sp(space, m, phi, dof_of, ndof, rowptr, colind, c, ORC_ZERO_THROW, lm, td, wl);
     if (!err) {
         const double sdt = sigma * dt, rdt = rho * dt, dtr = dt * rho, e2 = eps * eps, re2 = rho * eps * eps,
                      e2r = eps * eps * rho, meps = -eps;
@@ -1409,11 +1483,29 @@ int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_
     return err;
 }
 
+int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                  const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                  const double *stab, const double *c, double dt, double eps, double rho, double sigma, double S,
+                  int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
+{
+    return ch_system_sp(1, m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, c, dt, eps, rho, sigma, S,
+                        brp, bcol, bval, rhs, pcdiag);
+}
+
+int orc_ch_system_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                     const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                     const double *stab, const double *c, double dt, double eps, double rho, double sigma,
+                     double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
+{
+    return ch_system_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, mass, lb, stab, c, dt, eps, rho, sigma, S,
+                        brp, bcol, bval, rhs, pcdiag);
+}
+
 /* one step: system + right-preconditioned GMRES(restart) from x = [mu; c] (in/out) */
-int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
-                const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
-                const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
-                int32_t restart, int32_t *max_iter, double *tol, int32_t rel)
+static int ch_step_sp(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                      const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                      const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                      int32_t restart, int32_t *max_iter, double *tol, int32_t rel)
 {
     const int64_t nnz = rowptr[ndof], n2 = 2 * ndof;
     int64_t *brp = (int64_t *)malloc(sizeof(int64_t) * (n2 + 1));
@@ -1423,11 +1515,29 @@ int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_ve
     double *pcd = (double *)malloc(sizeof(double) * (n2 + 1));
     int err = (!brp || !bcol || !bval || !rhs || !pcd) ? ORC_E_NOMEM : ORC_OK;
     if (!err)
-        err = orc_ch_system(m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, x + ndof, dt, eps, rho,
-                            sigma, S, brp, bcol, bval, rhs, pcd);
+        err = ch_system_sp(space, m, phi, dof_of, ndof, rowptr, colind, mass, lb, stab, x + ndof, dt, eps, rho,
+                           sigma, S, brp, bcol, bval, rhs, pcd);
     if (!err) err = orc_gmres_diag(n2, brp, bcol, bval, pcd, rhs, x, restart, max_iter, tol, rel, 0);
     free(brp); free(bcol); free(bval); free(rhs); free(pcd);
     return err;
+}
+
+int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                int32_t restart, int32_t *max_iter, double *tol, int32_t rel)
+{
+    return ch_step_sp(1, m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, x, dt, eps, rho, sigma, S,
+                      restart, max_iter, tol, rel);
+}
+
+int orc_ch_step_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                   const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                   const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                   int32_t restart, int32_t *max_iter, double *tol, int32_t rel)
+{
+    return ch_step_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, mass, lb, stab, x, dt, eps, rho, sigma, S,
+                      restart, max_iter, tol, rel);
 }
 
 /* ------------------------------------------------------------------------ */

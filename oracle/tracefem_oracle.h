@@ -167,6 +167,22 @@ int orc_ch_step(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_ve
                 const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
                 int32_t restart, int32_t *max_iter, double *tol, int32_t rel);
 
+/* P2 interface space: the same pieces with dof_of_node over the half-grid, the P2 local mass and
+ * the P2 mobility matrix (mobility = P2 interpolant of the nodal mobilities) */
+int orc_tet_mobility_p2(const orc_mesh_t *m, const double *phi, int64_t tet, const double cv[10], int zero_policy,
+                        double out[100]);
+int orc_ch_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                       const int64_t *rowptr, const int32_t *colind, const double *c, int zero_policy,
+                       double *lm, double *timeder, double *well);
+int orc_ch_system_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                     const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                     const double *stab, const double *c, double dt, double eps, double rho, double sigma,
+                     double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag);
+int orc_ch_step_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
+                   const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                   const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
+                   int32_t restart, int32_t *max_iter, double *tol, int32_t rel);
+
 #ifdef __cplusplus
 }
 #endif

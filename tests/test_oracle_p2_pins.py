@@ -178,3 +178,66 @@ def test_torus96_counts():
     assert nd == 95552
     r = O.assemble_p2(om, phi, dof, nd, zero_policy=O.ZERO_ALLOW)
     assert r["colind"].size == 2177312
+
+
+# ---------------------------------------------------------------- Cahn-Hilliard on the P2 space
+def mob(x):
+    return np.maximum((1.0 - x) * x, 0.0)
+
+
+def test_p2_mobility_on_plane_closed_form():
+    """x = 0.33: for u = y (|grad_G u| = 1) u' LM u = int_G m_h, m_h the P2 interpolant of the
+    nodal mobilities, = sum_i Mob(c_i) int_G h_i = Mob(c) . b with b the constant-load vector.
+    Also LM = Mob(c) LB for a constant concentration."""
+    m, phi, om, dof, nd, r = plane_setup()
+    rng = np.random.default_rng(5)
+    c = rng.uniform(-0.1, 1.1, nd)
+    lm, td, wl = O.ch_assemble(om, phi, dof, nd, r["rowptr"], r["colind"], c, space=2)
+    LM = csr(dict(r, lm=lm), "lm", nd)
+    u = interp(m, dof, nd, lambda x, y, z: y)
+    assert abs(u @ (LM @ u) - mob(c) @ r["rhs"]) < 1e-13
+    lm3, _, _ = O.ch_assemble(om, phi, dof, nd, r["rowptr"], r["colind"], np.full(nd, 0.3), space=2)
+    assert np.abs(lm3 - mob(0.3) * r["lb"]).max() <= 1e-13 * np.abs(r["lb"]).max()
+
+
+def test_p2_ch_vectors_and_symmetry():
+    """M c and M f'(c) are the assembled P2 mass applied to c and f'(c); LM is symmetric and
+    annihilates constants."""
+    m, phi = cases.torus_case(12)
+    om = mesh_of(m)
+    dof, nd = O.number_p2if(om, phi)
+    r = O.assemble_p2(om, phi, dof, nd)
+    c = 0.5 + np.random.default_rng(2).uniform(-0.05, 0.05, nd)
+    lm, td, wl = O.ch_assemble(om, phi, dof, nd, r["rowptr"], r["colind"], c, space=2)
+    M = csr(r, "mass", nd)
+    absM = abs(M)
+    fp = (c - 1.0) * c * (c - 0.5)
+    assert np.all(np.abs(td - M @ c) <= 1e-14 * (absM @ np.abs(c)))
+    assert np.all(np.abs(wl - M @ fp) <= 1e-14 * (absM @ np.abs(fp)))
+    LM = csr(dict(r, lm=lm), "lm", nd)
+    assert np.abs(LM @ np.ones(nd)).max() <= 1e-11 * np.abs(lm).max()
+    assert abs(LM - LM.T).max() <= 1e-15 * np.abs(lm).max()
+
+
+def test_p2_ch_step_direct_solve_and_conservation():
+    import scipy.sparse.linalg as spla
+    m, phi = cases.torus_case(10)
+    om = mesh_of(m)
+    dof, nd = O.number_p2if(om, phi)
+    r = O.assemble_p2(om, phi, dof, nd)
+    rng = np.random.default_rng(3)
+    c = 0.5 + rng.uniform(-0.05, 0.05, nd)
+    mu = np.zeros(nd)
+    M = csr(r, "mass", nd)
+    total0 = np.ones(nd) @ (M @ c)
+    for _ in range(2):
+        o = O.ch_step(om, phi, dof, nd, r["rowptr"], r["colind"], r["mass"], r["lb"], r["stab"], c, mu,
+                      dt=1e-4, eps=0.05, rho=m.h, tol=1e-10, space=2)
+        assert o["ok"]
+        K = sp.csr_matrix((o["val"], o["col"], o["rowptr"]), shape=(2 * nd, 2 * nd))
+        xd = spla.spsolve(K.tocsc(), o["rhs"])
+        x = np.concatenate([o["mu"], o["c"]])
+        assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+        c, mu = o["c"], o["mu"]
+        assert abs(np.ones(nd) @ (M @ c) - total0) <= 1e-8 * abs(total0)
+    assert 0.0 < c.min() and c.max() < 1.0
