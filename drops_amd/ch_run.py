@@ -24,10 +24,11 @@ def initial_state(ndof, seed):
     return 0.5 + rng.uniform(-0.05, 0.05, ndof), np.zeros(ndof)
 
 
-def make_stepper(ctx, mesh, phi, restart=50, tol=1e-8, max_iter=20000):
+def make_stepper(ctx, mesh, phi, restart=50, tol=1e-8, max_iter=20000, space=1):
     ctx.mesh(mesh)
     ctx.levelset(phi)
-    return ctx.cahn_hilliard(DT, EPS, mesh.h, sigma=SIGMA, S=S_STAB, restart=restart, tol=tol, max_iter=max_iter)
+    return ctx.cahn_hilliard(DT, EPS, mesh.h, sigma=SIGMA, S=S_STAB, restart=restart, tol=tol, max_iter=max_iter,
+                             space=space)
 
 
 def run(ch, seed, steps, record=False, guess=None):

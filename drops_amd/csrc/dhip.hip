@@ -6,6 +6,7 @@
 #include "dhip_p2.hpp"
 #include "dhip_krylov.hpp"
 #include "dhip_ch.hpp"
+#include "dhip_ch_p2.hpp"
 #include "dhip_mlp.hpp"
 
 #include <algorithm>
@@ -1400,6 +1401,9 @@ struct dhip_ch {
     DevBuf hist[3];
     int nhist = 0;
     DevBuf feat, pred, xstart;
+    // P2 space: contribution lists of the entries and rows, row of each entry
+    int space = 1;
+    DevBuf eptr, econt, rptr, rcont, erow;
     dhip_csr K;
     bool prepared = false;
 };
@@ -1413,13 +1417,27 @@ static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
 static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
                       &h->col, &h->mass, &h->lb, &h->stab, &h->geom, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
-                      &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred, &h->xstart};
+                      &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred, &h->xstart,
+                      &h->eptr, &h->econt, &h->rptr, &h->rcont, &h->erow};
     for (DevBuf *b : bufs) b->release();
 }
 
 static int ch_prepare_impl(dhip_ch *h) {
     dhip_ctx *c = h->ctx;
     const int64_t N = h->ncut, R = h->ndof;
+    if (h->space == 2) {
+        ChTetP2Args ta{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->massloc.as<double>(), h->geom.as<double>(),
+                       h->x.as<double>() + R, h->tet.as<double>()};
+        if (N > 0) DHIP_LAUNCH(c, "ch_tet_p2", k_ch_tet_p2, blocks_for(N, CHTET2_WAVES), 64 * CHTET2_WAVES, 0, ta);
+        ChEntryP2Args ea{h->nnz, h->erow.as<int32_t>(), h->rowptr.as<int32_t>(), h->col.as<int32_t>(),
+                         h->eptr.as<int32_t>(), h->econt.as<int32_t>(), h->tet.as<double>(), h->mass.as<double>(),
+                         h->stab.as<double>(), h->bval.as<double>(), h->pcdiag.as<double>(), h->k};
+        if (h->nnz > 0) DHIP_LAUNCH(c, "ch_entries_p2", k_ch_entries_p2, blocks_for(h->nnz, 256), 256, 0, ea);
+        ChRhsP2Args ra{R, h->rptr.as<int32_t>(), h->rcont.as<int32_t>(), h->tet.as<double>(), h->rhs.as<double>(), h->k};
+        if (R > 0) DHIP_LAUNCH(c, "ch_rhs_p2", k_ch_rhs_p2, blocks_for(R, 256), 256, 0, ra);
+        h->prepared = true;
+        return DHIP_OK;
+    }
     ChTetArgs ta{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->massloc.as<double>(), h->geom.as<double>(),
                  h->x.as<double>() + R, h->tet.as<double>()};
     if (N > 0) DHIP_LAUNCH(c, "ch_tet", k_ch_tet, blocks_for(N, 256), 256, 0, ta);
@@ -1461,20 +1479,83 @@ static int ch_solve_impl(dhip_ch *h, int32_t *iters, double *resid) {
     return rc;
 }
 
-extern "C" {
+// P2 space: contribution lists of the entries and rows (counts, scans, fills, per-list sorts)
+static int ch_contrib_lists(dhip_ch *h) {
+    dhip_ctx *c = h->ctx;
+    const int64_t N = h->ncut, R = h->ndof, Z = h->nnz;
+    DHIP_HIPCHK(c, h->eptr.ensure(sizeof(int32_t) * (Z + 1)));
+    DHIP_HIPCHK(c, h->rptr.ensure(sizeof(int32_t) * (R + 1)));
+    DHIP_HIPCHK(c, h->erow.ensure(sizeof(int32_t) * (Z + 1)));
+    DevBuf ecnt, rcnt;
+    DHIP_HIPCHK(c, ecnt.ensure(sizeof(int32_t) * (Z + 1)));
+    DHIP_HIPCHK(c, rcnt.ensure(sizeof(int32_t) * (R + 1)));
+    auto zero = [&](DevBuf &b, int64_t n) {
+        DHIP_LAUNCH(c, "ch_zero", k_zero_u32, blocks_for(n, 256), 256, 0, b.as<uint32_t>(), (int)n);
+        return DHIP_OK;
+    };
+    int rc = zero(ecnt, Z + 1);
+    if (!rc) rc = zero(rcnt, R + 1);
+    if (rc) return rc;
+    DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+    ContribArgs ca{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->rowptr.as<int32_t>(), h->col.as<int32_t>(),
+                   ecnt.as<int32_t>(), rcnt.as<int32_t>(), h->eptr.as<int32_t>(), h->rptr.as<int32_t>(),
+                   nullptr, nullptr, c->errflag.as<int32_t>()};
+    if (N > 0) DHIP_LAUNCH(c, "ch_contrib_count", k_ch_contrib<false>, blocks_for(100 * N, 256), 256, 0, ca);
+    int64_t etot = 0, rtot = 0;
+    rc = scan_exclusive_i32(c, ecnt.as<int32_t>(), h->eptr.as<int32_t>(), Z, &etot);
+    if (!rc) rc = scan_exclusive_i32(c, rcnt.as<int32_t>(), h->rptr.as<int32_t>(), R, &rtot);
+    if (rc) return rc;
+    int32_t fl = 0;
+    DHIP_HIPCHK(c, hipMemcpyAsync(&fl, c->errflag.p, sizeof fl, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (fl & 8) return set_error(c, DHIP_E_STATE, "a local entry has no slot in the P2 pattern");
+    {
+        const int32_t e32 = (int32_t)etot, r32 = (int32_t)rtot;
+        DHIP_HIPCHK(c, hipMemcpyAsync(h->eptr.as<int32_t>() + Z, &e32, sizeof e32, hipMemcpyHostToDevice, c->stream));
+        DHIP_HIPCHK(c, hipMemcpyAsync(h->rptr.as<int32_t>() + R, &r32, sizeof r32, hipMemcpyHostToDevice, c->stream));
+    }
+    DHIP_HIPCHK(c, h->econt.ensure(sizeof(int32_t) * (etot + 1)));
+    DHIP_HIPCHK(c, h->rcont.ensure(sizeof(int32_t) * (rtot + 1)));
+    rc = zero(ecnt, Z + 1);
+    if (!rc) rc = zero(rcnt, R + 1);
+    if (rc) return rc;
+    ca.econt = h->econt.as<int32_t>();
+    ca.rcont = h->rcont.as<int32_t>();
+    if (N > 0) DHIP_LAUNCH(c, "ch_contrib_fill", k_ch_contrib<true>, blocks_for(100 * N, 256), 256, 0, ca);
+    if (Z > 0) DHIP_LAUNCH(c, "ch_sort_lists", k_sort_lists, blocks_for(Z, 256), 256, 0, Z, h->eptr.as<int32_t>(),
+                           h->econt.as<int32_t>());
+    if (R > 0) DHIP_LAUNCH(c, "ch_sort_lists", k_sort_lists, blocks_for(R, 256), 256, 0, R, h->rptr.as<int32_t>(),
+                           h->rcont.as<int32_t>());
+    if (R > 0) DHIP_LAUNCH(c, "ch_entry_rows", k_entry_rows, blocks_for(R, 256), 256, 0, R, h->rowptr.as<int32_t>(),
+                           h->erow.as<int32_t>());
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
 
-int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p, dhip_ch_t **out, int64_t *ndof) {
+static int ch_create(dhip_ctx *c, int32_t zero_policy, const dhip_ch_params_t *p, dhip_ch_t **out, int64_t *ndof,
+                     int space) {
     if (!c || !p || !out) return DHIP_E_ARG;
     if (!(p->dt > 0.0) || p->restart < 2 || p->restart > KMAX || p->max_iter < 1 || !(p->tol > 0.0))
         return set_error(c, DHIP_E_ARG, "invalid Cahn-Hilliard parameters");
     DHIP_HIPCHK(c, hipSetDevice(c->device));
     const double one3[3] = {1.0, 1.0, 1.0};
     int rc = classify_impl(c, zero_policy, nullptr, false);
-    if (!rc) rc = local_impl(c, one3, DHIP_RHS_NONE, nullptr);
-    if (!rc) rc = number_impl(c, -1.0, nullptr);
-    if (!rc) rc = build_impl(c, one3, nullptr);
+    if (space == 2) {
+        if (!rc && c->ncut >= (int64_t)(INT32_MAX / 100))
+            return set_error(c, DHIP_E_CAPACITY, "%lld cut tets exceed the P2 stepper's key range",
+                             (long long)c->ncut);
+        if (!rc) rc = local_p2_impl(c, one3, DHIP_RHS_NONE, nullptr);
+        if (!rc) rc = number_p2_impl(c, -1.0, nullptr);
+        if (!rc) rc = build_p2_impl(c, one3, nullptr);
+    } else {
+        if (!rc) rc = local_impl(c, one3, DHIP_RHS_NONE, nullptr);
+        if (!rc) rc = number_impl(c, -1.0, nullptr);
+        if (!rc) rc = build_impl(c, one3, nullptr);
+    }
     if (rc) return rc;
+    const int nloc = space == 2 ? 10 : 4;
     dhip_ch *h = new dhip_ch();
+    h->space = space;
     h->ctx = c;
     h->p = *p;
     h->mesh = c->mesh;
@@ -1500,16 +1581,18 @@ int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p
     if ((rc = ch_copy(c, h->phi, c->phi, sizeof(double) * c->phi_count)) ||
         (rc = ch_copy(c, h->cut, c->cut.p, sizeof(int64_t) * N)) ||
         (rc = ch_copy(c, h->cellinfo, c->cellinfo.p, sizeof(uint32_t) * c->mesh.ncell)) ||
-        (rc = ch_copy(c, h->flags, c->flags.p, N)) || (rc = ch_copy(c, h->dofs, c->dofs.p, sizeof(int32_t) * 4 * N)) ||
+        (rc = ch_copy(c, h->flags, c->flags.p, N)) ||
+        (rc = ch_copy(c, h->dofs, c->dofs.p, sizeof(int32_t) * nloc * N)) ||
         (rc = ch_copy(c, h->rowowner, c->rowowner.p, sizeof(int32_t) * R)) ||
-        (rc = ch_copy(c, h->massloc, c->loc.p, sizeof(double) * 16 * N)) ||
+        (rc = ch_copy(c, h->massloc, c->loc.p, sizeof(double) * nloc * nloc * N)) ||
         (rc = ch_copy(c, h->rowptr, c->rowptr.p, sizeof(int32_t) * (R + 1))) ||
         (rc = ch_copy(c, h->col, c->col.p, sizeof(int32_t) * Z)) ||
         (rc = ch_copy(c, h->mass, c->val[0].p, sizeof(double) * Z)) ||
         (rc = ch_copy(c, h->lb, c->val[1].p, sizeof(double) * Z)) ||
         (rc = ch_copy(c, h->stab, c->val[2].p, sizeof(double) * Z)))
         return fail(rc);
-    if (h->geom.ensure(sizeof(double) * CHG_N * (N + 1)) || h->tet.ensure(sizeof(double) * CHT_N * (N + 1)) ||
+    const int geom_n = space == 2 ? CHG2_N : CHG_N, tet_n = space == 2 ? CHT2_N : CHT_N;
+    if (h->geom.ensure(sizeof(double) * geom_n * (N + 1)) || h->tet.ensure(sizeof(double) * tet_n * (N + 1)) ||
         h->brp.ensure(sizeof(int32_t) * (2 * R + 1)) ||
         h->bcol.ensure(sizeof(int32_t) * (4 * Z + 1)) || h->bval.ensure(sizeof(double) * (4 * Z + 1)) ||
         h->pcdiag.ensure(sizeof(double) * (2 * R + 1)) || h->rhs.ensure(sizeof(double) * (2 * R + 1)) ||
@@ -1525,12 +1608,22 @@ int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p
         hipLaunchKernelGGL(k_ch_block, dim3(blocks_for(R + 1, 128)), dim3(128), 0, c->stream, ba);
         if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_block launch failed"));
     }
-    if (N > 0) {
+    if (N > 0 && space == 2) {
+        ChGeomP2Args ga{h->mesh, h->phi.as<double>(), h->cut.as<int64_t>(), N, h->flags.as<uint8_t>(),
+                        h->geom.as<double>(), c->errflag.as<int32_t>()};
+        ProfScope ps(c, "ch_geom_p2");
+        hipLaunchKernelGGL(k_ch_geom_p2, dim3((unsigned)N), dim3(64), 0, c->stream, ga);
+        if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_geom_p2 launch failed"));
+    } else if (N > 0) {
         ChGeomArgs ga{h->mesh, h->phi.as<double>(), h->cut.as<int64_t>(), N, h->flags.as<uint8_t>(),
                       h->geom.as<double>()};
         ProfScope ps(c, "ch_geom");
         hipLaunchKernelGGL(k_ch_geom, dim3(blocks_for(N, LOCAL_BLOCK)), dim3(LOCAL_BLOCK), 0, c->stream, ga);
         if (hipGetLastError() != hipSuccess) return fail(set_error(c, DHIP_E_HIP, "k_ch_geom launch failed"));
+    }
+    if (space == 2) {
+        rc = ch_contrib_lists(h);
+        if (rc) return fail(rc);
     }
     h->K.ctx = c;
     h->K.n = 2 * R;
@@ -1542,6 +1635,17 @@ int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p
     *out = h;
     if (ndof) *ndof = R;
     return flush_profile(c);
+}
+
+extern "C" {
+
+int dhip_ch_create(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p, dhip_ch_t **out, int64_t *ndof) {
+    return ch_create(c, zero_policy, p, out, ndof, 1);
+}
+
+int dhip_ch_create_p2(dhip_ctx_t *c, int32_t zero_policy, const dhip_ch_params_t *p, dhip_ch_t **out,
+                      int64_t *ndof) {
+    return ch_create(c, zero_policy, p, out, ndof, 2);
 }
 
 void dhip_ch_destroy(dhip_ch_t *h) {

@@ -86,20 +86,10 @@ struct P2Shared {
     int nf;
 };
 
-__global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
-    __shared__ P2Shared sh;
-    const int64_t c = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (c >= a.ncut) return;
-    double X[4][3], ls[10];
-    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
-    double G1[4][3];
-    const double det = p1_grad(X, G1);
-    double v[10];
-#pragma unroll
-    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
-
-    // ---- patch facets: sub-tet lt, triangle tr, in the order of make_patch ----
+// Patch facets of the tet (sub-tet lt, triangle tr, in the order of make_patch) into sh.vb / sh.af,
+// lanes 0..nf-1 one facet each.  Returns the facet count, or -1 when a sub-tet lies entirely in
+// the zero level set (the reference's "full body" error).  Uniform over the wave.
+__device__ inline int p2_patch_facets(P2Shared &sh, const double X[4][3], const double ls[10], int lane) {
     int ntri[8], nf = 0;
     bool zero_body = false;
 #pragma unroll
@@ -111,10 +101,7 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
         ntri[lt] = c_tab.rp_size[s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40];
         nf += ntri[lt];
     }
-    if (zero_body) {
-        if (lane == 0) atomicOr(a.err, 2);
-        return;
-    }
+    if (zero_body) return -1;
     if (lane < nf) {
         int lt = 0, tr = lane;
         while (tr >= ntri[lt]) tr -= ntri[lt++];
@@ -134,6 +121,65 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
             for (int cc = 0; cc < 4; ++cc) sh.vb[lane][j][cc] = vb[j][cc];
         sh.af[lane] = af;
     }
+    return nf;
+}
+
+// Stage composite-Quad5 point base + lane (< nq): weight, P2 basis values, surface gradients
+// projected with the P2 level-set normal and (rhs_kind != NONE) the load value.
+__device__ inline void p2_stage_points(P2Shared &sh, int base, int nq, int lane, const double X[4][3],
+                                       const double G1[4][3], const double v[10], int rhs_kind, double rhs_p0) {
+    const int q = base + lane;
+    if (q >= nq) return;
+    const int f = q / 7, i = q - 7 * (q / 7);
+    double nb[4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        double sv = sh.vb[f][0][cc] * c_tab.q5n[i][0];
+        sv += sh.vb[f][1][cc] * c_tab.q5n[i][1];
+        sv += sh.vb[f][2][cc] * c_tab.q5n[i][2];
+        nb[cc] = sv;
+    }
+    sh.w[lane] = sh.af[f] * c_tab.q5w[i];
+    double h[10];
+    p2_basis(nb, h);
+    double n3[3];
+    p2_unit_normal(v, nb, G1, n3);
+#pragma unroll
+    for (int d = 0; d < 10; ++d) {
+        double g[3];
+        p2_grad_one(d, nb, G1, g);
+        const double dn = dot3(g, n3);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) sh.sg[lane][d][cc] = g[cc] - dn * n3[cc];
+        sh.h[lane][d] = h[d];
+    }
+    double fv = 0.;
+    if (rhs_kind != DHIP_RHS_NONE) {
+        double x[3];
+        b2w(X, nb, x);
+        fv = rhs_fun(rhs_kind, rhs_p0, x);
+    }
+    sh.f[lane] = fv;
+}
+
+__global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
+    __shared__ P2Shared sh;
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (c >= a.ncut) return;
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+    double G1[4][3];
+    const double det = p1_grad(X, G1);
+    double v[10];
+#pragma unroll
+    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+
+    const int nf = p2_patch_facets(sh, X, ls, lane);
+    if (nf < 0) {
+        if (lane == 0) atomicOr(a.err, 2);
+        return;
+    }
     const int64_t N = a.ncut;
     double *L = a.loc;
     if (nf == 0) {
@@ -149,39 +195,7 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
     double am = 0., al = 0., hat = 0., rh = 0.;
     const int nq = 7 * nf;
     for (int base = 0; base < nq; base += P2_CHUNK) {
-        const int q = base + lane;
-        if (q < nq) {
-            const int f = q / 7, i = q - 7 * (q / 7);
-            double nb[4];
-#pragma unroll
-            for (int cc = 0; cc < 4; ++cc) {
-                double sv = sh.vb[f][0][cc] * c_tab.q5n[i][0];
-                sv += sh.vb[f][1][cc] * c_tab.q5n[i][1];
-                sv += sh.vb[f][2][cc] * c_tab.q5n[i][2];
-                nb[cc] = sv;
-            }
-            sh.w[lane] = sh.af[f] * c_tab.q5w[i];
-            double h[10];
-            p2_basis(nb, h);
-            double n3[3];
-            p2_unit_normal(v, nb, G1, n3);
-#pragma unroll
-            for (int d = 0; d < 10; ++d) {
-                double g[3];
-                p2_grad_one(d, nb, G1, g);
-                const double dn = dot3(g, n3);
-#pragma unroll
-                for (int cc = 0; cc < 3; ++cc) sh.sg[lane][d][cc] = g[cc] - dn * n3[cc];
-                sh.h[lane][d] = h[d];
-            }
-            double fv = 0.;
-            if (a.rhs_kind != DHIP_RHS_NONE) {
-                double x[3];
-                b2w(X, nb, x);
-                fv = rhs_fun(a.rhs_kind, a.rhs_p0, x);
-            }
-            sh.f[lane] = fv;
-        }
+        p2_stage_points(sh, base, nq, lane, X, G1, v, a.rhs_kind, a.rhs_p0);
         __syncthreads();
         const int cnt = nq - base < P2_CHUNK ? nq - base : P2_CHUNK;
         if (lane < 55) {

@@ -35,7 +35,7 @@ EXPORTS = [
     "dhip_mlp_create", "dhip_mlp_destroy", "dhip_mlp_params", "dhip_mlp_normalization", "dhip_mlp_features",
     "dhip_mlp_targets", "dhip_mlp_predict", "dhip_mlp_grad", "dhip_mlp_train_step", "dhip_ch_history",
     "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned", "dhip_local_p2", "dhip_number_p2if",
-    "dhip_build_p2", "dhip_assemble_p2", "dhip_get_local_p2", "dhip_get_dof_nodes",
+    "dhip_build_p2", "dhip_assemble_p2", "dhip_get_local_p2", "dhip_get_dof_nodes", "dhip_ch_create_p2",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -109,6 +109,7 @@ def load(path=LIB_PATH):
     L.dhip_gmres.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.dhip_gmres_diag.argtypes = [P, P, VP, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.dhip_ch_create.argtypes = [P, C.c_int32, C.POINTER(ChParams), C.POINTER(P), I64]
+    L.dhip_ch_create_p2.argtypes = [P, C.c_int32, C.POINTER(ChParams), C.POINTER(P), I64]
     L.dhip_ch_destroy.argtypes = [P]
     L.dhip_ch_destroy.restype = None
     L.dhip_ch_set_state.argtypes = [P, VP, VP, C.c_int32]
@@ -178,15 +179,18 @@ class Csr:
 
 class CahnHilliard:
     """Device Cahn-Hilliard stepper (CahnHilliardcGP1CL mirror, theta = 1, resting interface).
-    Unknowns [mu; c] on the P1 interface space of the context's mesh and level set."""
+    Unknowns [mu; c] on the P1 (space=1) or P2 (space=2) interface space of the context's mesh
+    and level set."""
 
     def __init__(self, ctx, dt, epsilon, rho, sigma=1.0, S=1.0, restart=50, max_iter=20000, tol=1e-8, rel=True,
-                 zero_policy=ZERO_THROW):
+                 zero_policy=ZERO_THROW, space=1):
         self.ctx, self.L = ctx, ctx.L
+        self.space = space
         self.params = ChParams(dt, epsilon, rho, sigma, S, restart, max_iter, tol, int(rel))
         self.h = C.c_void_p()
         n = C.c_int64()
-        ctx.check(self.L.dhip_ch_create(ctx.h, zero_policy, C.byref(self.params), C.byref(self.h), C.byref(n)))
+        create = self.L.dhip_ch_create_p2 if space == 2 else self.L.dhip_ch_create
+        ctx.check(create(ctx.h, zero_policy, C.byref(self.params), C.byref(self.h), C.byref(n)))
         self.n = n.value
         ctx._csrs.add(self)
 

@@ -248,6 +248,13 @@ typedef struct {
  * starts at zero. */
 int dhip_ch_create(dhip_ctx_t *ctx, int32_t zero_policy, const dhip_ch_params_t *params, dhip_ch_t **ch,
                    int64_t *ndof);
+/* The same stepper on the P2 interface space (dhip_assemble_p2 with coefficients 1): P2 mass,
+ * Laplace-Beltrami and stabilisation blocks, and the mobility matrix with the P2 interpolant of
+ * the nodal mobilities, LM_ij = sum_q w_q (sum_d Mob(c_d) h_d)(P grad h_i . P grad h_j) on the
+ * composite Quad5 rule (the P2 form of LocalLaplaceMobilityP1CL, surfactant/ifacetransp.h:1731-1766,
+ * with LocalAssembler's P2 surface gradients, :1049-1053).  Every other dhip_ch_* call applies. */
+int dhip_ch_create_p2(dhip_ctx_t *ctx, int32_t zero_policy, const dhip_ch_params_t *params, dhip_ch_t **ch,
+                      int64_t *ndof);
 void dhip_ch_destroy(dhip_ch_t *ch);
 int dhip_ch_set_state(dhip_ch_t *ch, const double *c, const double *mu, int32_t where);
 int dhip_ch_get_state(dhip_ch_t *ch, double *c, double *mu, int32_t where);
