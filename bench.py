@@ -201,14 +201,15 @@ def probe_sweep(ctx, ndof, x_dev, zero_dev):
     A.close()
 
 
-def gmres_probe_sweep(cells=96):
+def gmres_probe_sweep(cells=96, space=1):
     """Per-iteration compute/synchronisation split of the persistent GMRES on the first step of the
-    Cahn-Hilliard trajectory (torus, `cells`^3, seed 0) for a sweep of rows per block."""
+    Cahn-Hilliard trajectory (torus, `cells`^3, seed 0, P1 or P2 space) for a sweep of rows per
+    block."""
     from drops_amd import ch_run
     m, phi = ch_run.torus_mesh(cells)
     with D.Context(0) as ctx:
-        ch = ch_run.make_stepper(ctx, m, phi)
-        for rows in (128, 192, 256, 384, 512):
+        ch = ch_run.make_stepper(ctx, m, phi, space=space)
+        for rows in ((128, 192, 256, 384, 512) if space == 1 else (768, 1024, 1536, 2048)):
             ctx.tune("pcg_rows_per_block", rows)
             c0, mu0 = ch_run.initial_state(ch.n, 0)
             ch.set_state(c0, mu0)
@@ -219,6 +220,7 @@ def gmres_probe_sweep(cells=96):
             print(json.dumps(dict(rows_per_block=rows, blocks=int(pr.shape[0]), iters=it,
                                   us_per_iter=float(pr[:, 3].mean() / it),
                                   compute_us_per_iter=float(pr[:, 1].mean() / it),
+                                  spmv_us_per_iter=float(pr[:, 0].mean() / it),
                                   sync_us_per_iter=float(pr[:, 2].mean() / it))), flush=True)
         ch.close()
 
@@ -241,6 +243,8 @@ def main():
     ap.add_argument("--no-ch", action="store_true", help="skip the Cahn-Hilliard trajectory leg")
     ap.add_argument("--ch-cells", type=int, default=96)
     ap.add_argument("--ch-steps", type=int, default=200)
+    ap.add_argument("--ch-space", type=int, default=2, choices=(1, 2),
+                    help="interface space of the Cahn-Hilliard leg (2: P2, the configuration's space)")
     ap.add_argument("--gmres-probe", action="store_true",
                     help="print the compute/sync split of the persistent GMRES on the Cahn-Hilliard system and exit")
     ap.add_argument("--probe-sweep", action="store_true",
@@ -253,7 +257,7 @@ def main():
         sys.exit(2)
 
     if args.gmres_probe:
-        gmres_probe_sweep(args.ch_cells)
+        gmres_probe_sweep(args.ch_cells, args.ch_space)
         return
     n = args.cells
     mesh = KuhnMesh.box(-2.0, 2.0, n, shift=config_shift(4.0 / n))

@@ -391,7 +391,8 @@ void dhip_destroy(dhip_ctx_t *c) {
     DevBuf *bufs[] = {&c->phi_own, &c->cellmask, &c->blockcnt, &c->cellinfo, &c->cut, &c->errflag,
                       &c->loc, &c->flags, &c->first, &c->isown, &c->ownrank, &c->dofs, &c->rowowner,
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
-                      &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage};
+                      &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
+                      &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -1117,17 +1118,19 @@ static int persistent_geometry(dhip_ctx *c, int64_t n, unsigned *nb, int *rpt) {
 
 template <class ArgT>
 static int launch_persistent(dhip_ctx *c, const char *name, const void *const kernels[5], int rpt, unsigned nb,
-                             ArgT &args) {
+                             ArgT &args, size_t lds = LDS_BYTES) {
     const int idx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
     const void *k = kernels[idx];
+    if (lds > (size_t)LDS_BYTES)
+        DHIP_HIPCHK(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, LDS_BYTES));
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, lds));
     if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
         return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
     void *argv[] = {&args};
     ProfScope ps(c, name);
     if (!c->coop_refused) {
-        const hipError_t e = hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream);
+        const hipError_t e = hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, lds, c->stream);
         if (e == hipSuccess) return DHIP_OK;
         // The cooperative-launch resource is per device and can be held by another process.
         // The grid is already capped at one block per CU (and checked against occupancy
@@ -1135,8 +1138,21 @@ static int launch_persistent(dhip_ctx *c, const char *name, const void *const ke
         (void)hipGetLastError();
         c->coop_refused = true;
     }
-    DHIP_HIPCHK(c, hipLaunchKernel(k, dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
+    DHIP_HIPCHK(c, hipLaunchKernel(k, dim3(nb), dim3(PB), argv, lds, c->stream));
     return DHIP_OK;
+}
+
+// Dynamic LDS of a persistent kernel: the device's per-block limit less the kernel's static LDS,
+// at most 128 KiB and never below the CSR-slice capacity.
+static size_t persistent_lds(dhip_ctx *c, const void *k) {
+    int maxs = 0;
+    hipFuncAttributes fa{};
+    if (hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) != hipSuccess ||
+        hipFuncGetAttributes(&fa, k) != hipSuccess)
+        return LDS_BYTES;
+    int64_t dyn = (int64_t)maxs - (int64_t)fa.sharedSizeBytes - 256;
+    dyn = std::min<int64_t>(dyn, 128 * 1024) & ~(int64_t)15;
+    return (size_t)std::max<int64_t>(dyn, LDS_BYTES);
 }
 
 extern "C" {
@@ -1289,7 +1305,35 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     double *outd = reinterpret_cast<double *>(outs + 4);
     GmresArgs ga{n, A->rowptr, A->col, A->val, diag, db, dx, c->kv[15].as<double>(),
                  c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
-                 outs, outd, outs + 1, nullptr};
+                 outs, outd, outs + 1, nullptr, nullptr, LDS_BYTES, nullptr, nullptr, nullptr, 0};
+    if (diag && !left_pc && A->nnz > 0) {
+        DHIP_HIPCHK(c, c->gm_sval.ensure(sizeof(double) * (A->nnz + 1)));
+        DHIP_LAUNCH(c, "gmres_scale", k_scale_vals, blocks_for(A->nnz, 256), 256, 0, A->nnz, A->col, A->val, diag,
+                    c->gm_sval.as<double>());
+        ga.sval = c->gm_sval.as<double>();
+    }
+    // Blocks whose CSR slice exceeds the LDS slice capacity read a SELL-64 copy instead (built
+    // when the average block would not fit).
+    const int64_t per = (n + nb - 1) / nb;
+    if (A->nnz > 0 && (per + 1) * 4 + (A->nnz / nb) * 12 + 16 > (int64_t)LDS_BYTES) {
+        const int spb = (int)((per + 63) / 64);
+        const int64_t nsl = (int64_t)nb * spb;
+        DHIP_HIPCHK(c, c->gm_sell_off.ensure(sizeof(int32_t) * (nsl + 1)));
+        DHIP_LAUNCH(c, "gmres_sell_width", k_sell_width, blocks_for(nsl, 256), 256, 0, n, per, spb, nsl, A->rowptr,
+                    c->gm_sell_off.as<int32_t>());
+        int64_t tot = 0;
+        rc = scan_exclusive_i32(c, c->gm_sell_off.as<int32_t>(), c->gm_sell_off.as<int32_t>(), nsl, &tot);
+        if (rc) return rc;
+        DHIP_HIPCHK(c, c->gm_sell_col.ensure(sizeof(int32_t) * (tot + 1)));
+        DHIP_HIPCHK(c, c->gm_sell_val.ensure(sizeof(double) * (tot + 1)));
+        DHIP_LAUNCH(c, "gmres_sell_fill", k_sell_fill, blocks_for(n, 256), 256, 0, n, per, spb, A->rowptr, A->col,
+                    A->val, (diag && !left_pc) ? diag : nullptr, c->gm_sell_off.as<int32_t>(),
+                    c->gm_sell_col.as<int32_t>(), c->gm_sell_val.as<double>());
+        ga.sell_off = c->gm_sell_off.as<int32_t>();
+        ga.sell_col = c->gm_sell_col.as<int32_t>();
+        ga.sell_val = c->gm_sell_val.as<double>();
+        ga.sell_spb = spb;
+    }
     if (c->probe_on) {
         DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
         ga.probe = c->probe.as<uint64_t>();
@@ -1299,7 +1343,10 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     const void *ks[5] = {regv ? (const void *)k_gmres<1, KREG> : (const void *)k_gmres<1, 0>,
                          (const void *)k_gmres<2, 0>, (const void *)k_gmres<4, 0>, (const void *)k_gmres<8, 0>,
                          (const void *)k_gmres<16, 0>};
-    rc = launch_persistent(c, "gmres", ks, rpt, nb, ga);
+    const int kidx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
+    const size_t lds = regv ? (size_t)LDS_BYTES : persistent_lds(c, ks[kidx]);
+    ga.lds_bytes = (int64_t)lds;
+    rc = launch_persistent(c, "gmres", ks, rpt, nb, ga, lds);
     if (rc) return rc;
     int32_t hs[4];
     double h_resid = 0.0;

@@ -128,6 +128,8 @@ struct dhip_ctx {
     dhip::DevBuf host_stage;  // pinned staging for scalars
     // Krylov work vectors (grow-only: no device allocation once warmed up)
     dhip::DevBuf kv[16];
+    dhip::DevBuf gm_sval;   // right-preconditioned GMRES: a_rc / d_c
+    dhip::DevBuf gm_sell_off, gm_sell_col, gm_sell_val;  // SELL-64 copy for large systems
     // optional per-block wall-clock probe of the persistent solver
     bool probe_on = false;
     int64_t pcg_rows_per_block = 0;   // 0: one row per thread of a 512-thread block

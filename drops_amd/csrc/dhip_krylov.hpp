@@ -747,7 +747,52 @@ struct GmresArgs {
     double *out_resid;
     int *out_status;
     uint64_t *probe;            // optional [gridDim.x][4] wall-clock ticks: setup, compute, sync, total
+    const double *sval;         // right preconditioning: values a_rc / d_c precomputed (or nullptr)
+    int64_t lds_bytes;          // dynamic LDS of the launch
+    // SELL-64 copy of the matrix (values as the slice would hold them) for blocks whose CSR slice
+    // does not fit in LDS; nullptr: such blocks read the CSR arrays
+    const int32_t *sell_off, *sell_col;
+    const double *sell_val;
+    int sell_spb;               // slices per block
 };
+
+// SELL-64 layout of a CSR matrix partitioned into blocks of `per` consecutive rows: the local rows
+// of a block form slices of 64; entry k of local row l sits at off[slice] + 64 k + (l % 64), so the
+// thread-per-row loads of entry k are coalesced over a wave.  width[s] = 64 * longest row.
+__global__ void k_sell_width(int64_t n, int64_t per, int spb, int64_t nslices, const int32_t *rp, int32_t *width) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslices) return;
+    const int64_t b = s / spb, j = s - b * spb;
+    int32_t w = 0;
+    for (int64_t l = 64 * j; l < 64 * j + 64 && l < per; ++l) {
+        const int64_t r = b * per + l;
+        if (r >= n) break;
+        const int32_t len = rp[r + 1] - rp[r];
+        w = len > w ? len : w;
+    }
+    width[s] = 64 * w;
+}
+
+__global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, const int32_t *col,
+                            const double *val, const double *diag, const int32_t *off, int32_t *scol,
+                            double *sval) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t b = r / per, l = r - b * per;
+    const int64_t base = (int64_t)off[b * spb + (l >> 6)] + (l & 63);
+    for (int32_t e = rp[r], k = 0; e < rp[r + 1]; ++e, ++k) {
+        const int32_t c = col[e];
+        scol[base + 64 * (int64_t)k] = c;
+        sval[base + 64 * (int64_t)k] = diag ? val[e] / diag[c] : val[e];
+    }
+}
+
+// a_rc / d_c for right preconditioning (the same quotient the LDS slice loader forms)
+__global__ void k_scale_vals(int64_t nnz, const int32_t *col, const double *val, const double *diag, double *out) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    out[e] = val[e] / diag[col[e]];
+}
 
 constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG, one row per thread)
 
@@ -755,7 +800,7 @@ template <int RPT, int KR>
 __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const uint64_t t_start = a.probe ? wall_clock64() : 0;
-    uint64_t t_sync = 0;
+    uint64_t t_sync = 0, t_spmv = 0;
     __shared__ double Hc[KMAX + 1];          // current Hessenberg column
     __shared__ double Hs[KMAX * KMAX / 2 + KMAX];  // rotated upper triangle, packed by columns
     __shared__ double cs[KMAX], sn[KMAX], sv[KMAX + 1], yv[KMAX];
@@ -766,13 +811,44 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     const int64_t r0 = (int64_t)blockIdx.x * per;
     const int64_t r1 = r0 + per < n ? r0 + per : n;
     const bool right = !a.left;
-    bool scaled;
+    bool in_lds;
     // right preconditioning: the slice holds a_rc / d_c (A M v); left: plain values (M A v)
-    const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, a.val, right ? a.diag : nullptr, r0, r1, scaled);
+    const bool pre = right && a.diag && a.sval;
+    const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, pre ? a.sval : a.val,
+                                              (right && !pre) ? a.diag : nullptr, r0, r1, in_lds);
+    const bool scaled = in_lds || pre;
     const double *dg = a.diag;
     // gathers in batches of 8 when the basis rows occupy registers
     constexpr int GB = KR > 0 ? 8 : MAXR;
+    const bool sell = !in_lds && a.sell_val;
+    const int32_t *sell_col = a.sell_col;
+    const double *sell_val = a.sell_val;
+    const int32_t *sell_off = a.sell_off + (sell ? (int64_t)blockIdx.x * a.sell_spb : 0);
     auto spmv_row = [&](int64_t l, auto &&yv_) {
+        if (sell) {
+            // the same sum as row_product, entries read from the coalesced SELL-64 copy
+            const int64_t base = (int64_t)sell_off[l >> 6] + (l & 63);
+            const int32_t len = a.rp[r0 + l + 1] - a.rp[r0 + l];
+            double s_ = 0.0;
+            for (int32_t q0 = 0; q0 < len; q0 += GB) {
+                const int cnt = len - q0 < GB ? len - q0 : GB;
+                int32_t cc[GB];
+                double vv[GB], xx[GB];
+#pragma unroll
+                for (int j = 0; j < GB; ++j)
+                    if (j < cnt) {
+                        cc[j] = sell_col[base + 64 * (int64_t)(q0 + j)];
+                        vv[j] = sell_val[base + 64 * (int64_t)(q0 + j)];
+                    }
+#pragma unroll
+                for (int j = 0; j < GB; ++j)
+                    if (j < cnt) xx[j] = yv_(cc[j]);
+#pragma unroll
+                for (int j = 0; j < GB; ++j)
+                    if (j < cnt) s_ += vv[j] * xx[j];
+            }
+            return s_;
+        }
         if (scaled || !right || !dg) return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, yv_);
         return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
     };
@@ -809,8 +885,24 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     // shifts the slots down by one and enters at the top, so every register index is static.
     double vreg[REGV ? KR : 1];
     int nvec = 0;
+    // Without a register basis and with the CSR slice in global memory, the first KL basis
+    // vectors of the block's rows are also kept in LDS: the dots and combinations read them there
+    // (each thread reads back only its own rows), the SpMV gathers read the global copy.
+    const int64_t nrb = r1 > r0 ? r1 - r0 : 0;
+    int KL = 0;
+    double *vl = reinterpret_cast<double *>(lds);
+    if constexpr (!REGV) {
+        if (!in_lds && nrb > 0) {
+            const int64_t fit = a.lds_bytes / (8 * nrb);
+            KL = (int)(fit < a.m ? fit : a.m);
+        }
+    }
+    auto vbasis = [&](int q, int64_t l) { return q < KL ? vl[(size_t)q * nrb + l] : a.V[(size_t)q * n + r0 + l]; };
     auto vstore = [&](int q, int64_t r, double v, bool mine) {
-        if (mine) a.V[(size_t)q * n + r] = v;
+        if (mine) {
+            a.V[(size_t)q * n + r] = v;
+            if (q < KL) vl[(size_t)q * nrb + (r - r0)] = v;
+        }
         if constexpr (REGV) {
 #pragma unroll
             for (int t = 0; t < KR - 1; ++t) vreg[t] = vreg[t + 1];
@@ -852,7 +944,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                     if (q < KT) {
 #pragma unroll
                         for (int k = 0; k < RPT; ++k)
-                            if (own[k]) sacc += wr[k] * (q < K ? a.V[(size_t)q * n + r0 + lr[k]] : wr[k]);
+                            if (own[k]) sacc += wr[k] * (q < K ? vbasis(q, lr[k]) : wr[k]);
                     }
                     p[u] = sacc;
                 }
@@ -906,7 +998,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
             for (int c0 = 0; c0 < cnt; c0 += 8) {
                 double vv[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) vv[q] = c0 + q < cnt ? a.V[(size_t)(c0 + q) * n + r] : 0.0;
+                for (int q = 0; q < 8; ++q) vv[q] = c0 + q < cnt ? vbasis(c0 + q, r - r0) : 0.0;
 #pragma unroll
                 for (int q = 0; q < 8; ++q)
                     if (c0 + q < cnt) w -= sign * (h[c0 + q] * vv[q]);
@@ -968,6 +1060,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         for (i = 0; i < m - 1 && j <= a.max_iter; ++i, ++j) {
             const double *vi = a.V + (size_t)i * n;
             // w = A M v_i (right) or M A v_i (left); h1 = V^T w
+            const uint64_t t_sp = a.probe ? wall_clock64() : 0;
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 if (!own[k]) continue;
@@ -975,6 +1068,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
                 if (!right) w = prec(k, w);
                 wr[k] = w;
             }
+            if (a.probe) t_spmv += wall_clock64() - t_sp;
             dots(i + 1, false, msh);
             if (!syncv(i + 1, red_sh)) { status = 2; goto done; }
             if (threadIdx.x <= i) Hc[threadIdx.x] = red_sh[threadIdx.x];
@@ -1070,7 +1164,7 @@ done:
     }
     if (a.probe && threadIdx.x == 0) {
         const uint64_t total = wall_clock64() - t_start;
-        a.probe[blockIdx.x * 4 + 0] = 0;
+        a.probe[blockIdx.x * 4 + 0] = t_spmv;   // part of the compute time spent in the SpMV
         a.probe[blockIdx.x * 4 + 1] = total - t_sync;
         a.probe[blockIdx.x * 4 + 2] = t_sync;
         a.probe[blockIdx.x * 4 + 3] = total;

@@ -92,7 +92,7 @@ int dhip_profile_enable(dhip_ctx_t *ctx, int32_t on);
 int dhip_profile_get(dhip_ctx_t *ctx, const char *name, double *total_ms, int64_t *count);
 int dhip_profile_reset(dhip_ctx_t *ctx);
 /* Wall-clock probe of the persistent FP64 PCG and of GMRES (100 MHz ticks per block:
- * setup, compute phases, synchronisations, total).  dhip_solver_probe(ctx, 1, ...)
+ * setup (GMRES: the SpMV part of the compute time), compute phases, synchronisations, total).  dhip_solver_probe(ctx, 1, ...)
  * arms it for the following solves; `ticks` (4 per block, may be NULL) receives
  * the last solve's values and `nblocks` its block count. */
 int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
