@@ -113,20 +113,24 @@ def cpu_baseline(mesh, phi, coef, lc, budget_s=30.0):
                 threads=threads, reps=len(times), iters=out[1]), out
 
 
-def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=3, with_cpu=True):
-    """Cahn-Hilliard trajectory on the torus (implicit Euler, GMRES(50) + block Jacobi to 1e-8):
-    the previous-step start against the learned start (trained weights from
-    drops_amd/data/ch_warmstart_torus<cells>.npz, produced by `python -m drops_amd.learn`),
-    on a seed the network was not trained on; the CPU port times the first steps."""
+def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=2):
+    """Cahn-Hilliard trajectory on the torus (implicit Euler, GMRES(50) + block Jacobi to 1e-8) on the
+    P2 (or P1) interface space: the previous-step start against the linear extrapolation and the
+    learned start (trained weights from drops_amd/data/ch_warmstart_torus<cells>[_p2].npz, produced
+    by `python -m drops_amd.learn`), on a seed the network was not trained on; the CPU port times
+    the first steps."""
     from drops_amd import ch_run, learn
-    path = os.path.join(learn.DATA_DIR, f"ch_warmstart_torus{cells}.npz")
+    sfx = "_p2" if space == 2 else ""
+    path = os.path.join(learn.DATA_DIR, f"ch_warmstart_torus{cells}{sfx}.npz")
+    if cpu_steps is None:
+        cpu_steps = 1 if space == 2 else 3
     m, phi = ch_run.torus_mesh(cells)
-    out = {"workload": f"P1 trace-FEM Cahn-Hilliard (CahnHilliardcGP1CL, theta 1), torus R=1 r=0.3 on "
-                       f"[-2,2]^3, {cells}^3 Kuhn cells, {steps} implicit Euler steps dt={ch_run.DT} "
-                       f"eps={ch_run.EPS}, GMRES(50) right block-Jacobi to rel. residual 1e-8",
-           "seed": seed}
+    out = {"workload": f"P{space} trace-FEM Cahn-Hilliard (CahnHilliardcGP1CL step, theta 1, P{space} interface "
+                       f"space), torus R=1 r=0.3 on [-2,2]^3, {cells}^3 Kuhn cells, {steps} implicit Euler steps "
+                       f"dt={ch_run.DT} eps={ch_run.EPS}, GMRES(50) right block-Jacobi to rel. residual 1e-8",
+           "seed": seed, "space": f"P{space}"}
     with D.Context(0) as ctx:
-        ch = ch_run.make_stepper(ctx, m, phi)
+        ch = ch_run.make_stepper(ctx, m, phi, space=space)
         out["dofs"] = 2 * ch.n
         if os.path.exists(path):
             mlp, meta = learn.load(ctx, path)
@@ -139,7 +143,8 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=3, with_cpu=True):
         for v, r in ev.items():
             out[v] = {"s_total": float(r["wall"].sum()), "s_per_step": float(r["wall"].mean()),
                       "gmres_iters": int(r["iters"].sum()), "iters_first_steps": r["iters"][:5].tolist(),
-                      "iters_last_steps": r["iters"][-3:].tolist(), "s_first_3_steps": float(r["wall"][:3].sum())}
+                      "iters_last_steps": r["iters"][-3:].tolist(), "s_first_steps": float(r["wall"][:cpu_steps].sum()),
+                      "first_steps": cpu_steps}
             if v != "previous":
                 out[v]["starts"] = r["starts"]
             log(f"[bench] ch {v}: {out[v]['s_total']:.3f} s for {steps} steps, {out[v]['gmres_iters']} GMRES iterations")
@@ -153,18 +158,24 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=3, with_cpu=True):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         om = O.make_mesh(m.orig, m.e, m.n)
-        dof, nd = O.number_p1if(om, phi, kind="omp")
-        r = O.assemble_p1(om, phi, dof, nd, coef=(1.0, 1.0, 1.0), kind="omp")
+        if space == 2:
+            dof, nd = O.number_p2if(om, phi, kind="omp")
+            r = O.assemble_p2(om, phi, dof, nd, coef=(1.0, 1.0, 1.0), kind="omp")
+        else:
+            dof, nd = O.number_p1if(om, phi, kind="omp")
+            r = O.assemble_p1(om, phi, dof, nd, coef=(1.0, 1.0, 1.0), kind="omp")
         c0, mu0 = ch_run.initial_state(nd, seed)
         x = np.concatenate([mu0, c0])
         t0 = time.perf_counter()
         for s in range(cpu_steps):
             O.ch_step_c(om, phi, dof, nd, r["rowptr"], r["colind"], r["mass"], r["lb"], r["stab"], x, ch_run.DT,
-                        ch_run.EPS, m.h, kind="omp")
+                        ch_run.EPS, m.h, kind="omp", space=space)
         cpu_s = time.perf_counter() - t0
         out["cpu_port"] = {"s_first_steps": cpu_s, "steps": cpu_steps,
+                           "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
                            "sample": f"first {cpu_steps} steps of the same seed (the costliest: most GMRES "
                                      f"iterations), oracle OpenMP build"}
+        out["speedup_gpu_vs_cpu_first_steps"] = cpu_s / out["previous"]["s_first_steps"]
     return out
 
 
@@ -353,7 +364,7 @@ def main():
         err["cpu"] = float(np.linalg.norm(xc - xs) / np.linalg.norm(xs))
     ctx.close()
 
-    ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu)
+    ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu, space=args.ch_space)
 
     v = results["fp64"]["s_per_solve"]
     line = {

@@ -56,19 +56,22 @@ class Recorder:
             self.ctx.free(p)
 
 
-def normalisation(ctx, mlp, recs, every=5):
-    """Feature mean/scale and target mean/scale from a subsample of steps."""
+def normalisation(ctx, mlp, recs, every=5, max_samples=4_000_000):
+    """Feature mean/scale and target mean/scale from a subsample of steps (and of DOFs, so that at
+    most about max_samples rows come to the host)."""
     n = recs[0].n
     mlp.set_normalization(np.zeros(NF), np.ones(NF), np.zeros(NO), np.ones(NO))
     Xd = ctx.malloc(4 * NF * n)
     Td = ctx.malloc(4 * NO * n)
     feats, targs = [], []
+    nsel = len(recs) * len(range(1, recs[0].steps, every))
+    stride = max(1, (nsel * n + max_samples - 1) // max_samples)
     for r in recs:
         for s in range(1, r.steps, every):
             mlp.features(r.hist(s), r.res + s * r.bytes, r.pcd + s * r.bytes, n, Xd)
             mlp.targets(r.state(s + 1), r.hist(s, 3), n, Td)
-            feats.append(ctx.to_host(Xd, (n, NF), np.float32).astype(np.float64))
-            targs.append(ctx.to_host(Td, (n, NO), np.float32).astype(np.float64))
+            feats.append(ctx.to_host(Xd, (n, NF), np.float32)[::stride].astype(np.float64))
+            targs.append(ctx.to_host(Td, (n, NO), np.float32)[::stride].astype(np.float64))
     ctx.free(Xd)
     ctx.free(Td)
     F, T = np.concatenate(feats), np.concatenate(targs)
@@ -173,10 +176,14 @@ def load(ctx, path):
     return mlp, json.loads(str(z["meta"]))
 
 
+def default_weights(space):
+    return os.path.join(DATA_DIR, "ch_warmstart_torus96_p2.npz" if space == 2 else "ch_warmstart_torus96.npz")
+
+
 def main(argv=None):
     """Train the warm-start network on seeded torus trajectories and report its effect on unseen
-    seeds:  python -m drops_amd.learn [--cells 96] [--steps 200] [--train-seeds 20]
-    [--test-seeds 5] [--iters 3000] [--out drops_amd/data/ch_warmstart_torus96.npz]"""
+    seeds:  python -m drops_amd.learn [--space 2] [--cells 96] [--steps 200] [--train-seeds 20]
+    [--test-seeds 5] [--iters 3000] [--out drops_amd/data/ch_warmstart_torus96_p2.npz]"""
     import argparse
     ap = argparse.ArgumentParser()
     ap.add_argument("--cells", type=int, default=96)
@@ -185,13 +192,16 @@ def main(argv=None):
     ap.add_argument("--test-seeds", type=int, default=5)
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=65536)
-    ap.add_argument("--out", default=os.path.join(DATA_DIR, "ch_warmstart_torus96.npz"))
+    ap.add_argument("--space", type=int, default=2, choices=(1, 2))
+    ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
+    if a.out is None:
+        a.out = default_weights(a.space)
     log = lambda *x: print(*x, flush=True)
     m, phi = ch_run.torus_mesh(a.cells)
     with D.Context(0) as ctx:
-        ch = ch_run.make_stepper(ctx, m, phi)
-        log(f"[learn] torus {a.cells}^3: {ch.n} DOFs per field, {a.steps} steps")
+        ch = ch_run.make_stepper(ctx, m, phi, space=a.space)
+        log(f"[learn] torus {a.cells}^3, P{a.space}: {ch.n} DOFs per field, {a.steps} steps")
         t0 = time.perf_counter()
         recs = []
         for seed in range(a.train_seeds):
@@ -222,7 +232,7 @@ def main(argv=None):
                             s_extrapolation=float(e["wall"].sum()), s_learned=float(q["wall"].sum()),
                             starts_learned=q["starts"], c_rel_diff=err))
             log(f"[learn] test seed {seed}: {res[-1]}")
-        meta = dict(cells=a.cells, steps=a.steps, train_seeds=list(range(a.train_seeds)), dt=ch_run.DT,
+        meta = dict(space=a.space, cells=a.cells, steps=a.steps, train_seeds=list(range(a.train_seeds)), dt=ch_run.DT,
                     epsilon=ch_run.EPS, iters=a.iters, batch=a.batch, final_loss=hist[-1][1], data_s=t_data,
                     train_s=t_train, test=res)
         os.makedirs(os.path.dirname(a.out), exist_ok=True)
