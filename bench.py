@@ -179,6 +179,89 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=
     return out
 
 
+def batch_leg(nprob=512, seed=0, steps=3, cpu_sample=8, with_cpu=True):
+    """Batched configuration: `nprob` uneven P1 problems (ellipsoids, n^3 cells with n in 16..64) assembled
+    and solved (FP32 inner PCG, FP64 refinement, rel. residual 1e-8) in one persistent launch replayed as a
+    hipGraph; workgroup geometry from the autotuner; a sample of problems checked against the oracle's FP64
+    solution and timed on the CPU port."""
+    from drops_amd import batch as BT
+    t0 = time.perf_counter()
+    probs = BT.make_problems(nprob, seed)
+    phi, off = BT.level_sets(probs)
+    meshes = [p[0] for p in probs]
+    log(f"[bench] batch: {nprob} problems, {phi.size} level-set values generated in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": f"{nprob} P1 trace-FEM problems (mass + LB + h*stab, f = 3xy - x^3) on ellipsoids, semi-axes "
+                       f"U(0.5,1.5), random rotations, [-2,2]^3 Kuhn meshes n^3 with n in 16..64 (seed {seed}); "
+                       f"assembly + Jacobi PCG (FP32 inner, FP64 refinement) to rel. residual 1e-8 in one launch"}
+    with D.Context(0) as ctx:
+        b = ctx.batch(meshes, phi, off, rhs_kind=D.RHS_3XY_MINUS_X3)
+        choice, timings = tune.tune_batch(ctx, b)
+        out["tuned"] = {"threads": choice[0], "blocks_per_cu": choice[1],
+                        "candidates_ms": {f"{k[0]}x{k[1]}": 1e3 * v for k, v in timings.items()}}
+        b.solve(tol=1e-8, graph=True)   # capture
+        ctx.profile_reset()
+        ctx.profile(True)
+        ts = []
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            b.solve(tol=1e-8, graph=True)
+            ts.append(time.perf_counter() - t1)
+        ctx.profile(False)
+        gms, gcnt = ctx.profile_get("batch_graph")
+        st = b.stats()
+        dur = st["end_s"] - st["start_s"]
+        span = st["end_s"].max() - st["start_s"].min()
+        n_ok = int((st["status"] == D.OK).sum())
+        # algorithmic bytes: level set read once, then per inner iteration the FP32 matrix (value +
+        # column) and five FP32 vectors, per refinement the FP64 matrix and vectors
+        alg = 0.0
+        for p in range(nprob):
+            alg += 8.0 * float(np.prod(meshes[p].halfgrid_shape))
+            alg += st["iters"][p] * (8.0 * st["nnz"][p] + 28.0 * st["ndof"][p])
+            alg += (st["outer"][p] + 1) * (12.0 * st["nnz"][p] + 32.0 * st["ndof"][p])
+        s_batch = gms / max(gcnt, 1) / 1e3
+        out.update({"s_per_batch": s_batch, "s_per_batch_wall": float(np.median(ts)),
+                    "s_per_surface_solve": s_batch / nprob, "problems_ok": n_ok,
+                    "dofs_total": int(st["ndof"].sum()), "nnz_total": int(st["nnz"].sum()),
+                    "cut_tets_total": int(st["ncut"].sum()), "dofs_range": [int(st["ndof"].min()), int(st["ndof"].max())],
+                    "inner_iters_mean": float(st["iters"].mean()), "refinements_mean": float(st["outer"].mean()),
+                    "resid_max": float(st["resid"].max()), "device_span_s": float(span),
+                    "longest_problem_s": float(dur.max()), "mean_problem_s": float(dur.mean()),
+                    "assembly_share_longest": float(((st["assembled_s"] - st["start_s"]) / np.maximum(dur, 1e-12))[dur.argmax()]),
+                    "algorithmic_gbs": alg / s_batch / 1e9, "algorithmic_frac_of_hbm_peak": alg / s_batch / 8e12})
+        log(f"[bench] batch: {s_batch * 1e3:.2f} ms per batch of {nprob} ({n_ok} converged), "
+            f"longest problem {dur.max() * 1e3:.1f} ms, mean {dur.mean() * 1e3:.1f} ms")
+        if with_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            errs, cpu_t, cpu_dofs = [], [], 0
+            for p in range(cpu_sample):
+                m = meshes[p]
+                sz = int(np.prod(m.halfgrid_shape))
+                ph = phi[off[p]:off[p] + sz].reshape(m.halfgrid_shape)
+                om = O.make_mesh(m.orig, m.e, m.n)
+                t1 = time.perf_counter()
+                dof, nd = O.number_p1if(om, ph, kind="omp")
+                r = O.assemble_p1(om, ph, dof, nd, coef=(1.0, 1.0, 1.0), rhs_kind=O.RHS_3XY_MINUS_X3, kind="omp")
+                comb = (1.0 * (r["mass"] + r["lb"])) + m.h * r["stab"]
+                xo, it, res, ok = O.pcg_jacobi(r["rowptr"], r["colind"], comb, r["rhs"], tol=1e-8, kind="omp")
+                cpu_t.append(time.perf_counter() - t1)
+                cpu_dofs += nd
+                xt, _, _, _ = O.pcg_jacobi(r["rowptr"], r["colind"], comb, r["rhs"], tol=1e-14, max_iter=100000)
+                g = b.problem(p)
+                errs.append(float(np.linalg.norm(g["x"] - xt) / np.linalg.norm(xt)))
+            per = float(np.mean(cpu_t))
+            out["err_vs_fp64_oracle_max"] = max(errs)
+            out["cpu_port"] = {"s_per_problem_mean": per, "sample": f"problems 0..{cpu_sample - 1} of the batch "
+                               f"({cpu_dofs} DOFs, mean n {np.mean([meshes[p].n[0] for p in range(cpu_sample)]):.1f} "
+                               f"vs {np.mean([m.n[0] for m in meshes]):.1f} for the batch), oracle OpenMP build",
+                               "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                               "s_per_batch_est": per * nprob}
+            out["speedup_vs_cpu_est"] = per * nprob / s_batch
+        b.close()
+    return out
+
+
 def measured_traffic(kernel, rows, ndof):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
     each scaled by a known-size copy), when it was measured at this block geometry and system size."""
@@ -252,6 +335,8 @@ def main():
                     help="also run one device-to-device copy of this many MiB (the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--no-ch", action="store_true", help="skip the Cahn-Hilliard trajectory leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems leg")
+    ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--ch-cells", type=int, default=96)
     ap.add_argument("--ch-steps", type=int, default=200)
     ap.add_argument("--ch-space", type=int, default=2, choices=(1, 2),
@@ -365,6 +450,7 @@ def main():
     ctx.close()
 
     ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu, space=args.ch_space)
+    bt = None if args.no_batch else batch_leg(args.batch_problems, with_cpu=not args.no_cpu)
 
     v = results["fp64"]["s_per_solve"]
     line = {
@@ -394,6 +480,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pcg64", rows_best, ndof),
                      "bytes_per_iter": bytes_per_iter, "iters": iters, "avg_launch_us": pk["avg_us"]},
         "learned": ch,
+        "batch": bt,
     }
     if cpu is not None:
         line["cpu_baseline"] = {"value": cpu["value"], "unit": "s per solve", "cores": cpu["threads"],

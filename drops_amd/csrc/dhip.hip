@@ -8,6 +8,7 @@
 #include "dhip_ch.hpp"
 #include "dhip_ch_p2.hpp"
 #include "dhip_mlp.hpp"
+#include "dhip_batch.hpp"
 
 #include <algorithm>
 #include <mutex>
@@ -1270,6 +1271,17 @@ int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
         c->pcg_rows_per_block = value;
         return DHIP_OK;
     }
+    if (!std::strcmp(key, "batch_threads")) {
+        if (value != 128 && value != 256 && value != 512)
+            return set_error(c, DHIP_E_ARG, "batch_threads must be 128, 256 or 512");
+        c->batch_threads = (int)value;
+        return DHIP_OK;
+    }
+    if (!std::strcmp(key, "batch_blocks_per_cu")) {
+        if (value < 1 || value > 8) return set_error(c, DHIP_E_ARG, "batch_blocks_per_cu must be 1..8");
+        c->batch_blocks_per_cu = (int)value;
+        return DHIP_OK;
+    }
     return set_error(c, DHIP_E_ARG, "unknown tuning key %s", key);
 }
 
@@ -2105,6 +2117,329 @@ int dhip_ch_solve_learned(dhip_ch_t *h, dhip_mlp_t *q, int32_t *iters, double *r
     if (start) DHIP_HIPCHK(c, hipMemcpyAsync(start, flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
     // the verified stepper path: x_n into the history, the start into the state, GMRES
     return dhip_ch_solve(h, h->xstart.as<double>(), DHIP_DEVICE, iters, resid);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched problems: planning, one persistent launch per solve (optionally a replayed hipGraph)
+// ---------------------------------------------------------------------------
+struct dhip_batch {
+    dhip_ctx *ctx = nullptr;
+    int32_t nprob = 0;
+    std::vector<BatchProb> hprob;
+    DevBuf prob, order, queue, phi_own;
+    DevBuf cellinfo, cut, flags, loc, first, slot, dofs, rowowner, rowlen, rowptr, col;
+    DevBuf vm, vl, vs, vc, rhs, diag, x, r64, d32, r32, z32, p32, q32, scol, sval, soff, stat;
+    BatchArgs args{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int g_threads = 0, g_grid = 0, g_max_iter = 0, g_rel = 0;
+    int64_t g_lds = 0;
+    double g_tol = 0.0;
+};
+
+static void batch_release(dhip_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    DevBuf *bufs[] = {&b->prob, &b->order, &b->queue, &b->phi_own, &b->cellinfo, &b->cut, &b->flags, &b->loc,
+                      &b->first, &b->slot, &b->dofs, &b->rowowner, &b->rowlen, &b->rowptr, &b->col, &b->vm,
+                      &b->vl, &b->vs, &b->vc, &b->rhs, &b->diag, &b->x, &b->r64, &b->d32, &b->r32,
+                      &b->z32, &b->p32, &b->q32, &b->scol, &b->sval, &b->soff, &b->stat};
+    for (DevBuf *d : bufs) d->release();
+}
+
+static const void *batch_kernel(int threads) {
+    return threads == 128 ? (const void *)k_batch<128>
+                          : threads == 512 ? (const void *)k_batch<512> : (const void *)k_batch<256>;
+}
+
+extern "C" {
+
+int dhip_batch_create(dhip_ctx_t *c, int32_t nprob, const dhip_batch_problem_t *probs, const double *phi,
+                      int64_t phi_count, int32_t where, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                      const double *rhs_param, double omit_bound, dhip_batch_t **out) {
+    if (!c || !probs || !phi || !out || nprob <= 0 || nprob > 65535) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    std::vector<BatchProb> hp(nprob);
+    int64_t cells = 0, max_cells = 0, max_verts = 0;
+    for (int32_t i = 0; i < nprob; ++i) {
+        const dhip_batch_problem_t &q = probs[i];
+        for (int k = 0; k < 3; ++k)
+            if (q.n[k] <= 0) return set_error(c, DHIP_E_ARG, "problem %d: cell counts must be positive", i);
+        const int64_t ncell = (int64_t)q.n[0] * q.n[1] * q.n[2];
+        const int64_t hg = (2LL * q.n[0] + 1) * (2LL * q.n[1] + 1) * (2LL * q.n[2] + 1);
+        if (ncell * 6 >= ((int64_t)1 << 26))
+            return set_error(c, DHIP_E_CAPACITY, "problem %d: %lld tets exceed the 2^26 supported", i,
+                             (long long)(ncell * 6));
+        if (q.phi_offset < 0 || q.phi_offset + hg > phi_count)
+            return set_error(c, DHIP_E_ARG, "problem %d: level set [%lld, %lld) outside the %lld values", i,
+                             (long long)q.phi_offset, (long long)(q.phi_offset + hg), (long long)phi_count);
+        MeshP &m = hp[i].m;
+        for (int cc = 0; cc < 3; ++cc) {
+            m.orig[cc] = q.orig[cc];
+            for (int k = 0; k < 3; ++k) m.off[k][cc] = 1.0 / (double)q.n[k] * q.e[k * 3 + cc];  // geom/builder.cpp:108-110
+            m.n[cc] = q.n[cc];
+        }
+        m.ncell = ncell;
+        for (int k = 0; k < 3; ++k) hp[i].lc[k] = q.lc[k];
+        hp[i].phi_off = q.phi_offset;
+        hp[i].cell_off = cells;
+        cells += ncell;
+        max_cells = std::max(max_cells, ncell);
+        max_verts = std::max<int64_t>(max_verts, (int64_t)(q.n[0] + 1) * (q.n[1] + 1) * (q.n[2] + 1));
+    }
+    dhip_batch *b = new dhip_batch();
+    b->ctx = c;
+    b->nprob = nprob;
+    auto fail = [&](int code) {
+        batch_release(b);
+        delete b;
+        return code;
+    };
+    const double *dphi = phi;
+    if (where != DHIP_DEVICE) {
+        if (b->phi_own.ensure(sizeof(double) * phi_count) ||
+            hipMemcpyAsync(b->phi_own.p, phi, sizeof(double) * phi_count, hipMemcpyHostToDevice, c->stream))
+            return fail(set_error(c, DHIP_E_HIP, "level-set upload failed"));
+        dphi = b->phi_own.as<double>();
+    }
+    // ---- planning: cut tets and cut-tet vertices per problem ----
+    DevBuf mask, cnt;
+    if (mask.ensure(cells + 8) || cnt.ensure(sizeof(int32_t) * 3 * nprob) ||
+        b->prob.ensure(sizeof(BatchProb) * nprob))
+        return fail(set_error(c, DHIP_E_NOMEM, "batch planning buffers"));
+    auto cleanup_plan = [&]() {
+        mask.release();
+        cnt.release();
+    };
+    if (hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * 3 * nprob, c->stream) ||
+        hipMemcpyAsync(b->prob.p, hp.data(), sizeof(BatchProb) * nprob, hipMemcpyHostToDevice, c->stream)) {
+        cleanup_plan();
+        return fail(set_error(c, DHIP_E_HIP, "batch planning setup failed"));
+    }
+    int32_t *ncut_d = cnt.as<int32_t>(), *zero_d = ncut_d + nprob, *nvert_d = zero_d + nprob;
+    hipLaunchKernelGGL(k_batch_cells, dim3(blocks_for(max_cells, BPLAN), nprob), dim3(BPLAN), 0, c->stream,
+                       b->prob.as<BatchProb>(), dphi, mask.as<uint8_t>(), ncut_d, zero_d);
+    hipLaunchKernelGGL(k_batch_verts, dim3(blocks_for(max_verts, BPLAN), nprob), dim3(BPLAN), 0, c->stream,
+                       b->prob.as<BatchProb>(), (const uint8_t *)mask.p, nvert_d);
+    std::vector<int32_t> hc(3 * nprob);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hc.data(), cnt.p, sizeof(int32_t) * 3 * nprob, hipMemcpyDeviceToHost, c->stream) ||
+        hipStreamSynchronize(c->stream)) {
+        cleanup_plan();
+        return fail(set_error(c, DHIP_E_HIP, "batch planning failed"));
+    }
+    cleanup_plan();
+    int64_t cut_tot = 0, row_tot = 0, nz_tot = 0, sell_tot = 0, slice_tot = 0;
+    for (int32_t i = 0; i < nprob; ++i) {
+        BatchProb &P = hp[i];
+        P.ncut_cap = hc[i];
+        P.row_cap = hc[2 * nprob + i];
+        P.nz_cap = 15 * P.row_cap;
+        P.cut_off = cut_tot;
+        P.row_off = row_tot;
+        P.nz_off = nz_tot;
+        P.sell_off = sell_tot;
+        P.slice_off = slice_tot;
+        cut_tot += P.ncut_cap;
+        row_tot += P.row_cap;
+        nz_tot += P.nz_cap;
+        sell_tot += 15 * 64 * ((P.row_cap + 63) / 64);
+        slice_tot += (P.row_cap + 63) / 64 + 1;
+        if (P.nz_cap >= ((int64_t)1 << 31)) return fail(set_error(c, DHIP_E_CAPACITY, "problem %d too large", i));
+    }
+    b->hprob = hp;
+    // largest first: the longest-processing-time order balances uneven problems over the workgroups
+    std::vector<int32_t> ord(nprob);
+    for (int32_t i = 0; i < nprob; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return hc[x] > hc[y]; });
+    bool ok = !b->order.ensure(sizeof(int32_t) * nprob) && !b->queue.ensure(64) &&
+              !b->cellinfo.ensure(sizeof(uint32_t) * (cells + 1)) && !b->cut.ensure(sizeof(int64_t) * (cut_tot + 1)) &&
+              !b->flags.ensure(cut_tot + 1) && !b->loc.ensure(sizeof(double) * LOC_N * (cut_tot + 1)) &&
+              !b->first.ensure(sizeof(int32_t) * 4 * (cut_tot + 1)) && !b->slot.ensure(sizeof(int32_t) * 4 * (cut_tot + 1)) &&
+              !b->dofs.ensure(sizeof(int32_t) * 4 * (cut_tot + 1)) && !b->rowowner.ensure(sizeof(int32_t) * (row_tot + 1)) &&
+              !b->rowlen.ensure(sizeof(int32_t) * (row_tot + 1)) &&
+              !b->rowptr.ensure(sizeof(int32_t) * (row_tot + nprob + 1)) && !b->col.ensure(sizeof(int32_t) * (nz_tot + 1)) &&
+              !b->vm.ensure(sizeof(double) * (nz_tot + 1)) && !b->vl.ensure(sizeof(double) * (nz_tot + 1)) &&
+              !b->vs.ensure(sizeof(double) * (nz_tot + 1)) && !b->vc.ensure(sizeof(double) * (nz_tot + 1)) &&
+              !b->scol.ensure(sizeof(int32_t) * (sell_tot + 1)) && !b->sval.ensure(sizeof(float) * (sell_tot + 1)) &&
+              !b->soff.ensure(sizeof(int32_t) * (slice_tot + 1)) && !b->rhs.ensure(sizeof(double) * (row_tot + 1)) &&
+              !b->diag.ensure(sizeof(double) * (row_tot + 1)) && !b->x.ensure(sizeof(double) * (row_tot + 1)) &&
+              !b->r64.ensure(sizeof(double) * (row_tot + 1)) && !b->d32.ensure(sizeof(float) * (row_tot + 1)) &&
+              !b->r32.ensure(sizeof(float) * (row_tot + 1)) && !b->z32.ensure(sizeof(float) * (row_tot + 1)) &&
+              !b->p32.ensure(sizeof(float) * (row_tot + 1)) && !b->q32.ensure(sizeof(float) * (row_tot + 1)) &&
+              !b->stat.ensure(sizeof(BatchStat) * nprob);
+    if (!ok) return fail(set_error(c, DHIP_E_NOMEM, "batch buffers (%lld cut tets, %lld rows, %lld entries)",
+                                   (long long)cut_tot, (long long)row_tot, (long long)nz_tot));
+    if (hipMemcpyAsync(b->prob.p, hp.data(), sizeof(BatchProb) * nprob, hipMemcpyHostToDevice, c->stream) ||
+        hipMemcpyAsync(b->order.p, ord.data(), sizeof(int32_t) * nprob, hipMemcpyHostToDevice, c->stream) ||
+        hipMemsetAsync(b->stat.p, 0, sizeof(BatchStat) * nprob, c->stream) || hipStreamSynchronize(c->stream))
+        return fail(set_error(c, DHIP_E_HIP, "batch table upload failed"));
+    BatchArgs &a = b->args;
+    a.prob = b->prob.as<BatchProb>();
+    a.order = b->order.as<int32_t>();
+    a.nprob = nprob;
+    a.queue = b->queue.as<int32_t>();
+    a.phi = dphi;
+    a.cellinfo = b->cellinfo.as<uint32_t>();
+    a.cut = b->cut.as<int64_t>();
+    a.flags = b->flags.as<uint8_t>();
+    a.loc = b->loc.as<double>();
+    a.first = b->first.as<int32_t>();
+    a.slot = b->slot.as<int32_t>();
+    a.dofs = b->dofs.as<int32_t>();
+    a.rowowner = b->rowowner.as<int32_t>();
+    a.rowlen = b->rowlen.as<int32_t>();
+    a.rowptr = b->rowptr.as<int32_t>();
+    a.col = b->col.as<int32_t>();
+    a.vm = b->vm.as<double>();
+    a.vl = b->vl.as<double>();
+    a.vs = b->vs.as<double>();
+    a.vc = b->vc.as<double>();
+    a.rhs = b->rhs.as<double>();
+    a.diag = b->diag.as<double>();
+    a.x = b->x.as<double>();
+    a.r64 = b->r64.as<double>();
+    a.d32 = b->d32.as<float>();
+    a.r32 = b->r32.as<float>();
+    a.z32 = b->z32.as<float>();
+    a.p32 = b->p32.as<float>();
+    a.q32 = b->q32.as<float>();
+    a.scol = b->scol.as<int32_t>();
+    a.sval = b->sval.as<float>();
+    a.soff = b->soff.as<int32_t>();
+    a.stat = b->stat.as<BatchStat>();
+    a.coef0 = coef ? coef[0] : 1.0;
+    a.coef1 = coef ? coef[1] : 1.0;
+    a.coef2 = coef ? coef[2] : 1.0;
+    a.rhs_kind = rhs_kind;
+    a.rhs_p0 = rhs_param ? rhs_param[0] : 0.0;
+    a.omit_bound = omit_bound;
+    a.zero_throw = zero_policy == DHIP_ZERO_THROW;
+    a.inner_tol = 1e-4;
+    a.inner_max = 400;
+    *out = b;
+    return DHIP_OK;
+}
+
+void dhip_batch_destroy(dhip_batch_t *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_release(b);
+    delete b;
+}
+
+int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel, int32_t use_graph) {
+    if (!b || max_iter < 1 || !(tol > 0.0)) return DHIP_E_ARG;
+    dhip_ctx *c = b->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int threads = c->batch_threads;
+    const void *k = batch_kernel(threads);
+    // dynamic LDS for the search direction: what the requested residency leaves of the CU's LDS
+    int maxs = 0;
+    hipFuncAttributes fa{};
+    DHIP_HIPCHK(c, hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+    DHIP_HIPCHK(c, hipFuncGetAttributes(&fa, k));
+    int64_t lds = ((int64_t)maxs / c->batch_blocks_per_cu - (int64_t)fa.sharedSizeBytes - 1024) & ~(int64_t)255;
+    lds = std::max<int64_t>(0, std::min<int64_t>(lds, 128 * 1024));
+    if (lds > 64 * 1024) DHIP_HIPCHK(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    int per_cu = 0;
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, (size_t)lds));
+    if (per_cu < 1) return set_error(c, DHIP_E_HIP, "batched solve kernel cannot be resident");
+    const int grid = (int)std::min<int64_t>(b->nprob, (int64_t)c->cu_count * std::min(per_cu, c->batch_blocks_per_cu));
+    b->args.lds_floats = lds / 4;
+    b->args.max_iter = max_iter;
+    b->args.tol = tol;
+    b->args.rel = rel;
+    BatchArgs a = b->args;
+    void *argv[] = {&a};
+    auto enqueue = [&]() -> hipError_t {
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, b->queue.as<uint32_t>(), 16);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        return hipLaunchKernel(k, dim3(grid), dim3(threads), argv, (size_t)lds, c->stream);
+    };
+    if (!use_graph) {
+        ProfScope ps(c, "batch");
+        DHIP_HIPCHK(c, enqueue());
+    } else {
+        const bool same = b->exec && b->g_threads == threads && b->g_grid == grid && b->g_max_iter == max_iter &&
+                          b->g_tol == tol && b->g_rel == rel && b->g_lds == lds;
+        if (!same) {
+            if (b->exec) (void)hipGraphExecDestroy(b->exec);
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->exec = nullptr;
+            b->graph = nullptr;
+            DHIP_HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+            DHIP_HIPCHK(c, e);
+            DHIP_HIPCHK(c, e2);
+            b->graph = g;
+            DHIP_HIPCHK(c, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+            b->g_threads = threads;
+            b->g_grid = grid;
+            b->g_max_iter = max_iter;
+            b->g_tol = tol;
+            b->g_rel = rel;
+            b->g_lds = lds;
+        }
+        ProfScope ps(c, "batch_graph");
+        DHIP_HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_batch_stats(dhip_batch_t *b, dhip_batch_stat_t *out) {
+    if (!b || !out) return DHIP_E_ARG;
+    dhip_ctx *c = b->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    std::vector<BatchStat> st(b->nprob);
+    DHIP_HIPCHK(c, hipMemcpyAsync(st.data(), b->stat.p, sizeof(BatchStat) * b->nprob, hipMemcpyDeviceToHost,
+                                  c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int32_t i = 0; i < b->nprob; ++i) {
+        out[i].status = st[i].status;
+        out[i].ncut = st[i].ncut;
+        out[i].ndof = st[i].ndof;
+        out[i].nnz = st[i].nnz;
+        out[i].iters = st[i].iters;
+        out[i].outer = st[i].outer;
+        out[i].resid = st[i].resid;
+        out[i].start_s = (double)st[i].t0 * 1e-8;
+        out[i].end_s = (double)st[i].t1 * 1e-8;
+        out[i].assembled_s = (double)st[i].t_asm * 1e-8;
+    }
+    return DHIP_OK;
+}
+
+int dhip_batch_problem(dhip_batch_t *b, int32_t p, int32_t *rowptr, int32_t *colind, double *comb, double *rhs,
+                       double *x) {
+    if (!b || p < 0 || p >= b->nprob) return DHIP_E_ARG;
+    dhip_ctx *c = b->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    BatchStat st{};
+    DHIP_HIPCHK(c, hipMemcpyAsync(&st, b->stat.as<BatchStat>() + p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (st.status != DHIP_OK && st.status != DHIP_E_NOT_CONVERGED)
+        return set_error(c, DHIP_E_STATE, "problem %d has no assembled system (status %d)", p, st.status);
+    const BatchProb &P = b->hprob[p];
+    const size_t nd = (size_t)st.ndof, nz = (size_t)st.nnz;
+    int rc = 0;
+    rc = rc ? rc : d2h(c, rowptr, b->rowptr.as<int32_t>() + P.row_off + p, sizeof(int32_t) * (nd + 1));
+    rc = rc ? rc : d2h(c, colind, b->col.as<int32_t>() + P.nz_off, sizeof(int32_t) * nz);
+    rc = rc ? rc : d2h(c, comb, b->vc.as<double>() + P.nz_off, sizeof(double) * nz);
+    rc = rc ? rc : d2h(c, rhs, b->rhs.as<double>() + P.row_off, sizeof(double) * nd);
+    rc = rc ? rc : d2h(c, x, b->x.as<double>() + P.row_off, sizeof(double) * nd);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
 }
 
 }  // extern "C"

@@ -132,14 +132,16 @@ struct ClassArgs {
     int zero_throw;
 };
 
-__global__ void __launch_bounds__(CLS_BLOCK) k_classify(ClassArgs a) {
-    const int64_t cell = (int64_t)blockIdx.x * CLS_BLOCK + threadIdx.x;
+// 6-bit cut mask of brick cell `cell` (equalSigns on each Kuhn tet's 10 values); `zero` reports an
+// exact-zero node in a cut tet
+__device__ inline int cell_cut_mask(const MeshP &m, const double *phi, int64_t cell, bool &zero) {
     int mask = 0;
-    if (cell < a.m.ncell) {
+    zero = false;
+    {
         int64_t i1, i2, i3;
-        cell_of(a.m, cell, i1, i2, i3);
-        const int64_t h1 = 2 * (int64_t)a.m.n[0] + 1, h2 = 2 * (int64_t)a.m.n[1] + 1;
-        const double *base = a.phi + ((2 * i3) * h2 + 2 * i2) * h1 + 2 * i1;
+        cell_of(m, cell, i1, i2, i3);
+        const int64_t h1 = 2 * (int64_t)m.n[0] + 1, h2 = 2 * (int64_t)m.n[1] + 1;
+        const double *base = phi + ((2 * i3) * h2 + 2 * i2) * h1 + 2 * i1;
         uint32_t pos = 0, neg = 0;
 #pragma unroll
         for (int z = 0; z < 3; ++z)
@@ -153,7 +155,6 @@ __global__ void __launch_bounds__(CLS_BLOCK) k_classify(ClassArgs a) {
                     neg |= (v < 0.) ? (1u << b) : 0u;
                 }
         if (pos != 0x7FFFFFFu && neg != 0x7FFFFFFu) {
-            bool zero = false;
 #pragma unroll
             for (int k = 0; k < 6; ++k) {
                 const uint32_t tm = c_tab.tetmask27[k];
@@ -162,8 +163,18 @@ __global__ void __launch_bounds__(CLS_BLOCK) k_classify(ClassArgs a) {
                     zero |= ((pos | neg) & tm) != tm;
                 }
             }
-            if (zero && a.zero_throw) atomicOr(a.err, 1);
         }
+    }
+    return mask;
+}
+
+__global__ void __launch_bounds__(CLS_BLOCK) k_classify(ClassArgs a) {
+    const int64_t cell = (int64_t)blockIdx.x * CLS_BLOCK + threadIdx.x;
+    int mask = 0;
+    if (cell < a.m.ncell) {
+        bool zero;
+        mask = cell_cut_mask(a.m, a.phi, cell, zero);
+        if (zero && a.zero_throw) atomicOr(a.err, 1);
         a.cellmask[cell] = (uint8_t)mask;
     }
     int tot;
@@ -322,9 +333,7 @@ struct LocalArgs {
 
 constexpr int LOCAL_BLOCK = 64;
 
-__global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
-    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
-    if (c >= a.ncut) return;
+__device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
     double X[4][3], ls[10];
     load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
     double G1[4][3];
@@ -497,6 +506,12 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
     }
 }
 
+__global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
+    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+    if (c >= a.ncut) return;
+    local_p1_one(a, c);
+}
+
 // ---------------------------------------------------------------------------
 // interface numbering
 // ---------------------------------------------------------------------------
@@ -509,9 +524,7 @@ struct QualArgs {
     double omit_bound;
 };
 
-__global__ void k_qualify(QualArgs a) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.ncut) return;
+__device__ inline void qualify_one(const QualArgs &a, int64_t c) {
     int f = a.flags[c] & 1;
     if (f) {
         double limit = 0.;
@@ -530,6 +543,12 @@ __global__ void k_qualify(QualArgs a) {
             if (a.loc[(int64_t)(LOC_HAT + i) * a.ncut + c] > limit) f |= 2 << i;
     }
     a.flags[c] = (uint8_t)f;
+}
+
+__global__ void k_qualify(QualArgs a) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.ncut) return;
+    qualify_one(a, c);
 }
 
 // Visit the tets containing grid vertex (x1,x2,x3) in ascending tet id; the
@@ -575,9 +594,7 @@ struct FirstArgs {
     int32_t *isown;
 };
 
-__global__ void k_first(FirstArgs a) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= 4 * a.ncut) return;
+__device__ inline void first_one(const FirstArgs &a, int64_t s) {
     int64_t x1, x2, x3;
     slot_vertex(a.m, a.cut, s, x1, x2, x3);
     int64_t found = -1;
@@ -590,6 +607,12 @@ __global__ void k_first(FirstArgs a) {
     });
     a.first[s] = (int32_t)found;
     a.isown[s] = found == s ? 1 : 0;
+}
+
+__global__ void k_first(FirstArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= 4 * a.ncut) return;
+    first_one(a, s);
 }
 
 struct DofArgs {
@@ -628,9 +651,7 @@ struct RowArgs {
     double lc0, lc1, lc2;
 };
 
-__global__ void k_rowcount(RowArgs a) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.ndof) return;
+__device__ inline void rowcount_one(const RowArgs &a, int64_t r) {
     int64_t x1, x2, x3;
     slot_vertex(a.m, a.cut, a.rowowner[r], x1, x2, x3);
     uint32_t present = 0;
@@ -643,9 +664,13 @@ __global__ void k_rowcount(RowArgs a) {
     a.rowlen[r] = __popc(present);
 }
 
-__global__ void k_rowfill(RowArgs a) {
+__global__ void k_rowcount(RowArgs a) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.ndof) return;
+    rowcount_one(a, r);
+}
+
+__device__ inline void rowfill_one(const RowArgs &a, int64_t r) {
     int64_t x1, x2, x3;
     slot_vertex(a.m, a.cut, a.rowowner[r], x1, x2, x3);
     uint32_t present = 0;
@@ -699,6 +724,12 @@ __global__ void k_rowfill(RowArgs a) {
         if (cols[s] == r) a.diag[r] = cv;
     }
     a.rhs[r] = rh;
+}
+
+__global__ void k_rowfill(RowArgs a) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.ndof) return;
+    rowfill_one(a, r);
 }
 
 }  // namespace dhip

@@ -36,6 +36,7 @@ EXPORTS = [
     "dhip_mlp_targets", "dhip_mlp_predict", "dhip_mlp_grad", "dhip_mlp_train_step", "dhip_ch_history",
     "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned", "dhip_local_p2", "dhip_number_p2if",
     "dhip_build_p2", "dhip_assemble_p2", "dhip_get_local_p2", "dhip_get_dof_nodes", "dhip_ch_create_p2",
+    "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -45,6 +46,19 @@ class ChParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("epsilon", C.c_double), ("rho", C.c_double), ("sigma", C.c_double),
                 ("S", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("rel", C.c_int32)]
+
+
+class BatchProblem(C.Structure):
+    """dhip_batch_problem_t"""
+    _fields_ = [("orig", C.c_double * 3), ("e", C.c_double * 9), ("n", C.c_int32 * 3), ("phi_offset", C.c_int64),
+                ("lc", C.c_double * 3)]
+
+
+class BatchStat(C.Structure):
+    """dhip_batch_stat_t"""
+    _fields_ = [("status", C.c_int32), ("ncut", C.c_int32), ("ndof", C.c_int32), ("nnz", C.c_int32),
+                ("iters", C.c_int32), ("outer", C.c_int32), ("resid", C.c_double), ("start_s", C.c_double),
+                ("end_s", C.c_double), ("assembled_s", C.c_double)]
 
 
 class DhipError(RuntimeError):
@@ -134,6 +148,13 @@ def load(path=LIB_PATH):
     L.dhip_ch_residual.argtypes = [P, VP]
     L.dhip_ch_predict_start.argtypes = [P, P, VP, C.c_int32, I32]
     L.dhip_ch_solve_learned.argtypes = [P, P, I32, D, I32]
+    L.dhip_batch_create.argtypes = [P, C.c_int32, C.POINTER(BatchProblem), VP, C.c_int64, C.c_int32, C.c_int32, D,
+                                    C.c_int32, D, C.c_double, C.POINTER(P)]
+    L.dhip_batch_solve.argtypes = [P, C.c_int32, C.c_double, C.c_int32, C.c_int32]
+    L.dhip_batch_stats.argtypes = [P, C.POINTER(BatchStat)]
+    L.dhip_batch_problem.argtypes = [P, C.c_int32, I32, I32, D, D, D]
+    L.dhip_batch_destroy.argtypes = [P]
+    L.dhip_batch_destroy.restype = None
     _lib = L
     return L
 
@@ -285,6 +306,62 @@ class CahnHilliard:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many independent P1 problems assembled and solved in one launch (dhip_batch_*).
+    meshes: list of KuhnMesh; phi: one float64 array holding every problem's half-grid level set
+    (host array, or a device pointer with phi_count and where=DEVICE); offsets: first value of
+    each problem in phi; lcs: per problem operator combination (default (1, 1, h) of its mesh)."""
+
+    def __init__(self, ctx, meshes, phi, offsets, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+                 omit_bound=-1.0, lcs=None, zero_policy=ZERO_THROW, phi_count=None, where=HOST):
+        self.ctx, self.L = ctx, ctx.L
+        self.n = len(meshes)
+        probs = (BatchProblem * self.n)()
+        for i, (m, off) in enumerate(zip(meshes, offsets)):
+            probs[i].orig[:] = list(m.orig)
+            probs[i].e[:] = [float(v) for row in m.e for v in row]
+            probs[i].n[:] = list(m.n)
+            probs[i].phi_offset = int(off)
+            probs[i].lc[:] = list(lcs[i]) if lcs is not None else [1.0, 1.0, m.h]
+        co = np.asarray(coef, np.float64)
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        self.h = C.c_void_p()
+        if where == HOST:
+            self._phi = np.ascontiguousarray(phi, np.float64)
+            ptr, count = _vp(self._phi), self._phi.size
+        else:
+            ptr, count = C.c_void_p(phi), int(phi_count)
+        ctx.check(self.L.dhip_batch_create(ctx.h, self.n, probs, ptr, count, where, zero_policy, _p(co, C.c_double),
+                                           rhs_kind, _p(rp, C.c_double), omit_bound, C.byref(self.h)))
+        ctx._csrs.add(self)
+
+    def solve(self, max_iter=20000, tol=1e-8, rel=True, graph=False):
+        self.ctx.check(self.L.dhip_batch_solve(self.h, int(max_iter), float(tol), int(rel), int(graph)))
+
+    def stats(self):
+        """dict of per-problem numpy arrays: status, ncut, ndof, nnz, iters, outer, resid, start_s, end_s"""
+        st = (BatchStat * self.n)()
+        self.ctx.check(self.L.dhip_batch_stats(self.h, st))
+        return {k: np.array([getattr(x, k) for x in st]) for k, _ in BatchStat._fields_}
+
+    def problem(self, p):
+        st = self.stats()
+        nd, nz = int(st["ndof"][p]), int(st["nnz"][p])
+        rp = np.zeros(nd + 1, np.int32)
+        col = np.zeros(max(nz, 1), np.int32)
+        val = np.zeros(max(nz, 1))
+        rhs, x = np.zeros(max(nd, 1)), np.zeros(max(nd, 1))
+        self.ctx.check(self.L.dhip_batch_problem(self.h, int(p), _p(rp, C.c_int32), _p(col, C.c_int32),
+                                                 _p(val, C.c_double), _p(rhs, C.c_double), _p(x, C.c_double)))
+        return dict(rowptr=rp, colind=col[:nz], comb=val[:nz], rhs=rhs[:nd], x=x[:nd])
+
+    def close(self):
+        if self.h and self.ctx.h:
+            self.L.dhip_batch_destroy(self.h)
+        self.h = None
+        self.ctx._csrs.discard(self)
 
 
 class Mlp:
@@ -609,6 +686,9 @@ class Context:
 
     def cahn_hilliard(self, dt, epsilon, rho, **kw):
         return CahnHilliard(self, dt, epsilon, rho, **kw)
+
+    def batch(self, meshes, phi, offsets, **kw):
+        return Batch(self, meshes, phi, offsets, **kw)
 
     def mlp(self, theta=None):
         return Mlp(self, theta)

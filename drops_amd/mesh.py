@@ -74,6 +74,22 @@ class KuhnMesh:
         X = 0.5 * (A + B)  # exact for vertices (A == B)
         return X[..., 0], X[..., 1], X[..., 2]
 
+    def halfgrid_axes(self):
+        """For an axis-aligned brick: the 1-D half-grid coordinates (x along j1, y along j2, z along
+        j3), equal to halfgrid_coords() broadcast (the other edge vectors add exact zeros)."""
+        for k in range(3):
+            for c in range(3):
+                if c != k and self.e[k][c] != 0.0:
+                    raise ValueError("halfgrid_axes needs an axis-aligned brick")
+        axes = []
+        for k in range(3):
+            nk = self.n[k]
+            i = np.arange(nk + 1, dtype=np.float64)
+            v = self.orig[k] + i * (1.0 / float(nk) * self.e[k][k])
+            j = np.arange(2 * nk + 1)
+            axes.append(0.5 * (v[j // 2] + v[(j + 1) // 2]))
+        return axes
+
     def sample_p2(self, fn):
         """P2 nodal interpolation of fn(x, y, z) (LevelsetP2CL::Init / VecDescCL::Interpolate)."""
         x, y, z = self.halfgrid_coords()

@@ -321,6 +321,48 @@ int dhip_ch_predict_start(dhip_ch_t *ch, dhip_mlp_t *mlp, double *x0, int32_t wh
  * *start (may be NULL) = 0 x_n, 1 network, 2 linear extrapolation */
 int dhip_ch_solve_learned(dhip_ch_t *ch, dhip_mlp_t *mlp, int32_t *iters, double *resid, int32_t *start);
 
+/* ---- batched problems (configuration 4: many uneven P1 problems in one launch) ----
+ * No reference counterpart: each problem is what a DROPS caller does once per level set --
+ * IdxDescCL::CreateNumbering (misc/problem.cpp:725-745), accumulate() of the P1 interface mass,
+ * Laplace-Beltrami and stabilisation accumulators (num/accumulator.h:265-268), LinComb, and
+ * PCGSolverCL<JACPcCL>::Solve (num/krylovsolver.h:1805-1835) -- for a set of level sets.  One
+ * workgroup owns one problem at a time (workgroup barriers only), problems are taken largest first
+ * from a queue, and the solve is Jacobi PCG with FP32 inner iterations and FP64 residuals and
+ * refinement from x = 0.  The assembled arrays of each problem equal those of dhip_assemble_p1. */
+typedef struct dhip_batch dhip_batch_t;
+typedef struct {
+    double orig[3];
+    double e[9];          /* row-major {e1, e2, e3} as dhip_mesh_kuhn */
+    int32_t n[3];
+    int64_t phi_offset;   /* first value of this problem's half-grid level set in the shared array */
+    double lc[3];         /* the solved operator 1*(lc[0]*MASS + lc[1]*LB) + lc[2]*STAB of this problem */
+} dhip_batch_problem_t;
+typedef struct {
+    int32_t status;       /* DHIP_OK, DHIP_E_NOT_CONVERGED, DHIP_E_ZERO_NODE, DHIP_E_ZERO_PATTERN, DHIP_E_CAPACITY */
+    int32_t ncut, ndof, nnz;
+    int32_t iters, outer; /* FP32 inner iterations in total, FP64 refinement steps */
+    double resid;         /* final FP64 relative residual */
+    double start_s, end_s;  /* device wall clock at the start and end of the problem in the last solve */
+    double assembled_s;     /* device wall clock when its system was assembled */
+} dhip_batch_stat_t;
+/* Plans the batch: counts each problem's cut tets and cut-tet vertices on the device and sizes
+ * the per-problem scratch.  phi holds all level sets (phi_count values, DHIP_HOST: copied;
+ * DHIP_DEVICE: kept, must stay valid).  coef/rhs_kind/omit_bound as dhip_assemble_p1. */
+int dhip_batch_create(dhip_ctx_t *ctx, int32_t nprob, const dhip_batch_problem_t *probs, const double *phi,
+                      int64_t phi_count, int32_t where, int32_t zero_policy, const double coef[3],
+                      int32_t rhs_kind, const double *rhs_param, double omit_bound, dhip_batch_t **batch);
+/* Assembles and solves every problem in one persistent launch (relative residual tol, at most
+ * max_iter inner iterations per problem).  use_graph: capture the launch into a hipGraph on the
+ * first call and replay it while the arguments stay the same.  Workgroup size and residency come
+ * from dhip_tune("batch_threads" 128/256/512, "batch_blocks_per_cu"). */
+int dhip_batch_solve(dhip_batch_t *batch, int32_t max_iter, double tol, int32_t rel, int32_t use_graph);
+/* per-problem results of the last solve (nprob entries) */
+int dhip_batch_stats(dhip_batch_t *batch, dhip_batch_stat_t *stats);
+/* host copies of problem p's CSR (comb values), load vector and solution (sizes from the stats) */
+int dhip_batch_problem(dhip_batch_t *batch, int32_t p, int32_t *rowptr, int32_t *colind, double *comb,
+                       double *rhs, double *x);
+void dhip_batch_destroy(dhip_batch_t *batch);
+
 #ifdef __cplusplus
 }
 #endif
