@@ -13,11 +13,16 @@
 //   BrickBuilderCL / MultiGridCL (Kuhn bricks only) .. geom/builder.cpp:67-178
 //   LevelsetP2CL::Phi (P2 level set) ................. levelset/levelset.h:69
 //   IdxDescCL(P1IF_FE)::CreateNumbering .............. misc/problem.cpp:725-745
+//   IdxDescCL(P2IF_FE): CreateNumbOnInterfaceP2 ....... misc/problem.cpp:571-636
 //   MatDescCL, VecDescCL ............................. misc/problem.h:650-768
 //   InterfaceCommonDataP1CL, InterfaceMatrixAccuCL,
 //   InterfaceVectorAccuCL, Local*P1CL, accumulate() .. surfactant/ifacetransp.h:103-156,
 //                                                      533-641, 768-906, 1657-1691, 1985-2057,
 //                                                      num/accumulator.h:80-268
+//   InterfaceCommonDataP2CL, LocalMassP2CL,
+//   LocalLaplaceBeltramiP2CL, LocalVectorP2CL ........ surfactant/ifacetransp.h:200-276, 2319-2403
+//     (on the planar patch with LocalAssembler's P2 forms, :1049-1076, 1545-1588 -- not on the
+//      QuaQua-mapped surface of the reference's P2 classes)
 // Differences (documented at each class): meshes are Kuhn bricks; level sets are
 // P2 nodal vectors on the half-spaced grid; right-hand sides and the
 // stabilisation normal are device-evaluable (closed-form kinds / the normalised
@@ -463,7 +468,7 @@ class MultiGridCL {
     }
 };
 
-enum FiniteElementT { P1IF_FE = 1 };
+enum FiniteElementT { P1IF_FE = 1, P2IF_FE = 2 };
 
 // P2 level set on the half-spaced grid (LevelsetP2CL::Phi)
 class VecDescCL;
@@ -503,14 +508,17 @@ class IdxDescCL {
     FiniteElementT fe_;
     double omit_bound_;
     size_t num_ = 0;
-    std::vector<int64_t> vertex_of_dof_;
+    std::vector<int64_t> vertex_of_dof_;   // P1IF_FE: grid vertex of each DOF
+    std::vector<int64_t> node_of_dof_;     // both spaces: half-grid node of each DOF
 
   public:
     explicit IdxDescCL(FiniteElementT fe = P1IF_FE, double omit_bound = -99) : fe_(fe), omit_bound_(omit_bound) {}
     FiniteElementT GetFE() const { return fe_; }
     size_t NumUnknowns() const { return num_; }
     const std::vector<int64_t> &vertex_of_dof() const { return vertex_of_dof_; }
-    // CreateNumbering (misc/problem.cpp:725-745) -> CreateNumbOnInterfaceVertex (:443-503), on the GPU
+    const std::vector<int64_t> &node_of_dof() const { return node_of_dof_; }
+    // CreateNumbering (misc/problem.cpp:725-745) -> CreateNumbOnInterfaceVertex (:443-503) for P1IF_FE,
+    // CreateNumbOnInterfaceP2 (:571-636) for P2IF_FE, on the GPU
     size_t CreateNumbering(unsigned, const MultiGridCL &mg, const VecDescCL *lsetp, const BndDataCL * = 0) {
         if (!lsetp) throw DROPSErrCL("IdxDescCL::CreateNumbering: no level set function for interface numbering given");
         DeviceCL &dev = DeviceCL::instance();
@@ -523,11 +531,18 @@ class IdxDescCL {
         int64_t ncut = 0, nd = 0;
         const double coef[3] = {1., 1., 1.};
         dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &ncut));
-        dev.check(dhip_local_p1(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
-        dev.check(dhip_number_p1if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
+        if (fe_ == P2IF_FE) {
+            dev.check(dhip_local_p2(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
+            dev.check(dhip_number_p2if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
+        } else {
+            dev.check(dhip_local_p1(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
+            dev.check(dhip_number_p1if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
+        }
         num_ = (size_t)nd;
-        vertex_of_dof_.assign(num_, 0);
-        if (num_) dev.check(dhip_get_dof_vertices(dev.ctx(), vertex_of_dof_.data()));
+        vertex_of_dof_.assign(fe_ == P1IF_FE ? num_ : 0, 0);
+        node_of_dof_.assign(num_, 0);
+        if (num_ && fe_ == P1IF_FE) dev.check(dhip_get_dof_vertices(dev.ctx(), vertex_of_dof_.data()));
+        if (num_) dev.check(dhip_get_dof_nodes(dev.ctx(), node_of_dof_.data()));
         return (size_t)ncut;
     }
 };
@@ -583,6 +598,30 @@ struct LocalNormalLaplaceBulkP1CL {
     double coef;
     LocalNormalLaplaceBulkP1CL(double D, double = 0., std::nullptr_t = nullptr, double = 0.) : coef(D) {}
 };
+// P2 interface space (surfactant/ifacetransp.h:200-276): the same patch data, P2 basis
+class InterfaceCommonDataP2CL : public TetraAccumulatorCL {
+  public:
+    const VecDescCL *ls;
+    InterfaceCommonDataP2CL(const VecDescCL &ls_arg, const BndDataCL &) : ls(&ls_arg) {}
+};
+struct LocalMassP2CL {                // surfactant/ifacetransp.h:2379-2403 (LocalAssembler::M_P2P2, :1579)
+    static const int device_op = DHIP_OP_MASS;
+    double coef;
+    explicit LocalMassP2CL(double alpha = 1.) : coef(alpha) {}
+};
+struct LocalLaplaceBeltramiP2CL {     // surfactant/ifacetransp.h:2319-2377 (buildSurfGradP2, :1049-1053)
+    static const int device_op = DHIP_OP_LB;
+    double coef;
+    explicit LocalLaplaceBeltramiP2CL(double D) : coef(D) {}
+};
+// The normal-gradient stabilisation on the P2 space: the volume integral of C_n_P1P1's form
+// (surfactant/ifacetransp.h:1545-1555) with the P2 basis; the reference has no P2 class for it.
+struct LocalNormalGradientStabP2CL {
+    static const int device_op = DHIP_OP_STAB;
+    double coef;
+    explicit LocalNormalGradientStabP2CL(double D) : coef(D) {}
+};
+
 // load vector of a device-evaluable right-hand side (surfactant/ifacetransp.h:768-798)
 struct DeviceFunctionCL {
     int kind = DHIP_RHS_NONE;
@@ -591,6 +630,10 @@ struct DeviceFunctionCL {
 struct LocalVectorP1CL {
     DeviceFunctionCL f;
     LocalVectorP1CL(DeviceFunctionCL fn, double = 0.) : f(fn) {}
+};
+struct LocalVectorP2CL {
+    DeviceFunctionCL f;
+    LocalVectorP2CL(DeviceFunctionCL fn, double = 0.) : f(fn) {}
 };
 
 template <class LocalMatrixT, class CommonT>
@@ -618,6 +661,7 @@ class InterfaceVectorAccuCL : public TetraAccumulatorCL {
 class TetraAccumulatorTupleCL {
   public:
     const InterfaceCommonDataP1CL *common = nullptr;
+    const InterfaceCommonDataP2CL *common2 = nullptr;
     struct MatEntry { MatDescCL *mat; int op; double coef; };
     std::vector<MatEntry> mats;
     VecDescCL *rhs = nullptr;
@@ -634,13 +678,26 @@ class TetraAccumulatorTupleCL {
         rhs = a->vec;
         rhs_fn = a->local.f;
     }
+    void push_back(InterfaceCommonDataP2CL *c) { common2 = c; }
+    template <class L>
+    void push_back(InterfaceMatrixAccuCL<L, InterfaceCommonDataP2CL> *a) {
+        for (auto &m : mats)
+            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P2 interface operator");
+        mats.push_back({a->mat, L::device_op, a->local.coef});
+    }
+    void push_back(InterfaceVectorAccuCL<LocalVectorP2CL, InterfaceCommonDataP2CL> *a) {
+        rhs = a->vec;
+        rhs_fn = a->local.f;
+    }
 };
 
 // accumulate(tuple, mg, level, bnd) (num/accumulator.h:265-268): one device pass
 // computes the patches and local matrices of all cut tets and scatters every
 // operator of the tuple; results are copied into the caller's MatDescCL/VecDescCL.
 inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsigned = 0, const BndDataCL * = 0) {
-    if (!t.common) throw DROPSErrCL("accumulate: InterfaceCommonDataP1CL missing from the tuple");
+    if (!t.common && !t.common2) throw DROPSErrCL("accumulate: no interface common data in the tuple");
+    if (t.common && t.common2) throw DROPSErrCL("accumulate: P1 and P2 interface accumulators in one tuple");
+    const bool p2 = t.common2 != nullptr;
     DeviceCL &dev = DeviceCL::instance();
     const auto &B = mg.brick();
     const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
@@ -648,14 +705,18 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
     const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
     double coef[3] = {1., 1., 1.};
     for (auto &m : t.mats) coef[m.op] = m.coef;
-    const VectorCL &phi = t.common->ls->Data;
+    const VectorCL &phi = p2 ? t.common2->ls->Data : t.common->ls->Data;
     dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
     dev.check(dhip_levelset_p2(dev.ctx(), phi.raw(), (int64_t)phi.size(), DHIP_HOST));
     int64_t ncut = 0, nd = 0, nnz = 0;
     const double param = t.rhs_fn.param;
     const double lc[3] = {1., 1., 1.};
-    dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param, -1.0,
-                               lc, &ncut, &nd, &nnz));
+    if (p2)
+        dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
+                                   -1.0, lc, &ncut, &nd, &nnz));
+    else
+        dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
+                                   -1.0, lc, &ncut, &nd, &nnz));
     std::vector<int32_t> rp(nd + 1), ci(nnz);
     std::vector<double> v[3], rhs(nd);
     for (auto &x : v) x.resize(nnz);

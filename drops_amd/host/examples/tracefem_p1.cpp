@@ -4,6 +4,8 @@
 //
 //   tracefem_p1 sphere20                    -> the 1156-DOF known answer (TestinterfaceP1FE.ref)
 //   tracefem_p1 c1 <n> <outdir>             -> configuration-1 problem, writes CSR/rhs/solution
+//   tracefem_p1 p2 <n> <outdir>             -> the same sphere on the P2 interface space (IdxDescCL(P2IF_FE),
+//                                              P2 accumulators), writes CSR/rhs/solution
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -96,11 +98,63 @@ static int c1(int n, const std::string &out) {
     return 0;
 }
 
+static int p2(int n, const std::string &out) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL lsbnd;
+    IdxDescCL ifaceidx(P2IF_FE);
+    ifaceidx.CreateNumbering(0, mg, &lset.Phi, &lsbnd);
+    std::printf("NumUnknowns: %zu\n", ifaceidx.NumUnknowns());
+    MatDescCL M(&ifaceidx, &ifaceidx), A(&ifaceidx, &ifaceidx), C(&ifaceidx, &ifaceidx);
+    VecDescCL b(&ifaceidx);
+    TetraAccumulatorTupleCL accus;
+    InterfaceCommonDataP2CL cdata(lset.Phi, lsbnd);
+    accus.push_back(&cdata);
+    InterfaceMatrixAccuCL<LocalMassP2CL, InterfaceCommonDataP2CL> mass(&M, LocalMassP2CL(), cdata, "mass");
+    InterfaceMatrixAccuCL<LocalLaplaceBeltramiP2CL, InterfaceCommonDataP2CL> lb(&A, LocalLaplaceBeltramiP2CL(1.), cdata, "LB");
+    InterfaceMatrixAccuCL<LocalNormalGradientStabP2CL, InterfaceCommonDataP2CL> stab(&C, LocalNormalGradientStabP2CL(1.), cdata, "stab");
+    DeviceFunctionCL f;
+    f.kind = DHIP_RHS_3XY_MINUS_X3;
+    InterfaceVectorAccuCL<LocalVectorP2CL, InterfaceCommonDataP2CL> rhs(&b, LocalVectorP2CL(f), cdata, "rhs");
+    accus.push_back(&mass);
+    accus.push_back(&lb);
+    accus.push_back(&stab);
+    accus.push_back(&rhs);
+    accumulate(accus, mg, 0, &lsbnd);
+    std::printf("nnz: %zu\n", M.Data.num_nonzeros());
+    MatrixCL L;
+    L.LinComb(1.0, M.Data, 1.0, A.Data, h, C.Data);
+    VectorCL x(L.num_rows());
+    JACPcCL pc;
+    PCGSolverCL<JACPcCL> solver(pc, 20000, 1e-10, true);
+    solver.Solve(L, x, b.Data, DummyExchangeCL());
+    std::printf("PCG iterations: %d\nPCG residual: %.6e\n", solver.GetIter(), solver.GetResid());
+    if (!out.empty()) {
+        std::vector<int64_t> rp(L.num_rows() + 1), ci(L.num_nonzeros());
+        for (size_t i = 0; i <= L.num_rows(); ++i) rp[i] = (int64_t)L.raw_row()[i];
+        for (size_t k = 0; k < L.num_nonzeros(); ++k) ci[k] = (int64_t)L.raw_col()[k];
+        write_bin(out + "/rowptr.bin", rp.data(), rp.size() * 8);
+        write_bin(out + "/colind.bin", ci.data(), ci.size() * 8);
+        write_bin(out + "/mass.bin", M.Data.raw_val(), M.Data.num_nonzeros() * 8);
+        write_bin(out + "/lb.bin", A.Data.raw_val(), A.Data.num_nonzeros() * 8);
+        write_bin(out + "/stab.bin", C.Data.raw_val(), C.Data.num_nonzeros() * 8);
+        write_bin(out + "/rhs.bin", b.Data.raw(), b.Data.size() * 8);
+        write_bin(out + "/x.bin", x.raw(), x.size() * 8);
+        write_bin(out + "/node_of_dof.bin", ifaceidx.node_of_dof().data(), ifaceidx.node_of_dof().size() * 8);
+    }
+    return 0;
+}
+
 int main(int argc, char **argv) {
     try {
         if (argc >= 2 && !std::strcmp(argv[1], "sphere20")) return sphere20();
         if (argc >= 3 && !std::strcmp(argv[1], "c1")) return c1(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
-        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir]\n", argv[0]);
+        if (argc >= 3 && !std::strcmp(argv[1], "p2")) return p2(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
+        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir] | p2 <n> [outdir]\n", argv[0]);
         return 2;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());

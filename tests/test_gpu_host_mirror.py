@@ -54,3 +54,22 @@ def test_c1_through_host_mirror(tmp_path):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
     _, itr, _, _ = O.pcg_jacobi(ref["rowptr"], ref["colind"], comb, ref["rhs"], tol=1e-10)
     assert abs(it - itr) <= max(2, 0.02 * itr)
+
+
+def test_p2_through_host_mirror(tmp_path):
+    """IdxDescCL(P2IF_FE) + the P2 accumulators through accumulate(): numbering, pattern and every value
+    equal to the oracle's P2 assembly; PCG to 1e-10."""
+    n = 16
+    out = run("p2", str(n), str(tmp_path))
+    ld = lambda name, dt: np.fromfile(tmp_path / f"{name}.bin", dtype=dt)
+    mesh, phi = cases.c1_sphere(n)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    dof, nd = O.number_p2if(om, phi)
+    ref = O.assemble_p2(om, phi, dof, nd, rhs_kind=O.RHS_3XY_MINUS_X3)
+    assert np.array_equal(ld("rowptr", np.int64), ref["rowptr"])
+    assert np.array_equal(ld("colind", np.int64), ref["colind"].astype(np.int64))
+    for k in ("mass", "lb", "stab", "rhs"):
+        assert np.array_equal(ld(k, np.float64), ref[k]), k
+    nod = ld("node_of_dof", np.int64)
+    assert np.array_equal(dof[nod], np.arange(nd))
+    assert float(re.search(r"PCG residual: (\S+)", out).group(1)) <= 1e-10
