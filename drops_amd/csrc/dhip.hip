@@ -11,6 +11,7 @@
 #include "dhip_batch.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 
 namespace dhip {
@@ -393,7 +394,7 @@ void dhip_destroy(dhip_ctx_t *c) {
                       &c->loc, &c->flags, &c->first, &c->isown, &c->ownrank, &c->dofs, &c->rowowner,
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
                       &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
-                      &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val};
+                      &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -1327,9 +1328,22 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     // Blocks whose CSR slice exceeds the LDS slice capacity read a SELL-64 copy instead (built
     // when the average block would not fit).
     const int64_t per = (n + nb - 1) / nb;
-    if (A->nnz > 0 && (per + 1) * 4 + (A->nnz / nb) * 12 + 16 > (int64_t)LDS_BYTES) {
+    const uint64_t key[4] = {A->pattern_id, (uint64_t)n, (uint64_t)A->nnz, (uint64_t)nb};
+    if (A->nnz > 0 && (per + 1) * 4 + (A->nnz / nb) * 12 + 16 > (int64_t)LDS_BYTES && A->pattern_id != 0 &&
+        std::equal(key, key + 4, c->gm_sell_key)) {
+        // the same pattern and block layout as the last solve: only the values change
+        const int spb = (int)((per + 63) / 64);
+        DHIP_LAUNCH(c, "gmres_sell_vals", k_sell_vals, blocks_for(A->nnz, 256), 256, 0, A->nnz,
+                    c->gm_sell_pos.as<int32_t>(), A->col, A->val, (diag && !left_pc) ? diag : nullptr,
+                    c->gm_sell_val.as<double>());
+        ga.sell_off = c->gm_sell_off.as<int32_t>();
+        ga.sell_col = c->gm_sell_col.as<int32_t>();
+        ga.sell_val = c->gm_sell_val.as<double>();
+        ga.sell_spb = spb;
+    } else if (A->nnz > 0 && (per + 1) * 4 + (A->nnz / nb) * 12 + 16 > (int64_t)LDS_BYTES) {
         const int spb = (int)((per + 63) / 64);
         const int64_t nsl = (int64_t)nb * spb;
+        for (int k = 0; k < 4; ++k) c->gm_sell_key[k] = 0;
         DHIP_HIPCHK(c, c->gm_sell_off.ensure(sizeof(int32_t) * (nsl + 1)));
         DHIP_LAUNCH(c, "gmres_sell_width", k_sell_width, blocks_for(nsl, 256), 256, 0, n, per, spb, nsl, A->rowptr,
                     c->gm_sell_off.as<int32_t>());
@@ -1338,9 +1352,15 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
         if (rc) return rc;
         DHIP_HIPCHK(c, c->gm_sell_col.ensure(sizeof(int32_t) * (tot + 1)));
         DHIP_HIPCHK(c, c->gm_sell_val.ensure(sizeof(double) * (tot + 1)));
+        int32_t *pos = nullptr;
+        if (A->pattern_id != 0) {
+            DHIP_HIPCHK(c, c->gm_sell_pos.ensure(sizeof(int32_t) * (A->nnz + 1)));
+            pos = c->gm_sell_pos.as<int32_t>();
+        }
         DHIP_LAUNCH(c, "gmres_sell_fill", k_sell_fill, blocks_for(n, 256), 256, 0, n, per, spb, A->rowptr, A->col,
                     A->val, (diag && !left_pc) ? diag : nullptr, c->gm_sell_off.as<int32_t>(),
-                    c->gm_sell_col.as<int32_t>(), c->gm_sell_val.as<double>());
+                    c->gm_sell_col.as<int32_t>(), c->gm_sell_val.as<double>(), pos);
+        if (A->pattern_id != 0) std::copy(key, key + 4, c->gm_sell_key);
         ga.sell_off = c->gm_sell_off.as<int32_t>();
         ga.sell_col = c->gm_sell_col.as<int32_t>();
         ga.sell_val = c->gm_sell_val.as<double>();
@@ -1684,7 +1704,9 @@ static int ch_create(dhip_ctx *c, int32_t zero_policy, const dhip_ch_params_t *p
         rc = ch_contrib_lists(h);
         if (rc) return fail(rc);
     }
+    static std::atomic<uint64_t> next_pattern{1};
     h->K.ctx = c;
+    h->K.pattern_id = next_pattern.fetch_add(1);
     h->K.n = 2 * R;
     h->K.nnz = 4 * Z;
     h->K.rowptr = h->brp.as<int32_t>();

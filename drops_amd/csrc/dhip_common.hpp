@@ -130,6 +130,8 @@ struct dhip_ctx {
     dhip::DevBuf kv[16];
     dhip::DevBuf gm_sval;   // right-preconditioned GMRES: a_rc / d_c
     dhip::DevBuf gm_sell_off, gm_sell_col, gm_sell_val;  // SELL-64 copy for large systems
+    dhip::DevBuf gm_sell_pos;                            // SELL slot of every CSR entry
+    uint64_t gm_sell_key[4] = {0, 0, 0, 0};              // pattern id, rows, entries, blocks of the layout
     // optional per-block wall-clock probe of the persistent solver
     bool probe_on = false;
     int64_t pcg_rows_per_block = 0;   // 0: one row per thread of a 512-thread block
@@ -146,6 +148,7 @@ struct dhip_csr {
     const int32_t *rowptr = nullptr, *col = nullptr;
     const double *val = nullptr;
     dhip::DevBuf own_rowptr, own_col, own_val;
+    uint64_t pattern_id = 0;   // nonzero: rowptr/col never change for this handle (solver layouts may be reused)
 };
 
 namespace dhip {

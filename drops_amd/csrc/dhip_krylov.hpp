@@ -775,7 +775,7 @@ __global__ void k_sell_width(int64_t n, int64_t per, int spb, int64_t nslices, c
 
 __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, const int32_t *col,
                             const double *val, const double *diag, const int32_t *off, int32_t *scol,
-                            double *sval) {
+                            double *sval, int32_t *pos) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t b = r / per, l = r - b * per;
@@ -784,7 +784,16 @@ __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, 
         const int32_t c = col[e];
         scol[base + 64 * (int64_t)k] = c;
         sval[base + 64 * (int64_t)k] = diag ? val[e] / diag[c] : val[e];
+        if (pos) pos[e] = (int32_t)(base + 64 * (int64_t)k);
     }
+}
+
+// new values on an unchanged pattern: one lane per CSR entry through the stored SELL slots
+__global__ void k_sell_vals(int64_t nnz, const int32_t *pos, const int32_t *col, const double *val,
+                            const double *diag, double *sval) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    sval[pos[e]] = diag ? val[e] / diag[col[e]] : val[e];
 }
 
 // a_rc / d_c for right preconditioning (the same quotient the LDS slice loader forms)
