@@ -92,6 +92,7 @@ def lib(kind="parity"):
     L.orc_ch_step_p2.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
     L.orc_lattice_tables.argtypes = [I32, I32]
+    L.orc_quad_tables.argtypes = [D, D, D, D]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
     _libs[name] = L
     return L
@@ -117,6 +118,13 @@ def lattice_tables():
     t = np.zeros((8, 4), np.int32)
     lib().orc_lattice_tables(_ptr(b, C.c_int32), _ptr(t, C.c_int32))
     return b, t
+
+
+def quad_tables():
+    """(q5n (7,3), q5w (7,), v5n (15,4), (C1, C2)) of the oracle's quadrature rules"""
+    q5n, q5w, v5n, v5c = np.zeros((7, 3)), np.zeros(7), np.zeros((15, 4)), np.zeros(2)
+    lib().orc_quad_tables(*[_ptr(a, C.c_double) for a in (q5n, q5w, v5n, v5c)])
+    return q5n, q5w, v5n, tuple(v5c)
 
 
 def refpatch_table():

@@ -196,6 +196,16 @@ void orc_lattice_tables(int32_t bary2[10][4], int32_t tets[8][4])
         for (int c = 0; c < 4; ++c) tets[i][c] = LAT_T[i][c];
 }
 
+void orc_quad_tables(double q5n[7][3], double q5w[7], double v5n[15][4], double v5c[2])
+{
+    init_tables();
+    memcpy(q5n, Q5_N, sizeof Q5_N);
+    memcpy(q5w, Q5_W, sizeof Q5_W);
+    memcpy(v5n, V5_N, sizeof V5_N);
+    v5c[0] = V5_C1;
+    v5c[1] = V5_C2;
+}
+
 void orc_refpatch_table(int32_t size[81], int32_t tri[81][2][3], int32_t bnd[81])
 {
     init_tables();

@@ -12,7 +12,8 @@
  *   - P1 interface numbering: tests/specifications/serial/TestinterfaceP1FE.ref
  *     (0, 4, 8, 4, 1156)
  *   - surface quadrature + patch: Testprincipallattice.ref (area 3.0909)
- *   - Quad2/Quad5 2D rules: exactness on polynomials (TestQuad5_2D.ref idea)
+ *   - Quad2/Quad5 2D and Quad5 volume rules: TestQuad5_2D.ref, TestQuad5.ref
+ *     (exactness on degree <= 5, P2 mass, brick integral; kept in tests/golden)
  *   - mass matrix: two independent routes (per-facet Quad5_2D vs composite
  *     Quad2), as tests/interfaceP1FE.cpp:140-230 does
  *   - Laplace-Beltrami: closed form on planar patches + the O(h^2)
@@ -60,6 +61,10 @@ enum { ORC_OP_MASS = 1, ORC_OP_LB = 2, ORC_OP_STAB = 4 };
 enum { ORC_RHS_NONE = 0, ORC_RHS_3XY_MINUS_X3 = 1, ORC_RHS_CONST = 2 };
 
 /* ---- tables (exposed for the table tests) ---- */
+/* quadrature tables: Quad5_2D nodes (barycentric on a triangle) and weights (sum 1/2), Quad5
+ * volume nodes (barycentric) and the weights of node groups 1 (nodes 1-4) and 2 (nodes 5-8);
+ * node 0 has 8/405, nodes 9-14 have 5/567 (num/discretize.cpp:153-296, num/discretize.tpp:754-762) */
+void orc_quad_tables(double q5n[7][3], double q5w[7], double v5n[15][4], double v5c[2]);
 void orc_lattice_tables(int32_t bary2[10][4], int32_t tets[8][4]);
 /* 81 patterns indexed by pattern_idx+40: size, 2 triangles x 3, boundary flag */
 void orc_refpatch_table(int32_t size[81], int32_t tri[81][2][3], int32_t bnd[81]);
