@@ -262,6 +262,49 @@ def batch_leg(nprob=512, seed=0, steps=3, cpu_sample=8, with_cpu=True):
     return out
 
 
+def formats_leg(ctx, g, with_cpu=True, reps=3):
+    """The operator of the solve written as MatrixMarket from its device handle (exportMTX)
+    and the load vector as a binary and a text file (VecDescCL::Write), best of reps; beside
+    it the same statements through C++ iostreams (the oracle's restatement, one thread)."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="dhip_formats_")
+    try:
+        A = ctx.csr_from_assembly(D.OP_COMB)
+        rhs_dev = ctx.device_arrays()[3]
+        n = g["rowptr"].size - 1
+        path = os.path.join(d, "L.mtx")
+
+        def best(f):
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                f()
+                ts.append(time.perf_counter() - t0)
+            return min(ts)
+
+        t_mtx = best(lambda: A.export(path, "mtx"))
+        mtx_bytes = os.path.getsize(path)
+        t_plain = best(lambda: A.export(os.path.join(d, "L.txt"), "plain"))
+        t_vb = best(lambda: ctx.vec_write(os.path.join(d, "b.dat"), rhs_dev, n, binary=True))
+        t_vt = best(lambda: ctx.vec_write(os.path.join(d, "b.txt"), rhs_dev, n, binary=False))
+        A.close()
+        out = {"mtx_s": t_mtx, "mtx_mb": mtx_bytes / 1e6, "mtx_mbs": mtx_bytes / 1e6 / t_mtx, "plain_s": t_plain,
+               "vec_binary_s": t_vb, "vec_text_s": t_vt, "threads": min(16, os.cpu_count() or 1),
+               "note": "device handle -> host -> file on the local disk, best of %d" % reps}
+        if with_cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle as O
+            comb = g["comb"]
+            ref = os.path.join(d, "ref.mtx")
+            out["iostream_mtx_s"] = best(lambda: O.fmt_matrix(ref, g["rowptr"], g["colind"], comb, fmt="mtx"))
+            with open(path, "rb") as a, open(ref, "rb") as b:
+                out["bytes_identical"] = a.read() == b.read()
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def measured_traffic(kernel, rows, ndof):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
     each scaled by a known-size copy), when it was measured at this block geometry and system size."""
@@ -447,6 +490,8 @@ def main():
             xg = results[name]["x"]
             err[name] = float(np.linalg.norm(xg - xs) / np.linalg.norm(xs))
         err["cpu"] = float(np.linalg.norm(xc - xs) / np.linalg.norm(xs))
+    fm = formats_leg(ctx, g, with_cpu=not args.no_cpu)
+    log(f"[bench] formats {fm}")
     ctx.close()
 
     ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu, space=args.ch_space)
@@ -481,6 +526,7 @@ def main():
                      "bytes_per_iter": bytes_per_iter, "iters": iters, "avg_launch_us": pk["avg_us"]},
         "learned": ch,
         "batch": bt,
+        "formats": fm,
     }
     if cpu is not None:
         line["cpu_baseline"] = {"value": cpu["value"], "unit": "s per solve", "cores": cpu["threads"],

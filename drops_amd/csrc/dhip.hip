@@ -9,6 +9,7 @@
 #include "dhip_ch_p2.hpp"
 #include "dhip_mlp.hpp"
 #include "dhip_batch.hpp"
+#include "dhip_io.hpp"
 
 #include <algorithm>
 #include <atomic>
@@ -220,13 +221,15 @@ static hipError_t upload_tables(int device) {
 // ---------------------------------------------------------------------------
 // errors, profiling
 // ---------------------------------------------------------------------------
+// messages of calls made without a context (file formats on host memory)
+static thread_local char g_err_noctx[256] = "null context";
+
 int set_error(dhip_ctx *ctx, int code, const char *fmt, ...) {
-    if (ctx) {
-        va_list ap;
-        va_start(ap, fmt);
-        std::vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
-        va_end(ap);
-    }
+    va_list ap;
+    va_start(ap, fmt);
+    if (ctx) std::vsnprintf(ctx->err, sizeof ctx->err, fmt, ap);
+    else std::vsnprintf(g_err_noctx, sizeof g_err_noctx, fmt, ap);
+    va_end(ap);
     return code;
 }
 
@@ -401,7 +404,7 @@ void dhip_destroy(dhip_ctx_t *c) {
     delete c;
 }
 
-const char *dhip_last_error(const dhip_ctx_t *c) { return c ? c->err : "null context"; }
+const char *dhip_last_error(const dhip_ctx_t *c) { return c ? c->err : g_err_noctx; }
 
 int dhip_profile_enable(dhip_ctx_t *c, int32_t on) {
     if (!c) return DHIP_E_ARG;
@@ -2462,6 +2465,240 @@ int dhip_batch_problem(dhip_batch_t *b, int32_t p, int32_t *rowptr, int32_t *col
     if (rc) return rc;
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return DHIP_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// on-disk formats (MatrixCL::exportMTX / out / in, VecDescBaseCL::Write / Read)
+// ---------------------------------------------------------------------------
+static int io_error(dhip_ctx *c, int rc, const char *what) {
+    const int code = rc == dhipio::IO_E_COUNT ? DHIP_E_ARG : rc == dhipio::IO_E_SPACE ? DHIP_E_CAPACITY : DHIP_E_IO;
+    return set_error(c, code, "%s: %s", what, dhipio::io_message(rc));
+}
+
+static bool bad_format(int32_t f) { return f != DHIP_MTX_MARKET && f != DHIP_MTX_PLAIN; }
+
+// runs f, turning allocation failures into DHIP_E_NOMEM (no exception crosses the C ABI)
+extern "C++" {
+template <class F>
+static int guarded(dhip_ctx *c, F &&f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        return set_error(c, DHIP_E_NOMEM, "host allocation failed");
+    } catch (const std::exception &e) {
+        return set_error(c, DHIP_E_IO, "%s", e.what());
+    }
+}
+}
+
+static int read_text(dhip_ctx *c, const char *path, std::string &s) {
+    const int rc = dhipio::slurp(path, s);
+    return rc ? io_error(c, rc, path) : DHIP_OK;
+}
+
+static int matrix_to_host(const dhip_csr *A, std::vector<int32_t> &rp, std::vector<int32_t> &col,
+                          std::vector<double> &val) {
+    dhip_ctx *c = A->ctx;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    rp.resize((size_t)A->n + 1);
+    col.resize((size_t)A->nnz + 1);
+    val.resize((size_t)A->nnz + 1);
+    DHIP_HIPCHK(c, hipMemcpyAsync(rp.data(), A->rowptr, sizeof(int32_t) * (A->n + 1), hipMemcpyDeviceToHost, c->stream));
+    if (A->nnz > 0) {
+        DHIP_HIPCHK(c, hipMemcpyAsync(col.data(), A->col, sizeof(int32_t) * A->nnz, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipMemcpyAsync(val.data(), A->val, sizeof(double) * A->nnz, hipMemcpyDeviceToHost, c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+int dhip_mtx_format(int32_t format, int64_t nrows, int64_t ncols, const int32_t *rowptr, const int32_t *colind,
+                    const double *val, char *buf, int64_t cap, int64_t *len) {
+    if (bad_format(format) || nrows < 0 || ncols < 0 || !rowptr || !len)
+        return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_format: bad arguments");
+    if (!buf) {
+        *len = dhipio::matrix_bound(format, nrows, rowptr[nrows]);
+        return DHIP_OK;
+    }
+    if (rowptr[nrows] > 0 && (!colind || !val)) return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_format: no entries");
+    return guarded(nullptr, [&] {
+        dhipio::Sink out;
+        out.buf = buf;
+        out.cap = cap;
+        const int rc = dhipio::format_matrix(out, format, nrows, ncols, rowptr, colind, val);
+        *len = out.len;
+        return rc ? io_error(nullptr, rc, "dhip_mtx_format") : DHIP_OK;
+    });
+}
+
+int dhip_mtx_write(const char *path, int32_t format, int64_t nrows, int64_t ncols, const int32_t *rowptr,
+                   const int32_t *colind, const double *val) {
+    if (!path || bad_format(format) || nrows < 0 || ncols < 0 || !rowptr)
+        return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_write: bad arguments");
+    if (rowptr[nrows] > 0 && (!colind || !val)) return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_write: no entries");
+    return guarded(nullptr, [&] {
+        const int rc = dhipio::to_file(path, [&](dhipio::Sink &out) {
+            return dhipio::format_matrix(out, format, nrows, ncols, rowptr, colind, val);
+        });
+        return rc ? io_error(nullptr, rc, path) : DHIP_OK;
+    });
+}
+
+int dhip_mtx_parse(const char *buf, int64_t len, int32_t format, int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                   int32_t *rowptr, int32_t *colind, double *val) {
+    if ((!buf && len > 0) || len < 0 || !nrows || !ncols || !nnz || bad_format(format) ||
+        (rowptr && (!colind || !val)))
+        return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_parse: bad arguments");
+    return guarded(nullptr, [&] {
+        const std::string s(buf ? buf : "", (size_t)len);
+        const int rc = dhipio::parse_matrix(s, format, nrows, ncols, nnz, rowptr, colind, val);
+        return rc ? io_error(nullptr, rc, "dhip_mtx_parse") : DHIP_OK;
+    });
+}
+
+int dhip_mtx_read(const char *path, int32_t format, int64_t *nrows, int64_t *ncols, int64_t *nnz, int32_t *rowptr,
+                  int32_t *colind, double *val) {
+    if (!path || !nrows || !ncols || !nnz || bad_format(format) || (rowptr && (!colind || !val)))
+        return set_error(nullptr, DHIP_E_ARG, "dhip_mtx_read: bad arguments");
+    return guarded(nullptr, [&] {
+        std::string s;
+        int rc = read_text(nullptr, path, s);
+        if (rc) return rc;
+        rc = dhipio::parse_matrix(s, format, nrows, ncols, nnz, rowptr, colind, val);
+        return rc ? io_error(nullptr, rc, path) : DHIP_OK;
+    });
+}
+
+int dhip_csr_export(const dhip_csr_t *A, const char *path, int32_t format) {
+    if (!A || !path) return DHIP_E_ARG;
+    dhip_ctx *c = A->ctx;
+    if (bad_format(format)) return set_error(c, DHIP_E_ARG, "unknown format");
+    return guarded(c, [&] {
+        std::vector<int32_t> rp, col;
+        std::vector<double> val;
+        int rc = matrix_to_host(A, rp, col, val);
+        if (rc) return rc;
+        rc = dhipio::to_file(path, [&](dhipio::Sink &out) {
+            return dhipio::format_matrix(out, format, A->n, A->n, rp.data(), col.data(), val.data());
+        });
+        return rc ? io_error(c, rc, path) : DHIP_OK;
+    });
+}
+
+int dhip_csr_import(dhip_ctx_t *c, const char *path, int32_t format, dhip_csr_t **A) {
+    if (!c || !path || !A) return DHIP_E_ARG;
+    if (bad_format(format)) return set_error(c, DHIP_E_ARG, "unknown format");
+    return guarded(c, [&] {
+        std::string s;
+        int rc = read_text(c, path, s);
+        if (rc) return rc;
+        int64_t nr = 0, nc = 0, nz = 0;
+        if ((rc = dhipio::parse_matrix(s, format, &nr, &nc, &nz, nullptr, nullptr, nullptr)))
+            return io_error(c, rc, path);
+        if (nr != nc)
+            return set_error(c, DHIP_E_ARG, "%s: %lld x %lld is not square", path, (long long)nr, (long long)nc);
+        std::vector<int32_t> rp((size_t)nr + 1), col((size_t)nz + 1);
+        std::vector<double> val((size_t)nz + 1);
+        if ((rc = dhipio::parse_matrix(s, format, &nr, &nc, &nz, rp.data(), col.data(), val.data())))
+            return io_error(c, rc, path);
+        return dhip_csr_create(c, nr, nz, rp.data(), col.data(), val.data(), DHIP_HOST, A);
+    });
+}
+
+// host copy of a caller vector (device: copied down; host: used in place)
+static int vector_on_host(dhip_ctx *c, const double *x, int64_t n, int32_t where, std::vector<double> &h,
+                          const double **src) {
+    *src = x;
+    if (where != DHIP_DEVICE) return DHIP_OK;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    h.resize((size_t)n + 1);
+    if (n > 0) DHIP_HIPCHK(c, hipMemcpyAsync(h.data(), x, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    *src = h.data();
+    return DHIP_OK;
+}
+
+static int vector_to_caller(dhip_ctx *c, double *x, const std::vector<double> &h, int64_t n, int32_t where) {
+    if (where != DHIP_DEVICE || n == 0) return DHIP_OK;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DHIP_HIPCHK(c, hipMemcpyAsync(x, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return DHIP_OK;
+}
+
+static bool bad_vec_args(dhip_ctx *c, const double *x, int64_t n, int32_t where) {
+    return n < 0 || (n > 0 && !x) || (where != DHIP_HOST && where != DHIP_DEVICE) || (where == DHIP_DEVICE && !c);
+}
+
+int dhip_vec_format(dhip_ctx_t *c, const double *x, int64_t n, int32_t where, int32_t binary, char *buf, int64_t cap,
+                    int64_t *len) {
+    if (bad_vec_args(c, x, n, where) || !len) return set_error(c, DHIP_E_ARG, "dhip_vec_format: bad arguments");
+    if (!buf) {
+        *len = dhipio::vec_bound(n, binary != 0);
+        return DHIP_OK;
+    }
+    return guarded(c, [&] {
+        std::vector<double> h;
+        const double *src;
+        int rc = vector_on_host(c, x, n, where, h, &src);
+        if (rc) return rc;
+        dhipio::Sink out;
+        out.buf = buf;
+        out.cap = cap;
+        rc = dhipio::format_vec(out, src, n, binary != 0);
+        *len = out.len;
+        return rc ? io_error(c, rc, "dhip_vec_format") : DHIP_OK;
+    });
+}
+
+int dhip_vec_write(dhip_ctx_t *c, const char *path, const double *x, int64_t n, int32_t where, int32_t binary) {
+    if (!path || bad_vec_args(c, x, n, where)) return set_error(c, DHIP_E_ARG, "dhip_vec_write: bad arguments");
+    return guarded(c, [&] {
+        std::vector<double> h;
+        const double *src;
+        int rc = vector_on_host(c, x, n, where, h, &src);
+        if (rc) return rc;
+        rc = dhipio::to_file(path, [&](dhipio::Sink &out) { return dhipio::format_vec(out, src, n, binary != 0); });
+        return rc ? io_error(c, rc, path) : DHIP_OK;
+    });
+}
+
+int dhip_vec_parse(dhip_ctx_t *c, const char *buf, int64_t len, double *x, int64_t n, int32_t where, int32_t binary) {
+    if ((!buf && len > 0) || len < 0 || bad_vec_args(c, x, n, where))
+        return set_error(c, DHIP_E_ARG, "dhip_vec_parse: bad arguments");
+    return guarded(c, [&] {
+        const std::string s(buf ? buf : "", (size_t)len);
+        std::vector<double> h(where == DHIP_DEVICE ? (size_t)n + 1 : 0);
+        int rc = dhipio::parse_vec(s, where == DHIP_DEVICE ? h.data() : x, n, binary != 0);
+        if (rc) return io_error(c, rc, "dhip_vec_parse");
+        return vector_to_caller(c, x, h, n, where);
+    });
+}
+
+int dhip_vec_read(dhip_ctx_t *c, const char *path, double *x, int64_t n, int32_t where, int32_t binary) {
+    if (!path || bad_vec_args(c, x, n, where)) return set_error(c, DHIP_E_ARG, "dhip_vec_read: bad arguments");
+    return guarded(c, [&] {
+        std::string s;
+        int rc = read_text(c, path, s);
+        if (rc) return rc;
+        std::vector<double> h(where == DHIP_DEVICE ? (size_t)n + 1 : 0);
+        rc = dhipio::parse_vec(s, where == DHIP_DEVICE ? h.data() : x, n, binary != 0);
+        if (rc) return io_error(c, rc, path);
+        return vector_to_caller(c, x, h, n, where);
+    });
+}
+
+int dhip_vec_count(const char *path, int32_t binary, int64_t *n) {
+    if (!path || !n) return set_error(nullptr, DHIP_E_ARG, "dhip_vec_count: bad arguments");
+    // the count is at the start of the file
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return io_error(nullptr, dhipio::IO_E_OPEN, path);
+    char head[64];
+    const size_t k = std::fread(head, 1, sizeof head, f);
+    std::fclose(f);
+    const int rc = dhipio::parse_vec_count(std::string(head, k), binary != 0, n);
+    return rc ? io_error(nullptr, rc, path) : DHIP_OK;
 }
 
 }  // extern "C"

@@ -236,6 +236,33 @@ __device__ inline double block_sum(double v) {
     return s;
 }
 
+// three block sums with one exchange through LDS (each rounded as block_sum rounds it)
+template <int BLOCK>
+__device__ inline void block_sum3(double &a, double &b, double &c) {
+    __shared__ double ws[BLOCK / 64][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    if (lane == 0) {
+        ws[w][0] = a;
+        ws[w][1] = b;
+        ws[w][2] = c;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0, sc = 0.0;
+#pragma unroll
+    for (int i = 0; i < BLOCK / 64; ++i) {
+        sa += ws[i][0];
+        sb += ws[i][1];
+        sc += ws[i][2];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+    c = sc;
+}
+
 // Exclusive scan of n int32 values (out may alias in); *total_host receives the sum.
 int scan_exclusive_i32(dhip_ctx *ctx, const int32_t *in, int32_t *out, int64_t n, int64_t *total_host);
 

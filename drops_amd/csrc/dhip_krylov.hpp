@@ -418,7 +418,8 @@ done:
 // with u = M r the Jacobi step.  w = A u is formed from the neighbours' values
 // of the previous synchronisation: u_c = (r_c - alpha (w_c + beta s_c)) / d_c,
 // using the column-scaled slice a_rc / d_c kept in LDS.  r, w and s are double
-// buffered by iteration parity.
+// buffered by iteration parity (separate arrays: a layout with the three values of
+// a row side by side measured slower, its synchronisations took longer).
 // ---------------------------------------------------------------------------
 struct PcgSrArgs {
     int64_t n;
@@ -516,8 +517,10 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
         p_rr += rr[k] * rr[k];
         p_bb += bv * bv;
     }
-    p_rr = block_sum<PB>(p_rr);
-    p_bb = block_sum<PB>(p_bb);
+    {
+        double unused = 0.0;
+        block_sum3<PB>(p_rr, p_bb, unused);
+    }
     if (!sync_reduce<PB>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; goto done; }
     {
         double normb = sqrt(red[1]);
@@ -544,8 +547,10 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
             p_g += rr[k] * ur[k];
             p_d += wr[k] * ur[k];
         }
-        p_g = block_sum<PB>(p_g);
-        p_d = block_sum<PB>(p_d);
+        {
+            double unused = 0.0;
+            block_sum3<PB>(p_g, p_d, unused);
+        }
         if (!sync_reduce<PB>(a.slots, ++gen, p_g, p_d, 0.0, red)) { status = 2; goto done; }
         double gamma = red[0], alpha = red[0] / red[1], beta = 0.0;
         const uint64_t t_setup = wall_clock64() - t_start;
@@ -573,9 +578,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
                 p_d2 += wr[k] * ur[k];
                 p_rr2 += rr[k] * rr[k];
             }
-            p_g2 = block_sum<PB>(p_g2);
-            p_d2 = block_sum<PB>(p_d2);
-            p_rr2 = block_sum<PB>(p_rr2);
+            block_sum3<PB>(p_g2, p_d2, p_rr2);
             if (a.probe) {
                 const uint64_t t1 = wall_clock64();
                 t_compute += t1 - t_mark;

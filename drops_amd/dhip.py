@@ -12,7 +12,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libdhip.so")
 
-OK, E_ZERO_NODE, E_ZERO_PATTERN, E_ARG, E_NOT_CONVERGED, E_NOMEM, E_HIP, E_STATE, E_CAPACITY = range(9)
+OK, E_ZERO_NODE, E_ZERO_PATTERN, E_ARG, E_NOT_CONVERGED, E_NOMEM, E_HIP, E_STATE, E_CAPACITY, E_IO = range(10)
+MTX_MARKET, MTX_PLAIN = 0, 1
 HOST, DEVICE = 0, 1
 ZERO_THROW, ZERO_ALLOW = 0, 1
 OP_MASS, OP_LB, OP_STAB, OP_COMB = range(4)
@@ -37,6 +38,8 @@ EXPORTS = [
     "dhip_ch_residual", "dhip_ch_predict_start", "dhip_ch_solve_learned", "dhip_local_p2", "dhip_number_p2if",
     "dhip_build_p2", "dhip_assemble_p2", "dhip_get_local_p2", "dhip_get_dof_nodes", "dhip_ch_create_p2",
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
+    "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
+    "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -155,6 +158,18 @@ def load(path=LIB_PATH):
     L.dhip_batch_problem.argtypes = [P, C.c_int32, I32, I32, D, D, D]
     L.dhip_batch_destroy.argtypes = [P]
     L.dhip_batch_destroy.restype = None
+    CP = C.c_char_p
+    L.dhip_mtx_write.argtypes = [CP, C.c_int32, C.c_int64, C.c_int64, I32, I32, D]
+    L.dhip_mtx_read.argtypes = [CP, C.c_int32, I64, I64, I64, I32, I32, D]
+    L.dhip_csr_export.argtypes = [P, CP, C.c_int32]
+    L.dhip_csr_import.argtypes = [P, CP, C.c_int32, C.POINTER(P)]
+    L.dhip_vec_write.argtypes = [P, CP, VP, C.c_int64, C.c_int32, C.c_int32]
+    L.dhip_vec_read.argtypes = [P, CP, VP, C.c_int64, C.c_int32, C.c_int32]
+    L.dhip_vec_count.argtypes = [CP, C.c_int32, I64]
+    L.dhip_mtx_format.argtypes = [C.c_int32, C.c_int64, C.c_int64, I32, I32, D, VP, C.c_int64, I64]
+    L.dhip_mtx_parse.argtypes = [VP, C.c_int64, C.c_int32, I64, I64, I64, I32, I32, D]
+    L.dhip_vec_format.argtypes = [P, VP, C.c_int64, C.c_int32, C.c_int32, VP, C.c_int64, I64]
+    L.dhip_vec_parse.argtypes = [P, VP, C.c_int64, VP, C.c_int64, C.c_int32, C.c_int32]
     _lib = L
     return L
 
@@ -171,6 +186,117 @@ def _vp(a):
     return C.c_void_p(a.ctypes.data)
 
 
+# ---- on-disk formats on host memory (no GPU needed) ----
+def _fmt(fmt):
+    return {"mtx": MTX_MARKET, "plain": MTX_PLAIN}[fmt] if isinstance(fmt, str) else int(fmt)
+
+
+def _check_noctx(L, rc):
+    if rc != OK:
+        raise DhipError(rc, L.dhip_last_error(None).decode())
+
+
+def mtx_write(path, rowptr, colind, val, ncols=None, fmt="mtx"):
+    """MatrixCL::exportMTX (fmt "mtx") or out() (fmt "plain") of a host CSR matrix."""
+    L = load()
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colind, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    n = rp.size - 1
+    _check_noctx(L, L.dhip_mtx_write(os.fsencode(path), _fmt(fmt), n, n if ncols is None else int(ncols),
+                                     _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_double)))
+
+
+def mtx_read(path, fmt="mtx"):
+    """dict(shape, rowptr, colind, val) of a matrix file (MatrixMarket or plain)."""
+    L = load()
+    nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+    b = os.fsencode(path)
+    _check_noctx(L, L.dhip_mtx_read(b, _fmt(fmt), C.byref(nr), C.byref(nc), C.byref(nz), None, None, None))
+    rp = np.zeros(nr.value + 1, np.int32)
+    ci = np.zeros(max(nz.value, 1), np.int32)
+    v = np.zeros(max(nz.value, 1), np.float64)
+    _check_noctx(L, L.dhip_mtx_read(b, _fmt(fmt), C.byref(nr), C.byref(nc), C.byref(nz), _p(rp, C.c_int32),
+                                    _p(ci, C.c_int32), _p(v, C.c_double)))
+    return dict(shape=(nr.value, nc.value), rowptr=rp, colind=ci[:nz.value], val=v[:nz.value])
+
+
+def mtx_format(rowptr, colind, val, ncols=None, fmt="mtx"):
+    """bytes of operator<< (fmt "mtx") or out() (fmt "plain") of a host CSR matrix"""
+    L = load()
+    rp = np.ascontiguousarray(rowptr, np.int32)
+    ci = np.ascontiguousarray(colind, np.int32)
+    v = np.ascontiguousarray(val, np.float64)
+    n = rp.size - 1
+    nc = n if ncols is None else int(ncols)
+    cap = C.c_int64()
+    args = (_fmt(fmt), n, nc, _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_double))
+    _check_noctx(L, L.dhip_mtx_format(*args, None, 0, C.byref(cap)))
+    buf = C.create_string_buffer(max(cap.value, 1))
+    ln = C.c_int64()
+    _check_noctx(L, L.dhip_mtx_format(*args, C.cast(buf, C.c_void_p), cap.value, C.byref(ln)))
+    return buf.raw[:ln.value]
+
+
+def mtx_parse(data, fmt="mtx"):
+    """dict(shape, rowptr, colind, val) of matrix text held in memory"""
+    L = load()
+    b = bytes(data)
+    nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+    _check_noctx(L, L.dhip_mtx_parse(b, len(b), _fmt(fmt), C.byref(nr), C.byref(nc), C.byref(nz), None, None, None))
+    rp = np.zeros(nr.value + 1, np.int32)
+    ci = np.zeros(max(nz.value, 1), np.int32)
+    v = np.zeros(max(nz.value, 1), np.float64)
+    _check_noctx(L, L.dhip_mtx_parse(b, len(b), _fmt(fmt), C.byref(nr), C.byref(nc), C.byref(nz),
+                                     _p(rp, C.c_int32), _p(ci, C.c_int32), _p(v, C.c_double)))
+    return dict(shape=(nr.value, nc.value), rowptr=rp, colind=ci[:nz.value], val=v[:nz.value])
+
+
+def vec_format(x, binary=True):
+    """bytes of VecDescBaseCL::Write of a host vector"""
+    L = load()
+    a = np.ascontiguousarray(x, np.float64)
+    cap = C.c_int64()
+    _check_noctx(L, L.dhip_vec_format(None, _vp(a), a.size, HOST, int(binary), None, 0, C.byref(cap)))
+    buf = C.create_string_buffer(max(cap.value, 1))
+    ln = C.c_int64()
+    _check_noctx(L, L.dhip_vec_format(None, _vp(a), a.size, HOST, int(binary), C.cast(buf, C.c_void_p), cap.value,
+                                      C.byref(ln)))
+    return buf.raw[:ln.value]
+
+
+def vec_parse(data, n, binary=True):
+    """VecDescBaseCL::Read of n values from bytes held in memory"""
+    L = load()
+    b = bytes(data)
+    out = np.zeros(max(int(n), 1), np.float64)
+    _check_noctx(L, L.dhip_vec_parse(None, b, len(b), _vp(out), int(n), HOST, int(binary)))
+    return out[:int(n)]
+
+
+def vec_write(path, x, binary=True):
+    """VecDescBaseCL::Write of a host vector."""
+    L = load()
+    a = np.ascontiguousarray(x, np.float64)
+    _check_noctx(L, L.dhip_vec_write(None, os.fsencode(path), _vp(a), a.size, HOST, int(binary)))
+
+
+def vec_count(path, binary=True):
+    L = load()
+    n = C.c_int64()
+    _check_noctx(L, L.dhip_vec_count(os.fsencode(path), int(binary), C.byref(n)))
+    return n.value
+
+
+def vec_read(path, n=None, binary=True):
+    """VecDescBaseCL::Read into a new host vector; n (default: the stored count) must match."""
+    L = load()
+    n = vec_count(path, binary) if n is None else int(n)
+    out = np.zeros(max(n, 1), np.float64)
+    _check_noctx(L, L.dhip_vec_read(None, os.fsencode(path), _vp(out), n, HOST, int(binary)))
+    return out[:n]
+
+
 class Csr:
     """Device CSR matrix handle (MatrixCL mirror).  Owned by its Context: closing the
     context closes its matrices first (a handle must not outlive its context)."""
@@ -184,6 +310,10 @@ class Csr:
         n, z = C.c_int64(), C.c_int64()
         self.ctx.L.dhip_csr_info(self.h, C.byref(n), C.byref(z))
         return n.value, z.value
+
+    def export(self, path, fmt="mtx"):
+        """MatrixCL::exportMTX (or out()) of the device matrix."""
+        self.ctx.check(self.ctx.L.dhip_csr_export(self.h, os.fsencode(path), _fmt(fmt)))
 
     def close(self):
         if self.h and self.ctx.h:
@@ -627,6 +757,20 @@ class Context:
         self.check(self.L.dhip_csr_create(self.h, rowptr.size - 1, colind.size, _vp(rowptr), _vp(colind), _vp(val),
                                           HOST, C.byref(h)))
         return Csr(self, h)
+
+    def csr_import(self, path, fmt="mtx"):
+        """a square matrix file (MatrixMarket or plain) into a new device matrix"""
+        h = C.c_void_p()
+        self.check(self.L.dhip_csr_import(self.h, os.fsencode(path), _fmt(fmt), C.byref(h)))
+        return Csr(self, h)
+
+    def vec_write(self, path, x_ptr, n, binary=True):
+        """VecDescBaseCL::Write of a device vector (x_ptr: device address)"""
+        self.check(self.L.dhip_vec_write(self.h, os.fsencode(path), _vp(x_ptr), int(n), DEVICE, int(binary)))
+
+    def vec_read(self, path, x_ptr, n, binary=True):
+        """VecDescBaseCL::Read into a device vector of n values"""
+        self.check(self.L.dhip_vec_read(self.h, os.fsencode(path), _vp(x_ptr), int(n), DEVICE, int(binary)))
 
     def csr_from_assembly(self, op=OP_COMB):
         h = C.c_void_p()

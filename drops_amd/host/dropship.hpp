@@ -32,7 +32,11 @@
 
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <functional>
+#include <istream>
+#include <iterator>
+#include <ostream>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -233,9 +237,85 @@ class SparseMatBaseCL {
         return LinComb(1., tmp, args...);
     }
 
+    // MatrixMarket file of the matrix (num/spmat.h:1084-1089)
+    SparseMatBaseCL &exportMTX(std::string const &path);
+    // needs the MATLAB engine (num/spmat.h:1095-1125); this build has none
+    SparseMatBaseCL &exportMAT(std::string const &) { throw std::invalid_argument("DROPS is compiled w/o matlab"); }
+
     template <class U>
     friend class SparseMatBuilderCL;
 };
+
+// ---------------------------------------------------------------------------
+// file formats (num/spmat.h:1183-1255, misc/problem.h:928-975), written by libdhip
+// ---------------------------------------------------------------------------
+namespace detail {
+inline void host_check(int rc) {
+    if (rc != DHIP_OK) throw DROPSErrCL(dhip_last_error(nullptr));
+}
+// int32 copies of the size_t CSR arrays
+inline void csr32(const MatrixCL &A, std::vector<int32_t> &rp, std::vector<int32_t> &ci) {
+    if (A.num_nonzeros() >= (size_t(1) << 31) || A.num_rows() >= (size_t(1) << 31))
+        throw DROPSErrCL("matrix too large for int32 indices");
+    rp.resize(A.num_rows() + 1);
+    ci.resize(A.num_nonzeros());
+    for (size_t i = 0; i <= A.num_rows(); ++i) rp[i] = (int32_t)A.row_beg(i);
+    for (size_t k = 0; k < A.num_nonzeros(); ++k) ci[k] = (int32_t)A.col_ind(k);
+}
+inline std::string format_matrix(const MatrixCL &A, int32_t fmt) {
+    std::vector<int32_t> rp, ci;
+    csr32(A, rp, ci);
+    int64_t cap = 0, len = 0;
+    host_check(dhip_mtx_format(fmt, (int64_t)A.num_rows(), (int64_t)A.num_cols(), rp.data(), ci.data(), A.raw_val(),
+                               nullptr, 0, &cap));
+    std::string s((size_t)cap, '\0');
+    host_check(dhip_mtx_format(fmt, (int64_t)A.num_rows(), (int64_t)A.num_cols(), rp.data(), ci.data(), A.raw_val(),
+                               &s[0], cap, &len));
+    s.resize((size_t)len);
+    return s;
+}
+inline std::string rest_of(std::istream &is) {
+    return std::string(std::istreambuf_iterator<char>(is), std::istreambuf_iterator<char>());
+}
+}  // namespace detail
+
+// MatrixMarket text (operator<<, num/spmat.h:1213-1226)
+inline std::ostream &operator<<(std::ostream &os, const MatrixCL &A) {
+    const std::string s = detail::format_matrix(A, DHIP_MTX_MARKET);
+    os.write(s.data(), (std::streamsize)s.size());
+    return os << std::flush;
+}
+// out() / in() (num/spmat.h:1228-1255)
+inline void out(std::ostream &os, const MatrixCL &A) {
+    const std::string s = detail::format_matrix(A, DHIP_MTX_PLAIN);
+    os.write(s.data(), (std::streamsize)s.size());
+    os.flush();
+}
+inline void in(std::istream &is, MatrixCL &A) {
+    const std::string s = detail::rest_of(is);
+    int64_t nr = 0, nc = 0, nz = 0;
+    detail::host_check(dhip_mtx_parse(s.data(), (int64_t)s.size(), DHIP_MTX_PLAIN, &nr, &nc, &nz, nullptr, nullptr,
+                                      nullptr));
+    std::vector<int32_t> rp((size_t)nr + 1), ci((size_t)nz + 1);
+    std::vector<double> v((size_t)nz + 1);
+    detail::host_check(dhip_mtx_parse(s.data(), (int64_t)s.size(), DHIP_MTX_PLAIN, &nr, &nc, &nz, rp.data(), ci.data(),
+                                      v.data()));
+    A.resize((size_t)nr, (size_t)nc, (size_t)nz);
+    for (int64_t i = 0; i <= nr; ++i) A.raw_row()[i] = (size_t)rp[i];
+    for (int64_t k = 0; k < nz; ++k) {
+        A.raw_col()[k] = (size_t)ci[k];
+        A.raw_val()[k] = v[k];
+    }
+}
+
+template <class T>
+SparseMatBaseCL<T> &SparseMatBaseCL<T>::exportMTX(std::string const &path) {
+    std::vector<int32_t> rp, ci;
+    detail::csr32(*this, rp, ci);
+    detail::host_check(dhip_mtx_write(path.c_str(), DHIP_MTX_MARKET, (int64_t)rows_, (int64_t)cols_, rp.data(),
+                                      ci.data(), raw_val()));
+    return *this;
+}
 
 // y = A x on the device (y_Ax, num/spmat.h:1683-1722)
 inline VectorCL operator*(const MatrixCL &A, const VectorCL &x) {
@@ -482,6 +562,9 @@ class VecDescCL {
     VecDescCL() {}
     explicit VecDescCL(IdxDescCL *idx);
     void SetIdx(IdxDescCL *idx);
+    // misc/problem.h:928-975: binary = size_t count + raw doubles, text = count and 15 digits per value
+    void Write(std::ostream &os, bool binary = false) const;
+    void Read(std::istream &is, bool binary = false);
 };
 
 struct LevelsetP2CL {
@@ -548,6 +631,33 @@ class IdxDescCL {
 };
 
 inline VecDescCL::VecDescCL(IdxDescCL *idx) { SetIdx(idx); }
+inline void VecDescCL::Write(std::ostream &os, bool binary) const {
+    const int64_t n = (int64_t)Data.size();
+    int64_t cap = 0, len = 0;
+    detail::host_check(dhip_vec_format(nullptr, Data.raw(), n, DHIP_HOST, binary, nullptr, 0, &cap));
+    std::string s((size_t)cap, '\0');
+    detail::host_check(dhip_vec_format(nullptr, Data.raw(), n, DHIP_HOST, binary, &s[0], cap, &len));
+    os.write(s.data(), (std::streamsize)len);
+}
+inline void VecDescCL::Read(std::istream &is, bool binary) {
+    const std::string s = detail::rest_of(is);
+    const size_t n = RowIdx ? RowIdx->NumUnknowns() : Data.size();
+    Data.resize(n);
+    const int rc = dhip_vec_parse(nullptr, s.data(), (int64_t)s.size(), Data.raw(), (int64_t)n, DHIP_HOST, binary);
+    if (rc == DHIP_E_ARG) throw DROPSErrCL("VecDescBaseCL::Read: Number of Unknowns does not match, wrong FE-type?");
+    detail::host_check(rc);
+}
+
+// WriteFEToFile / ReadFEFromFile (out/output.cpp:335-344, 358-369) for the non-extended spaces
+inline void WriteFEToFile(const VecDescCL &v, MultiGridCL &, std::string filename, bool binary = false) {
+    const int rc = dhip_vec_write(nullptr, filename.c_str(), v.Data.raw(), (int64_t)v.Data.size(), DHIP_HOST, binary);
+    if (rc != DHIP_OK) throw DROPSErrCL("WriteFEToFile: Cannot open file " + filename + " for writing");
+}
+inline void ReadFEFromFile(VecDescCL &v, MultiGridCL &, std::string filename, bool binary = false) {
+    std::ifstream file(filename.c_str(), std::ios::binary);
+    if (!file) throw DROPSErrCL("ReadFEFromFile: Cannot open file " + filename);
+    v.Read(file, binary);
+}
 inline void VecDescCL::SetIdx(IdxDescCL *idx) {
     RowIdx = idx;
     Data.resize(idx ? idx->NumUnknowns() : 0, 0.0);

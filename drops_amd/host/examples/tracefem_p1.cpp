@@ -3,10 +3,12 @@
 // Kuhn brick, P2 level set, interface numbering, accumulator tuple, LinComb, PCG.
 //
 //   tracefem_p1 sphere20                    -> the 1156-DOF known answer (TestinterfaceP1FE.ref)
-//   tracefem_p1 c1 <n> <outdir>             -> configuration-1 problem, writes CSR/rhs/solution
+//   tracefem_p1 c1 <n> <outdir>             -> configuration-1 problem, writes CSR/rhs/solution, and the
+//                                              operator / load vector in the reference's file formats
 //   tracefem_p1 p2 <n> <outdir>             -> the same sphere on the P2 interface space (IdxDescCL(P2IF_FE),
 //                                              P2 accumulators), writes CSR/rhs/solution
 #include <cstdio>
+#include <fstream>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -94,6 +96,26 @@ static int c1(int n, const std::string &out) {
         write_bin(out + "/rhs.bin", b.Data.raw(), b.Data.size() * 8);
         write_bin(out + "/x.bin", x.raw(), x.size() * 8);
         write_bin(out + "/vertex_of_dof.bin", ifaceidx.vertex_of_dof().data(), ifaceidx.vertex_of_dof().size() * 8);
+        // the reference's file formats: exportMTX, out(), WriteFEToFile (binary and text)
+        L.exportMTX(out + "/L.mtx");
+        {
+            std::ofstream os(out + "/L.txt");
+            DROPS::out(os, L);
+        }
+        WriteFEToFile(b, mg, out + "/rhs.dat", true);
+        WriteFEToFile(b, mg, out + "/rhs.txt", false);
+        VecDescCL back(&ifaceidx);
+        ReadFEFromFile(back, mg, out + "/rhs.dat", true);
+        bool same = back.Data.size() == b.Data.size();
+        for (size_t i = 0; same && i < b.Data.size(); ++i) same = back.Data[i] == b.Data[i];
+        MatrixCL L2;
+        {
+            std::ifstream is(out + "/L.txt");
+            DROPS::in(is, L2);
+        }
+        same = same && L2.num_rows() == L.num_rows() && L2.num_nonzeros() == L.num_nonzeros();
+        for (size_t k = 0; same && k < L.num_nonzeros(); ++k) same = L2.col_ind(k) == L.col_ind(k);
+        std::printf("formats round trip: %s\n", same ? "ok" : "MISMATCH");
     }
     return 0;
 }

@@ -46,6 +46,7 @@ extern "C" {
 #define DHIP_E_HIP 6
 #define DHIP_E_STATE 7           /* call out of order (e.g. number before classify) */
 #define DHIP_E_CAPACITY 8        /* size beyond what the library supports */
+#define DHIP_E_IO 9              /* file cannot be opened, written or parsed */
 
 #define DHIP_HOST 0
 #define DHIP_DEVICE 1
@@ -362,6 +363,48 @@ int dhip_batch_stats(dhip_batch_t *batch, dhip_batch_stat_t *stats);
 int dhip_batch_problem(dhip_batch_t *batch, int32_t p, int32_t *rowptr, int32_t *colind, double *comb,
                        double *rhs, double *x);
 void dhip_batch_destroy(dhip_batch_t *batch);
+
+/* ---- on-disk formats ----
+ * Matrices: DHIP_MTX_MARKET is MatrixCL::exportMTX, i.e. operator<< (num/spmat.h:1084-1089,
+ * 1213-1226): "%%MatrixMarket matrix coordinate real general", "rows cols nnz", one line
+ * "row+1 col+1 value" per entry in CSR order, value as printf %.15e.  DHIP_MTX_PLAIN is
+ * out()/in() (num/spmat.h:1228-1255): "rows cols nnz", then row pointers, column indices and
+ * values (%g, the stream's default 6 digits), each item followed by a blank, one line per array.
+ * MatrixCL::exportMAT needs the MATLAB engine and has no counterpart here.
+ * Vectors: VecDescBaseCL::Write / Read (misc/problem.h:928-975): binary = the count as a
+ * 64-bit size_t and the raw doubles; text = the count, six blanks, every value as %.15g
+ * followed by a blank, a newline.  Reading checks the stored count against n (the reference's
+ * "Number of Unknowns does not match": DHIP_E_ARG).  Calls on host memory need no context
+ * (ctx may be NULL; dhip_last_error(NULL) then holds the message of this thread's last failure). */
+#define DHIP_MTX_MARKET 0
+#define DHIP_MTX_PLAIN 1
+/* host arrays; rowptr[nrows] entries.  _format writes to a caller buffer of cap bytes
+ * (buf == NULL: *len = an upper bound of the size; a short buffer gives DHIP_E_CAPACITY) and
+ * sets *len to the bytes written: what operator<< / out() put on a stream. */
+int dhip_mtx_write(const char *path, int32_t format, int64_t nrows, int64_t ncols, const int32_t *rowptr,
+                   const int32_t *colind, const double *val);
+int dhip_mtx_format(int32_t format, int64_t nrows, int64_t ncols, const int32_t *rowptr, const int32_t *colind,
+                    const double *val, char *buf, int64_t cap, int64_t *len);
+/* host arrays.  rowptr == NULL: sizes only (*nnz = the stored count, an upper bound).  Otherwise
+ * rowptr (nrows+1), colind and val (*nnz from the first call) are filled and *nnz is the final
+ * count: MatrixMarket entries may come in any order and are sorted by row, then column, with
+ * repeated (row, col) entries summed in file order.  _parse reads len bytes of a buffer. */
+int dhip_mtx_read(const char *path, int32_t format, int64_t *nrows, int64_t *ncols, int64_t *nnz, int32_t *rowptr,
+                  int32_t *colind, double *val);
+int dhip_mtx_parse(const char *buf, int64_t len, int32_t format, int64_t *nrows, int64_t *ncols, int64_t *nnz,
+                   int32_t *rowptr, int32_t *colind, double *val);
+/* a device matrix to a file / a square matrix from a file into a new handle */
+int dhip_csr_export(const dhip_csr_t *A, const char *path, int32_t format);
+int dhip_csr_import(dhip_ctx_t *ctx, const char *path, int32_t format, dhip_csr_t **A);
+/* vectors from / to files or caller buffers (as dhip_mtx_format / dhip_mtx_parse) */
+int dhip_vec_write(dhip_ctx_t *ctx, const char *path, const double *x, int64_t n, int32_t where, int32_t binary);
+int dhip_vec_format(dhip_ctx_t *ctx, const double *x, int64_t n, int32_t where, int32_t binary, char *buf,
+                    int64_t cap, int64_t *len);
+int dhip_vec_read(dhip_ctx_t *ctx, const char *path, double *x, int64_t n, int32_t where, int32_t binary);
+int dhip_vec_parse(dhip_ctx_t *ctx, const char *buf, int64_t len, double *x, int64_t n, int32_t where,
+                   int32_t binary);
+/* the count stored in a vector file */
+int dhip_vec_count(const char *path, int32_t binary, int64_t *n);
 
 #ifdef __cplusplus
 }

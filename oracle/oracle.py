@@ -410,3 +410,57 @@ def ch_step(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, mu, 
     x, it, res, ok = gmres_diag(brp, bcol, bval, pcd, b, x0, restart=restart, max_iter=max_iter, tol=tol)
     return dict(mu=x[:ndof], c=x[ndof:], iters=it, res=res, ok=ok, lm=lm, timeder=timeder, well=well,
                 rhs=b, pcdiag=pcd, rowptr=brp, col=bcol, val=bval, x0=x0)
+
+
+# ---------------------------------------------------------------- file formats (iostreams)
+_fmt_lib = None
+
+
+def fmt_lib():
+    global _fmt_lib
+    if _fmt_lib is None:
+        L = C.CDLL(os.path.join(HERE, "_build", "libformat_oracle.so"))
+        I64p, Dp, CP = C.POINTER(C.c_int64), C.POINTER(C.c_double), C.c_char_p
+        L.orc_fmt_mtx.argtypes = [CP, C.c_int64, C.c_int64, I64p, I64p, Dp]
+        L.orc_fmt_plain.argtypes = [CP, C.c_int64, C.c_int64, I64p, I64p, Dp]
+        L.orc_read_plain.argtypes = [CP, I64p, I64p, I64p, I64p, I64p, Dp]
+        L.orc_fmt_vec.argtypes = [CP, Dp, C.c_int64, C.c_int]
+        L.orc_read_vec.argtypes = [CP, Dp, C.c_int64, C.c_int]
+        _fmt_lib = L
+    return _fmt_lib
+
+
+def fmt_matrix(path, rowptr, colind, val, ncols=None, fmt="mtx"):
+    """operator<< (fmt "mtx", the body of exportMTX) or out() (fmt "plain") through iostreams"""
+    rp = np.ascontiguousarray(rowptr, np.int64)
+    ci = np.ascontiguousarray(colind, np.int64)
+    v = np.ascontiguousarray(val, np.float64)
+    n = rp.size - 1
+    f = fmt_lib().orc_fmt_mtx if fmt == "mtx" else fmt_lib().orc_fmt_plain
+    _check(f(os.fsencode(path), n, n if ncols is None else ncols, _ptr(rp, C.c_int64), _ptr(ci, C.c_int64),
+             _ptr(v, C.c_double)), "fmt_matrix")
+
+
+def read_plain(path):
+    """in(): dict(shape, rowptr, colind, val)"""
+    nr, nc, nz = C.c_int64(), C.c_int64(), C.c_int64()
+    b = os.fsencode(path)
+    _check(fmt_lib().orc_read_plain(b, C.byref(nr), C.byref(nc), C.byref(nz), None, None, None), "read_plain")
+    rp = np.zeros(nr.value + 1, np.int64)
+    ci = np.zeros(max(nz.value, 1), np.int64)
+    v = np.zeros(max(nz.value, 1))
+    _check(fmt_lib().orc_read_plain(b, C.byref(nr), C.byref(nc), C.byref(nz), _ptr(rp, C.c_int64),
+                                    _ptr(ci, C.c_int64), _ptr(v, C.c_double)), "read_plain")
+    return dict(shape=(nr.value, nc.value), rowptr=rp, colind=ci[:nz.value], val=v[:nz.value])
+
+
+def fmt_vector(path, x, binary=True):
+    a = np.ascontiguousarray(x, np.float64)
+    _check(fmt_lib().orc_fmt_vec(os.fsencode(path), _ptr(a, C.c_double), a.size, int(binary)), "fmt_vector")
+
+
+def read_vector(path, n, binary=True):
+    """VecDescBaseCL::Read; raises OracleError(1) when the stored count differs from n"""
+    out = np.zeros(max(n, 1))
+    _check(fmt_lib().orc_read_vec(os.fsencode(path), _ptr(out, C.c_double), n, int(binary)), "read_vector")
+    return out[:n]

@@ -54,6 +54,14 @@ def test_c1_through_host_mirror(tmp_path):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
     _, itr, _, _ = O.pcg_jacobi(ref["rowptr"], ref["colind"], comb, ref["rhs"], tol=1e-10)
     assert abs(it - itr) <= max(2, 0.02 * itr)
+    # exportMTX, out() and WriteFEToFile of the mirror: the iostream restatement's bytes
+    assert "formats round trip: ok" in out
+    for name, fmt in (("L.mtx", "mtx"), ("L.txt", "plain")):
+        O.fmt_matrix(str(tmp_path / ("ref_" + name)), ref["rowptr"], ref["colind"], comb, fmt=fmt)
+        assert (tmp_path / name).read_bytes() == (tmp_path / ("ref_" + name)).read_bytes(), name
+    for name, binary in (("rhs.dat", True), ("rhs.txt", False)):
+        O.fmt_vector(str(tmp_path / ("ref_" + name)), ref["rhs"], binary=binary)
+        assert (tmp_path / name).read_bytes() == (tmp_path / ("ref_" + name)).read_bytes(), name
 
 
 def test_p2_through_host_mirror(tmp_path):
