@@ -509,7 +509,7 @@ int dhip_levelset_p2(dhip_ctx_t *c, const double *phi, int64_t count, int32_t wh
     return DHIP_OK;
 }
 
-static int classify_impl(dhip_ctx *c, int32_t zero_policy, int64_t *ncut, bool sync_count) {
+static int classify_impl(dhip_ctx *c, int32_t zero_policy, int64_t *ncut, bool sync_count, double band = -1.0) {
     if (!c->has_mesh || !c->phi) return set_error(c, DHIP_E_STATE, "mesh and level set required");
     const int64_t ncell = c->mesh.ncell;
     const unsigned nb = blocks_for(ncell, CLS_BLOCK);
@@ -518,7 +518,9 @@ static int classify_impl(dhip_ctx *c, int32_t zero_policy, int64_t *ncut, bool s
     DHIP_HIPCHK(c, c->cellinfo.ensure(sizeof(uint32_t) * ncell));
     DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
     ClassArgs ca{c->mesh, c->phi, c->cellmask.as<uint8_t>(), c->blockcnt.as<int32_t>(), c->errflag.as<int32_t>(),
-                 zero_policy == DHIP_ZERO_THROW};
+                 zero_policy == DHIP_ZERO_THROW, band};
+    c->stage = 0;
+    c->band = band;
     DHIP_LAUNCH(c, "classify", k_classify, nb, CLS_BLOCK, 0, ca);
     int64_t total = 0;
     int rc = scan_exclusive_i32(c, c->blockcnt.as<int32_t>(), c->blockcnt.as<int32_t>(), nb, &total);
@@ -556,7 +558,7 @@ static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const
     for (int k = 0; k < 3; ++k) c->coef[k] = coef ? coef[k] : 1.0;
     LocalArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->coef[0], c->coef[1], c->coef[2], rhs_kind,
                  rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
-                 c->errflag.as<int32_t>()};
+                 c->errflag.as<int32_t>(), c->band >= 0. ? 1 : 0};
     if (N > 0) DHIP_LAUNCH(c, "local_p1", k_local_p1, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la);
     int rc = check_err_flag(c);
     if (rc) return rc;
@@ -582,7 +584,8 @@ static int number_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
     DHIP_HIPCHK(c, c->ownrank.ensure(sizeof(int32_t) * (S + 1)));
     DHIP_HIPCHK(c, c->dofs.ensure(sizeof(int32_t) * (S + 1)));
     if (N > 0) {
-        QualArgs qa{c->mesh, c->cut.as<int64_t>(), N, c->loc.as<double>(), c->flags.as<uint8_t>(), omit_bound};
+        QualArgs qa{c->mesh, c->cut.as<int64_t>(), N, c->loc.as<double>(), c->flags.as<uint8_t>(), omit_bound,
+                    c->band >= 0. ? 1 : 0};
         DHIP_LAUNCH(c, "qualify", k_qualify, blocks_for(N, 256), 256, 0, qa);
         FirstArgs fa{c->mesh, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->flags.as<uint8_t>(),
                      c->first.as<int32_t>(), c->isown.as<int32_t>()};
@@ -670,11 +673,27 @@ int dhip_assemble_p1(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], i
     return flush_profile(c);
 }
 
+int dhip_assemble_p1_band(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                          const double *rhs_param, double width, const double lc[3], int64_t *nband, int64_t *ndof,
+                          int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    if (!(width >= 0.)) return set_error(c, DHIP_E_ARG, "narrow-band width must be >= 0");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = classify_impl(c, zero_policy, nband, false, width);
+    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param);
+    if (!rc) rc = number_impl(c, -1.0, ndof);
+    if (!rc) rc = build_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
 // ---------------------------------------------------------------------------
 // P2 interface space
 // ---------------------------------------------------------------------------
 static int local_p2_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
     if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_local_p2");
+    if (c->band >= 0.) return set_error(c, DHIP_E_STATE, "the narrow band is a P1 assembly");
     const int64_t N = c->ncut;
     DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC2_N * (N + 1)));
     DHIP_HIPCHK(c, c->flags.ensure(N + 1));

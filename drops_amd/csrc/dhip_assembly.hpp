@@ -130,11 +130,15 @@ struct ClassArgs {
     int32_t *blockcnt;
     int32_t *err;
     int zero_throw;
+    double band = -1.0;   // >= 0: narrow band of that width
 };
 
 // 6-bit cut mask of brick cell `cell` (equalSigns on each Kuhn tet's 10 values); `zero` reports an
-// exact-zero node in a cut tet
-__device__ inline int cell_cut_mask(const MeshP &m, const double *phi, int64_t cell, bool &zero) {
+// exact-zero node in a cut tet.  With band >= 0 the mask also holds the uncut tets whose smallest
+// |value| is at most band: distance() <= band (geom/subtriangulation.tpp:38-46,
+// NarrowBandCommonDataP1CL::visit, surfactant/ifacetransp.h:688-700).
+__device__ inline int cell_cut_mask(const MeshP &m, const double *phi, int64_t cell, bool &zero,
+                                    double band = -1.0) {
     int mask = 0;
     zero = false;
     {
@@ -164,6 +168,19 @@ __device__ inline int cell_cut_mask(const MeshP &m, const double *phi, int64_t c
                 }
             }
         }
+        if (band >= 0. && mask != 63) {
+            double mn[6] = {INFINITY, INFINITY, INFINITY, INFINITY, INFINITY, INFINITY};
+#pragma unroll
+            for (int b = 0; b < 27; ++b) {
+                const double av = fabs(base[((b / 9) * h2 + (b / 3) % 3) * h1 + b % 3]);
+#pragma unroll
+                for (int k = 0; k < 6; ++k)
+                    if ((c_tab.tetmask27[k] >> b) & 1u) mn[k] = av < mn[k] ? av : mn[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                if (mn[k] <= band) mask |= 1 << k;
+        }
     }
     return mask;
 }
@@ -173,7 +190,7 @@ __global__ void __launch_bounds__(CLS_BLOCK) k_classify(ClassArgs a) {
     int mask = 0;
     if (cell < a.m.ncell) {
         bool zero;
-        mask = cell_cut_mask(a.m, a.phi, cell, zero);
+        mask = cell_cut_mask(a.m, a.phi, cell, zero, a.band);
         if (zero && a.zero_throw) atomicOr(a.err, 1);
         a.cellmask[cell] = (uint8_t)mask;
     }
@@ -327,11 +344,56 @@ struct LocalArgs {
     int rhs_kind;
     double rhs_p0;
     double *loc;      // SoA [LOC_N][ncut]
-    uint8_t *flags;   // bit0: non-empty patch
+    uint8_t *flags;   // bit0: contributes (non-empty patch, or a narrow-band tet); bit5: band tet without patch
     int32_t *err;
+    int band = 0;     // narrow band: listed tets without a patch still carry the stabilisation
 };
 
 constexpr int LOCAL_BLOCK = 64;
+
+// normal-gradient stabilisation on the Quad5 volume rule, normal = grad(phi_h)/|grad(phi_h)|
+// (LocalNormalLaplaceBulkP1CL::setup, surfactant/ifacetransp.h:2005-2051)
+__device__ inline void stab_p1(const LocalArgs &a, const double ls[10], const double G1[4][3], double det,
+                               int64_t c) {
+    const int64_t N = a.ncut;
+    double *L = a.loc;
+    {
+        double v[10];
+#pragma unroll
+        for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+        double R[10], S[10];
+        for (int qn = 0; qn < 15; ++qn) {
+            double n3[3];
+            p2_unit_normal(v, c_tab.v5n[qn], G1, n3);
+            double U[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) U[i] = dot3(n3, G1[i]);
+            int o = 0;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j <= i; ++j, ++o) {
+                    const double f = U[i] * U[j];
+                    if (qn == 0) R[o] = 8. / 405. * f;
+                    else if (qn == 1 || qn == 5 || qn == 9) S[o] = f;
+                    else S[o] += f;
+                    if (qn == 4) R[o] = R[o] + c_tab.v5c1 * S[o];
+                    else if (qn == 8) R[o] = R[o] + c_tab.v5c2 * S[o];
+                    else if (qn == 14) R[o] = R[o] + 5. / 567. * S[o];
+                }
+        }
+        const double ad = fabs(det) / 6;
+        int o = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) {
+                const double val = a.coef2 * (ad * R[o++]);
+                L[(LOC_STAB + i * 4 + j) * N + c] = val;
+                L[(LOC_STAB + j * 4 + i) * N + c] = val;
+            }
+    }
+}
 
 __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
     double X[4][3], ls[10];
@@ -429,9 +491,11 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
 
     const int64_t N = a.ncut;
     double *L = a.loc;
-    a.flags[c] = nfacet > 0 ? 1 : 0;
+    a.flags[c] = nfacet > 0 ? 1 : (a.band ? 0x21 : 0);
     if (nfacet == 0) {
         for (int e = 0; e < LOC_N; ++e) L[e * N + c] = 0.;
+        // NarrowBandMatrixAccuP1CL: the stabilisation of a band tet does not need the patch
+        if (a.band) stab_p1(a, ls, G1, det, c);
         return;
     }
     // mass: diagonal without alpha, off-diagonal alpha * integral
@@ -466,44 +530,7 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
         L[(LOC_HAT + i) * N + c] = hat[i];
     }
     L[LOC_AREA * N + c] = area;
-
-    // normal-gradient stabilisation on the Quad5 volume rule, normal = grad(phi_h)/|grad(phi_h)|
-    {
-        double v[10];
-#pragma unroll
-        for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
-        double R[10], S[10];
-        for (int qn = 0; qn < 15; ++qn) {
-            double n3[3];
-            p2_unit_normal(v, c_tab.v5n[qn], G1, n3);
-            double U[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) U[i] = dot3(n3, G1[i]);
-            int o = 0;
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j <= i; ++j, ++o) {
-                    const double f = U[i] * U[j];
-                    if (qn == 0) R[o] = 8. / 405. * f;
-                    else if (qn == 1 || qn == 5 || qn == 9) S[o] = f;
-                    else S[o] += f;
-                    if (qn == 4) R[o] = R[o] + c_tab.v5c1 * S[o];
-                    else if (qn == 8) R[o] = R[o] + c_tab.v5c2 * S[o];
-                    else if (qn == 14) R[o] = R[o] + 5. / 567. * S[o];
-                }
-        }
-        const double ad = fabs(det) / 6;
-        int o = 0;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j <= i; ++j) {
-                const double val = a.coef2 * (ad * R[o++]);
-                L[(LOC_STAB + i * 4 + j) * N + c] = val;
-                L[(LOC_STAB + j * 4 + i) * N + c] = val;
-            }
-    }
+    stab_p1(a, ls, G1, det, c);
 }
 
 __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
@@ -522,11 +549,15 @@ struct QualArgs {
     const double *loc;
     uint8_t *flags;
     double omit_bound;
+    int band = 0;   // narrow band: every vertex of a listed tet (CreateNumbNearInterfaceVertex)
 };
 
 __device__ inline void qualify_one(const QualArgs &a, int64_t c) {
     int f = a.flags[c] & 1;
-    if (f) {
+    const int keep = a.flags[c] & 0x20;
+    if (f && a.band) {
+        f |= 0x1E;
+    } else if (f) {
         double limit = 0.;
         if (a.omit_bound >= 0.) {
             const int64_t t = a.cut[c];
@@ -542,7 +573,7 @@ __device__ inline void qualify_one(const QualArgs &a, int64_t c) {
         for (int i = 0; i < 4; ++i)
             if (a.loc[(int64_t)(LOC_HAT + i) * a.ncut + c] > limit) f |= 2 << i;
     }
-    a.flags[c] = (uint8_t)f;
+    a.flags[c] = (uint8_t)(f | keep);
 }
 
 __global__ void k_qualify(QualArgs a) {

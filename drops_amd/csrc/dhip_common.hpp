@@ -117,6 +117,7 @@ struct dhip_ctx {
     // assembly state: 0 none, 1 classified, 2 local, 3 numbered, 4 built
     int stage = 0;
     int space = 1;   // 1: P1 interface space, 2: P2 interface space (set by the local step)
+    double band = -1.0;  // >= 0: the listed tets are the narrow band of this width (P1 only)
     int zero_policy = DHIP_ZERO_THROW;
     int64_t ncut = 0, ndof = 0, nnz = 0;
     double coef[3] = {1, 1, 1};

@@ -40,6 +40,7 @@ EXPORTS = [
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
+    "dhip_assemble_p1_band",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -108,6 +109,7 @@ def load(path=LIB_PATH):
     L.dhip_number_p2if.argtypes = [P, C.c_double, I64]
     L.dhip_build_p2.argtypes = [P, D, I64]
     L.dhip_assemble_p2.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
+    L.dhip_assemble_p1_band.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
     L.dhip_get_local_p2.argtypes = [P, D, C.POINTER(C.c_uint8)]
     L.dhip_get_dof_nodes.argtypes = [P, I64]
     L.dhip_get_tet_dofs.argtypes = [P, I32]
@@ -685,6 +687,20 @@ class Context:
         a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
         self.check(self.L.dhip_assemble_p1(self.h, zero_policy, _p(co, C.c_double), rhs_kind, _p(rp, C.c_double),
                                            omit_bound, _p(lcv, C.c_double), C.byref(a), C.byref(b), C.byref(c)))
+        self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 1
+        return a.value, b.value, c.value
+
+    def assemble_p1_band(self, width, zero_policy=ZERO_THROW, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE,
+                         rhs_param=None, lc=(1.0, 1.0, 1.0)):
+        """Narrow-band P1 assembly: (nband, ndof, nnz); results through csr(), cut_tets() (the band
+        tets), local() (flags bit 5: band tet without patch), dof_vertices()."""
+        co = np.asarray(coef, np.float64)
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        lcv = np.asarray(lc, np.float64)
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(self.L.dhip_assemble_p1_band(self.h, zero_policy, _p(co, C.c_double), rhs_kind,
+                                                _p(rp, C.c_double), float(width), _p(lcv, C.c_double), C.byref(a),
+                                                C.byref(b), C.byref(c)))
         self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 1
         return a.value, b.value, c.value
 

@@ -140,6 +140,20 @@ int dhip_build_p1(dhip_ctx_t *ctx, const double lc[3], int64_t *nnz);
 int dhip_assemble_p1(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
                      const double *rhs_param, double omit_bound, const double lc[3],
                      int64_t *ncut, int64_t *ndof, int64_t *nnz);
+/* The narrow-band assembly of SurfactantNarrowBandStblP1CL::Update (surfactant/ifacetransp.cpp:1110-1172):
+ * the listed tets are those with distance() <= width (NarrowBandCommonDataP1CL,
+ * surfactant/ifacetransp.h:647-700; cut tets have distance 0), numbered by
+ * CreateNumbNearInterfaceVertex (misc/problem.cpp:509-555: every vertex of every band tet, first
+ * encounter in mesh order); mass, Laplace-Beltrami and load come from the cut tets
+ * (InterfaceMatrixAccuCL), the normal stabilisation from every band tet (NarrowBandMatrixAccuP1CL<
+ * LocalNormalLaplaceBulkP1CL>, surfactant/ifacetransp.h:703-760).  The CSR pattern is the
+ * stabilisation's (it contains the others'); mass and LB hold 0.0 on the entries only band tets
+ * touch, where the reference's own mass and LB matrices have no entry.  *nband = listed tets.
+ * The getters below (CSR, cut tets = band tets, local data; flags bit 5 = band tet without patch)
+ * apply as after dhip_assemble_p1. */
+int dhip_assemble_p1_band(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3], int32_t rhs_kind,
+                          const double *rhs_param, double width, const double lc[3], int64_t *nband,
+                          int64_t *ndof, int64_t *nnz);
 
 /* ---- P2 interface space on the same cut tets and planar patch ----
  * Local forms of LocalAssembler (surfactant/ifacetransp.h:914-1619) with the P2 basis

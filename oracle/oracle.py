@@ -93,6 +93,10 @@ def lib(kind="parity"):
                               C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
     L.orc_lattice_tables.argtypes = [I32, I32]
     L.orc_quad_tables.argtypes = [D, D, D, D]
+    L.orc_classify_band.argtypes = [M, D, C.c_double, C.c_int, C.c_int64, I64, I64, C.POINTER(C.c_uint8)]
+    L.orc_number_p1band.argtypes = [M, D, C.c_double, C.c_int, I32, I64]
+    L.orc_assemble_p1band.argtypes = [M, D, I32, C.c_int64, C.c_double, D, C.c_int, D, C.c_int, C.c_int64,
+                                      I64, I64, I32, D, I64, I64, I32, D, D, D]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
     _libs[name] = L
     return L
@@ -464,3 +468,50 @@ def read_vector(path, n, binary=True):
     out = np.zeros(max(n, 1))
     _check(fmt_lib().orc_read_vec(os.fsencode(path), _ptr(out, C.c_double), n, int(binary)), "read_vector")
     return out[:n]
+
+
+# ---------------------------------------------------------------- narrow band
+def classify_band(mesh, phi, width, zero_policy=ZERO_THROW):
+    """(band tets in mesh order, is_cut flags) of distance() <= width"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    n = C.c_int64()
+    _check(lib().orc_classify_band(C.byref(mesh), _ptr(phi, C.c_double), width, zero_policy, 0, C.byref(n), None,
+                                   None), "classify_band")
+    t = np.zeros(max(n.value, 1), np.int64)
+    f = np.zeros(max(n.value, 1), np.uint8)
+    _check(lib().orc_classify_band(C.byref(mesh), _ptr(phi, C.c_double), width, zero_policy, n.value, C.byref(n),
+                                   _ptr(t, C.c_int64), _ptr(f, C.c_uint8)), "classify_band")
+    return t[:n.value], f[:n.value]
+
+
+def number_p1band(mesh, phi, width, zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    nv = (mesh.n[0] + 1) * (mesh.n[1] + 1) * (mesh.n[2] + 1)
+    dof = np.empty(nv, np.int32)
+    nd = C.c_int64()
+    _check(lib().orc_number_p1band(C.byref(mesh), _ptr(phi, C.c_double), width, zero_policy, _ptr(dof, C.c_int32),
+                                   C.byref(nd)), "number_p1band")
+    return dof, nd.value
+
+
+def assemble_p1band(mesh, phi, dof, ndof, width, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
+                    zero_policy=ZERO_THROW):
+    """dict(stab=(rowptr, colind, val), surf=(rowptr, colind, mass, lb), rhs): the stabilisation on its
+    own (band) pattern, mass and LB on theirs (cut tets)"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    dof = np.ascontiguousarray(dof, np.int32)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    cap = 27 * max(ndof, 1)
+    rps, rpm = np.zeros(ndof + 1, np.int64), np.zeros(ndof + 1, np.int64)
+    cs, cm = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    vs, vm, vl = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+    rhs = np.zeros(max(ndof, 1))
+    ns, nm = C.c_int64(), C.c_int64()
+    _check(lib().orc_assemble_p1band(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof, C.c_int32), ndof, width,
+                                     _ptr(co, C.c_double), rhs_kind, _ptr(rp, C.c_double), zero_policy, cap,
+                                     C.byref(ns), _ptr(rps, C.c_int64), _ptr(cs, C.c_int32), _ptr(vs, C.c_double),
+                                     C.byref(nm), _ptr(rpm, C.c_int64), _ptr(cm, C.c_int32), _ptr(vm, C.c_double),
+                                     _ptr(vl, C.c_double), _ptr(rhs, C.c_double)), "assemble_p1band")
+    return dict(stab=(rps, cs[:ns.value], vs[:ns.value]), surf=(rpm, cm[:nm.value], vm[:nm.value], vl[:nm.value]),
+                rhs=rhs[:ndof])

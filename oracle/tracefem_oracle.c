@@ -548,6 +548,35 @@ static double rhs_fun(int kind, const double *param, const double x[3])
 }
 
 /* All per-tet local quantities of one cut tet with a non-empty patch. */
+/* LocalNormalLaplaceBulkP1CL::setup, surfactant/ifacetransp.h:2005-2051 (the same formula for the
+ * InterfaceCommonDataP1CL and the NarrowBandCommonDataP1CL overloads): D * Quad5(U_i U_j, |det|/6),
+ * U_i = n . grad(phi_i) on the whole tet */
+static void local_stab(const tet_t *T, const double coef[3], double stab[16])
+{
+    double G1[4][3];
+    const double det = p1_gradients(T->X, G1);
+    {
+        double v[10];
+        for (int d = 0; d < 10; ++d) v[d] = T->ls[LAT_OF_P2[d]];
+        double U[4][15];
+        for (int q = 0; q < 15; ++q) {
+            double n[3];
+            p2_normal(v, V5_N[q], G1, n);
+            for (int i = 0; i < 4; ++i) U[i][q] = dot3(n, G1[i]);
+        }
+        const double ad = fabs(det) / 6;
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j <= i; ++j) {
+                double f[15];
+                for (int q = 0; q < 15; ++q) f[q] = U[i][q] * U[j][q];
+                const double val = ad * (((8. / 405. * f[0] + V5_C1 * (((f[1] + f[2]) + f[3]) + f[4]))
+                                          + V5_C2 * (((f[5] + f[6]) + f[7]) + f[8]))
+                                         + 5. / 567. * (((((f[9] + f[10]) + f[11]) + f[12]) + f[13]) + f[14]));
+                stab[i * 4 + j] = stab[j * 4 + i] = coef[2] * val;
+            }
+    }
+}
+
 static void local_quantities(const tet_t *T, const patch_t *p, const double coef[3], int rhs_kind,
                              const double *rhs_param, orc_local_t *o)
 {
@@ -602,28 +631,7 @@ static void local_quantities(const tet_t *T, const patch_t *p, const double coef
                 o->lb[i * 4 + j] = o->lb[j * 4 + i] = fac * s;
             }
     }
-    /* ---- LocalNormalLaplaceBulkP1CL::setup(tet, InterfaceCommonDataP1CL),
-     *      surfactant/ifacetransp.h:2032-2051: D * Quad5(U_i U_j, |det|/6), U_i = n . grad(phi_i) ---- */
-    {
-        double v[10];
-        for (int d = 0; d < 10; ++d) v[d] = T->ls[LAT_OF_P2[d]];
-        double U[4][15];
-        for (int q = 0; q < 15; ++q) {
-            double n[3];
-            p2_normal(v, V5_N[q], G1, n);
-            for (int i = 0; i < 4; ++i) U[i][q] = dot3(n, G1[i]);
-        }
-        const double ad = fabs(det) / 6;
-        for (int i = 0; i < 4; ++i)
-            for (int j = 0; j <= i; ++j) {
-                double f[15];
-                for (int q = 0; q < 15; ++q) f[q] = U[i][q] * U[j][q];
-                const double val = ad * (((8. / 405. * f[0] + V5_C1 * (((f[1] + f[2]) + f[3]) + f[4]))
-                                          + V5_C2 * (((f[5] + f[6]) + f[7]) + f[8]))
-                                         + 5. / 567. * (((((f[9] + f[10]) + f[11]) + f[12]) + f[13]) + f[14]));
-                o->stab[i * 4 + j] = o->stab[j * 4 + i] = coef[2] * val;
-            }
-    }
+    local_stab(T, coef, o->stab);
     /* ---- LocalVectorP1CL::setup, surfactant/ifacetransp.h:785-797 (Quad5 composite) ---- */
     if (rhs_kind != ORC_RHS_NONE) {
         nq = composite_nodes(p, 7, Q5_N, Q5_W, node, w);
@@ -929,6 +937,182 @@ int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
     free(cut);
     free(rec);
     free(rows);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Narrow band (SurfactantNarrowBandStblP1CL::Update, surfactant/ifacetransp.cpp:1110-1172)         */
+/* ------------------------------------------------------------------------ */
+
+/* distance() of the 10 lattice values, geom/subtriangulation.tpp:38-46: min |f| when all share one
+ * strict sign, else 0 (a cut tet; with an exact zero the reference throws) */
+static int tet_distance(const double ls[10], double *d)
+{
+    if (equal_signs(ls, 10)) {
+        double m = fabs(ls[0]);
+        for (int l = 1; l < 10; ++l)
+            if (fabs(ls[l]) < m) m = fabs(ls[l]);
+        *d = m;
+        return ORC_OK;
+    }
+    *d = 0.;
+    return has_zero(ls, 10) ? ORC_E_ZERO_NODE : ORC_OK;
+}
+
+/* Tets with distance() <= width in mesh order (NarrowBandCommonDataP1CL::visit,
+ * surfactant/ifacetransp.h:688-700); is_cut[c] = 1 for the cut ones */
+int orc_classify_band(const orc_mesh_t *m, const double *phi, double width, int zero_policy, int64_t cap,
+                      int64_t *nband, int64_t *band_tets, uint8_t *is_cut)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !nband || width < 0.) return ORC_E_ARG;
+    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    int64_t c = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        tet_t T;
+        load_tet_ls(m, phi, t, &T);
+        double d;
+        const int err = tet_distance(T.ls, &d);
+        if (err && zero_policy == ORC_ZERO_THROW) return err;
+        if (d > width) continue;
+        if (band_tets) {
+            if (c >= cap) return ORC_E_CAPACITY;
+            band_tets[c] = t;
+            if (is_cut) is_cut[c] = equal_signs(T.ls, 10) ? 0 : 1;
+        }
+        ++c;
+    }
+    *nband = c;
+    return ORC_OK;
+}
+
+/* IdxDescCL::CreateNumbering(..., width) -> CreateNumbNearInterfaceVertex, misc/problem.cpp:509-555:
+ * every vertex of every band tet, first encounter in mesh order (no omit_bound test) */
+int orc_number_p1band(const orc_mesh_t *m, const double *phi, double width, int zero_policy,
+                      int32_t *dof_of_vertex, int64_t *ndof)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !ndof || width < 0.) return ORC_E_ARG;
+    const int64_t nv = (int64_t)(m->n[0] + 1) * (m->n[1] + 1) * (m->n[2] + 1);
+    for (int64_t v = 0; v < nv; ++v) dof_of_vertex[v] = -1;
+    const int64_t nt = 6LL * m->n[0] * m->n[1] * m->n[2];
+    int64_t counter = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        tet_t T;
+        load_tet(m, phi, t, &T);
+        double d;
+        const int err = tet_distance(T.ls, &d);
+        if (err && zero_policy == ORC_ZERO_THROW) return err;
+        if (d > width) continue;
+        for (int i = 0; i < 4; ++i)
+            if (dof_of_vertex[T.vid[i]] < 0) dof_of_vertex[T.vid[i]] = (int32_t)counter++;
+    }
+    *ndof = counter;
+    return ORC_OK;
+}
+
+/* Build() of one brow_t array into CSR (first value slot only or all three) */
+static int build_rows(brow_t *rows, int64_t ndof, int64_t cap, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                      double *v0, double *v1, double *v2)
+{
+    rowptr[0] = 0;
+    for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
+    *nnz = rowptr[ndof];
+    if (*nnz > cap) return ORC_E_CAPACITY;
+    if (!colind) return ORC_OK;
+    for (int64_t r = 0; r < ndof; ++r) {
+        brow_t *R = &rows[r];
+        int32_t ord[ROWCAP];
+        for (int k = 0; k < R->n; ++k) ord[k] = k;
+        for (int a = 1; a < R->n; ++a) {
+            int32_t x = ord[a];
+            int b = a - 1;
+            while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
+            ord[b + 1] = x;
+        }
+        for (int k = 0; k < R->n; ++k) {
+            const int64_t dst = rowptr[r] + k;
+            colind[dst] = R->col[ord[k]];
+            if (v0) v0[dst] = R->v[0][ord[k]];
+            if (v1) v1[dst] = R->v[1][ord[k]];
+            if (v2) v2[dst] = R->v[2][ord[k]];
+        }
+    }
+    return ORC_OK;
+}
+
+/* The accumulate() of SurfactantNarrowBandStblP1CL::Update on the narrow-band numbering: mass,
+ * Laplace-Beltrami and load through InterfaceCommonDataP1CL (cut tets with a non-empty patch; their
+ * own builders, so their own pattern), the normal stabilisation through NarrowBandMatrixAccuP1CL
+ * over every band tet (its own pattern, surfactant/ifacetransp.h:746-755).  Each builder sums its
+ * contributions in mesh order. */
+int orc_assemble_p1band(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                        double width, const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                        int64_t cap_nnz, int64_t *nnz_s, int64_t *rowptr_s, int32_t *colind_s, double *val_stab,
+                        int64_t *nnz_m, int64_t *rowptr_m, int32_t *colind_m, double *val_mass, double *val_lb,
+                        double *rhs)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !nnz_s || !rowptr_s || !nnz_m || !rowptr_m || !coef)
+        return ORC_E_ARG;
+    int64_t nb = 0;
+    int err = orc_classify_band(m, phi, width, zero_policy, 0, &nb, NULL, NULL);
+    if (err) return err;
+    int64_t *band = (int64_t *)malloc(sizeof(int64_t) * (nb > 0 ? nb : 1));
+    uint8_t *cutf = (uint8_t *)malloc(nb > 0 ? nb : 1);
+    brow_t *rs = (brow_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow_t));
+    brow_t *rm = (brow_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow_t));
+    if (!band || !cutf || !rs || !rm) {
+        free(band); free(cutf); free(rs); free(rm);
+        return ORC_E_NOMEM;
+    }
+    orc_classify_band(m, phi, width, zero_policy, nb, &nb, band, cutf);
+    if (rhs)
+        for (int64_t i = 0; i < ndof; ++i) rhs[i] = 0.;
+    for (int64_t c = 0; c < nb && !err; ++c) {
+        tet_t T;
+        load_tet(m, phi, band[c], &T);
+        int32_t numb[4];
+        for (int i = 0; i < 4; ++i) numb[i] = dof_of_vertex[T.vid[i]];
+        orc_local_t L;
+        memset(&L, 0, sizeof L);
+        int nonempty = 0;
+        if (cutf[c]) {
+            patch_t p;
+            err = make_patch(T.ls, &p);
+            if (err) break;
+            if (p.nf > 0) {
+                patch_geometry(T.X, &p);
+                local_quantities(&T, &p, coef, rhs_kind, rhs_param, &L);
+                nonempty = 1;
+            }
+        }
+        double stab[16];
+        local_stab(&T, coef, stab);
+        for (int i = 0; i < 4 && !err; ++i) {
+            if (numb[i] < 0) continue;
+            for (int j = 0; j < 4 && !err; ++j) {
+                if (numb[j] < 0) continue;
+                if (nonempty) {
+                    const double v3[3] = {L.mass[i * 4 + j], L.lb[i * 4 + j], 0.};
+                    err = builder_add(&rm[numb[i]], numb[j], v3);
+                }
+                if (!err) {
+                    const double s3[3] = {stab[i * 4 + j], 0., 0.};
+                    err = builder_add(&rs[numb[i]], numb[j], s3);
+                }
+            }
+        }
+        if (rhs && nonempty)
+            for (int i = 0; i < 4; ++i)
+                if (numb[i] >= 0) rhs[numb[i]] += L.rhs[i];
+    }
+    if (!err) err = build_rows(rs, ndof, cap_nnz, nnz_s, rowptr_s, colind_s, val_stab, NULL, NULL);
+    if (!err) err = build_rows(rm, ndof, cap_nnz, nnz_m, rowptr_m, colind_m, val_mass, val_lb, NULL);
+    free(band);
+    free(cutf);
+    free(rs);
+    free(rm);
     return err;
 }
 

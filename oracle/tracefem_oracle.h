@@ -117,6 +117,21 @@ int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
                     int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
                     double *val_mass, double *val_lb, double *val_stab, double *rhs);
 
+/* ---- narrow band (SurfactantNarrowBandStblP1CL::Update, surfactant/ifacetransp.cpp:1110-1172):
+ *      band = tets with distance() <= width (cut tets have distance 0); numbering of every vertex of
+ *      every band tet (CreateNumbNearInterfaceVertex, misc/problem.cpp:509-555); mass / LB / load over
+ *      the cut tets on their own pattern (_m), the normal stabilisation over the band on its own
+ *      pattern (_s) ---- */
+int orc_classify_band(const orc_mesh_t *m, const double *phi, double width, int zero_policy, int64_t cap,
+                      int64_t *nband, int64_t *band_tets, uint8_t *is_cut);
+int orc_number_p1band(const orc_mesh_t *m, const double *phi, double width, int zero_policy,
+                      int32_t *dof_of_vertex, int64_t *ndof);
+int orc_assemble_p1band(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                        double width, const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                        int64_t cap_nnz, int64_t *nnz_s, int64_t *rowptr_s, int32_t *colind_s, double *val_stab,
+                        int64_t *nnz_m, int64_t *rowptr_m, int32_t *colind_m, double *val_mass, double *val_lb,
+                        double *rhs);
+
 /* surface integral of 1 over the patch (tests/principallattice.cpp:296-330) */
 int orc_surface_area(const orc_mesh_t *m, const double *phi, int zero_policy, double *area);
 
