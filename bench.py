@@ -305,6 +305,40 @@ def formats_leg(ctx, g, with_cpu=True, reps=3):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def band_leg(ctx, mesh, phi, coef, lc, width, reps=5, with_cpu=True):
+    """Narrow-band assembly (SurfactantNarrowBandStblP1CL::Update's tuple) of the same level set,
+    band width `width`: best wall time of reps, per-kernel HIP-event averages, and the oracle's
+    serial restatement timed once beside it."""
+    ts = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        nb, nd, nnz = ctx.assemble_p1_band(width, coef=coef, rhs_kind=D.RHS_3XY_MINUS_X3, lc=lc)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ctx.profile_reset()
+    ctx.profile(True)
+    ctx.assemble_p1_band(width, coef=coef, rhs_kind=D.RHS_3XY_MINUS_X3, lc=lc)
+    kern = {}
+    for k in ("classify", "compact", "local_p1", "qualify", "first_owner", "dofs", "rowcount", "rowfill"):
+        ms, cnt = ctx.profile_get(k)
+        if cnt:
+            kern[k] = round(1e3 * ms / cnt, 2)
+    ctx.profile(False)
+    out = {"width": width, "band_tets": nb, "dofs": nd, "nnz": nnz, "s_per_assembly": min(ts), "kernels_us": kern}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+        t0 = time.perf_counter()
+        dof, ndc = O.number_p1band(om, phi, width)
+        O.assemble_p1band(om, phi, dof, ndc, width, coef=coef, rhs_kind=O.RHS_3XY_MINUS_X3)
+        out["cpu_port_s"] = time.perf_counter() - t0
+        out["cpu_threads"] = 1
+        out["dofs_match"] = bool(ndc == nd)
+    return out
+
+
 def measured_traffic(kernel, rows, ndof):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
     each scaled by a known-size copy), when it was measured at this block geometry and system size."""
@@ -492,6 +526,8 @@ def main():
         err["cpu"] = float(np.linalg.norm(xc - xs) / np.linalg.norm(xs))
     fm = formats_leg(ctx, g, with_cpu=not args.no_cpu)
     log(f"[bench] formats {fm}")
+    bd = band_leg(ctx, mesh, phi, coef, lc, 2.0 * mesh.h, with_cpu=not args.no_cpu)
+    log(f"[bench] band {bd}")
     ctx.close()
 
     ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu, space=args.ch_space)
@@ -527,6 +563,7 @@ def main():
         "learned": ch,
         "batch": bt,
         "formats": fm,
+        "band": bd,
     }
     if cpu is not None:
         line["cpu_baseline"] = {"value": cpu["value"], "unit": "s per solve", "cores": cpu["threads"],

@@ -600,9 +600,11 @@ class IdxDescCL {
     size_t NumUnknowns() const { return num_; }
     const std::vector<int64_t> &vertex_of_dof() const { return vertex_of_dof_; }
     const std::vector<int64_t> &node_of_dof() const { return node_of_dof_; }
-    // CreateNumbering (misc/problem.cpp:725-745) -> CreateNumbOnInterfaceVertex (:443-503) for P1IF_FE,
-    // CreateNumbOnInterfaceP2 (:571-636) for P2IF_FE, on the GPU
-    size_t CreateNumbering(unsigned, const MultiGridCL &mg, const VecDescCL *lsetp, const BndDataCL * = 0) {
+    // CreateNumbering (misc/problem.cpp:725-745, 811-824) -> CreateNumbOnInterfaceVertex (:443-503) for
+    // P1IF_FE, CreateNumbOnInterfaceP2 (:571-636) for P2IF_FE, CreateNumbNearInterfaceVertex (:509-555)
+    // for width > 0 (P1IF_FE), on the GPU
+    size_t CreateNumbering(unsigned, const MultiGridCL &mg, const VecDescCL *lsetp, const BndDataCL * = 0,
+                           double width = 0.) {
         if (!lsetp) throw DROPSErrCL("IdxDescCL::CreateNumbering: no level set function for interface numbering given");
         DeviceCL &dev = DeviceCL::instance();
         const auto &B = mg.brick();
@@ -613,11 +615,18 @@ class IdxDescCL {
         dev.check(dhip_levelset_p2(dev.ctx(), lsetp->Data.raw(), (int64_t)lsetp->Data.size(), DHIP_HOST));
         int64_t ncut = 0, nd = 0;
         const double coef[3] = {1., 1., 1.};
-        dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &ncut));
-        if (fe_ == P2IF_FE) {
+        if (width > 0.) {
+            if (fe_ != P1IF_FE) throw DROPSErrCL("IdxDescCL::CreateNumbering: the narrow band is a P1 numbering");
+            int64_t nnz = 0;
+            const double lc[3] = {1., 1., 1.};
+            dev.check(dhip_assemble_p1_band(dev.ctx(), DHIP_ZERO_THROW, coef, DHIP_RHS_NONE, nullptr, width, lc, &ncut,
+                                            &nd, &nnz));
+        } else if (fe_ == P2IF_FE) {
+            dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &ncut));
             dev.check(dhip_local_p2(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
             dev.check(dhip_number_p2if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
         } else {
+            dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &ncut));
             dev.check(dhip_local_p1(dev.ctx(), coef, DHIP_RHS_NONE, nullptr));
             dev.check(dhip_number_p1if(dev.ctx(), omit_bound_ < 0 ? -1.0 : omit_bound_, &nd));
         }
@@ -732,6 +741,28 @@ struct LocalNormalGradientStabP2CL {
     explicit LocalNormalGradientStabP2CL(double D) : coef(D) {}
 };
 
+// Narrow band (surfactant/ifacetransp.h:647-760): the tets with distance() <= dist; the matrix
+// accumulator over them carries the normal stabilisation (LocalNormalLaplaceBulkP1CL's narrow-band
+// setup, surfactant/ifacetransp.h:2005-2027).
+class NarrowBandCommonDataP1CL : public TetraAccumulatorCL {
+  public:
+    const VecDescCL *ls;
+    double dist;
+    NarrowBandCommonDataP1CL(const VecDescCL &ls_arg, const BndDataCL &, const double &dst)
+        : ls(&ls_arg), dist(dst) {}
+};
+template <class LocalMatrixT>
+class NarrowBandMatrixAccuP1CL : public TetraAccumulatorCL {
+  public:
+    MatDescCL *mat;
+    LocalMatrixT local;
+    const NarrowBandCommonDataP1CL *bdata;
+    std::string name;
+    NarrowBandMatrixAccuP1CL(MatDescCL *m, const LocalMatrixT &l, const NarrowBandCommonDataP1CL &b,
+                             std::string n = std::string())
+        : mat(m), local(l), bdata(&b), name(n) {}
+};
+
 // load vector of a device-evaluable right-hand side (surfactant/ifacetransp.h:768-798)
 struct DeviceFunctionCL {
     int kind = DHIP_RHS_NONE;
@@ -788,6 +819,16 @@ class TetraAccumulatorTupleCL {
         rhs = a->vec;
         rhs_fn = a->local.f;
     }
+    const NarrowBandCommonDataP1CL *band = nullptr;
+    void push_back(NarrowBandCommonDataP1CL *b) { band = b; }
+    template <class L>
+    void push_back(NarrowBandMatrixAccuP1CL<L> *a) {
+        static_assert(L::device_op == DHIP_OP_STAB, "the narrow band carries the normal stabilisation");
+        for (auto &m : mats)
+            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P1 interface operator");
+        if (a->bdata != band) throw DROPSErrCL("accumulate: push the band's common data first");
+        mats.push_back({a->mat, L::device_op, a->local.coef});
+    }
     void push_back(InterfaceCommonDataP2CL *c) { common2 = c; }
     template <class L>
     void push_back(InterfaceMatrixAccuCL<L, InterfaceCommonDataP2CL> *a) {
@@ -821,9 +862,16 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
     int64_t ncut = 0, nd = 0, nnz = 0;
     const double param = t.rhs_fn.param;
     const double lc[3] = {1., 1., 1.};
+    if (t.band && p2) throw DROPSErrCL("accumulate: the narrow band is a P1 assembly");
     if (p2)
         dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
                                    -1.0, lc, &ncut, &nd, &nnz));
+    else if (t.band)
+        // SurfactantNarrowBandStblP1CL::Update's tuple: the surface forms on the cut tets, the
+        // stabilisation on the band; every matrix comes back on the band pattern (mass and LB hold
+        // 0.0 where only band tets couple two DOFs)
+        dev.check(dhip_assemble_p1_band(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE,
+                                        &param, t.band->dist, lc, &ncut, &nd, &nnz));
     else
         dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
                                    -1.0, lc, &ncut, &nd, &nnz));

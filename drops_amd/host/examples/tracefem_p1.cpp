@@ -5,6 +5,7 @@
 //   tracefem_p1 sphere20                    -> the 1156-DOF known answer (TestinterfaceP1FE.ref)
 //   tracefem_p1 c1 <n> <outdir>             -> configuration-1 problem, writes CSR/rhs/solution, and the
 //                                              operator / load vector in the reference's file formats
+//   tracefem_p1 band <n> <width> <outdir>   -> the narrow-band tuple of SurfactantNarrowBandStblP1CL::Update
 //   tracefem_p1 p2 <n> <outdir>             -> the same sphere on the P2 interface space (IdxDescCL(P2IF_FE),
 //                                              P2 accumulators), writes CSR/rhs/solution
 #include <cstdio>
@@ -120,6 +121,57 @@ static int c1(int n, const std::string &out) {
     return 0;
 }
 
+// SurfactantNarrowBandStblP1CL::Update's tuple (surfactant/ifacetransp.cpp:1128-1166): interface
+// mass / LB / load with InterfaceCommonDataP1CL, the stabilisation with the narrow band, on the
+// near-interface numbering
+static int band(int n, double width, const std::string &out) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL lsbnd;
+    IdxDescCL idx(P1IF_FE);
+    idx.CreateNumbering(0, mg, &lset.Phi, &lsbnd, width);
+    std::printf("NumUnknowns: %zu\n", idx.NumUnknowns());
+    MatDescCL M(&idx, &idx), A(&idx, &idx), C(&idx, &idx);
+    VecDescCL b(&idx);
+    TetraAccumulatorTupleCL accus;
+    InterfaceCommonDataP1CL cdata(lset.Phi, lsbnd);
+    accus.push_back(&cdata);
+    InterfaceMatrixAccuCL<LocalInterfaceMassP1CL, InterfaceCommonDataP1CL> mass(&M, LocalInterfaceMassP1CL(), cdata, "mass");
+    accus.push_back(&mass);
+    InterfaceMatrixAccuCL<LocalLaplaceBeltramiP1CL, InterfaceCommonDataP1CL> lb(&A, LocalLaplaceBeltramiP1CL(1.), cdata, "LB");
+    accus.push_back(&lb);
+    NarrowBandCommonDataP1CL bdata(lset.Phi, lsbnd, width);
+    accus.push_back(&bdata);
+    NarrowBandMatrixAccuP1CL<LocalNormalLaplaceBulkP1CL> sb(&C, LocalNormalLaplaceBulkP1CL(1.), bdata, "NormalLaplaceBulk");
+    accus.push_back(&sb);
+    DeviceFunctionCL f;
+    f.kind = DHIP_RHS_3XY_MINUS_X3;
+    InterfaceVectorAccuCL<LocalVectorP1CL, InterfaceCommonDataP1CL> rhs(&b, LocalVectorP1CL(f), cdata, "load");
+    accus.push_back(&rhs);
+    accumulate(accus, mg, 0, &lsbnd);
+    std::printf("nnz: %zu\n", C.Data.num_nonzeros());
+    MatrixCL L;
+    L.LinComb(1.0, M.Data, 1.0, A.Data, h, C.Data);
+    if (!out.empty()) {
+        std::vector<int64_t> rp(L.num_rows() + 1), ci(L.num_nonzeros());
+        for (size_t i = 0; i <= L.num_rows(); ++i) rp[i] = (int64_t)L.raw_row()[i];
+        for (size_t k = 0; k < L.num_nonzeros(); ++k) ci[k] = (int64_t)L.raw_col()[k];
+        write_bin(out + "/rowptr.bin", rp.data(), rp.size() * 8);
+        write_bin(out + "/colind.bin", ci.data(), ci.size() * 8);
+        write_bin(out + "/mass.bin", M.Data.raw_val(), M.Data.num_nonzeros() * 8);
+        write_bin(out + "/lb.bin", A.Data.raw_val(), A.Data.num_nonzeros() * 8);
+        write_bin(out + "/stab.bin", C.Data.raw_val(), C.Data.num_nonzeros() * 8);
+        write_bin(out + "/comb.bin", L.raw_val(), L.num_nonzeros() * 8);
+        write_bin(out + "/rhs.bin", b.Data.raw(), b.Data.size() * 8);
+        write_bin(out + "/vertex_of_dof.bin", idx.vertex_of_dof().data(), idx.vertex_of_dof().size() * 8);
+    }
+    return 0;
+}
+
 static int p2(int n, const std::string &out) {
     const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
     const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
@@ -176,7 +228,10 @@ int main(int argc, char **argv) {
         if (argc >= 2 && !std::strcmp(argv[1], "sphere20")) return sphere20();
         if (argc >= 3 && !std::strcmp(argv[1], "c1")) return c1(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
         if (argc >= 3 && !std::strcmp(argv[1], "p2")) return p2(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
-        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir] | p2 <n> [outdir]\n", argv[0]);
+        if (argc >= 4 && !std::strcmp(argv[1], "band"))
+            return band(std::atoi(argv[2]), std::atof(argv[3]), argc >= 5 ? argv[4] : "");
+        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir] | p2 <n> [outdir] | band <n> <width> [outdir]\n",
+                     argv[0]);
         return 2;
     } catch (const std::exception &e) {
         std::fprintf(stderr, "error: %s\n", e.what());

@@ -81,3 +81,33 @@ def test_p2_through_host_mirror(tmp_path):
     nod = ld("node_of_dof", np.int64)
     assert np.array_equal(dof[nod], np.arange(nd))
     assert float(re.search(r"PCG residual: (\S+)", out).group(1)) <= 1e-10
+
+
+def test_band_through_host_mirror(tmp_path):
+    """IdxDescCL::CreateNumbering(..., width) + NarrowBandCommonDataP1CL + NarrowBandMatrixAccuP1CL
+    through accumulate() (SurfactantNarrowBandStblP1CL::Update's tuple): numbering, stabilisation,
+    surface forms, load and the LinComb bit-exact against the oracle's narrow band."""
+    n, width = 16, 0.3
+    out = run("band", str(n), str(width), str(tmp_path))
+    ld = lambda name, dt: np.fromfile(tmp_path / f"{name}.bin", dtype=dt)
+    mesh, phi = cases.c1_sphere(n)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    dof, nd = O.number_p1band(om, phi, width)
+    assert re.search(rf"NumUnknowns: {nd}\b", out)
+    assert np.array_equal(dof[ld("vertex_of_dof", np.int64)], np.arange(nd))
+    ref = O.assemble_p1band(om, phi, dof, nd, width, rhs_kind=O.RHS_3XY_MINUS_X3)
+    rps, cs, vs = ref["stab"]
+    assert np.array_equal(ld("rowptr", np.int64), rps) and np.array_equal(ld("colind", np.int64), cs.astype(np.int64))
+    assert np.array_equal(ld("stab", np.float64), vs)
+    assert np.array_equal(ld("rhs", np.float64), ref["rhs"])
+    # surface forms: the oracle's own entries, 0.0 elsewhere on the band pattern
+    rpm, cm, vm, vl = ref["surf"]
+    m_on, l_on, hit = np.zeros(cs.size), np.zeros(cs.size), np.zeros(cs.size, bool)
+    for r in range(nd):
+        lo, hi = rps[r], rps[r + 1]
+        for q in range(rpm[r], rpm[r + 1]):
+            k = lo + np.searchsorted(cs[lo:hi], cm[q])
+            m_on[k], l_on[k], hit[k] = vm[q], vl[q], True
+    assert np.array_equal(ld("mass", np.float64), m_on) and np.array_equal(ld("lb", np.float64), l_on)
+    comb = np.where(hit, 1.0 * (m_on + l_on) + mesh.h * vs, mesh.h * vs)
+    assert np.array_equal(ld("comb", np.float64), comb)
