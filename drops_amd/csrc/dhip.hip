@@ -1300,6 +1300,11 @@ int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
         c->batch_threads = (int)value;
         return DHIP_OK;
     }
+    if (!std::strcmp(key, "batch_prepass")) {
+        if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "batch_prepass must be 0 or 1");
+        c->batch_prepass = (int)value;
+        return DHIP_OK;
+    }
     if (!std::strcmp(key, "batch_blocks_per_cu")) {
         if (value < 1 || value > 8) return set_error(c, DHIP_E_ARG, "batch_blocks_per_cu must be 1..8");
         c->batch_blocks_per_cu = (int)value;
@@ -2175,10 +2180,13 @@ struct dhip_batch {
     DevBuf prob, order, queue, phi_own;
     DevBuf cellinfo, cut, flags, loc, first, slot, dofs, rowowner, rowlen, rowptr, col;
     DevBuf vm, vl, vs, vc, rhs, diag, x, r64, d32, r32, z32, p32, q32, scol, sval, soff, stat;
+    DevBuf pre_mask, pre_blk, pre_flag;   // pre-pass: cell masks, block counts, zero / error flags
     BatchArgs args{};
+    PrepassArgs pre{};
+    int32_t max_cells = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int g_threads = 0, g_grid = 0, g_max_iter = 0, g_rel = 0;
+    int g_threads = 0, g_grid = 0, g_max_iter = 0, g_rel = 0, g_prepass = -1;
     int64_t g_lds = 0;
     double g_tol = 0.0;
 };
@@ -2191,7 +2199,8 @@ static void batch_release(dhip_batch *b) {
     DevBuf *bufs[] = {&b->prob, &b->order, &b->queue, &b->phi_own, &b->cellinfo, &b->cut, &b->flags, &b->loc,
                       &b->first, &b->slot, &b->dofs, &b->rowowner, &b->rowlen, &b->rowptr, &b->col, &b->vm,
                       &b->vl, &b->vs, &b->vc, &b->rhs, &b->diag, &b->x, &b->r64, &b->d32, &b->r32,
-                      &b->z32, &b->p32, &b->q32, &b->scol, &b->sval, &b->soff, &b->stat};
+                      &b->z32, &b->p32, &b->q32, &b->scol, &b->sval, &b->soff, &b->stat, &b->pre_mask,
+                      &b->pre_blk, &b->pre_flag};
     for (DevBuf *d : bufs) d->release();
 }
 
@@ -2315,7 +2324,9 @@ int dhip_batch_create(dhip_ctx_t *c, int32_t nprob, const dhip_batch_problem_t *
               !b->r64.ensure(sizeof(double) * (row_tot + 1)) && !b->d32.ensure(sizeof(float) * (row_tot + 1)) &&
               !b->r32.ensure(sizeof(float) * (row_tot + 1)) && !b->z32.ensure(sizeof(float) * (row_tot + 1)) &&
               !b->p32.ensure(sizeof(float) * (row_tot + 1)) && !b->q32.ensure(sizeof(float) * (row_tot + 1)) &&
-              !b->stat.ensure(sizeof(BatchStat) * nprob);
+              !b->stat.ensure(sizeof(BatchStat) * nprob) && !b->pre_mask.ensure(cells + 8) &&
+              !b->pre_blk.ensure(sizeof(int32_t) * ((int64_t)nprob * blocks_for(max_cells, BPLAN) + 1)) &&
+              !b->pre_flag.ensure(sizeof(int32_t) * (2 * (int64_t)nprob + 1));
     if (!ok) return fail(set_error(c, DHIP_E_NOMEM, "batch buffers (%lld cut tets, %lld rows, %lld entries)",
                                    (long long)cut_tot, (long long)row_tot, (long long)nz_tot));
     if (hipMemcpyAsync(b->prob.p, hp.data(), sizeof(BatchProb) * nprob, hipMemcpyHostToDevice, c->stream) ||
@@ -2365,6 +2376,28 @@ int dhip_batch_create(dhip_ctx_t *c, int32_t nprob, const dhip_batch_problem_t *
     a.zero_throw = zero_policy == DHIP_ZERO_THROW;
     a.inner_tol = 1e-4;
     a.inner_max = 400;
+    a.pre_zero = b->pre_flag.as<int32_t>();
+    a.pre_err = a.pre_zero + nprob;
+    PrepassArgs &q = b->pre;
+    q.prob = a.prob;
+    q.nprob = nprob;
+    q.phi = dphi;
+    q.cellmask = b->pre_mask.as<uint8_t>();
+    q.blkcnt = b->pre_blk.as<int32_t>();
+    q.max_blocks = (int32_t)blocks_for(max_cells, BPLAN);
+    q.zero = b->pre_flag.as<int32_t>();
+    q.perr = q.zero + nprob;
+    q.cellinfo = a.cellinfo;
+    q.cut = a.cut;
+    q.cut_total = cut_tot;
+    q.flags = a.flags;
+    q.loc = a.loc;
+    q.coef0 = a.coef0;
+    q.coef1 = a.coef1;
+    q.coef2 = a.coef2;
+    q.rhs_kind = a.rhs_kind;
+    q.rhs_p0 = a.rhs_p0;
+    b->max_cells = (int32_t)max_cells;
     *out = b;
     return DHIP_OK;
 }
@@ -2399,10 +2432,24 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
     b->args.max_iter = max_iter;
     b->args.tol = tol;
     b->args.rel = rel;
+    b->args.prepass = c->batch_prepass;
     BatchArgs a = b->args;
     void *argv[] = {&a};
+    const PrepassArgs q = b->pre;
     auto enqueue = [&]() -> hipError_t {
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, b->queue.as<uint32_t>(), 16);
+        if (a.prepass) {
+            const int nflag = 2 * b->nprob;
+            hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(nflag, 256)), dim3(256), 0, c->stream,
+                               b->pre_flag.as<uint32_t>(), nflag);
+            const dim3 g2(q.max_blocks, b->nprob);
+            hipLaunchKernelGGL(k_batch_cut_count, g2, dim3(BPLAN), 0, c->stream, q);
+            hipLaunchKernelGGL(k_batch_cut_scan, dim3(b->nprob), dim3(BPLAN), 0, c->stream, q);
+            hipLaunchKernelGGL(k_batch_cut_fill, g2, dim3(BPLAN), 0, c->stream, q);
+            if (q.cut_total > 0)
+                hipLaunchKernelGGL(k_batch_local, dim3(blocks_for(q.cut_total, LOCAL_BLOCK)), dim3(LOCAL_BLOCK), 0,
+                                   c->stream, q);
+        }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         return hipLaunchKernel(k, dim3(grid), dim3(threads), argv, (size_t)lds, c->stream);
@@ -2412,7 +2459,7 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
         DHIP_HIPCHK(c, enqueue());
     } else {
         const bool same = b->exec && b->g_threads == threads && b->g_grid == grid && b->g_max_iter == max_iter &&
-                          b->g_tol == tol && b->g_rel == rel && b->g_lds == lds;
+                          b->g_tol == tol && b->g_rel == rel && b->g_lds == lds && b->g_prepass == a.prepass;
         if (!same) {
             if (b->exec) (void)hipGraphExecDestroy(b->exec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -2432,6 +2479,7 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
             b->g_tol = tol;
             b->g_rel = rel;
             b->g_lds = lds;
+            b->g_prepass = a.prepass;
         }
         ProfScope ps(c, "batch_graph");
         DHIP_HIPCHK(c, hipGraphLaunch(b->exec, c->stream));

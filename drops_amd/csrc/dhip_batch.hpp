@@ -88,6 +88,113 @@ __global__ void __launch_bounds__(BPLAN) k_batch_verts(const BatchProb *prob, co
     if (threadIdx.x == 0 && cnt > 0.0) atomicAdd(&nvert[blockIdx.y], (int32_t)cnt);
 }
 
+// exclusive scan of a[0..n) in place by the whole workgroup; returns the total
+template <int BB>
+__device__ inline int64_t wg_scan_inplace(int32_t *a, int64_t n) {
+    int64_t carry = 0;
+    for (int64_t i0 = 0; i0 < n; i0 += BB) {
+        const int64_t i = i0 + threadIdx.x;
+        const int v = i < n ? a[i] : 0;
+        int tot;
+        const int off = block_excl_scan<BB>(v, &tot);
+        if (i < n) a[i] = (int32_t)(carry + off);
+        carry += tot;
+    }
+    __syncthreads();
+    return carry;
+}
+
+// ---------------------------------------------------------------------------
+// pre-pass (tuning key batch_prepass): classification, compaction and local matrices of every
+// problem over the whole grid, so that a large problem's most expensive per-tet work does not
+// lengthen its workgroup's span.  The same per-item functions produce the same arrays as the
+// in-kernel path.  On the 512-problem set it measured slower (77.1 against 73.9 ms per batch: the
+// largest problem's assembly left its span, but every problem then entered its solve at once and
+// the solves slowed more), so the autotuner decides per problem set.
+//   k_batch_cut_count   per (problem, block of cells): cell masks, the block's cut-tet count,
+//                       the problem's exact-zero flag
+//   k_batch_cut_scan    per problem: exclusive scan of its block counts
+//   k_batch_cut_fill    per (problem, block of cells): cut list in cell order and cell words
+//   k_batch_local       one thread per cut tet of any problem: local_p1_one
+// ---------------------------------------------------------------------------
+struct PrepassArgs {
+    const BatchProb *prob;
+    int32_t nprob;
+    const double *phi;
+    uint8_t *cellmask;          // per problem cell (cell_off layout)
+    int32_t *blkcnt;            // [nprob][max_blocks]
+    int32_t max_blocks;
+    int32_t *zero;              // per problem: exact-zero node in a cut tet
+    int32_t *perr;              // per problem: local error bits (2: all-zero sub-tet)
+    uint32_t *cellinfo;
+    int64_t *cut;
+    int64_t cut_total;
+    uint8_t *flags;
+    double *loc;
+    double coef0, coef1, coef2;
+    int rhs_kind;
+    double rhs_p0;
+};
+
+__global__ void __launch_bounds__(BPLAN) k_batch_cut_count(PrepassArgs a) {
+    const int p = blockIdx.y;
+    const BatchProb &P = a.prob[p];
+    const int64_t cell = (int64_t)blockIdx.x * BPLAN + threadIdx.x;
+    int mk = 0;
+    bool z = false;
+    if (cell < P.m.ncell) {
+        mk = cell_cut_mask(P.m, a.phi + P.phi_off, cell, z);
+        a.cellmask[P.cell_off + cell] = (uint8_t)mk;
+    }
+    int tot;
+    block_excl_scan<BPLAN>(__popc(mk), &tot);
+    const double zs = block_sum<BPLAN>(z ? 1.0 : 0.0);
+    if (threadIdx.x == 0) {
+        a.blkcnt[(int64_t)p * a.max_blocks + blockIdx.x] = tot;
+        if (zs > 0.0) atomicOr(&a.zero[p], 1);
+    }
+}
+
+__global__ void __launch_bounds__(BPLAN) k_batch_cut_scan(PrepassArgs a) {
+    const int p = blockIdx.x;
+    const int64_t nb = (a.prob[p].m.ncell + BPLAN - 1) / BPLAN;
+    wg_scan_inplace<BPLAN>(a.blkcnt + (int64_t)p * a.max_blocks, nb);
+}
+
+__global__ void __launch_bounds__(BPLAN) k_batch_cut_fill(PrepassArgs a) {
+    const int p = blockIdx.y;
+    const BatchProb &P = a.prob[p];
+    const int64_t cell = (int64_t)blockIdx.x * BPLAN + threadIdx.x;
+    const int mk = cell < P.m.ncell ? a.cellmask[P.cell_off + cell] : 0;
+    const int off = block_excl_scan<BPLAN>(__popc(mk), nullptr);
+    if (cell >= P.m.ncell) return;
+    int64_t b = (int64_t)a.blkcnt[(int64_t)p * a.max_blocks + blockIdx.x] + off;
+    a.cellinfo[P.cell_off + cell] = ((uint32_t)b << 6) | (uint32_t)mk;
+    for (int k = 0; k < 6; ++k)
+        if (mk & (1 << k)) {
+            if (b < P.ncut_cap) a.cut[P.cut_off + b] = cell * 6 + k;
+            ++b;
+        }
+}
+
+__global__ void __launch_bounds__(LOCAL_BLOCK) k_batch_local(PrepassArgs a) {
+    const int64_t g = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+    if (g >= a.cut_total) return;
+    // the problem whose cut slots hold g: the last one with cut_off <= g
+    int lo = 0, hi = a.nprob - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.prob[mid].cut_off <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchProb &P = a.prob[lo];
+    const int64_t c = g - P.cut_off;
+    if (c < 0 || c >= P.ncut_cap) return;
+    LocalArgs la{P.m, a.phi + P.phi_off, a.cut + P.cut_off, P.ncut_cap, a.coef0, a.coef1, a.coef2, a.rhs_kind,
+                 a.rhs_p0, a.loc + (int64_t)LOC_N * P.cut_off, a.flags + P.cut_off, &a.perr[lo]};
+    local_p1_one(la, c);
+}
+
 // ---------------------------------------------------------------------------
 // the persistent solve
 // ---------------------------------------------------------------------------
@@ -121,6 +228,8 @@ struct BatchArgs {
     int rel;
     double inner_tol;
     int inner_max;
+    int prepass = 0;            // 1: cut lists, cell words, flags and local data come from the pre-pass
+    const int32_t *pre_zero = nullptr, *pre_err = nullptr;
 };
 
 // two workgroup sums with one LDS round
@@ -144,22 +253,6 @@ __device__ inline void block_sum2(double &u, double &v) {
     __syncthreads();
     u = su;
     v = sv;
-}
-
-// exclusive scan of a[0..n) in place by the whole workgroup; returns the total
-template <int BB>
-__device__ inline int64_t wg_scan_inplace(int32_t *a, int64_t n) {
-    int64_t carry = 0;
-    for (int64_t i0 = 0; i0 < n; i0 += BB) {
-        const int64_t i = i0 + threadIdx.x;
-        const int v = i < n ? a[i] : 0;
-        int tot;
-        const int off = block_excl_scan<BB>(v, &tot);
-        if (i < n) a[i] = (int32_t)(carry + off);
-        carry += tot;
-    }
-    __syncthreads();
-    return carry;
 }
 
 // Row loops of the inner iteration in groups: a thread's rows r = tid + (g + u BB) are handled G at a
@@ -240,7 +333,12 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
     __syncthreads();
     int64_t ncut = 0;
     bool over = false;
-    for (int64_t c0 = 0; c0 < m.ncell; c0 += BB) {
+    if (a.prepass) {
+        ncut = P.ncut_cap;   // the planned count is exact: same level set, same classification
+        if (a.pre_zero[p]) sh_zero = 1;
+        if (a.pre_err[p] & 2) sh_err = 2;
+    }
+    for (int64_t c0 = 0; c0 < (a.prepass ? 0 : m.ncell); c0 += BB) {
         const int64_t cell = c0 + tid;
         int mk = 0;
         if (cell < m.ncell) {
@@ -270,8 +368,10 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
     int64_t nd = 0, nnz = 0;
     if (st.status == DHIP_OK) {
         // ---- local matrices ----
-        LocalArgs la{m, phi, cut, ncut, a.coef0, a.coef1, a.coef2, a.rhs_kind, a.rhs_p0, loc, flags, &sh_err};
-        for (int64_t c = tid; c < ncut; c += BB) local_p1_one(la, c);
+        if (!a.prepass) {
+            LocalArgs la{m, phi, cut, ncut, a.coef0, a.coef1, a.coef2, a.rhs_kind, a.rhs_p0, loc, flags, &sh_err};
+            for (int64_t c = tid; c < ncut; c += BB) local_p1_one(la, c);
+        }
         __syncthreads();
         if (sh_err & 2) st.status = DHIP_E_ZERO_PATTERN;
     }

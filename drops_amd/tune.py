@@ -33,24 +33,27 @@ def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, cand
     return choice, timings
 
 
-BATCH_CANDIDATES = ((512, 1), (256, 1), (256, 2))
+# (threads, workgroups per CU, grid-wide pre-pass of classification and local matrices)
+BATCH_CANDIDATES = ((512, 1, 0), (512, 1, 1), (256, 1, 0), (256, 2, 0))
 
 
 def tune_batch(ctx, batch, tol=1e-8, candidates=BATCH_CANDIDATES, reps=1):
-    """Workgroup size and residency of the batched solve: times one launch per candidate on the
-    caller's problem set and leaves the fastest on the context.  Returns ((threads, blocks_per_cu),
-    {candidate: seconds})."""
+    """Workgroup size, residency and pre-pass of the batched solve: times one launch per candidate
+    on the caller's problem set and leaves the fastest on the context.  Returns
+    ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
     timings = {}
-    for threads, bpc in candidates:
+    for threads, bpc, pre in candidates:
         ctx.tune("batch_threads", threads)
         ctx.tune("batch_blocks_per_cu", bpc)
+        ctx.tune("batch_prepass", pre)
         best = np.inf
         for _ in range(reps):
             t0 = time.perf_counter()
             batch.solve(tol=tol)
             best = min(best, time.perf_counter() - t0)
-        timings[(threads, bpc)] = best
+        timings[(threads, bpc, pre)] = best
     choice = min(timings, key=timings.get)
     ctx.tune("batch_threads", choice[0])
     ctx.tune("batch_blocks_per_cu", choice[1])
+    ctx.tune("batch_prepass", choice[2])
     return choice, timings

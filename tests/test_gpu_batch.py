@@ -95,6 +95,20 @@ def test_batch_graph_replay_and_tiles(ctx):
                 assert np.linalg.norm(g["x"] - ref[p]["x"]) <= 1e-6 * np.linalg.norm(ref[p]["x"])
         finally:
             ctx.tune("batch_threads", 256)
+    # classification and local matrices by a grid-wide pre-pass instead of inside each problem's
+    # workgroup: the same assembled systems, iteration counts and solutions bit for bit
+    ctx.tune("batch_prepass", 1)
+    try:
+        for graph in (False, True):
+            b.solve(tol=1e-8, graph=graph)
+            st = b.stats()
+            assert np.array_equal(st["iters"], ref_st["iters"]) and np.array_equal(st["ncut"], ref_st["ncut"])
+            for p in range(len(meshes)):
+                g = b.problem(p)
+                for k in ("rowptr", "colind", "comb", "rhs", "x"):
+                    assert np.array_equal(g[k], ref[p][k]), (p, k)
+    finally:
+        ctx.tune("batch_prepass", 0)
     b.close()
 
 
