@@ -2539,7 +2539,10 @@ int dhip_batch_problem(dhip_batch_t *b, int32_t p, int32_t *rowptr, int32_t *col
 // on-disk formats (MatrixCL::exportMTX / out / in, VecDescBaseCL::Write / Read)
 // ---------------------------------------------------------------------------
 static int io_error(dhip_ctx *c, int rc, const char *what) {
-    const int code = rc == dhipio::IO_E_COUNT ? DHIP_E_ARG : rc == dhipio::IO_E_SPACE ? DHIP_E_CAPACITY : DHIP_E_IO;
+    const int code = rc == dhipio::IO_E_COUNT   ? DHIP_E_ARG
+                     : rc == dhipio::IO_E_SPACE ? DHIP_E_CAPACITY
+                     : rc == dhipio::IO_E_NOMEM ? DHIP_E_NOMEM
+                                                : DHIP_E_IO;
     return set_error(c, code, "%s: %s", what, dhipio::io_message(rc));
 }
 

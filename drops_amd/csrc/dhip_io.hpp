@@ -34,7 +34,8 @@
 
 namespace dhipio {
 
-enum { IO_OK = 0, IO_E_OPEN = 1, IO_E_WRITE = 2, IO_E_FORMAT = 3, IO_E_COUNT = 4, IO_E_RANGE = 5, IO_E_SPACE = 6 };
+enum { IO_OK = 0, IO_E_OPEN = 1, IO_E_WRITE = 2, IO_E_FORMAT = 3, IO_E_COUNT = 4, IO_E_RANGE = 5, IO_E_SPACE = 6,
+       IO_E_NOMEM = 7 };
 
 enum { FMT_MARKET = 0, FMT_PLAIN = 1 };
 
@@ -87,20 +88,34 @@ inline void append(std::string &s, const char *fmt, A... a) {
 }
 
 // Formats the items [b_t, b_{t+1}) of every part t with fmt(lo, hi, out) in parallel and emits
-// the parts in order.
+// the parts in order.  A part whose thread cannot be started is formatted by the caller; an
+// allocation failure inside a worker is reported, never thrown across the thread.
 template <class F>
 inline int write_parts(Sink &out, const std::vector<int64_t> &bounds, F &&fmt) {
     const int parts = (int)bounds.size() - 1;
     std::vector<std::string> buf(parts);
-    if (parts == 1) {
-        fmt(bounds[0], bounds[1], buf[0]);
-    } else {
-        std::vector<std::thread> th;
-        th.reserve(parts);
-        for (int t = 0; t < parts; ++t)
-            th.emplace_back([&, t] { fmt(bounds[t], bounds[t + 1], buf[t]); });
-        for (auto &x : th) x.join();
+    std::vector<char> failed(parts, 0);
+    auto run = [&](int t) {
+        try {
+            fmt(bounds[t], bounds[t + 1], buf[t]);
+        } catch (...) {
+            failed[t] = 1;
+        }
+    };
+    std::vector<std::thread> th;
+    int started = 0;
+    if (parts > 1) {
+        try {
+            th.reserve(parts);
+            for (; started < parts; ++started) th.emplace_back(run, started);
+        } catch (...) {
+            // fewer threads than parts: the rest runs here
+        }
     }
+    for (int t = started; t < parts; ++t) run(t);
+    for (auto &x : th) x.join();
+    for (int t = 0; t < parts; ++t)
+        if (failed[t]) return IO_E_NOMEM;
     for (auto &s : buf)
         if (int rc = out.put(s)) return rc;
     return IO_OK;
@@ -369,6 +384,7 @@ inline const char *io_message(int rc) {
     case IO_E_COUNT: return "Number of Unknowns does not match, wrong FE-type?";
     case IO_E_RANGE: return "index or size out of range";
     case IO_E_SPACE: return "buffer too small";
+    case IO_E_NOMEM: return "host allocation failed";
     default: return "ok";
     }
 }

@@ -37,11 +37,12 @@ def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, cand
 BATCH_CANDIDATES = ((512, 1, 0), (512, 1, 1), (256, 1, 0), (256, 2, 0))
 
 
-def tune_batch(ctx, batch, tol=1e-8, candidates=BATCH_CANDIDATES, reps=1):
-    """Workgroup size, residency and pre-pass of the batched solve: times one launch per candidate
-    on the caller's problem set and leaves the fastest on the context.  Returns
-    ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
+def tune_batch(ctx, batch, tol=1e-8, candidates=BATCH_CANDIDATES, reps=2):
+    """Workgroup size, residency and pre-pass of the batched solve: times `reps` launches per
+    candidate (best of) on the caller's problem set, after one untimed launch, and leaves the
+    fastest on the context.  Returns ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
     timings = {}
+    batch.solve(tol=tol)   # first launch: code object load and first-touch costs stay out of the timings
     for threads, bpc, pre in candidates:
         ctx.tune("batch_threads", threads)
         ctx.tune("batch_blocks_per_cu", bpc)

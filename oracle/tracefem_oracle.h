@@ -21,6 +21,10 @@
  *   - normal stabilisation: closed form for planar level sets
  *   - CSR builder / SpMV / Kahan dot: dense brute force
  *   - PCG / GMRES: dense direct solve on small systems
+ *   - narrow band: band and near-interface numbering by brute force, width 0 against the
+ *     interface assembly, surface forms under relabelling, planar closed form of the band
+ *     stabilisation (tests/test_oracle_band_pins.py)
+ * The file formats are restated separately with C++ iostreams (format_oracle.cpp).
  */
 #ifndef TRACEFEM_ORACLE_H
 #define TRACEFEM_ORACLE_H
