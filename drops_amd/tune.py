@@ -38,19 +38,20 @@ BATCH_CANDIDATES = ((512, 1, 0), (512, 1, 1), (256, 1, 0), (256, 2, 0))
 
 
 def tune_batch(ctx, batch, tol=1e-8, candidates=BATCH_CANDIDATES, reps=2):
-    """Workgroup size, residency and pre-pass of the batched solve: times `reps` launches per
-    candidate (best of) on the caller's problem set, after one untimed launch, and leaves the
-    fastest on the context.  Returns ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
+    """Workgroup size, residency and pre-pass of the batched solve: times `reps` graph replays per
+    candidate (best of; the capture is untimed) on the caller's problem set, after one untimed
+    launch, and leaves the fastest on the context.  Returns ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
     timings = {}
     batch.solve(tol=tol)   # first launch: code object load and first-touch costs stay out of the timings
     for threads, bpc, pre in candidates:
         ctx.tune("batch_threads", threads)
         ctx.tune("batch_blocks_per_cu", bpc)
         ctx.tune("batch_prepass", pre)
+        batch.solve(tol=tol, graph=True)   # captures this candidate's graph (untimed)
         best = np.inf
         for _ in range(reps):
             t0 = time.perf_counter()
-            batch.solve(tol=tol)
+            batch.solve(tol=tol, graph=True)
             best = min(best, time.perf_counter() - t0)
         timings[(threads, bpc, pre)] = best
     choice = min(timings, key=timings.get)
