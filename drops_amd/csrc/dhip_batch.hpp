@@ -108,9 +108,9 @@ __device__ inline int64_t wg_scan_inplace(int32_t *a, int64_t n) {
 // pre-pass (tuning key batch_prepass): classification, compaction and local matrices of every
 // problem over the whole grid, so that a large problem's most expensive per-tet work does not
 // lengthen its workgroup's span.  The same per-item functions produce the same arrays as the
-// in-kernel path.  On the 512-problem set it measured slower (77.1 against 73.9 ms per batch: the
-// largest problem's assembly left its span, but every problem then entered its solve at once and
-// the solves slowed more), so the autotuner decides per problem set.
+// in-kernel path.  The largest problem's assembly leaves its span, but every problem then enters its
+// solve at once and the solves slow down; which effect wins differs between runs, so the autotuner
+// decides per problem set.
 //   k_batch_cut_count   per (problem, block of cells): cell masks, the block's cut-tet count,
 //                       the problem's exact-zero flag
 //   k_batch_cut_scan    per problem: exclusive scan of its block counts
