@@ -98,7 +98,8 @@ int dhip_profile_reset(dhip_ctx_t *ctx);
  * the last solve's values and `nblocks` its block count. */
 int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
 /* Tuning parameters (chosen by drops_amd/tune.py): "pcg_rows_per_block" -- rows owned
- * by one block of the persistent solvers (0 = library default, 256). */
+ * by one block of the persistent solvers (0 = library default, 256); "batch_threads",
+ * "batch_blocks_per_cu", "batch_prepass" -- see dhip_batch_solve. */
 int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);
 /* Device buffers for callers without their own allocator (bench, tests). */
 int dhip_malloc(dhip_ctx_t *ctx, void **dptr, int64_t bytes);
@@ -369,7 +370,9 @@ int dhip_batch_create(dhip_ctx_t *ctx, int32_t nprob, const dhip_batch_problem_t
 /* Assembles and solves every problem in one persistent launch (relative residual tol, at most
  * max_iter inner iterations per problem).  use_graph: capture the launch into a hipGraph on the
  * first call and replay it while the arguments stay the same.  Workgroup size and residency come
- * from dhip_tune("batch_threads" 128/256/512, "batch_blocks_per_cu"). */
+ * from dhip_tune("batch_threads" 128/256/512, "batch_blocks_per_cu" 1..8); "batch_prepass" 1 runs
+ * classification, compaction and local matrices of all problems over the whole grid before the
+ * per-problem launch (same results; 0, the default, does them inside each problem's workgroup). */
 int dhip_batch_solve(dhip_batch_t *batch, int32_t max_iter, double tol, int32_t rel, int32_t use_graph);
 /* per-problem results of the last solve (nprob entries) */
 int dhip_batch_stats(dhip_batch_t *batch, dhip_batch_stat_t *stats);
