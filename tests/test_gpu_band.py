@@ -30,7 +30,7 @@ def scatter(rp_to, ci_to, rp, ci, val):
 
 
 CASES = [("c1_sphere16", 0.0), ("c1_sphere16", 0.25), ("ellipsoid_ragged", 0.3), ("skew_brick", 0.12),
-         ("torus24", 0.05)]
+         ("torus24", 0.05), ("c1_sphere16", 100.0)]   # the last: every tet of the mesh is in the band
 
 
 @pytest.mark.parametrize("name,width", CASES)
@@ -103,3 +103,20 @@ def test_band_zero_node_and_state(ctx):
     with pytest.raises(D.DhipError) as e:
         ctx.assemble_p1_band(-1.0)
     assert e.value.code == D.E_ARG
+
+
+def test_band_empty(ctx):
+    """no tet within the width of a level set that does not meet the mesh: nothing listed, no rows"""
+    m = KuhnMesh.box(0.0, 1.0, 6)
+    phi = m.sample_p2(plane_x(3.0))     # phi = x - 3 <= -2 on the mesh
+    print("[gpu band] empty", flush=True)
+    ctx.mesh(m)
+    ctx.levelset(phi)
+    nb, nd, nnz = ctx.assemble_p1_band(0.5)
+    assert (nb, nd, nnz) == (0, 0, 0)
+    om = O.make_mesh(m.orig, m.e, m.n)
+    assert O.classify_band(om, phi, 0.5)[0].size == 0
+    nb, nd, nnz = ctx.assemble_p1_band(2.5)      # the tets with a node at x >= 0.5
+    assert nb == O.classify_band(om, phi, 2.5)[0].size and 0 < nb < 6 * 6 ** 3
+    nb, nd, nnz = ctx.assemble_p1_band(3.5)      # |phi| <= 3: the whole mesh
+    assert nb == 6 * 6 ** 3 and nd == 7 ** 3
