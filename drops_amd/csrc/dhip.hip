@@ -326,6 +326,7 @@ int scan_exclusive_i32(dhip_ctx *ctx, const int32_t *in, int32_t *out, int64_t n
         DHIP_LAUNCH(ctx, "scan_reduce", k_scan_reduce, (unsigned)nb, 256, 0, in, n, bsum);
     }
     DHIP_LAUNCH(ctx, "scan_top", k_scan_top, 1, 256, 0, bsum, nb, tot);
+    ctx->scan_total_dev = tot;
     if (nb > 0) {
         DHIP_LAUNCH(ctx, "scan_final", k_scan_final, (unsigned)nb, 256, 0, in, out, n, bsum);
     }
@@ -335,6 +336,20 @@ int scan_exclusive_i32(dhip_ctx *ctx, const int32_t *in, int32_t *out, int64_t n
         DHIP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
         *total_host = t;
     }
+    return DHIP_OK;
+}
+
+// the total of the last scan_exclusive_i32 and the error flags, with one synchronisation
+static int read_total_and_flags(dhip_ctx *ctx, int64_t *total) {
+    int32_t t = 0, f = 0;
+    DHIP_HIPCHK(ctx, hipMemcpyAsync(&t, ctx->scan_total_dev, sizeof t, hipMemcpyDeviceToHost, ctx->stream));
+    DHIP_HIPCHK(ctx, hipMemcpyAsync(&f, ctx->errflag.p, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
+    DHIP_HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *total = t;
+    if (f & 2)
+        return set_error(ctx, DHIP_E_ZERO_PATTERN, "found a full body in the zero level set, grid is too coarse");
+    if (f & 1)
+        return set_error(ctx, DHIP_E_ZERO_NODE, "levelset is exactly zero at node(s) of a cut tetra");
     return DHIP_OK;
 }
 
@@ -522,14 +537,15 @@ static int classify_impl(dhip_ctx *c, int32_t zero_policy, int64_t *ncut, bool s
     c->stage = 0;
     c->band = band;
     DHIP_LAUNCH(c, "classify", k_classify, nb, CLS_BLOCK, 0, ca);
-    int64_t total = 0;
-    int rc = scan_exclusive_i32(c, c->blockcnt.as<int32_t>(), c->blockcnt.as<int32_t>(), nb, &total);
+    int rc = scan_exclusive_i32(c, c->blockcnt.as<int32_t>(), c->blockcnt.as<int32_t>(), nb, nullptr);
     if (rc) return rc;
     (void)sync_count;
+    // one readback: the cut-tet count (the scan's total) and the error flags
+    int64_t total = 0;
+    rc = read_total_and_flags(c, &total);
+    if (rc) return rc;
     if (total >= ((int64_t)1 << 26))
         return set_error(c, DHIP_E_CAPACITY, "%lld cut tets exceed the 2^26 supported", (long long)total);
-    rc = check_err_flag(c);
-    if (rc) return rc;
     DHIP_HIPCHK(c, c->cut.ensure(sizeof(int64_t) * (total + 1)));
     CompactArgs pa{ncell, c->cellmask.as<uint8_t>(), c->blockcnt.as<int32_t>(), c->cellinfo.as<uint32_t>(),
                    c->cut.as<int64_t>()};
@@ -550,7 +566,9 @@ int dhip_classify(dhip_ctx_t *c, int32_t zero_policy, int64_t *ncut) {
     return flush_profile(c);
 }
 
-static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
+// check: read the error flags now; the assembly pipeline leaves that to the numbering's readback
+static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param,
+                      bool check = true) {
     if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_local_p1");
     const int64_t N = c->ncut;
     DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC_N * (N + 1)));
@@ -560,8 +578,10 @@ static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const
                  rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
                  c->errflag.as<int32_t>(), c->band >= 0. ? 1 : 0};
     if (N > 0) DHIP_LAUNCH(c, "local_p1", k_local_p1, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la);
-    int rc = check_err_flag(c);
-    if (rc) return rc;
+    if (check) {
+        const int rc = check_err_flag(c);
+        if (rc) return rc;
+    }
     c->stage = 2;
     c->space = 1;
     return DHIP_OK;
@@ -592,7 +612,10 @@ static int number_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
         DHIP_LAUNCH(c, "first_owner", k_first, blocks_for(S, 256), 256, 0, fa);
     }
     int64_t nd = 0;
-    int rc = scan_exclusive_i32(c, c->isown.as<int32_t>(), c->ownrank.as<int32_t>(), S, &nd);
+    int rc = scan_exclusive_i32(c, c->isown.as<int32_t>(), c->ownrank.as<int32_t>(), S, nullptr);
+    if (rc) return rc;
+    // one readback: the DOF count and the local step's error flags
+    rc = read_total_and_flags(c, &nd);
     if (rc) return rc;
     DHIP_HIPCHK(c, c->rowowner.ensure(sizeof(int32_t) * (nd + 1)));
     if (N > 0) {
@@ -631,11 +654,8 @@ static int build_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
     int64_t total = 0;
     int rc = scan_exclusive_i32(c, c->rowlen.as<int32_t>(), c->rowptr.as<int32_t>(), R, &total);
     if (rc) return rc;
-    {
-        int32_t t32 = (int32_t)total;  // rowptr[R] = nnz
-        DHIP_HIPCHK(c, hipMemcpyAsync(c->rowptr.as<int32_t>() + R, &t32, sizeof t32, hipMemcpyHostToDevice,
-                                      c->stream));
-    }
+    // rowptr[R] = nnz, from the scan's device total (no host-to-device copy)
+    DHIP_LAUNCH(c, "rowptr_last", k_copy_i32, 1, 64, 0, c->scan_total_dev, c->rowptr.as<int32_t>() + R);
     DHIP_HIPCHK(c, c->col.ensure(sizeof(int32_t) * (total + 1)));
     for (int k = 0; k < 4; ++k) DHIP_HIPCHK(c, c->val[k].ensure(sizeof(double) * (total + 1)));
     ra.col = c->col.as<int32_t>();
@@ -665,7 +685,7 @@ int dhip_assemble_p1(dhip_ctx_t *c, int32_t zero_policy, const double coef[3], i
     if (!c) return DHIP_E_ARG;
     DHIP_HIPCHK(c, hipSetDevice(c->device));
     int rc = classify_impl(c, zero_policy, ncut, false);
-    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param);
+    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param, false);
     if (!rc) rc = number_impl(c, omit_bound, ndof);
     if (!rc) rc = build_impl(c, lc, nnz);
     if (rc) return rc;
@@ -680,7 +700,7 @@ int dhip_assemble_p1_band(dhip_ctx_t *c, int32_t zero_policy, const double coef[
     if (!(width >= 0.)) return set_error(c, DHIP_E_ARG, "narrow-band width must be >= 0");
     DHIP_HIPCHK(c, hipSetDevice(c->device));
     int rc = classify_impl(c, zero_policy, nband, false, width);
-    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param);
+    if (!rc) rc = local_impl(c, coef, rhs_kind, rhs_param, false);
     if (!rc) rc = number_impl(c, -1.0, ndof);
     if (!rc) rc = build_impl(c, lc, nnz);
     if (rc) return rc;
@@ -1148,7 +1168,14 @@ static int launch_persistent(dhip_ctx *c, const char *name, const void *const ke
     if (lds > (size_t)LDS_BYTES)
         DHIP_HIPCHK(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     int per_cu = 0;
-    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, lds));
+    const auto key = std::make_pair(k, lds);
+    const auto hit = c->occupancy.find(key);
+    if (hit != c->occupancy.end()) {
+        per_cu = hit->second;
+    } else {
+        DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PB, lds));
+        c->occupancy[key] = per_cu;
+    }
     if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
         return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
     void *argv[] = {&args};

@@ -414,6 +414,7 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
         for (int q = 0; q < 4; ++q) s[q] = sgn(ls[c_tab.lat_t[lt][q]]);
         if (!s[0] && !s[1] && !s[2] && !s[3]) {
             atomicOr(a.err, 2);
+            a.flags[c] = 0;   // defined for the steps that run before the error is read
             return;
         }
         const int pid = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;

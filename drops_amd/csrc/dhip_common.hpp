@@ -95,6 +95,9 @@ struct dhip_ctx {
     hipStream_t stream = nullptr;
     char err[512] = {0};
     int cu_count = 0;
+    // resident blocks per CU of a persistent kernel at a dynamic LDS size (queried once)
+    std::map<std::pair<const void *, size_t>, int> occupancy;
+    int32_t *scan_total_dev = nullptr;   // device total of the last scan_exclusive_i32
 
     // profiling: per-kernel accumulated milliseconds / launches
     bool prof = false;

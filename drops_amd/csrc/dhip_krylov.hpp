@@ -88,6 +88,10 @@ __global__ void k_zero_u32(uint32_t *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+__global__ void k_copy_i32(const int32_t *src, int32_t *dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
+}
+
 // ---------------------------------------------------------------------------
 // Grid synchronisation for cooperative (co-resident) launches.
 //
