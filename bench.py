@@ -197,8 +197,9 @@ def batch_leg(nprob=512, seed=0, steps=3, cpu_sample=8, with_cpu=True):
         b = ctx.batch(meshes, phi, off, rhs_kind=D.RHS_3XY_MINUS_X3)
         choice, timings = tune.tune_batch(ctx, b)
         out["tuned"] = {"threads": choice[0], "blocks_per_cu": choice[1], "prepass": choice[2],
-                        "candidates_ms": {f"{k[0]}x{k[1]}{'+pre' if k[2] else ''}": 1e3 * v
-                                          for k, v in timings.items()}}
+                        "solve_threads": choice[3] or None,
+                        "candidates_ms": {f"{k[0]}x{k[1]}{'+pre' if k[2] else ''}{f'+pcg{k[3]}' if k[3] else ''}":
+                                          1e3 * v for k, v in timings.items()}}
         b.solve(tol=1e-8, graph=True)   # capture
         ctx.profile_reset()
         ctx.profile(True)

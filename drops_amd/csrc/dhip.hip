@@ -1327,6 +1327,17 @@ int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
         c->batch_threads = (int)value;
         return DHIP_OK;
     }
+    if (!std::strcmp(key, "batch_split")) {
+        if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "batch_split must be 0 or 1");
+        c->batch_split = (int)value;
+        return DHIP_OK;
+    }
+    if (!std::strcmp(key, "batch_solve_threads")) {
+        if (value != 256 && value != 512 && value != 1024)
+            return set_error(c, DHIP_E_ARG, "batch_solve_threads must be 256, 512 or 1024");
+        c->batch_solve_threads = (int)value;
+        return DHIP_OK;
+    }
     if (!std::strcmp(key, "batch_prepass")) {
         if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "batch_prepass must be 0 or 1");
         c->batch_prepass = (int)value;
@@ -2213,7 +2224,7 @@ struct dhip_batch {
     int32_t max_cells = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
-    int g_threads = 0, g_grid = 0, g_max_iter = 0, g_rel = 0, g_prepass = -1;
+    int g_threads = 0, g_grid = 0, g_max_iter = 0, g_rel = 0, g_prepass = -1, g_split = -1, g_sthreads = 0;
     int64_t g_lds = 0;
     double g_tol = 0.0;
 };
@@ -2234,6 +2245,30 @@ static void batch_release(dhip_batch *b) {
 static const void *batch_kernel(int threads) {
     return threads == 128 ? (const void *)k_batch<128>
                           : threads == 512 ? (const void *)k_batch<512> : (const void *)k_batch<256>;
+}
+
+static const void *batch_solve_kernel(int threads) {
+    return threads == 1024 ? (const void *)k_batch_solve<1024>
+                           : threads == 512 ? (const void *)k_batch_solve<512> : (const void *)k_batch_solve<256>;
+}
+
+// Dynamic LDS (the search direction) and grid of a per-problem batch kernel at a residency of
+// `per_cu_want` workgroups per CU.
+static int batch_launch_shape(dhip_ctx *c, const void *k, int threads, int per_cu_want, int32_t nprob, int64_t *lds,
+                              int *grid) {
+    int maxs = 0;
+    hipFuncAttributes fa{};
+    DHIP_HIPCHK(c, hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
+    DHIP_HIPCHK(c, hipFuncGetAttributes(&fa, k));
+    int64_t l = ((int64_t)maxs / per_cu_want - (int64_t)fa.sharedSizeBytes - 1024) & ~(int64_t)255;
+    l = std::max<int64_t>(0, std::min<int64_t>(l, 128 * 1024));
+    if (l > 64 * 1024) DHIP_HIPCHK(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l));
+    int per_cu = 0;
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, (size_t)l));
+    if (per_cu < 1) return set_error(c, DHIP_E_HIP, "batched kernel cannot be resident");
+    *lds = l;
+    *grid = (int)std::min<int64_t>(nprob, (int64_t)c->cu_count * std::min(per_cu, per_cu_want));
+    return DHIP_OK;
 }
 
 extern "C" {
@@ -2444,18 +2479,21 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
     const int threads = c->batch_threads;
     const void *k = batch_kernel(threads);
     // dynamic LDS for the search direction: what the requested residency leaves of the CU's LDS
-    int maxs = 0;
-    hipFuncAttributes fa{};
-    DHIP_HIPCHK(c, hipDeviceGetAttribute(&maxs, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, c->device));
-    DHIP_HIPCHK(c, hipFuncGetAttributes(&fa, k));
-    int64_t lds = ((int64_t)maxs / c->batch_blocks_per_cu - (int64_t)fa.sharedSizeBytes - 1024) & ~(int64_t)255;
-    lds = std::max<int64_t>(0, std::min<int64_t>(lds, 128 * 1024));
-    if (lds > 64 * 1024) DHIP_HIPCHK(c, hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    int per_cu = 0;
-    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, (size_t)lds));
-    if (per_cu < 1) return set_error(c, DHIP_E_HIP, "batched solve kernel cannot be resident");
-    const int grid = (int)std::min<int64_t>(b->nprob, (int64_t)c->cu_count * std::min(per_cu, c->batch_blocks_per_cu));
-    b->args.lds_floats = lds / 4;
+    int64_t lds = 0;
+    int grid = 0;
+    int rc = batch_launch_shape(c, k, threads, c->batch_blocks_per_cu, b->nprob, &lds, &grid);
+    if (rc) return rc;
+    const int split = c->batch_split;
+    const int sthreads = c->batch_solve_threads;
+    const void *ks = batch_solve_kernel(sthreads);
+    int64_t slds = 0;
+    int sgrid = 0;
+    if (split) {
+        rc = batch_launch_shape(c, ks, sthreads, 1, b->nprob, &slds, &sgrid);
+        if (rc) return rc;
+    }
+    b->args.split = split;
+    b->args.lds_floats = (split ? slds : lds) / 4;
     b->args.max_iter = max_iter;
     b->args.tol = tol;
     b->args.rel = rel;
@@ -2479,14 +2517,17 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
         }
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        return hipLaunchKernel(k, dim3(grid), dim3(threads), argv, (size_t)lds, c->stream);
+        e = hipLaunchKernel(k, dim3(grid), dim3(threads), argv, split ? 0 : (size_t)lds, c->stream);
+        if (e != hipSuccess || !split) return e;
+        return hipLaunchKernel(ks, dim3(sgrid), dim3(sthreads), argv, (size_t)slds, c->stream);
     };
     if (!use_graph) {
         ProfScope ps(c, "batch");
         DHIP_HIPCHK(c, enqueue());
     } else {
         const bool same = b->exec && b->g_threads == threads && b->g_grid == grid && b->g_max_iter == max_iter &&
-                          b->g_tol == tol && b->g_rel == rel && b->g_lds == lds && b->g_prepass == a.prepass;
+                          b->g_tol == tol && b->g_rel == rel && b->g_lds == lds && b->g_prepass == a.prepass &&
+                          b->g_split == split && b->g_sthreads == sthreads;
         if (!same) {
             if (b->exec) (void)hipGraphExecDestroy(b->exec);
             if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -2507,6 +2548,8 @@ int dhip_batch_solve(dhip_batch_t *b, int32_t max_iter, double tol, int32_t rel,
             b->g_rel = rel;
             b->g_lds = lds;
             b->g_prepass = a.prepass;
+            b->g_split = split;
+            b->g_sthreads = sthreads;
         }
         ProfScope ps(c, "batch_graph");
         DHIP_HIPCHK(c, hipGraphLaunch(b->exec, c->stream));

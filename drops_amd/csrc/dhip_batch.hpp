@@ -229,6 +229,7 @@ struct BatchArgs {
     double inner_tol;
     int inner_max;
     int prepass = 0;            // 1: cut lists, cell words, flags and local data come from the pre-pass
+    int split = 0;              // 1: k_batch assembles only, k_batch_solve runs the PCGs
     const int32_t *pre_zero = nullptr, *pre_err = nullptr;
 };
 
@@ -304,7 +305,12 @@ __device__ __forceinline__ double sell_spmv_group(const int32_t *soff, const int
 }
 
 template <int BB>
-__device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
+__device__ void batch_pcg(const BatchArgs &a, int p, float *lds_p, BatchStat &st);
+
+// Assembly of problem p, then (solve) its PCG; without solve the stat entry records the assembly
+// and k_batch_solve runs the PCG.
+template <int BB>
+__device__ void batch_problem(const BatchArgs &a, int p, float *lds_p, bool solve) {
     __shared__ int sh_zero, sh_err;   // workgroup flags (LDS: no stale cache lines)
     const BatchProb P = a.prob[p];
     const MeshP m = P.m;
@@ -439,6 +445,30 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
         }
         __syncthreads();
         st.t_asm = wall_clock64();
+        if (solve) batch_pcg<BB>(a, p, lds_p, st);
+    }
+    st.t1 = wall_clock64();
+    if (tid == 0) a.stat[p] = st;
+    __syncthreads();
+}
+
+// Jacobi PCG of an assembled problem: FP64 residual and refinement, FP32 inner iterations on the
+// SELL-64 copy; st receives iterations, residual and status.
+template <int BB>
+__device__ void batch_pcg(const BatchArgs &a, int p, float *lds_p, BatchStat &st) {
+    const BatchProb P = a.prob[p];
+    const int tid = threadIdx.x;
+    const int64_t nd = st.ndof;
+    const int32_t *rowptr = a.rowptr + P.row_off + p;
+    const int32_t *col = a.col + P.nz_off;
+    const double *vc = a.vc + P.nz_off;
+    const double *rhs = a.rhs + P.row_off, *diag = a.diag + P.row_off;
+    double *x = a.x + P.row_off, *r64 = a.r64 + P.row_off;
+    float *d32 = a.d32 + P.row_off, *r32 = a.r32 + P.row_off, *z32 = a.z32 + P.row_off;
+    float *p32 = a.p32 + P.row_off, *q32 = a.q32 + P.row_off;
+    const int32_t *scol = a.scol + P.sell_off, *soff = a.soff + P.slice_off;
+    const float *sval = a.sval + P.sell_off;
+    {
 
         // ---- Jacobi PCG: FP64 residual and refinement, FP32 inner iterations ----
         int total_it = 0, outer = 0, status = 1;
@@ -447,7 +477,8 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
         for (;;) {
             double p_rr = 0.0, p_bb = 0.0;
             for (int64_t r = tid; r < nd; r += BB) {
-                const double s = row_product<double>(rowptr, col, vc, 0, r, [&](int32_t c) { return x[c]; });
+                const double s =
+                    row_product<double, (BB >= 1024 ? 8 : MAXR)>(rowptr, col, vc, 0, r, [&](int32_t c) { return x[c]; });
                 const double bv = rhs[r];
                 const double rv = bv - s;
                 r64[r] = rv;
@@ -483,7 +514,7 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
             float beta = 0.0f;
             for (int it = 1; it <= a.inner_max && total_it < a.max_iter; ++it) {
                 ++total_it;
-                constexpr int GU = 8;   // rows per group of the vector loops
+                constexpr int GU = BB >= 1024 ? 4 : 8;   // rows per group of the vector loops
                 for (int64_t r0 = tid; r0 < nd; r0 += (int64_t)GU * BB) {
                     float zl[GU], pl[GU];
 #pragma unroll
@@ -501,7 +532,8 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
                     }
                 }
                 __syncthreads();
-                const double lam = block_sum<BB>(sell_spmv_group<BB, 4>(soff, scol, sval, rowptr, pv, q32, nd));
+                const double lam =
+                    block_sum<BB>(sell_spmv_group<BB, (BB >= 1024 ? 1 : 4)>(soff, scol, sval, rowptr, pv, q32, nd));
                 const float alpha = (float)(rho / lam);
                 double p_rr2 = 0.0, p_zr2 = 0.0;
                 for (int64_t r0 = tid; r0 < nd; r0 += (int64_t)GU * BB) {
@@ -546,9 +578,6 @@ __device__ void batch_problem(const BatchArgs &a, int p, float *lds_p) {
         st.resid = resid;
         st.status = status == 0 ? DHIP_OK : DHIP_E_NOT_CONVERGED;
     }
-    st.t1 = wall_clock64();
-    if (tid == 0) a.stat[p] = st;
-    __syncthreads();
 }
 
 template <int BB>
@@ -564,7 +593,32 @@ __global__ void __launch_bounds__(BB) k_batch(BatchArgs a) {
         const int p = sh_p;
         __syncthreads();
         if (p < 0) break;
-        batch_problem<BB>(a, p, reinterpret_cast<float *>(lds));
+        batch_problem<BB>(a, p, reinterpret_cast<float *>(lds), !a.split);
+    }
+}
+
+// The PCG of every assembled problem (split mode): problems from the second queue counter in the
+// same largest-first order, BB threads each.
+template <int BB>
+__global__ void __launch_bounds__(BB) k_batch_solve(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ int sh_p;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int q = atomicAdd(a.queue + 1, 1);
+            sh_p = q < a.nprob ? a.order[q] : -1;
+        }
+        __syncthreads();
+        const int p = sh_p;
+        __syncthreads();
+        if (p < 0) break;
+        BatchStat st = a.stat[p];
+        if (st.status == DHIP_OK) {
+            batch_pcg<BB>(a, p, reinterpret_cast<float *>(lds), st);
+            st.t1 = wall_clock64();
+            if (threadIdx.x == 0) a.stat[p] = st;
+        }
+        __syncthreads();
     }
 }
 

@@ -33,29 +33,38 @@ def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, cand
     return choice, timings
 
 
-# (threads, workgroups per CU, grid-wide pre-pass of classification and local matrices)
-BATCH_CANDIDATES = ((512, 1, 0), (512, 1, 1), (256, 1, 0), (256, 2, 0))
+# (threads, workgroups per CU, grid-wide pre-pass of classification and local matrices,
+#  threads per problem of a separate PCG launch or 0 for none)
+BATCH_CANDIDATES = ((512, 1, 0, 0), (512, 1, 1, 0), (256, 1, 0, 0), (512, 1, 1, 1024), (512, 1, 1, 512),
+                    (512, 1, 0, 1024))
 
 
 def tune_batch(ctx, batch, tol=1e-8, candidates=BATCH_CANDIDATES, reps=2):
-    """Workgroup size, residency and pre-pass of the batched solve: times `reps` graph replays per
-    candidate (best of; the capture is untimed) on the caller's problem set, after one untimed
-    launch, and leaves the fastest on the context.  Returns ((threads, blocks_per_cu, prepass), {candidate: seconds})."""
+    """Workgroup size, residency, pre-pass and separate PCG launch of the batched solve: times `reps`
+    graph replays per candidate (best of; the capture is untimed) on the caller's problem set, after
+    one untimed launch, and leaves the fastest on the context.  Returns
+    ((threads, blocks_per_cu, prepass, solve_threads), {candidate: seconds})."""
     timings = {}
     batch.solve(tol=tol)   # first launch: code object load and first-touch costs stay out of the timings
-    for threads, bpc, pre in candidates:
+    for threads, bpc, pre, sthreads in candidates:
         ctx.tune("batch_threads", threads)
         ctx.tune("batch_blocks_per_cu", bpc)
         ctx.tune("batch_prepass", pre)
+        ctx.tune("batch_split", 1 if sthreads else 0)
+        if sthreads:
+            ctx.tune("batch_solve_threads", sthreads)
         batch.solve(tol=tol, graph=True)   # captures this candidate's graph (untimed)
         best = np.inf
         for _ in range(reps):
             t0 = time.perf_counter()
             batch.solve(tol=tol, graph=True)
             best = min(best, time.perf_counter() - t0)
-        timings[(threads, bpc, pre)] = best
+        timings[(threads, bpc, pre, sthreads)] = best
     choice = min(timings, key=timings.get)
     ctx.tune("batch_threads", choice[0])
     ctx.tune("batch_blocks_per_cu", choice[1])
     ctx.tune("batch_prepass", choice[2])
+    ctx.tune("batch_split", 1 if choice[3] else 0)
+    if choice[3]:
+        ctx.tune("batch_solve_threads", choice[3])
     return choice, timings

@@ -109,6 +109,26 @@ def test_batch_graph_replay_and_tiles(ctx):
                     assert np.array_equal(g[k], ref[p][k]), (p, k)
     finally:
         ctx.tune("batch_prepass", 0)
+    # assembly launch and a separate PCG launch: with the same workgroup size bit for bit, with
+    # 1024 threads per problem to the same tolerance
+    ctx.tune("batch_split", 1)
+    try:
+        for sthreads in (256, 1024):
+            ctx.tune("batch_solve_threads", sthreads)
+            for graph in (False, True):
+                b.solve(tol=1e-8, graph=graph)
+                st = b.stats()
+                assert np.all(st["status"] == D.OK) and np.all(st["resid"] <= 1e-8)
+                for p in range(len(meshes)):
+                    g = b.problem(p)
+                    assert np.array_equal(g["comb"], ref[p]["comb"]) and np.array_equal(g["rhs"], ref[p]["rhs"])
+                    if sthreads == 256:
+                        assert np.array_equal(g["x"], ref[p]["x"]), p
+                    else:
+                        assert np.linalg.norm(g["x"] - ref[p]["x"]) <= 1e-6 * np.linalg.norm(ref[p]["x"])
+    finally:
+        ctx.tune("batch_split", 0)
+        ctx.tune("batch_solve_threads", 1024)
     b.close()
 
 
