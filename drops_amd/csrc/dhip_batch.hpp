@@ -13,6 +13,10 @@
 //                                   the per-problem scratch; the rows of a problem are at most its
 //                                   cut-tet vertices, each row has at most 15 entries)
 //   k_batch<BB>                     the persistent solve, BB threads per problem
+//   k_batch_solve<BB>               with the tuning key batch_split: k_batch stops after assembly and
+//                                   this second launch runs the PCGs, BB (up to 1024) threads per
+//                                   problem, largest first; the solve is bandwidth-bound per problem,
+//                                   so more threads than the assembly's workgroup shorten the span
 #pragma once
 
 #include "dhip_assembly.hpp"

@@ -99,7 +99,8 @@ int dhip_profile_reset(dhip_ctx_t *ctx);
 int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
 /* Tuning parameters (chosen by drops_amd/tune.py): "pcg_rows_per_block" -- rows owned
  * by one block of the persistent solvers (0 = library default, 256); "batch_threads",
- * "batch_blocks_per_cu", "batch_prepass" -- see dhip_batch_solve. */
+ * "batch_blocks_per_cu", "batch_prepass", "batch_split", "batch_solve_threads" -- see
+ * dhip_batch_solve. */
 int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);
 /* Device buffers for callers without their own allocator (bench, tests). */
 int dhip_malloc(dhip_ctx_t *ctx, void **dptr, int64_t bytes);
@@ -372,7 +373,11 @@ int dhip_batch_create(dhip_ctx_t *ctx, int32_t nprob, const dhip_batch_problem_t
  * first call and replay it while the arguments stay the same.  Workgroup size and residency come
  * from dhip_tune("batch_threads" 128/256/512, "batch_blocks_per_cu" 1..8); "batch_prepass" 1 runs
  * classification, compaction and local matrices of all problems over the whole grid before the
- * per-problem launch (same results; 0, the default, does them inside each problem's workgroup). */
+ * per-problem launch (same results; 0, the default, does them inside each problem's workgroup).
+ * "batch_split" 1 ends the per-problem launch after assembly and runs the PCGs in a second
+ * launch with "batch_solve_threads" (256/512/1024, default 1024) threads per problem, largest
+ * first (bit-identical to the single launch when both use the same number of threads; with
+ * another number the reductions add their partial sums in another order). */
 int dhip_batch_solve(dhip_batch_t *batch, int32_t max_iter, double tol, int32_t rel, int32_t use_graph);
 /* per-problem results of the last solve (nprob entries) */
 int dhip_batch_stats(dhip_batch_t *batch, dhip_batch_stat_t *stats);
