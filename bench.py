@@ -427,8 +427,8 @@ def main():
                          "rows per block and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; the multi-GPU sharding is "
-                                   "described in DESIGN.md and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "bench.py runs on exactly one GPU; sharding the solve across "
+                                   "GPUs is not implemented", "n_gpus": args.gpus}))
         sys.exit(2)
 
     if args.gmres_probe:

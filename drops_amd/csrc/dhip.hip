@@ -1263,6 +1263,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
                      {c->kv[0].as<double>(), c->kv[1].as<double>()}, {c->kv[2].as<double>(), c->kv[3].as<double>()},
                      {c->kv[4].as<double>(), c->kv[5].as<double>()}, c->bar.as<SyncSlot>(), *max_iter, *tol, rel,
                      outs, outd, outs + 1, nullptr};
+        pa.xcd_map = c->xcd_map;
         if (c->probe_on) {
             DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
             pa.probe = c->probe.as<uint64_t>();
@@ -1276,6 +1277,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
                    c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(), c->bar.as<SyncSlot>(),
                    *max_iter, *tol, rel, outs, outd, outs + 1};
+        pa.xcd_map = c->xcd_map;
         const void *ks[5] = {(const void *)k_pcg64<1>, (const void *)k_pcg64<2>, (const void *)k_pcg64<4>,
                              (const void *)k_pcg64<8>, (const void *)k_pcg64<16>};
         rc = launch_persistent(c, "pcg64_classic", ks, rpt, nb, pa);
@@ -1286,6 +1288,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         PcgMixArgs pa{n, A->rowptr, A->col, A->val, diag, val32, db, dx, c->kv[6].as<float>(), c->kv[7].as<float>(),
                       c->kv[8].as<float>(), c->bar.as<SyncSlot>(), *max_iter, *tol, rel, 1e-4, 400, outs, outd,
                       outs + 1, outs + 2};
+        pa.xcd_map = c->xcd_map;
         const void *ks[5] = {(const void *)k_pcg_mixed<1>, (const void *)k_pcg_mixed<2>, (const void *)k_pcg_mixed<4>,
                              (const void *)k_pcg_mixed<8>, (const void *)k_pcg_mixed<16>};
         rc = launch_persistent(c, "pcg_mixed", ks, rpt, nb, pa);
@@ -1316,6 +1319,11 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
 
 int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
     if (!c || !key) return DHIP_E_ARG;
+    if (!std::strcmp(key, "xcd_map")) {
+        if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "xcd_map must be 0 or 1");
+        c->xcd_map = (int)value;
+        return DHIP_OK;
+    }
     if (!std::strcmp(key, "pcg_rows_per_block")) {
         if (value < 0 || value > 16 * PB) return set_error(c, DHIP_E_ARG, "pcg_rows_per_block out of range");
         c->pcg_rows_per_block = value;
@@ -1384,6 +1392,7 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     GmresArgs ga{n, A->rowptr, A->col, A->val, diag, db, dx, c->kv[15].as<double>(),
                  c->bar.as<SyncSlot>(), c->red_tmp.as<double>(), m, *max_iter, *tol, rel, left_pc ? 1 : 0,
                  outs, outd, outs + 1, nullptr, nullptr, LDS_BYTES, nullptr, nullptr, nullptr, 0};
+    ga.xcd_map = c->xcd_map;
     if (diag && !left_pc && A->nnz > 0) {
         DHIP_HIPCHK(c, c->gm_sval.ensure(sizeof(double) * (A->nnz + 1)));
         DHIP_LAUNCH(c, "gmres_scale", k_scale_vals, blocks_for(A->nnz, 256), 256, 0, A->nnz, A->col, A->val, diag,

@@ -141,6 +141,7 @@ struct dhip_ctx {
     int64_t pcg_rows_per_block = 0;   // 0: one row per thread of a 512-thread block
     int batch_threads = 256;          // workgroup size of the batched solve (128, 256, 512)
     int batch_blocks_per_cu = 2;      // resident workgroups per CU of the batched solve
+    int xcd_map = 1;                  // persistent solvers: each XCD's blocks own one contiguous run of rows
     int batch_split = 0;              // batched solve: assembly launch, then a separate PCG launch
     int batch_solve_threads = 1024;   // threads per problem of the separate PCG launch
     int batch_prepass = 0;            // batched solve: classification and local matrices over the whole grid first

@@ -117,6 +117,7 @@ struct alignas(128) SyncSlot {
 };
 
 constexpr uint32_t SPIN_LIMIT = 1u << 22;  // polls; a healthy barrier takes microseconds
+constexpr int POLL_CHUNKS = 4;             // slots polled per lane in one round (covers 256 blocks)
 
 __device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
@@ -144,29 +145,39 @@ __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, doubl
         }
         int ok = 1;
         double v[SYNC_VALS] = {0.0, 0.0, 0.0};
-        for (int base = 0; base < nb; base += 64) {
-            const int l = base + lane;
-            if (l < nb) {
-                uint64_t *w = slots[l].w[par];
-                uint64_t x[2 * SYNC_VALS];
-                uint32_t spins = 0;
-                for (;;) {
-                    bool ready = true;
+        // lane l polls slots l, l + 64, l + 128, l + 192 with all loads of a round in flight
+        // together (one polling round trip, not one per 64 blocks), then adds them in that order
+        for (int base = 0; base < nb; base += POLL_CHUNKS * 64) {
+            uint64_t x[POLL_CHUNKS][2 * SYNC_VALS];
+            uint32_t spins = 0;
+            for (;;) {
+                bool ready = true;
 #pragma unroll
-                    for (int i = 0; i < 2 * SYNC_VALS; ++i) {
-                        x[i] = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ready &= (uint32_t)(x[i] >> 32) == g;
-                    }
-                    if (ready) break;
-                    if (++spins > SPIN_LIMIT) {
-                        ok = 0;
-                        break;
+                for (int j = 0; j < POLL_CHUNKS; ++j) {
+                    const int l = base + j * 64 + lane;
+                    if (l < nb) {
+                        const uint64_t *w = slots[l].w[par];
+#pragma unroll
+                        for (int i = 0; i < 2 * SYNC_VALS; ++i) {
+                            x[j][i] = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            ready &= (uint32_t)(x[j][i] >> 32) == g;
+                        }
                     }
                 }
-#pragma unroll
-                for (int i = 0; i < SYNC_VALS; ++i)
-                    v[i] += __longlong_as_double((long long)((x[2 * i] & 0xffffffffull) | (x[2 * i + 1] << 32)));
+                if (ready) break;
+                if (++spins > SPIN_LIMIT) {
+                    ok = 0;
+                    break;
+                }
             }
+#pragma unroll
+            for (int j = 0; j < POLL_CHUNKS; ++j)
+                if (base + j * 64 + lane < nb) {
+#pragma unroll
+                    for (int i = 0; i < SYNC_VALS; ++i)
+                        v[i] += __longlong_as_double(
+                            (long long)((x[j][2 * i] & 0xffffffffull) | (x[j][2 * i + 1] << 32)));
+                }
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
@@ -198,6 +209,21 @@ __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, doubl
 // two grid synchronisations: after q = A p (lambda) and after the vector
 // update (|r|^2, z.r).
 // ---------------------------------------------------------------------------
+// Blocks of a launch are dispatched to the 8 XCDs round robin (block b to XCD b % 8).  Position of
+// block b when each XCD's blocks are to own consecutive row ranges: the blocks of the XCDs before
+// its own, then b / 8.  A bijection on [0, nb) whatever the dispatch does.
+__device__ inline unsigned xcd_position(unsigned b, unsigned nb) {
+    const unsigned x = b % 8, q = nb / 8, rem = nb % 8;
+    return x * q + (x < rem ? x : rem) + b / 8;
+}
+
+// Row range of this block among gridDim.x equal ranges: range index blockIdx.x, or with xcd_map the
+// block's position above, so that the rows a block gathers from its neighbours were mostly written
+// by blocks of its own XCD and are read from the L2 they share.
+__device__ inline unsigned row_range_index(int xcd_map) {
+    return xcd_map ? xcd_position(blockIdx.x, gridDim.x) : blockIdx.x;
+}
+
 constexpr int PB = 512;                   // threads per block of the persistent solvers
 constexpr int LDS_BYTES = 64 * 1024;      // dynamic LDS per block for the CSR slice
 
@@ -214,6 +240,7 @@ struct PcgArgs {
     int *out_iter;
     double *out_resid;
     int *out_status;            // 0 converged, 1 not converged, 2 synchronisation timeout
+    int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
 };
 
 // CSR slice of a block: LDS copy when it fits, else the global arrays.
@@ -297,7 +324,7 @@ template <int RPT>
 __global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
     const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
     const Slice<double> S = load_slice<double>(lds, a.rp, a.col, a.val, r0, r1);
     uint32_t gen = 0;
@@ -440,6 +467,7 @@ struct PcgSrArgs {
     double *out_resid;
     int *out_status;
     uint64_t *probe;            // optional [gridDim.x][4] wall-clock ticks: setup, compute, sync, total
+    int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
 };
 
 // LDS copy of the slice with values divided by the diagonal of their column.
@@ -484,7 +512,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
     const uint64_t t_start = wall_clock64();
     uint64_t t_compute = 0, t_sync = 0, t_mark = 0;
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
     const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
     bool scaled;
     const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, a.val, a.diag, r0, r1, scaled);
@@ -648,16 +676,21 @@ __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t 
             __hip_atomic_store(&slots[blockIdx.x].w[par][0], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         int ok = 1;
-        for (int base = 0; base < nb; base += 64) {
-            const int l = base + lane;
-            if (l < nb) {
-                uint32_t spins = 0;
-                while ((uint32_t)(__hip_atomic_load(&slots[l].w[par][0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >>
-                                  32) != g) {
-                    if (++spins > SPIN_LIMIT) {
-                        ok = 0;
-                        break;
-                    }
+        for (int base = 0; base < nb; base += POLL_CHUNKS * 64) {
+            uint32_t spins = 0;
+            for (;;) {
+                bool ready = true;
+#pragma unroll
+                for (int j = 0; j < POLL_CHUNKS; ++j) {
+                    const int l = base + j * 64 + lane;
+                    if (l < nb)
+                        ready &= (uint32_t)(__hip_atomic_load(&slots[l].w[par][0], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT) >> 32) == g;
+                }
+                if (ready) break;
+                if (++spins > SPIN_LIMIT) {
+                    ok = 0;
+                    break;
                 }
             }
         }
@@ -761,6 +794,7 @@ struct GmresArgs {
     const int32_t *sell_off, *sell_col;
     const double *sell_val;
     int sell_spb;               // slices per block
+    int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
 };
 
 // SELL-64 layout of a CSR matrix partitioned into blocks of `per` consecutive rows: the local rows
@@ -824,7 +858,8 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     __shared__ double msh[KMAX + 2];         // this block's partial sums
     const int64_t n = a.n;
     const int64_t per = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const unsigned range = row_range_index(a.xcd_map);
+    const int64_t r0 = (int64_t)range * per;
     const int64_t r1 = r0 + per < n ? r0 + per : n;
     const bool right = !a.left;
     bool in_lds;
@@ -839,7 +874,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     const bool sell = !in_lds && a.sell_val;
     const int32_t *sell_col = a.sell_col;
     const double *sell_val = a.sell_val;
-    const int32_t *sell_off = a.sell_off + (sell ? (int64_t)blockIdx.x * a.sell_spb : 0);
+    const int32_t *sell_off = a.sell_off + (sell ? (int64_t)range * a.sell_spb : 0);
     auto spmv_row = [&](int64_t l, auto &&yv_) {
         if (sell) {
             // the same sum as row_product, entries read from the coalesced SELL-64 copy
@@ -1217,13 +1252,14 @@ struct PcgMixArgs {
     double *out_resid;
     int *out_status;
     int *out_outer;
+    int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
 };
 
 template <int RPT>
 __global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blockIdx.x * per;
+    const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
     const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
     const Slice<float> S = load_slice<float>(lds, a.rp, a.col, a.val32, r0, r1);
     uint32_t gen = 0;

@@ -98,7 +98,10 @@ int dhip_profile_reset(dhip_ctx_t *ctx);
  * the last solve's values and `nblocks` its block count. */
 int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nblocks);
 /* Tuning parameters (chosen by drops_amd/tune.py): "pcg_rows_per_block" -- rows owned
- * by one block of the persistent solvers (0 = library default, 256); "batch_threads",
+ * by one block of the persistent solvers (0 = library default, 256); "xcd_map" -- 1 (default):
+ * the blocks of each XCD own one contiguous run of row ranges, so most gathered neighbour rows
+ * come from the XCD's own L2; 0: range i belongs to block i (same results up to the order in
+ * which the blocks' partial sums are added); "batch_threads",
  * "batch_blocks_per_cu", "batch_prepass", "batch_split", "batch_solve_threads" -- see
  * dhip_batch_solve. */
 int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);

@@ -107,3 +107,37 @@ def test_pcg_on_uploaded_matrix(ctx):
         xd = np.linalg.solve(M.toarray(), b)
         assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-10
     A.close()
+
+
+@pytest.mark.parametrize("variant", [D.FP64, D.FP64_CLASSIC, D.FP32_IR, "gmres"])
+def test_xcd_row_mapping(ctx, variant):
+    """Persistent solvers with each XCD's blocks owning one contiguous run of rows (xcd_map 1,
+    the default) against the plain block order (xcd_map 0), on a block count that is not a
+    multiple of 8 (47 rows per block): the same system solved to the same tolerance, the
+    solutions within the solve's accuracy of each other and of the oracle's tight solution."""
+    g, A = assembled_system(ctx, 48)
+    rp = g["rowptr"].astype(np.int64)
+    b = g["rhs"]
+    n = b.size
+    assert (n + 46) // 47 % 8 != 0
+    xs, _, _, _ = O.pcg_jacobi(rp, g["colind"], g["comb"], b, tol=1e-13)
+    out = {}
+    try:
+        ctx.tune("pcg_rows_per_block", 47)
+        for xm in (0, 1):
+            ctx.tune("xcd_map", xm)
+            print(f"[gpu krylov] xcd_map={xm} variant={variant}", flush=True)
+            if variant == "gmres":
+                x, it, res, ok = ctx.gmres(A, b, tol=1e-10)
+            else:
+                x, it, res, ok = ctx.pcg(A, b, tol=1e-10, precision=variant)
+            assert ok and res <= 1e-10
+            assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+            out[xm] = (x, it)
+    finally:
+        ctx.tune("pcg_rows_per_block", 0)
+        ctx.tune("xcd_map", 1)
+    (x0, it0), (x1, it1) = out[0], out[1]
+    assert abs(it0 - it1) <= max(2, 0.02 * it0)
+    assert np.linalg.norm(x0 - x1) / np.linalg.norm(x0) < 1e-8
+    A.close()
