@@ -242,8 +242,8 @@ hipError_t take_event(dhip_ctx *ctx, hipEvent_t *ev) {
     return hipEventCreate(ev);
 }
 
-ProfScope::ProfScope(dhip_ctx *c, const char *n) : ctx(c), name(n) {
-    if (!ctx->prof) return;
+ProfScope::ProfScope(dhip_ctx *c, const char *n, bool solver) : ctx(c), name(n) {
+    if (!(ctx->prof == 1 || (ctx->prof == 2 && solver))) return;
     if (take_event(ctx, &a) != hipSuccess || take_event(ctx, &b) != hipSuccess) {
         a = b = nullptr;
         return;
@@ -252,7 +252,7 @@ ProfScope::ProfScope(dhip_ctx *c, const char *n) : ctx(c), name(n) {
 }
 
 ProfScope::~ProfScope() {
-    if (!ctx->prof || !a) return;
+    if (!a) return;
     (void)hipEventRecord(b, ctx->stream);
     ctx->pending.push_back({name, a, b});
 }
@@ -427,7 +427,8 @@ int dhip_profile_enable(dhip_ctx_t *c, int32_t on) {
         int rc = flush_profile(c);
         if (rc) return rc;
     }
-    c->prof = on != 0;
+    if (on < 0 || on > 2) return set_error(c, DHIP_E_ARG, "profile mode must be 0, 1 or 2");
+    c->prof = on;
     return DHIP_OK;
 }
 
@@ -1179,7 +1180,7 @@ static int launch_persistent(dhip_ctx *c, const char *name, const void *const ke
     if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
         return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
     void *argv[] = {&args};
-    ProfScope ps(c, name);
+    ProfScope ps(c, name, true);
     if (!c->coop_refused) {
         const hipError_t e = hipLaunchCooperativeKernel(k, dim3(nb), dim3(PB), argv, lds, c->stream);
         if (e == hipSuccess) return DHIP_OK;

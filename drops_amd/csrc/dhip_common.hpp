@@ -100,7 +100,7 @@ struct dhip_ctx {
     int32_t *scan_total_dev = nullptr;   // device total of the last scan_exclusive_i32
 
     // profiling: per-kernel accumulated milliseconds / launches
-    bool prof = false;
+    int prof = 0;                     // event timing: 0 off, 1 every launch, 2 persistent solvers only
     struct Pending { std::string name; hipEvent_t a, b; };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
@@ -176,7 +176,7 @@ struct ProfScope {
     dhip_ctx *ctx;
     const char *name;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(dhip_ctx *c, const char *n);
+    ProfScope(dhip_ctx *c, const char *n, bool solver = false);
     ~ProfScope();
 };
 hipError_t take_event(dhip_ctx *ctx, hipEvent_t *ev);

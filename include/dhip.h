@@ -86,9 +86,11 @@ typedef struct dhip_csr dhip_csr_t;
 int dhip_create(dhip_ctx_t **ctx, int32_t device);
 void dhip_destroy(dhip_ctx_t *ctx);
 const char *dhip_last_error(const dhip_ctx_t *ctx);
-/* Per-kernel HIP-event timing on the context's stream (0 = off).  While on,
- * every library kernel launch is bracketed by events; dhip_profile_get returns
- * the accumulated milliseconds and launch count of kernel `name`. */
+/* Per-kernel HIP-event timing on the context's stream: 0 = off, 1 = every library kernel
+ * launch is bracketed by events, 2 = only the persistent solver launches ("pcg64",
+ * "pcg64_classic", "pcg_mixed", "gmres"), so that timing them adds nothing to the other
+ * launches.  dhip_profile_get returns the accumulated milliseconds and launch count of
+ * kernel `name`. */
 int dhip_profile_enable(dhip_ctx_t *ctx, int32_t on);
 int dhip_profile_get(dhip_ctx_t *ctx, const char *name, double *total_ms, int64_t *count);
 int dhip_profile_reset(dhip_ctx_t *ctx);

@@ -141,3 +141,26 @@ def test_xcd_row_mapping(ctx, variant):
     assert abs(it0 - it1) <= max(2, 0.02 * it0)
     assert np.linalg.norm(x0 - x1) / np.linalg.norm(x0) < 1e-8
     A.close()
+
+
+def test_profile_solver_only(ctx):
+    """profile mode 2 brackets only the persistent solver launches with events; mode 1 every launch"""
+    mesh, phi = cases.c1_sphere(24)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    try:
+        for mode, expect_classify in ((2, 0), (1, 1)):
+            ctx.profile_reset()
+            ctx.profile(mode)
+            ctx.assemble_p1(coef=(1.0, 1.0, 1.0), rhs_kind=D.RHS_3XY_MINUS_X3, lc=(1.0, 1.0, mesh.h))
+            A = ctx.csr_from_assembly(D.OP_COMB)
+            ctx.pcg(A, ctx.csr()["rhs"], tol=1e-10)
+            A.close()
+            ms, cnt = ctx.profile_get("pcg64")
+            assert cnt == 1 and ms > 0.0
+            assert ctx.profile_get("classify")[1] == expect_classify
+        with pytest.raises(D.DhipError) as e:
+            ctx.profile(3)
+        assert e.value.code == D.E_ARG
+    finally:
+        ctx.profile(0)
