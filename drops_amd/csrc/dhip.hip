@@ -12,6 +12,7 @@
 #include "dhip_io.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <atomic>
 #include <mutex>
 
@@ -1145,6 +1146,14 @@ int dhip_axpy(dhip_ctx_t *c, int64_t n, double a, const double *x, double *y, in
 
 // Launch geometry of the persistent solvers: one row per thread when the rows
 // fit 512 x (CUs) threads, else RPT rows per thread; one block per CU at most.
+// Results the persistent solvers leave in the context's staging buffer (int32 slots 0..3: iterations,
+// status, outer iterations, spare; then the final residual at byte 16), read back with one copy.
+struct SolverResult {
+    int32_t i[4];
+    double resid;
+};
+static_assert(sizeof(SolverResult) == 24 && offsetof(SolverResult, resid) == 16, "staging layout");
+
 static int persistent_geometry(dhip_ctx *c, int64_t n, unsigned *nb, int *rpt) {
     const int64_t cap_blocks = c->cu_count;
     // default 256 rows per block: measured optimum of compute (gathers) vs synchronisation
@@ -1297,15 +1306,15 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     } else {
         return set_error(c, DHIP_E_ARG, "unknown precision");
     }
-    int32_t hs[4];
-    DHIP_HIPCHK(c, hipMemcpyAsync(hs, outs, sizeof hs, hipMemcpyDeviceToHost, c->stream));
-    DHIP_HIPCHK(c, hipMemcpyAsync(&h_resid, outd, sizeof h_resid, hipMemcpyDeviceToHost, c->stream));
+    SolverResult res;
+    DHIP_HIPCHK(c, hipMemcpyAsync(&res, outs, sizeof res, hipMemcpyDeviceToHost, c->stream));
     if (where != DHIP_DEVICE && n > 0)
         DHIP_HIPCHK(c, hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
-    h_iter = hs[0];
-    h_status = hs[1];
-    h_outer = hs[2];
+    h_iter = res.i[0];
+    h_status = res.i[1];
+    h_outer = res.i[2];
+    h_resid = res.resid;
     (void)h_outer;
     bb.release();
     bx.release();
@@ -1455,11 +1464,11 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     ga.lds_bytes = (int64_t)lds;
     rc = launch_persistent(c, "gmres", ks, rpt, nb, ga, lds);
     if (rc) return rc;
-    int32_t hs[4];
-    double h_resid = 0.0;
-    DHIP_HIPCHK(c, hipMemcpyAsync(hs, outs, sizeof hs, hipMemcpyDeviceToHost, c->stream));
-    DHIP_HIPCHK(c, hipMemcpyAsync(&h_resid, outd, sizeof h_resid, hipMemcpyDeviceToHost, c->stream));
+    SolverResult res;
+    DHIP_HIPCHK(c, hipMemcpyAsync(&res, outs, sizeof res, hipMemcpyDeviceToHost, c->stream));
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int32_t hs[2] = {res.i[0], res.i[1]};
+    const double h_resid = res.resid;
     *max_iter = hs[0];
     *tol = h_resid;
     if (hs[1] == 2) return set_error(c, DHIP_E_HIP, "grid synchronisation timed out in the persistent solver");
