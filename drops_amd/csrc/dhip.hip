@@ -1088,7 +1088,10 @@ int dhip_spmv(dhip_ctx_t *c, const dhip_csr_t *A, const double *x, double *y, in
         dy = by.as<double>();
     }
     if (A->n > 0)
-        DHIP_LAUNCH(c, "spmv", k_spmv, blocks_for(A->n * SPMV_G, KB), KB, 0, A->n, A->rowptr, A->col, A->val, dx, dy);
+    {
+        const int rb = spmv_tile_rows(A->n, A->nnz);
+        DHIP_LAUNCH(c, "spmv", k_spmv, blocks_for(A->n, rb), SPMV_T, 0, A->n, rb, A->rowptr, A->col, A->val, dx, dy);
+    }
     if (where != DHIP_DEVICE && A->n > 0)
         DHIP_HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * A->n, hipMemcpyDeviceToHost, c->stream));
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -37,6 +37,8 @@ def test_spmv_dot_axpy(ctx):
     yr = O.spmv(rp, g["colind"], g["comb"], x)
     S = sp.csr_matrix((np.abs(g["comb"]), g["colind"], rp), shape=(n, n))
     assert np.all(np.abs(y - yr) <= 1e-15 * (S @ np.abs(x)) + 1e-300)
+    # each row is summed in column order, as the oracle's (and the reference's y_Ax) serial loop
+    assert np.array_equal(y, yr)
     d = ctx.dot(x, y)
     dr = O.kahan_dot(x, y)
     assert abs(d - dr) <= 1e-14 * (np.abs(x) @ np.abs(y))
@@ -164,3 +166,25 @@ def test_profile_solver_only(ctx):
         assert e.value.code == D.E_ARG
     finally:
         ctx.profile(0)
+
+
+def test_spmv_row_lengths_and_fallback(ctx):
+    """SpMV tiles whose entries fit the LDS stage and tiles that do not (rows of 0 to 400
+    entries, the long ones grouped so that whole tiles overflow): every row summed in column
+    order, bit-identical to the oracle's serial loop."""
+    rng = np.random.default_rng(5)
+    n = 3000
+    lens = rng.integers(0, 20, size=n)
+    lens[1000:1100] = rng.integers(200, 400, size=100)   # tiles of these rows exceed the stage
+    lens[2000:2010] = 0
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(n, size=k, replace=False)) for k in lens]).astype(np.int32)
+    val = rng.normal(size=col.size)
+    x = rng.normal(size=n)
+    print(f"[gpu krylov] spmv ragged n={n} nnz={col.size}", flush=True)
+    A = ctx.csr_create(rp.astype(np.int32), col, val)
+    y = ctx.spmv(A, x)
+    A.close()
+    assert np.array_equal(y, O.spmv(rp, col, val, x))
+    assert np.all(y[2000:2010] == 0.0)
