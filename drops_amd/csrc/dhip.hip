@@ -1090,7 +1090,8 @@ int dhip_spmv(dhip_ctx_t *c, const dhip_csr_t *A, const double *x, double *y, in
     if (A->n > 0)
     {
         const int rb = spmv_tile_rows(A->n, A->nnz);
-        DHIP_LAUNCH(c, "spmv", k_spmv, blocks_for(A->n, rb), SPMV_T, 0, A->n, rb, A->rowptr, A->col, A->val, dx, dy);
+        DHIP_LAUNCH(c, "spmv", k_spmv<false>, blocks_for(A->n, rb), SPMV_T, 0, A->n, rb, A->rowptr, A->col, A->val, dx,
+                    nullptr, dy);
     }
     if (where != DHIP_DEVICE && A->n > 0)
         DHIP_HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * A->n, hipMemcpyDeviceToHost, c->stream));
@@ -2140,9 +2141,11 @@ int dhip_ch_residual(dhip_ch_t *h, double *r) {
     if (!h->prepared) return set_error(c, DHIP_E_STATE, "dhip_ch_prepare must precede dhip_ch_residual");
     DHIP_HIPCHK(c, hipSetDevice(c->device));
     const int64_t n2 = 2 * h->ndof;
-    if (n2 > 0)
-        DHIP_LAUNCH(c, "ch_residual", k_ch_residual, blocks_for(n2, 256), 256, 0, n2, h->K.rowptr, h->K.col, h->K.val,
-                    h->rhs.as<double>(), h->x.as<double>(), r);
+    if (n2 > 0) {
+        const int rb = spmv_tile_rows(n2, h->K.nnz);
+        DHIP_LAUNCH(c, "ch_residual", k_spmv<true>, blocks_for(n2, rb), SPMV_T, 0, n2, rb, h->K.rowptr, h->K.col,
+                    h->K.val, h->x.as<double>(), h->rhs.as<double>(), r);
+    }
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return flush_profile(c);
 }
@@ -2164,8 +2167,9 @@ static int ch_learned_start(dhip_ch *h, dhip_mlp *q, double *x0, int32_t **flag_
     int32_t *flag = reinterpret_cast<int32_t *>(part + 4 * SCALE_BLOCKS);
     const double *xn = h->x.as<double>();
     // residual of the new system at the current state, features, prediction
-    DHIP_LAUNCH(c, "ch_residual", k_ch_residual, blocks_for(n2, 256), 256, 0, n2, h->K.rowptr, h->K.col, h->K.val,
-                h->rhs.as<double>(), xn, res);
+    const int rb = spmv_tile_rows(n2, h->K.nnz);
+    DHIP_LAUNCH(c, "ch_residual", k_spmv<true>, blocks_for(n2, rb), SPMV_T, 0, n2, rb, h->K.rowptr, h->K.col, h->K.val,
+                xn, h->rhs.as<double>(), res);
     const double *hist[4] = {xn, h->hist[0].as<double>(), h->hist[1].as<double>(), h->hist[2].as<double>()};
     const int levels = 1 + h->nhist;
     int rc = mlp_features_impl(q, hist, levels, res, h->pcdiag.as<double>(), n, h->feat.as<float>());

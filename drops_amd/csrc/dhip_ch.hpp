@@ -297,13 +297,6 @@ __global__ void k_ch_block(ChBlockArgs a) {
 }
 
 // r = b - K x (one lane per row of the block system, row gathers batched)
-__global__ void k_ch_residual(int64_t n2, const int32_t *rp, const int32_t *col, const double *val, const double *b,
-                              const double *x, double *r) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n2) return;
-    r[i] = b[i] - row_product<double>(rp, col, val, 0, i, [&](int32_t c) { return x[c]; });
-}
-
 // |b - K x_k|^2 of three candidate starts and |b|^2 per block (block-order sums later)
 constexpr int RES3_B = 256;
 __global__ void __launch_bounds__(RES3_B) k_ch_three_resid(int64_t n2, const int32_t *rp, const int32_t *col,

@@ -37,10 +37,12 @@ __device__ __forceinline__ double row_dot(const int32_t *rp, const int32_t *col,
 // with more than SPMV_CAP entries sums its rows straight from global memory, one row per thread, in
 // the same order.  The 16 KB stage keeps up to 8 blocks per CU resident (a 32 KB stage measured
 // 3.5 TB/s, 16 KB 4.7 TB/s, 8 KB 4.5 TB/s on a 315 MB matrix of 11-entry rows).
+// RESID: y = b - A x instead of y = A x (b - the serial row sum).
 constexpr int SPMV_T = 256, SPMV_CAP = 2048, SPMV_PT = SPMV_CAP / SPMV_T;
 
+template <bool RESID>
 __global__ void __launch_bounds__(SPMV_T) k_spmv(int64_t n, int rb, const int32_t *rp, const int32_t *col,
-                                                 const double *val, const double *x, double *y) {
+                                                 const double *val, const double *x, const double *b, double *y) {
     __shared__ double prod[SPMV_CAP];
     const int64_t r0 = (int64_t)blockIdx.x * rb;
     if (r0 >= n) return;   // the same for every thread of the block
@@ -67,13 +69,13 @@ __global__ void __launch_bounds__(SPMV_T) k_spmv(int64_t n, int rb, const int32_
         if (r < r1) {
             double s = 0.0;
             for (int32_t k = rp[r] - z0, e = rp[r + 1] - z0; k < e; ++k) s += prod[k];
-            y[r] = s;
+            y[r] = RESID ? b[r] - s : s;
         }
     } else {
         for (int64_t r = r0 + threadIdx.x; r < r1; r += SPMV_T) {
             double s = 0.0;
             for (int32_t k = rp[r], e = rp[r + 1]; k < e; ++k) s += val[k] * x[col[k]];
-            y[r] = s;
+            y[r] = RESID ? b[r] - s : s;
         }
     }
 }
