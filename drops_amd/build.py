@@ -41,7 +41,7 @@ HOST_EXAMPLES = {"tracefem_p1": os.path.join(HERE, "host", "examples", "tracefem
 
 
 def build_host_examples(verbose=False):
-    """C++ drivers written against the host mirror (drops_amd/host/dropship.hpp), linked to libdhip."""
+    """C++ example programs written against the host mirror (drops_amd/host/dropship.hpp), linked to libdhip."""
     out = {}
     for name, src in HOST_EXAMPLES.items():
         exe = os.path.join(HERE, "lib", name)

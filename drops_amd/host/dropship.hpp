@@ -239,7 +239,7 @@ class SparseMatBaseCL {
 
     // MatrixMarket file of the matrix (num/spmat.h:1084-1089)
     SparseMatBaseCL &exportMTX(std::string const &path);
-    // needs the MATLAB engine (num/spmat.h:1095-1125); this build has none
+    // needs the MATLAB engine (num/spmat.h:1095-1125); the library has none
     SparseMatBaseCL &exportMAT(std::string const &) { throw std::invalid_argument("DROPS is compiled w/o matlab"); }
 
     template <class U>

@@ -1,4 +1,4 @@
-// tracefem_p1.cpp -- a DROPS-style driver of the P1 trace-FEM path written against
+// tracefem_p1.cpp -- a DROPS-style program for the P1 trace-FEM path written against
 // the host mirror (dropship.hpp), in the shape of SetupLBP1 / the interfaceP1FE test:
 // Kuhn brick, P2 level set, interface numbering, accumulator tuple, LinComb, PCG.
 //

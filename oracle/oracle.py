@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (test infrastructure only).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
-module, as the checker.  It loads oracle/_build/liboracle.so (parity build,
+module, to check results against.  It loads oracle/_build/liboracle.so (parity build,
 no FMA contraction) or liboracle_omp.so (OpenMP build with the reference's
 release flags, used for CPU timing).
 """

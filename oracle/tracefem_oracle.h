@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the DROPS
  * (56th/drops) algorithms, used by tests/, by __graft_entry__.smoke() and by
- * the cpu_baseline leg of bench.py as the checker.  No product code path
+ * the cpu_baseline leg of bench.py to check results.  No product code path
  * links, loads or calls it.
  *
  * Every function names the reference file:line it restates (paths relative

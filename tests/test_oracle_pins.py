@@ -1,7 +1,7 @@
 """Pins of the CPU oracle (no GPU): the reference's known answers, brute-force
 checks of the tables, closed forms for planar interfaces and exact triangle
 integrals, so that a dropped term, a wrong index or a transposed operand in the
-oracle fails here before it can be used as the GPU's checker."""
+oracle fails here before GPU results are compared with it."""
 import itertools
 
 import numpy as np
