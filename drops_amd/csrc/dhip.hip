@@ -370,6 +370,8 @@ static int check_err_flag(dhip_ctx *ctx) {
 
 using namespace dhip;
 
+static int noalloc_setup(dhip_ctx *c);
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -392,9 +394,19 @@ int dhip_create(dhip_ctx_t **out, int32_t device) {
     }
     c->cu_count = prop.multiProcessorCount;
     if (c->errflag.ensure(256) != hipSuccess || c->bar.ensure(256) != hipSuccess) {
+        c->errflag.release();
         (void)hipStreamDestroy(c->stream);
         delete c;
         return DHIP_E_NOMEM;
+    }
+    // once per context (and so per device it runs on): the device address of the allocation-free
+    // solver's scratch array
+    if (noalloc_setup(c) != DHIP_OK) {
+        c->errflag.release();
+        c->bar.release();
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return DHIP_E_HIP;
     }
     *out = c;
     return DHIP_OK;
@@ -1220,7 +1232,135 @@ static size_t persistent_lds(dhip_ctx *c, const void *k) {
     return (size_t)std::max<int64_t>(dyn, LDS_BYTES);
 }
 
+// Work buffers of the FP64 single-reduction PCG: the Jacobi diagonal and the exchanged r, w, s
+// (double buffered), the synchronisation slots (one per block) and the 24-byte result record.
+struct PcgFp64Bufs {
+    double *diag;
+    double *vec[6];   // R0, R1, W0, W1, S0, S1
+    SyncSlot *slots;  // at least one per block of the launch
+    SolverResult *res;
+};
+
+// Diagonal scan, slot clearing (a fill kernel: no memset node) and the persistent FP64 PCG launch.
+// `alloc_ok` false: nothing on this path may allocate (no profiling events, no occupancy cache insert).
+// `max_blocks` > 0: the slots hold at most that many blocks.
+static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double *db, double *dx,
+                           const PcgFp64Bufs &w, int32_t max_iter, double tol, int32_t rel, bool alloc_ok,
+                           unsigned max_blocks = 0) {
+    const int64_t n = A->n;
+    unsigned nb = 1;
+    int rpt = 1;
+    int rc = persistent_geometry(c, n, &nb, &rpt);
+    if (rc) return rc;
+    if (max_blocks && nb > max_blocks)
+        return set_error(c, DHIP_E_CAPACITY, "%u blocks exceed the %u synchronisation slots of the scratch", nb,
+                         max_blocks);
+    const unsigned slot_words = (unsigned)(nb * sizeof(SyncSlot) / 4);
+    if (alloc_ok) {
+        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, w.diag,
+                    (float *)nullptr);
+        DHIP_LAUNCH(c, "zero_slots", k_zero_u32, blocks_for(slot_words, 256), 256, 0,
+                    reinterpret_cast<uint32_t *>(w.slots), (int)slot_words);
+    } else {
+        hipLaunchKernelGGL(k_diag, dim3(blocks_for(n, KB)), dim3(KB), 0, c->stream, n, A->rowptr, A->col, A->val,
+                           w.diag, (float *)nullptr);
+        DHIP_HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(slot_words, 256)), dim3(256), 0, c->stream,
+                           reinterpret_cast<uint32_t *>(w.slots), (int)slot_words);
+        DHIP_HIPCHK(c, hipGetLastError());
+    }
+    int32_t *outs = w.res->i;
+    double *outd = &w.res->resid;
+    PcgSrArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? w.diag : nullptr, db, dx,
+                 {w.vec[0], w.vec[1]}, {w.vec[2], w.vec[3]}, {w.vec[4], w.vec[5]}, w.slots, max_iter, tol, rel,
+                 outs, outd, outs + 1, nullptr};
+    pa.xcd_map = c->xcd_map;
+    const void *ks[5] = {(const void *)k_pcg64_sr<1>, (const void *)k_pcg64_sr<2>, (const void *)k_pcg64_sr<4>,
+                         (const void *)k_pcg64_sr<8>, (const void *)k_pcg64_sr<16>};
+    if (alloc_ok) {
+        if (c->probe_on) {
+            DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
+            pa.probe = c->probe.as<uint64_t>();
+            c->probe_blocks = nb;
+        }
+        return launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+    }
+    // allocation-free launch: occupancy queried directly, no event timing
+    const int idx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
+    int per_cu = 0;
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[idx], PB, LDS_BYTES));
+    if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
+        return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
+    void *argv[] = {&pa};
+    if (!c->coop_refused) {
+        const hipError_t e = hipLaunchCooperativeKernel(ks[idx], dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream);
+        if (e == hipSuccess) return DHIP_OK;
+        (void)hipGetLastError();
+        c->coop_refused = true;
+    }
+    DHIP_HIPCHK(c, hipLaunchKernel(ks[idx], dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
+    return DHIP_OK;
+}
+
+// Per-device scratch of dhip_pcg_noalloc: the slots of up to 256 blocks, the result record, then the
+// diagonal and the six exchanged vectors of at most DHIP_NOALLOC_MAX_ROWS rows.
+constexpr unsigned NOALLOC_MAX_BLOCKS = 256;
+constexpr size_t NOALLOC_SLOT_BYTES = NOALLOC_MAX_BLOCKS * sizeof(SyncSlot);
+constexpr size_t NOALLOC_RES_BYTES = 256;
+constexpr size_t NOALLOC_VEC_BYTES = sizeof(double) * (size_t)DHIP_NOALLOC_MAX_ROWS;
+constexpr size_t NOALLOC_BYTES = NOALLOC_SLOT_BYTES + NOALLOC_RES_BYTES + 7 * NOALLOC_VEC_BYTES;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// Device address of this device's copy of `scratch` (the host-side name is only a stand-in).
+// Called once per context, at creation, with the context's device current.
+static int noalloc_setup(dhip_ctx *c) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        c->noalloc_scratch = nullptr;
+        return set_error(c, DHIP_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    }
+    c->noalloc_scratch = static_cast<unsigned char *>(p);
+    return DHIP_OK;
+}
+
 extern "C" {
+
+int dhip_pcg_noalloc(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
+                     int32_t *max_iter, double *tol, int32_t rel) {
+    if (!c || !A || !b || !x || !max_iter || !tol) return DHIP_E_ARG;
+    if (pc != DHIP_PC_NONE && pc != DHIP_PC_JACOBI) return set_error(c, DHIP_E_ARG, "unknown preconditioner");
+    if (A->n > DHIP_NOALLOC_MAX_ROWS)
+        return set_error(c, DHIP_E_CAPACITY, "%lld rows exceed the allocation-free solver's scratch (max %d rows)",
+                         (long long)A->n, DHIP_NOALLOC_MAX_ROWS);
+    if (!c->noalloc_scratch) return set_error(c, DHIP_E_STATE, "scratch address of this device is unknown");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = A->n;
+    if (n == 0) {
+        *max_iter = 0;
+        *tol = 0.0;
+        return DHIP_OK;
+    }
+    unsigned char *s = c->noalloc_scratch;
+    PcgFp64Bufs w;
+    w.slots = reinterpret_cast<SyncSlot *>(s);
+    w.res = reinterpret_cast<SolverResult *>(s + NOALLOC_SLOT_BYTES);
+    double *v = reinterpret_cast<double *>(s + NOALLOC_SLOT_BYTES + NOALLOC_RES_BYTES);
+    w.diag = v;
+    for (int k = 0; k < 6; ++k) w.vec[k] = v + (size_t)(k + 1) * DHIP_NOALLOC_MAX_ROWS;
+    int rc = pcg_fp64_launch(c, A, pc, b, x, w, *max_iter, *tol, rel, false, NOALLOC_MAX_BLOCKS);
+    if (rc) return rc;
+    SolverResult res;
+    DHIP_HIPCHK(c, hipMemcpyAsync(&res, w.res, sizeof res, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    *max_iter = res.i[0];
+    *tol = res.resid;
+    if (res.i[1] == 2) return set_error(c, DHIP_E_HIP, "grid barrier timed out in the persistent solver");
+    if (res.i[1] != 0)
+        return set_error(c, DHIP_E_NOT_CONVERGED, "PCG did not converge: resid %g after %d", res.resid, res.i[0]);
+    return DHIP_OK;
+}
 
 int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, double *x, int32_t where,
              int32_t *max_iter, double *tol, int32_t rel, int32_t precision) {
@@ -1257,36 +1397,32 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
     double *diag = c->kv[12].as<double>();
     float *diag32 = c->kv[13].as<float>();
     float *val32 = c->kv[14].as<float>();
-    if (n > 0)
-        DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, diag,
-                    mixed ? diag32 : nullptr);
-    if (mixed && A->nnz > 0)
-        DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, val32);
+    int h_iter = 0, h_status = 1, h_outer = 0;
+    double h_resid = 0.0;
     unsigned nb = 1;
     int rpt = 1;
     rc = persistent_geometry(c, n, &nb, &rpt);
     if (rc) return rc;
     DHIP_HIPCHK(c, c->bar.ensure(sizeof(SyncSlot) * (nb + 1)));
-    DHIP_LAUNCH(c, "zero_slots", k_zero_u32, blocks_for((int64_t)nb * sizeof(SyncSlot) / 4, 256), 256, 0,
-                c->bar.as<uint32_t>(), (int)(nb * sizeof(SyncSlot) / 4));
-    int h_iter = 0, h_status = 1, h_outer = 0;
-    double h_resid = 0.0;
     if (precision == DHIP_FP64) {
-        for (int k = 0; k < 6; ++k) DHIP_HIPCHK(c, c->kv[k].ensure(sizeof(double) * (n + 1)));
-        PcgSrArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
-                     {c->kv[0].as<double>(), c->kv[1].as<double>()}, {c->kv[2].as<double>(), c->kv[3].as<double>()},
-                     {c->kv[4].as<double>(), c->kv[5].as<double>()}, c->bar.as<SyncSlot>(), *max_iter, *tol, rel,
-                     outs, outd, outs + 1, nullptr};
-        pa.xcd_map = c->xcd_map;
-        if (c->probe_on) {
-            DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
-            pa.probe = c->probe.as<uint64_t>();
-            c->probe_blocks = nb;
-        }
-        const void *ks[5] = {(const void *)k_pcg64_sr<1>, (const void *)k_pcg64_sr<2>, (const void *)k_pcg64_sr<4>,
-                             (const void *)k_pcg64_sr<8>, (const void *)k_pcg64_sr<16>};
-        rc = launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+        PcgFp64Bufs w;
+        w.diag = diag;
+        for (int k = 0; k < 6; ++k) w.vec[k] = c->kv[k].as<double>();
+        w.slots = c->bar.as<SyncSlot>();
+        w.res = reinterpret_cast<SolverResult *>(outs);
+        rc = pcg_fp64_launch(c, A, pc, db, dx, w, *max_iter, *tol, rel, true);
         if (rc) return rc;
+    } else {
+        if (n > 0)
+            DHIP_LAUNCH(c, "csr_diag", k_diag, blocks_for(n, KB), KB, 0, n, A->rowptr, A->col, A->val, diag,
+                        mixed ? diag32 : nullptr);
+        if (mixed && A->nnz > 0)
+            DHIP_LAUNCH(c, "csr_to_f32", k_to_f32, blocks_for(A->nnz, KB), KB, 0, A->nnz, A->val, val32);
+        DHIP_LAUNCH(c, "zero_slots", k_zero_u32, blocks_for((int64_t)nb * sizeof(SyncSlot) / 4, 256), 256, 0,
+                    c->bar.as<uint32_t>(), (int)(nb * sizeof(SyncSlot) / 4));
+    }
+    if (precision == DHIP_FP64) {
+        // launched by pcg_fp64_launch above
     } else if (precision == DHIP_FP64_CLASSIC) {
         PcgArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? diag : nullptr, db, dx,
                    c->kv[1].as<double>(), c->kv[2].as<double>(), c->kv[3].as<double>(), c->bar.as<SyncSlot>(),

@@ -148,6 +148,7 @@ struct dhip_ctx {
     bool coop_refused = false;        // cooperative launch unavailable: same grid, regular launch
     int probe_blocks = 0;
     dhip::DevBuf probe;
+    unsigned char *noalloc_scratch = nullptr;   // this device's copy of the allocation-free solver's scratch
 };
 
 struct dhip_csr {

@@ -40,7 +40,7 @@ EXPORTS = [
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
-    "dhip_assemble_p1_band",
+    "dhip_assemble_p1_band", "dhip_pcg_noalloc",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -125,6 +125,7 @@ def load(path=LIB_PATH):
     L.dhip_dot.argtypes = [P, C.c_int64, VP, VP, D, C.c_int32]
     L.dhip_axpy.argtypes = [P, C.c_int64, C.c_double, VP, VP, C.c_int32]
     L.dhip_pcg.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, I32, D, C.c_int32, C.c_int32]
+    L.dhip_pcg_noalloc.argtypes = [P, P, C.c_int32, VP, VP, I32, D, C.c_int32]
     L.dhip_gmres.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.dhip_gmres_diag.argtypes = [P, P, VP, VP, VP, C.c_int32, C.c_int32, I32, D, C.c_int32, C.c_int32]
     L.dhip_ch_create.argtypes = [P, C.c_int32, C.POINTER(ChParams), C.POINTER(P), I64]

@@ -228,6 +228,20 @@ int dhip_axpy(dhip_ctx_t *ctx, int64_t n, double a, const double *x, double *y, 
  * refinement loop; *max_iter then counts inner iterations. */
 int dhip_pcg(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
              int32_t where, int32_t *max_iter, double *tol, int32_t rel, int32_t precision);
+
+/* Allocation-free FP64 PCG.  No counterpart in the reference: it serves callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same solver, arguments and results as dhip_pcg with precision DHIP_FP64 and
+ * where = DHIP_DEVICE (b and x are device pointers), but the Jacobi diagonal, the exchanged work
+ * vectors, the synchronisation slots and the result record live in one fixed scratch array of the
+ * library (a static device array, one copy per GPU, held until the process ends; about 56 MiB).
+ * Systems above DHIP_NOALLOC_MAX_ROWS rows are rejected with DHIP_E_CAPACITY.  The call allocates
+ * nothing, on the device or the host.  Every call on one GPU uses that GPU's array (the slots are
+ * cleared by a fill kernel on each call): calls that overlap in time on one GPU, from several
+ * contexts or host threads, race and must be serialised by the caller. */
+#define DHIP_NOALLOC_MAX_ROWS 1048576
+int dhip_pcg_noalloc(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
+                     int32_t *max_iter, double *tol, int32_t rel);
 /* GMRES(A, x, b, ex, M, m, max_iter, tol, rel, calc2norm=false, method),
  * num/krylovsolver.h:320-463 (serial variant: m-1 Arnoldi steps per cycle). */
 int dhip_gmres(dhip_ctx_t *ctx, const dhip_csr_t *A, int32_t pc, const double *b, double *x,
