@@ -77,3 +77,24 @@ def test_gmres_not_converged_and_restart_two(ctx):
     with pytest.raises(D.DhipError):
         ctx.gmres(A, b, restart=1, tol=1e-10)
     A.close()
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_gmres_restart_m_takes_m_minus_1_steps(ctx, left):
+    """As the oracle pin: on a diagonal matrix with k distinct eigenvalues GMRES(k+1) converges in
+    its first cycle after exactly k steps and GMRES(k) needs a second cycle (m-1 Arnoldi steps per
+    cycle, num/krylovsolver.h:320-463)."""
+    for k in (3, 5, 8):
+        print(f"[gmres m-1] k={k} left={left}", flush=True)
+        rng = np.random.default_rng(k)
+        n = 8 * k
+        d = np.repeat(np.arange(1, k + 1, dtype=float), n // k)
+        rng.shuffle(d)
+        b = rng.normal(size=n)
+        A = ctx.csr_create(np.arange(n + 1), np.arange(n), d)
+        x, it, res, ok = ctx.gmres(A, b, pc=D.PC_NONE, restart=k + 1, tol=1e-10, max_iter=500, left_pc=left)
+        assert ok and it == k, (k, it)
+        assert np.abs(x - b / d).max() < 1e-12
+        x, it, res, ok = ctx.gmres(A, b, pc=D.PC_NONE, restart=k, tol=1e-10, max_iter=500, left_pc=left)
+        assert ok and it > k, (k, it)
+        A.close()

@@ -272,6 +272,32 @@ def test_pcg_and_gmres_against_direct_solve():
     assert ok
 
 
+def _diag_k_distinct(k, n, seed):
+    rng = np.random.default_rng(seed)
+    d = np.repeat(np.arange(1, k + 1, dtype=float), n // k)
+    rng.shuffle(d)
+    return np.arange(n + 1, dtype=np.int64), np.arange(n, dtype=np.int32), d, rng.normal(size=n)
+
+
+@pytest.mark.parametrize("left", [False, True])
+def test_gmres_restart_m_takes_m_minus_1_steps(left):
+    """The reference's GMRES(m) runs m-1 Arnoldi steps per cycle (num/krylovsolver.h:320-463).
+    A diagonal matrix with k distinct eigenvalues has a minimal polynomial of degree k, so the
+    exact solution lies in the k-th Krylov space and no earlier one (generic b): with restart
+    k+1 the first cycle reaches it after exactly k steps; with restart k a cycle has only k-1
+    steps and a second cycle is needed.  A GMRES(m) with m steps per cycle fails the second
+    assertion, one with m-2 steps the first."""
+    for k in (3, 5, 8):
+        rp, col, d, b = _diag_k_distinct(k, 8 * k, k)
+        x, it, res, ok = O.gmres_jacobi(rp, col, d, b, restart=k + 1, tol=1e-10, max_iter=500, left_pc=left,
+                                        use_pc=False)
+        assert ok and it == k, (k, it)
+        assert np.abs(x - b / d).max() < 1e-12
+        x, it, res, ok = O.gmres_jacobi(rp, col, d, b, restart=k, tol=1e-10, max_iter=500, left_pc=left,
+                                        use_pc=False)
+        assert ok and it > k, (k, it)
+
+
 def test_pcg_failure_is_reported():
     M = _random_csr(100, 0.05, 7)
     b = np.ones(100)
