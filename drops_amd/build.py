@@ -10,8 +10,6 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 SOURCES = [os.path.join(HERE, "csrc", "dhip.hip")]
-DEPS = [os.path.join(HERE, "csrc", f) for f in ("dhip_common.hpp", "dhip_assembly.hpp", "dhip_krylov.hpp")] + [
-    os.path.join(ROOT, "include", "dhip.h")]
 LIB = os.path.join(HERE, "lib", "libdhip.so")
 
 HIP_FLAGS = [
@@ -45,7 +43,7 @@ def build_host_examples(verbose=False):
     out = {}
     for name, src in HOST_EXAMPLES.items():
         exe = os.path.join(HERE, "lib", name)
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", exe + ".tmp", src, "-L" + os.path.dirname(LIB), "-ldhip",
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-fopenmp", "-o", exe + ".tmp", src, "-L" + os.path.dirname(LIB), "-ldhip",
                "-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd), flush=True)

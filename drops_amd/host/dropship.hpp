@@ -46,6 +46,10 @@
 #include <valarray>
 #include <vector>
 #include <algorithm>
+#include <type_traits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "../../include/dhip.h"
 
@@ -524,13 +528,33 @@ struct BrickBuilderCL {
     }
 };
 
+class TetraCL;
+class ColorClassesCL;
+
+// Kuhn tets of a brick cell (geom/builder.cpp:155-175): corner offsets (d1, d2, d3) of local vertices 0..3
+static const int KUHN_TET[6][4][3] = {{{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {1, 1, 1}},
+                                      {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {1, 1, 1}},
+                                      {{0, 0, 0}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}},
+                                      {{0, 0, 0}, {0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+                                      {{0, 0, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}},
+                                      {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {1, 1, 1}}};
+// P2 edges in the reference's order (geom/topo.cpp:45-46)
+static const int P2_EDGE[6][2] = {{0, 1}, {0, 2}, {1, 2}, {0, 3}, {1, 3}, {2, 3}};
+
 class MultiGridCL {
     BrickBuilderCL brick_;
+    mutable std::shared_ptr<ColorClassesCL> colors_;
 
   public:
     explicit MultiGridCL(const BrickBuilderCL &b) : brick_(b) {}
     const BrickBuilderCL &brick() const { return brick_; }
     size_t GetNumTriangTetra() const { return 6ul * brick_.n[0] * brick_.n[1] * brick_.n[2]; }
+    size_t GetNumVertices() const { return (size_t)(brick_.n[0] + 1) * (brick_.n[1] + 1) * (brick_.n[2] + 1); }
+    // tet t of the triangulation, t = 6 * cell + k in the reference's creation order
+    TetraCL GetTetra(size_t t) const;
+    // the greedy vertex colouring of the triangulation, computed on first use and kept
+    // (MultiGridCL::GetColorClasses, geom/multigrid.cpp:1344-1353)
+    const ColorClassesCL &GetColorClasses(int = 0, const void * = nullptr) const;
     // world coordinates of half-grid point j (vertex for even j, Kuhn edge barycentre for odd j)
     void halfgrid_coord(long j1, long j2, long j3, double x[3]) const {
         auto vc = [&](long i1, long i2, long i3, double *v) {
@@ -547,6 +571,107 @@ class MultiGridCL {
         for (int c = 0; c < 3; ++c) x[c] = 0.5 * (a[c] + b[c]);
     }
 };
+
+// A tetrahedron of the Kuhn triangulation as accumulators see it (the parts of TetraCL,
+// geom/simplex.h, that a visitor on a brick uses): its position in the triangulation, the
+// global numbers and coordinates of its vertices, and the half-grid nodes of its P2 degrees of
+// freedom (vertices 0..3, then the edges in the reference's order) -- the index into a P2 level
+// set of LevelsetP2CL::Phi.
+class TetraCL {
+    const MultiGridCL *mg_ = nullptr;
+    size_t t_ = 0;
+    size_t vid_[4] = {0, 0, 0, 0};
+    long node_[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+  public:
+    TetraCL() {}
+    TetraCL(const MultiGridCL &mg, size_t t) : mg_(&mg), t_(t) {
+        const auto &B = mg.brick();
+        const size_t n1 = B.n[0], n2 = B.n[1];
+        const size_t cell = t / 6, k = t % 6;
+        const long i1 = (long)(cell % n1), i2 = (long)((cell / n1) % n2), i3 = (long)(cell / (n1 * n2));
+        const long h1 = 2 * B.n[0] + 1, h2 = 2 * B.n[1] + 1;
+        long j[4][3];
+        for (int v = 0; v < 4; ++v) {
+            const long a = i1 + KUHN_TET[k][v][0], b = i2 + KUHN_TET[k][v][1], c = i3 + KUHN_TET[k][v][2];
+            vid_[v] = (size_t)a + (n1 + 1) * ((size_t)b + (n2 + 1) * (size_t)c);
+            j[v][0] = 2 * a;
+            j[v][1] = 2 * b;
+            j[v][2] = 2 * c;
+            node_[v] = (j[v][2] * h2 + j[v][1]) * h1 + j[v][0];
+        }
+        for (int e = 0; e < 6; ++e) {
+            const long *p = j[P2_EDGE[e][0]], *q = j[P2_EDGE[e][1]];
+            node_[4 + e] = (((p[2] + q[2]) / 2) * h2 + (p[1] + q[1]) / 2) * h1 + (p[0] + q[0]) / 2;
+        }
+    }
+    size_t GetId() const { return t_; }
+    size_t GetVertexNum(int i) const { return vid_[i]; }
+    long GetP2Node(int d) const { return node_[d]; }
+    void GetVertexCoord(int i, double x[3]) const {
+        const auto &B = mg_->brick();
+        const size_t n1 = B.n[0] + 1, n2 = B.n[1] + 1;
+        const size_t v = vid_[i];
+        mg_->halfgrid_coord(2 * (long)(v % n1), 2 * (long)((v / n1) % n2), 2 * (long)(v / (n1 * n2)), x);
+    }
+    // |det(v1 - v0, v2 - v0, v3 - v0)| / 6
+    double GetVolume() const {
+        double x[4][3];
+        for (int i = 0; i < 4; ++i) GetVertexCoord(i, x[i]);
+        double a[3][3];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) a[r][c] = x[r + 1][c] - x[0][c];
+        const double d = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                         a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                         a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+        return std::fabs(d) / 6.0;
+    }
+};
+
+inline TetraCL MultiGridCL::GetTetra(size_t t) const { return TetraCL(*this, t); }
+
+// ColorClassesCL::compute_color_classes (geom/multigrid.cpp:1238-1319): tets in mesh order, each
+// given the smallest colour that none of its four vertices carries yet; the tets of one colour
+// share no vertex, so one colour's visits may run in parallel.  Each class is sorted (the
+// reference sorts by address, which on a freshly built brick is the creation order).
+class ColorClassesCL {
+    std::vector<std::vector<size_t>> colors_;
+
+  public:
+    typedef std::vector<size_t> ColorClassT;
+    ColorClassesCL() {}
+    explicit ColorClassesCL(const MultiGridCL &mg) { compute_color_classes(mg); }
+    void compute_color_classes(const MultiGridCL &mg) {
+        const size_t nt = mg.GetNumTriangTetra(), nv = mg.GetNumVertices();
+        // a Kuhn tet shares a vertex with at most 70 others, so the greedy colour is below 71 and
+        // two 64-bit words per vertex hold every colour it can carry
+        std::vector<uint64_t> used(2 * nv, 0);
+        colors_.assign(128, ColorClassT());
+        for (size_t t = 0; t < nt; ++t) {
+            const TetraCL T(mg, t);
+            uint64_t u[2] = {0, 0};
+            for (int i = 0; i < 4; ++i)
+                for (int w = 0; w < 2; ++w) u[w] |= used[2 * T.GetVertexNum(i) + w];
+            int c = 0;
+            while (c < 128 && (u[c >> 6] >> (c & 63) & 1u)) ++c;
+            if (c == 128) throw DROPSErrCL("ColorClassesCL: more than 128 colours");
+            colors_[c].push_back(t);
+            for (int i = 0; i < 4; ++i) used[2 * T.GetVertexNum(i) + (c >> 6)] |= uint64_t(1) << (c & 63);
+        }
+        size_t k = 0;
+        while (k < colors_.size() && !colors_[k].empty()) std::sort(colors_[k].begin(), colors_[k].end()), ++k;
+        colors_.resize(k);
+    }
+    size_t num_colors() const { return colors_.size(); }
+    std::vector<ColorClassT>::const_iterator begin() const { return colors_.begin(); }
+    std::vector<ColorClassT>::const_iterator end() const { return colors_.end(); }
+    const ColorClassT &operator[](size_t c) const { return colors_[c]; }
+};
+
+inline const ColorClassesCL &MultiGridCL::GetColorClasses(int, const void *) const {
+    if (!colors_) colors_ = std::make_shared<ColorClassesCL>(*this);
+    return *colors_;
+}
 
 enum FiniteElementT { P1IF_FE = 1, P2IF_FE = 2 };
 
@@ -686,17 +811,128 @@ class MatDescCL {
 };
 
 // ---------------------------------------------------------------------------
-// accumulators (the assembly plug-in interface)
+// accumulators (the assembly plug-in interface, num/accumulator.h:42-268)
 // ---------------------------------------------------------------------------
-class TetraAccumulatorCL {
-  public:
-    virtual ~TetraAccumulatorCL() {}
-};
+class InterfaceCommonDataP1CL;
+class InterfaceCommonDataP2CL;
+class NarrowBandCommonDataP1CL;
 
+namespace detail {
+// What the device-assembled accumulators of a tuple ask of one libdhip pass.
+struct DevicePlan {
+    InterfaceCommonDataP1CL *common = nullptr;
+    InterfaceCommonDataP2CL *common2 = nullptr;
+    const NarrowBandCommonDataP1CL *band = nullptr;
+    struct MatEntry { MatDescCL *mat; int op; double coef; };
+    std::vector<MatEntry> mats;
+    VecDescCL *rhs = nullptr;
+    int rhs_kind = DHIP_RHS_NONE;
+    double rhs_param = 0.;
+    void add_matrix(MatDescCL *m, int op, double coef, const char *space) {
+        for (auto &e : mats)
+            if (e.op == op) throw DROPSErrCL(std::string("accumulate: one accumulator per ") + space + " interface operator");
+        mats.push_back({m, op, coef});
+    }
+};
+inline int thread_num() {
+#ifdef _OPENMP
+    return omp_get_thread_num();
+#else
+    return 0;
+#endif
+}
+inline int max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+}  // namespace detail
+
+// AccumulatorCL<VisitedT> (num/accumulator.h:42-66): begin_accumulation before the first
+// visit, visit exactly once per visited object, finalize_accumulation after the last, clone for
+// each further thread of the colour-class loop.  One addition: device_plan() -- an accumulator
+// that the library assembles on the GPU describes its part of the device pass there and returns
+// true; every other accumulator (the default) is visited on the host.
+template <class VisitedT>
+class AccumulatorCL {
+  public:
+    AccumulatorCL(bool modified = false) : modified_(modified) {}
+    virtual void begin_accumulation() {}
+    virtual void finalize_accumulation() {}
+    virtual void visit(const VisitedT &t) = 0;
+    virtual AccumulatorCL *clone(int clone_id) = 0;
+    virtual ~AccumulatorCL() {}
+    bool &modified() { return modified_; }
+    virtual bool device_plan(detail::DevicePlan &) { return false; }
+
+  protected:
+    bool modified_;
+};
+typedef AccumulatorCL<TetraCL> TetraAccumulatorCL;
+
+namespace detail {
+// The per-tet interface data of one device pass, for host visitors next to the device accumulators:
+// the cut tets (mesh order) with their patch area and non-empty flag.
+struct CutTable {
+    std::vector<int64_t> tets;
+    std::vector<double> area;
+    std::vector<uint8_t> nonempty;
+    long find(size_t t) const {
+        const auto it = std::lower_bound(tets.begin(), tets.end(), (int64_t)t);
+        return it != tets.end() && *it == (int64_t)t ? (long)(it - tets.begin()) : -1;
+    }
+};
+}  // namespace detail
+
+// Interface common data (surfactant/ifacetransp.h:103-156).  The device pass computes the patches
+// of all cut tets; on the host, visit(t) provides what the reference's object holds for t to the
+// accumulators after it in the tuple: the level-set values on the order-2 lattice (ls_loc), the
+// P1 basis (p1), empty() (no interface patch in t) and, as an addition, the patch area of t.
+// get_clone() returns the copy of the calling thread, as the reference's.
 class InterfaceCommonDataP1CL : public TetraAccumulatorCL {
+  protected:
+    std::vector<InterfaceCommonDataP1CL *> *clones_ = nullptr;
+    long cur_ = -1;
+
   public:
     const VecDescCL *ls;
-    InterfaceCommonDataP1CL(const VecDescCL &ls_arg, const BndDataCL &) : ls(&ls_arg) {}
+    std::shared_ptr<const detail::CutTable> cut;   // set by accumulate() after the device pass
+    std::valarray<double> ls_loc;                  // lattice-2 values (PrincipalLatticeCL vertex order)
+    double p1[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+
+    InterfaceCommonDataP1CL(const VecDescCL &ls_arg, const BndDataCL &) : ls(&ls_arg), ls_loc(10) {}
+    bool empty() const { return cur_ < 0 || !cut->nonempty[cur_]; }
+    double surface_area() const { return empty() ? 0. : cut->area[cur_]; }
+    const InterfaceCommonDataP1CL &get_clone() const {
+        const int tid = detail::thread_num();
+        return tid == 0 || !clones_ ? *this : *(*clones_)[tid];
+    }
+    void begin_accumulation() override {
+        delete clones_;
+        clones_ = new std::vector<InterfaceCommonDataP1CL *>((size_t)detail::max_threads(), this);
+    }
+    void finalize_accumulation() override {
+        delete clones_;
+        clones_ = nullptr;
+    }
+    void visit(const TetraCL &t) override {
+        // lattice-2 vertex of P2 dof d (geom/principallattice.cpp:102-106): 0, 4, 7, 9, 1, 2, 5, 3, 6, 8
+        static const int lat_of_p2[10] = {0, 4, 7, 9, 1, 2, 5, 3, 6, 8};
+        for (int d = 0; d < 10; ++d) ls_loc[lat_of_p2[d]] = ls->Data[(size_t)t.GetP2Node(d)];
+        cur_ = cut ? cut->find(t.GetId()) : -1;
+    }
+    TetraAccumulatorCL *clone(int clone_id) override {
+        auto *c = new InterfaceCommonDataP1CL(*this);
+        c->clones_ = nullptr;
+        if (clones_) (*clones_)[(size_t)clone_id] = c;
+        return c;
+    }
+    bool device_plan(detail::DevicePlan &p) override {
+        p.common = this;
+        return true;
+    }
 };
 
 // local P1 interface operators: coefficient carriers for the device assembly
@@ -717,11 +953,21 @@ struct LocalNormalLaplaceBulkP1CL {
     double coef;
     LocalNormalLaplaceBulkP1CL(double D, double = 0., std::nullptr_t = nullptr, double = 0.) : coef(D) {}
 };
-// P2 interface space (surfactant/ifacetransp.h:200-276): the same patch data, P2 basis
-class InterfaceCommonDataP2CL : public TetraAccumulatorCL {
+// P2 interface space (surfactant/ifacetransp.h:200-276): the same patch data, P2 basis.  On the
+// host it offers what the P1 class offers; the patch area is the sum of the tet's P2 mass matrix.
+class InterfaceCommonDataP2CL : public InterfaceCommonDataP1CL {
   public:
-    const VecDescCL *ls;
-    InterfaceCommonDataP2CL(const VecDescCL &ls_arg, const BndDataCL &) : ls(&ls_arg) {}
+    InterfaceCommonDataP2CL(const VecDescCL &ls_arg, const BndDataCL &b) : InterfaceCommonDataP1CL(ls_arg, b) {}
+    TetraAccumulatorCL *clone(int clone_id) override {
+        auto *c = new InterfaceCommonDataP2CL(*this);
+        c->clones_ = nullptr;
+        if (clones_) (*clones_)[(size_t)clone_id] = c;
+        return c;
+    }
+    bool device_plan(detail::DevicePlan &p) override {
+        p.common2 = this;
+        return true;
+    }
 };
 struct LocalMassP2CL {                // surfactant/ifacetransp.h:2379-2403 (LocalAssembler::M_P2P2, :1579)
     static const int device_op = DHIP_OP_MASS;
@@ -741,18 +987,31 @@ struct LocalNormalGradientStabP2CL {
     explicit LocalNormalGradientStabP2CL(double D) : coef(D) {}
 };
 
+// The accumulators assembled by the device pass do their work there; on the host their visits
+// are empty, and clones are plain copies.
+template <class Derived>
+class DeviceAccumulatorCL : public TetraAccumulatorCL {
+  public:
+    void visit(const TetraCL &) override {}
+    TetraAccumulatorCL *clone(int) override { return new Derived(static_cast<const Derived &>(*this)); }
+};
+
 // Narrow band (surfactant/ifacetransp.h:647-760): the tets with distance() <= dist; the matrix
 // accumulator over them carries the normal stabilisation (LocalNormalLaplaceBulkP1CL's narrow-band
 // setup, surfactant/ifacetransp.h:2005-2027).
-class NarrowBandCommonDataP1CL : public TetraAccumulatorCL {
+class NarrowBandCommonDataP1CL : public DeviceAccumulatorCL<NarrowBandCommonDataP1CL> {
   public:
     const VecDescCL *ls;
     double dist;
     NarrowBandCommonDataP1CL(const VecDescCL &ls_arg, const BndDataCL &, const double &dst)
         : ls(&ls_arg), dist(dst) {}
+    bool device_plan(detail::DevicePlan &p) override {
+        p.band = this;
+        return true;
+    }
 };
 template <class LocalMatrixT>
-class NarrowBandMatrixAccuP1CL : public TetraAccumulatorCL {
+class NarrowBandMatrixAccuP1CL : public DeviceAccumulatorCL<NarrowBandMatrixAccuP1CL<LocalMatrixT>> {
   public:
     MatDescCL *mat;
     LocalMatrixT local;
@@ -761,6 +1020,12 @@ class NarrowBandMatrixAccuP1CL : public TetraAccumulatorCL {
     NarrowBandMatrixAccuP1CL(MatDescCL *m, const LocalMatrixT &l, const NarrowBandCommonDataP1CL &b,
                              std::string n = std::string())
         : mat(m), local(l), bdata(&b), name(n) {}
+    bool device_plan(detail::DevicePlan &p) override {
+        static_assert(LocalMatrixT::device_op == DHIP_OP_STAB, "the narrow band carries the normal stabilisation");
+        if (bdata != p.band) throw DROPSErrCL("accumulate: push the band's common data first");
+        p.add_matrix(mat, LocalMatrixT::device_op, local.coef, "P1");
+        return true;
+    }
 };
 
 // load vector of a device-evaluable right-hand side (surfactant/ifacetransp.h:768-798)
@@ -778,7 +1043,7 @@ struct LocalVectorP2CL {
 };
 
 template <class LocalMatrixT, class CommonT>
-class InterfaceMatrixAccuCL : public TetraAccumulatorCL {
+class InterfaceMatrixAccuCL : public DeviceAccumulatorCL<InterfaceMatrixAccuCL<LocalMatrixT, CommonT>> {
   public:
     MatDescCL *mat;
     LocalMatrixT local;
@@ -786,10 +1051,17 @@ class InterfaceMatrixAccuCL : public TetraAccumulatorCL {
     std::string name;
     InterfaceMatrixAccuCL(MatDescCL *m, const LocalMatrixT &l, const CommonT &c, std::string n = std::string())
         : mat(m), local(l), cdata(&c), name(n) {}
+    bool device_plan(detail::DevicePlan &p) override {
+        const bool p2 = std::is_same<CommonT, InterfaceCommonDataP2CL>::value;
+        if ((p2 ? (const void *)p.common2 : (const void *)p.common) != (const void *)cdata)
+            throw DROPSErrCL("accumulate: push the interface common data before its accumulators");
+        p.add_matrix(mat, LocalMatrixT::device_op, local.coef, p2 ? "P2" : "P1");
+        return true;
+    }
 };
 
 template <class LocalVectorT, class CommonT>
-class InterfaceVectorAccuCL : public TetraAccumulatorCL {
+class InterfaceVectorAccuCL : public DeviceAccumulatorCL<InterfaceVectorAccuCL<LocalVectorT, CommonT>> {
   public:
     VecDescCL *vec;
     LocalVectorT local;
@@ -797,105 +1069,174 @@ class InterfaceVectorAccuCL : public TetraAccumulatorCL {
     std::string name;
     InterfaceVectorAccuCL(VecDescCL *v, const LocalVectorT &l, const CommonT &c, std::string n = std::string())
         : vec(v), local(l), cdata(&c), name(n) {}
+    bool device_plan(detail::DevicePlan &p) override {
+        const bool p2 = std::is_same<CommonT, InterfaceCommonDataP2CL>::value;
+        if ((p2 ? (const void *)p.common2 : (const void *)p.common) != (const void *)cdata)
+            throw DROPSErrCL("accumulate: push the interface common data before its accumulators");
+        p.rhs = vec;
+        p.rhs_kind = local.f.kind;
+        p.rhs_param = local.f.param;
+        return true;
+    }
 };
 
+// AccumulatorTupleCL<TetraCL> (num/accumulator.h:80-189): accumulators in registration order;
+// operator()(begin, end) visits tets in the given order, operator()(colors) visits each colour
+// class in parallel over the threads (OpenMP) with one clone of every accumulator per further
+// thread, and merges the modified flags.  Accumulators registered with push_back_acquire are
+// deleted with the tuple.
 class TetraAccumulatorTupleCL {
-  public:
-    const InterfaceCommonDataP1CL *common = nullptr;
-    const InterfaceCommonDataP2CL *common2 = nullptr;
-    struct MatEntry { MatDescCL *mat; int op; double coef; };
-    std::vector<MatEntry> mats;
-    VecDescCL *rhs = nullptr;
-    DeviceFunctionCL rhs_fn;
+    std::vector<TetraAccumulatorCL *> accus_, owned_;
 
-    void push_back(InterfaceCommonDataP1CL *c) { common = c; }
-    template <class L>
-    void push_back(InterfaceMatrixAccuCL<L, InterfaceCommonDataP1CL> *a) {
-        for (auto &m : mats)
-            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P1 interface operator");
-        mats.push_back({a->mat, L::device_op, a->local.coef});
+  public:
+    TetraAccumulatorTupleCL() {}
+    TetraAccumulatorTupleCL(const TetraAccumulatorTupleCL &) = delete;
+    TetraAccumulatorTupleCL &operator=(const TetraAccumulatorTupleCL &) = delete;
+    ~TetraAccumulatorTupleCL() {
+        for (auto *a : owned_) delete a;
     }
-    void push_back(InterfaceVectorAccuCL<LocalVectorP1CL, InterfaceCommonDataP1CL> *a) {
-        rhs = a->vec;
-        rhs_fn = a->local.f;
+    void push_back(TetraAccumulatorCL *p) { accus_.push_back(p); }
+    void push_back_acquire(TetraAccumulatorCL *p) {
+        push_back(p);
+        owned_.push_back(p);
     }
-    const NarrowBandCommonDataP1CL *band = nullptr;
-    void push_back(NarrowBandCommonDataP1CL *b) { band = b; }
-    template <class L>
-    void push_back(NarrowBandMatrixAccuP1CL<L> *a) {
-        static_assert(L::device_op == DHIP_OP_STAB, "the narrow band carries the normal stabilisation");
-        for (auto &m : mats)
-            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P1 interface operator");
-        if (a->bdata != band) throw DROPSErrCL("accumulate: push the band's common data first");
-        mats.push_back({a->mat, L::device_op, a->local.coef});
+    const std::vector<TetraAccumulatorCL *> &accumulators() const { return accus_; }
+
+    void operator()(const MultiGridCL &mg, size_t begin, size_t end) {
+        for (auto *a : accus_) a->begin_accumulation();
+        for (size_t t = begin; t < end; ++t) {
+            const TetraCL T(mg, t);
+            for (auto *a : accus_) a->visit(T);
+        }
+        for (auto *a : accus_) a->finalize_accumulation();
     }
-    void push_back(InterfaceCommonDataP2CL *c) { common2 = c; }
-    template <class L>
-    void push_back(InterfaceMatrixAccuCL<L, InterfaceCommonDataP2CL> *a) {
-        for (auto &m : mats)
-            if (m.op == L::device_op) throw DROPSErrCL("accumulate: one accumulator per P2 interface operator");
-        mats.push_back({a->mat, L::device_op, a->local.coef});
-    }
-    void push_back(InterfaceVectorAccuCL<LocalVectorP2CL, InterfaceCommonDataP2CL> *a) {
-        rhs = a->vec;
-        rhs_fn = a->local.f;
+    void operator()(const MultiGridCL &mg, const ColorClassesCL &colors) {
+        for (auto *a : accus_) a->begin_accumulation();
+        const int nth = detail::max_threads();
+        std::vector<std::vector<TetraAccumulatorCL *>> clones((size_t)nth);
+        clones[0] = accus_;
+        for (int i = 1; i < nth; ++i)
+            for (auto *a : accus_) clones[(size_t)i].push_back(a->clone(i));
+        for (const auto &color : colors) {
+            const long nc = (long)color.size();
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+            {
+                auto &mine = clones[(size_t)detail::thread_num()];
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic)
+#endif
+                for (long j = 0; j < nc; ++j) {
+                    const TetraCL T(mg, color[(size_t)j]);
+                    for (auto *a : mine) a->visit(T);
+                }
+            }
+        }
+        for (size_t i = 0; i < accus_.size(); ++i)
+            for (int j = 1; j < nth; ++j)
+                if ((accus_[i]->modified() = accus_[i]->modified() || clones[(size_t)j][i]->modified())) break;
+        for (int j = 1; j < nth; ++j)
+            for (auto *a : clones[(size_t)j]) delete a;
+        for (auto *a : accus_) a->finalize_accumulation();
     }
 };
 
-// accumulate(tuple, mg, level, bnd) (num/accumulator.h:265-268): one device pass
-// computes the patches and local matrices of all cut tets and scatters every
-// operator of the tuple; results are copied into the caller's MatDescCL/VecDescCL.
+// accumulate(tuple, mg, level, bnd) (num/accumulator.h:215-268).  The accumulators that the
+// library assembles (interface common data with its matrix and load accumulators, P1 or P2, or
+// the narrow band) run as one device pass, whose matrices and vector are written into the
+// caller's MatDescCL / VecDescCL.  Any other accumulator in the tuple is visited on the host over
+// all tets of the triangulation, with the tuple's interface common data presenting each tet's
+// patch to it: in mesh order with one thread, or over the colour classes with OpenMP threads, as
+// the reference's accumulate does.
 inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsigned = 0, const BndDataCL * = 0) {
-    if (!t.common && !t.common2) throw DROPSErrCL("accumulate: no interface common data in the tuple");
-    if (t.common && t.common2) throw DROPSErrCL("accumulate: P1 and P2 interface accumulators in one tuple");
-    const bool p2 = t.common2 != nullptr;
-    DeviceCL &dev = DeviceCL::instance();
-    const auto &B = mg.brick();
-    const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
-                         B.e[2][0], B.e[2][1], B.e[2][2]};
-    const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
-    double coef[3] = {1., 1., 1.};
-    for (auto &m : t.mats) coef[m.op] = m.coef;
-    const VectorCL &phi = p2 ? t.common2->ls->Data : t.common->ls->Data;
-    dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
-    dev.check(dhip_levelset_p2(dev.ctx(), phi.raw(), (int64_t)phi.size(), DHIP_HOST));
-    int64_t ncut = 0, nd = 0, nnz = 0;
-    const double param = t.rhs_fn.param;
-    const double lc[3] = {1., 1., 1.};
-    if (t.band && p2) throw DROPSErrCL("accumulate: the narrow band is a P1 assembly");
-    if (p2)
-        dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
-                                   -1.0, lc, &ncut, &nd, &nnz));
-    else if (t.band)
-        // SurfactantNarrowBandStblP1CL::Update's tuple: the surface forms on the cut tets, the
-        // stabilisation on the band; every matrix comes back on the band pattern (mass and LB hold
-        // 0.0 where only band tets couple two DOFs)
-        dev.check(dhip_assemble_p1_band(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE,
-                                        &param, t.band->dist, lc, &ncut, &nd, &nnz));
-    else
-        dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, t.rhs ? t.rhs_fn.kind : DHIP_RHS_NONE, &param,
-                                   -1.0, lc, &ncut, &nd, &nnz));
-    std::vector<int32_t> rp(nd + 1), ci(nnz);
-    std::vector<double> v[3], rhs(nd);
-    for (auto &x : v) x.resize(nnz);
-    dev.check(dhip_get_csr(dev.ctx(), rp.data(), ci.data(), v[0].data(), v[1].data(), v[2].data(), nullptr,
-                           rhs.data()));
-    for (auto &m : t.mats) {
-        if (m.mat->RowIdx && m.mat->RowIdx->NumUnknowns() != (size_t)nd)
-            throw DROPSErrCL("accumulate: numbering of the matrix does not match the level set");
-        MatrixCL &A = m.mat->Data;
-        A.resize(nd, nd, nnz);
-        for (int64_t i = 0; i <= nd; ++i) A.raw_row()[i] = (size_t)rp[i];
-        for (int64_t k = 0; k < nnz; ++k) {
-            A.raw_col()[k] = (size_t)ci[k];
-            A.raw_val()[k] = v[m.op][k];
+    detail::DevicePlan plan;
+    bool host_visitors = false;
+    for (auto *a : t.accumulators())
+        if (!a->device_plan(plan)) host_visitors = true;
+    if (plan.common && plan.common2) throw DROPSErrCL("accumulate: P1 and P2 interface accumulators in one tuple");
+    if (plan.band && plan.common2) throw DROPSErrCL("accumulate: the narrow band is a P1 assembly");
+    const bool device = plan.common || plan.common2 || plan.band;
+    if (!device && !plan.mats.empty()) throw DROPSErrCL("accumulate: no interface common data in the tuple");
+    if (device) {
+        const bool p2 = plan.common2 != nullptr;
+        DeviceCL &dev = DeviceCL::instance();
+        const auto &B = mg.brick();
+        const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                             B.e[2][0], B.e[2][1], B.e[2][2]};
+        const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+        double coef[3] = {1., 1., 1.};
+        for (auto &m : plan.mats) coef[m.op] = m.coef;
+        const VecDescCL *lsd = p2 ? plan.common2->ls : plan.common ? plan.common->ls : plan.band->ls;
+        const VectorCL &phi = lsd->Data;
+        dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+        dev.check(dhip_levelset_p2(dev.ctx(), phi.raw(), (int64_t)phi.size(), DHIP_HOST));
+        int64_t ncut = 0, nd = 0, nnz = 0;
+        const double param = plan.rhs_param;
+        const int rk = plan.rhs ? plan.rhs_kind : DHIP_RHS_NONE;
+        const double lc[3] = {1., 1., 1.};
+        if (p2)
+            dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, rk, &param, -1.0, lc, &ncut, &nd, &nnz));
+        else if (plan.band)
+            // SurfactantNarrowBandStblP1CL::Update's tuple: the surface forms on the cut tets, the
+            // stabilisation on the band; every matrix comes back on the band pattern (mass and LB hold
+            // 0.0 where only band tets couple two DOFs)
+            dev.check(dhip_assemble_p1_band(dev.ctx(), DHIP_ZERO_THROW, coef, rk, &param, plan.band->dist, lc, &ncut,
+                                            &nd, &nnz));
+        else
+            dev.check(dhip_assemble_p1(dev.ctx(), DHIP_ZERO_THROW, coef, rk, &param, -1.0, lc, &ncut, &nd, &nnz));
+        std::vector<int32_t> rp(nd + 1), ci(nnz);
+        std::vector<double> v[3], rhs(nd);
+        for (auto &x : v) x.resize(nnz);
+        dev.check(dhip_get_csr(dev.ctx(), rp.data(), ci.data(), v[0].data(), v[1].data(), v[2].data(), nullptr,
+                               rhs.data()));
+        for (auto &m : plan.mats) {
+            if (m.mat->RowIdx && m.mat->RowIdx->NumUnknowns() != (size_t)nd)
+                throw DROPSErrCL("accumulate: numbering of the matrix does not match the level set");
+            MatrixCL &A = m.mat->Data;
+            A.resize(nd, nd, nnz);
+            for (int64_t i = 0; i <= nd; ++i) A.raw_row()[i] = (size_t)rp[i];
+            for (int64_t k = 0; k < nnz; ++k) {
+                A.raw_col()[k] = (size_t)ci[k];
+                A.raw_val()[k] = v[m.op][k];
+            }
+        }
+        if (plan.rhs) {
+            // InterfaceVectorAccuCL adds into the vector (surfactant/ifacetransp.h:635-637)
+            if (plan.rhs->Data.size() != (size_t)nd) plan.rhs->Data.resize(nd, 0.0);
+            for (int64_t i = 0; i < nd; ++i) plan.rhs->Data[i] += rhs[i];
+        }
+        InterfaceCommonDataP1CL *cd = p2 ? plan.common2 : plan.common;
+        if (host_visitors && cd && !plan.band) {
+            auto tab = std::make_shared<detail::CutTable>();
+            tab->tets.resize((size_t)ncut);
+            tab->area.resize((size_t)ncut);
+            tab->nonempty.resize((size_t)ncut);
+            if (ncut) {
+                dev.check(dhip_get_cut_tets(dev.ctx(), tab->tets.data()));
+                if (p2) {
+                    std::vector<double> loc((size_t)ncut * 320);
+                    dev.check(dhip_get_local_p2(dev.ctx(), loc.data(), tab->nonempty.data()));
+                    for (int64_t c = 0; c < ncut; ++c) {  // 1^T M 1 of the tet's P2 mass matrix
+                        double a = 0.;
+                        for (int k = 0; k < 100; ++k) a += loc[(size_t)c * 320 + k];
+                        tab->area[(size_t)c] = a;
+                    }
+                } else {
+                    std::vector<double> loc((size_t)ncut * 57);
+                    dev.check(dhip_get_local(dev.ctx(), loc.data(), tab->nonempty.data()));
+                    for (int64_t c = 0; c < ncut; ++c) tab->area[(size_t)c] = loc[(size_t)c * 57 + 56];
+                }
+                for (auto &f : tab->nonempty) f &= 1u;
+            }
+            cd->cut = tab;
         }
     }
-    if (t.rhs) {
-        // InterfaceVectorAccuCL adds into the vector (surfactant/ifacetransp.h:635-637)
-        if (t.rhs->Data.size() != (size_t)nd) t.rhs->Data.resize(nd, 0.0);
-        for (int64_t i = 0; i < nd; ++i) t.rhs->Data[i] += rhs[i];
-    }
+    if (!host_visitors) return;
+    if (detail::max_threads() > 1)
+        t(mg, mg.GetColorClasses());
+    else
+        t(mg, 0, mg.GetNumTriangTetra());
 }
 
 }  // namespace DROPS

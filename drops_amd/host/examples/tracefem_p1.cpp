@@ -8,6 +8,9 @@
 //   tracefem_p1 band <n> <width> <outdir>   -> the narrow-band tuple of SurfactantNarrowBandStblP1CL::Update
 //   tracefem_p1 p2 <n> <outdir>             -> the same sphere on the P2 interface space (IdxDescCL(P2IF_FE),
 //                                              P2 accumulators), writes CSR/rhs/solution
+//   tracefem_p1 accus <n> <p1|p2>           -> user-written accumulators (cut-tet count, patch area, visit order)
+//                                              in one tuple with the device-assembled interface accumulators
+//   tracefem_p1 colors <n>                  -> ColorClassesCL of the n^3 brick, checked (host only)
 #include <cstdio>
 #include <fstream>
 #include <cstdlib>
@@ -223,14 +226,160 @@ static int p2(int n, const std::string &out) {
     return 0;
 }
 
+// A user accumulator as a DROPS application writes one (num/accumulator.h:42-66): it counts the
+// visited tets, the tets with an interface patch and sums the patch areas through the tuple's
+// interface common data, and records the visit order and the begin/finalize calls.
+template <class CommonT>
+class SurfaceStatsAccuCL : public TetraAccumulatorCL {
+    const CommonT *cdata_;
+    SurfaceStatsAccuCL *root_;
+    int tid_;
+
+  public:
+    size_t visited = 0, cut = 0, begins = 0, finals = 0, clones = 0;
+    bool mesh_order = true;
+    size_t last = 0;
+    double area = 0.;
+    std::vector<size_t> per_thread_visits;
+    std::vector<double> per_thread_area;
+    std::vector<size_t> per_thread_cut;
+    std::vector<uint8_t> seen;
+
+    explicit SurfaceStatsAccuCL(const CommonT &c) : cdata_(&c), root_(this), tid_(0) {}
+    void begin_accumulation() override {
+        ++begins;
+        const int nth = detail::max_threads();
+        per_thread_visits.assign((size_t)nth, 0);
+        per_thread_area.assign((size_t)nth, 0.);
+        per_thread_cut.assign((size_t)nth, 0);
+    }
+    void visit(const TetraCL &t) override {
+        SurfaceStatsAccuCL &R = *root_;
+        if (R.seen.empty()) return;
+        const auto &cd = cdata_->get_clone();
+        ++R.per_thread_visits[(size_t)tid_];
+        ++R.seen[t.GetId()];
+        if (tid_ == 0 && R.clones == 0) {
+            if (R.per_thread_visits[0] > 1 && t.GetId() <= R.last) R.mesh_order = false;
+            R.last = t.GetId();
+        }
+        if (!cd.empty()) {
+            ++R.per_thread_cut[(size_t)tid_];
+            R.per_thread_area[(size_t)tid_] += cd.surface_area();
+        }
+    }
+    TetraAccumulatorCL *clone(int clone_id) override {
+        auto *c = new SurfaceStatsAccuCL(*cdata_);
+        c->root_ = this;
+        c->tid_ = clone_id;
+        ++clones;
+        return c;
+    }
+    void finalize_accumulation() override {
+        ++finals;
+        for (size_t i = 0; i < per_thread_visits.size(); ++i) {
+            visited += per_thread_visits[i];
+            cut += per_thread_cut[i];
+            area += per_thread_area[i];
+        }
+    }
+    void size_for(const MultiGridCL &mg) { seen.assign(mg.GetNumTriangTetra(), 0); }
+};
+
+template <class CommonT, class LocalMassT, class LocalLBT>
+static int accus_case(int n, FiniteElementT fe) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL lsbnd;
+    IdxDescCL ifaceidx(fe);
+    ifaceidx.CreateNumbering(0, mg, &lset.Phi, &lsbnd);
+    MatDescCL M(&ifaceidx, &ifaceidx), A(&ifaceidx, &ifaceidx);
+    TetraAccumulatorTupleCL accus;
+    CommonT cdata(lset.Phi, lsbnd);
+    accus.push_back(&cdata);
+    InterfaceMatrixAccuCL<LocalMassT, CommonT> mass(&M, LocalMassT(1.), cdata, "mass");
+    accus.push_back(&mass);
+    auto *stats = new SurfaceStatsAccuCL<CommonT>(cdata);
+    stats->size_for(mg);
+    accus.push_back_acquire(stats);
+    InterfaceMatrixAccuCL<LocalLBT, CommonT> lb(&A, LocalLBT(1.), cdata, "LB");
+    accus.push_back(&lb);
+    accumulate(accus, mg, 0, &lsbnd);
+    size_t once = 0;
+    for (uint8_t v : stats->seen) once += v == 1;
+    double msum = 0.;
+    for (size_t k = 0; k < M.Data.num_nonzeros(); ++k) msum += M.Data.raw_val()[k];
+    std::printf("threads: %d\ncolors: %zu\ntets: %zu\nvisited: %zu\nvisited once: %zu\ncut: %zu\n"
+                "begin: %zu\nfinalize: %zu\nclones: %zu\nmesh order: %d\narea: %.17g\nmass sum: %.17g\n"
+                "unknowns: %zu\nnnz: %zu\n",
+                detail::max_threads(), detail::max_threads() > 1 ? mg.GetColorClasses().num_colors() : (size_t)1,
+                mg.GetNumTriangTetra(), stats->visited, once, stats->cut, stats->begins, stats->finals,
+                stats->clones, stats->mesh_order ? 1 : 0, stats->area, msum, ifaceidx.NumUnknowns(),
+                M.Data.num_nonzeros());
+    return 0;
+}
+
+// ColorClassesCL on the n^3 brick: every tet in exactly one class, no two tets of a class share a
+// vertex, classes sorted, the greedy choice (smallest colour free at the tet's vertices when it is
+// reached in mesh order) reproduced by a second pass over the classes.
+static int colors(int n) {
+    const double o[3] = {0, 0, 0}, e1[3] = {1, 0, 0}, e2[3] = {0, 1, 0}, e3[3] = {0, 0, 1};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    const ColorClassesCL &cc = mg.GetColorClasses();
+    const size_t nt = mg.GetNumTriangTetra();
+    std::vector<int> color_of(nt, -1);
+    bool ok = true;
+    for (size_t c = 0; c < cc.num_colors(); ++c) {
+        const auto &cls = cc[c];
+        ok &= std::is_sorted(cls.begin(), cls.end()) && !cls.empty();
+        std::vector<uint8_t> vtx(mg.GetNumVertices(), 0);
+        for (size_t t : cls) {
+            ok &= color_of[t] == -1;
+            color_of[t] = (int)c;
+            const TetraCL T = mg.GetTetra(t);
+            for (int i = 0; i < 4; ++i) ok &= vtx[T.GetVertexNum(i)]++ == 0;
+        }
+    }
+    std::vector<std::vector<int>> at_vertex(mg.GetNumVertices());
+    for (size_t t = 0; t < nt && ok; ++t) {
+        ok &= color_of[t] >= 0;
+        const TetraCL T = mg.GetTetra(t);
+        std::vector<int> used;
+        for (int i = 0; i < 4; ++i)
+            for (int c : at_vertex[T.GetVertexNum(i)]) used.push_back(c);
+        int c = 0;
+        while (std::find(used.begin(), used.end(), c) != used.end()) ++c;
+        ok &= c == color_of[t];
+        for (int i = 0; i < 4; ++i) at_vertex[T.GetVertexNum(i)].push_back(color_of[t]);
+    }
+    std::printf("colors: %zu\ntets: %zu\nvalid: %d\nsizes:", cc.num_colors(), nt, ok ? 1 : 0);
+    for (size_t c = 0; c < cc.num_colors(); ++c) std::printf(" %zu", cc[c].size());
+    std::printf("\n");
+    return ok ? 0 : 1;
+}
+
 int main(int argc, char **argv) {
     try {
         if (argc >= 2 && !std::strcmp(argv[1], "sphere20")) return sphere20();
         if (argc >= 3 && !std::strcmp(argv[1], "c1")) return c1(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
         if (argc >= 3 && !std::strcmp(argv[1], "p2")) return p2(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
+        if (argc >= 4 && !std::strcmp(argv[1], "accus")) {
+            if (!std::strcmp(argv[3], "p2"))
+                return accus_case<InterfaceCommonDataP2CL, LocalMassP2CL, LocalLaplaceBeltramiP2CL>(std::atoi(argv[2]),
+                                                                                                    P2IF_FE);
+            return accus_case<InterfaceCommonDataP1CL, LocalInterfaceMassP1CL, LocalLaplaceBeltramiP1CL>(
+                std::atoi(argv[2]), P1IF_FE);
+        }
+        if (argc >= 3 && !std::strcmp(argv[1], "colors")) return colors(std::atoi(argv[2]));
         if (argc >= 4 && !std::strcmp(argv[1], "band"))
             return band(std::atoi(argv[2]), std::atof(argv[3]), argc >= 5 ? argv[4] : "");
-        std::fprintf(stderr, "usage: %s sphere20 | c1 <n> [outdir] | p2 <n> [outdir] | band <n> <width> [outdir]\n",
+        std::fprintf(stderr,
+                     "usage: %s sphere20 | c1 <n> [outdir] | p2 <n> [outdir] | band <n> <width> [outdir] | "
+                     "accus <n> <p1|p2> | colors <n>\n",
                      argv[0]);
         return 2;
     } catch (const std::exception &e) {

@@ -111,3 +111,43 @@ def test_band_through_host_mirror(tmp_path):
     assert np.array_equal(ld("mass", np.float64), m_on) and np.array_equal(ld("lb", np.float64), l_on)
     comb = np.where(hit, 1.0 * (m_on + l_on) + mesh.h * vs, mesh.h * vs)
     assert np.array_equal(ld("comb", np.float64), comb)
+
+
+@pytest.mark.parametrize("space", ["p1", "p2"])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_user_accumulator_next_to_device_accumulators(space, threads):
+    """A user-written TetraAccumulatorCL (num/accumulator.h:42-66) in one tuple with the interface
+    common data and two device-assembled matrix accumulators: accumulate() runs the device pass and
+    visits every tet of the triangulation on the host exactly once -- in mesh order with one thread,
+    over the colour classes with cloned accumulators with four -- with begin/finalize called once.
+    Through the common data the user accumulator sees exactly the cut tets with a patch, and their
+    patch areas sum to 1^T M 1 of the device mass matrix (the basis is a partition of unity)."""
+    n = 16
+    print(f"[host mirror] accus {n} {space} threads={threads}", flush=True)
+    r = subprocess.run([EXE, "accus", str(n), space], capture_output=True, text=True, timeout=120,
+                       env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+    assert r.returncode == 0, r.stderr
+    v = dict(re.findall(r"^([a-z ]+): (\S+)$", r.stdout, flags=re.M))
+    tets = 6 * n ** 3
+    assert int(v["tets"]) == tets and int(v["visited"]) == tets and int(v["visited once"]) == tets
+    assert int(v["begin"]) == 1 and int(v["finalize"]) == 1
+    assert int(v["threads"]) == threads
+    if threads == 1:
+        assert int(v["mesh order"]) == 1 and int(v["clones"]) == 0
+    else:
+        assert int(v["clones"]) == threads - 1 and int(v["colors"]) >= 27
+    mesh, phi = cases.c1_sphere(n)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    cut = O.classify(om, phi)
+    # no exact zeros (tie-break shift): every cut tet has a sign change on a lattice edge, so a patch
+    assert int(v["cut"]) == cut.size
+    if space == "p1":
+        dof, nd = O.number_p1if(om, phi)
+        ref = O.assemble_p1(om, phi, dof, nd)
+    else:
+        dof, nd = O.number_p2if(om, phi)
+        ref = O.assemble_p2(om, phi, dof, nd)
+    assert int(v["unknowns"]) == nd and int(v["nnz"]) == ref["colind"].size
+    area, msum = float(v["area"]), float(v["mass sum"])
+    assert abs(msum - ref["mass"].sum()) <= 1e-13 * abs(msum)
+    assert abs(area - msum) <= 1e-12 * abs(msum)
