@@ -71,6 +71,9 @@ def lib(kind="parity"):
     L.orc_assemble_p2.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64,
                                   I64, I32, D, D, D, D]
     L.orc_surface_area.argtypes = [M, D, C.c_int, D]
+    L.orc_color_classes.argtypes = [M, I64, I64, I64]
+    L.orc_assemble_p1_colored.argtypes = [M, D, I32, C.c_int64, D, C.c_int, D, C.c_int, C.c_int64, I64, I64,
+                                          C.c_int64, I64, I64, I32, D, D, D, D]
     L.orc_tet_patch.argtypes = [M, D, C.c_int64, C.c_int, C.POINTER(Patch)]
     L.orc_spmv.argtypes = [C.c_int64, I64, I32, D, D, D]
     L.orc_kahan_dot.argtypes = [C.c_int64, D, D]
@@ -203,6 +206,43 @@ def assemble_p1(mesh, phi, dof_of_vertex, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=R
                              C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
                              _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(vs, C.c_double),
                              _ptr(rhs, C.c_double)), "assemble_p1")
+    k = nnz.value
+    return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
+                stab=vs[:k].copy(), rhs=rhs[:ndof].copy())
+
+
+def color_classes(mesh, kind="parity"):
+    """ColorClassesCL of the brick: (class_start[ncolors + 1], class_tets[6 n^3])"""
+    nt = 6 * mesh.n[0] * mesh.n[1] * mesh.n[2]
+    start = np.zeros(129, np.int64)
+    tets = np.zeros(max(nt, 1), np.int64)
+    nc = C.c_int64()
+    _check(lib(kind).orc_color_classes(C.byref(mesh), C.byref(nc), _ptr(start, C.c_int64), _ptr(tets, C.c_int64)),
+           "color_classes")
+    return start[:nc.value + 1].copy(), tets[:nt]
+
+
+def assemble_p1_colored(mesh, phi, dof_of_vertex, ndof, colors, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE,
+                        rhs_param=None, zero_policy=ZERO_THROW, kind="omp", cap=None):
+    """The colour-parallel accumulate() (OpenMP build by default): the same matrices as
+    assemble_p1 up to the order in which each entry's contributions are summed."""
+    phi = np.ascontiguousarray(phi, np.float64)
+    co = np.asarray(coef, np.float64)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    start, tets = colors
+    if cap is None:
+        cap = 16 * max(ndof, 1)
+    rowptr = np.zeros(ndof + 1, np.int64)
+    col = np.zeros(cap, np.int32)
+    vm, vl, vs = (np.zeros(cap) for _ in range(3))
+    rhs = np.zeros(max(ndof, 1))
+    nnz = C.c_int64()
+    _check(lib(kind).orc_assemble_p1_colored(C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of_vertex, C.c_int32),
+                                             ndof, _ptr(co, C.c_double), rhs_kind, _ptr(rp, C.c_double), zero_policy,
+                                             start.size - 1, _ptr(start, C.c_int64), _ptr(tets, C.c_int64), cap,
+                                             C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
+                                             _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(vs, C.c_double),
+                                             _ptr(rhs, C.c_double)), "assemble_p1_colored")
     k = nnz.value
     return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
                 stab=vs[:k].copy(), rhs=rhs[:ndof].copy())

@@ -843,6 +843,36 @@ static int builder_add(brow_t *r, int32_t col, const double val[3])
     return ORC_OK;
 }
 
+/* Build(), num/spmat.h:621-677: row sizes, prefix sum, per-row sort by column */
+static int build_rows(const brow_t *rows, int64_t ndof, int64_t cap_nnz, int64_t *nnz, int64_t *rowptr,
+                      int32_t *colind, double *val_mass, double *val_lb, double *val_stab)
+{
+    rowptr[0] = 0;
+    for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
+    *nnz = rowptr[ndof];
+    if (*nnz > cap_nnz) return ORC_E_CAPACITY;
+    if (!colind) return ORC_OK;
+    for (int64_t r = 0; r < ndof; ++r) {
+        const brow_t *R = &rows[r];
+        int32_t ord[ROWCAP];
+        for (int k = 0; k < R->n; ++k) ord[k] = k;
+        for (int a = 1; a < R->n; ++a) {
+            int32_t x = ord[a];
+            int b = a - 1;
+            while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
+            ord[b + 1] = x;
+        }
+        for (int k = 0; k < R->n; ++k) {
+            const int64_t dst = rowptr[r] + k;
+            colind[dst] = R->col[ord[k]];
+            if (val_mass) val_mass[dst] = R->v[0][ord[k]];
+            if (val_lb) val_lb[dst] = R->v[1][ord[k]];
+            if (val_stab) val_stab[dst] = R->v[2][ord[k]];
+        }
+    }
+    return ORC_OK;
+}
+
 typedef struct {
     int64_t t;
     int err;
@@ -907,35 +937,130 @@ int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
             for (int i = 0; i < 4; ++i)
                 if (numb[i] >= 0) rhs[numb[i]] += L->rhs[i];
     }
-    /* Build(): row sizes, prefix sum, per-row sort by column */
-    if (!err) {
-        rowptr[0] = 0;
-        for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
-        *nnz = rowptr[ndof];
-        if (*nnz > cap_nnz) err = ORC_E_CAPACITY;
-    }
-    if (!err && colind) {
-        for (int64_t r = 0; r < ndof; ++r) {
-            brow_t *R = &rows[r];
-            int32_t ord[ROWCAP];
-            for (int k = 0; k < R->n; ++k) ord[k] = k;
-            for (int a = 1; a < R->n; ++a) {
-                int32_t x = ord[a];
-                int b = a - 1;
-                while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
-                ord[b + 1] = x;
-            }
-            for (int k = 0; k < R->n; ++k) {
-                const int64_t dst = rowptr[r] + k;
-                colind[dst] = R->col[ord[k]];
-                if (val_mass) val_mass[dst] = R->v[0][ord[k]];
-                if (val_lb) val_lb[dst] = R->v[1][ord[k]];
-                if (val_stab) val_stab[dst] = R->v[2][ord[k]];
-            }
-        }
-    }
+    if (!err) err = build_rows(rows, ndof, cap_nnz, nnz, rowptr, colind, val_mass, val_lb, val_stab);
     free(cut);
     free(rec);
+    free(rows);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Colour classes and the colour-parallel accumulate() of the OpenMP build    */
+/* ------------------------------------------------------------------------ */
+
+/* ColorClassesCL::compute_color_classes, geom/multigrid.cpp:1238-1319: tets in mesh order, each
+ * given the smallest colour not yet carried by one of its four vertices (a Kuhn tet shares a
+ * vertex with at most 70 others: two 64-bit words per vertex).  Classes are sorted by tet (the
+ * reference sorts by address, the creation order of a fresh brick).  class_start[c] .. [c+1]
+ * index class_tets; returns the number of colours, at most 128. */
+int orc_color_classes(const orc_mesh_t *m, int64_t *ncolors, int64_t *class_start, int64_t *class_tets)
+{
+    init_tables();
+    if (!check_mesh(m) || !ncolors || !class_start || !class_tets) return ORC_E_ARG;
+    const int64_t n1 = m->n[0], n2 = m->n[1], n3 = m->n[2];
+    const int64_t nt = 6 * n1 * n2 * n3, nv = (n1 + 1) * (n2 + 1) * (n3 + 1);
+    uint64_t *used = (uint64_t *)calloc((size_t)(2 * nv), sizeof(uint64_t));
+    uint8_t *color = (uint8_t *)malloc((size_t)(nt > 0 ? nt : 1));
+    if (!used || !color) {
+        free(used);
+        free(color);
+        return ORC_E_NOMEM;
+    }
+    int64_t count[129] = {0};
+    int err = ORC_OK;
+    for (int64_t t = 0; t < nt && !err; ++t) {
+        const int64_t cell = t / 6, k = t % 6;
+        const int64_t i1 = cell % n1, i2 = (cell / n1) % n2, i3 = cell / (n1 * n2);
+        int64_t vid[4];
+        uint64_t u[2] = {0, 0};
+        for (int q = 0; q < 4; ++q) {
+            const int *d = KUHN[k][q];
+            vid[q] = ((i3 + d[2]) * (n2 + 1) + i2 + d[1]) * (n1 + 1) + i1 + d[0];
+            u[0] |= used[2 * vid[q]];
+            u[1] |= used[2 * vid[q] + 1];
+        }
+        int c = 0;
+        while (c < 128 && (u[c >> 6] >> (c & 63) & 1u)) ++c;
+        if (c == 128) { err = ORC_E_CAPACITY; break; }
+        color[t] = (uint8_t)c;
+        ++count[c];
+        for (int q = 0; q < 4; ++q) used[2 * vid[q] + (c >> 6)] |= (uint64_t)1 << (c & 63);
+    }
+    int64_t nc = 0;
+    while (nc < 128 && count[nc] > 0) ++nc;
+    if (!err) {
+        class_start[0] = 0;
+        for (int64_t c = 0; c < nc; ++c) class_start[c + 1] = class_start[c] + count[c];
+        int64_t fill[128];
+        for (int64_t c = 0; c < nc; ++c) fill[c] = class_start[c];
+        for (int64_t t = 0; t < nt; ++t) class_tets[fill[color[t]]++] = t;  /* ascending within a class */
+        *ncolors = nc;
+    }
+    free(used);
+    free(color);
+    return err;
+}
+
+/* accumulate() with colour classes (num/accumulator.h:161-188, 215-228): for each colour, the
+ * threads visit its tets (schedule(dynamic)), each visit being InterfaceCommonDataP1CL::visit
+ * (lattice values, equalSigns, patch) followed by the matrix and vector accumulators'
+ * local setup and update_global_matrix into the shared builders.  Tets of one colour share no
+ * vertex, so no builder row is written by two threads; entries are summed in colour order (the
+ * reference's OpenMP rounding, not its serial mesh order). */
+int orc_assemble_p1_colored(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                            const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                            int64_t ncolors, const int64_t *class_start, const int64_t *class_tets,
+                            int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                            double *val_mass, double *val_lb, double *val_stab, double *rhs)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !dof_of_vertex || !nnz || !rowptr || !coef || !class_start || !class_tets)
+        return ORC_E_ARG;
+    brow_t *rows = (brow_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow_t));
+    if (!rows) return ORC_E_NOMEM;
+    if (rhs)
+        for (int64_t i = 0; i < ndof; ++i) rhs[i] = 0.;
+    int err = ORC_OK;
+    for (int64_t c = 0; c < ncolors; ++c) {
+        const int64_t b0 = class_start[c], b1 = class_start[c + 1];
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int64_t j = b0; j < b1; ++j) {
+            int e = ORC_OK;
+            tet_t T;
+            load_tet_ls(m, phi, class_tets[j], &T);
+            if (equal_signs(T.ls, 10)) continue;
+            if (zero_policy == ORC_ZERO_THROW && has_zero(T.ls, 10)) e = ORC_E_ZERO_NODE;
+            patch_t p;
+            if (!e) {
+                load_tet(m, phi, class_tets[j], &T);
+                e = make_patch(T.ls, &p);
+            }
+            if (!e && p.nf > 0) {
+                orc_local_t L;
+                patch_geometry(T.X, &p);
+                local_quantities(&T, &p, coef, rhs_kind, rhs_param, &L);
+                int32_t numb[4];
+                for (int i = 0; i < 4; ++i) numb[i] = dof_of_vertex[T.vid[i]];
+                for (int i = 0; i < 4 && !e; ++i) {
+                    if (numb[i] < 0) continue;
+                    for (int jj = 0; jj < 4 && !e; ++jj) {
+                        if (numb[jj] < 0) continue;
+                        const double v3[3] = {L.mass[i * 4 + jj], L.lb[i * 4 + jj], L.stab[i * 4 + jj]};
+                        e = builder_add(&rows[numb[i]], numb[jj], v3);
+                    }
+                }
+                if (rhs)
+                    for (int i = 0; i < 4; ++i)
+                        if (numb[i] >= 0) rhs[numb[i]] += L.rhs[i];
+            }
+            if (e) {
+#pragma omp critical(orc_color_err)
+                if (!err) err = e;
+            }
+        }
+        if (err) break;
+    }
+    if (!err) err = build_rows(rows, ndof, cap_nnz, nnz, rowptr, colind, val_mass, val_lb, val_stab);
     free(rows);
     return err;
 }
@@ -1008,36 +1133,6 @@ int orc_number_p1band(const orc_mesh_t *m, const double *phi, double width, int 
             if (dof_of_vertex[T.vid[i]] < 0) dof_of_vertex[T.vid[i]] = (int32_t)counter++;
     }
     *ndof = counter;
-    return ORC_OK;
-}
-
-/* Build() of one brow_t array into CSR (first value slot only or all three) */
-static int build_rows(brow_t *rows, int64_t ndof, int64_t cap, int64_t *nnz, int64_t *rowptr, int32_t *colind,
-                      double *v0, double *v1, double *v2)
-{
-    rowptr[0] = 0;
-    for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
-    *nnz = rowptr[ndof];
-    if (*nnz > cap) return ORC_E_CAPACITY;
-    if (!colind) return ORC_OK;
-    for (int64_t r = 0; r < ndof; ++r) {
-        brow_t *R = &rows[r];
-        int32_t ord[ROWCAP];
-        for (int k = 0; k < R->n; ++k) ord[k] = k;
-        for (int a = 1; a < R->n; ++a) {
-            int32_t x = ord[a];
-            int b = a - 1;
-            while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
-            ord[b + 1] = x;
-        }
-        for (int k = 0; k < R->n; ++k) {
-            const int64_t dst = rowptr[r] + k;
-            colind[dst] = R->col[ord[k]];
-            if (v0) v0[dst] = R->v[0][ord[k]];
-            if (v1) v1[dst] = R->v[1][ord[k]];
-            if (v2) v2[dst] = R->v[2][ord[k]];
-        }
-    }
     return ORC_OK;
 }
 

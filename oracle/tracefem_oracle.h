@@ -121,6 +121,16 @@ int orc_assemble_p1(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
                     int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
                     double *val_mass, double *val_lb, double *val_stab, double *rhs);
 
+/* ---- colour classes (ColorClassesCL::compute_color_classes, geom/multigrid.cpp:1238-1319) and the
+ *      colour-parallel accumulate() of the OpenMP build (num/accumulator.h:161-188): class_start has
+ *      room for 129 entries, class_tets for 6 n1 n2 n3 ---- */
+int orc_color_classes(const orc_mesh_t *m, int64_t *ncolors, int64_t *class_start, int64_t *class_tets);
+int orc_assemble_p1_colored(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_vertex, int64_t ndof,
+                            const double coef[3], int rhs_kind, const double *rhs_param, int zero_policy,
+                            int64_t ncolors, const int64_t *class_start, const int64_t *class_tets,
+                            int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
+                            double *val_mass, double *val_lb, double *val_stab, double *rhs);
+
 /* ---- narrow band (SurfactantNarrowBandStblP1CL::Update, surfactant/ifacetransp.cpp:1110-1172):
  *      band = tets with distance() <= width (cut tets have distance 0); numbering of every vertex of
  *      every band tet (CreateNumbNearInterfaceVertex, misc/problem.cpp:509-555); mass / LB / load over
