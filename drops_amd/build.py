@@ -9,11 +9,18 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-SOURCES = [os.path.join(HERE, "csrc", "dhip.hip")]
+CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libdhip.so")
+OBJ_DIR = os.path.join(HERE, "lib", "obj")
+# translation units: the library proper, and the persistent solver kernels in eleven shares
+# (the longest compilations first, so that eight parallel jobs finish together)
+UNITS = [(f"dhip_solvers_{k}", os.path.join(CSRC, "dhip_solvers.hip"), [f"-DDHIP_SOLVER_PART={k}"])
+         for k in (7, 10)] + [("dhip", os.path.join(CSRC, "dhip.hip"), [])] + [
+    (f"dhip_solvers_{k}", os.path.join(CSRC, "dhip_solvers.hip"), [f"-DDHIP_SOLVER_PART={k}"])
+    for k in (0, 1, 2, 8, 9, 6, 5, 4, 3)]
 
 HIP_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # products and sums are rounded one by one, in the reference's order
     "-ffp-contract=off",
     # no packed-FP32 instructions
@@ -21,16 +28,47 @@ HIP_FLAGS = [
 ]
 
 
-def build_lib(verbose=False):
-    os.makedirs(os.path.dirname(LIB), exist_ok=True)
+def _sources_mtime():
+    files = [os.path.join(CSRC, f) for f in os.listdir(CSRC)] + [os.path.join(ROOT, "include", "dhip.h")]
+    return max(os.path.getmtime(f) for f in files)
+
+
+def build_lib(verbose=False, jobs=8, incremental=False):
+    """Compiles every translation unit (in parallel, `jobs` at a time) and links libdhip.so.
+    incremental=True skips units whose object is newer than every source and header."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    newest = _sources_mtime()
+    procs, pending = [], []
+    for name, src, defs in UNITS:
+        obj = os.path.join(OBJ_DIR, name + ".o")
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > newest:
+            continue
+        pending.append([HIPCC] + HIP_FLAGS + defs + ["-c", "-o", obj + ".tmp", src])
+    failed = []
+    while pending or procs:
+        while pending and len(procs) < jobs:
+            cmd = pending.pop(0)
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+        cmd, p = procs.pop(0)
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            failed.append(out)
+        else:
+            os.replace(cmd[cmd.index("-o") + 1], cmd[cmd.index("-o") + 1][:-4])
+    if failed:
+        sys.stderr.write("\n".join(failed))
+        raise RuntimeError("hipcc failed for libdhip")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + HIP_FLAGS + ["-o", tmp] + SOURCES
+    objs = [os.path.join(OBJ_DIR, name + ".o") for name, _, _ in UNITS]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for libdhip")
+        raise RuntimeError("link failed for libdhip")
     os.replace(tmp, LIB)
     return LIB
 

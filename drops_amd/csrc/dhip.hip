@@ -371,6 +371,7 @@ static int check_err_flag(dhip_ctx *ctx) {
 using namespace dhip;
 
 static int noalloc_setup(dhip_ctx *c);
+static size_t persistent_lds(dhip_ctx *c, const void *k);
 
 // ===========================================================================
 // C ABI
@@ -398,6 +399,19 @@ int dhip_create(dhip_ctx_t **out, int32_t device) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return DHIP_E_NOMEM;
+    }
+    // the tagged PCG kernels' dynamic LDS (their attribute raised here, once, so that the
+    // allocation-free entry point launches them as they are)
+    {
+        const void *kt[3] = {(const void *)k_pcg64_tx<1>, (const void *)k_pcg64_tx<2>, (const void *)k_pcg64_tx<4>};
+        size_t lim = persistent_lds(c, kt[2]);
+        for (const void *k : kt) lim = std::min(lim, persistent_lds(c, k));
+        c->tx_lds = lim;
+        for (const void *k : kt)
+            if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim) != hipSuccess) {
+                (void)hipGetLastError();
+                c->tx_lds = LDS_BYTES;
+            }
     }
     // once per context (and so per device it runs on): the device address of the allocation-free
     // solver's scratch array
@@ -1239,14 +1253,16 @@ struct PcgFp64Bufs {
     double *vec[6];   // R0, R1, W0, W1, S0, S1
     SyncSlot *slots;  // at least one per block of the launch
     SolverResult *res;
+    uint4 *tx = nullptr;   // tagged records of r, w, s ([3][2][n], 96 n bytes); nullptr: none
 };
 
 // Diagonal scan, slot clearing (a fill kernel: no memset node) and the persistent FP64 PCG launch.
 // `alloc_ok` false: nothing on this path may allocate (no profiling events, no occupancy cache insert).
-// `max_blocks` > 0: the slots hold at most that many blocks.
+// `max_blocks` > 0: the slots hold at most that many blocks.  `tagged`: the tagged-exchange kernel
+// k_pcg64_tx (rows per thread 1, 2 or 4), else k_pcg64_sr with the context's exchange.
 static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double *db, double *dx,
                            const PcgFp64Bufs &w, int32_t max_iter, double tol, int32_t rel, bool alloc_ok,
-                           unsigned max_blocks = 0) {
+                           unsigned max_blocks, bool tagged) {
     const int64_t n = A->n;
     unsigned nb = 1;
     int rpt = 1;
@@ -1271,34 +1287,71 @@ static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const dou
     }
     int32_t *outs = w.res->i;
     double *outd = &w.res->resid;
-    PcgSrArgs pa{n, A->rowptr, A->col, A->val, pc == DHIP_PC_JACOBI ? w.diag : nullptr, db, dx,
-                 {w.vec[0], w.vec[1]}, {w.vec[2], w.vec[3]}, {w.vec[4], w.vec[5]}, w.slots, max_iter, tol, rel,
-                 outs, outd, outs + 1, nullptr};
+    const double *diag = pc == DHIP_PC_JACOBI ? w.diag : nullptr;
+    tagged = tagged && w.tx && rpt <= 4;
+    const int kr = rpt;   // selects the instantiation below
+    PcgSrArgs pa{n, A->rowptr, A->col, A->val, diag, db, dx, {w.vec[0], w.vec[1]}, {w.vec[2], w.vec[3]},
+                 {w.vec[4], w.vec[5]}, w.slots, max_iter, tol, rel, outs, outd, outs + 1, nullptr};
     pa.xcd_map = c->xcd_map;
-    const void *ks[5] = {(const void *)k_pcg64_sr<1>, (const void *)k_pcg64_sr<2>, (const void *)k_pcg64_sr<4>,
-                         (const void *)k_pcg64_sr<8>, (const void *)k_pcg64_sr<16>};
+    PcgTxArgs ta{n, A->rowptr, A->col, A->val, diag, db, dx, w.tx, w.tx + 2 * n, w.tx + 4 * n, w.slots,
+                 max_iter, tol, rel, outs, outd, outs + 1, nullptr};
+    ta.xcd_map = c->xcd_map;
+    const void *ks_f[5] = {(const void *)k_pcg64_sr<1, false>, (const void *)k_pcg64_sr<2, false>,
+                           (const void *)k_pcg64_sr<4, false>, (const void *)k_pcg64_sr<8, false>,
+                           (const void *)k_pcg64_sr<16, false>};
+    const void *ks_c[5] = {(const void *)k_pcg64_sr<1, true>, (const void *)k_pcg64_sr<2, true>,
+                           (const void *)k_pcg64_sr<4, true>, (const void *)k_pcg64_sr<8, true>,
+                           (const void *)k_pcg64_sr<16, true>};
+    const void *ks_t[5] = {(const void *)k_pcg64_tx<1>, (const void *)k_pcg64_tx<2>, (const void *)k_pcg64_tx<4>,
+                           nullptr, nullptr};
+    const void *const *ks = tagged ? ks_t : c->coherent_exchange ? ks_c : ks_f;
+    const int idx = kr == 1 ? 0 : kr == 2 ? 1 : kr == 4 ? 2 : kr == 8 ? 3 : 4;
+    // the tagged kernel keeps its slice and halo in up to 128 KiB of LDS
+    const size_t lds = tagged ? c->tx_lds : (size_t)LDS_BYTES;
+    ta.lds_bytes = (int64_t)lds;
     if (alloc_ok) {
         if (c->probe_on) {
             DHIP_HIPCHK(c, c->probe.ensure(sizeof(uint64_t) * 4 * (nb + 1)));
-            pa.probe = c->probe.as<uint64_t>();
+            pa.probe = ta.probe = c->probe.as<uint64_t>();
             c->probe_blocks = nb;
         }
-        return launch_persistent(c, "pcg64", ks, rpt, nb, pa);
+        return tagged ? launch_persistent(c, "pcg64", ks, kr, nb, ta, lds)
+                      : launch_persistent(c, "pcg64", ks, kr, nb, pa);
     }
     // allocation-free launch: occupancy queried directly, no event timing
-    const int idx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
     int per_cu = 0;
-    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[idx], PB, LDS_BYTES));
+    DHIP_HIPCHK(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ks[idx], PB, lds));
     if ((int64_t)per_cu * c->cu_count < (int64_t)nb)
         return set_error(c, DHIP_E_HIP, "persistent kernel cannot keep %u blocks resident", nb);
-    void *argv[] = {&pa};
+    void *argv_s[] = {&pa};
+    void *argv_t[] = {&ta};
+    void **argv = tagged ? argv_t : argv_s;
     if (!c->coop_refused) {
-        const hipError_t e = hipLaunchCooperativeKernel(ks[idx], dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream);
+        const hipError_t e = hipLaunchCooperativeKernel(ks[idx], dim3(nb), dim3(PB), argv, lds, c->stream);
         if (e == hipSuccess) return DHIP_OK;
         (void)hipGetLastError();
         c->coop_refused = true;
     }
-    DHIP_HIPCHK(c, hipLaunchKernel(ks[idx], dim3(nb), dim3(PB), argv, LDS_BYTES, c->stream));
+    DHIP_HIPCHK(c, hipLaunchKernel(ks[idx], dim3(nb), dim3(PB), argv, lds, c->stream));
+    return DHIP_OK;
+}
+
+// The FP64 solve with the result record read back into *out: the tagged kernel when the context
+// asks for it and the buffers allow, relaunched as k_pcg64_sr when a row is longer than it handles.
+static int pcg_fp64_run(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double *db, double *dx,
+                        const PcgFp64Bufs &w, int32_t max_iter, double tol, int32_t rel, bool alloc_ok,
+                        unsigned max_blocks, SolverResult *out) {
+    const bool tagged = c->coherent_exchange == 2 && w.tx;
+    int rc = pcg_fp64_launch(c, A, pc, db, dx, w, max_iter, tol, rel, alloc_ok, max_blocks, tagged);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipMemcpyAsync(out, w.res, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (tagged && out->i[1] == 3) {
+        rc = pcg_fp64_launch(c, A, pc, db, dx, w, max_iter, tol, rel, alloc_ok, max_blocks, false);
+        if (rc) return rc;
+        DHIP_HIPCHK(c, hipMemcpyAsync(out, w.res, sizeof *out, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     return DHIP_OK;
 }
 
@@ -1309,6 +1362,9 @@ constexpr size_t NOALLOC_SLOT_BYTES = NOALLOC_MAX_BLOCKS * sizeof(SyncSlot);
 constexpr size_t NOALLOC_RES_BYTES = 256;
 constexpr size_t NOALLOC_VEC_BYTES = sizeof(double) * (size_t)DHIP_NOALLOC_MAX_ROWS;
 constexpr size_t NOALLOC_BYTES = NOALLOC_SLOT_BYTES + NOALLOC_RES_BYTES + 7 * NOALLOC_VEC_BYTES;
+// the tagged records of r, w, s (96 bytes a row) share the six vectors' 48 bytes a row: they fit
+// systems of up to half the maximum rows
+constexpr int64_t NOALLOC_TX_ROWS = DHIP_NOALLOC_MAX_ROWS / 2;
 static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
 
 // Device address of this device's copy of `scratch` (the host-side name is only a stand-in).
@@ -1349,11 +1405,10 @@ int dhip_pcg_noalloc(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const doubl
     double *v = reinterpret_cast<double *>(s + NOALLOC_SLOT_BYTES + NOALLOC_RES_BYTES);
     w.diag = v;
     for (int k = 0; k < 6; ++k) w.vec[k] = v + (size_t)(k + 1) * DHIP_NOALLOC_MAX_ROWS;
-    int rc = pcg_fp64_launch(c, A, pc, b, x, w, *max_iter, *tol, rel, false, NOALLOC_MAX_BLOCKS);
-    if (rc) return rc;
+    if (n <= NOALLOC_TX_ROWS) w.tx = reinterpret_cast<uint4 *>(w.vec[0]);
     SolverResult res;
-    DHIP_HIPCHK(c, hipMemcpyAsync(&res, w.res, sizeof res, hipMemcpyDeviceToHost, c->stream));
-    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    int rc = pcg_fp64_run(c, A, pc, b, x, w, *max_iter, *tol, rel, false, NOALLOC_MAX_BLOCKS, &res);
+    if (rc) return rc;
     *max_iter = res.i[0];
     *tol = res.resid;
     if (res.i[1] == 2) return set_error(c, DHIP_E_HIP, "grid barrier timed out in the persistent solver");
@@ -1410,7 +1465,12 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         for (int k = 0; k < 6; ++k) w.vec[k] = c->kv[k].as<double>();
         w.slots = c->bar.as<SyncSlot>();
         w.res = reinterpret_cast<SolverResult *>(outs);
-        rc = pcg_fp64_launch(c, A, pc, db, dx, w, *max_iter, *tol, rel, true);
+        if (c->coherent_exchange == 2) {
+            DHIP_HIPCHK(c, c->kv[16].ensure((size_t)96 * (n + 1)));
+            w.tx = c->kv[16].as<uint4>();
+        }
+        SolverResult fres;
+        rc = pcg_fp64_run(c, A, pc, db, dx, w, *max_iter, *tol, rel, true, 0, &fres);
         if (rc) return rc;
     } else {
         if (n > 0)
@@ -1469,6 +1529,11 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
 
 int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
     if (!c || !key) return DHIP_E_ARG;
+    if (!std::strcmp(key, "coherent_exchange")) {
+        if (value < 0 || value > 2) return set_error(c, DHIP_E_ARG, "coherent_exchange must be 0, 1 or 2");
+        c->coherent_exchange = (int)value;
+        return DHIP_OK;
+    }
     if (!std::strcmp(key, "xcd_map")) {
         if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "xcd_map must be 0 or 1");
         c->xcd_map = (int)value;
@@ -1596,9 +1661,13 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
         c->probe_blocks = nb;
     }
     const bool regv = rpt == 1 && m <= KREG;
-    const void *ks[5] = {regv ? (const void *)k_gmres<1, KREG> : (const void *)k_gmres<1, 0>,
-                         (const void *)k_gmres<2, 0>, (const void *)k_gmres<4, 0>, (const void *)k_gmres<8, 0>,
-                         (const void *)k_gmres<16, 0>};
+    const void *ks_f[5] = {regv ? (const void *)k_gmres<1, KREG, false> : (const void *)k_gmres<1, 0, false>,
+                           (const void *)k_gmres<2, 0, false>, (const void *)k_gmres<4, 0, false>,
+                           (const void *)k_gmres<8, 0, false>, (const void *)k_gmres<16, 0, false>};
+    const void *ks_c[5] = {regv ? (const void *)k_gmres<1, KREG, true> : (const void *)k_gmres<1, 0, true>,
+                           (const void *)k_gmres<2, 0, true>, (const void *)k_gmres<4, 0, true>,
+                           (const void *)k_gmres<8, 0, true>, (const void *)k_gmres<16, 0, true>};
+    const void *const *ks = c->coherent_exchange ? ks_c : ks_f;
     const int kidx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
     const size_t lds = regv ? (size_t)LDS_BYTES : persistent_lds(c, ks[kidx]);
     ga.lds_bytes = (int64_t)lds;

@@ -131,7 +131,7 @@ struct dhip_ctx {
     dhip::DevBuf scan_tmp, red_tmp, bar;
     dhip::DevBuf host_stage;  // pinned staging for scalars
     // Krylov work vectors (grow-only: no device allocation once warmed up)
-    dhip::DevBuf kv[16];
+    dhip::DevBuf kv[17];
     dhip::DevBuf gm_sval;   // right-preconditioned GMRES: a_rc / d_c
     dhip::DevBuf gm_sell_off, gm_sell_col, gm_sell_val;  // SELL-64 copy for large systems
     dhip::DevBuf gm_sell_pos;                            // SELL slot of every CSR entry
@@ -142,6 +142,7 @@ struct dhip_ctx {
     int batch_threads = 256;          // workgroup size of the batched solve (128, 256, 512)
     int batch_blocks_per_cu = 2;      // resident workgroups per CU of the batched solve
     int xcd_map = 1;                  // persistent solvers: each XCD's blocks own one contiguous run of rows
+    int coherent_exchange = 2;        // persistent solvers' exchange: 0 fenced, 1 coherent atomics, 2 tagged records (FP64 PCG)
     int batch_split = 0;              // batched solve: assembly launch, then a separate PCG launch
     int batch_solve_threads = 1024;   // threads per problem of the separate PCG launch
     int batch_prepass = 0;            // batched solve: classification and local matrices over the whole grid first
@@ -149,6 +150,7 @@ struct dhip_ctx {
     int probe_blocks = 0;
     dhip::DevBuf probe;
     unsigned char *noalloc_scratch = nullptr;   // this device's copy of the allocation-free solver's scratch
+    size_t tx_lds = 0;                          // dynamic LDS of the tagged PCG kernels
 };
 
 struct dhip_csr {

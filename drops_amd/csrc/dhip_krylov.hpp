@@ -9,6 +9,10 @@
 // so every block takes the same convergence decision.
 #pragma once
 
+// The persistent solver kernels (k_pcg64, k_pcg64_sr, k_gmres, k_pcg_mixed) are defined in the
+// translation unit that defines DHIP_SOLVER_KERNELS (dhip_solvers.hip, which also instantiates
+// them); every other unit sees their declarations.
+
 #include "dhip_common.hpp"
 
 namespace dhip {
@@ -40,6 +44,7 @@ __device__ __forceinline__ double row_dot(const int32_t *rp, const int32_t *col,
 // RESID: y = b - A x instead of y = A x (b - the serial row sum).
 constexpr int SPMV_T = 256, SPMV_CAP = 2048, SPMV_PT = SPMV_CAP / SPMV_T;
 
+#ifndef DHIP_SOLVER_KERNELS
 template <bool RESID>
 __global__ void __launch_bounds__(SPMV_T) k_spmv(int64_t n, int rb, const int32_t *rp, const int32_t *col,
                                                  const double *val, const double *x, const double *b, double *y) {
@@ -79,6 +84,7 @@ __global__ void __launch_bounds__(SPMV_T) k_spmv(int64_t n, int rb, const int32_
         }
     }
 }
+#endif
 
 // rows per SpMV tile: the largest power of two <= 256 whose tile of mean-length rows, with 25 %
 // headroom, fits the LDS stage
@@ -89,6 +95,7 @@ inline int spmv_tile_rows(int64_t n, int64_t nnz) {
     return rb;
 }
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_dot_partial(int64_t n, const double *x, const double *y, double *part) {
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x; i < n; i += (int64_t)gridDim.x * KB)
@@ -96,20 +103,26 @@ __global__ void __launch_bounds__(KB) k_dot_partial(int64_t n, const double *x, 
     s = block_sum<KB>(s);
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_reduce(const double *part, int np, double *out) {
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += KB) s += part[i];
     s = block_sum<KB>(s);
     if (threadIdx.x == 0) *out = s;
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_axpy(int64_t n, double a, const double *x, double *y) {
     const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
     if (i < n) y[i] += a * x[i];
 }
+#endif
 
 // diagonal located by a row scan (SolveGSstep<PB_JAC0>, num/precond.h:163-168)
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_diag(int64_t n, const int32_t *rp, const int32_t *col, const double *val,
                                              double *diag, float *diag32) {
     const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
@@ -121,20 +134,27 @@ __global__ void __launch_bounds__(KB) k_diag(int64_t n, const int32_t *rp, const
     diag[i] = d;
     if (diag32) diag32[i] = (float)d;
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_to_f32(int64_t n, const double *a, float *b) {
     const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_zero_u32(uint32_t *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_copy_i32(const int32_t *src, int32_t *dst) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *dst = *src;
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Grid synchronisation for cooperative (co-resident) launches.
@@ -165,8 +185,29 @@ constexpr int POLL_CHUNKS = 4;             // slots polled per lane in one round
 
 __device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
-// Returns false in every thread of a block that saw a timeout.
-template <int BLOCK>
+// Vector values exchanged between the blocks of a persistent solver.  COH = false: plain stores and
+// loads, made visible by the agent-scope release and acquire fences of the synchronisation, which
+// write back and invalidate the L2 of every block's XCD on each synchronisation.  COH = true: each
+// exchanged value is stored and loaded as a relaxed agent-scope atomic -- coherent across the XCDs
+// by itself -- and the synchronisation runs without fences: a block publishes its slot only after
+// s_waitcnt vmcnt(0) has seen all its stores complete, and a reader issues its loads only after it
+// has seen every slot of the generation.
+template <bool COH>
+__device__ __forceinline__ double xload(const double *p) {
+    if (COH) return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *p;
+}
+template <bool COH>
+__device__ __forceinline__ void xstore(double *p, double v) {
+    if (COH)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
+// Returns false in every thread of a block that saw a timeout.  FENCES = false: for solvers whose
+// exchanged values are coherent atomics (xload / xstore with COH = true).
+template <int BLOCK, bool FENCES = true>
 __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, double a1, double a2, double out[3]) {
     __shared__ double sh[SYNC_VALS];
     __shared__ int ok_sh;
@@ -176,7 +217,7 @@ __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, doubl
         const int lane = threadIdx.x, nb = gridDim.x, par = g & 1;
         const uint64_t tag = (uint64_t)g << 32;
         if (lane == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (FENCES) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             uint64_t *w = slots[blockIdx.x].w[par];
             const double in[SYNC_VALS] = {a0, a1, a2};
 #pragma unroll
@@ -223,7 +264,7 @@ __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, doubl
                             (long long)((x[j][2 * i] & 0xffffffffull) | (x[j][2 * i + 1] << 32)));
                 }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (FENCES) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
         for (int i = 0; i < SYNC_VALS; ++i) v[i] = wave_sum(v[i]);
         const bool all_ok = __ballot(ok) == __ballot(1);
@@ -364,8 +405,54 @@ __device__ __forceinline__ ACC row_product(const int32_t *rp, const int32_t *col
     return s;
 }
 
+// The row product of row_product summed in two halves, columns [0, H) and [H, len) each in column
+// order, then added: the summation of k_pcg64_sr, which k_pcg64_tx reproduces over its LDS halo.
+template <int H, class F>
+__device__ __forceinline__ double row_product_halves(const int32_t *rp, const int32_t *col, const double *val,
+                                                     int32_t off, int64_t l, F &&xv) {
+    const int32_t b = rp[l] - off, e = rp[l + 1] - off;
+    const int len = e - b;
+    double s0 = 0.0, s1 = 0.0;
+    int32_t cc[H];
+    double vv[H], xx[H];
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+        if (j < len) {
+            cc[j] = col[b + j];
+            vv[j] = val[b + j];
+        }
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+        if (j < len) xx[j] = xv(cc[j]);
+#pragma unroll
+    for (int j = 0; j < H; ++j)
+        if (j < len) s0 += vv[j] * xx[j];
+    for (int32_t q0 = b + H; q0 < e; q0 += H) {
+        const int cnt = e - q0 < H ? e - q0 : H;
+#pragma unroll
+        for (int j = 0; j < H; ++j)
+            if (j < cnt) {
+                cc[j] = col[q0 + j];
+                vv[j] = val[q0 + j];
+            }
+#pragma unroll
+        for (int j = 0; j < H; ++j)
+            if (j < cnt) xx[j] = xv(cc[j]);
+#pragma unroll
+        for (int j = 0; j < H; ++j)
+            if (j < cnt) s1 += vv[j] * xx[j];
+    }
+    return s0 + s1;
+}
+
+constexpr int TX_HALF = 8;   // columns of the first half of a row (row_product_halves)
+
 template <int RPT>
-__global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a) {
+__global__ void __launch_bounds__(PB) k_pcg64(PcgArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
@@ -482,6 +569,7 @@ done:
         *a.out_status = status;
     }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // persistent single-reduction PCG (FP64): the Chronopoulos-Gear arrangement of
@@ -550,8 +638,12 @@ __device__ inline Slice<double> load_slice_scaled(unsigned char *lds, const int3
     return S;
 }
 
-template <int RPT>
-__global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
+template <int RPT, bool COH>
+__global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
     extern __shared__ __align__(16) unsigned char lds[];
     const uint64_t t_start = wall_clock64();
     uint64_t t_compute = 0, t_sync = 0, t_mark = 0;
@@ -563,8 +655,8 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
     const double *dg = a.diag;
     // sum_c (a_rc / d_c) * y_c over local row l, y gathered by yv
     auto scaled_row = [&](int64_t l, auto &&yv) {
-        if (scaled || !dg) return row_product<double>(S.rp, S.col, S.val, S.off, l, yv);
-        return row_product<double>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv(c) / dg[c]; });
+        if (scaled || !dg) return row_product_halves<TX_HALF>(S.rp, S.col, S.val, S.off, l, yv);
+        return row_product_halves<TX_HALF>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv(c) / dg[c]; });
     };
     uint32_t gen = 0;
     double red[3];
@@ -589,7 +681,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
         xr[k] = a.x[r];
         rr[k] = bv - ax;
         dr[k] = dg ? dg[r] : 1.0;
-        a.R[0][r] = rr[k];
+        xstore<COH>(&a.R[0][r], rr[k]);
         p_rr += rr[k] * rr[k];
         p_bb += bv * bv;
     }
@@ -597,7 +689,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
         double unused = 0.0;
         block_sum3<PB>(p_rr, p_bb, unused);
     }
-    if (!sync_reduce<PB>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; goto done; }
+    if (!sync_reduce<PB, !COH>(a.slots, ++gen, p_rr, p_bb, 0.0, red)) { status = 2; goto done; }
     {
         double normb = sqrt(red[1]);
         if (normb == 0.0 || !a.rel) normb = 1.0;
@@ -615,11 +707,11 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
             if (!own[k]) continue;
             const int64_t r = r0 + lr[k];
             ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
-            wr[k] = scaled_row(lr[k], [&](int32_t c) { return R0[c]; });
+            wr[k] = scaled_row(lr[k], [&](int32_t c) { return xload<COH>(&R0[c]); });
             sr[k] = 0.0;
             pr[k] = 0.0;
-            a.W[0][r] = wr[k];
-            a.S[0][r] = 0.0;
+            xstore<COH>(&a.W[0][r], wr[k]);
+            xstore<COH>(&a.S[0][r], 0.0);
             p_g += rr[k] * ur[k];
             p_d += wr[k] * ur[k];
         }
@@ -627,7 +719,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
             double unused = 0.0;
             block_sum3<PB>(p_g, p_d, unused);
         }
-        if (!sync_reduce<PB>(a.slots, ++gen, p_g, p_d, 0.0, red)) { status = 2; goto done; }
+        if (!sync_reduce<PB, !COH>(a.slots, ++gen, p_g, p_d, 0.0, red)) { status = 2; goto done; }
         double gamma = red[0], alpha = red[0] / red[1], beta = 0.0;
         const uint64_t t_setup = wall_clock64() - t_start;
         if (a.probe && threadIdx.x == 0) a.probe[blockIdx.x * 4 + 0] = t_setup;
@@ -646,10 +738,12 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
                 xr[k] += alpha * pr[k];
                 rr[k] = rr[k] - alpha * sr[k];
                 ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
-                wr[k] = scaled_row(lr[k], [&](int32_t c) { return Rc[c] - alpha * (Wc[c] + beta * Sc[c]); });
-                a.R[nxt][r] = rr[k];
-                a.W[nxt][r] = wr[k];
-                a.S[nxt][r] = sr[k];
+                wr[k] = scaled_row(lr[k], [&](int32_t c) {
+                    return xload<COH>(&Rc[c]) - alpha * (xload<COH>(&Wc[c]) + beta * xload<COH>(&Sc[c]));
+                });
+                xstore<COH>(&a.R[nxt][r], rr[k]);
+                xstore<COH>(&a.W[nxt][r], wr[k]);
+                xstore<COH>(&a.S[nxt][r], sr[k]);
                 p_g2 += rr[k] * ur[k];
                 p_d2 += wr[k] * ur[k];
                 p_rr2 += rr[k] * rr[k];
@@ -660,7 +754,7 @@ __global__ void __launch_bounds__(PB) k_pcg64_sr(PcgSrArgs a) {
                 t_compute += t1 - t_mark;
                 t_mark = t1;
             }
-            if (!sync_reduce<PB>(a.slots, ++gen, p_g2, p_d2, p_rr2, red)) { status = 2; goto done; }
+            if (!sync_reduce<PB, !COH>(a.slots, ++gen, p_g2, p_d2, p_rr2, red)) { status = 2; goto done; }
             if (a.probe) t_sync += wall_clock64() - t_mark;
             resid = sqrt(red[2]) / normb;
             if (resid <= a.tol) {
@@ -691,6 +785,457 @@ done:
         *a.out_status = status;
     }
 }
+#endif
+
+// ---------------------------------------------------------------------------
+// Tagged exchange: every exchanged double travels with the generation that wrote it, as one
+// 16-byte record {lo32, gen, hi32, gen} written and read with device-coherent (sc1) 128-bit
+// buffer accesses.  A reader takes a value only when both 64-bit halves carry the generation it
+// expects (each half is single-copy atomic), and reloads it otherwise, so a block needs neither
+// a store acknowledgement nor a flag before its neighbours may read its rows: the data are the
+// flag.  The reductions still go through the slots, published without fences and collected
+// while the gathers are in flight.  Records are double buffered by generation parity.
+// ---------------------------------------------------------------------------
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+struct TxVec {
+    __amdgpu_buffer_rsrc_t rsrc;   // [2][n] records of 16 bytes
+    int64_t n;
+};
+
+__device__ __forceinline__ TxVec tx_vec(uint4 *base, int64_t n) {
+    TxVec v;
+    v.rsrc = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)(2 * n * 16), 0x00020000);
+    v.n = n;
+    return v;
+}
+
+constexpr int TX_SC1 = 16;   // cache policy: device-coherent (sc1)
+
+__device__ __forceinline__ void tx_store(const TxVec &v, uint32_t g, int64_t i, double x) {
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const uint32_t off = (uint32_t)(((g & 1) * v.n + i) * 16);
+    const u32x4 rec = {(unsigned)(b & 0xffffffffu), g, (unsigned)(b >> 32), g};
+    __builtin_amdgcn_raw_buffer_store_b128(rec, v.rsrc, off, 0, TX_SC1);
+}
+
+__device__ __forceinline__ u32x4 tx_load_raw(const TxVec &v, uint32_t g, int64_t i) {
+    const uint32_t off = (uint32_t)(((g & 1) * v.n + i) * 16);
+    return __builtin_amdgcn_raw_buffer_load_b128(v.rsrc, off, 0, TX_SC1);
+}
+
+__device__ __forceinline__ bool tx_ready(const u32x4 &r, uint32_t g) { return r[1] == g && r[3] == g; }
+
+__device__ __forceinline__ double tx_value(const u32x4 &r) {
+    return __longlong_as_double((long long)((uint64_t)r[0] | ((uint64_t)r[2] << 32)));
+}
+
+// Publish three block partials in this block's slot for generation g (no fence: the partials are
+// the slot's own tagged words).  Called by one thread.
+__device__ __forceinline__ void slot_publish3(SyncSlot *slots, uint32_t g, double a0, double a1, double a2) {
+    uint64_t *w = slots[blockIdx.x].w[g & 1];
+    const uint64_t tag = (uint64_t)g << 32;
+    const double in[SYNC_VALS] = {a0, a1, a2};
+#pragma unroll
+    for (int i = 0; i < SYNC_VALS; ++i) {
+        const uint64_t bits = __double_as_longlong(in[i]);
+        __hip_atomic_store(&w[2 * i], tag | (bits & 0xffffffffull), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&w[2 * i + 1], tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Collect generation g of every block's slot by one wave and sum the partials in the order of
+// sync_reduce (lane l adds slots l, l + 64, l + 128, l + 192, then the wave sum), so that the totals
+// are the ones sync_reduce gives.  Lane 0 returns them; false after a timeout.
+__device__ inline bool slot_collect3(SyncSlot *slots, uint32_t g, double out[3]) {
+    const int lane = threadIdx.x & 63, nb = gridDim.x, par = g & 1;
+    int ok = 1;
+    double v[SYNC_VALS] = {0.0, 0.0, 0.0};
+    for (int base = 0; base < nb; base += POLL_CHUNKS * 64) {
+        uint64_t x[POLL_CHUNKS][2 * SYNC_VALS];
+        uint32_t spins = 0;
+        for (;;) {
+            bool ready = true;
+#pragma unroll
+            for (int j = 0; j < POLL_CHUNKS; ++j) {
+                const int l = base + j * 64 + lane;
+                if (l < nb) {
+                    const uint64_t *w = slots[l].w[par];
+#pragma unroll
+                    for (int i = 0; i < 2 * SYNC_VALS; ++i) {
+                        x[j][i] = __hip_atomic_load(&w[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ready &= (uint32_t)(x[j][i] >> 32) == g;
+                    }
+                }
+            }
+            if (ready) break;
+            if (++spins > SPIN_LIMIT) {
+                ok = 0;
+                break;
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < POLL_CHUNKS; ++j)
+            if (base + j * 64 + lane < nb) {
+#pragma unroll
+                for (int i = 0; i < SYNC_VALS; ++i)
+                    v[i] += __longlong_as_double((long long)((x[j][2 * i] & 0xffffffffull) | (x[j][2 * i + 1] << 32)));
+            }
+    }
+#pragma unroll
+    for (int i = 0; i < SYNC_VALS; ++i) out[i] = wave_sum(v[i]);
+    return __ballot(ok) == __ballot(1);
+}
+
+// Persistent FP64 PCG, the single-reduction arrangement of k_pcg64_sr with the tagged exchange and
+// the block's halo in LDS.  At set-up each block lists the distinct columns of its rows (an LDS
+// bitmap over their range, ranked by a block scan) and rewrites its CSR slice to indices into that
+// list.  Per iteration the block loads the r, w, s records of generation it + 2 once per distinct
+// column it does not own -- one tagged load per value instead of one per matrix entry -- into LDS,
+// while one wave collects the block partials of the same generation; its own rows' values enter
+// the LDS copy from registers.  The row products then read LDS only.  One row per thread as in
+// k_pcg64_sr, rows summed in row_product_halves' order, block partials added as there: the
+// iterates equal k_pcg64_sr's.  A block whose slice and halo do not fit the LDS makes every block
+// leave with status 3 after the first reduction (the host then runs k_pcg64_sr).
+struct PcgTxArgs {
+    int64_t n;
+    const int32_t *rp, *col;
+    const double *val, *diag;   // diag == nullptr: no preconditioner
+    const double *b;
+    double *x;
+    uint4 *R, *W, *S;           // [2][n] tagged records each
+    SyncSlot *slots;
+    int max_iter;
+    double tol;
+    int rel;
+    int *out_iter;
+    double *out_resid;
+    int *out_status;            // 0 converged, 1 not converged, 2 timeout, 3 slice and halo over the LDS
+    uint64_t *probe;            // optional [gridDim.x][4] wall-clock ticks: setup, compute, wait, total
+    int xcd_map = 0;
+    int64_t lds_bytes = 0;      // dynamic LDS of the launch
+};
+
+__device__ inline int block_min_int(int v) {
+    __shared__ int ws[PB / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int t = __shfl_xor(v, o, 64);
+        v = t < v ? t : v;
+    }
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int m = ws[0];
+#pragma unroll
+    for (int i = 1; i < PB / 64; ++i) m = ws[i] < m ? ws[i] : m;
+    __syncthreads();
+    return m;
+}
+
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_pcg64_tx(PcgTxArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ double red_sh[SYNC_VALS];
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_wait = 0;
+    bool tbad = false;   // this thread saw a timeout
+    const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
+    const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
+    const int64_t nr = r1 > r0 ? r1 - r0 : 0;
+    const int32_t nz0 = nr ? a.rp[r0] : 0;
+    const int64_t nzb = nr ? (int64_t)a.rp[r1] - nz0 : 0;
+    const double *dg = a.diag;
+    const TxVec TR = tx_vec(a.R, a.n), TW = tx_vec(a.W, a.n), TS = tx_vec(a.S, a.n);
+    // ---- set-up A: the distinct columns of the block and the slice in local column indices ----
+    int cmn = 0x7fffffff, cmx = -1;
+    for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+        const int c = a.col[nz0 + k];
+        cmn = c < cmn ? c : cmn;
+        cmx = c > cmx ? c : cmx;
+    }
+    cmn = block_min_int(cmn);
+    cmx = -block_min_int(-cmx);
+    const int64_t range = nzb ? (int64_t)cmx - cmn + 1 : 0;
+    const int64_t nwords = (range + 31) / 32;
+    // LDS: lrp[nr + 1] | lcol[nzb] | bits[nwords] | pre[nwords] | U[nu] | (16-aligned) lval[nzb] | X[3][nu]
+    int32_t *lrp = reinterpret_cast<int32_t *>(lds);
+    int32_t *lcol = lrp + nr + 1;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lcol + nzb);
+    int32_t *pre = reinterpret_cast<int32_t *>(bits + nwords);
+    int32_t *U = pre + nwords;
+    bool fits = 4 * (size_t)(nr + 1 + nzb + 2 * nwords) <= (size_t)a.lds_bytes;
+    int nu = 0;
+    double *lval = nullptr, *XR = nullptr, *XW = nullptr, *XS = nullptr;
+    if (fits && nzb > 0) {
+        for (int64_t w = threadIdx.x; w < nwords; w += PB) bits[w] = 0u;
+        __syncthreads();
+        for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+            const int d = a.col[nz0 + k] - cmn;
+            atomicOr(&bits[d >> 5], 1u << (d & 31));
+        }
+        __syncthreads();
+        // rank of every set bit: each thread takes a run of words
+        const int64_t wpt = (nwords + PB - 1) / PB;
+        const int64_t w0 = threadIdx.x * wpt, w1 = w0 + wpt < nwords ? w0 + wpt : nwords;
+        int cnt = 0;
+        for (int64_t w = w0; w < w1; ++w) cnt += __popc(bits[w]);
+        int tot = 0;
+        int off = block_excl_scan<PB>(cnt, &tot);
+        nu = tot;
+        const size_t int_bytes = 4 * (size_t)(nr + 1 + nzb + 2 * nwords + nu);
+        const size_t lval_off = (int_bytes + 15) & ~(size_t)15;
+        // the same in every thread (the scan total)
+        fits = lval_off + 8 * (size_t)(nzb + 3 * (int64_t)nu) <= (size_t)a.lds_bytes;
+        if (fits) {
+            lval = reinterpret_cast<double *>(lds + lval_off);
+            XR = lval + nzb;
+            XW = XR + nu;
+            XS = XW + nu;
+            for (int64_t w = w0; w < w1; ++w) {
+                pre[w] = off;
+                uint32_t m = bits[w];
+                while (m) {
+                    const int bpos = __ffs(m) - 1;
+                    U[off++] = cmn + (int)(32 * w) + bpos;
+                    m &= m - 1;
+                }
+            }
+            __syncthreads();
+            for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = a.rp[r0 + i] - nz0;
+            for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+                const int c = a.col[nz0 + k], d = c - cmn;
+                lcol[k] = pre[d >> 5] + __popc(bits[d >> 5] & ((1u << (d & 31)) - 1u));
+                lval[k] = dg ? a.val[nz0 + k] / dg[c] : a.val[nz0 + k];
+            }
+            __syncthreads();
+        }
+    } else if (fits) {
+        for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = 0;   // rows without entries
+        __syncthreads();
+    }
+    // local index of global column c (-1: not a column of the block)
+    auto local_of = [&](int64_t c) -> int {
+        const int64_t d = c - cmn;
+        if (d < 0 || d >= range) return -1;
+        const uint32_t w = bits[d >> 5], m = 1u << (d & 31);
+        return (w & m) ? pre[d >> 5] + __popc(w & (m - 1u)) : -1;
+    };
+    // row sum over LDS in row_product_halves' order: columns [0, 8), then the rest
+    auto lds_row = [&](int64_t l, auto &&yv) {
+        const int32_t b0 = lrp[l], e0 = lrp[l + 1];
+        const int32_t m0 = e0 - b0 < TX_HALF ? e0 : b0 + TX_HALF;
+        double s0 = 0.0, s1 = 0.0;
+        for (int32_t q = b0; q < m0; ++q) s0 += lval[q] * yv(lcol[q]);
+        for (int32_t q = m0; q < e0; ++q) s1 += lval[q] * yv(lcol[q]);
+        return s0 + s1;
+    };
+    // the halo of generation g: r (and w, s when three) of every distinct column not owned by the
+    // block, loaded together and reloaded until written
+    auto halo = [&](uint32_t g, bool three) {
+        for (int t0 = 0; t0 < nu; t0 += PB) {
+            const int t = t0 + threadIdx.x;
+            const bool mine = t < nu && !(U[t] >= r0 && U[t] < r1);
+            if (!mine) continue;
+            const int64_t c = U[t];
+            u32x4 vr = tx_load_raw(TR, g, c), vw, vs;
+            if (three) {
+                vw = tx_load_raw(TW, g, c);
+                vs = tx_load_raw(TS, g, c);
+            }
+            uint32_t spins = 0;
+            for (;;) {
+                bool ok = tx_ready(vr, g);
+                if (three) ok = ok && tx_ready(vw, g) && tx_ready(vs, g);
+                if (ok) break;
+                if (++spins > SPIN_LIMIT) {
+                    tbad = true;
+                    break;
+                }
+                // the buffer-load builtin reads memory the compiler assumes unchanged: without this
+                // barrier it reuses the first load's result and the reload never happens
+                __asm__ __volatile__("" ::: "memory");
+                if (!tx_ready(vr, g)) vr = tx_load_raw(TR, g, c);
+                if (three) {
+                    if (!tx_ready(vw, g)) vw = tx_load_raw(TW, g, c);
+                    if (!tx_ready(vs, g)) vs = tx_load_raw(TS, g, c);
+                }
+            }
+            XR[t] = tx_value(vr);
+            if (three) {
+                XW[t] = tx_value(vw);
+                XS[t] = tx_value(vs);
+            }
+        }
+    };
+    // the block partials of generation g into red_sh; false in every thread when any thread of the
+    // block has seen a timeout
+    auto collect = [&](uint32_t g) {
+        const uint64_t t0 = a.probe ? wall_clock64() : 0;
+        if (threadIdx.x < 64) {
+            double o[3];
+            const bool ok = slot_collect3(a.slots, g, o);
+            if (threadIdx.x == 0) {
+                red_sh[0] = o[0];
+                red_sh[1] = o[1];
+                red_sh[2] = o[2];
+                if (!ok) {
+                    tbad = true;
+                    a.slots[blockIdx.x].timeout = 1;
+                }
+            }
+        }
+        const bool any = __syncthreads_or(tbad);
+        if (a.probe) t_wait += wall_clock64() - t0;
+        return !any;
+    };
+    int status = 1, it_done = a.max_iter;
+    double resid = 0.0, normb = 1.0;
+    double xr[RPT], rr[RPT], ur[RPT], wr[RPT], pr[RPT], sr[RPT], dr[RPT];
+    int lo[RPT];
+    bool own[RPT];
+    int64_t lr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = lr[k] < nr;
+        lo[k] = (own[k] && fits) ? local_of(r0 + lr[k]) : -1;
+    }
+    // ---- set-up B: r0 = b - A x0 (generation 1), |r|^2, |b|^2, and whether every block fits ----
+    {
+        double p_rr = 0.0, p_bb = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            const double ax = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
+            const double bv = a.b[r];
+            xr[k] = a.x[r];
+            rr[k] = bv - ax;
+            dr[k] = dg ? dg[r] : 1.0;
+            tx_store(TR, 1, r, rr[k]);
+            if (lo[k] >= 0) XR[lo[k]] = rr[k];
+            p_rr += rr[k] * rr[k];
+            p_bb += bv * bv;
+        }
+        double unused = 0.0;
+        block_sum3<PB>(p_rr, p_bb, unused);
+        if (threadIdx.x == 0) slot_publish3(a.slots, 1, p_rr, p_bb, fits ? 0.0 : 1.0);
+    }
+    if (fits) halo(1, false);
+    if (!collect(1)) { status = 2; goto done; }
+    if (red_sh[2] != 0.0) { status = 3; goto done; }
+    {
+        normb = sqrt(red_sh[1]);
+        if (normb == 0.0 || !a.rel) normb = 1.0;
+        resid = sqrt(red_sh[0]) / normb;
+        if (resid <= a.tol) {
+            status = 0;
+            it_done = 0;
+            goto done;
+        }
+        // w0 = A M r0 (generation 2 of r, w, s), gamma0 = (r0, u0), delta0 = (w0, u0)
+        double p_g = 0.0, p_d = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
+            wr[k] = lds_row(lr[k], [&](int q) { return XR[q]; });
+            sr[k] = 0.0;
+            pr[k] = 0.0;
+            tx_store(TR, 2, r, rr[k]);
+            tx_store(TW, 2, r, wr[k]);
+            tx_store(TS, 2, r, 0.0);
+            p_g += rr[k] * ur[k];
+            p_d += wr[k] * ur[k];
+        }
+        double unused = 0.0;
+        block_sum3<PB>(p_g, p_d, unused);
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (lo[k] >= 0) {
+                XW[lo[k]] = wr[k];
+                XS[lo[k]] = 0.0;
+            }
+        if (threadIdx.x == 0) slot_publish3(a.slots, 2, p_g, p_d, 0.0);
+    }
+    {
+        double gamma = 0.0, alpha = 0.0, beta = 0.0;
+        if (a.probe && threadIdx.x == 0) a.probe[blockIdx.x * 4 + 0] = wall_clock64() - t_start;
+        for (int it = 0;; ++it) {
+            const uint32_t gd = (uint32_t)it + 2;
+            halo(gd, true);
+            if (!collect(gd)) { status = 2; break; }
+            if (it == 0) {
+                gamma = red_sh[0];
+                alpha = red_sh[0] / red_sh[1];
+                beta = 0.0;
+            } else {
+                resid = sqrt(red_sh[2]) / normb;
+                if (resid <= a.tol) {
+                    status = 0;
+                    it_done = it;
+                    break;
+                }
+                const double gamma_new = red_sh[0];
+                beta = gamma_new / gamma;
+                alpha = gamma_new / (red_sh[1] - beta * gamma_new / alpha);
+                gamma = gamma_new;
+            }
+            if (it == a.max_iter) break;
+            double p_g2 = 0.0, p_d2 = 0.0, p_rr2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                pr[k] = ur[k] + beta * pr[k];
+                sr[k] = wr[k] + beta * sr[k];
+                xr[k] += alpha * pr[k];
+                rr[k] = rr[k] - alpha * sr[k];
+                ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
+                wr[k] = lds_row(lr[k], [&](int q) { return XR[q] - alpha * (XW[q] + beta * XS[q]); });
+                tx_store(TR, gd + 1, r, rr[k]);
+                tx_store(TW, gd + 1, r, wr[k]);
+                tx_store(TS, gd + 1, r, sr[k]);
+                p_g2 += rr[k] * ur[k];
+                p_d2 += wr[k] * ur[k];
+                p_rr2 += rr[k] * rr[k];
+            }
+            block_sum3<PB>(p_g2, p_d2, p_rr2);
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (lo[k] >= 0) {
+                    XR[lo[k]] = rr[k];
+                    XW[lo[k]] = wr[k];
+                    XS[lo[k]] = sr[k];
+                }
+            if (threadIdx.x == 0) slot_publish3(a.slots, gd + 1, p_g2, p_d2, p_rr2);
+        }
+    }
+done:
+    if (status == 0 || status == 1) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k] && it_done > 0) a.x[r0 + lr[k]] = xr[k];
+    }
+    if (a.probe && threadIdx.x == 0) {
+        const uint64_t total = wall_clock64() - t_start;
+        a.probe[blockIdx.x * 4 + 1] = total - t_wait - a.probe[blockIdx.x * 4 + 0];
+        a.probe[blockIdx.x * 4 + 2] = t_wait;
+        a.probe[blockIdx.x * 4 + 3] = total;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.out_iter = it_done;
+        *a.out_resid = resid;
+        *a.out_status = status;
+    }
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // Vector-payload synchronisation: up to KMAX partial sums per block.  The
@@ -701,22 +1246,23 @@ done:
 // ---------------------------------------------------------------------------
 constexpr int KMAX = 64;
 
-template <int BLOCK>
+template <int BLOCK, bool FENCES = true>
 __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t g, const double *mine, int K,
                                        double *out_sh) {
+    constexpr bool COH = !FENCES;
     static_assert(BLOCK == 512, "the chunked reduction maps 8 waves to 8 chunks of blocks");
     __shared__ int ok_sh;
     __shared__ double chunk_sh[BLOCK / 64][KMAX];
     const int nb = gridDim.x, par = g & 1;
     double *P = parts + (size_t)par * nb * KMAX;
-    if (threadIdx.x < K) P[(size_t)blockIdx.x * KMAX + threadIdx.x] = mine[threadIdx.x];
+    if (threadIdx.x < K) xstore<COH>(&P[(size_t)blockIdx.x * KMAX + threadIdx.x], mine[threadIdx.x]);
     wait_own_stores();
     __syncthreads();
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         const uint64_t tag = (uint64_t)g << 32;
         if (lane == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            if (FENCES) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __hip_atomic_store(&slots[blockIdx.x].w[par][0], tag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         int ok = 1;
@@ -738,7 +1284,7 @@ __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t 
                 }
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (FENCES) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const bool all_ok = __ballot(ok) == __ballot(1);
         if (lane == 0) {
             ok_sh = all_ok ? 1 : 0;
@@ -759,7 +1305,7 @@ __device__ inline bool sync_reduce_vec(SyncSlot *slots, double *parts, uint32_t 
 #pragma unroll
                 for (int q = 0; q < BATCH; ++q) {
                     const int b = ch + NCH * (q0 + q);
-                    v[q] = b < nb ? P[(size_t)b * KMAX + kk] : 0.0;
+                    v[q] = b < nb ? xload<COH>(&P[(size_t)b * KMAX + kk]) : 0.0;
                 }
 #pragma unroll
                 for (int q = 0; q < BATCH; ++q)
@@ -844,6 +1390,7 @@ struct GmresArgs {
 // SELL-64 layout of a CSR matrix partitioned into blocks of `per` consecutive rows: the local rows
 // of a block form slices of 64; entry k of local row l sits at off[slice] + 64 k + (l % 64), so the
 // thread-per-row loads of entry k are coalesced over a wave.  width[s] = 64 * longest row.
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_sell_width(int64_t n, int64_t per, int spb, int64_t nslices, const int32_t *rp, int32_t *width) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslices) return;
@@ -857,7 +1404,9 @@ __global__ void k_sell_width(int64_t n, int64_t per, int spb, int64_t nslices, c
     }
     width[s] = 64 * w;
 }
+#endif
 
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, const int32_t *col,
                             const double *val, const double *diag, const int32_t *off, int32_t *scol,
                             double *sval, int32_t *pos) {
@@ -872,26 +1421,35 @@ __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, 
         if (pos) pos[e] = (int32_t)(base + 64 * (int64_t)k);
     }
 }
+#endif
 
 // new values on an unchanged pattern: one lane per CSR entry through the stored SELL slots
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_sell_vals(int64_t nnz, const int32_t *pos, const int32_t *col, const double *val,
                             const double *diag, double *sval) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     sval[pos[e]] = diag ? val[e] / diag[col[e]] : val[e];
 }
+#endif
 
 // a_rc / d_c for right preconditioning (the same quotient the LDS slice loader forms)
+#ifndef DHIP_SOLVER_KERNELS
 __global__ void k_scale_vals(int64_t nnz, const int32_t *col, const double *val, const double *diag, double *out) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
     out[e] = val[e] / diag[col[e]];
 }
+#endif
 
 constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG, one row per thread)
 
-template <int RPT, int KR>
-__global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
+template <int RPT, int KR, bool COH>
+__global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
     extern __shared__ __align__(16) unsigned char lds[];
     const uint64_t t_start = a.probe ? wall_clock64() : 0;
     uint64_t t_sync = 0, t_spmv = 0;
@@ -952,7 +1510,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     uint32_t gen = 0;
     auto syncv = [&](int K, double *out) {
         const uint64_t t0 = a.probe ? wall_clock64() : 0;
-        const bool ok = sync_reduce_vec<PB>(a.slots, a.parts, ++gen, msh, K, out);
+        const bool ok = sync_reduce_vec<PB, !COH>(a.slots, a.parts, ++gen, msh, K, out);
         if (a.probe) t_sync += wall_clock64() - t0;
         return ok;
     };
@@ -995,7 +1553,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
     auto vbasis = [&](int q, int64_t l) { return q < KL ? vl[(size_t)q * nrb + l] : a.V[(size_t)q * n + r0 + l]; };
     auto vstore = [&](int q, int64_t r, double v, bool mine) {
         if (mine) {
-            a.V[(size_t)q * n + r] = v;
+            xstore<COH>(&a.V[(size_t)q * n + r], v);
             if (q < KL) vl[(size_t)q * nrb + (r - r0)] = v;
         }
         if constexpr (REGV) {
@@ -1064,7 +1622,8 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         for (int k = 0; k < RPT; ++k) {
             if (!own[k]) continue;
             const int64_t r = r0 + lr[k];
-            const double ax = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
+            const double ax =
+                row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return xload<COH>(&a.x[c]); });
             double rv = a.b[r] - ax;
             double bv = a.b[r];
             if (!right) {
@@ -1159,7 +1718,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
 #pragma unroll
             for (int k = 0; k < RPT; ++k) {
                 if (!own[k]) continue;
-                double w = spmv_row(lr[k], [&](int32_t c) { return vi[c]; });
+                double w = spmv_row(lr[k], [&](int32_t c) { return xload<COH>(&vi[c]); });
                 if (!right) w = prec(k, w);
                 wr[k] = w;
             }
@@ -1236,7 +1795,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a) {
         // restart: publish x, r = b - A x (left: M r)
 #pragma unroll
         for (int k = 0; k < RPT; ++k)
-            if (own[k]) a.x[r0 + lr[k]] = xr[k];
+            if (own[k]) xstore<COH>(&a.x[r0 + lr[k]], xr[k]);
         if (!syncv(0, red_sh)) { status = 2; goto done; }
         {
             double p_rr, p_bb;
@@ -1270,6 +1829,7 @@ done:
         *a.out_status = status;
     }
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // persistent PCG with FP32 inner iterations and FP64 iterative refinement
@@ -1300,7 +1860,11 @@ struct PcgMixArgs {
 };
 
 template <int RPT>
-__global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
+__global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
     const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
@@ -1430,5 +1994,6 @@ __global__ void __launch_bounds__(PB) k_pcg_mixed(PcgMixArgs a) {
         *a.out_outer = outer;
     }
 }
+#endif
 
 }  // namespace dhip
