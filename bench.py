@@ -360,6 +360,10 @@ def band_leg(ctx, mesh, phi, coef, lc, width, reps=5, with_cpu=True):
     return out
 
 
+# the FP64 PCG kernel each exchange runs (rocprofv3 names it so in its summaries)
+PCG_KERNEL = {0: "k_pcg64_sr", 1: "k_pcg64_sr", 2: "k_pcg64_tx"}
+
+
 def measured_traffic(kernel, rows, ndof):
     """HBM bytes per launch of `kernel` from the committed PMC summary (FETCH_SIZE + WRITE_SIZE passes,
     each scaled by a known-size copy), when it was measured at this block geometry and system size."""
@@ -453,6 +457,9 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-copies", type=int, default=80)
+    ap.add_argument("--exchange", type=int, default=2, choices=(0, 1, 2),
+                    help="with --rows-per-block: the persistent solvers' exchange (0 fenced, 1 coherent atomics, "
+                         "2 tagged records with the halo in LDS)")
     ap.add_argument("--rows-per-block", type=int, default=0,
                     help="fix the persistent solver's rows per block instead of tuning it (profiling runs)")
     ap.add_argument("--variants", default="fp64,fp64_classic,fp32ir",
@@ -509,15 +516,16 @@ def main():
         ctx.free(dst)
         log(f"[bench] copy kernel: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
     if args.rows_per_block:
-        rows_best = args.rows_per_block
+        rows_best, exch_best = args.rows_per_block, args.exchange
         ctx.tune("pcg_rows_per_block", rows_best)
+        ctx.tune("coherent_exchange", exch_best)
     else:
-        # block geometry of the persistent solver, tuned on this matrix outside the timed region
+        # block geometry and exchange of the persistent solver, tuned on this matrix outside the timed region
         A0 = ctx.csr_from_assembly(D.OP_COMB)
         rhs0 = ctx.device_arrays()[3]
-        rows_best, tune_t = tune.tune_pcg(ctx, A0, rhs0, x_dev, zero_dev, ndof, tol=TOL)
+        (rows_best, exch_best), tune_t = tune.tune_pcg(ctx, A0, rhs0, x_dev, zero_dev, ndof, tol=TOL)
         A0.close()
-        log(f"[bench] tuned pcg_rows_per_block={rows_best} {tune_t}")
+        log(f"[bench] tuned pcg_rows_per_block={rows_best} coherent_exchange={exch_best} {tune_t}")
 
     forms = {"fp64": D.FP64, "fp64_classic": D.FP64_CLASSIC, "fp32ir": D.FP32_IR}
     variants = [v for v in args.variants.split(",") if v]
@@ -617,15 +625,16 @@ def main():
         "config": {"workload": "P1 TraceFEM LB+mass+h*normal-stab, sphere |x|^2-1 on [-2,2]^3, "
                                f"{n}^3 Kuhn cells, assembly + Jacobi-PCG to rel. residual 1e-8",
                    "cells": n, "cut_tets": ncut, "dofs": ndof, "nnz": nnz,
-                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_rows_per_block": rows_best},
+                   "pcg_iters_fp64": results["fp64"]["iters"], "pcg_rows_per_block": rows_best,
+                   "pcg_exchange": {0: "fenced", 1: "coherent", 2: "tagged_lds_halo"}[exch_best]},
         "variants": {k: {"s_per_solve": r["s_per_solve"], "iters": r["iters"]} for k, r in results.items()},
         "err_vs_fp64_oracle": err or None,
         "spmv_hbm_gbs": spmv["gbs"] if spmv else None,
         "spmv": {k: spmv[k] for k in ("nnz", "rows", "csr_mb", "avg_us", "copies")} if spmv else None,
         "kernels_us": {k: round(v2["avg_us"], 2) for k, v2 in results["fp64"]["kernels"].items()},
         "classify_gbs": cls_bytes / (cls["avg_us"] * 1e-6) / 1e9,
-        "roofline": {"kernel": "pcg64", "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("pcg64", rows_best, ndof),
+        "roofline": {"kernel": PCG_KERNEL[exch_best], "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(PCG_KERNEL[exch_best], rows_best, ndof),
                      "bytes_per_iter": bytes_per_iter, "iters": iters, "avg_launch_us": pk["avg_us"]},
         "learned": ch,
         "batch": bt,

@@ -1,10 +1,13 @@
-"""Autotuner of the persistent solvers' block geometry.
+"""Autotuner of the persistent solvers' block geometry and exchange, and of the batched solve.
 
 The persistent PCG/GMRES kernels give each block a contiguous range of rows.
 More rows per block make a block's gather phase longer; more blocks make every
-grid synchronisation (one slot per block to poll) longer.  `tune_pcg` times one
-solve per candidate on the caller's matrix and leaves the fastest setting on the
-context (`dhip_tune("pcg_rows_per_block", ...)`).
+grid synchronisation (one slot per block to poll) longer.  The blocks exchange
+their rows' values either as coherent atomics behind a slot synchronisation, or
+as generation-tagged records gathered once per distinct column into LDS.
+`tune_pcg` times solves of every (rows per block, exchange) candidate on the
+caller's matrix and leaves the fastest setting on the context
+(`dhip_tune("pcg_rows_per_block", ...)`, `dhip_tune("coherent_exchange", ...)`).
 """
 import time
 
@@ -12,24 +15,29 @@ import numpy as np
 
 from . import dhip as D
 
-CANDIDATES = (128, 192, 256, 384, 512)
+CANDIDATES = (128, 192, 256, 320, 384, 512)
+EXCHANGES = (1, 2)   # coherent atomics, tagged records with the halo in LDS
 
 
-def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, candidates=CANDIDATES, reps=2):
-    """Returns (best_rows_per_block, {rows: seconds}).  b_dev/x_dev/zero_dev are device pointers;
-    x is reset from zero_dev before each solve."""
+def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, candidates=CANDIDATES,
+             exchanges=EXCHANGES, reps=2):
+    """Returns ((rows_per_block, exchange), {(rows, exchange): seconds}).  b_dev/x_dev/zero_dev are
+    device pointers; x is reset from zero_dev before each solve."""
     timings = {}
-    for rows in candidates:
-        ctx.tune("pcg_rows_per_block", rows)
-        best = np.inf
-        for _ in range(reps):
-            ctx.copy_d2d(x_dev, zero_dev, 8 * n)
-            t0 = time.perf_counter()
-            ctx.pcg(A, b_dev, x_dev, tol=tol, precision=precision, where=D.DEVICE)
-            best = min(best, time.perf_counter() - t0)
-        timings[rows] = best
+    for exch in exchanges:
+        ctx.tune("coherent_exchange", exch)
+        for rows in candidates:
+            ctx.tune("pcg_rows_per_block", rows)
+            best = np.inf
+            for _ in range(reps):
+                ctx.copy_d2d(x_dev, zero_dev, 8 * n)
+                t0 = time.perf_counter()
+                ctx.pcg(A, b_dev, x_dev, tol=tol, precision=precision, where=D.DEVICE)
+                best = min(best, time.perf_counter() - t0)
+            timings[(rows, exch)] = best
     choice = min(timings, key=timings.get)
-    ctx.tune("pcg_rows_per_block", choice)
+    ctx.tune("pcg_rows_per_block", choice[0])
+    ctx.tune("coherent_exchange", choice[1])
     return choice, timings
 
 
