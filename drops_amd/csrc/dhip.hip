@@ -1778,6 +1778,10 @@ struct dhip_ch {
     DevBuf eptr, econt, rptr, rcont, erow;
     dhip_csr K;
     bool prepared = false;
+    // BDF loop: the solver's vector [mu; c], the BDF1 concentration, c_{n-1}, c_e, c_r, error parts
+    DevBuf bdf_xl, bdf_c1, bdf_cprev, bdf_ce, bdf_cr, bdf_part;
+    int64_t bdf_step = 0;
+    double bdf_t = 0.0, bdf_dt = 0.0, bdf_e = 0.0;
 };
 
 static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
@@ -1790,36 +1794,58 @@ static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
                       &h->col, &h->mass, &h->lb, &h->stab, &h->geom, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
                       &h->rhs, &h->x, &h->hist[0], &h->hist[1], &h->hist[2], &h->feat, &h->pred, &h->xstart,
-                      &h->eptr, &h->econt, &h->rptr, &h->rcont, &h->erow};
+                      &h->eptr, &h->econt, &h->rptr, &h->rcont, &h->erow, &h->bdf_xl, &h->bdf_c1,
+                      &h->bdf_cprev, &h->bdf_ce, &h->bdf_cr, &h->bdf_part};
     for (DevBuf *b : bufs) b->release();
 }
 
-static int ch_prepare_impl(dhip_ch *h) {
+// The step system from the concentration conc; conc_r != nullptr: a BDF2 step with rhs3 = M conc_r.
+static int ch_prepare_with(dhip_ch *h, const double *conc, const double *conc_r, const ChCoef &k) {
     dhip_ctx *c = h->ctx;
     const int64_t N = h->ncut, R = h->ndof;
+    const int32_t bdf = conc_r != nullptr;
     if (h->space == 2) {
         ChTetP2Args ta{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->massloc.as<double>(), h->geom.as<double>(),
-                       h->x.as<double>() + R, h->tet.as<double>()};
+                       conc, conc_r, h->tet.as<double>()};
         if (N > 0) DHIP_LAUNCH(c, "ch_tet_p2", k_ch_tet_p2, blocks_for(N, CHTET2_WAVES), 64 * CHTET2_WAVES, 0, ta);
         ChEntryP2Args ea{h->nnz, h->erow.as<int32_t>(), h->rowptr.as<int32_t>(), h->col.as<int32_t>(),
                          h->eptr.as<int32_t>(), h->econt.as<int32_t>(), h->tet.as<double>(), h->mass.as<double>(),
-                         h->stab.as<double>(), h->bval.as<double>(), h->pcdiag.as<double>(), h->k};
+                         h->stab.as<double>(), h->bval.as<double>(), h->pcdiag.as<double>(), k};
         if (h->nnz > 0) DHIP_LAUNCH(c, "ch_entries_p2", k_ch_entries_p2, blocks_for(h->nnz, 256), 256, 0, ea);
-        ChRhsP2Args ra{R, h->rptr.as<int32_t>(), h->rcont.as<int32_t>(), h->tet.as<double>(), h->rhs.as<double>(), h->k};
+        ChRhsP2Args ra{R, h->rptr.as<int32_t>(), h->rcont.as<int32_t>(), h->tet.as<double>(), h->rhs.as<double>(), k,
+                       bdf};
         if (R > 0) DHIP_LAUNCH(c, "ch_rhs_p2", k_ch_rhs_p2, blocks_for(R, 256), 256, 0, ra);
         h->prepared = true;
         return DHIP_OK;
     }
     ChTetArgs ta{N, h->flags.as<uint8_t>(), h->dofs.as<int32_t>(), h->massloc.as<double>(), h->geom.as<double>(),
-                 h->x.as<double>() + R, h->tet.as<double>()};
+                 conc, conc_r, h->tet.as<double>()};
     if (N > 0) DHIP_LAUNCH(c, "ch_tet", k_ch_tet, blocks_for(N, 256), 256, 0, ta);
     ChRowArgs ra{h->mesh, h->cut.as<int64_t>(), h->cellinfo.as<uint32_t>(), h->flags.as<uint8_t>(),
                  h->dofs.as<int32_t>(), h->rowowner.as<int32_t>(), R, N, h->tet.as<double>(),
                  h->rowptr.as<int32_t>(), h->mass.as<double>(), h->stab.as<double>(), h->bval.as<double>(),
-                 h->pcdiag.as<double>(), h->rhs.as<double>(), h->k};
+                 h->pcdiag.as<double>(), h->rhs.as<double>(), k, bdf};
     if (R > 0) DHIP_LAUNCH(c, "ch_rows", k_ch_rows, blocks_for(R, 128), 128, 0, ra);
     h->prepared = true;
     return DHIP_OK;
+}
+
+static int ch_prepare_impl(dhip_ch *h) { return ch_prepare_with(h, h->x.as<double>() + h->ndof, nullptr, h->k); }
+
+// the per-step coefficients of a time step dt
+static ChCoef ch_coef(const dhip_ch_params_t &p, double dt) {
+    ChCoef k{};
+    const double eps = p.epsilon, rho = p.rho;
+    k.sdt = p.sigma * dt;
+    k.rdt = rho * dt;
+    k.dtr = dt * rho;
+    k.dt = dt;
+    k.S = p.S;
+    k.e2 = eps * eps;
+    k.re2 = rho * eps * eps;
+    k.e2r = eps * eps * rho;
+    k.meps = -eps;
+    return k;
 }
 
 // The current state becomes the newest history level (the ring keeps three).
@@ -1846,6 +1872,8 @@ static int ch_solve_impl(dhip_ch *h, int32_t *iters, double *resid) {
     rc = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), h->x.as<double>(), m, &it, &tol,
                     h->p.rel, 0);
     h->prepared = false;
+    h->bdf_step = 0;
+    h->bdf_e = 0.0;
     if (iters) *iters = it;
     if (resid) *resid = tol;
     return rc;
@@ -1935,16 +1963,7 @@ static int ch_create(dhip_ctx *c, int32_t zero_policy, const dhip_ch_params_t *p
     h->ndof = c->ndof;
     h->nnz = c->nnz;
     const int64_t N = h->ncut, R = h->ndof, Z = h->nnz;
-    const double eps = p->epsilon, dt = p->dt, rho = p->rho;
-    h->k.sdt = p->sigma * dt;
-    h->k.rdt = rho * dt;
-    h->k.dtr = dt * rho;
-    h->k.dt = dt;
-    h->k.S = p->S;
-    h->k.e2 = eps * eps;
-    h->k.re2 = rho * eps * eps;
-    h->k.e2r = eps * eps * rho;
-    h->k.meps = -eps;
+    h->k = ch_coef(*p, p->dt);
     auto fail = [&](int code) {
         ch_release(h);
         delete h;
@@ -2043,6 +2062,8 @@ int dhip_ch_set_state(dhip_ch_t *h, const double *c_in, const double *mu_in, int
     DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     h->prepared = false;
     h->nhist = 0;
+    h->bdf_step = 0;
+    h->bdf_t = h->bdf_dt = h->bdf_e = 0.0;
     return DHIP_OK;
 }
 
@@ -2116,6 +2137,135 @@ int dhip_ch_step(dhip_ch_t *h, int32_t nsteps, int32_t *iters, double *resid) {
             return rc;
         }
     }
+    return flush_profile(c);
+}
+
+}  // extern "C"
+
+// F(e, dt), surfphasesep/surfphasesep.cpp:117-123
+static double bdf_step_size(const dhip_bdf_params_t &p, double e, double dt) {
+    if (!e) return p.step_size;
+    const double f = p.rho * std::sqrt(p.tol / e) * dt;
+    if (f > p.max_step) return p.max_step;
+    if (f < p.min_step) return p.min_step;
+    return f;
+}
+
+// One step of the loop at surfphasesep/surfphasesep.cpp:397-491 (see dhip_ch_bdf_step).
+static int ch_bdf_one(dhip_ch *h, const dhip_bdf_params_t &bp, dhip_bdf_stat_t *stat) {
+    dhip_ctx *c = h->ctx;
+    const int64_t R = h->ndof;
+    const size_t vb = sizeof(double) * R;
+    int m = 0;
+    int rc = gmres_restart(c, h->p.restart, h->p.max_iter, &m);
+    if (rc) return rc;
+    const unsigned nb = std::max(1u, std::min(1024u, blocks_for(R, KB)));
+    DHIP_HIPCHK(c, h->bdf_xl.ensure(2 * vb + 8));
+    DHIP_HIPCHK(c, h->bdf_c1.ensure(vb + 8));
+    DHIP_HIPCHK(c, h->bdf_cprev.ensure(vb + 8));
+    DHIP_HIPCHK(c, h->bdf_ce.ensure(vb + 8));
+    DHIP_HIPCHK(c, h->bdf_cr.ensure(vb + 8));
+    DHIP_HIPCHK(c, h->bdf_part.ensure(sizeof(double) * (2 * nb + 2)));
+    double *xl = h->bdf_xl.as<double>(), *c1 = h->bdf_c1.as<double>(), *cn = h->x.as<double>() + R;
+    double *part = h->bdf_part.as<double>();
+    // the solver's vector: the reference keeps one lhs across solves, equal to the state between steps
+    DHIP_HIPCHK(c, hipMemcpyAsync(xl, h->x.p, 2 * vb, hipMemcpyDeviceToDevice, c->stream));
+    const int64_t i = h->bdf_step + 1;
+    const double t_prev = h->bdf_t, dt_prev = h->bdf_dt;
+    double dt = h->bdf_dt, e = h->bdf_e, resid = 0.0;
+    int32_t tries = 0, it1 = 0, it2 = 0;
+    auto solve = [&](int32_t *acc) {
+        int32_t it = h->p.max_iter;
+        double tl = h->p.tol;
+        const int r = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), xl, m, &it, &tl, h->p.rel, 0);
+        *acc += it;
+        resid = tl;
+        return r;
+    };
+    for (;;) {
+        if (bp.max_tries > 0 && tries == bp.max_tries) {
+            rc = set_error(c, DHIP_E_NOT_CONVERGED, "BDF step %lld: error %g above %g after %d attempts",
+                           (long long)i, e, bp.tol, tries);
+            break;
+        }
+        ++tries;
+        if (i == 1 || bp.adaptive) dt = bdf_step_size(bp, e, dt);
+        // BDF1: the implicit-Euler system of step dt at c_n
+        rc = ch_prepare_with(h, cn, nullptr, ch_coef(h->p, dt));
+        if (!rc) rc = solve(&it1);
+        if (rc) break;
+        DHIP_HIPCHK(c, hipMemcpyAsync(c1, xl + R, vb, hipMemcpyDeviceToDevice, c->stream));
+        if (i == 1) break;
+        const double r = dt / dt_prev;
+        const double dte = (1. + r) * dt / (1. + 2. * r);
+        const double w1 = 1. + r, w2 = -r, w3 = (1. + r) * (1. + r) / (1. + 2. * r), w4 = -(r * r) / (1. + 2. * r);
+        DHIP_LAUNCH(c, "ch_bdf_combine", k_ch_bdf_combine, blocks_for(R, KB), KB, 0, R, (const double *)cn,
+                    (const double *)h->bdf_cprev.as<double>(), w1, w2, w3, w4, h->bdf_ce.as<double>(),
+                    h->bdf_cr.as<double>());
+        rc = ch_prepare_with(h, h->bdf_ce.as<double>(), h->bdf_cr.as<double>(), ch_coef(h->p, dte));
+        if (!rc) rc = solve(&it2);
+        if (rc) break;
+        DHIP_LAUNCH(c, "ch_mnorm", k_ch_mnorm, nb, KB, 0, R, (const int32_t *)h->rowptr.as<int32_t>(),
+                    (const int32_t *)h->col.as<int32_t>(), (const double *)h->mass.as<double>(), (const double *)c1,
+                    (const double *)(xl + R), part, part + nb);
+        DHIP_LAUNCH(c, "ch_mnorm_reduce", k_reduce, 1, KB, 0, (const double *)part, (int)nb, part + 2 * nb);
+        DHIP_LAUNCH(c, "ch_mnorm_reduce", k_reduce, 1, KB, 0, (const double *)(part + nb), (int)nb,
+                    part + 2 * nb + 1);
+        double dc[2];
+        DHIP_HIPCHK(c, hipMemcpyAsync(dc, part + 2 * nb, sizeof dc, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        e = std::sqrt(dc[0] / dc[1]);
+        if (!(bp.adaptive && e > bp.tol && bp.min_step < dt && dt < bp.max_step)) break;
+    }
+    if (stat) {
+        stat->step = rc ? h->bdf_step : i;
+        stat->t = rc ? t_prev : t_prev + dt;
+        stat->dt = dt;
+        stat->e = e;
+        stat->tries = tries;
+        stat->iters_bdf1 = it1;
+        stat->iters_bdf2 = it2;
+        stat->reserved = 0;
+        stat->resid = resid;
+    }
+    h->prepared = false;
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipMemcpyAsync(h->bdf_cprev.p, cn, vb, hipMemcpyDeviceToDevice, c->stream));
+    DHIP_HIPCHK(c, hipMemcpyAsync(h->x.p, xl, 2 * vb, hipMemcpyDeviceToDevice, c->stream));
+    h->bdf_step = i;
+    h->bdf_t = t_prev + dt;
+    h->bdf_dt = dt;
+    h->bdf_e = e;
+    h->nhist = 0;
+    return DHIP_OK;
+}
+
+extern "C" {
+
+int dhip_ch_bdf_step(dhip_ch_t *h, const dhip_bdf_params_t *p, int32_t nsteps, dhip_bdf_stat_t *stats) {
+    if (!h || !p || nsteps < 0) return DHIP_E_ARG;
+    dhip_ctx *c = h->ctx;
+    if (!(p->step_size > 0.0) || !(p->tol > 0.0) || !(p->rho > 0.0) || !(p->min_step >= 0.0) ||
+        !(p->max_step > 0.0) || p->max_tries < 0)
+        return set_error(c, DHIP_E_ARG, "invalid BDF parameters");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    for (int32_t s = 0; s < nsteps; ++s) {
+        if (h->ndof == 0) {
+            // no interface: the loop still advances its time
+            const int64_t i = h->bdf_step + 1;
+            if (i == 1 || p->adaptive) h->bdf_dt = bdf_step_size(*p, h->bdf_e, h->bdf_dt);
+            h->bdf_step = i;
+            h->bdf_t += h->bdf_dt;
+            if (stats) stats[s] = dhip_bdf_stat_t{i, h->bdf_t, h->bdf_dt, h->bdf_e, 1, 0, 0, 0, 0.0};
+            continue;
+        }
+        int rc = ch_bdf_one(h, *p, stats ? stats + s : nullptr);
+        if (rc) {
+            flush_profile(c);
+            return rc;
+        }
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
     return flush_profile(c);
 }
 

@@ -36,8 +36,8 @@ __device__ __forceinline__ double mobility_fun(double x) {
 // Potential_prime_function, surfactant/ifacetransp.cpp:3360-3364
 __device__ __forceinline__ double potential_prime(double x) { return (x - 1.) * x * (x - 0.5); }
 
-constexpr int CHT_N = 24;  // 16 mobility, 4 M c, 4 M f'(c)
-constexpr int CHT_MOB = 0, CHT_TD = 16, CHT_WL = 20;
+constexpr int CHT_N = 28;  // 16 mobility, 4 M c, 4 M f'(c), 4 M c_r (BDF2 steps only)
+constexpr int CHT_MOB = 0, CHT_TD = 16, CHT_WL = 20, CHT_TR = 24;
 constexpr int CHG_N = 40;  // T[k][ij]: 4 vertices x 10 lower-triangular (i, j) pairs
 
 struct ChGeomArgs {
@@ -125,12 +125,14 @@ struct ChTetArgs {
     const int32_t *dofs;     // [ncut][4], -1 where the vertex has no DOF
     const double *loc;       // local mass (alpha = 1) from k_local_p1
     const double *T;         // k_ch_geom
-    const double *conc;      // c on the DOFs
+    const double *conc;      // c on the DOFs (the extrapolated c_e of a BDF2 step)
+    const double *conc_r;    // BDF2: the c_r of the time-derivative right-hand side; nullptr otherwise
     double *out;             // SoA [CHT_N][ncut]
 };
 
 // Per step: the local mobility matrix from the vertex concentrations and the local products
-// M c, M f'(c) (LocalMatVecP1CL<LocalInterfaceMassP1CL>, surfactant/ifacetransp.h:800-828).
+// M c, M f'(c) (LocalMatVecP1CL<LocalInterfaceMassP1CL>, surfactant/ifacetransp.h:800-828);
+// a BDF2 step adds M c_r.
 __global__ void k_ch_tet(ChTetArgs a) {
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.ncut) return;
@@ -140,17 +142,19 @@ __global__ void k_ch_tet(ChTetArgs a) {
         for (int e = 0; e < CHT_N; ++e) O[e * N + c] = 0.;
         return;
     }
-    double xc[4], mv[4];
+    const bool bdf = a.conc_r != nullptr;
+    double xc[4], mv[4], xr[4];
     int32_t nb[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         nb[i] = a.dofs[c * 4 + i];
         xc[i] = nb[i] >= 0 ? a.conc[nb[i]] : 0.;
+        xr[i] = nb[i] >= 0 && bdf ? a.conc_r[nb[i]] : 0.;
         mv[i] = mobility_fun(xc[i]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        double vt = 0., vw = 0.;
+        double vt = 0., vw = 0., vr = 0.;
         if (nb[i] >= 0) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -158,10 +162,12 @@ __global__ void k_ch_tet(ChTetArgs a) {
                 const double mij = a.loc[(int64_t)(LOC_MASS + i * 4 + j) * N + c];
                 vt += mij * xc[j];
                 vw += mij * potential_prime(xc[j]);
+                vr += mij * xr[j];
             }
         }
         O[(CHT_TD + i) * N + c] = vt;
         O[(CHT_WL + i) * N + c] = vw;
+        if (bdf) O[(CHT_TR + i) * N + c] = vr;
     }
     int o = 0;
 #pragma unroll
@@ -203,6 +209,7 @@ struct ChRowArgs {
     double *pcdiag;         // [2 ndof]
     double *rhs;            // [2 ndof]
     ChCoef k;
+    int32_t bdf;            // rhs3 from the M c_r products of k_ch_tet
 };
 
 __global__ void k_ch_rows(ChRowArgs a) {
@@ -218,7 +225,7 @@ __global__ void k_ch_rows(ChRowArgs a) {
         cols[s] = 0;
         lm[s] = 0.0;
     }
-    double td = 0.0, wl = 0.0;
+    double td = 0.0, wl = 0.0, tr = 0.0;
     const int64_t N = a.ncut;
     for_incident_cut_tets(a.m, a.cellinfo, x1, x2, x3, [&](int64_t ci, int k, int li) {
         if (!(a.flags[ci] & 1)) return false;
@@ -237,6 +244,7 @@ __global__ void k_ch_rows(ChRowArgs a) {
         }
         td += a.tet[(int64_t)(CHT_TD + li) * N + ci];
         wl += a.tet[(int64_t)(CHT_WL + li) * N + ci];
+        if (a.bdf) tr += a.tet[(int64_t)(CHT_TR + li) * N + ci];
         return false;
     });
     const int64_t e0 = a.rowptr[r];
@@ -254,8 +262,9 @@ __global__ void k_ch_rows(ChRowArgs a) {
         // Precond3.LinComb(1, Mass, sigma dt, LaplaceM, dt rho, Volume_stab)
         if (cols[s] == r) a.pcdiag[r] = 1.0 * (1.0 * a.mass[e] + a.k.sdt * lm[s]) + a.k.dtr * st;
     }
-    // rhs3 = theta (timeder + dt load3), rhs4 = -well + S timeder + load4 (no sources)
-    a.rhs[r] = 1.0 * (td + a.k.dt * 0.0);
+    // rhs3 = theta (timeder + dt load3), rhs4 = -well + S timeder + load4 (no sources); a BDF2
+    // step takes the time-derivative part of rhs3 from c_r
+    a.rhs[r] = 1.0 * ((a.bdf ? tr : td) + a.k.dt * 0.0);
     a.rhs[a.ndof + r] = ((-1.0) * wl + a.k.S * td) + 0.0;
 }
 
@@ -293,6 +302,44 @@ __global__ void k_ch_block(ChBlockArgs a) {
         a.bval[bot + len + k] = 1.0 * (a.k.e2 * l + a.k.S * m) + a.k.re2 * s;
         // Precond4.LinComb(1, Mass, eps^2, Laplace, eps^2 rho, Volume_stab)
         if (cj == r) a.pcdiag[n + r] = 1.0 * (1.0 * m + a.k.e2 * l) + a.k.e2r * s;
+    }
+}
+
+// BDF2 concentrations from c_n and c_{n-1} (surfphasesep/surfphasesep.cpp:451-456): the extrapolation
+// c_e = w1 c_n + w2 c_{n-1} and the time-derivative vector c_r = w3 c_n + w4 c_{n-1} of the system scaled
+// by dt_eff = 1/alpha.
+__global__ void __launch_bounds__(KB) k_ch_bdf_combine(int64_t n, const double *cn, const double *cp, double w1,
+                                                       double w2, double w3, double w4, double *ce, double *cr) {
+    const int64_t i = (int64_t)blockIdx.x * KB + threadIdx.x;
+    if (i >= n) return;
+    const double a = cn[i], b = cp[i];
+    ce[i] = w1 * a + w2 * b;
+    cr[i] = w3 * a + w4 * b;
+}
+
+// Parts of the step error e^2 = (d, M d) / (c2, M c2), d = c2 - c1 (surfphasesep/surfphasesep.cpp:
+// 477-479): per block the sums over its rows of d_r (M d)_r and c2_r (M c2)_r; k_reduce sums the parts.
+__global__ void __launch_bounds__(KB) k_ch_mnorm(int64_t n, const int32_t *rp, const int32_t *col, const double *m,
+                                                 const double *c1, const double *c2, double *part_d,
+                                                 double *part_c) {
+    double sd = 0., sc = 0.;
+    for (int64_t r = (int64_t)blockIdx.x * KB + threadIdx.x; r < n; r += (int64_t)gridDim.x * KB) {
+        double md = 0., mc = 0.;
+        for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+            const int32_t j = col[q];
+            const double x2 = c2[j];
+            md += m[q] * (x2 - c1[j]);
+            mc += m[q] * x2;
+        }
+        const double x2 = c2[r];
+        sd += (x2 - c1[r]) * md;
+        sc += x2 * mc;
+    }
+    sd = block_sum<KB>(sd);
+    sc = block_sum<KB>(sc);
+    if (threadIdx.x == 0) {
+        part_d[blockIdx.x] = sd;
+        part_c[blockIdx.x] = sc;
     }
 }
 

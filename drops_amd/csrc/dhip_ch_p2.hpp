@@ -23,8 +23,8 @@
 namespace dhip {
 
 constexpr int CHG2_N = 550;   // T: 10 nodal mobilities x 55 lower-triangular pairs
-constexpr int CHT2_N = 120;   // 100 local mobility entries, 10 M c, 10 M f'(c)
-constexpr int CHT2_TD = 100, CHT2_WL = 110;
+constexpr int CHT2_N = 130;   // 100 local mobility entries, 10 M c, 10 M f'(c), 10 M c_r (BDF2)
+constexpr int CHT2_TD = 100, CHT2_WL = 110, CHT2_TR = 120;
 
 struct ChGeomP2Args {
     MeshP m;
@@ -162,7 +162,8 @@ struct ChTetP2Args {
     const int32_t *dofs;     // [ncut][10]
     const double *massloc;   // P2 local mass, SoA [100][ncut]
     const double *T;         // [ncut][10][55]
-    const double *conc;      // c on the DOFs
+    const double *conc;      // c on the DOFs (the extrapolated c_e of a BDF2 step)
+    const double *conc_r;    // BDF2: the c_r of the time-derivative right-hand side; nullptr otherwise
     double *out;             // [ncut][CHT2_N]
 };
 
@@ -195,11 +196,13 @@ __global__ void __launch_bounds__(64 * CHTET2_WAVES) k_ch_tet_p2(ChTetP2Args a) 
         O[pi * 10 + pj] = lm;
         O[pj * 10 + pi] = lm;
     }
-    // LocalMatVecP1CL-style products M c and M f'(c) with the P2 local mass, lane i < 10
+    // LocalMatVecP1CL-style products M c and M f'(c) with the P2 local mass, lane i < 10; a BDF2
+    // step adds M c_r
     const int i = lane;
     if (i < 10) {
         const int64_t N = a.ncut;
-        double vt = 0., vw = 0.;
+        const bool bdf = a.conc_r != nullptr;
+        double vt = 0., vw = 0., vr = 0.;
         if (nb[i] >= 0) {
 #pragma unroll
             for (int j = 0; j < 10; ++j) {
@@ -207,10 +210,12 @@ __global__ void __launch_bounds__(64 * CHTET2_WAVES) k_ch_tet_p2(ChTetP2Args a) 
                 const double mij = a.massloc[(int64_t)(i * 10 + j) * N + c];
                 vt += mij * xc[j];
                 vw += mij * potential_prime(xc[j]);
+                if (bdf) vr += mij * a.conc_r[nb[j]];
             }
         }
         O[CHT2_TD + i] = vt;
         O[CHT2_WL + i] = vw;
+        if (bdf) O[CHT2_TR + i] = vr;
     }
 }
 
@@ -247,21 +252,24 @@ struct ChRhsP2Args {
     const double *tet;
     double *rhs;
     ChCoef k;
+    int32_t bdf;             // rhs3 from the M c_r products of k_ch_tet_p2
 };
 
 __global__ void k_ch_rhs_p2(ChRhsP2Args a) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.ndof) return;
-    double td = 0.0, wl = 0.0;
+    double td = 0.0, wl = 0.0, tr = 0.0;
     for (int32_t q = a.rptr[r]; q < a.rptr[r + 1]; ++q) {
         const int32_t key = a.rcont[q];
         const int32_t c = key / 10;
         const int li = key - 10 * c;
         td += a.tet[(int64_t)c * CHT2_N + CHT2_TD + li];
         wl += a.tet[(int64_t)c * CHT2_N + CHT2_WL + li];
+        if (a.bdf) tr += a.tet[(int64_t)c * CHT2_N + CHT2_TR + li];
     }
-    // rhs3 = theta (timeder + dt load3), rhs4 = -well + S timeder + load4 (no sources)
-    a.rhs[r] = 1.0 * (td + a.k.dt * 0.0);
+    // rhs3 = theta (timeder + dt load3), rhs4 = -well + S timeder + load4 (no sources); a BDF2
+    // step takes the time-derivative part of rhs3 from c_r
+    a.rhs[r] = 1.0 * ((a.bdf ? tr : td) + a.k.dt * 0.0);
     a.rhs[a.ndof + r] = ((-1.0) * wl + a.k.S * td) + 0.0;
 }
 

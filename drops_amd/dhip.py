@@ -40,7 +40,7 @@ EXPORTS = [
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
-    "dhip_assemble_p1_band", "dhip_pcg_noalloc",
+    "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -50,6 +50,19 @@ class ChParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("epsilon", C.c_double), ("rho", C.c_double), ("sigma", C.c_double),
                 ("S", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("rel", C.c_int32)]
+
+
+class BdfParams(C.Structure):
+    """dhip_bdf_params_t"""
+    _fields_ = [("step_size", C.c_double), ("tol", C.c_double), ("rho", C.c_double), ("min_step", C.c_double),
+                ("max_step", C.c_double), ("adaptive", C.c_int32), ("max_tries", C.c_int32)]
+
+
+class BdfStat(C.Structure):
+    """dhip_bdf_stat_t"""
+    _fields_ = [("step", C.c_int64), ("t", C.c_double), ("dt", C.c_double), ("e", C.c_double),
+                ("tries", C.c_int32), ("iters_bdf1", C.c_int32), ("iters_bdf2", C.c_int32),
+                ("reserved", C.c_int32), ("resid", C.c_double)]
 
 
 class BatchProblem(C.Structure):
@@ -139,6 +152,7 @@ def load(path=LIB_PATH):
                                  C.POINTER(VP), I64, I64]
     L.dhip_ch_solve.argtypes = [P, VP, C.c_int32, I32, D]
     L.dhip_ch_step.argtypes = [P, C.c_int32, I32, D]
+    L.dhip_ch_bdf_step.argtypes = [P, C.POINTER(BdfParams), C.c_int32, C.POINTER(BdfStat)]
     F = C.POINTER(C.c_float)
     L.dhip_mlp_create.argtypes = [P, C.POINTER(P)]
     L.dhip_mlp_destroy.argtypes = [P]
@@ -427,6 +441,16 @@ class CahnHilliard:
         res = np.zeros(nsteps)
         self.ctx.check(self.L.dhip_ch_step(self.h, nsteps, _p(its, C.c_int32), _p(res, C.c_double)))
         return its, res
+
+    def bdf_step(self, nsteps=1, step_size=0.01, tol=0.005, rho=0.9, min_step=0.0, max_step=float("inf"),
+                 adaptive=True, max_tries=0):
+        """nsteps of the adaptive BDF1/BDF2 loop (dhip_ch_bdf_step); one dict per step with
+        step, t, dt, e, tries, iters_bdf1, iters_bdf2, resid."""
+        prm = BdfParams(step_size, tol, rho, min_step, max_step, int(adaptive), max_tries)
+        st = (BdfStat * max(nsteps, 1))()
+        self.ctx.check(self.L.dhip_ch_bdf_step(self.h, C.byref(prm), nsteps, st))
+        keys = ("step", "t", "dt", "e", "tries", "iters_bdf1", "iters_bdf2", "resid")
+        return [{k: getattr(st[i], k) for k in keys} for i in range(nsteps)]
 
     def close(self):
         if self.h and self.ctx.h:

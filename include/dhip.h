@@ -307,6 +307,44 @@ int dhip_ch_solve(dhip_ch_t *ch, const double *x0, int32_t where, int32_t *iters
 /* nsteps x (prepare + solve from the previous step); iters/resid (may be NULL) get one entry per step. */
 int dhip_ch_step(dhip_ch_t *ch, int32_t nsteps, int32_t *iters, double *resid);
 
+/* ---- BDF1/BDF2 with the adaptive step size ----
+ * The time loop of surfphasesep/surfphasesep.cpp:396-491 on this stepper's system, replacing the
+ * reference's per-attempt reassembly + Belos solves.  Each attempt of step i (i = 1, 2, ...):
+ *   dt = F(e, dt) on the first step or when adaptive, F from :117-123 (step_size for e == 0, else
+ *        rho sqrt(tol/e) dt clamped to [min_step, max_step]);
+ *   BDF1: the implicit-Euler system of step dt from c_n, solved from the solver's previous vector;
+ *   i == 1: the BDF1 solution is taken (:446-450);
+ *   BDF2 (r = dt/dt_prev): mobility, f' and S M c at c_e = (1+r) c_n - r c_{n-1}, effective step
+ *        dt_eff = (1+r) dt / (1+2r) = 1/alpha and rhs3 = M c_r, c_r = ((1+r)^2 c_n - r^2 c_{n-1})/(1+2r)
+ *        (F_chi of :455 times dt_eff), solved from the BDF1 solution;
+ *   e = sqrt((d, M d) / (c2, M c2)), d = c2 - c1 with the assembled mass matrix (:477-479);
+ * repeated while adaptive && e > tol && min_step < dt < max_step (:483).  The last solve's [mu; c]
+ * becomes the state, c_n becomes c_{n-1}.  The system's unknowns are scaled as in dhip_ch_step
+ * (the time-derivative row times dt_eff).  The reference retries without bound: max_tries > 0
+ * ends a step after that many rejected attempts with DHIP_E_NOT_CONVERGED, state unchanged; a
+ * GMRES solve that does not converge ends the step the same way.  dhip_ch_set_state restarts the
+ * loop (step 0, t = 0, e = 0); dhip_ch_solve and dhip_ch_step, which keep the creation dt, make the
+ * next BDF step a first step again (step 0, e = 0, t kept). */
+typedef struct {
+    double step_size;   /* Time.StepSize */
+    double tol;         /* Time.Adaptive.Tol */
+    double rho;         /* Time.Adaptive.rho */
+    double min_step;    /* Time.Adaptive.MinStepSize (0: none) */
+    double max_step;    /* Time.Adaptive.MaxStepSize (+inf: none) */
+    int32_t adaptive;   /* Time.Adaptive.Use */
+    int32_t max_tries;  /* 0: no bound */
+} dhip_bdf_params_t;
+typedef struct {
+    int64_t step;       /* steps taken since the state was set (the reference's i) */
+    double t, dt, e;    /* time reached, accepted step, its error (0 until the first BDF2 solve) */
+    int32_t tries;      /* attempts of this step */
+    int32_t iters_bdf1, iters_bdf2;  /* GMRES iterations summed over the attempts */
+    int32_t reserved;
+    double resid;       /* last solve's residual */
+} dhip_bdf_stat_t;
+/* nsteps time steps; stats (may be NULL) get one entry per step taken. */
+int dhip_ch_bdf_step(dhip_ch_t *ch, const dhip_bdf_params_t *params, int32_t nsteps, dhip_bdf_stat_t *stats);
+
 /* ---- learned warm start (no reference counterpart: an addition of this library) ----
  * A per-DOF network 12 -> 64 -> 64 -> 64 -> 2 (tanh hidden layers, bf16 weights and layer inputs,
  * float32 accumulation on the matrix cores) predicts, for the step n -> n+1, a correction of the

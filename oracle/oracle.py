@@ -44,6 +44,23 @@ class Patch(C.Structure):
                 ("normal", (C.c_double * 3) * 16), ("ls", C.c_double * 10)]
 
 
+class BdfParams(C.Structure):
+    """orc_bdf_params_t (surfphasesep/surfphasesep.cpp:112-123)"""
+    _fields_ = [("step_size", C.c_double), ("tol", C.c_double), ("rho", C.c_double), ("min_step", C.c_double),
+                ("max_step", C.c_double), ("adaptive", C.c_int32), ("max_tries", C.c_int32)]
+
+
+class BdfState(C.Structure):
+    """orc_bdf_state_t"""
+    _fields_ = [("step", C.c_int64), ("t", C.c_double), ("dt", C.c_double), ("e", C.c_double)]
+
+
+class BdfStat(C.Structure):
+    """orc_bdf_stat_t"""
+    _fields_ = [("tries", C.c_int32), ("iters_bdf1", C.c_int32), ("iters_bdf2", C.c_int32), ("resid", C.c_double),
+                ("dt", C.c_double), ("e", C.c_double)]
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
 
@@ -94,6 +111,16 @@ def lib(kind="parity"):
                               C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
     L.orc_ch_step_p2.argtypes = [M, D, I32, C.c_int64, I64, I32, D, D, D, D, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_double, C.c_int32, I32, D, C.c_int32]
+    BP, BS, BT = C.POINTER(BdfParams), C.POINTER(BdfState), C.POINTER(BdfStat)
+    L.orc_bdf2_weights.argtypes = [C.c_double, C.c_double, D]
+    L.orc_bdf2_weights.restype = None
+    L.orc_bdf_step_size.argtypes = [BP, C.c_double, C.c_double]
+    L.orc_bdf_step_size.restype = C.c_double
+    L.orc_ch_system_bdf.argtypes = [C.c_int, M, D, I32, C.c_int64, I64, I32, D, D, D, D, D, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, I64, I32, D, D, D]
+    L.orc_ch_bdf_step.argtypes = [C.c_int, M, D, I32, C.c_int64, I64, I32, D, D, D, D, D, BS, BP, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_double,
+                                  C.c_int32, BT]
     L.orc_lattice_tables.argtypes = [I32, I32]
     L.orc_quad_tables.argtypes = [D, D, D, D]
     L.orc_classify_band.argtypes = [M, D, C.c_double, C.c_int, C.c_int64, I64, I64, C.POINTER(C.c_uint8)]
@@ -454,6 +481,80 @@ def ch_step(mesh, phi, dof_of_vertex, ndof, rowptr, col, mass, lb, stab, c, mu, 
     x, it, res, ok = gmres_diag(brp, bcol, bval, pcd, b, x0, restart=restart, max_iter=max_iter, tol=tol)
     return dict(mu=x[:ndof], c=x[ndof:], iters=it, res=res, ok=ok, lm=lm, timeder=timeder, well=well,
                 rhs=b, pcdiag=pcd, rowptr=brp, col=bcol, val=bval, x0=x0)
+
+
+def bdf2_weights(dt, dt_prev, kind="parity"):
+    """(dt_eff, w1, w2, w3, w4) of orc_bdf2_weights: c_e = w1 c_n + w2 c_{n-1}, c_r = w3 c_n + w4 c_{n-1}."""
+    w = np.zeros(5)
+    lib(kind).orc_bdf2_weights(dt, dt_prev, _ptr(w, C.c_double))
+    return tuple(w)
+
+
+def bdf_params(step_size, tol, rho=0.9, min_step=0.0, max_step=float("inf"), adaptive=True, max_tries=0):
+    return BdfParams(step_size, tol, rho, min_step, max_step, int(adaptive), max_tries)
+
+
+def bdf_step_size(params, e, dt, kind="parity"):
+    """F(e, dt) of surfphasesep/surfphasesep.cpp:117-123."""
+    return lib(kind).orc_bdf_step_size(C.byref(params), e, dt)
+
+
+def ch_system_bdf(mesh, phi, dof_of, ndof, rowptr, col, mass, lb, stab, c_e, c_r, dt_eff, eps, rho, sigma=1.0,
+                  S=1.0, kind="parity", space=1):
+    """BDF2 step system (orc_ch_system_bdf): mobility and f' from c_e, rhs3 = M c_r, effective step dt_eff."""
+    nnz = col.size
+    brp = np.zeros(2 * ndof + 1, np.int64)
+    bcol = np.zeros(4 * nnz, np.int32)
+    bval, rhs, pcd = np.zeros(4 * nnz), np.zeros(2 * ndof), np.zeros(2 * ndof)
+    f64 = lambda a: _ptr(np.ascontiguousarray(a, np.float64), C.c_double)
+    _check(lib(kind).orc_ch_system_bdf(space, C.byref(mesh), _ptr(phi, C.c_double), _ptr(dof_of, C.c_int32), ndof,
+                                       _ptr(np.ascontiguousarray(rowptr, np.int64), C.c_int64),
+                                       _ptr(np.ascontiguousarray(col, np.int32), C.c_int32), f64(mass), f64(lb),
+                                       f64(stab), f64(c_e), f64(c_r), dt_eff, eps, rho, sigma, S,
+                                       _ptr(brp, C.c_int64), _ptr(bcol, C.c_int32), _ptr(bval, C.c_double),
+                                       _ptr(rhs, C.c_double), _ptr(pcd, C.c_double)), "ch_system_bdf")
+    return brp, bcol, bval, rhs, pcd
+
+
+class BdfStepper:
+    """The adaptive BDF1/BDF2 time loop of surfphasesep/surfphasesep.cpp:396-491 on the step system
+    of the Cahn-Hilliard stepper (orc_ch_bdf_step).  x = [mu; c] is the accepted state."""
+
+    def __init__(self, mesh, phi, dof_of, ndof, r, c0, mu0, params, eps, rho, sigma=1.0, S=1.0, restart=50,
+                 max_iter=20000, tol=1e-10, rel=True, kind="parity", space=1):
+        self.args = (mesh, phi, dof_of, ndof, np.ascontiguousarray(r["rowptr"], np.int64),
+                     np.ascontiguousarray(r["colind"], np.int32), np.ascontiguousarray(r["mass"], np.float64),
+                     np.ascontiguousarray(r["lb"], np.float64), np.ascontiguousarray(r["stab"], np.float64))
+        self.n = ndof
+        self.x = np.concatenate([np.asarray(mu0, np.float64), np.asarray(c0, np.float64)])
+        self.cprev = np.zeros(ndof)
+        self.state = BdfState(0, 0.0, 0.0, 0.0)
+        self.params = params
+        self.phys = (eps, rho, sigma, S)
+        self.solver = (restart, max_iter, tol, int(rel))
+        self.kind, self.space = kind, space
+
+    @property
+    def c(self):
+        return self.x[self.n:]
+
+    @property
+    def mu(self):
+        return self.x[:self.n]
+
+    def step(self):
+        """One time step; returns dict(step, t, dt, e, tries, iters_bdf1, iters_bdf2, resid)."""
+        mesh, phi, dof, nd, rp, col, mass, lb, stab = self.args
+        st = BdfStat()
+        _check(lib(self.kind).orc_ch_bdf_step(self.space, C.byref(mesh), _ptr(phi, C.c_double),
+                                              _ptr(dof, C.c_int32), nd, _ptr(rp, C.c_int64),
+                                              _ptr(col, C.c_int32), _ptr(mass, C.c_double),
+                                              _ptr(lb, C.c_double), _ptr(stab, C.c_double),
+                                              _ptr(self.x, C.c_double), _ptr(self.cprev, C.c_double),
+                                              C.byref(self.state), C.byref(self.params), *self.phys,
+                                              *self.solver, C.byref(st)), "ch_bdf_step")
+        return dict(step=self.state.step, t=self.state.t, dt=self.state.dt, e=self.state.e, tries=st.tries,
+                    iters_bdf1=st.iters_bdf1, iters_bdf2=st.iters_bdf2, resid=st.resid)
 
 
 # ---------------------------------------------------------------- file formats (iostreams)

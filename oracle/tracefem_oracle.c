@@ -1727,17 +1727,29 @@ int orc_ch_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *do
  * no wind, no sources: block CSR (SparseMatBaseCL(A, B, C, D), num/spmat.h:719-751), right-hand
  * side [M c; -M f'(c) + S M c] and the block-Jacobi diagonal [diag Precond3; diag Precond4]
  * (:1567-1570), each LinComb evaluated in the reference's order. */
+/* c_r == NULL: the implicit-Euler step from c.  Otherwise a BDF2 step: mobility, M f'(c) and
+ * S M c from the extrapolation c (= c_e) and the time-derivative part of rhs3 = M c_r, with dt the
+ * effective step 1/alpha; the M c_r products are LocalMatVec accumulations like M c. */
 static int ch_system_sp(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
                         const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
-                        const double *stab, const double *c, double dt, double eps, double rho, double sigma,
-                        double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
+                        const double *stab, const double *c, const double *c_r, double dt, double eps, double rho,
+                        double sigma, double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs,
+                        double *pcdiag)
 {
     const int64_t nnz = rowptr[ndof];
     double *lm = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
     double *td = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
     double *wl = (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1));
-    if (!lm || !td || !wl) { free(lm); free(td); free(wl); return ORC_E_NOMEM; }
+    double *tr = c_r ? (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1)) : td;
+    double *lm2 = c_r ? (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1)) : lm;
+    double *wl2 = c_r ? (double *)malloc(sizeof(double) * (ndof > 0 ? ndof : 1)) : wl;
+    if (!lm || !td || !wl || !tr || !lm2 || !wl2) {
+        free(lm); free(td); free(wl);
+        if (c_r) { free(tr); free(lm2); free(wl2); }
+        return ORC_E_NOMEM;
+    }
     int err = ch_assemble_sp(space, m, phi, dof_of, ndof, rowptr, colind, c, ORC_ZERO_THROW, lm, td, wl);
+    if (!err && c_r) err = ch_assemble_sp(space, m, phi, dof_of, ndof, rowptr, colind, c_r, ORC_ZERO_THROW, lm2, tr, wl2);
     if (!err) {
         const double sdt = sigma * dt, rdt = rho * dt, dtr = dt * rho, e2 = eps * eps, re2 = rho * eps * eps,
                      e2r = eps * eps * rho, meps = -eps;
@@ -1763,12 +1775,13 @@ static int ch_system_sp(int space, const orc_mesh_t *m, const double *phi, const
                     pcdiag[ndof + i] = 1.0 * (1.0 * mass[e] + e2 * lb[e]) + e2r * stab[e];
                 }
             }
-            rhs[i] = 1.0 * (td[i] + dt * 0.0);
+            rhs[i] = 1.0 * (tr[i] + dt * 0.0);
             rhs[ndof + i] = ((-1.0) * wl[i] + S * td[i]) + 0.0;
         }
         brp[2 * ndof] = 4 * nnz;
     }
     free(lm); free(td); free(wl);
+    if (c_r) { free(tr); free(lm2); free(wl2); }
     return err;
 }
 
@@ -1777,8 +1790,8 @@ int orc_ch_system(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_
                   const double *stab, const double *c, double dt, double eps, double rho, double sigma, double S,
                   int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
 {
-    return ch_system_sp(1, m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, c, dt, eps, rho, sigma, S,
-                        brp, bcol, bval, rhs, pcdiag);
+    return ch_system_sp(1, m, phi, dof_of_vertex, ndof, rowptr, colind, mass, lb, stab, c, NULL, dt, eps, rho, sigma,
+                        S, brp, bcol, bval, rhs, pcdiag);
 }
 
 int orc_ch_system_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of_node, int64_t ndof,
@@ -1786,8 +1799,8 @@ int orc_ch_system_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_
                      const double *stab, const double *c, double dt, double eps, double rho, double sigma,
                      double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs, double *pcdiag)
 {
-    return ch_system_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, mass, lb, stab, c, dt, eps, rho, sigma, S,
-                        brp, bcol, bval, rhs, pcdiag);
+    return ch_system_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, mass, lb, stab, c, NULL, dt, eps, rho, sigma,
+                        S, brp, bcol, bval, rhs, pcdiag);
 }
 
 /* one step: system + right-preconditioned GMRES(restart) from x = [mu; c] (in/out) */
@@ -1804,8 +1817,8 @@ static int ch_step_sp(int space, const orc_mesh_t *m, const double *phi, const i
     double *pcd = (double *)malloc(sizeof(double) * (n2 + 1));
     int err = (!brp || !bcol || !bval || !rhs || !pcd) ? ORC_E_NOMEM : ORC_OK;
     if (!err)
-        err = ch_system_sp(space, m, phi, dof_of, ndof, rowptr, colind, mass, lb, stab, x + ndof, dt, eps, rho,
-                           sigma, S, brp, bcol, bval, rhs, pcd);
+        err = ch_system_sp(space, m, phi, dof_of, ndof, rowptr, colind, mass, lb, stab, x + ndof, NULL, dt, eps,
+                           rho, sigma, S, brp, bcol, bval, rhs, pcd);
     if (!err) err = orc_gmres_diag(n2, brp, bcol, bval, pcd, rhs, x, restart, max_iter, tol, rel, 0);
     free(brp); free(bcol); free(bval); free(rhs); free(pcd);
     return err;
@@ -1827,6 +1840,141 @@ int orc_ch_step_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of
 {
     return ch_step_sp(2, m, phi, dof_of_node, ndof, rowptr, colind, mass, lb, stab, x, dt, eps, rho, sigma, S,
                       restart, max_iter, tol, rel);
+}
+
+/* ---- BDF1/BDF2 with the adaptive step size, surfphasesep/surfphasesep.cpp:112-123, 396-491 ---- */
+
+/* r = dt / dt_prev.  The BDF2 row of the time derivative, alpha chi - F_chi with
+ * alpha = (1 + 2r) / ((1 + r) dt) and F_chi = (1 + r)/dt M chi_n - r^2/((1 + r) dt) M chi_{n-1}
+ * (:451-468), scaled by dt_eff = 1/alpha to the implicit-Euler layout of the step system:
+ *   M c + dt_eff A mu = M c_r,  c_r = w3 c_n + w4 c_{n-1}.
+ * The extrapolation chi_extrap = (1 + r) chi_n - r chi_{n-1} (:452) gives c_e = w1 c_n + w2 c_{n-1}.
+ * w = {dt_eff, w1, w2, w3, w4}. */
+void orc_bdf2_weights(double dt, double dt_prev, double w[5])
+{
+    const double r = dt / dt_prev;
+    w[0] = (1. + r) * dt / (1. + 2. * r);
+    w[1] = 1. + r;
+    w[2] = -r;
+    w[3] = (1. + r) * (1. + r) / (1. + 2. * r);
+    w[4] = -(r * r) / (1. + 2. * r);
+}
+
+/* the step-size function F(e, dt) (:117-123): the initial step for e == 0, else
+ * rho sqrt(tol / e) dt clamped to [min_step, max_step] */
+double orc_bdf_step_size(const orc_bdf_params_t *p, double e, double dt)
+{
+    if (!e) return p->step_size;
+    const double f = p->rho * sqrt(p->tol / e) * dt;
+    if (f > p->max_step) return p->max_step;
+    if (f < p->min_step) return p->min_step;
+    return f;
+}
+
+int orc_ch_system_bdf(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                      const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                      const double *stab, const double *c_e, const double *c_r, double dt_eff, double eps,
+                      double rho, double sigma, double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs,
+                      double *pcdiag)
+{
+    if ((space != 1 && space != 2) || !c_r) return ORC_E_ARG;
+    return ch_system_sp(space, m, phi, dof_of, ndof, rowptr, colind, mass, lb, stab, c_e, c_r, dt_eff, eps, rho,
+                        sigma, S, brp, bcol, bval, rhs, pcdiag);
+}
+
+/* One time step t_{i-1} -> t_i of the loop at :397-491.  x = [mu; c] holds the accepted state,
+ * cprev the concentration one step back (read from the second step on), st the loop's carried
+ * values.  Every attempt: dt = F(e, dt) on the first step or when adaptive; a BDF1 solve from the
+ * solver's previous vector (the reference keeps one lhs across solves, :319); on the first step
+ * the BDF1 solution is taken and the loop ends (:446-450); else a BDF2 solve started from the BDF1
+ * solution and the error e = sqrt((d, M d) / (c2, M c2)), d = c2 - c1 (:477-479).  Retried while
+ * adaptive and e > tol and min_step < dt < max_step (:483).  On acceptance the last solve's
+ * [mu; c] becomes the state (:485-488).  The reference retries without bound; max_tries > 0 ends
+ * the step with ORC_E_NOT_CONVERGED after that many attempts (state unchanged). */
+int orc_ch_bdf_step(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                    const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                    const double *stab, double *x, double *cprev, orc_bdf_state_t *st, const orc_bdf_params_t *bp,
+                    double eps, double rho, double sigma, double S, int32_t restart, int32_t max_iter, double tol,
+                    int32_t rel, orc_bdf_stat_t *stat)
+{
+    if ((space != 1 && space != 2) || !x || !cprev || !st || !bp || ndof < 0) return ORC_E_ARG;
+    const int64_t n = ndof, n2 = 2 * ndof, nnz = rowptr[ndof];
+    int64_t *brp = (int64_t *)malloc(sizeof(int64_t) * (n2 + 1));
+    int32_t *bcol = (int32_t *)malloc(sizeof(int32_t) * (4 * nnz + 1));
+    double *bval = (double *)malloc(sizeof(double) * (4 * nnz + 1));
+    double *rhs = (double *)malloc(sizeof(double) * (n2 + 1));
+    double *pcd = (double *)malloc(sizeof(double) * (n2 + 1));
+    double *xl = (double *)malloc(sizeof(double) * (n2 + 1));
+    double *c1 = (double *)malloc(sizeof(double) * (n + 1));
+    double *ce = (double *)malloc(sizeof(double) * (n + 1));
+    double *cr = (double *)malloc(sizeof(double) * (n + 1));
+    int err = (!brp || !bcol || !bval || !rhs || !pcd || !xl || !c1 || !ce || !cr) ? ORC_E_NOMEM : ORC_OK;
+    const int64_t i = st->step + 1;
+    const double t_prev = st->t, dt_prev = st->dt;
+    double dt = st->dt, e = st->e, resid = 0.;
+    int32_t tries = 0, it1 = 0, it2 = 0;
+    if (!err) memcpy(xl, x, sizeof(double) * n2);
+    while (!err) {
+        if (bp->max_tries > 0 && tries == bp->max_tries) { err = ORC_E_NOT_CONVERGED; break; }
+        ++tries;
+        if (i == 1 || bp->adaptive) dt = orc_bdf_step_size(bp, e, dt);
+        /* BDF1: the implicit-Euler system from chi_n with alpha = 1/dt */
+        err = ch_system_sp(space, m, phi, dof_of, n, rowptr, colind, mass, lb, stab, x + n, NULL, dt, eps, rho,
+                           sigma, S, brp, bcol, bval, rhs, pcd);
+        int32_t it = max_iter;
+        double tl = tol;
+        if (!err) err = orc_gmres_diag(n2, brp, bcol, bval, pcd, rhs, xl, restart, &it, &tl, rel, 0);
+        it1 += it;
+        resid = tl;
+        if (err) break;
+        memcpy(c1, xl + n, sizeof(double) * n);
+        if (i == 1) break;
+        double w[5];
+        orc_bdf2_weights(dt, dt_prev, w);
+        for (int64_t k = 0; k < n; ++k) {
+            ce[k] = w[1] * x[n + k] + w[2] * cprev[k];
+            cr[k] = w[3] * x[n + k] + w[4] * cprev[k];
+        }
+        err = ch_system_sp(space, m, phi, dof_of, n, rowptr, colind, mass, lb, stab, ce, cr, w[0], eps, rho, sigma,
+                           S, brp, bcol, bval, rhs, pcd);
+        it = max_iter;
+        tl = tol;
+        if (!err) err = orc_gmres_diag(n2, brp, bcol, bval, pcd, rhs, xl, restart, &it, &tl, rel, 0);
+        it2 += it;
+        resid = tl;
+        if (err) break;
+        double dd = 0., cc = 0.;
+        for (int64_t r = 0; r < n; ++r) {
+            double md = 0., mc = 0.;
+            for (int64_t q = rowptr[r]; q < rowptr[r + 1]; ++q) {
+                const int32_t j = colind[q];
+                md += mass[q] * (xl[n + j] - c1[j]);
+                mc += mass[q] * xl[n + j];
+            }
+            dd += (xl[n + r] - c1[r]) * md;
+            cc += xl[n + r] * mc;
+        }
+        e = sqrt(dd / cc);
+        if (!(bp->adaptive && e > bp->tol && bp->min_step < dt && dt < bp->max_step)) break;
+    }
+    if (!err) {
+        memcpy(cprev, x + n, sizeof(double) * n);
+        memcpy(x, xl, sizeof(double) * n2);
+        st->step = i;
+        st->t = t_prev + dt;
+        st->dt = dt;
+        st->e = e;
+    }
+    if (stat) {
+        stat->tries = tries;
+        stat->iters_bdf1 = it1;
+        stat->iters_bdf2 = it2;
+        stat->resid = resid;
+        stat->dt = dt;
+        stat->e = e;
+    }
+    free(brp); free(bcol); free(bval); free(rhs); free(pcd); free(xl); free(c1); free(ce); free(cr);
+    return err;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -217,6 +217,38 @@ int orc_ch_step_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_of
                    const double *stab, double *x, double dt, double eps, double rho, double sigma, double S,
                    int32_t restart, int32_t *max_iter, double *tol, int32_t rel);
 
+/* ---- BDF1/BDF2 with the adaptive step size (surfphasesep/surfphasesep.cpp:112-123, 396-491) ---- */
+typedef struct {
+    double step_size;  /* Time.StepSize: F(0, dt) */
+    double tol;        /* Time.Adaptive.Tol */
+    double rho;        /* Time.Adaptive.rho */
+    double min_step;   /* Time.Adaptive.MinStepSize (0: none) */
+    double max_step;   /* Time.Adaptive.MaxStepSize (inf: none) */
+    int32_t adaptive;  /* Time.Adaptive.Use */
+    int32_t max_tries; /* attempts per step before ORC_E_NOT_CONVERGED (0: no bound, as the reference) */
+} orc_bdf_params_t;
+typedef struct {
+    int64_t step;      /* steps taken (the reference's i) */
+    double t, dt, e;   /* time reached, last accepted step, last error (0 until a BDF2 step) */
+} orc_bdf_state_t;
+typedef struct {
+    int32_t tries, iters_bdf1, iters_bdf2;  /* attempts, GMRES iterations summed over them */
+    double resid, dt, e;                      /* last solve's residual, last attempt's dt and e */
+} orc_bdf_stat_t;
+
+void orc_bdf2_weights(double dt, double dt_prev, double w[5]);
+double orc_bdf_step_size(const orc_bdf_params_t *p, double e, double dt);
+int orc_ch_system_bdf(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                      const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                      const double *stab, const double *c_e, const double *c_r, double dt_eff, double eps,
+                      double rho, double sigma, double S, int64_t *brp, int32_t *bcol, double *bval, double *rhs,
+                      double *pcdiag);
+int orc_ch_bdf_step(int space, const orc_mesh_t *m, const double *phi, const int32_t *dof_of, int64_t ndof,
+                    const int64_t *rowptr, const int32_t *colind, const double *mass, const double *lb,
+                    const double *stab, double *x, double *cprev, orc_bdf_state_t *st, const orc_bdf_params_t *bp,
+                    double eps, double rho, double sigma, double S, int32_t restart, int32_t max_iter, double tol,
+                    int32_t rel, orc_bdf_stat_t *stat);
+
 #ifdef __cplusplus
 }
 #endif
