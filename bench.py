@@ -132,12 +132,17 @@ def cpu_baseline(mesh, phi, coef, lc, budget_s=30.0, reps=5):
                 colors=int(colors[0].size - 1), cpu_model=cpu_model()), out
 
 
+VARIANTS = ("previous", "extrapolation", "learned", "previous_mixed", "learned_mixed")
+
+
 def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=2):
     """Cahn-Hilliard trajectory on the torus (implicit Euler, GMRES(50) + block Jacobi to 1e-8) on the
     P2 (or P1) interface space: the previous-step start against the linear extrapolation and the
     learned start (trained weights from drops_amd/data/ch_warmstart_torus<cells>[_p2].npz, produced
-    by `python -m drops_amd.learn`), on a seed the network was not trained on; the CPU port times
-    the first steps."""
+    by `python -m drops_amd.learn`), on a seed the network was not trained on, each in FP64 and the
+    previous-step and learned starts also with FP32 Arnoldi cycles inside FP64 restarts ("_mixed");
+    the CPU port times the first steps, and every variant's state after those steps is compared
+    with the port's FP64 state."""
     from drops_amd import ch_run, learn
     sfx = "_p2" if space == 2 else ""
     path = os.path.join(learn.DATA_DIR, f"ch_warmstart_torus{cells}{sfx}.npz")
@@ -154,11 +159,11 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=
         if os.path.exists(path):
             mlp, meta = learn.load(ctx, path)
             out["trained_on_seeds"] = meta.get("train_seeds")
-            ev = learn.evaluate(ch, mlp, seed, steps)
+            ev = learn.evaluate(ch, mlp, seed, steps, variants=VARIANTS, first=cpu_steps)
             mlp.close()
         else:
             log(f"[bench] {path} missing: previous-step start only")
-            ev = learn.evaluate(ch, None, seed, steps, variants=("previous",))
+            ev = learn.evaluate(ch, None, seed, steps, variants=("previous", "previous_mixed"), first=cpu_steps)
         for v, r in ev.items():
             out[v] = {"s_total": float(r["wall"].sum()), "s_per_step": float(r["wall"].mean()),
                       "gmres_iters": int(r["iters"].sum()), "iters_first_steps": r["iters"][:5].tolist(),
@@ -167,11 +172,20 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=
             if v != "previous":
                 out[v]["starts"] = r["starts"]
             log(f"[bench] ch {v}: {out[v]['s_total']:.3f} s for {steps} steps, {out[v]['gmres_iters']} GMRES iterations")
-        if "learned" in ev:
-            out["speedup_learned_vs_previous"] = out["previous"]["s_total"] / out["learned"]["s_total"]
-            out["speedup_extrapolation_vs_previous"] = out["previous"]["s_total"] / out["extrapolation"]["s_total"]
-            cp, cl = ev["previous"]["state"][0], ev["learned"]["state"][0]
-            out["c_rel_diff_learned_vs_previous"] = float(np.linalg.norm(cl - cp) / np.linalg.norm(cp))
+        cp = ev["previous"]["state"][0]
+        for v in ev:
+            if v != "previous":
+                out[f"speedup_{v}_vs_previous"] = out["previous"]["s_total"] / out[v]["s_total"]
+                cv = ev[v]["state"][0]
+                out[v]["c_rel_diff_vs_previous_final"] = float(np.linalg.norm(cv - cp) / np.linalg.norm(cp))
+        if "learned_mixed" in ev:
+            out["learned_mixed_vs_plain"] = {
+                "what": "learned start + FP32 Arnoldi cycles in FP64 restarts against the plain FP64 GMRES "
+                        "from the previous step, same seed, same tolerance (both accept only rel. residual "
+                        "<= 1e-8; the mixed form checks it on the FP64 residual)",
+                "s_plain": out["previous"]["s_total"], "s_learned_mixed": out["learned_mixed"]["s_total"],
+                "speedup": out["speedup_learned_mixed_vs_previous"],
+                "iters_plain": out["previous"]["gmres_iters"], "iters_learned_mixed": out["learned_mixed"]["gmres_iters"]}
         ch.close()
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -195,6 +209,25 @@ def ch_leg(cells=96, steps=200, seed=1000, cpu_steps=None, with_cpu=True, space=
                            "sample": f"first {cpu_steps} steps of the same seed (the costliest: most GMRES "
                                      f"iterations), oracle OpenMP build"}
         out["speedup_gpu_vs_cpu_first_steps"] = cpu_s / out["previous"]["s_first_steps"]
+        # every variant's state after the same first steps against the FP64 oracle's: the timed port
+        # run (the same GMRES to 1e-8, which the FP64 device solve follows iterate for iterate) and
+        # the port's solve of the same steps to 1e-13 (the solution itself, up to conditioning)
+        xt = np.concatenate([mu0, c0])
+        for s in range(cpu_steps):
+            O.ch_step_c(om, phi, dof, nd, r["rowptr"], r["colind"], r["mass"], r["lb"], r["stab"], xt, ch_run.DT,
+                        ch_run.EPS, m.h, tol=1e-13, max_iter=200000, kind="omp", space=space)
+        rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        for v, res in ev.items():
+            cg, mug = res["state_first"]
+            xg = np.concatenate([mug, cg])
+            out[v]["err_vs_fp64_oracle_first_steps"] = rel(xg, x)
+            out[v]["c_err_vs_fp64_oracle_first_steps"] = rel(cg, x[nd:])
+            out[v]["err_vs_fp64_oracle_1e-13_first_steps"] = rel(xg, xt)
+            out[v]["c_err_vs_fp64_oracle_1e-13_first_steps"] = rel(cg, xt[nd:])
+        if "learned_mixed_vs_plain" in out:
+            for k in ("err_vs_fp64_oracle_1e-13_first_steps", "c_err_vs_fp64_oracle_1e-13_first_steps"):
+                out["learned_mixed_vs_plain"][k] = out["learned_mixed"][k]
+                out["learned_mixed_vs_plain"][k.replace("err", "err_plain", 1)] = out["previous"][k]
     return out
 
 
@@ -415,14 +448,16 @@ def gmres_probe_sweep(cells=96, space=1):
     """Per-iteration compute/synchronisation split of the persistent GMRES on the first step of the
     Cahn-Hilliard trajectory (torus, `cells`^3, seed 0, P1 or P2 space) for a sweep of rows per
     block, with the fenced and the coherent (fence-free) exchange; the two must give the same
-    solution bit for bit at one geometry."""
+    solution bit for bit at one geometry.  A third pass runs the FP32 Arnoldi cycles (coherent
+    exchange), whose solution differs within the tolerance."""
     from drops_amd import ch_run
     m, phi = ch_run.torus_mesh(cells)
     with D.Context(0) as ctx:
         ch = ch_run.make_stepper(ctx, m, phi, space=space)
         ref = {}
-        for coh in (0, 1):
+        for coh, prec in ((0, D.FP64), (1, D.FP64), (1, D.FP32_IR)):
             ctx.tune("coherent_exchange", coh)
+            ch.set_precision(prec)
             for rows in ((128, 256, 384) if space == 1 else (768, 1024)):
                 ctx.tune("pcg_rows_per_block", rows)
                 c0, mu0 = ch_run.initial_state(ch.n, 0)
@@ -437,13 +472,17 @@ def gmres_probe_sweep(cells=96, space=1):
                 x = np.concatenate([mu, c])
                 if coh == 0:
                     ref[rows] = x
-                print(json.dumps(dict(coherent_exchange=coh, rows_per_block=rows, blocks=int(pr.shape[0]), iters=it,
+                print(json.dumps(dict(coherent_exchange=coh, precision="fp32_cycles" if prec else "fp64",
+                                      rows_per_block=rows, blocks=int(pr.shape[0]), iters=it,
                                       ms_per_solve=1e3 * (t1 - t0),
                                       us_per_iter=float(pr[:, 3].mean() / it),
                                       compute_us_per_iter=float(pr[:, 1].mean() / it),
                                       spmv_us_per_iter=float(pr[:, 0].mean() / it),
                                       sync_us_per_iter=float(pr[:, 2].mean() / it),
-                                      identical_to_fenced=bool(np.array_equal(x, ref[rows])))), flush=True)
+                                      identical_to_fenced=bool(np.array_equal(x, ref[rows])),
+                                      rel_diff_to_fenced=float(np.linalg.norm(x - ref[rows]) /
+                                                               np.linalg.norm(ref[rows])))), flush=True)
+        ch.set_precision(D.FP64)
         ctx.tune("coherent_exchange", 2)
         ctx.tune("pcg_rows_per_block", 0)
         ch.close()

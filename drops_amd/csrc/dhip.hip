@@ -439,7 +439,8 @@ void dhip_destroy(dhip_ctx_t *c) {
                       &c->loc, &c->flags, &c->first, &c->isown, &c->ownrank, &c->dofs, &c->rowowner,
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
                       &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
-                      &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos};
+                      &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos,
+                      &c->gm_sell_val32};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -1534,6 +1535,12 @@ int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
         c->coherent_exchange = (int)value;
         return DHIP_OK;
     }
+    if (!std::strcmp(key, "gmres_precision")) {
+        if (value != DHIP_FP64 && value != DHIP_FP32_IR)
+            return set_error(c, DHIP_E_ARG, "gmres_precision must be DHIP_FP64 or DHIP_FP32_IR");
+        c->gmres_precision = (int)value;
+        return DHIP_OK;
+    }
     if (!std::strcmp(key, "xcd_map")) {
         if (value != 0 && value != 1) return set_error(c, DHIP_E_ARG, "xcd_map must be 0 or 1");
         c->xcd_map = (int)value;
@@ -1589,8 +1596,9 @@ int dhip_solver_probe(dhip_ctx_t *c, int32_t on, uint64_t *ticks, int32_t *nbloc
 }  // extern "C"
 
 // Persistent GMRES(m) on device vectors: diag == nullptr means no preconditioner.
+// f32: Arnoldi cycles on FP32 basis and matrix values inside the FP64 restart residual (k_gmres F32).
 static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const double *db, double *dx, int m,
-                      int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc) {
+                      int32_t *max_iter, double *tol, int32_t rel, int32_t left_pc, bool f32 = false) {
     const int64_t n = A->n;
     DHIP_HIPCHK(c, c->kv[15].ensure(sizeof(double) * (size_t)m * n + 8));
     unsigned nb = 1;
@@ -1622,9 +1630,15 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
         std::equal(key, key + 4, c->gm_sell_key)) {
         // the same pattern and block layout as the last solve: only the values change
         const int spb = (int)((per + 63) / 64);
+        float *sv32 = nullptr;
+        if (f32) {
+            DHIP_HIPCHK(c, c->gm_sell_val32.ensure(c->gm_sell_val.bytes / 2 + 8));
+            sv32 = c->gm_sell_val32.as<float>();
+        }
         DHIP_LAUNCH(c, "gmres_sell_vals", k_sell_vals, blocks_for(A->nnz, 256), 256, 0, A->nnz,
                     c->gm_sell_pos.as<int32_t>(), A->col, A->val, (diag && !left_pc) ? diag : nullptr,
-                    c->gm_sell_val.as<double>());
+                    c->gm_sell_val.as<double>(), sv32);
+        ga.sell_val32 = sv32;
         ga.sell_off = c->gm_sell_off.as<int32_t>();
         ga.sell_col = c->gm_sell_col.as<int32_t>();
         ga.sell_val = c->gm_sell_val.as<double>();
@@ -1641,6 +1655,12 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
         if (rc) return rc;
         DHIP_HIPCHK(c, c->gm_sell_col.ensure(sizeof(int32_t) * (tot + 1)));
         DHIP_HIPCHK(c, c->gm_sell_val.ensure(sizeof(double) * (tot + 1)));
+        float *sv32 = nullptr;
+        if (f32) {
+            DHIP_HIPCHK(c, c->gm_sell_val32.ensure(sizeof(float) * (tot + 1)));
+            sv32 = c->gm_sell_val32.as<float>();
+        }
+        ga.sell_val32 = sv32;
         int32_t *pos = nullptr;
         if (A->pattern_id != 0) {
             DHIP_HIPCHK(c, c->gm_sell_pos.ensure(sizeof(int32_t) * (A->nnz + 1)));
@@ -1648,7 +1668,7 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
         }
         DHIP_LAUNCH(c, "gmres_sell_fill", k_sell_fill, blocks_for(n, 256), 256, 0, n, per, spb, A->rowptr, A->col,
                     A->val, (diag && !left_pc) ? diag : nullptr, c->gm_sell_off.as<int32_t>(),
-                    c->gm_sell_col.as<int32_t>(), c->gm_sell_val.as<double>(), pos);
+                    c->gm_sell_col.as<int32_t>(), c->gm_sell_val.as<double>(), pos, sv32);
         if (A->pattern_id != 0) std::copy(key, key + 4, c->gm_sell_key);
         ga.sell_off = c->gm_sell_off.as<int32_t>();
         ga.sell_col = c->gm_sell_col.as<int32_t>();
@@ -1667,7 +1687,10 @@ static int gmres_core(dhip_ctx *c, const dhip_csr *A, const double *diag, const 
     const void *ks_c[5] = {regv ? (const void *)k_gmres<1, KREG, true> : (const void *)k_gmres<1, 0, true>,
                            (const void *)k_gmres<2, 0, true>, (const void *)k_gmres<4, 0, true>,
                            (const void *)k_gmres<8, 0, true>, (const void *)k_gmres<16, 0, true>};
-    const void *const *ks = c->coherent_exchange ? ks_c : ks_f;
+    const void *ks_32[5] = {regv ? (const void *)k_gmres<1, KREG, true, true> : (const void *)k_gmres<1, 0, true, true>,
+                            (const void *)k_gmres<2, 0, true, true>, (const void *)k_gmres<4, 0, true, true>,
+                            (const void *)k_gmres<8, 0, true, true>, (const void *)k_gmres<16, 0, true, true>};
+    const void *const *ks = f32 ? ks_32 : c->coherent_exchange ? ks_c : ks_f;
     const int kidx = rpt == 1 ? 0 : rpt == 2 ? 1 : rpt == 4 ? 2 : rpt == 8 ? 3 : 4;
     const size_t lds = regv ? (size_t)LDS_BYTES : persistent_lds(c, ks[kidx]);
     ga.lds_bytes = (int64_t)lds;
@@ -1726,7 +1749,7 @@ static int gmres_entry(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double 
                     c->kv[12].as<double>(), nullptr);
         diag = c->kv[12].as<double>();
     }
-    rc = gmres_core(c, A, diag, db, dx, m, max_iter, tol, rel, left_pc);
+    rc = gmres_core(c, A, diag, db, dx, m, max_iter, tol, rel, left_pc, c->gmres_precision == DHIP_FP32_IR);
     if ((rc == DHIP_OK || rc == DHIP_E_NOT_CONVERGED) && where != DHIP_DEVICE) {
         if (hipMemcpyAsync(x, dx, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
             hipStreamSynchronize(c->stream) != hipSuccess)
@@ -1782,6 +1805,7 @@ struct dhip_ch {
     DevBuf bdf_xl, bdf_c1, bdf_cprev, bdf_ce, bdf_cr, bdf_part;
     int64_t bdf_step = 0;
     double bdf_t = 0.0, bdf_dt = 0.0, bdf_e = 0.0;
+    int32_t precision = DHIP_FP64;   // GMRES cycles: DHIP_FP64 or DHIP_FP32_IR (FP32 Arnoldi)
 };
 
 static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
@@ -1870,7 +1894,7 @@ static int ch_solve_impl(dhip_ch *h, int32_t *iters, double *resid) {
     int32_t it = h->p.max_iter;
     double tol = h->p.tol;
     rc = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), h->x.as<double>(), m, &it, &tol,
-                    h->p.rel, 0);
+                    h->p.rel, 0, h->precision == DHIP_FP32_IR);
     h->prepared = false;
     h->bdf_step = 0;
     h->bdf_e = 0.0;
@@ -2177,7 +2201,8 @@ static int ch_bdf_one(dhip_ch *h, const dhip_bdf_params_t &bp, dhip_bdf_stat_t *
     auto solve = [&](int32_t *acc) {
         int32_t it = h->p.max_iter;
         double tl = h->p.tol;
-        const int r = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), xl, m, &it, &tl, h->p.rel, 0);
+        const int r = gmres_core(c, &h->K, h->pcdiag.as<double>(), h->rhs.as<double>(), xl, m, &it, &tl, h->p.rel, 0,
+                                 h->precision == DHIP_FP32_IR);
         *acc += it;
         resid = tl;
         return r;
@@ -2241,6 +2266,14 @@ static int ch_bdf_one(dhip_ch *h, const dhip_bdf_params_t &bp, dhip_bdf_stat_t *
 }
 
 extern "C" {
+
+int dhip_ch_set_precision(dhip_ch_t *h, int32_t precision) {
+    if (!h) return DHIP_E_ARG;
+    if (precision != DHIP_FP64 && precision != DHIP_FP32_IR)
+        return set_error(h->ctx, DHIP_E_ARG, "precision must be DHIP_FP64 or DHIP_FP32_IR");
+    h->precision = precision;
+    return DHIP_OK;
+}
 
 int dhip_ch_bdf_step(dhip_ch_t *h, const dhip_bdf_params_t *p, int32_t nsteps, dhip_bdf_stat_t *stats) {
     if (!h || !p || nsteps < 0) return DHIP_E_ARG;

@@ -134,6 +134,8 @@ struct dhip_ctx {
     dhip::DevBuf kv[17];
     dhip::DevBuf gm_sval;   // right-preconditioned GMRES: a_rc / d_c
     dhip::DevBuf gm_sell_off, gm_sell_col, gm_sell_val;  // SELL-64 copy for large systems
+    dhip::DevBuf gm_sell_val32;                          // its values rounded to float (FP32 Arnoldi)
+    int gmres_precision = 0;                             // DHIP_FP64 or DHIP_FP32_IR for dhip_gmres*
     dhip::DevBuf gm_sell_pos;                            // SELL slot of every CSR entry
     uint64_t gm_sell_key[4] = {0, 0, 0, 0};              // pattern id, rows, entries, blocks of the layout
     // optional per-block wall-clock probe of the persistent solver

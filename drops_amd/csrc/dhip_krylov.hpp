@@ -15,6 +15,8 @@
 
 #include "dhip_common.hpp"
 
+#include <type_traits>
+
 namespace dhip {
 
 constexpr int KB = 256;       // threads per block of the vector kernels
@@ -192,13 +194,13 @@ __device__ __forceinline__ void wait_own_stores() { asm volatile("s_waitcnt vmcn
 // by itself -- and the synchronisation runs without fences: a block publishes its slot only after
 // s_waitcnt vmcnt(0) has seen all its stores complete, and a reader issues its loads only after it
 // has seen every slot of the generation.
-template <bool COH>
-__device__ __forceinline__ double xload(const double *p) {
-    if (COH) return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <bool COH, class T>
+__device__ __forceinline__ T xload(const T *p) {
+    if (COH) return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return *p;
 }
-template <bool COH>
-__device__ __forceinline__ void xstore(double *p, double v) {
+template <bool COH, class T>
+__device__ __forceinline__ void xstore(T *p, T v) {
     if (COH)
         __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
@@ -602,25 +604,27 @@ struct PcgSrArgs {
     int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
 };
 
-// LDS copy of the slice with values divided by the diagonal of their column.
-__device__ inline Slice<double> load_slice_scaled(unsigned char *lds, const int32_t *rp, const int32_t *col,
-                                                  const double *val, const double *diag, int64_t r0, int64_t r1,
-                                                  bool &scaled) {
-    Slice<double> S;
+// LDS copy of the slice with values divided by the diagonal of their column, held as VT (the
+// quotient rounded once).  When it does not fit: the global arrays, with val set only for VT = double.
+template <class VT = double>
+__device__ inline Slice<VT> load_slice_scaled(unsigned char *lds, const int32_t *rp, const int32_t *col,
+                                              const double *val, const double *diag, int64_t r0, int64_t r1,
+                                              bool &scaled) {
+    Slice<VT> S;
     S.r0 = r0;
     const int64_t nr = r1 > r0 ? r1 - r0 : 0;
     const int32_t nz0 = nr ? rp[r0] : 0;
     const int64_t nz = nr ? (int64_t)rp[r1] - nz0 : 0;
-    const size_t need = (size_t)(nr + 1) * 4 + (size_t)nz * 12 + 16;
+    const size_t need = (size_t)(nr + 1) * 4 + (size_t)nz * (4 + sizeof(VT)) + 16;
     scaled = false;
     if (nr && need <= (size_t)LDS_BYTES) {
         int32_t *lrp = reinterpret_cast<int32_t *>(lds);
-        double *lval = reinterpret_cast<double *>(lds + (((nr + 1) * 4 + 15) & ~(int64_t)15));
+        VT *lval = reinterpret_cast<VT *>(lds + (((nr + 1) * 4 + 15) & ~(int64_t)15));
         int32_t *lcol = reinterpret_cast<int32_t *>(lval + nz);
         for (int64_t i = threadIdx.x; i <= nr; i += blockDim.x) lrp[i] = rp[r0 + i] - nz0;
         for (int64_t i = threadIdx.x; i < nz; i += blockDim.x) {
             const int32_t c = col[nz0 + i];
-            lval[i] = diag ? val[nz0 + i] / diag[c] : val[nz0 + i];
+            lval[i] = (VT)(diag ? val[nz0 + i] / diag[c] : val[nz0 + i]);
             lcol[i] = c;
         }
         __syncthreads();
@@ -632,7 +636,8 @@ __device__ inline Slice<double> load_slice_scaled(unsigned char *lds, const int3
     } else {
         S.rp = rp + r0;
         S.col = col + nz0;
-        S.val = val + nz0;
+        if constexpr (sizeof(VT) == sizeof(double)) S.val = val + nz0;
+        else S.val = nullptr;
         S.off = nz0;
     }
     return S;
@@ -1385,6 +1390,7 @@ struct GmresArgs {
     const double *sell_val;
     int sell_spb;               // slices per block
     int xcd_map = 0;            // 1: the blocks of one XCD own one contiguous run of rows
+    const float *sell_val32 = nullptr;  // FP32 Arnoldi: the SELL-64 values rounded to float
 };
 
 // SELL-64 layout of a CSR matrix partitioned into blocks of `per` consecutive rows: the local rows
@@ -1409,15 +1415,17 @@ __global__ void k_sell_width(int64_t n, int64_t per, int spb, int64_t nslices, c
 #ifndef DHIP_SOLVER_KERNELS
 __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, const int32_t *col,
                             const double *val, const double *diag, const int32_t *off, int32_t *scol,
-                            double *sval, int32_t *pos) {
+                            double *sval, int32_t *pos, float *sval32) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int64_t b = r / per, l = r - b * per;
     const int64_t base = (int64_t)off[b * spb + (l >> 6)] + (l & 63);
     for (int32_t e = rp[r], k = 0; e < rp[r + 1]; ++e, ++k) {
         const int32_t c = col[e];
+        const double v = diag ? val[e] / diag[c] : val[e];
         scol[base + 64 * (int64_t)k] = c;
-        sval[base + 64 * (int64_t)k] = diag ? val[e] / diag[c] : val[e];
+        sval[base + 64 * (int64_t)k] = v;
+        if (sval32) sval32[base + 64 * (int64_t)k] = (float)v;
         if (pos) pos[e] = (int32_t)(base + 64 * (int64_t)k);
     }
 }
@@ -1426,10 +1434,12 @@ __global__ void k_sell_fill(int64_t n, int64_t per, int spb, const int32_t *rp, 
 // new values on an unchanged pattern: one lane per CSR entry through the stored SELL slots
 #ifndef DHIP_SOLVER_KERNELS
 __global__ void k_sell_vals(int64_t nnz, const int32_t *pos, const int32_t *col, const double *val,
-                            const double *diag, double *sval) {
+                            const double *diag, double *sval, float *sval32) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nnz) return;
-    sval[pos[e]] = diag ? val[e] / diag[col[e]] : val[e];
+    const double v = diag ? val[e] / diag[col[e]] : val[e];
+    sval[pos[e]] = v;
+    if (sval32) sval32[pos[e]] = (float)v;
 }
 #endif
 
@@ -1444,12 +1454,18 @@ __global__ void k_scale_vals(int64_t nnz, const int32_t *col, const double *val,
 
 constexpr int KREG = 50;  // basis vectors kept in registers (restart m <= KREG, one row per thread)
 
-template <int RPT, int KR, bool COH>
+// F32: the Arnoldi cycles store the Krylov basis and read the matrix values (LDS slice, SELL copy)
+// as float, accumulating every product, dot and combination in double; the restart residual
+// b - A x, the Hessenberg least squares and x stay FP64, and a cycle that meets the tolerance by
+// its estimate is accepted only when the recomputed FP64 residual meets it too (iterative
+// refinement with one GMRES cycle as the inner solve).
+template <int RPT, int KR, bool COH, bool F32 = false>
 __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
 #ifndef DHIP_SOLVER_KERNELS
     ;
 #else
 {
+    using BT = typename std::conditional<F32, float, double>::type;
     extern __shared__ __align__(16) unsigned char lds[];
     const uint64_t t_start = a.probe ? wall_clock64() : 0;
     uint64_t t_sync = 0, t_spmv = 0;
@@ -1467,43 +1483,53 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
     bool in_lds;
     // right preconditioning: the slice holds a_rc / d_c (A M v); left: plain values (M A v)
     const bool pre = right && a.diag && a.sval;
-    const Slice<double> S = load_slice_scaled(lds, a.rp, a.col, pre ? a.sval : a.val,
+    const Slice<BT> S = load_slice_scaled<BT>(lds, a.rp, a.col, pre ? a.sval : a.val,
                                               (right && !pre) ? a.diag : nullptr, r0, r1, in_lds);
     const bool scaled = in_lds || pre;
     const double *dg = a.diag;
+    // the global CSR values of a slice that is not in LDS (FP64 in both forms)
+    const double *gval = (pre ? a.sval : a.val) + S.off;
     // gathers in batches of 8 when the basis rows occupy registers
     constexpr int GB = KR > 0 ? 8 : MAXR;
-    const bool sell = !in_lds && a.sell_val;
+    const BT *sell_val;
+    if constexpr (F32) sell_val = a.sell_val32;
+    else sell_val = a.sell_val;
+    const bool sell = !in_lds && sell_val;
     const int32_t *sell_col = a.sell_col;
-    const double *sell_val = a.sell_val;
     const int32_t *sell_off = a.sell_off + (sell ? (int64_t)range * a.sell_spb : 0);
+    // SELL batches: one wave per CU-quarter runs here, so a batch may hold a whole long row in
+    // registers; each batch costs two dependent memory round trips (entries, then gathers)
+    constexpr int GBS = KR > 0 ? 8 : (F32 ? 40 : 20);
     auto spmv_row = [&](int64_t l, auto &&yv_) {
         if (sell) {
             // the same sum as row_product, entries read from the coalesced SELL-64 copy
+            using XT = decltype(yv_(0));
             const int64_t base = (int64_t)sell_off[l >> 6] + (l & 63);
             const int32_t len = a.rp[r0 + l + 1] - a.rp[r0 + l];
             double s_ = 0.0;
-            for (int32_t q0 = 0; q0 < len; q0 += GB) {
-                const int cnt = len - q0 < GB ? len - q0 : GB;
-                int32_t cc[GB];
-                double vv[GB], xx[GB];
+            for (int32_t q0 = 0; q0 < len; q0 += GBS) {
+                const int cnt = len - q0 < GBS ? len - q0 : GBS;
+                int32_t cc[GBS];
+                BT vv[GBS];
+                XT xx[GBS];
 #pragma unroll
-                for (int j = 0; j < GB; ++j)
+                for (int j = 0; j < GBS; ++j)
                     if (j < cnt) {
                         cc[j] = sell_col[base + 64 * (int64_t)(q0 + j)];
                         vv[j] = sell_val[base + 64 * (int64_t)(q0 + j)];
                     }
 #pragma unroll
-                for (int j = 0; j < GB; ++j)
+                for (int j = 0; j < GBS; ++j)
                     if (j < cnt) xx[j] = yv_(cc[j]);
 #pragma unroll
-                for (int j = 0; j < GB; ++j)
-                    if (j < cnt) s_ += vv[j] * xx[j];
+                for (int j = 0; j < GBS; ++j)
+                    if (j < cnt) s_ += (double)vv[j] * (double)xx[j];
             }
             return s_;
         }
-        if (scaled || !right || !dg) return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, yv_);
-        return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
+        if (in_lds) return row_product<double, GB>(S.rp, S.col, S.val, S.off, l, yv_);
+        if (scaled || !right || !dg) return row_product<double, GB>(S.rp, S.col, gval, S.off, l, yv_);
+        return row_product<double, GB>(S.rp, S.col, gval, S.off, l, [&](int32_t c) { return yv_(c) / dg[c]; });
     };
     auto col_of = [](int k) { return k * (k + 1) / 2; };  // start of packed column k (rows 0..k)
     const int m = a.m;
@@ -1543,23 +1569,27 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
     // (each thread reads back only its own rows), the SpMV gathers read the global copy.
     const int64_t nrb = r1 > r0 ? r1 - r0 : 0;
     int KL = 0;
-    double *vl = reinterpret_cast<double *>(lds);
+    BT *vl = reinterpret_cast<BT *>(lds);
+    BT *V = reinterpret_cast<BT *>(a.V);
     if constexpr (!REGV) {
         if (!in_lds && nrb > 0) {
-            const int64_t fit = a.lds_bytes / (8 * nrb);
+            const int64_t fit = a.lds_bytes / ((int64_t)sizeof(BT) * nrb);
             KL = (int)(fit < a.m ? fit : a.m);
         }
     }
-    auto vbasis = [&](int q, int64_t l) { return q < KL ? vl[(size_t)q * nrb + l] : a.V[(size_t)q * n + r0 + l]; };
+    auto vbasis = [&](int q, int64_t l) -> double {
+        return q < KL ? vl[(size_t)q * nrb + l] : V[(size_t)q * n + r0 + l];
+    };
     auto vstore = [&](int q, int64_t r, double v, bool mine) {
+        const BT vb = (BT)v;
         if (mine) {
-            xstore<COH>(&a.V[(size_t)q * n + r], v);
-            if (q < KL) vl[(size_t)q * nrb + (r - r0)] = v;
+            xstore<COH>(&V[(size_t)q * n + r], vb);
+            if (q < KL) vl[(size_t)q * nrb + (r - r0)] = vb;
         }
         if constexpr (REGV) {
 #pragma unroll
             for (int t = 0; t < KR - 1; ++t) vreg[t] = vreg[t + 1];
-            vreg[KR - 1] = v;
+            vreg[KR - 1] = (double)vb;
         }
     };
     // block sums of w . V[q] (q < K) and, with `self`, w . w as entry K
@@ -1712,7 +1742,7 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
         int i;
         bool converged = false;
         for (i = 0; i < m - 1 && j <= a.max_iter; ++i, ++j) {
-            const double *vi = a.V + (size_t)i * n;
+            const BT *vi = V + (size_t)i * n;
             // w = A M v_i (right) or M A v_i (left); h1 = V^T w
             const uint64_t t_sp = a.probe ? wall_clock64() : 0;
 #pragma unroll
@@ -1787,11 +1817,12 @@ __global__ void __launch_bounds__(PB) k_gmres(GmresArgs a)
             // v_{i+1} visible to all blocks before the next SpMV
             if (!syncv(0, red_sh)) { status = 2; goto done; }
         }
-        if (converged) {
+        if (converged && !F32) {
             status = 0;
             break;
         }
-        update_x(i - 1);
+        // FP32 cycles: a converged estimate is checked against the FP64 residual below
+        if (!converged) update_x(i - 1);
         // restart: publish x, r = b - A x (left: M r)
 #pragma unroll
         for (int k = 0; k < RPT; ++k)

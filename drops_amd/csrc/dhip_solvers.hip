@@ -1,13 +1,13 @@
 // dhip_solvers.hip -- the persistent Krylov solver kernels (FP64 PCG in the reference's order and in
 // the single-reduction arrangement, FP32-inner / FP64-refined PCG, GMRES), compiled apart from the
-// rest of libdhip.  The file is compiled once per DHIP_SOLVER_PART (0 .. 10), each compilation
+// rest of libdhip.  The file is compiled once per DHIP_SOLVER_PART (0 .. 13), each compilation
 // instantiating one share of the kernels, so that the shares build in parallel; the host code in
 // dhip.hip launches them through their declarations in dhip_krylov.hpp.
 #define DHIP_SOLVER_KERNELS
 #include "dhip_krylov.hpp"
 
 #ifndef DHIP_SOLVER_PART
-#error "compile with -DDHIP_SOLVER_PART=<0..10>"
+#error "compile with -DDHIP_SOLVER_PART=<0..13>"
 #endif
 
 namespace dhip {
@@ -58,6 +58,16 @@ template __global__ void k_gmres<16, 0, true>(GmresArgs);
 template __global__ void k_pcg64_tx<1>(PcgTxArgs);
 template __global__ void k_pcg64_tx<2>(PcgTxArgs);
 template __global__ void k_pcg64_tx<4>(PcgTxArgs);
+#elif DHIP_SOLVER_PART == 11
+// FP32 Arnoldi cycles (coherent exchange)
+template __global__ void k_gmres<1, KREG, true, true>(GmresArgs);
+template __global__ void k_gmres<1, 0, true, true>(GmresArgs);
+template __global__ void k_gmres<2, 0, true, true>(GmresArgs);
+template __global__ void k_gmres<4, 0, true, true>(GmresArgs);
+#elif DHIP_SOLVER_PART == 12
+template __global__ void k_gmres<8, 0, true, true>(GmresArgs);
+#elif DHIP_SOLVER_PART == 13
+template __global__ void k_gmres<16, 0, true, true>(GmresArgs);
 #endif
 
 }  // namespace dhip

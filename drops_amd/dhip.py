@@ -40,7 +40,7 @@ EXPORTS = [
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
-    "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step",
+    "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -153,6 +153,7 @@ def load(path=LIB_PATH):
     L.dhip_ch_solve.argtypes = [P, VP, C.c_int32, I32, D]
     L.dhip_ch_step.argtypes = [P, C.c_int32, I32, D]
     L.dhip_ch_bdf_step.argtypes = [P, C.POINTER(BdfParams), C.c_int32, C.POINTER(BdfStat)]
+    L.dhip_ch_set_precision.argtypes = [P, C.c_int32]
     F = C.POINTER(C.c_float)
     L.dhip_mlp_create.argtypes = [P, C.POINTER(P)]
     L.dhip_mlp_destroy.argtypes = [P]
@@ -441,6 +442,10 @@ class CahnHilliard:
         res = np.zeros(nsteps)
         self.ctx.check(self.L.dhip_ch_step(self.h, nsteps, _p(its, C.c_int32), _p(res, C.c_double)))
         return its, res
+
+    def set_precision(self, precision):
+        """FP64 (default) or FP32_IR: FP32 Arnoldi cycles inside FP64 restarts (dhip_ch_set_precision)."""
+        self.ctx.check(self.L.dhip_ch_set_precision(self.h, precision))
 
     def bdf_step(self, nsteps=1, step_size=0.01, tol=0.005, rho=0.9, min_step=0.0, max_step=float("inf"),
                  adaptive=True, max_tries=0):

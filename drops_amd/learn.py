@@ -131,30 +131,37 @@ def null_network(ctx, mlp):
     return null
 
 
-def evaluate(ch, mlp, seed, steps, variants=("previous", "extrapolation", "learned")):
+def evaluate(ch, mlp, seed, steps, variants=("previous", "extrapolation", "learned"), first=1):
     """The same trajectory from the previous-step start, from the linear extrapolation alone
-    (zero network) and from the learned start.  Per-step iterations and wall time (prepare +
-    solve) of each."""
+    (zero network) and from the learned start; a "_mixed" suffix runs the variant with FP32
+    Arnoldi cycles inside FP64 restarts (dhip_ch_set_precision FP32_IR).  Per-step iterations and
+    wall time (prepare + solve) of each, the final state and the state after `first` steps."""
     out = {}
     null = null_network(ch.ctx, mlp) if "extrapolation" in variants and mlp is not None else None
     for name in variants:
-        net = null if name == "extrapolation" else mlp
+        base = name[:-len("_mixed")] if name.endswith("_mixed") else name
+        net = null if base == "extrapolation" else mlp
+        ch.set_precision(D.FP32_IR if name.endswith("_mixed") else D.FP64)
         c0, mu0 = ch_run.initial_state(ch.n, seed)
         ch.set_state(c0, mu0)
         its = np.zeros(steps, np.int32)
         wall = np.zeros(steps)
         used = [0, 0, 0]
+        state_first = None
         for s in range(steps):
             t0 = time.perf_counter()
             ch.prepare()
-            if name != "previous":
+            if base != "previous":
                 its[s], _, _, u = ch.solve_learned(net)
                 used[u] += 1
             else:
                 its[s], _, _ = ch.solve()
             wall[s] = time.perf_counter() - t0
-        out[name] = dict(iters=its, wall=wall, state=ch.state(),
+            if s + 1 == first:
+                state_first = ch.state()
+        out[name] = dict(iters=its, wall=wall, state=ch.state(), state_first=state_first,
                          starts=dict(previous=used[0], network=used[1], extrapolation=used[2]))
+    ch.set_precision(D.FP64)
     if null is not None:
         null.close()
     return out

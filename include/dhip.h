@@ -103,7 +103,13 @@ int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nbl
  * by one block of the persistent solvers (0 = library default, 256); "xcd_map" -- 1 (default):
  * the blocks of each XCD own one contiguous run of row ranges, so most gathered neighbour rows
  * come from the XCD's own L2; 0: range i belongs to block i (same results up to the order in
- * which the blocks' partial sums are added); "batch_threads",
+ * which the blocks' partial sums are added); "coherent_exchange" -- how the persistent solvers'
+ * blocks exchange vectors: 0 release/acquire fences around every grid reduction, 1 coherent
+ * (L2-scope) loads and stores of the exchanged values without fences, 2 (default) as 1 and the FP64
+ * PCG with generation-tagged 16-byte records and its block's gathered values staged in LDS (the
+ * same iterates bit for bit while a block's slice fits its LDS; the single-reduction kernel
+ * otherwise); "gmres_precision" -- DHIP_FP64 (default) or DHIP_FP32_IR for dhip_gmres and
+ * dhip_gmres_diag, as dhip_ch_set_precision; "batch_threads",
  * "batch_blocks_per_cu", "batch_prepass", "batch_split", "batch_solve_threads" -- see
  * dhip_batch_solve. */
 int dhip_tune(dhip_ctx_t *ctx, const char *key, int64_t value);
@@ -306,6 +312,13 @@ int dhip_ch_system(dhip_ch_t *ch, const int32_t **rowptr, const int32_t **col, c
 int dhip_ch_solve(dhip_ch_t *ch, const double *x0, int32_t where, int32_t *iters, double *resid);
 /* nsteps x (prepare + solve from the previous step); iters/resid (may be NULL) get one entry per step. */
 int dhip_ch_step(dhip_ch_t *ch, int32_t nsteps, int32_t *iters, double *resid);
+/* GMRES precision of every solve of this stepper (default DHIP_FP64).  DHIP_FP32_IR: the
+ * Arnoldi cycles of GMRES(m) store the Krylov basis and read the block matrix (scaled by the
+ * preconditioner) in FP32, accumulating in FP64; each restart recomputes b - K x and updates x in
+ * FP64 -- iterative refinement whose inner solve is one FP32 cycle -- and the tolerance is
+ * accepted only on the recomputed FP64 residual.  The reference's GMRES (num/krylovsolver.h:
+ * 320-463) is FP64 throughout; the solution meets the same FP64 residual bound. */
+int dhip_ch_set_precision(dhip_ch_t *ch, int32_t precision);
 
 /* ---- BDF1/BDF2 with the adaptive step size ----
  * The time loop of surfphasesep/surfphasesep.cpp:396-491 on this stepper's system, replacing the
