@@ -1128,6 +1128,54 @@ int dhip_spmv(dhip_ctx_t *c, const dhip_csr_t *A, const double *x, double *y, in
     return flush_profile(c);
 }
 
+int dhip_spmv_t(dhip_ctx_t *c, const dhip_csr_t *A, int64_t ncols, const double *x, double *y, int32_t where) {
+    if (!c || !A || !x || !y || ncols < 0) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = A->n, nnz = A->nnz, m = ncols;
+    DevBuf bx, by, cnt, trp, tcol, tval;
+    const double *dx;
+    int rc = stage_in(c, bx, x, n, where, &dx);
+    if (rc) return rc;
+    double *dy = y;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, by.ensure(sizeof(double) * (m + 1)));
+        dy = by.as<double>();
+    }
+    if (m > 0) {
+        DHIP_HIPCHK(c, cnt.ensure(sizeof(int32_t) * (m + 1)));
+        DHIP_HIPCHK(c, trp.ensure(sizeof(int32_t) * (m + 1)));
+        DHIP_HIPCHK(c, tcol.ensure(sizeof(int32_t) * (nnz + 1)));
+        DHIP_HIPCHK(c, tval.ensure(sizeof(double) * (nnz + 1)));
+        DHIP_LAUNCH(c, "spmv_t_zero", k_zero_u32, blocks_for(m + 1, 256), 256, 0, cnt.as<uint32_t>(), (int)(m + 1));
+        DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+        if (nnz > 0)
+            DHIP_LAUNCH(c, "spmv_t_count", k_tcount, blocks_for(nnz, 256), 256, 0, nnz, m, A->col, cnt.as<int32_t>(),
+                        c->errflag.as<int32_t>());
+        int64_t tot = 0;
+        rc = scan_exclusive_i32(c, cnt.as<int32_t>(), trp.as<int32_t>(), m, &tot);
+        if (rc) return rc;
+        int32_t fl = 0;
+        DHIP_HIPCHK(c, hipMemcpyAsync(&fl, c->errflag.p, sizeof fl, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (fl) return set_error(c, DHIP_E_ARG, "dhip_spmv_t: a column index is not below ncols = %lld", (long long)m);
+        const int32_t t32 = (int32_t)tot;
+        DHIP_HIPCHK(c, hipMemcpyAsync(trp.as<int32_t>() + m, &t32, sizeof t32, hipMemcpyHostToDevice, c->stream));
+        DHIP_LAUNCH(c, "spmv_t_zero", k_zero_u32, blocks_for(m + 1, 256), 256, 0, cnt.as<uint32_t>(), (int)(m + 1));
+        if (n > 0)
+            DHIP_LAUNCH(c, "spmv_t_fill", k_tfill, blocks_for(n, 256), 256, 0, n, A->rowptr, A->col, A->val,
+                        (const int32_t *)trp.as<int32_t>(), cnt.as<int32_t>(), tcol.as<int32_t>(), tval.as<double>());
+        DHIP_LAUNCH(c, "spmv_t_sort", k_tsort, blocks_for(m, 256), 256, 0, m, (const int32_t *)trp.as<int32_t>(),
+                    tcol.as<int32_t>(), tval.as<double>());
+        const int rb = spmv_tile_rows(m, nnz);
+        DHIP_LAUNCH(c, "spmv_t", k_spmv<false>, blocks_for(m, rb), SPMV_T, 0, m, rb, (const int32_t *)trp.as<int32_t>(),
+                    (const int32_t *)tcol.as<int32_t>(), (const double *)tval.as<double>(), dx, nullptr, dy);
+        if (where != DHIP_DEVICE)
+            DHIP_HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
 int dhip_dot(dhip_ctx_t *c, int64_t n, const double *x, const double *y, double *result, int32_t where) {
     if (!c || !x || !y || !result || n < 0) return DHIP_E_ARG;
     DHIP_HIPCHK(c, hipSetDevice(c->device));

@@ -696,7 +696,7 @@ __device__ inline void rowcount_one(const RowArgs &a, int64_t r) {
     a.rowlen[r] = __popc(present);
 }
 
-__global__ void k_rowcount(RowArgs a) {
+__global__ void __launch_bounds__(128) k_rowcount(RowArgs a) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.ndof) return;
     rowcount_one(a, r);
@@ -758,7 +758,9 @@ __device__ inline void rowfill_one(const RowArgs &a, int64_t r) {
     a.rhs[r] = rh;
 }
 
-__global__ void k_rowfill(RowArgs a) {
+constexpr int ROW_BLOCK = 128;
+
+__global__ void __launch_bounds__(ROW_BLOCK) k_rowfill(RowArgs a) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.ndof) return;
     rowfill_one(a, r);

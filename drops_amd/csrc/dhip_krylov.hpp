@@ -97,6 +97,52 @@ inline int spmv_tile_rows(int64_t n, int64_t nnz) {
     return rb;
 }
 
+// The transposed pattern for y = A^T x (y_ATx, num/spmat.h:1729-1747): column counts, their scan,
+// a fill pass, then each transposed row sorted by original row index -- the order in which y_ATx
+// adds the products into y_j -- so that k_spmv over it gives y_ATx's sums bit for bit.
+#ifndef DHIP_SOLVER_KERNELS
+__global__ void k_tcount(int64_t nnz, int64_t ncols, const int32_t *col, int32_t *cnt, int32_t *err) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    const int32_t j = col[e];
+    if (j < 0 || j >= ncols) {
+        atomicOr(err, 1);
+        return;
+    }
+    atomicAdd(&cnt[j], 1);
+}
+
+__global__ void k_tfill(int64_t n, const int32_t *rp, const int32_t *col, const double *val, const int32_t *trp,
+                        int32_t *fill, int32_t *tcol, double *tval) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int32_t q = rp[i]; q < rp[i + 1]; ++q) {
+        const int32_t j = col[q];
+        const int32_t p = trp[j] + atomicAdd(&fill[j], 1);
+        tcol[p] = (int32_t)i;
+        tval[p] = val[q];
+    }
+}
+
+__global__ void k_tsort(int64_t m, const int32_t *trp, int32_t *tcol, double *tval) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int32_t b = trp[j], e = trp[j + 1];
+    for (int32_t k = b + 1; k < e; ++k) {
+        const int32_t c = tcol[k];
+        const double v = tval[k];
+        int32_t q = k - 1;
+        while (q >= b && tcol[q] > c) {
+            tcol[q + 1] = tcol[q];
+            tval[q + 1] = tval[q];
+            --q;
+        }
+        tcol[q + 1] = c;
+        tval[q + 1] = v;
+    }
+}
+#endif
+
 #ifndef DHIP_SOLVER_KERNELS
 __global__ void __launch_bounds__(KB) k_dot_partial(int64_t n, const double *x, const double *y, double *part) {
     double s = 0.0;

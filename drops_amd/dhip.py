@@ -40,7 +40,7 @@ EXPORTS = [
     "dhip_batch_create", "dhip_batch_solve", "dhip_batch_stats", "dhip_batch_problem", "dhip_batch_destroy",
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
-    "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision",
+    "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision", "dhip_spmv_t",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -135,6 +135,7 @@ def load(path=LIB_PATH):
     L.dhip_csr_destroy.restype = None
     L.dhip_csr_info.argtypes = [P, I64, I64]
     L.dhip_spmv.argtypes = [P, P, VP, VP, C.c_int32]
+    L.dhip_spmv_t.argtypes = [P, P, C.c_int64, VP, VP, C.c_int32]
     L.dhip_dot.argtypes = [P, C.c_int64, VP, VP, D, C.c_int32]
     L.dhip_axpy.argtypes = [P, C.c_int64, C.c_double, VP, VP, C.c_int32]
     L.dhip_pcg.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, I32, D, C.c_int32, C.c_int32]
@@ -827,6 +828,14 @@ class Context:
         x = np.ascontiguousarray(x, np.float64)
         y = np.zeros(A.shape[0])
         self.check(self.L.dhip_spmv(self.h, A.h, _vp(x), _vp(y), HOST))
+        return y
+
+    def spmv_t(self, A, x, ncols=None):
+        """A^T x (dhip_spmv_t); ncols defaults to the row count (square A)."""
+        x = np.ascontiguousarray(x, np.float64)
+        m = A.shape[0] if ncols is None else ncols
+        y = np.zeros(m)
+        self.check(self.L.dhip_spmv_t(self.h, A.h, m, _vp(x), _vp(y), HOST))
         return y
 
     def dot(self, x, y):

@@ -330,6 +330,34 @@ inline VectorCL operator*(const MatrixCL &A, const VectorCL &x) {
     return y;
 }
 
+// transp_mul(A, x) = A^T x (num/spmat.h:1748-1760, y_ATx :1729-1747) on the device, summed in
+// y_ATx's order
+inline VectorCL transp_mul(const MatrixCL &A, const VectorCL &x) {
+    if (A.num_rows() != x.size()) throw DROPSErrCL("transp_mul: incompatible dimensions");
+    DeviceCL &dev = DeviceCL::instance();
+    VectorCL y(A.num_cols());
+    if (A.num_cols() == 0) return y;
+    dev.check(dhip_spmv_t(dev.ctx(), A.device(dev), (int64_t)A.num_cols(), x.raw(), y.raw(), DHIP_HOST));
+    return y;
+}
+
+// transpose(M, Mt) (num/spmat.h:1667-1675): every stored entry of M, rows of Mt in ascending
+// column order (a counting sort by column; the reference goes through SparseMatBuilderCL)
+inline void transpose(const MatrixCL &M, MatrixCL &Mt) {
+    const size_t n = M.num_rows(), m = M.num_cols(), nz = M.num_nonzeros();
+    Mt.resize(m, n, nz);
+    std::vector<size_t> cnt(m + 1, 0);
+    for (size_t k = 0; k < nz; ++k) ++cnt[M.col_ind(k) + 1];
+    for (size_t j = 0; j < m; ++j) cnt[j + 1] += cnt[j];
+    for (size_t j = 0; j <= m; ++j) Mt.raw_row()[j] = cnt[j];
+    for (size_t i = 0; i < n; ++i)
+        for (size_t k = M.row_beg(i); k < M.row_beg(i + 1); ++k) {
+            const size_t p = cnt[M.col_ind(k)]++;
+            Mt.raw_col()[p] = i;
+            Mt.raw_val()[p] = M.raw_val()[k];
+        }
+}
+
 // MatrixBuilderCL semantics: every touched (i,j) is stored, rows sorted on Build() (num/spmat.h:551-677)
 template <class T>
 class SparseMatBuilderCL {
@@ -502,6 +530,104 @@ class GMResSolverCL : public SolverBaseCL {
         res_ = tol_;
         iter_ = maxiter_;
         GMRES(A, x, b, ex, pc_, restart_, iter_, res_, rel_, false, method_);
+    }
+};
+
+// Block-Jacobi preconditioner by given diagonals: z_i = r_i / d_i.  The Cahn-Hilliard stepper's
+// preconditioner (the diagonals of Precond3 and Precond4, surfactant/ifacetransp.cpp:1567-1570)
+// in place of the reference's DiagBlockPcT of inner PCG solves.
+class DiagBlockJacobiPcCL {
+    VectorCL d_;
+
+  public:
+    DiagBlockJacobiPcCL() {}
+    explicit DiagBlockJacobiPcCL(const VectorCL &d) : d_(d) {}
+    void SetDiag(const VectorCL &d) { d_.resize(d.size()); d_ = d; }
+    const VectorCL &diag() const { return d_; }
+    double GetOmega() const { return 1.0; }
+};
+
+inline bool GMRES(const MatrixCL &A, VectorCL &x, const VectorCL &b, const DummyExchangeCL &, DiagBlockJacobiPcCL &M,
+                  int m, int &max_iter, double &tol, bool measure_relative_tol = true, bool = false,
+                  PreMethGMRES method = RightPreconditioning, std::ostream * = 0) {
+    DeviceCL &dev = DeviceCL::instance();
+    if (M.diag().size() != b.size()) throw DROPSErrCL("GMRES: the block-Jacobi diagonal does not match the system");
+    if (x.size() != b.size()) x.resize(b.size(), 0.0);
+    int32_t it = max_iter;
+    double t = tol;
+    const int rc = dhip_gmres_diag(dev.ctx(), A.device(dev), M.diag().raw(), b.raw(), x.raw(), DHIP_HOST, m, &it, &t,
+                                   measure_relative_tol ? 1 : 0, method == LeftPreconditioning ? 1 : 0);
+    if (rc != DHIP_OK && rc != DHIP_E_NOT_CONVERGED) dev.check(rc);
+    max_iter = it;
+    tol = t;
+    return rc == DHIP_OK;
+}
+
+namespace detail {
+// The block matrix [A B; C D] in the layout of the SparseMatBaseCL(A, B, C, D) constructor
+// (num/spmat.h:719-751): row i of the top half holds row i of A, then row i of B with its
+// columns shifted by A's column count; the bottom half likewise with C and D.
+inline MatrixCL block_matrix(const MatrixCL &A, const MatrixCL &B, const MatrixCL &C, const MatrixCL &D) {
+    if (A.num_rows() != B.num_rows() || C.num_rows() != D.num_rows() || A.num_cols() != C.num_cols() ||
+        B.num_cols() != D.num_cols())
+        throw DROPSErrCL("BlockMatrixSolverCL: the blocks do not fit together");
+    const size_t n0 = A.num_rows(), n1 = C.num_rows(), c0 = A.num_cols();
+    MatrixCL M;
+    M.resize(n0 + n1, c0 + B.num_cols(), A.num_nonzeros() + B.num_nonzeros() + C.num_nonzeros() + D.num_nonzeros());
+    size_t k = 0;
+    auto put = [&](const MatrixCL &X, size_t i, size_t shift) {
+        for (size_t q = X.row_beg(i); q < X.row_beg(i + 1); ++q, ++k) {
+            M.raw_col()[k] = X.col_ind(q) + shift;
+            M.raw_val()[k] = X.raw_val()[q];
+        }
+    };
+    for (size_t i = 0; i < n0; ++i) {
+        M.raw_row()[i] = k;
+        put(A, i, 0);
+        put(B, i, c0);
+    }
+    for (size_t i = 0; i < n1; ++i) {
+        M.raw_row()[n0 + i] = k;
+        put(C, i, 0);
+        put(D, i, c0);
+    }
+    M.raw_row()[n0 + n1] = k;
+    return M;
+}
+}  // namespace detail
+
+// BlockMatrixSolverCL (num/oseensolver.h:178-239): solves [A B; C D] [v; p] = [b; c] with the
+// given Krylov solver on the assembled block matrix (the reference applies the blocks one by one
+// through BlockMatrixBaseCL; the device solver takes the merged CSR).
+template <class SolverT>
+class BlockMatrixSolverCL {
+    SolverT &solver_;
+
+  public:
+    explicit BlockMatrixSolverCL(SolverT &solver) : solver_(solver) {}
+    SolverT &GetSolver() { return solver_; }
+    int GetIter() const { return solver_.GetIter(); }
+    double GetResid() const { return solver_.GetResid(); }
+    // Solve(A, B, C, D, v, p, b, c, vel_ex, pr_ex) (:221-237)
+    void Solve(const MatrixCL &A, const MatrixCL &B, const MatrixCL &C, const MatrixCL &D, VectorCL &v, VectorCL &p,
+               const VectorCL &b, const VectorCL &c, const DummyExchangeCL &, const DummyExchangeCL &) {
+        const MatrixCL M = detail::block_matrix(A, B, C, D);
+        const size_t n0 = A.num_rows(), m0 = A.num_cols();
+        VectorCL rhs(M.num_rows()), x(M.num_cols());
+        rhs[std::slice(0, n0, 1)] = b;
+        rhs[std::slice(n0, M.num_rows() - n0, 1)] = c;
+        x[std::slice(0, m0, 1)] = v;
+        x[std::slice(m0, M.num_cols() - m0, 1)] = p;
+        solver_.Solve(M, x, rhs, DummyExchangeCL());
+        v = x[std::slice(0, m0, 1)];
+        p = x[std::slice(m0, M.num_cols() - m0, 1)];
+    }
+    // Solve(A, B, C, v, p, b, c, ...) (:203-218): the saddle-point form [A B^T; B C]
+    void Solve(const MatrixCL &A, const MatrixCL &B, const MatrixCL &C, VectorCL &v, VectorCL &p, const VectorCL &b,
+               const VectorCL &c, const DummyExchangeCL &ex0, const DummyExchangeCL &ex1) {
+        MatrixCL BT;
+        transpose(B, BT);
+        Solve(A, BT, B, C, v, p, b, c, ex0, ex1);
     }
 };
 
@@ -1238,5 +1364,149 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
     else
         t(mg, 0, mg.GetNumTriangTetra());
 }
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard stepper (CahnHilliardcGP1CL, surfactant/ifacetransp.h:2947-2990)
+// ---------------------------------------------------------------------------
+typedef std::function<double(const double *, double)> InstatScalarFunction;
+
+// The statistics of the last block solve (the reference's block_gm_.GetIter / GetResid).
+class CHSolveStatsCL {
+    friend class CahnHilliardcGP1CL;
+    int iter_ = -1;
+    double resid_ = -1.;
+
+  public:
+    int GetIter() const { return iter_; }
+    double GetResid() const { return resid_; }
+};
+
+// The reference's class with theta = 1 (DoStep's BDF1), a resting interface (v == nullptr: no
+// convection or mass-divergence matrices) and no right-hand-side functions, on the device stepper
+// dhip_ch_*: the interface numbering, the step system (mobility from the current concentration,
+// the blocks of DoStep, Precond3/Precond4 diagonals) and GMRES(restart) all run on the GPU.  The
+// preconditioner is block Jacobi with the Precond3/Precond4 diagonals in place of the
+// reference's inner PCG solves.  ic / imu hold the concentration and chemical potential after each
+// DoStep.  The time step is new_t - t; a different step from the last one re-creates the device
+// system.
+class CahnHilliardcGP1CL {
+  public:
+    IdxDescCL idx;
+    VecDescCL ic, imu;
+
+  private:
+    MultiGridCL &mg_;
+    VecDescCL &lset_;
+    double sigma_, epsilon_, rho_, S_;
+    int iter_;
+    double tol_;
+    int restart_ = 10;   // GMRes_(block_pc_, 10, iter, tol, false, ...) (:2888)
+    int precision_ = DHIP_FP64;
+    double t_ = 0., dt_ = 0.;
+    dhip_ch_t *h_ = nullptr;
+    CHSolveStatsCL stats_;
+
+    void release() {
+        if (h_) dhip_ch_destroy(h_);
+        h_ = nullptr;
+    }
+    void make(double dt) {
+        DeviceCL &dev = DeviceCL::instance();
+        release();
+        const auto &B = mg_.brick();
+        const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                             B.e[2][0], B.e[2][1], B.e[2][2]};
+        const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+        dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+        dev.check(dhip_levelset_p2(dev.ctx(), lset_.Data.raw(), (int64_t)lset_.Data.size(), DHIP_HOST));
+        dhip_ch_params_t p{};
+        p.dt = dt;
+        p.epsilon = epsilon_;
+        p.rho = rho_;
+        p.sigma = sigma_;
+        p.S = S_;
+        p.restart = restart_;
+        p.max_iter = iter_;
+        p.tol = tol_;
+        p.rel = 0;   // the reference's block GMRES measures the absolute residual
+        int64_t nd = 0;
+        dev.check(dhip_ch_create(dev.ctx(), DHIP_ZERO_THROW, &p, &h_, &nd));
+        if ((size_t)nd != idx.NumUnknowns())
+            throw DROPSErrCL("CahnHilliardcGP1CL: the stepper's numbering differs from idx");
+        dev.check(dhip_ch_set_precision(h_, precision_));
+        dt_ = dt;
+    }
+
+  public:
+    // CahnHilliardcGP1CL(mg, theta, sigma, epsilon, v, Bnd_v, lset_vd, lsetbnd, normal, width, rho, S,
+    // iter, tol, ...) (:2960-2969); v must be nullptr and theta 1, the normal is the level set's
+    CahnHilliardcGP1CL(MultiGridCL &mg, double theta, double sigma, double epsilon, VecDescCL *v, const BndDataCL &,
+                       VecDescCL &lset_vd, const BndDataCL &, const void * /*normal*/, const double & /*width*/,
+                       double rho, double S, int iter = 999, double tol = 1.1e-7)
+        : idx(P1IF_FE), mg_(mg), lset_(lset_vd), sigma_(sigma), epsilon_(epsilon), rho_(rho), S_(S), iter_(iter),
+          tol_(tol) {
+        if (theta != 1.0) throw DROPSErrCL("CahnHilliardcGP1CL: the device stepper implements theta = 1");
+        if (v) throw DROPSErrCL("CahnHilliardcGP1CL: the device stepper implements a resting interface (v == 0)");
+    }
+    ~CahnHilliardcGP1CL() { release(); }
+    CahnHilliardcGP1CL(const CahnHilliardcGP1CL &) = delete;
+    CahnHilliardcGP1CL &operator=(const CahnHilliardcGP1CL &) = delete;
+
+    // idx.CreateNumbering + ic/imu.SetIdx, as the application does before SetInitialValue
+    void CreateNumbering(const BndDataCL *bnd = 0) {
+        idx.CreateNumbering(0, mg_, &lset_, bnd);
+        ic.SetIdx(&idx);
+        imu.SetIdx(&idx);
+        release();
+    }
+    // SetInitialValue(icmu, icc, ics, t) (surfactant/ifacetransp.cpp:1477-1490): the functions
+    // interpolated at the DOF vertices (no third species here)
+    void SetInitialValue(InstatScalarFunction icmu, InstatScalarFunction icc, InstatScalarFunction = nullptr,
+                         double t = 0.) {
+        if (idx.NumUnknowns() == 0 && ic.Data.size() == 0) CreateNumbering();
+        const auto &B = mg_.brick();
+        const long n1 = B.n[0] + 1, n2 = B.n[1] + 1;
+        const auto &vod = idx.vertex_of_dof();
+        for (size_t i = 0; i < vod.size(); ++i) {
+            const int64_t v = vod[i];
+            double x[3];
+            mg_.halfgrid_coord(2 * (long)(v % n1), 2 * (long)((v / n1) % n2), 2 * (long)(v / (n1 * n2)), x);
+            ic.Data[i] = icc ? icc(x, t) : 0.;
+            imu.Data[i] = icmu ? icmu(x, t) : 0.;
+        }
+        t_ = t;
+    }
+    void SetTime(double t) { t_ = t; }
+    double GetTime() const { return t_; }
+    // not in the reference: GMRES precision of the solves (DHIP_FP32_IR: FP32 Arnoldi cycles in
+    // FP64 restarts)
+    void SetPrecision(int p) {
+        precision_ = p;
+        if (h_) DeviceCL::instance().check(dhip_ch_set_precision(h_, p));
+    }
+    void SetRestart(int m) {
+        restart_ = m;
+        release();
+    }
+    // InitTimeStep (:1492-1527): the old level set and velocity are kept by the caller's objects here
+    void InitTimeStep() {}
+    // DoStep(new_t) (:1858-1868): InitStep + DoStep(rhs3, rhs4) + CommitStep
+    void DoStep(double new_t) {
+        const double dt = new_t - t_;
+        if (!(dt > 0.)) throw DROPSErrCL("CahnHilliardcGP1CL::DoStep: new_t must lie after the current time");
+        if (!h_ || dt != dt_) make(dt);
+        DeviceCL &dev = DeviceCL::instance();
+        dev.check(dhip_ch_set_state(h_, ic.Data.raw(), imu.Data.raw(), DHIP_HOST));
+        int32_t it = 0;
+        double res = 0.;
+        const int rc = dhip_ch_step(h_, 1, &it, &res);
+        if (rc != DHIP_OK && rc != DHIP_E_NOT_CONVERGED) dev.check(rc);
+        dev.check(dhip_ch_get_state(h_, ic.Data.raw(), imu.Data.raw(), DHIP_HOST));
+        stats_.iter_ = it;
+        stats_.resid_ = res;
+        t_ = new_t;
+    }
+    const CHSolveStatsCL &GetSolver() const { return stats_; }
+};
 
 }  // namespace DROPS

@@ -11,6 +11,10 @@
 //   tracefem_p1 accus <n> <p1|p2>           -> user-written accumulators (cut-tet count, patch area, visit order)
 //                                              in one tuple with the device-assembled interface accumulators
 //   tracefem_p1 colors <n>                  -> ColorClassesCL of the n^3 brick, checked (host only)
+//   tracefem_p1 ch <n> <steps> <outdir>     -> CahnHilliardcGP1CL on the torus: SetInitialValue, DoStep x steps;
+//                                              writes the initial and final c, mu and the iterations
+//   tracefem_p1 block <n> <outdir>          -> BlockMatrixSolverCL<GMResSolverCL<DiagBlockJacobiPcCL>> on the
+//                                              CH blocks from the accumulators, transp_mul and transpose
 #include <cstdio>
 #include <fstream>
 #include <cstdlib>
@@ -226,6 +230,128 @@ static int p2(int n, const std::string &out) {
     return 0;
 }
 
+// torus R = 1, r = 0.3 on [-2,2]^3 moved by -s, s = 1e-3 h (1,2,3)/sqrt(14), as the c1 problem
+static void torus_setup(int n, MultiGridCL *&mg, LevelsetP2CL &lset) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    mg = new MultiGridCL(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    lset.Init(*mg, [](const double *p, double) {
+        const double q = std::sqrt(p[0] * p[0] + p[1] * p[1]) - 1.0;
+        return (q * q + p[2] * p[2]) - 0.3 * 0.3;
+    });
+}
+
+// the smooth initial concentration of the CH examples
+static double ch_c0(const double *p, double) { return 0.5 + 0.05 * std::sin(3.0 * p[0]) * std::cos(2.0 * p[1] + p[2]); }
+
+static int ch(int n, int steps, const std::string &out) {
+    MultiGridCL *mg = nullptr;
+    LevelsetP2CL lset;
+    torus_setup(n, mg, lset);
+    BndDataCL bnd;
+    const double h = 4.0 / n, dt = 1e-4, width = 0.;
+    CahnHilliardcGP1CL ch(*mg, 1.0, 1.0, 0.05, nullptr, bnd, lset.Phi, bnd, nullptr, width, h, 1.0, 20000, 1e-13);
+    ch.SetRestart(50);
+    ch.CreateNumbering(&bnd);
+    ch.SetInitialValue([](const double *, double) { return 0.0; }, ch_c0);
+    std::printf("NumUnknowns: %zu\n", ch.idx.NumUnknowns());
+    write_bin(out + "/c0.bin", ch.ic.Data.raw(), ch.ic.Data.size() * 8);
+    std::vector<int32_t> its;
+    double t = 0.;
+    for (int s = 0; s < steps; ++s) {
+        ch.InitTimeStep();
+        t += dt;
+        ch.DoStep(t);
+        its.push_back(ch.GetSolver().GetIter());
+        std::printf("step %d: iter %d resid %.3e\n", s + 1, ch.GetSolver().GetIter(), ch.GetSolver().GetResid());
+    }
+    write_bin(out + "/c.bin", ch.ic.Data.raw(), ch.ic.Data.size() * 8);
+    write_bin(out + "/mu.bin", ch.imu.Data.raw(), ch.imu.Data.size() * 8);
+    write_bin(out + "/iters.bin", its.data(), its.size() * 4);
+    write_bin(out + "/vertex_of_dof.bin", ch.idx.vertex_of_dof().data(), ch.idx.vertex_of_dof().size() * 8);
+    delete mg;
+    return 0;
+}
+
+// The blocks of CahnHilliardcGP1CL::DoStep with the mobility matrix replaced by the Laplace-Beltrami
+// matrix (constant mobility 1), assembled by the accumulators, solved by BlockMatrixSolverCL.
+static int block(int n, const std::string &out) {
+    MultiGridCL *mg = nullptr;
+    LevelsetP2CL lset;
+    torus_setup(n, mg, lset);
+    BndDataCL bnd;
+    IdxDescCL idx(P1IF_FE);
+    idx.CreateNumbering(0, *mg, &lset.Phi, &bnd);
+    MatDescCL M(&idx, &idx), A(&idx, &idx), C(&idx, &idx);
+    TetraAccumulatorTupleCL accus;
+    InterfaceCommonDataP1CL cdata(lset.Phi, bnd);
+    accus.push_back(&cdata);
+    InterfaceMatrixAccuCL<LocalInterfaceMassP1CL, InterfaceCommonDataP1CL> mass(&M, LocalInterfaceMassP1CL(), cdata, "mass");
+    InterfaceMatrixAccuCL<LocalLaplaceBeltramiP1CL, InterfaceCommonDataP1CL> lb(&A, LocalLaplaceBeltramiP1CL(1.), cdata, "LB");
+    InterfaceMatrixAccuCL<LocalNormalLaplaceBulkP1CL, InterfaceCommonDataP1CL> st(&C, LocalNormalLaplaceBulkP1CL(1.), cdata, "stab");
+    accus.push_back(&mass);
+    accus.push_back(&lb);
+    accus.push_back(&st);
+    accumulate(accus, *mg, 0, &bnd);
+    const size_t nd = idx.NumUnknowns();
+    const double dt = 1e-4, eps = 0.05, rho = 4.0 / n, sigma = 1.0, S = 1.0;
+    MatrixCL A_, B_, C_, D_;
+    A_.LinComb(sigma * dt, A.Data, rho * dt, C.Data);
+    B_.LinComb(1.0, M.Data, 0.0, M.Data);
+    C_.LinComb(0., A.Data, -eps, M.Data);
+    D_.LinComb(eps * eps, A.Data, S, M.Data, rho * eps * eps, C.Data);
+    VectorCL c0(nd), mu(nd), c(nd);
+    for (size_t i = 0; i < nd; ++i) c0[i] = 0.5 + 0.05 * std::sin(0.37 * (double)i);
+    c = c0;
+    const VectorCL rhs3 = M.Data * c0;
+    VectorCL rhs4(nd);
+    for (size_t i = 0; i < nd; ++i) rhs4[i] = S * rhs3[i];
+    // the block-Jacobi diagonal: Precond3 = M + sigma dt A + dt rho C, Precond4 = M + eps^2 A + eps^2 rho C
+    MatrixCL P3, P4;
+    P3.LinComb(1.0, M.Data, sigma * dt, A.Data, dt * rho, C.Data);
+    P4.LinComb(1.0, M.Data, eps * eps, A.Data, eps * eps * rho, C.Data);
+    VectorCL d(2 * nd);
+    for (size_t i = 0; i < nd; ++i)
+        for (size_t k = P3.row_beg(i); k < P3.row_beg(i + 1); ++k)
+            if (P3.col_ind(k) == i) {
+                d[i] = P3.raw_val()[k];
+                d[nd + i] = P4.raw_val()[k];
+            }
+    DiagBlockJacobiPcCL pc(d);
+    GMResSolverCL<DiagBlockJacobiPcCL> gm(pc, 50, 20000, 1e-10, true, false, RightPreconditioning);
+    BlockMatrixSolverCL<GMResSolverCL<DiagBlockJacobiPcCL>> solver(gm);
+    solver.Solve(A_, B_, C_, D_, mu, c, rhs3, rhs4, DummyExchangeCL(), DummyExchangeCL());
+    std::printf("block GMRES iterations: %d\nblock GMRES residual: %.6e\n", solver.GetIter(), solver.GetResid());
+    // y_ATx on the device and transpose on the host: (C_^T x) against (transpose(C_)) x
+    const VectorCL yt = transp_mul(C_, c);
+    MatrixCL CT;
+    transpose(C_, CT);
+    const VectorCL yt2 = CT * c;
+    double dmax = 0.;
+    for (size_t i = 0; i < nd; ++i) dmax = std::max(dmax, std::fabs(yt[i] - yt2[i]));
+    std::printf("transp_mul vs transpose: %.3e\n", dmax);
+    auto dump = [&](const std::string &name, const MatrixCL &X) {
+        std::vector<int64_t> rp(X.num_rows() + 1), ci(X.num_nonzeros());
+        for (size_t i = 0; i <= X.num_rows(); ++i) rp[i] = (int64_t)X.raw_row()[i];
+        for (size_t k = 0; k < X.num_nonzeros(); ++k) ci[k] = (int64_t)X.raw_col()[k];
+        write_bin(out + "/" + name + "_rowptr.bin", rp.data(), rp.size() * 8);
+        write_bin(out + "/" + name + "_colind.bin", ci.data(), ci.size() * 8);
+        write_bin(out + "/" + name + "_val.bin", X.raw_val(), X.num_nonzeros() * 8);
+    };
+    dump("A", A_);
+    dump("B", B_);
+    dump("C", C_);
+    dump("D", D_);
+    write_bin(out + "/rhs3.bin", rhs3.raw(), nd * 8);
+    write_bin(out + "/rhs4.bin", rhs4.raw(), nd * 8);
+    write_bin(out + "/mu.bin", mu.raw(), nd * 8);
+    write_bin(out + "/c.bin", c.raw(), nd * 8);
+    write_bin(out + "/yt.bin", yt.raw(), nd * 8);
+    delete mg;
+    return 0;
+}
+
 // A user accumulator as a DROPS application writes one (num/accumulator.h:42-66): it counts the
 // visited tets, the tets with an interface patch and sums the patch areas through the tuple's
 // interface common data, and records the visit order and the begin/finalize calls.
@@ -375,6 +501,8 @@ int main(int argc, char **argv) {
                 std::atoi(argv[2]), P1IF_FE);
         }
         if (argc >= 3 && !std::strcmp(argv[1], "colors")) return colors(std::atoi(argv[2]));
+        if (argc >= 5 && !std::strcmp(argv[1], "ch")) return ch(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
+        if (argc >= 4 && !std::strcmp(argv[1], "block")) return block(std::atoi(argv[2]), argv[3]);
         if (argc >= 4 && !std::strcmp(argv[1], "band"))
             return band(std::atoi(argv[2]), std::atof(argv[3]), argc >= 5 ? argv[4] : "");
         std::fprintf(stderr,

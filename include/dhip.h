@@ -220,6 +220,11 @@ int dhip_csr_info(const dhip_csr_t *A, int64_t *nrows, int64_t *nnz);
 
 /* ---- vector core (num/spmat.h:164-235, 1683-1722) ---- */
 int dhip_spmv(dhip_ctx_t *ctx, const dhip_csr_t *A, const double *x, double *y, int32_t where);
+/* y = A^T x (transp_mul / y_ATx, num/spmat.h:1729-1757): y has ncols entries, x has A's row count.
+ * The transposed pattern is built for the call (counts, scan, fill, each row sorted by original row)
+ * so that every y_j is summed in the row order of y_ATx, bit for bit; a column index >= ncols is
+ * DHIP_E_ARG. */
+int dhip_spmv_t(dhip_ctx_t *ctx, const dhip_csr_t *A, int64_t ncols, const double *x, double *y, int32_t where);
 int dhip_dot(dhip_ctx_t *ctx, int64_t n, const double *x, const double *y, double *result,
              int32_t where);
 int dhip_axpy(dhip_ctx_t *ctx, int64_t n, double a, const double *x, double *y, int32_t where);

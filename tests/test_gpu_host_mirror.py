@@ -8,6 +8,8 @@ import subprocess
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 import cases
 import oracle as O
@@ -151,3 +153,51 @@ def test_user_accumulator_next_to_device_accumulators(space, threads):
     area, msum = float(v["area"]), float(v["mass sum"])
     assert abs(msum - ref["mass"].sum()) <= 1e-13 * abs(msum)
     assert abs(area - msum) <= 1e-12 * abs(msum)
+
+
+def test_ch_class_through_host_mirror(tmp_path):
+    """CahnHilliardcGP1CL of the mirror (CreateNumbering, SetInitialValue, DoStep x 5 with dt 1e-4)
+    against five oracle steps from the same interpolated initial state: numbering equal, trajectory
+    within 1e-6 (GMRES tolerances 1e-13 absolute on the device -- the class's absolute residual --,
+    1e-12 relative in the oracle)."""
+    n, steps = 16, 5
+    out = run("ch", str(n), str(steps), str(tmp_path))
+    assert len(re.findall(r"step \d+: iter \d+", out)) == steps
+    ld = lambda name, dt: np.fromfile(tmp_path / f"{name}.bin", dtype=dt)
+    m, phi = cases.torus_case(n)
+    om = O.make_mesh(m.orig, m.e, m.n)
+    dof, nd = O.number_p1if(om, phi)
+    vod = ld("vertex_of_dof", np.int64)
+    assert vod.size == nd and np.array_equal(dof[vod], np.arange(nd))
+    r = O.assemble_p1(om, phi, dof, nd, coef=(1.0, 1.0, 1.0))
+    c, mu = ld("c0", np.float64), np.zeros(nd)
+    for _ in range(steps):
+        o = O.ch_step(om, phi, dof, nd, r["rowptr"], r["colind"], r["mass"], r["lb"], r["stab"], c, mu, dt=1e-4,
+                      eps=0.05, rho=m.h, tol=1e-12, restart=50)
+        c, mu = o["c"], o["mu"]
+    assert np.linalg.norm(ld("c", np.float64) - c) <= 1e-6 * np.linalg.norm(c)
+    assert np.linalg.norm(ld("mu", np.float64) - mu) <= 1e-6 * np.linalg.norm(mu)
+
+
+def test_block_solver_through_host_mirror(tmp_path):
+    """BlockMatrixSolverCL<GMResSolverCL<DiagBlockJacobiPcCL>>::Solve(A, B, C, D, ...) against a direct
+    solve of scipy's bmat([[A, B], [C, D]]); transp_mul on the device equals transpose() + SpMV bit for
+    bit and scipy's C^T x to rounding."""
+    n = 16
+    out = run("block", str(n), str(tmp_path))
+    assert float(re.search(r"transp_mul vs transpose: (\S+)", out).group(1)) == 0.0
+    ld = lambda name, dt: np.fromfile(tmp_path / f"{name}.bin", dtype=dt)
+    mats = {}
+    for k in "ABCD":
+        rp, ci, v = ld(k + "_rowptr", np.int64), ld(k + "_colind", np.int64), ld(k + "_val", np.float64)
+        mats[k] = sp.csr_matrix((v, ci, rp), shape=(rp.size - 1, rp.size - 1))
+    K = sp.bmat([[mats["A"], mats["B"]], [mats["C"], mats["D"]]]).tocsc()
+    b = np.concatenate([ld("rhs3", np.float64), ld("rhs4", np.float64)])
+    xd = spla.spsolve(K, b)
+    x = np.concatenate([ld("mu", np.float64), ld("c", np.float64)])
+    assert np.linalg.norm(b - K @ x) <= 2e-10 * np.linalg.norm(b)
+    assert np.linalg.norm(x - xd) <= 1e-6 * np.linalg.norm(xd)
+    yt = ld("yt", np.float64)
+    yref = mats["C"].T @ ld("c", np.float64)
+    bound = (abs(mats["C"]).T @ np.abs(ld("c", np.float64))).max()
+    assert np.abs(yt - yref).max() <= 1e-14 * bound
