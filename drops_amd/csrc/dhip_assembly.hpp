@@ -395,7 +395,15 @@ __device__ inline void stab_p1(const LocalArgs &a, const double ls[10], const do
     }
 }
 
-__device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
+// The local matrices of one cut tet.  ROLE < 0: every output; otherwise one group, so that four
+// waves can share a tet: 0 mass + numbering integrals, 1 Laplace-Beltrami, 2 load vector, area and
+// the flags, 3 stabilisation.  Each output is summed by the same statements in the same order in
+// every form, so the parts equal the whole bit for bit.
+template <int ROLE>
+__device__ inline void local_p1_part(const LocalArgs &a, int64_t c) {
+    constexpr bool ALL = ROLE < 0;
+    constexpr bool MASS = ALL || ROLE == 0, LB = ALL || ROLE == 1, LOAD = ALL || ROLE == 2, STAB = ALL || ROLE == 3;
+    constexpr bool FACETS = MASS || LB || LOAD;
     double X[4][3], ls[10];
     load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
     double G1[4][3];
@@ -413,12 +421,18 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) s[q] = sgn(ls[c_tab.lat_t[lt][q]]);
         if (!s[0] && !s[1] && !s[2] && !s[3]) {
-            atomicOr(a.err, 2);
-            a.flags[c] = 0;   // defined for the steps that run before the error is read
+            if (LOAD) {
+                atomicOr(a.err, 2);
+                a.flags[c] = 0;   // defined for the steps that run before the error is read
+            }
             return;
         }
         const int pid = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;
         const int ntri = c_tab.rp_size[pid];
+        if constexpr (!FACETS) {
+            nfacet += ntri;
+            continue;
+        }
         const bool bnd = c_tab.rp_bnd[pid] != 0;
         for (int tr = 0; tr < ntri; ++tr) {
             double vb[3][4];
@@ -429,30 +443,32 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
             double af = bnd ? 0.5 : 1.;
             af *= absdet;
             // Quad2 composite: mass + numbering integrals
+            if (MASS) {
 #pragma unroll
-            for (int qn = 0; qn < 4; ++qn) {
-                double nb[4];
+                for (int qn = 0; qn < 4; ++qn) {
+                    double nb[4];
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    double sv = vb[0][cc] * c_tab.q2n[qn][0];
-                    sv += vb[1][cc] * c_tab.q2n[qn][1];
-                    sv += vb[2][cc] * c_tab.q2n[qn][2];
-                    nb[cc] = sv;
+                    for (int cc = 0; cc < 4; ++cc) {
+                        double sv = vb[0][cc] * c_tab.q2n[qn][0];
+                        sv += vb[1][cc] * c_tab.q2n[qn][1];
+                        sv += vb[2][cc] * c_tab.q2n[qn][2];
+                        nb[cc] = sv;
+                    }
+                    const double wq = af * c_tab.q2w[qn];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        mdiag[i] += wq * (nb[i] * nb[i]);
+                        hat[i] += wq * hat_sq_eval(i, nb);
+                    }
+                    int o = 0;
+#pragma unroll
+                    for (int i = 1; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < i; ++j) moff[o++] += wq * (nb[j] * nb[i]);
                 }
-                const double wq = af * c_tab.q2w[qn];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    mdiag[i] += wq * (nb[i] * nb[i]);
-                    hat[i] += wq * hat_sq_eval(i, nb);
-                }
-                int o = 0;
-#pragma unroll
-                for (int i = 1; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < i; ++j) moff[o++] += wq * (nb[j] * nb[i]);
             }
             // Laplace-Beltrami: projected gradients per facet
-            {
+            if (LB) {
                 double q[4][3];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -467,24 +483,26 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
                     for (int j = 0; j <= i; ++j) lbs[o++] += dot3(q[i], q[j]) * absdet;
             }
             // Quad5 composite: load vector and area
+            if (LOAD) {
 #pragma unroll
-            for (int qn = 0; qn < 7; ++qn) {
-                double nb[4];
+                for (int qn = 0; qn < 7; ++qn) {
+                    double nb[4];
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) {
-                    double sv = vb[0][cc] * c_tab.q5n[qn][0];
-                    sv += vb[1][cc] * c_tab.q5n[qn][1];
-                    sv += vb[2][cc] * c_tab.q5n[qn][2];
-                    nb[cc] = sv;
-                }
-                const double wq = af * c_tab.q5w[qn];
-                area += wq * 1.;
-                if (a.rhs_kind != DHIP_RHS_NONE) {
-                    double x[3];
-                    b2w(X, nb, x);
-                    const double f = rhs_fun(a.rhs_kind, a.rhs_p0, x);
+                    for (int cc = 0; cc < 4; ++cc) {
+                        double sv = vb[0][cc] * c_tab.q5n[qn][0];
+                        sv += vb[1][cc] * c_tab.q5n[qn][1];
+                        sv += vb[2][cc] * c_tab.q5n[qn][2];
+                        nb[cc] = sv;
+                    }
+                    const double wq = af * c_tab.q5w[qn];
+                    area += wq * 1.;
+                    if (a.rhs_kind != DHIP_RHS_NONE) {
+                        double x[3];
+                        b2w(X, nb, x);
+                        const double f = rhs_fun(a.rhs_kind, a.rhs_p0, x);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) rhs[i] += wq * (f * nb[i]);
+                        for (int i = 0; i < 4; ++i) rhs[i] += wq * (f * nb[i]);
+                    }
                 }
             }
         }
@@ -492,15 +510,29 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
 
     const int64_t N = a.ncut;
     double *L = a.loc;
-    a.flags[c] = nfacet > 0 ? 1 : (a.band ? 0x21 : 0);
+    if (LOAD) a.flags[c] = nfacet > 0 ? 1 : (a.band ? 0x21 : 0);
     if (nfacet == 0) {
-        for (int e = 0; e < LOC_N; ++e) L[e * N + c] = 0.;
-        // NarrowBandMatrixAccuP1CL: the stabilisation of a band tet does not need the patch
-        if (a.band) stab_p1(a, ls, G1, det, c);
+        auto zero = [&](int e0, int cnt) {
+            for (int e = e0; e < e0 + cnt; ++e) L[e * N + c] = 0.;
+        };
+        if (MASS) {
+            zero(LOC_MASS, 16);
+            zero(LOC_HAT, 4);
+        }
+        if (LB) zero(LOC_LB, 16);
+        if (LOAD) {
+            zero(LOC_RHS, 4);
+            zero(LOC_AREA, 1);
+        }
+        if (STAB) {
+            zero(LOC_STAB, 16);
+            // NarrowBandMatrixAccuP1CL: the stabilisation of a band tet does not need the patch
+            if (a.band) stab_p1(a, ls, G1, det, c);
+        }
         return;
     }
     // mass: diagonal without alpha, off-diagonal alpha * integral
-    {
+    if (MASS) {
         int o = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -512,8 +544,10 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
                 L[(LOC_MASS + j * 4 + i) * N + c] = v;
             }
         }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) L[(LOC_HAT + i) * N + c] = hat[i];
     }
-    {
+    if (LB) {
         const double fac = a.coef1 * 0.5;
         int o = 0;
 #pragma unroll
@@ -525,15 +559,18 @@ __device__ inline void local_p1_one(const LocalArgs &a, int64_t c) {
                 L[(LOC_LB + j * 4 + i) * N + c] = v;
             }
     }
+    if (LOAD) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        L[(LOC_RHS + i) * N + c] = rhs[i];
-        L[(LOC_HAT + i) * N + c] = hat[i];
+        for (int i = 0; i < 4; ++i) L[(LOC_RHS + i) * N + c] = rhs[i];
+        L[LOC_AREA * N + c] = area;
     }
-    L[LOC_AREA * N + c] = area;
-    stab_p1(a, ls, G1, det, c);
+    if (STAB) stab_p1(a, ls, G1, det, c);
 }
 
+__device__ inline void local_p1_one(const LocalArgs &a, int64_t c) { local_p1_part<-1>(a, c); }
+
+// One lane per cut tet.  Splitting the outputs over four waves or four launches (local_p1_part<0..3>)
+// was measured slower: the facet geometry, which every group needs, dominates the cost.
 __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
     const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
     if (c >= a.ncut) return;
