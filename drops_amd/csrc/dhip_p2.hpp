@@ -83,6 +83,7 @@ struct P2Shared {
     double w[P2_CHUNK];
     double f[P2_CHUNK];
     double U[15][10];
+    double n5[15][3];   // P2 level-set unit normal at the 15 volume points
     int nf;
 };
 
@@ -216,12 +217,19 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
     }
 
     // ---- normal-gradient stabilisation: U[q][d] = n(x_q) . grad h_d(x_q), Quad5 volume rule ----
+    // (the normal once per point, then one lane per (point, basis function))
+    if (lane < 15) {
+        double n3[3];
+        p2_unit_normal(v, c_tab.v5n[lane], G1, n3);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) sh.n5[lane][cc] = n3[cc];
+    }
+    __syncthreads();
     for (int t = lane; t < 150; t += 64) {
         const int q = t / 10, d = t - 10 * (t / 10);
-        double n3[3], g[3];
-        p2_unit_normal(v, c_tab.v5n[q], G1, n3);
+        double g[3];
         p2_grad_one(d, c_tab.v5n[q], G1, g);
-        sh.U[q][d] = dot3(n3, g);
+        sh.U[q][d] = dot3(sh.n5[q], g);
     }
     __syncthreads();
     if (lane < 55) {
