@@ -888,6 +888,37 @@ class IdxDescCL {
         if (num_) dev.check(dhip_get_dof_nodes(dev.ctx(), node_of_dof_.data()));
         return (size_t)ncut;
     }
+    // DistributeDOFs(level, mg, levelSet, dist) (misc/problem.cpp:283-336) for P1IF_FE: every vertex of
+    // every tet with distance() <= dist gets the next number on first encounter in tet order -- the
+    // near-interface numbering of the narrow band (CreateNumbNearInterfaceVertex) at width dist, with
+    // no integral test; dist = 0 takes the cut tets.  Returns the number of marked tets.  (The
+    // reference's re-mapping of the vectors registered on the old numbering is left to the caller.)
+    size_t DistributeDOFs(unsigned level, const MultiGridCL &mg, const VecDescCL *levelSet = nullptr,
+                          double dist = 0.) {
+        if (fe_ != P1IF_FE) throw DROPSErrCL("IdxDescCL::DistributeDOFs: implemented for P1IF_FE");
+        if (!levelSet) throw DROPSErrCL("IdxDescCL::DistributeDOFs: the interface space needs the level set");
+        if (!(dist >= 0.)) throw DROPSErrCL("IdxDescCL::DistributeDOFs: dist must be >= 0");
+        DeviceCL &dev = DeviceCL::instance();
+        const auto &B = mg.brick();
+        const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                             B.e[2][0], B.e[2][1], B.e[2][2]};
+        const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+        dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+        dev.check(dhip_levelset_p2(dev.ctx(), levelSet->Data.raw(), (int64_t)levelSet->Data.size(), DHIP_HOST));
+        int64_t ntet = 0, nd = 0, nnz = 0;
+        const double coef[3] = {1., 1., 1.}, lc[3] = {1., 1., 1.};
+        (void)level;
+        dev.check(dhip_assemble_p1_band(dev.ctx(), DHIP_ZERO_THROW, coef, DHIP_RHS_NONE, nullptr, dist, lc, &ntet, &nd,
+                                        &nnz));
+        num_ = (size_t)nd;
+        vertex_of_dof_.assign(num_, 0);
+        node_of_dof_.assign(num_, 0);
+        if (num_) {
+            dev.check(dhip_get_dof_vertices(dev.ctx(), vertex_of_dof_.data()));
+            dev.check(dhip_get_dof_nodes(dev.ctx(), node_of_dof_.data()));
+        }
+        return (size_t)ntet;
+    }
 };
 
 inline VecDescCL::VecDescCL(IdxDescCL *idx) { SetIdx(idx); }

@@ -15,6 +15,8 @@
 //                                              writes the initial and final c, mu and the iterations
 //   tracefem_p1 block <n> <outdir>          -> BlockMatrixSolverCL<GMResSolverCL<DiagBlockJacobiPcCL>> on the
 //                                              CH blocks from the accumulators, transp_mul and transpose
+//   tracefem_p1 distribute <n> <dist> <outdir> -> IdxDescCL::DistributeDOFs on the torus: marked tets and the
+//                                              grid vertex of every DOF
 #include <cstdio>
 #include <fstream>
 #include <cstdlib>
@@ -352,6 +354,18 @@ static int block(int n, const std::string &out) {
     return 0;
 }
 
+static int distribute(int n, double dist, const std::string &out) {
+    MultiGridCL *mg = nullptr;
+    LevelsetP2CL lset;
+    torus_setup(n, mg, lset);
+    IdxDescCL idx(P1IF_FE);
+    const size_t ntet = idx.DistributeDOFs(0, *mg, &lset.Phi, dist);
+    std::printf("marked tets: %zu\nNumUnknowns: %zu\n", ntet, idx.NumUnknowns());
+    write_bin(out + "/vertex_of_dof.bin", idx.vertex_of_dof().data(), idx.vertex_of_dof().size() * 8);
+    delete mg;
+    return 0;
+}
+
 // A user accumulator as a DROPS application writes one (num/accumulator.h:42-66): it counts the
 // visited tets, the tets with an interface patch and sums the patch areas through the tuple's
 // interface common data, and records the visit order and the begin/finalize calls.
@@ -503,6 +517,8 @@ int main(int argc, char **argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "colors")) return colors(std::atoi(argv[2]));
         if (argc >= 5 && !std::strcmp(argv[1], "ch")) return ch(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
         if (argc >= 4 && !std::strcmp(argv[1], "block")) return block(std::atoi(argv[2]), argv[3]);
+        if (argc >= 5 && !std::strcmp(argv[1], "distribute"))
+            return distribute(std::atoi(argv[2]), std::atof(argv[3]), argv[4]);
         if (argc >= 4 && !std::strcmp(argv[1], "band"))
             return band(std::atoi(argv[2]), std::atof(argv[3]), argc >= 5 ? argv[4] : "");
         std::fprintf(stderr,

@@ -201,3 +201,32 @@ def test_block_solver_through_host_mirror(tmp_path):
     yref = mats["C"].T @ ld("c", np.float64)
     bound = (abs(mats["C"]).T @ np.abs(ld("c", np.float64))).max()
     assert np.abs(yt - yref).max() <= 1e-14 * bound
+
+
+@pytest.mark.parametrize("dist", [0.0, 0.1])
+def test_distribute_dofs_through_host_mirror(tmp_path, dist):
+    """IdxDescCL::DistributeDOFs (misc/problem.cpp:283-336) on the torus: the marked tets are those with
+    distance() <= dist and the DOFs are every vertex of a marked tet, numbered on first encounter in tet
+    order -- checked against a numpy restatement from the definition."""
+    n = 16
+    out = run("distribute", str(n), repr(dist), str(tmp_path))
+    ntet = int(re.search(r"marked tets: (\d+)", out).group(1))
+    vod = np.fromfile(tmp_path / "vertex_of_dof.bin", dtype=np.int64)
+    m, phi = cases.torus_case(n)
+    n1, n2, _ = m.n
+    h1, h2 = 2 * n1 + 1, 2 * n2 + 1
+    nodes = cases.tet_p2_nodes(m, np.arange(6 * n1 * n2 * m.n[2]))   # half-grid nodes of every tet
+    vals = np.asarray(phi).reshape(-1)[nodes]
+    j = nodes[:, :4]
+    tets = (((j // (h1 * h2)) // 2) * (n2 + 1) + ((j // h1) % h2) // 2) * (n1 + 1) + (j % h1) // 2
+    same = np.all(vals > 0, axis=1) | np.all(vals < 0, axis=1)
+    dist_t = np.where(same, np.abs(vals).min(axis=1), 0.0)
+    marked = np.flatnonzero(dist_t <= dist)
+    assert ntet == marked.size
+    seen, order = set(), []
+    for t in marked:
+        for v in tets[t]:
+            if v not in seen:
+                seen.add(v)
+                order.append(v)
+    assert np.array_equal(vod, np.array(order, np.int64))
