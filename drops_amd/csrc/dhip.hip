@@ -529,6 +529,7 @@ int dhip_mesh_kuhn(dhip_ctx_t *c, const double orig[3], const double e[9], const
     c->stage = 0;
     c->phi = nullptr;
     c->phi_count = 0;
+    ++c->geo_gen;
     return DHIP_OK;
 }
 
@@ -550,6 +551,7 @@ int dhip_levelset_p2(dhip_ctx_t *c, const double *phi, int64_t count, int32_t wh
     }
     c->phi_count = need;
     c->stage = 0;
+    ++c->geo_gen;
     return DHIP_OK;
 }
 
@@ -595,6 +597,19 @@ int dhip_classify(dhip_ctx_t *c, int32_t zero_policy, int64_t *ncut) {
     return flush_profile(c);
 }
 
+// The given right-hand side values, if they belong to the current cut list (DHIP_RHS_VALUES).
+static int rhs_values_for(dhip_ctx *c, int32_t rhs_kind, const double **vals, const int32_t **off) {
+    *vals = nullptr;
+    *off = nullptr;
+    if (rhs_kind != DHIP_RHS_VALUES) return DHIP_OK;
+    if (!c->qp_have_vals || c->qp_gen != c->geo_gen || c->qp_band != c->band || c->qp_ncut != c->ncut)
+        return set_error(c, DHIP_E_STATE,
+                         "DHIP_RHS_VALUES needs dhip_quad_points and dhip_rhs_values on the current cut tets");
+    *vals = c->qp_vals.as<double>();
+    *off = c->qp_off.as<int32_t>();
+    return DHIP_OK;
+}
+
 // check: read the error flags now; the assembly pipeline leaves that to the numbering's readback
 static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const double *rhs_param,
                       bool check = true) {
@@ -603,9 +618,13 @@ static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const
     DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC_N * (N + 1)));
     DHIP_HIPCHK(c, c->flags.ensure(N + 1));
     for (int k = 0; k < 3; ++k) c->coef[k] = coef ? coef[k] : 1.0;
+    const double *rv = nullptr;
+    const int32_t *ro = nullptr;
+    int rc0 = rhs_values_for(c, rhs_kind, &rv, &ro);
+    if (rc0) return rc0;
     LocalArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->coef[0], c->coef[1], c->coef[2], rhs_kind,
                  rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
-                 c->errflag.as<int32_t>(), c->band >= 0. ? 1 : 0};
+                 c->errflag.as<int32_t>(), c->band >= 0. ? 1 : 0, rv, ro};
     if (N > 0) DHIP_LAUNCH(c, "local_p1", k_local_p1, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la);
     if (check) {
         const int rc = check_err_flag(c);
@@ -614,6 +633,73 @@ static int local_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, const
     c->stage = 2;
     c->space = 1;
     return DHIP_OK;
+}
+
+// Maximal points of one tet (16 facets x 7): the slack after the given values, so that a tet whose
+// patch turns out to be invalid (a sub-tet in the zero level set, reported by the local pass) reads
+// inside the buffer.
+constexpr int64_t QP_SLACK = 112;
+
+int dhip_quad_points(dhip_ctx_t *c, int64_t *npoints, double *xyz, int32_t where) {
+    if (!c || !npoints) return DHIP_E_ARG;
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_quad_points");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t N = c->ncut;
+    DevBuf cnt;
+    DHIP_HIPCHK(c, cnt.ensure(sizeof(int32_t) * (N + 1)));
+    DHIP_HIPCHK(c, c->qp_off.ensure(sizeof(int32_t) * (N + 1)));
+    LocalArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, 1., 1., 1., DHIP_RHS_NONE, 0., nullptr, nullptr,
+                 nullptr, 0};
+    int64_t total = 0;
+    if (N > 0) {
+        DHIP_LAUNCH(c, "quad_points_count", k_quad_points<true>, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la,
+                    cnt.as<int32_t>(), (const int32_t *)nullptr, (double *)nullptr);
+        int rc = scan_exclusive_i32(c, cnt.as<int32_t>(), c->qp_off.as<int32_t>(), N, &total);
+        if (rc) return rc;
+    }
+    const int32_t t32 = (int32_t)total;
+    DHIP_HIPCHK(c, hipMemcpyAsync(c->qp_off.as<int32_t>() + N, &t32, sizeof t32, hipMemcpyHostToDevice, c->stream));
+    c->qp_n = total;
+    c->qp_ncut = N;
+    c->qp_gen = c->geo_gen;
+    c->qp_band = c->band;
+    c->qp_have_vals = false;
+    *npoints = total;
+    if (xyz && total > 0) {
+        DevBuf tmp;
+        double *dst = xyz;
+        if (where != DHIP_DEVICE) {
+            DHIP_HIPCHK(c, tmp.ensure(sizeof(double) * 3 * total));
+            dst = tmp.as<double>();
+        }
+        DHIP_LAUNCH(c, "quad_points", k_quad_points<false>, blocks_for(N, LOCAL_BLOCK), LOCAL_BLOCK, 0, la,
+                    (int32_t *)nullptr, (const int32_t *)c->qp_off.as<int32_t>(), dst);
+        if (where != DHIP_DEVICE)
+            DHIP_HIPCHK(c, hipMemcpyAsync(xyz, dst, sizeof(double) * 3 * total, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_rhs_values(dhip_ctx_t *c, const double *values, int64_t npoints, int32_t where) {
+    if (!c || (npoints > 0 && !values) || npoints < 0) return DHIP_E_ARG;
+    if (c->qp_n < 0 || c->qp_gen != c->geo_gen)
+        return set_error(c, DHIP_E_STATE, "dhip_quad_points must precede dhip_rhs_values on this level set");
+    if (npoints != c->qp_n)
+        return set_error(c, DHIP_E_ARG, "%lld values for %lld quadrature points", (long long)npoints,
+                         (long long)c->qp_n);
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DHIP_HIPCHK(c, c->qp_vals.ensure(sizeof(double) * (npoints + QP_SLACK)));
+    if (npoints > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(c->qp_vals.p, values, sizeof(double) * npoints,
+                                      where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                      c->stream));
+    DHIP_LAUNCH(c, "rhs_values_slack", k_zero_u32, 1, 256, 0, (uint32_t *)(c->qp_vals.as<double>() + npoints),
+                (int)(2 * QP_SLACK));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->qp_have_vals = true;
+    return flush_profile(c);
 }
 
 int dhip_local_p1(dhip_ctx_t *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {
@@ -747,9 +833,13 @@ static int local_p2_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, co
     DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC2_N * (N + 1)));
     DHIP_HIPCHK(c, c->flags.ensure(N + 1));
     for (int k = 0; k < 3; ++k) c->coef[k] = coef ? coef[k] : 1.0;
+    const double *rv = nullptr;
+    const int32_t *ro = nullptr;
+    int rc0 = rhs_values_for(c, rhs_kind, &rv, &ro);
+    if (rc0) return rc0;
     LocalP2Args la{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->coef[0], c->coef[1], c->coef[2], rhs_kind,
                    rhs_param ? rhs_param[0] : 0.0, c->loc.as<double>(), c->flags.as<uint8_t>(),
-                   c->errflag.as<int32_t>()};
+                   c->errflag.as<int32_t>(), rv, ro};
     if (N > 0) DHIP_LAUNCH(c, "local_p2", k_local_p2, (unsigned)N, 64, 0, la);
     int rc = check_err_flag(c);
     if (rc) return rc;

@@ -347,6 +347,8 @@ struct LocalArgs {
     uint8_t *flags;   // bit0: contributes (non-empty patch, or a narrow-band tet); bit5: band tet without patch
     int32_t *err;
     int band = 0;     // narrow band: listed tets without a patch still carry the stabilisation
+    const double *rhs_vals = nullptr;   // DHIP_RHS_VALUES: f at every patch quadrature point
+    const int32_t *rhs_off = nullptr;   //   first point of each cut tet
 };
 
 constexpr int LOCAL_BLOCK = 64;
@@ -415,6 +417,7 @@ __device__ inline void local_p1_part(const LocalArgs &a, int64_t c) {
     double hat[4] = {0., 0., 0., 0.}, rhs[4] = {0., 0., 0., 0.};
     double area = 0.;
     int nfacet = 0;
+    const double *fvals = (LOAD && a.rhs_kind == DHIP_RHS_VALUES) ? a.rhs_vals + a.rhs_off[c] : nullptr;
 
     for (int lt = 0; lt < 8; ++lt) {
         int s[4];
@@ -497,9 +500,14 @@ __device__ inline void local_p1_part(const LocalArgs &a, int64_t c) {
                     const double wq = af * c_tab.q5w[qn];
                     area += wq * 1.;
                     if (a.rhs_kind != DHIP_RHS_NONE) {
-                        double x[3];
-                        b2w(X, nb, x);
-                        const double f = rhs_fun(a.rhs_kind, a.rhs_p0, x);
+                        double f;
+                        if (fvals) {
+                            f = fvals[7 * (nfacet - 1) + qn];
+                        } else {
+                            double x[3];
+                            b2w(X, nb, x);
+                            f = rhs_fun(a.rhs_kind, a.rhs_p0, x);
+                        }
 #pragma unroll
                         for (int i = 0; i < 4; ++i) rhs[i] += wq * (f * nb[i]);
                     }
@@ -575,6 +583,60 @@ __global__ void __launch_bounds__(LOCAL_BLOCK) k_local_p1(LocalArgs a) {
     const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
     if (c >= a.ncut) return;
     local_p1_one(a, c);
+}
+
+// The composite-Quad5 points of every cut tet's patch in the load vector's order (facets in
+// make_patch order, 7 nodes each): COUNT writes 7 x facets per tet, else the world coordinates from
+// off[c] on.  A sub-tet lying in the zero level set gives no points (the local pass reports it).
+template <bool COUNT>
+__global__ void __launch_bounds__(LOCAL_BLOCK) k_quad_points(LocalArgs a, int32_t *cnt, const int32_t *off,
+                                                             double *xyz) {
+    const int64_t c = (int64_t)blockIdx.x * LOCAL_BLOCK + threadIdx.x;
+    if (c >= a.ncut) return;
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+    int pids[8], nfacet = 0;
+    for (int lt = 0; lt < 8; ++lt) {
+        int s[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) s[q] = sgn(ls[c_tab.lat_t[lt][q]]);
+        if (!s[0] && !s[1] && !s[2] && !s[3]) {
+            if (COUNT) cnt[c] = 0;   // no points: the local pass raises the error for this tet
+            return;
+        }
+        pids[lt] = s[0] + 3 * s[1] + 9 * s[2] + 27 * s[3] + 40;
+        nfacet += c_tab.rp_size[pids[lt]];
+    }
+    if (COUNT) {
+        cnt[c] = 7 * nfacet;
+        return;
+    }
+    int64_t k = off[c];
+    for (int lt = 0; lt < 8; ++lt) {
+        const int pid = pids[lt];
+        const int ntri = c_tab.rp_size[pid];
+        for (int tr = 0; tr < ntri; ++tr) {
+            double vb[3][4];
+            facet_vertices(ls, lt, pid, tr, vb);
+#pragma unroll
+            for (int qn = 0; qn < 7; ++qn) {
+                double nb[4];
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) {
+                    double sv = vb[0][cc] * c_tab.q5n[qn][0];
+                    sv += vb[1][cc] * c_tab.q5n[qn][1];
+                    sv += vb[2][cc] * c_tab.q5n[qn][2];
+                    nb[cc] = sv;
+                }
+                double x[3];
+                b2w(X, nb, x);
+                xyz[3 * k + 0] = x[0];
+                xyz[3 * k + 1] = x[1];
+                xyz[3 * k + 2] = x[2];
+                ++k;
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------

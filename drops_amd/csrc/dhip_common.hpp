@@ -117,6 +117,14 @@ struct dhip_ctx {
     int64_t phi_count = 0;
     dhip::DevBuf phi_own;
 
+    uint64_t geo_gen = 1;   // bumped by every new mesh or level set
+    // right-hand side values at the patch quadrature points (DHIP_RHS_VALUES)
+    dhip::DevBuf qp_off, qp_vals;
+    int64_t qp_n = -1, qp_ncut = -1;
+    uint64_t qp_gen = 0;
+    double qp_band = -2.0;
+    bool qp_have_vals = false;
+
     // assembly state: 0 none, 1 classified, 2 local, 3 numbered, 4 built
     int stage = 0;
     int space = 1;   // 1: P1 interface space, 2: P2 interface space (set by the local step)

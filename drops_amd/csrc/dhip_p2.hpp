@@ -73,6 +73,8 @@ struct LocalP2Args {
     double *loc;      // SoA [LOC2_N][ncut]
     uint8_t *flags;   // bit0: non-empty patch
     int32_t *err;
+    const double *rhs_vals = nullptr;   // DHIP_RHS_VALUES: f at every patch quadrature point
+    const int32_t *rhs_off = nullptr;   //   first point of each cut tet
 };
 
 struct P2Shared {
@@ -126,9 +128,11 @@ __device__ inline int p2_patch_facets(P2Shared &sh, const double X[4][3], const 
 }
 
 // Stage composite-Quad5 point base + lane (< nq): weight, P2 basis values, surface gradients
-// projected with the P2 level-set normal and (rhs_kind != NONE) the load value.
+// projected with the P2 level-set normal and (rhs_kind != NONE) the load value -- read from fvals
+// (the tet's given values, DHIP_RHS_VALUES) or evaluated.
 __device__ inline void p2_stage_points(P2Shared &sh, int base, int nq, int lane, const double X[4][3],
-                                       const double G1[4][3], const double v[10], int rhs_kind, double rhs_p0) {
+                                       const double G1[4][3], const double v[10], int rhs_kind, double rhs_p0,
+                                       const double *fvals = nullptr) {
     const int q = base + lane;
     if (q >= nq) return;
     const int f = q / 7, i = q - 7 * (q / 7);
@@ -155,7 +159,9 @@ __device__ inline void p2_stage_points(P2Shared &sh, int base, int nq, int lane,
         sh.h[lane][d] = h[d];
     }
     double fv = 0.;
-    if (rhs_kind != DHIP_RHS_NONE) {
+    if (rhs_kind == DHIP_RHS_VALUES) {
+        fv = fvals[q];
+    } else if (rhs_kind != DHIP_RHS_NONE) {
         double x[3];
         b2w(X, nb, x);
         fv = rhs_fun(rhs_kind, rhs_p0, x);
@@ -196,7 +202,8 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
     double am = 0., al = 0., hat = 0., rh = 0.;
     const int nq = 7 * nf;
     for (int base = 0; base < nq; base += P2_CHUNK) {
-        p2_stage_points(sh, base, nq, lane, X, G1, v, a.rhs_kind, a.rhs_p0);
+        p2_stage_points(sh, base, nq, lane, X, G1, v, a.rhs_kind, a.rhs_p0,
+                        a.rhs_kind == DHIP_RHS_VALUES ? a.rhs_vals + a.rhs_off[c] : nullptr);
         __syncthreads();
         const int cnt = nq - base < P2_CHUNK ? nq - base : P2_CHUNK;
         if (lane < 55) {

@@ -17,7 +17,7 @@ MTX_MARKET, MTX_PLAIN = 0, 1
 HOST, DEVICE = 0, 1
 ZERO_THROW, ZERO_ALLOW = 0, 1
 OP_MASS, OP_LB, OP_STAB, OP_COMB = range(4)
-RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST = range(3)
+RHS_NONE, RHS_3XY_MINUS_X3, RHS_CONST, RHS_VALUES = range(4)
 PC_NONE, PC_JACOBI = 0, 1
 FP64, FP32_IR, FP64_CLASSIC = 0, 1, 2
 LOC_N = 57
@@ -41,6 +41,7 @@ EXPORTS = [
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
     "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision", "dhip_spmv_t",
+    "dhip_quad_points", "dhip_rhs_values",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -136,6 +137,8 @@ def load(path=LIB_PATH):
     L.dhip_csr_info.argtypes = [P, I64, I64]
     L.dhip_spmv.argtypes = [P, P, VP, VP, C.c_int32]
     L.dhip_spmv_t.argtypes = [P, P, C.c_int64, VP, VP, C.c_int32]
+    L.dhip_quad_points.argtypes = [P, I64, VP, C.c_int32]
+    L.dhip_rhs_values.argtypes = [P, VP, C.c_int64, C.c_int32]
     L.dhip_dot.argtypes = [P, C.c_int64, VP, VP, D, C.c_int32]
     L.dhip_axpy.argtypes = [P, C.c_int64, C.c_double, VP, VP, C.c_int32]
     L.dhip_pcg.argtypes = [P, P, C.c_int32, VP, VP, C.c_int32, I32, D, C.c_int32, C.c_int32]
@@ -829,6 +832,20 @@ class Context:
         y = np.zeros(A.shape[0])
         self.check(self.L.dhip_spmv(self.h, A.h, _vp(x), _vp(y), HOST))
         return y
+
+    def quad_points(self):
+        """(npoints, 3) world coordinates of the patches' composite Quad5 points (dhip_quad_points)."""
+        n = C.c_int64()
+        self.check(self.L.dhip_quad_points(self.h, C.byref(n), None, HOST))
+        xyz = np.zeros((max(n.value, 1), 3))
+        if n.value:
+            self.check(self.L.dhip_quad_points(self.h, C.byref(n), _vp(xyz), HOST))
+        return xyz[:n.value]
+
+    def rhs_values(self, values):
+        """f at the points of quad_points() for the next DHIP_RHS_VALUES assembly (dhip_rhs_values)."""
+        v = np.ascontiguousarray(values, np.float64)
+        self.check(self.L.dhip_rhs_values(self.h, _vp(v) if v.size else None, v.size, HOST))
 
     def spmv_t(self, A, x, ncols=None):
         """A^T x (dhip_spmv_t); ncols defaults to the row count (square A)."""

@@ -985,6 +985,8 @@ struct DevicePlan {
     VecDescCL *rhs = nullptr;
     int rhs_kind = DHIP_RHS_NONE;
     double rhs_param = 0.;
+    std::function<double(const double *, double)> rhs_fun;   // DHIP_RHS_VALUES
+    double rhs_time = 0.;
     void add_matrix(MatDescCL *m, int op, double coef, const char *space) {
         for (auto &e : mats)
             if (e.op == op) throw DROPSErrCL(std::string("accumulate: one accumulator per ") + space + " interface operator");
@@ -1185,18 +1187,26 @@ class NarrowBandMatrixAccuP1CL : public DeviceAccumulatorCL<NarrowBandMatrixAccu
     }
 };
 
-// load vector of a device-evaluable right-hand side (surfactant/ifacetransp.h:768-798)
+// load vector (surfactant/ifacetransp.h:768-798): f(x, t) evaluated on the host at the patches'
+// quadrature points, which the library lists (dhip_quad_points), or a device-evaluable kind
+typedef std::function<double(const double *, double)> InstatScalarFunction;
 struct DeviceFunctionCL {
     int kind = DHIP_RHS_NONE;
     double param = 0.;
 };
 struct LocalVectorP1CL {
     DeviceFunctionCL f;
+    InstatScalarFunction fun;
+    double time = 0.;
     LocalVectorP1CL(DeviceFunctionCL fn, double = 0.) : f(fn) {}
+    LocalVectorP1CL(InstatScalarFunction fn, double t) : fun(std::move(fn)), time(t) { f.kind = DHIP_RHS_VALUES; }
 };
 struct LocalVectorP2CL {
     DeviceFunctionCL f;
+    InstatScalarFunction fun;
+    double time = 0.;
     LocalVectorP2CL(DeviceFunctionCL fn, double = 0.) : f(fn) {}
+    LocalVectorP2CL(InstatScalarFunction fn, double t) : fun(std::move(fn)), time(t) { f.kind = DHIP_RHS_VALUES; }
 };
 
 template <class LocalMatrixT, class CommonT>
@@ -1233,6 +1243,8 @@ class InterfaceVectorAccuCL : public DeviceAccumulatorCL<InterfaceVectorAccuCL<L
         p.rhs = vec;
         p.rhs_kind = local.f.kind;
         p.rhs_param = local.f.param;
+        p.rhs_fun = local.fun;
+        p.rhs_time = local.time;
         return true;
     }
 };
@@ -1332,6 +1344,19 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
         const double param = plan.rhs_param;
         const int rk = plan.rhs ? plan.rhs_kind : DHIP_RHS_NONE;
         const double lc[3] = {1., 1., 1.};
+        if (rk == DHIP_RHS_VALUES) {
+            // f at the quadrature points of the patches of the cut tets (the points LocalVectorP1CL /
+            // LocalVectorP2CL evaluate it at), evaluated here, summed by the device pass
+            if (plan.band) throw DROPSErrCL("accumulate: a load function on the narrow band is not supported");
+            if (!plan.rhs_fun) throw DROPSErrCL("accumulate: no load function");
+            int64_t nc = 0, np = 0;
+            dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &nc));
+            dev.check(dhip_quad_points(dev.ctx(), &np, nullptr, DHIP_HOST));
+            std::vector<double> xyz((size_t)(3 * np) + 3), fv((size_t)np + 1);
+            if (np) dev.check(dhip_quad_points(dev.ctx(), &np, xyz.data(), DHIP_HOST));
+            for (int64_t q = 0; q < np; ++q) fv[(size_t)q] = plan.rhs_fun(&xyz[(size_t)(3 * q)], plan.rhs_time);
+            dev.check(dhip_rhs_values(dev.ctx(), fv.data(), np, DHIP_HOST));
+        }
         if (p2)
             dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, rk, &param, -1.0, lc, &ncut, &nd, &nnz));
         else if (plan.band)
@@ -1399,8 +1424,6 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
 // ---------------------------------------------------------------------------
 // Cahn-Hilliard stepper (CahnHilliardcGP1CL, surfactant/ifacetransp.h:2947-2990)
 // ---------------------------------------------------------------------------
-typedef std::function<double(const double *, double)> InstatScalarFunction;
-
 // The statistics of the last block solve (the reference's block_gm_.GetIter / GetResid).
 class CHSolveStatsCL {
     friend class CahnHilliardcGP1CL;

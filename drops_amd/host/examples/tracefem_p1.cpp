@@ -17,6 +17,8 @@
 //                                              CH blocks from the accumulators, transp_mul and transpose
 //   tracefem_p1 distribute <n> <dist> <outdir> -> IdxDescCL::DistributeDOFs on the torus: marked tets and the
 //                                              grid vertex of every DOF
+//   tracefem_p1 userrhs <n> <p1|p2>         -> the load vector of a host function f(x, t) (LocalVectorP1CL /
+//                                              LocalVectorP2CL with a std::function) against the device kind
 #include <cstdio>
 #include <fstream>
 #include <cstdlib>
@@ -366,6 +368,56 @@ static int distribute(int n, double dist, const std::string &out) {
     return 0;
 }
 
+// The configuration-1 sphere's load vector of f = 3xy - x^3 given as a host function and as the
+// device-evaluated kind; prints the largest difference (0 when the points and the sums agree).
+template <class CommonT, class LocalT>
+static double load_vector(MultiGridCL &mg, LevelsetP2CL &lset, IdxDescCL &idx, const LocalT &local) {
+    BndDataCL bnd;
+    VecDescCL b(&idx);
+    TetraAccumulatorTupleCL accus;
+    CommonT cdata(lset.Phi, bnd);
+    accus.push_back(&cdata);
+    InterfaceVectorAccuCL<LocalT, CommonT> rhs(&b, local, cdata, "load");
+    accus.push_back(&rhs);
+    accumulate(accus, mg, 0, &bnd);
+    double s = 0.;
+    for (size_t i = 0; i < b.Data.size(); ++i) s += std::fabs(b.Data[i]);
+    static VectorCL last;
+    double d = 0.;
+    if (last.size() == b.Data.size())
+        for (size_t i = 0; i < b.Data.size(); ++i) d = std::max(d, std::fabs(last[i] - b.Data[i]));
+    last.resize(b.Data.size());
+    last = b.Data;
+    std::printf("sum |b| = %.17g\n", s);
+    return d;
+}
+
+static int userrhs(int n, const std::string &space) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL bnd;
+    const bool p2 = space == "p2";
+    IdxDescCL idx(p2 ? P2IF_FE : P1IF_FE);
+    idx.CreateNumbering(0, mg, &lset.Phi, &bnd);
+    DeviceFunctionCL kind;
+    kind.kind = DHIP_RHS_3XY_MINUS_X3;
+    InstatScalarFunction f = [](const double *x, double) { return 3.0 * x[0] * x[1] - x[0] * x[0] * x[0]; };
+    double d;
+    if (p2) {
+        load_vector<InterfaceCommonDataP2CL>(mg, lset, idx, LocalVectorP2CL(kind));
+        d = load_vector<InterfaceCommonDataP2CL>(mg, lset, idx, LocalVectorP2CL(f, 0.));
+    } else {
+        load_vector<InterfaceCommonDataP1CL>(mg, lset, idx, LocalVectorP1CL(kind));
+        d = load_vector<InterfaceCommonDataP1CL>(mg, lset, idx, LocalVectorP1CL(f, 0.));
+    }
+    std::printf("user rhs vs device rhs: %.3e\n", d);
+    return 0;
+}
+
 // A user accumulator as a DROPS application writes one (num/accumulator.h:42-66): it counts the
 // visited tets, the tets with an interface patch and sums the patch areas through the tuple's
 // interface common data, and records the visit order and the begin/finalize calls.
@@ -517,6 +569,7 @@ int main(int argc, char **argv) {
         if (argc >= 3 && !std::strcmp(argv[1], "colors")) return colors(std::atoi(argv[2]));
         if (argc >= 5 && !std::strcmp(argv[1], "ch")) return ch(std::atoi(argv[2]), std::atoi(argv[3]), argv[4]);
         if (argc >= 4 && !std::strcmp(argv[1], "block")) return block(std::atoi(argv[2]), argv[3]);
+        if (argc >= 4 && !std::strcmp(argv[1], "userrhs")) return userrhs(std::atoi(argv[2]), argv[3]);
         if (argc >= 5 && !std::strcmp(argv[1], "distribute"))
             return distribute(std::atoi(argv[2]), std::atof(argv[3]), argv[4]);
         if (argc >= 4 && !std::strcmp(argv[1], "band"))

@@ -67,6 +67,7 @@ extern "C" {
 #define DHIP_RHS_NONE 0
 #define DHIP_RHS_3XY_MINUS_X3 1  /* f(x) = 3 x y - x^3 */
 #define DHIP_RHS_CONST 2         /* f(x) = param[0] */
+#define DHIP_RHS_VALUES 3        /* f given at the patches' quadrature points (dhip_quad_points, dhip_rhs_values) */
 
 /* preconditioners */
 #define DHIP_PC_NONE 0           /* DummyPcCL, num/precond.h:1032-1065 */
@@ -140,6 +141,16 @@ int dhip_classify(dhip_ctx_t *ctx, int32_t zero_policy, int64_t *ncut);
  * (coef[1] = D), normal-gradient stabilisation (coef[2] = D), load vector,
  * the interface-numbering integrals of phi_i^2 and the patch area. */
 int dhip_local_p1(dhip_ctx_t *ctx, const double coef[3], int32_t rhs_kind, const double *rhs_param);
+/* Right-hand sides from any function (LocalVectorP1CL / LocalVectorP2CL with an InstatScalarFunction f,
+ * surfactant/ifacetransp.h:768-798, which evaluates f at the vertices of the composite Quad5 domain of
+ * each tet's patch): dhip_quad_points lists those points in world coordinates -- cut tets in cut-list
+ * order, per tet the patch facets in make_patch order, 7 Quad5_2D nodes per facet -- after dhip_classify
+ * (or an assembly) on the current mesh and level set; xyz == NULL gives the count only, else xyz holds
+ * 3 * npoints doubles (host or device by where).  The caller evaluates f there and hands the values to
+ * dhip_rhs_values; the next local pass with rhs_kind DHIP_RHS_VALUES on the same cut list (P1, P2 or
+ * narrow band) sums them exactly as it sums a built-in kind. */
+int dhip_quad_points(dhip_ctx_t *ctx, int64_t *npoints, double *xyz, int32_t where);
+int dhip_rhs_values(dhip_ctx_t *ctx, const double *values, int64_t npoints, int32_t where);
 /* First-encounter P1 interface numbering, IdxDescCL::CreateNumbering ->
  * CreateNumbOnInterfaceVertex (misc/problem.cpp:443-503, 725-745).
  * omit_bound < 0 numbers every vertex with a positive phi_i^2 integral. */
