@@ -394,7 +394,7 @@ def band_leg(ctx, mesh, phi, coef, lc, width, reps=5, with_cpu=True):
 
 
 # the FP64 PCG kernel each exchange runs (rocprofv3 names it so in its summaries)
-PCG_KERNEL = {0: "k_pcg64_sr", 1: "k_pcg64_sr", 2: "k_pcg64_tx"}
+PCG_KERNEL = {0: "k_pcg64_sr", 1: "k_pcg64_sr", 2: "k_pcg64_tx", 3: "k_pcg64_pipe"}
 
 
 def measured_traffic(kernel, rows, ndof):
@@ -412,13 +412,14 @@ def measured_traffic(kernel, rows, ndof):
 
 def probe_sweep(ctx, ndof, x_dev, zero_dev):
     """Per-iteration compute / synchronisation split of the FP64 PCG for a sweep of rows per block,
-    with the three exchanges between blocks (0 fenced, 1 coherent atomics, 2 tagged records with the
-    halo in LDS); at one geometry all three must give the same iterates bit for bit (as long as the
-    single-reduction kernel's slice fits its LDS, so that both scale the values alike)."""
+    with the exchanges between blocks (0 fenced, 1 coherent atomics, 2 tagged records with the halo in
+    LDS, 3 the pipelined arrangement over tagged records); at one geometry 0-2 must give the same
+    iterates bit for bit (as long as the single-reduction kernel's slice fits its LDS, so that both
+    scale the values alike); 3 runs other recurrences and is compared by its distance."""
     A = ctx.csr_from_assembly(D.OP_COMB)
     rhs = ctx.device_arrays()[3]
     ref = {}
-    for coh in (0, 1, 2):
+    for coh in (0, 1, 2, 3):
         ctx.tune("coherent_exchange", coh)
         for rows in (128, 256, 384, 512, 1024):
             ctx.tune("pcg_rows_per_block", rows)
@@ -438,7 +439,9 @@ def probe_sweep(ctx, ndof, x_dev, zero_dev):
                                   compute_us_per_iter=float((pr[:, 1] / it).mean()),
                                   sync_us_per_iter=float((pr[:, 2] / it).mean()),
                                   setup_us=float(pr[:, 0].mean()),
-                                  identical_to_fenced=bool(np.array_equal(x, ref[rows])))), flush=True)
+                                  identical_to_fenced=bool(np.array_equal(x, ref[rows])),
+                                  rel_diff_to_fenced=float(np.linalg.norm(x - ref[rows]) /
+                                                           np.linalg.norm(ref[rows])))), flush=True)
     ctx.tune("coherent_exchange", 2)
     ctx.tune("pcg_rows_per_block", 0)
     A.close()
@@ -496,7 +499,7 @@ def main():
     ap.add_argument("--cells", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-copies", type=int, default=80)
-    ap.add_argument("--exchange", type=int, default=2, choices=(0, 1, 2),
+    ap.add_argument("--exchange", type=int, default=2, choices=(0, 1, 2, 3),
                     help="with --rows-per-block: the persistent solvers' exchange (0 fenced, 1 coherent atomics, "
                          "2 tagged records with the halo in LDS)")
     ap.add_argument("--rows-per-block", type=int, default=0,
@@ -665,7 +668,8 @@ def main():
                                f"{n}^3 Kuhn cells, assembly + Jacobi-PCG to rel. residual 1e-8",
                    "cells": n, "cut_tets": ncut, "dofs": ndof, "nnz": nnz,
                    "pcg_iters_fp64": results["fp64"]["iters"], "pcg_rows_per_block": rows_best,
-                   "pcg_exchange": {0: "fenced", 1: "coherent", 2: "tagged_lds_halo"}[exch_best]},
+                   "pcg_exchange": {0: "fenced", 1: "coherent", 2: "tagged_lds_halo",
+                                    3: "pipelined_tagged_lds_halo"}[exch_best]},
         "variants": {k: {"s_per_solve": r["s_per_solve"], "iters": r["iters"]} for k, r in results.items()},
         "err_vs_fp64_oracle": err or None,
         "spmv_hbm_gbs": spmv["gbs"] if spmv else None,

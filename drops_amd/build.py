@@ -12,12 +12,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib", "libdhip.so")
 OBJ_DIR = os.path.join(HERE, "lib", "obj")
-# translation units: the library proper, and the persistent solver kernels in fourteen shares
+# translation units: the library proper, and the persistent solver kernels in fifteen shares
 # (the longest compilations first, so that eight parallel jobs finish together)
 UNITS = [(f"dhip_solvers_{k}", os.path.join(CSRC, "dhip_solvers.hip"), [f"-DDHIP_SOLVER_PART={k}"])
          for k in (7, 10, 13)] + [("dhip", os.path.join(CSRC, "dhip.hip"), [])] + [
     (f"dhip_solvers_{k}", os.path.join(CSRC, "dhip_solvers.hip"), [f"-DDHIP_SOLVER_PART={k}"])
-    for k in (12, 0, 1, 2, 8, 9, 11, 6, 5, 4, 3)]
+    for k in (12, 0, 1, 2, 8, 14, 9, 11, 6, 5, 4, 3)]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

@@ -403,7 +403,9 @@ int dhip_create(dhip_ctx_t **out, int32_t device) {
     // the tagged PCG kernels' dynamic LDS (their attribute raised here, once, so that the
     // allocation-free entry point launches them as they are)
     {
-        const void *kt[3] = {(const void *)k_pcg64_tx<1>, (const void *)k_pcg64_tx<2>, (const void *)k_pcg64_tx<4>};
+        const void *kt[6] = {(const void *)k_pcg64_tx<1>, (const void *)k_pcg64_tx<2>, (const void *)k_pcg64_tx<4>,
+                             (const void *)k_pcg64_pipe<1>, (const void *)k_pcg64_pipe<2>,
+                             (const void *)k_pcg64_pipe<4>};
         size_t lim = persistent_lds(c, kt[2]);
         for (const void *k : kt) lim = std::min(lim, persistent_lds(c, k));
         c->tx_lds = lim;
@@ -1428,6 +1430,20 @@ static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const dou
     double *outd = &w.res->resid;
     const double *diag = pc == DHIP_PC_JACOBI ? w.diag : nullptr;
     tagged = tagged && w.tx && rpt <= 4;
+    if (tagged && n > 0) {
+        // Every solve numbers its generations from 1, so records left in the buffer by an earlier solve
+        // carry tags this solve will wait for: a reader that arrives before the writer would take the
+        // old value.  Clearing them (tag 0, never used) leaves only this solve's records acceptable.
+        const int64_t words = 24 * n;   // three vectors of [2][n] 16-byte records
+        if (alloc_ok) {
+            DHIP_LAUNCH(c, "zero_records", k_zero_u32, blocks_for(words, 256), 256, 0,
+                        reinterpret_cast<uint32_t *>(w.tx), (int)words);
+        } else {
+            hipLaunchKernelGGL(k_zero_u32, dim3(blocks_for(words, 256)), dim3(256), 0, c->stream,
+                               reinterpret_cast<uint32_t *>(w.tx), (int)words);
+            DHIP_HIPCHK(c, hipGetLastError());
+        }
+    }
     const int kr = rpt;   // selects the instantiation below
     PcgSrArgs pa{n, A->rowptr, A->col, A->val, diag, db, dx, {w.vec[0], w.vec[1]}, {w.vec[2], w.vec[3]},
                  {w.vec[4], w.vec[5]}, w.slots, max_iter, tol, rel, outs, outd, outs + 1, nullptr};
@@ -1443,7 +1459,11 @@ static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const dou
                            (const void *)k_pcg64_sr<16, true>};
     const void *ks_t[5] = {(const void *)k_pcg64_tx<1>, (const void *)k_pcg64_tx<2>, (const void *)k_pcg64_tx<4>,
                            nullptr, nullptr};
-    const void *const *ks = tagged ? ks_t : c->coherent_exchange ? ks_c : ks_f;
+    const void *ks_p[5] = {(const void *)k_pcg64_pipe<1>, (const void *)k_pcg64_pipe<2>,
+                           (const void *)k_pcg64_pipe<4>, nullptr, nullptr};
+    // exchange 3: the pipelined arrangement over the same tagged records
+    const bool pipe = tagged && c->coherent_exchange == 3;
+    const void *const *ks = pipe ? ks_p : tagged ? ks_t : c->coherent_exchange ? ks_c : ks_f;
     const int idx = kr == 1 ? 0 : kr == 2 ? 1 : kr == 4 ? 2 : kr == 8 ? 3 : 4;
     // the tagged kernel keeps its slice and halo in up to 128 KiB of LDS
     const size_t lds = tagged ? c->tx_lds : (size_t)LDS_BYTES;
@@ -1480,7 +1500,7 @@ static int pcg_fp64_launch(dhip_ctx *c, const dhip_csr *A, int32_t pc, const dou
 static int pcg_fp64_run(dhip_ctx *c, const dhip_csr *A, int32_t pc, const double *db, double *dx,
                         const PcgFp64Bufs &w, int32_t max_iter, double tol, int32_t rel, bool alloc_ok,
                         unsigned max_blocks, SolverResult *out) {
-    const bool tagged = c->coherent_exchange == 2 && w.tx;
+    const bool tagged = c->coherent_exchange >= 2 && w.tx;
     int rc = pcg_fp64_launch(c, A, pc, db, dx, w, max_iter, tol, rel, alloc_ok, max_blocks, tagged);
     if (rc) return rc;
     DHIP_HIPCHK(c, hipMemcpyAsync(out, w.res, sizeof *out, hipMemcpyDeviceToHost, c->stream));
@@ -1604,7 +1624,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
         for (int k = 0; k < 6; ++k) w.vec[k] = c->kv[k].as<double>();
         w.slots = c->bar.as<SyncSlot>();
         w.res = reinterpret_cast<SolverResult *>(outs);
-        if (c->coherent_exchange == 2) {
+        if (c->coherent_exchange >= 2) {
             DHIP_HIPCHK(c, c->kv[16].ensure((size_t)96 * (n + 1)));
             w.tx = c->kv[16].as<uint4>();
         }
@@ -1669,7 +1689,7 @@ int dhip_pcg(dhip_ctx_t *c, const dhip_csr_t *A, int32_t pc, const double *b, do
 int dhip_tune(dhip_ctx_t *c, const char *key, int64_t value) {
     if (!c || !key) return DHIP_E_ARG;
     if (!std::strcmp(key, "coherent_exchange")) {
-        if (value < 0 || value > 2) return set_error(c, DHIP_E_ARG, "coherent_exchange must be 0, 1 or 2");
+        if (value < 0 || value > 3) return set_error(c, DHIP_E_ARG, "coherent_exchange must be 0, 1, 2 or 3");
         c->coherent_exchange = (int)value;
         return DHIP_OK;
     }

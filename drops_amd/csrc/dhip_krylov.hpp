@@ -938,6 +938,118 @@ __device__ inline bool slot_collect3(SyncSlot *slots, uint32_t g, double out[3])
     return __ballot(ok) == __ballot(1);
 }
 
+__device__ inline int block_min_int(int v) {
+    __shared__ int ws[PB / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int t = __shfl_xor(v, o, 64);
+        v = t < v ? t : v;
+    }
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int m = ws[0];
+#pragma unroll
+    for (int i = 1; i < PB / 64; ++i) m = ws[i] < m ? ws[i] : m;
+    __syncthreads();
+    return m;
+}
+
+// A block's CSR slice with block-local column indices in LDS: the distinct columns of rows
+// [r0, r0 + nr) in an LDS bitmap over [cmn, cmn + range), ranked by a block scan into the list U
+// (nu entries, ascending), the slice's columns rewritten as indices into U and its values divided by
+// the diagonal of their column (dg != nullptr), and nvec halo arrays X[nvec][nu].  Layout:
+// lrp[nr + 1] | lcol[nzb] | bits[nwords] | pre[nwords] | U[nu] | (16-aligned) lval[nzb] | X[nvec][nu].
+// fits is the same in every thread; false: the LDS holds neither (the arrays are then unusable).
+struct TxLocal {
+    int32_t *lrp, *lcol, *pre, *U;
+    uint32_t *bits;
+    double *lval, *X;
+    int cmn, nu;
+    int64_t range;
+    bool fits;
+};
+
+__device__ inline TxLocal tx_local_setup(unsigned char *lds, int64_t lds_bytes, const int32_t *rp, const int32_t *col,
+                                         const double *val, const double *dg, int64_t r0, int64_t nr, int32_t nz0,
+                                         int64_t nzb, int nvec) {
+    TxLocal T{};
+    int cmn = 0x7fffffff, cmx = -1;
+    for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+        const int c = col[nz0 + k];
+        cmn = c < cmn ? c : cmn;
+        cmx = c > cmx ? c : cmx;
+    }
+    cmn = block_min_int(cmn);
+    cmx = -block_min_int(-cmx);
+    const int64_t range = nzb ? (int64_t)cmx - cmn + 1 : 0;
+    const int64_t nwords = (range + 31) / 32;
+    int32_t *lrp = reinterpret_cast<int32_t *>(lds);
+    int32_t *lcol = lrp + nr + 1;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lcol + nzb);
+    int32_t *pre = reinterpret_cast<int32_t *>(bits + nwords);
+    int32_t *U = pre + nwords;
+    bool fits = 4 * (size_t)(nr + 1 + nzb + 2 * nwords) <= (size_t)lds_bytes;
+    int nu = 0;
+    double *lval = nullptr, *X = nullptr;
+    if (fits && nzb > 0) {
+        for (int64_t w = threadIdx.x; w < nwords; w += PB) bits[w] = 0u;
+        __syncthreads();
+        for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+            const int d = col[nz0 + k] - cmn;
+            atomicOr(&bits[d >> 5], 1u << (d & 31));
+        }
+        __syncthreads();
+        // rank of every set bit: each thread takes a run of words
+        const int64_t wpt = (nwords + PB - 1) / PB;
+        const int64_t w0 = threadIdx.x * wpt, w1 = w0 + wpt < nwords ? w0 + wpt : nwords;
+        int cnt = 0;
+        for (int64_t w = w0; w < w1; ++w) cnt += __popc(bits[w]);
+        int tot = 0;
+        int off = block_excl_scan<PB>(cnt, &tot);
+        nu = tot;
+        const size_t int_bytes = 4 * (size_t)(nr + 1 + nzb + 2 * nwords + nu);
+        const size_t lval_off = (int_bytes + 15) & ~(size_t)15;
+        // the same in every thread (the scan total)
+        fits = lval_off + 8 * (size_t)(nzb + (int64_t)nvec * nu) <= (size_t)lds_bytes;
+        if (fits) {
+            lval = reinterpret_cast<double *>(lds + lval_off);
+            X = lval + nzb;
+            for (int64_t w = w0; w < w1; ++w) {
+                pre[w] = off;
+                uint32_t m = bits[w];
+                while (m) {
+                    const int bpos = __ffs(m) - 1;
+                    U[off++] = cmn + (int)(32 * w) + bpos;
+                    m &= m - 1;
+                }
+            }
+            __syncthreads();
+            for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = rp[r0 + i] - nz0;
+            for (int64_t k = threadIdx.x; k < nzb; k += PB) {
+                const int c = col[nz0 + k], d = c - cmn;
+                lcol[k] = pre[d >> 5] + __popc(bits[d >> 5] & ((1u << (d & 31)) - 1u));
+                lval[k] = dg ? val[nz0 + k] / dg[c] : val[nz0 + k];
+            }
+            __syncthreads();
+        }
+    } else if (fits) {
+        for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = 0;   // rows without entries
+        __syncthreads();
+    }
+    T.lrp = lrp;
+    T.lcol = lcol;
+    T.pre = pre;
+    T.U = U;
+    T.bits = bits;
+    T.lval = lval;
+    T.X = X;
+    T.cmn = cmn;
+    T.nu = nu;
+    T.range = range;
+    T.fits = fits;
+    return T;
+}
+
 // Persistent FP64 PCG, the single-reduction arrangement of k_pcg64_sr with the tagged exchange and
 // the block's halo in LDS.  At set-up each block lists the distinct columns of its rows (an LDS
 // bitmap over their range, ranked by a block scan) and rewrites its CSR slice to indices into that
@@ -967,21 +1079,6 @@ struct PcgTxArgs {
     int64_t lds_bytes = 0;      // dynamic LDS of the launch
 };
 
-__device__ inline int block_min_int(int v) {
-    __shared__ int ws[PB / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const int t = __shfl_xor(v, o, 64);
-        v = t < v ? t : v;
-    }
-    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
-    __syncthreads();
-    int m = ws[0];
-#pragma unroll
-    for (int i = 1; i < PB / 64; ++i) m = ws[i] < m ? ws[i] : m;
-    __syncthreads();
-    return m;
-}
 
 template <int RPT>
 __global__ void __launch_bounds__(PB) k_pcg64_tx(PcgTxArgs a)
@@ -1003,72 +1100,14 @@ __global__ void __launch_bounds__(PB) k_pcg64_tx(PcgTxArgs a)
     const double *dg = a.diag;
     const TxVec TR = tx_vec(a.R, a.n), TW = tx_vec(a.W, a.n), TS = tx_vec(a.S, a.n);
     // ---- set-up A: the distinct columns of the block and the slice in local column indices ----
-    int cmn = 0x7fffffff, cmx = -1;
-    for (int64_t k = threadIdx.x; k < nzb; k += PB) {
-        const int c = a.col[nz0 + k];
-        cmn = c < cmn ? c : cmn;
-        cmx = c > cmx ? c : cmx;
-    }
-    cmn = block_min_int(cmn);
-    cmx = -block_min_int(-cmx);
-    const int64_t range = nzb ? (int64_t)cmx - cmn + 1 : 0;
-    const int64_t nwords = (range + 31) / 32;
-    // LDS: lrp[nr + 1] | lcol[nzb] | bits[nwords] | pre[nwords] | U[nu] | (16-aligned) lval[nzb] | X[3][nu]
-    int32_t *lrp = reinterpret_cast<int32_t *>(lds);
-    int32_t *lcol = lrp + nr + 1;
-    uint32_t *bits = reinterpret_cast<uint32_t *>(lcol + nzb);
-    int32_t *pre = reinterpret_cast<int32_t *>(bits + nwords);
-    int32_t *U = pre + nwords;
-    bool fits = 4 * (size_t)(nr + 1 + nzb + 2 * nwords) <= (size_t)a.lds_bytes;
-    int nu = 0;
-    double *lval = nullptr, *XR = nullptr, *XW = nullptr, *XS = nullptr;
-    if (fits && nzb > 0) {
-        for (int64_t w = threadIdx.x; w < nwords; w += PB) bits[w] = 0u;
-        __syncthreads();
-        for (int64_t k = threadIdx.x; k < nzb; k += PB) {
-            const int d = a.col[nz0 + k] - cmn;
-            atomicOr(&bits[d >> 5], 1u << (d & 31));
-        }
-        __syncthreads();
-        // rank of every set bit: each thread takes a run of words
-        const int64_t wpt = (nwords + PB - 1) / PB;
-        const int64_t w0 = threadIdx.x * wpt, w1 = w0 + wpt < nwords ? w0 + wpt : nwords;
-        int cnt = 0;
-        for (int64_t w = w0; w < w1; ++w) cnt += __popc(bits[w]);
-        int tot = 0;
-        int off = block_excl_scan<PB>(cnt, &tot);
-        nu = tot;
-        const size_t int_bytes = 4 * (size_t)(nr + 1 + nzb + 2 * nwords + nu);
-        const size_t lval_off = (int_bytes + 15) & ~(size_t)15;
-        // the same in every thread (the scan total)
-        fits = lval_off + 8 * (size_t)(nzb + 3 * (int64_t)nu) <= (size_t)a.lds_bytes;
-        if (fits) {
-            lval = reinterpret_cast<double *>(lds + lval_off);
-            XR = lval + nzb;
-            XW = XR + nu;
-            XS = XW + nu;
-            for (int64_t w = w0; w < w1; ++w) {
-                pre[w] = off;
-                uint32_t m = bits[w];
-                while (m) {
-                    const int bpos = __ffs(m) - 1;
-                    U[off++] = cmn + (int)(32 * w) + bpos;
-                    m &= m - 1;
-                }
-            }
-            __syncthreads();
-            for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = a.rp[r0 + i] - nz0;
-            for (int64_t k = threadIdx.x; k < nzb; k += PB) {
-                const int c = a.col[nz0 + k], d = c - cmn;
-                lcol[k] = pre[d >> 5] + __popc(bits[d >> 5] & ((1u << (d & 31)) - 1u));
-                lval[k] = dg ? a.val[nz0 + k] / dg[c] : a.val[nz0 + k];
-            }
-            __syncthreads();
-        }
-    } else if (fits) {
-        for (int64_t i = threadIdx.x; i <= nr; i += PB) lrp[i] = 0;   // rows without entries
-        __syncthreads();
-    }
+    const TxLocal TL = tx_local_setup(lds, a.lds_bytes, a.rp, a.col, a.val, dg, r0, nr, nz0, nzb, 3);
+    int32_t *lrp = TL.lrp, *lcol = TL.lcol, *pre = TL.pre, *U = TL.U;
+    uint32_t *bits = TL.bits;
+    const int cmn = TL.cmn;
+    const int64_t range = TL.range;
+    const bool fits = TL.fits;
+    const int nu = TL.nu;
+    double *lval = TL.lval, *XR = TL.X, *XW = TL.X ? TL.X + nu : nullptr, *XS = TL.X ? TL.X + 2 * nu : nullptr;
     // local index of global column c (-1: not a column of the block)
     auto local_of = [&](int64_t c) -> int {
         const int64_t d = c - cmn;
@@ -1282,6 +1321,288 @@ done:
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *a.out_iter = it_done;
+        *a.out_resid = resid;
+        *a.out_status = status;
+    }
+}
+#endif
+
+// Persistent FP64 PCG in the pipelined arrangement (Ghysels and Vanroose, "Hiding global synchronization
+// latency in the preconditioned Conjugate Gradient algorithm", Parallel Computing 40 (2014), Alg. 4, with
+// Jacobi preconditioning M = diag(A)^-1): per iteration the product n = A M w needs only w, which every
+// block publishes as tagged records at the end of the previous iteration together with its partials of
+// gamma = (r, u), delta = (w, u) and |r|^2.  So each iteration loads the halo of w, runs the row
+// products from LDS, and only then collects the partials, whose wait overlaps the halo and the products;
+// one vector is exchanged instead of three.  The recurrences
+//   z = n + beta z, q = M w + beta q, s = w + beta s, p = u + beta p,
+//   x += alpha p, r -= alpha s, u -= alpha q, w -= alpha z
+// are CG's in exact arithmetic (same alphas and betas); their rounding drifts from b - A x, so a
+// residual that meets the tolerance is checked once as b - A x (x exchanged as records) and, if it
+// does not meet it, the recurrences restart from that residual.  The LDS slice, the halo and the
+// set-up follow k_pcg64_tx; a block whose slice and halo do not fit makes every block leave with
+// status 3.
+template <int RPT>
+__global__ void __launch_bounds__(PB) k_pcg64_pipe(PcgTxArgs a)
+#ifndef DHIP_SOLVER_KERNELS
+    ;
+#else
+{
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ double red_sh[SYNC_VALS];
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_wait = 0;
+    bool tbad = false;
+    const int64_t per = (a.n + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)row_range_index(a.xcd_map) * per;
+    const int64_t r1 = r0 + per < a.n ? r0 + per : a.n;
+    const int64_t nr = r1 > r0 ? r1 - r0 : 0;
+    const int32_t nz0 = nr ? a.rp[r0] : 0;
+    const int64_t nzb = nr ? (int64_t)a.rp[r1] - nz0 : 0;
+    const double *dg = a.diag;
+    const TxVec TR = tx_vec(a.R, a.n), TW = tx_vec(a.W, a.n), TX = tx_vec(a.S, a.n);
+    const TxLocal TL = tx_local_setup(lds, a.lds_bytes, a.rp, a.col, a.val, dg, r0, nr, nz0, nzb, 1);
+    const bool fits = TL.fits;
+    const int nu = TL.nu;
+    double *XH = TL.X;   // the halo of the vector being exchanged (own rows from registers)
+    auto local_of = [&](int64_t c) -> int {
+        const int64_t d = c - TL.cmn;
+        if (d < 0 || d >= TL.range) return -1;
+        const uint32_t w = TL.bits[d >> 5], m = 1u << (d & 31);
+        return (w & m) ? TL.pre[d >> 5] + __popc(w & (m - 1u)) : -1;
+    };
+    auto lds_row = [&](int64_t l) {
+        const int32_t b0 = TL.lrp[l], e0 = TL.lrp[l + 1];
+        const int32_t m0 = e0 - b0 < TX_HALF ? e0 : b0 + TX_HALF;
+        double s0 = 0.0, s1 = 0.0;
+        for (int32_t q = b0; q < m0; ++q) s0 += TL.lval[q] * XH[TL.lcol[q]];
+        for (int32_t q = m0; q < e0; ++q) s1 += TL.lval[q] * XH[TL.lcol[q]];
+        return s0 + s1;
+    };
+    // the halo of V at generation g into XH, columns the block does not own, reloaded until written
+    auto halo = [&](const TxVec &V, uint32_t g) {
+        for (int t0 = 0; t0 < nu; t0 += PB) {
+            const int t = t0 + threadIdx.x;
+            if (!(t < nu && !(TL.U[t] >= r0 && TL.U[t] < r1))) continue;
+            const int64_t c = TL.U[t];
+            u32x4 v = tx_load_raw(V, g, c);
+            uint32_t spins = 0;
+            while (!tx_ready(v, g)) {
+                if (++spins > SPIN_LIMIT) {
+                    tbad = true;
+                    break;
+                }
+                // the buffer-load builtin reads memory the compiler assumes unchanged: without this
+                // barrier it reuses the first load's result and the reload never happens
+                __asm__ __volatile__("" ::: "memory");
+                v = tx_load_raw(V, g, c);
+            }
+            XH[t] = tx_value(v);
+        }
+    };
+    auto collect = [&](uint32_t g) {
+        const uint64_t t0 = a.probe ? wall_clock64() : 0;
+        if (threadIdx.x < 64) {
+            double o[3];
+            const bool ok = slot_collect3(a.slots, g, o);
+            if (threadIdx.x == 0) {
+                red_sh[0] = o[0];
+                red_sh[1] = o[1];
+                red_sh[2] = o[2];
+                if (!ok) {
+                    tbad = true;
+                    a.slots[blockIdx.x].timeout = 1;
+                }
+            }
+        }
+        const bool any = __syncthreads_or(tbad);
+        if (a.probe) t_wait += wall_clock64() - t0;
+        return !any;
+    };
+    if (a.probe && threadIdx.x == 0) a.probe[blockIdx.x * 4 + 0] = 0;
+    int status = 1, it = 0, restarts = 0;
+    uint32_t gen = 0;
+    double resid = 0.0, normb = 1.0, alpha = 0.0, gamma = 0.0;
+    double xr[RPT], rr[RPT], ur[RPT], wr[RPT], zr[RPT], qr[RPT], sr[RPT], pr[RPT], dr[RPT], nrv[RPT];
+    int lo[RPT];
+    bool own[RPT];
+    int64_t lr[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        lr[k] = threadIdx.x + (int64_t)k * PB;
+        own[k] = lr[k] < nr;
+        lo[k] = (own[k] && fits) ? local_of(r0 + lr[k]) : -1;
+        xr[k] = rr[k] = ur[k] = wr[k] = zr[k] = qr[k] = sr[k] = pr[k] = nrv[k] = 0.0;
+        dr[k] = 1.0;
+    }
+    // ---- set-up: r0 = b - A x0 (records and slot of generation 1), |r|^2, |b|^2, whether every block fits
+    {
+        double p_rr = 0.0, p_bb = 0.0;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (!own[k]) continue;
+            const int64_t r = r0 + lr[k];
+            const double ax = row_product<double>(a.rp, a.col, a.val, 0, r, [&](int32_t c) { return a.x[c]; });
+            const double bv = a.b[r];
+            xr[k] = a.x[r];
+            rr[k] = bv - ax;
+            dr[k] = dg ? dg[r] : 1.0;
+            tx_store(TR, 1, r, rr[k]);
+            p_rr += rr[k] * rr[k];
+            p_bb += bv * bv;
+        }
+        double unused = 0.0;
+        block_sum3<PB>(p_rr, p_bb, unused);
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (lo[k] >= 0) XH[lo[k]] = rr[k];
+        gen = 1;
+        if (threadIdx.x == 0) slot_publish3(a.slots, gen, p_rr, p_bb, fits ? 0.0 : 1.0);
+    }
+    if (fits) halo(TR, gen);
+    if (!collect(gen)) { status = 2; goto done; }
+    if (red_sh[2] != 0.0) { status = 3; goto done; }
+    normb = sqrt(red_sh[1]);
+    if (normb == 0.0 || !a.rel) normb = 1.0;
+    resid = sqrt(red_sh[0]) / normb;
+    if (resid <= a.tol) {
+        status = 0;
+        goto done;
+    }
+    if (a.probe && threadIdx.x == 0) a.probe[blockIdx.x * 4 + 0] = wall_clock64() - t_start;
+    for (;;) {
+        // ---- (re)start from r (registers) with the halo of r in XH: u = M r, w = A M r ----
+        {
+            double p_g = 0.0, p_d = 0.0, p_r = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                ur[k] = dg ? 1.0 * rr[k] / dr[k] : rr[k];
+                wr[k] = lds_row(lr[k]);
+                zr[k] = qr[k] = sr[k] = pr[k] = 0.0;
+                p_g += rr[k] * ur[k];
+                p_d += wr[k] * ur[k];
+                p_r += rr[k] * rr[k];
+            }
+            ++gen;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (own[k]) tx_store(TW, gen, r0 + lr[k], wr[k]);
+            block_sum3<PB>(p_g, p_d, p_r);
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (lo[k] >= 0) XH[lo[k]] = wr[k];
+            if (threadIdx.x == 0) slot_publish3(a.slots, gen, p_g, p_d, p_r);
+        }
+        bool first = true, converged = false;
+        for (;; ++it) {
+            // n = A M w from the halo of w; then the partials of the same generation
+            halo(TW, gen);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (own[k]) nrv[k] = lds_row(lr[k]);
+            if (!collect(gen)) { status = 2; goto done; }
+            resid = sqrt(red_sh[2]) / normb;
+            if (resid <= a.tol) {
+                converged = true;
+                break;
+            }
+            if (it == a.max_iter) break;
+            const double gamma_new = red_sh[0];
+            double beta;
+            if (first) {
+                beta = 0.0;
+                alpha = gamma_new / red_sh[1];
+                first = false;
+            } else {
+                beta = gamma_new / gamma;
+                alpha = gamma_new / (red_sh[1] - beta * gamma_new / alpha);
+            }
+            gamma = gamma_new;
+            double p_g = 0.0, p_d = 0.0, p_r = 0.0;
+            ++gen;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const double m = dg ? 1.0 * wr[k] / dr[k] : wr[k];
+                zr[k] = nrv[k] + beta * zr[k];
+                qr[k] = m + beta * qr[k];
+                sr[k] = wr[k] + beta * sr[k];
+                pr[k] = ur[k] + beta * pr[k];
+                xr[k] += alpha * pr[k];
+                rr[k] = rr[k] - alpha * sr[k];
+                ur[k] = ur[k] - alpha * qr[k];
+                wr[k] = wr[k] - alpha * zr[k];
+                tx_store(TW, gen, r0 + lr[k], wr[k]);
+                p_g += rr[k] * ur[k];
+                p_d += wr[k] * ur[k];
+                p_r += rr[k] * rr[k];
+            }
+            block_sum3<PB>(p_g, p_d, p_r);
+#pragma unroll
+            for (int k = 0; k < RPT; ++k)
+                if (lo[k] >= 0) XH[lo[k]] = wr[k];
+            if (threadIdx.x == 0) slot_publish3(a.slots, gen, p_g, p_d, p_r);
+        }
+        if (!converged) break;   // max_iter: status 1
+        // ---- the true residual b - A x (x exchanged as records; its generation carries an empty slot) ----
+        ++gen;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (own[k]) tx_store(TX, gen, r0 + lr[k], xr[k]);
+            if (lo[k] >= 0) XH[lo[k]] = xr[k];
+        }
+        if (threadIdx.x == 0) slot_publish3(a.slots, gen, 0.0, 0.0, 0.0);
+        halo(TX, gen);
+        if (!collect(gen)) { status = 2; goto done; }
+        {
+            double p_r = 0.0;
+#pragma unroll
+            for (int k = 0; k < RPT; ++k) {
+                if (!own[k]) continue;
+                const int64_t r = r0 + lr[k];
+                const double ax = row_product<double>(a.rp, a.col, a.val, 0, r,
+                                                      [&](int32_t c) { return XH[local_of(c)]; });
+                rr[k] = a.b[r] - ax;
+                p_r += rr[k] * rr[k];
+            }
+            double u0 = 0.0, u1 = 0.0;
+            block_sum3<PB>(p_r, u0, u1);
+            ++gen;
+            if (threadIdx.x == 0) slot_publish3(a.slots, gen, p_r, 0.0, 0.0);
+            if (!collect(gen)) { status = 2; goto done; }
+            resid = sqrt(red_sh[0]) / normb;
+            if (resid <= a.tol) {
+                status = 0;
+                break;
+            }
+        }
+        if (it >= a.max_iter || ++restarts > 20) break;
+        // ---- restart from the true residual: its halo (generation with an empty slot) ----
+        ++gen;
+#pragma unroll
+        for (int k = 0; k < RPT; ++k) {
+            if (own[k]) tx_store(TR, gen, r0 + lr[k], rr[k]);
+            if (lo[k] >= 0) XH[lo[k]] = rr[k];
+        }
+        if (threadIdx.x == 0) slot_publish3(a.slots, gen, 0.0, 0.0, 0.0);
+        halo(TR, gen);
+        if (!collect(gen)) { status = 2; goto done; }
+    }
+done:
+    if (status == 0 || status == 1) {
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (own[k] && it > 0) a.x[r0 + lr[k]] = xr[k];
+    }
+    if (a.probe && threadIdx.x == 0) {
+        const uint64_t total = wall_clock64() - t_start;
+        a.probe[blockIdx.x * 4 + 1] = total - t_wait - a.probe[blockIdx.x * 4 + 0];
+        a.probe[blockIdx.x * 4 + 2] = t_wait;
+        a.probe[blockIdx.x * 4 + 3] = total;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.out_iter = status == 0 ? it : a.max_iter;
         *a.out_resid = resid;
         *a.out_status = status;
     }

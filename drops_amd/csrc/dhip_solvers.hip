@@ -1,13 +1,13 @@
 // dhip_solvers.hip -- the persistent Krylov solver kernels (FP64 PCG in the reference's order and in
 // the single-reduction arrangement, FP32-inner / FP64-refined PCG, GMRES), compiled apart from the
-// rest of libdhip.  The file is compiled once per DHIP_SOLVER_PART (0 .. 13), each compilation
+// rest of libdhip.  The file is compiled once per DHIP_SOLVER_PART (0 .. 14), each compilation
 // instantiating one share of the kernels, so that the shares build in parallel; the host code in
 // dhip.hip launches them through their declarations in dhip_krylov.hpp.
 #define DHIP_SOLVER_KERNELS
 #include "dhip_krylov.hpp"
 
 #ifndef DHIP_SOLVER_PART
-#error "compile with -DDHIP_SOLVER_PART=<0..13>"
+#error "compile with -DDHIP_SOLVER_PART=<0..14>"
 #endif
 
 namespace dhip {
@@ -68,6 +68,10 @@ template __global__ void k_gmres<4, 0, true, true>(GmresArgs);
 template __global__ void k_gmres<8, 0, true, true>(GmresArgs);
 #elif DHIP_SOLVER_PART == 13
 template __global__ void k_gmres<16, 0, true, true>(GmresArgs);
+#elif DHIP_SOLVER_PART == 14
+template __global__ void k_pcg64_pipe<1>(PcgTxArgs);
+template __global__ void k_pcg64_pipe<2>(PcgTxArgs);
+template __global__ void k_pcg64_pipe<4>(PcgTxArgs);
 #endif
 
 }  // namespace dhip

@@ -16,7 +16,7 @@ import numpy as np
 from . import dhip as D
 
 CANDIDATES = (128, 192, 256, 320, 384, 512)
-EXCHANGES = (1, 2)   # coherent atomics, tagged records with the halo in LDS
+EXCHANGES = (1, 2, 3)   # coherent atomics, tagged records with the halo in LDS, pipelined over them
 
 
 def tune_pcg(ctx, A, b_dev, x_dev, zero_dev, n, tol=1e-8, precision=D.FP64, candidates=CANDIDATES,
