@@ -109,7 +109,10 @@ int dhip_solver_probe(dhip_ctx_t *ctx, int32_t on, uint64_t *ticks, int32_t *nbl
  * (L2-scope) loads and stores of the exchanged values without fences, 2 (default) as 1 and the FP64
  * PCG with generation-tagged 16-byte records and its block's gathered values staged in LDS (the
  * same iterates bit for bit while a block's slice fits its LDS; the single-reduction kernel
- * otherwise); "gmres_precision" -- DHIP_FP64 (default) or DHIP_FP32_IR for dhip_gmres and
+ * otherwise), 3 as 2 with the FP64 PCG in the pipelined arrangement of CG (one vector of records
+ * per iteration, the wait for the block partials overlapped with the next product; iterates equal
+ * CG's up to rounding, and b - A x is checked, restarting from it if needed, before it stops);
+ * "gmres_precision" -- DHIP_FP64 (default) or DHIP_FP32_IR for dhip_gmres and
  * dhip_gmres_diag, as dhip_ch_set_precision; "batch_threads",
  * "batch_blocks_per_cu", "batch_prepass", "batch_split", "batch_solve_threads" -- see
  * dhip_batch_solve. */
