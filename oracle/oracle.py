@@ -61,6 +61,16 @@ class BdfStat(C.Structure):
                 ("dt", C.c_double), ("e", C.c_double)]
 
 
+class QuaQua(C.Structure):
+    """orc_quaqua_t: the QuaQuaMapperCL parameters (LevelsetMapper.Iter / Tol / ArmijoConstant, damping steps)"""
+    _fields_ = [("maxiter", C.c_int32), ("tol", C.c_double), ("armijo_c", C.c_double), ("max_damping", C.c_int32)]
+
+
+def quaqua_params(maxiter=20, tol=1e-7, armijo_c=1e-2, max_damping=8):
+    """the reference's parameter files (param/surfactant/surfactant/surfactant.json: LevelsetMapper)"""
+    return QuaQua(maxiter, tol, armijo_c, max_damping)
+
+
 def _ptr(a, ctype):
     return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
 
@@ -128,6 +138,13 @@ def lib(kind="parity"):
     L.orc_assemble_p1band.argtypes = [M, D, I32, C.c_int64, C.c_double, D, C.c_int, D, C.c_int, C.c_int64,
                                       I64, I64, I32, D, I64, I64, I32, D, D, D]
     L.orc_refpatch_table.argtypes = [I32, I32, I32]
+    QQ = C.POINTER(QuaQua)
+    L.orc_p2_recovered_gradient.argtypes = [M, D, D]
+    L.orc_quaqua_point.argtypes = [M, D, D, C.c_int64, D, QQ, D, D, D, D, I32]
+    L.orc_quaqua_winv.argtypes = [M, D, D, C.c_int64, D, D, QQ, D]
+    L.orc_tet_local_p2qq.argtypes = [M, D, D, C.c_int64, QQ, C.c_int, D, C.c_int, C.POINTER(LocalP2)]
+    L.orc_assemble_p2qq.argtypes = [M, D, D, I32, C.c_int64, QQ, C.c_int, D, C.c_int, C.c_int64, I64, I64, I32,
+                                    D, D, D]
     _libs[name] = L
     return L
 
@@ -319,6 +336,79 @@ def assemble_p2(mesh, phi, dof_of_node, ndof, coef=(1.0, 1.0, 1.0), rhs_kind=RHS
     k = nnz.value
     return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(),
                 stab=vs[:k].copy(), rhs=rhs[:ndof].copy())
+
+
+def recovered_gradient(mesh, phi):
+    """averaging_P2_gradient_recovery: the recovered gradient at every half-grid node, shape (nodes, 3)"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    nn = (2 * mesh.n[0] + 1) * (2 * mesh.n[1] + 1) * (2 * mesh.n[2] + 1)
+    g = np.zeros(3 * nn)
+    _check(lib().orc_p2_recovered_gradient(C.byref(mesh), _ptr(phi, C.c_double), _ptr(g, C.c_double)),
+           "p2_recovered_gradient")
+    return g.reshape(nn, 3)
+
+
+def quaqua_point(mesh, phi, grad, tet, xb, params=None):
+    """base point (barycentric in the tet, world), dh, Jacobian dph (3x3) and Newton iterations"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    g = np.ascontiguousarray(grad, np.float64).reshape(-1)
+    xb = np.ascontiguousarray(xb, np.float64)
+    P = params if params is not None else quaqua_params()
+    bxb, y, dh, dph, it = np.zeros(4), np.zeros(3), C.c_double(), np.zeros(9), C.c_int32()
+    _check(lib().orc_quaqua_point(C.byref(mesh), _ptr(phi, C.c_double), _ptr(g, C.c_double), int(tet),
+                                  _ptr(xb, C.c_double), C.byref(P), _ptr(bxb, C.c_double), _ptr(y, C.c_double),
+                                  C.byref(dh), _ptr(dph, C.c_double), C.byref(it)), "quaqua_point")
+    return bxb, y, dh.value, dph.reshape(3, 3), it.value
+
+
+def quaqua_winv(mesh, phi, grad, tet, xb, nl, params=None):
+    """W^-1 (3x3) of the gradient transformation at xb for the plane normal nl"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    g = np.ascontiguousarray(grad, np.float64).reshape(-1)
+    xb = np.ascontiguousarray(xb, np.float64)
+    nl = np.ascontiguousarray(nl, np.float64)
+    P = params if params is not None else quaqua_params()
+    w = np.zeros(9)
+    _check(lib().orc_quaqua_winv(C.byref(mesh), _ptr(phi, C.c_double), _ptr(g, C.c_double), int(tet),
+                                 _ptr(xb, C.c_double), _ptr(nl, C.c_double), C.byref(P), _ptr(w, C.c_double)),
+           "quaqua_winv")
+    return w.reshape(3, 3)
+
+
+def tet_local_p2qq(mesh, phi, grad, tet, params=None, rhs_kind=RHS_NONE, rhs_param=None, zero_policy=ZERO_THROW):
+    phi = np.ascontiguousarray(phi, np.float64)
+    g = np.ascontiguousarray(grad, np.float64).reshape(-1)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    P = params if params is not None else quaqua_params()
+    out = LocalP2()
+    _check(lib().orc_tet_local_p2qq(C.byref(mesh), _ptr(phi, C.c_double), _ptr(g, C.c_double), int(tet),
+                                    C.byref(P), rhs_kind, _ptr(rp, C.c_double), zero_policy, C.byref(out)),
+           "tet_local_p2qq")
+    return out
+
+
+def assemble_p2qq(mesh, phi, grad, dof_of_node, ndof, params=None, rhs_kind=RHS_NONE, rhs_param=None,
+                  zero_policy=ZERO_THROW, kind="parity"):
+    """P2 mass, Laplace-Beltrami and load vector on the QuaQua-mapped surface:
+    dict(rowptr, colind, mass, lb, rhs), int64 row pointers"""
+    phi = np.ascontiguousarray(phi, np.float64)
+    g = np.ascontiguousarray(grad, np.float64).reshape(-1)
+    rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+    P = params if params is not None else quaqua_params()
+    L = lib(kind)
+    ncut = classify(mesh, phi, zero_policy).size
+    cap = 100 * max(ncut, 1)
+    rowptr = np.zeros(ndof + 1, np.int64)
+    col = np.zeros(cap, np.int32)
+    vm, vl = np.zeros(cap), np.zeros(cap)
+    rhs = np.zeros(max(ndof, 1))
+    nnz = C.c_int64()
+    _check(L.orc_assemble_p2qq(C.byref(mesh), _ptr(phi, C.c_double), _ptr(g, C.c_double),
+                               _ptr(dof_of_node, C.c_int32), ndof, C.byref(P), rhs_kind, _ptr(rp, C.c_double),
+                               zero_policy, cap, C.byref(nnz), _ptr(rowptr, C.c_int64), _ptr(col, C.c_int32),
+                               _ptr(vm, C.c_double), _ptr(vl, C.c_double), _ptr(rhs, C.c_double)), "assemble_p2qq")
+    k = nnz.value
+    return dict(rowptr=rowptr, colind=col[:k].copy(), mass=vm[:k].copy(), lb=vl[:k].copy(), rhs=rhs[:ndof].copy())
 
 
 def surface_area(mesh, phi, zero_policy=ZERO_ALLOW):

@@ -323,8 +323,9 @@ typedef struct {
 } patch_t;
 
 /* geom/subtriangulation.tpp:194-231 (make_patch), :149-177 (make_sub_facet),
- * geom/subtriangulation.h:514-527 (MergeCutPolicyCL), misc/container.h:403-410 (ConvexComb) */
-static int make_patch(const double ls[10], patch_t *p)
+ * geom/subtriangulation.h:514-527 (MergeCutPolicyCL), misc/container.h:403-410 (ConvexComb);
+ * nsub lattice tets sub[] over lattice vertices with barycentric coordinates B[] and values ls[] */
+static int make_patch_on(const double *ls, int nsub, const int (*sub)[4], const double (*B)[4], patch_t *p)
 {
     int ekey[MAXV][2];
     double cutb[MAXV][4];
@@ -335,8 +336,8 @@ static int make_patch(const double ls[10], patch_t *p)
     int zf[3 * MAXF], zj[3 * MAXF], nzu = 0;
     for (int i = 0; i < 10; ++i) copied[i] = -1;
     p->nf = 0;
-    for (int lt = 0; lt < 8; ++lt) {
-        const int *v = LAT_T[lt];
+    for (int lt = 0; lt < nsub; ++lt) {
+        const int *v = sub[lt];
         double lls[4];
         int s[4];
         for (int q = 0; q < 4; ++q) {
@@ -354,7 +355,7 @@ static int make_patch(const double ls[10], patch_t *p)
                     const int lv = v[rep];
                     if (copied[lv] < 0) {
                         copied[lv] = nzero;
-                        memcpy(zerob[nzero], LAT_B[lv], sizeof zerob[0]);
+                        memcpy(zerob[nzero], B[lv], sizeof zerob[0]);
                         ++nzero;
                     }
                     out[j] = copied[lv];
@@ -371,7 +372,7 @@ static int make_patch(const double ls[10], patch_t *p)
                     if (found < 0) {
                         const double alpha = lls[v0] / (lls[v0] - lls[v1]);
                         for (int c = 0; c < 4; ++c)
-                            cutb[ncut][c] = (1.0 - alpha) * LAT_B[a][c] + alpha * LAT_B[b][c];
+                            cutb[ncut][c] = (1.0 - alpha) * B[a][c] + alpha * B[b][c];
                         ekey[ncut][0] = ka;
                         ekey[ncut][1] = kb;
                         found = ncut++;
@@ -389,6 +390,21 @@ static int make_patch(const double ls[10], patch_t *p)
     memcpy(p->vb, cutb, sizeof(double) * 4 * ncut);
     memcpy(p->vb + ncut, zerob, sizeof(double) * 4 * nzero);
     return ORC_OK;
+}
+
+/* the patch of the P2 level set on lattice 2 (its eight sub-tets) */
+static int make_patch(const double ls[10], patch_t *p)
+{
+    return make_patch_on(ls, 8, (const int (*)[4])LAT_T, (const double (*)[4])LAT_B, p);
+}
+
+/* the patch on lattice 1: the tet itself with the level set at its four vertices (lattice-1 vertex order
+ * is the tet's vertex order, geom/principallattice.cpp:33-44) */
+static int make_patch_l1(const double ls4[4], patch_t *p)
+{
+    static const int sub1[1][4] = {{0, 1, 2, 3}};
+    static const double B1[4][4] = {{1., 0., 0., 0.}, {0., 1., 0., 0.}, {0., 0., 1., 0.}, {0., 0., 0., 1.}};
+    return make_patch_on(ls4, 1, sub1, B1, p);
 }
 
 /* Bary2WorldCoordCL: SMatrixCL<3,4> * b, geom/simplex.h:867-881, misc/container.h:966-973 */
@@ -1503,6 +1519,633 @@ int orc_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
                 if (val_mass) val_mass[dst] = R->v[0][ord[k]];
                 if (val_lb) val_lb[dst] = R->v[1][ord[k]];
                 if (val_stab) val_stab[dst] = R->v[2][ord[k]];
+            }
+        }
+    }
+    free(cut); free(loc); free(numb); free(st); free(rows);
+    return err;
+}
+
+/* ------------------------------------------------------------------------ */
+/* QuaQua-mapped P2 geometry: InterfaceCommonDataP2CL + QuaQuaMapperCL with  */
+/* LocalMassP2CL, LocalLaplaceBeltramiP2CL, LocalVectorP2CL                   */
+/* ------------------------------------------------------------------------ */
+
+/* Householder QR of a small row-major R x C matrix: QRDecompCL<R,C>::prepare_solve,
+ * apply_reflection, apply_Q and Solve, misc/container.h:1210-1300 (std::pow(x, 2) taken as x * x,
+ * the correctly rounded square). */
+typedef struct {
+    int R, C;
+    double a[16], d[4], beta[4];
+} qr_t;
+#define QA(q, i, j) ((q)->a[(i) * (q)->C + (j)])
+
+static int qr_prepare(qr_t *q) /* 1: rank deficient (the reference throws) */
+{
+    for (int j = 0; j < q->C; ++j) {
+        double sigma = 0.;
+        for (int i = j; i < q->R; ++i) sigma += QA(q, i, j) * QA(q, i, j);
+        if (sigma == 0.) return 1;
+        q->d[j] = (QA(q, j, j) < 0 ? 1. : -1.) * sqrt(sigma);
+        q->beta[j] = 1. / (q->d[j] * QA(q, j, j) - sigma);
+        QA(q, j, j) -= q->d[j];
+        for (int k = j + 1; k < q->C; ++k) {
+            double sp = 0.;
+            for (int i = j; i < q->R; ++i) sp += QA(q, i, j) * QA(q, i, k);
+            sp *= q->beta[j];
+            for (int i = j; i < q->R; ++i) QA(q, i, k) += QA(q, i, j) * sp;
+        }
+    }
+    return 0;
+}
+
+static void qr_reflect(const qr_t *q, int j, double *b)
+{
+    double sp = 0.;
+    for (int i = j; i < q->R; ++i) sp += QA(q, i, j) * b[i];
+    sp *= q->beta[j];
+    for (int i = j; i < q->R; ++i) b[i] += QA(q, i, j) * sp;
+}
+
+static void qr_apply_Q(const qr_t *q, double *b)
+{
+    for (int j = q->C - 1; j >= 0; --j) qr_reflect(q, j, b);
+}
+
+static void qr_solve(const qr_t *q, double *b)
+{
+    for (int j = 0; j < q->C; ++j) qr_reflect(q, j, b);
+    for (int i = q->C - 1; i >= 0; --i) {
+        for (int j = i + 1; j < q->C; ++j) b[i] -= QA(q, i, j) * b[j];
+        b[i] /= q->d[i];
+    }
+}
+
+/* GetTrafoTr, geom/multigrid.h:627-647 */
+static void trafo_tr(const double X[4][3], double T[3][3], double *det)
+{
+    double M[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[j][i] = X[i + 1][j] - X[0][j];
+    *det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1])
+         - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0])
+         + M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    const double d = *det;
+    T[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / d;
+    T[0][1] = (M[2][0] * M[1][2] - M[1][0] * M[2][2]) / d;
+    T[0][2] = (M[1][0] * M[2][1] - M[2][0] * M[1][1]) / d;
+    T[1][0] = (M[2][1] * M[0][2] - M[0][1] * M[2][2]) / d;
+    T[1][1] = (M[0][0] * M[2][2] - M[2][0] * M[0][2]) / d;
+    T[1][2] = (M[2][0] * M[0][1] - M[0][0] * M[2][1]) / d;
+    T[2][0] = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) / d;
+    T[2][1] = (M[1][0] * M[0][2] - M[0][0] * M[1][2]) / d;
+    T[2][2] = (M[0][0] * M[1][1] - M[1][0] * M[0][1]) / d;
+}
+
+/* FE_P2CL::DH0Ref .. DH9Ref, num/fe.tpp:122-190 */
+static void p2_dhref(int i, double v1, double v2, double v3, double g[3])
+{
+    switch (i) {
+    case 0: g[0] = g[1] = g[2] = 4. * (v1 + v2 + v3) - 3.; break;
+    case 1: g[0] = 4. * v1 - 1.; g[1] = g[2] = 0.; break;
+    case 2: g[0] = g[2] = 0.; g[1] = 4 * v2 - 1.; break;
+    case 3: g[0] = 0.; g[1] = 0.; g[2] = 4. * v3 - 1.; break;
+    case 4: g[0] = 4. * (1. - (2. * v1 + v2 + v3)); g[1] = g[2] = -4. * v1; break;
+    case 5: g[0] = g[2] = -4. * v2; g[1] = 4. * (1. - (2. * v2 + v1 + v3)); break;
+    case 6: g[0] = 4. * v2; g[1] = 4. * v1; g[2] = 0.; break;
+    case 7: g[0] = g[1] = -4. * v3; g[2] = 4. * (1. - (2. * v3 + v1 + v2)); break;
+    case 8: g[0] = 4. * v3; g[1] = 0.; g[2] = 4. * v1; break;
+    default: g[0] = 0.; g[1] = 4. * v3; g[2] = 4. * v2; break;
+    }
+}
+
+/* 3x3 matrix times vector, misc/container.h:975-984 */
+static void mv3(const double A[3][3], const double v[3], double r[3])
+{
+    for (int i = 0; i < 3; ++i) r[i] = A[i][0] * v[0] + A[i][1] * v[1] + A[i][2] * v[2];
+}
+
+/* inner_prod of 3-vectors, misc/container.h:460-464; norm = sqrt of it */
+static double ip3(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static double norm3(const double a[3]) { return sqrt(ip3(a, a)); }
+
+/* one tet of the mapped geometry: the cached quantities of base_point_newton_cacheCL::set_tetra
+ * (levelset/levelsetmapper.cpp:77-100) and the World2BaryCoordCL factorisation (geom/simplex.cpp:456-484) */
+typedef struct {
+    double X[4][3];
+    double T[3][3], det, h;
+    double G[10][4][3];  /* P2DiscCL::GetGradients (num/discretize.h:1430-1431): T * DH_iRef at the vertices */
+    double v[10];        /* level set in P2 dof order */
+    double gh[10][3];    /* recovered gradient at the P2 nodes */
+    qr_t w2b;
+} qqtet_t;
+
+static void qq_gradients(const double X[4][3], double T[3][3], double *det, double G[10][4][3])
+{
+    static const double ref[4][3] = {{0., 0., 0.}, {1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+    trafo_tr(X, T, det);
+    for (int i = 0; i < 10; ++i)
+        for (int k = 0; k < 4; ++k) {
+            double g[3];
+            p2_dhref(i, ref[k][0], ref[k][1], ref[k][2], g);
+            mv3((const double (*)[3])T, g, G[i][k]);
+        }
+}
+
+/* LocalP1CL<Point3DCL> at barycentric b (LinearCombinationCL::do_it, num/fe.h:44-53) */
+static void p1v_eval(const double G[4][3], const double b[4], double r[3])
+{
+    for (int c = 0; c < 3; ++c) r[c] = b[0] * G[0][c] + b[1] * G[1][c] + b[2] * G[2][c] + b[3] * G[3][c];
+}
+
+/* LocalP2CL<Point3DCL> at barycentric b */
+static void p2v_eval(const double g[10][3], const double b[4], double r[3])
+{
+    double h[10];
+    p2_basis(b, h);
+    for (int c = 0; c < 3; ++c) {
+        double s = h[0] * g[0][c];
+        for (int d = 1; d < 10; ++d) s += h[d] * g[d][c];
+        r[c] = s;
+    }
+}
+
+/* LocalP2GradientCL::set_tetra, num/gradient_recovery.cpp:62-76: the gradient of the tet's P2 level
+ * set at its vertices, and at the edge midpoints the mean of the two vertex values */
+static void local_p2_gradient(const double G[10][4][3], const double v[10], double out[10][3])
+{
+    double p1[4][3] = {{0.}};
+    for (int i = 0; i < 10; ++i)
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) p1[k][c] += v[i] * G[i][k][c];
+    for (int k = 0; k < 4; ++k) memcpy(out[k], p1[k], sizeof p1[k]);
+    for (int e = 0; e < 6; ++e)
+        for (int c = 0; c < 3; ++c) out[4 + e][c] = 0.5 * (p1[VOE[e][0]][c] + p1[VOE[e][1]][c]);
+}
+
+/* averaging_P2_gradient_recovery (num/gradient_recovery.cpp:82-89) through OswaldProjectionP2AccuCL::visit
+ * (num/oswald_projection.h:102-125): every tet of the mesh in id order, each node's value the running mean
+ * ((n - 1)/n) avg + (1/n) local over the tets that contain it.  grad: 3 doubles per half-grid node. */
+int orc_p2_recovered_gradient(const orc_mesh_t *m, const double *phi, double *grad)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !grad) return ORC_E_ARG;
+    const int64_t nn = (2LL * m->n[0] + 1) * (2LL * m->n[1] + 1) * (2LL * m->n[2] + 1);
+    const int64_t ntet = 6LL * m->n[0] * m->n[1] * m->n[2];
+    double *cnt = (double *)calloc((size_t)nn, sizeof(double));
+    if (!cnt) return ORC_E_NOMEM;
+    for (int64_t i = 0; i < 3 * nn; ++i) grad[i] = 0.;
+    for (int64_t t = 0; t < ntet; ++t) {
+        tet_t T;
+        load_tet(m, phi, t, &T);
+        double Tm[3][3], det, G[10][4][3], v[10], loc[10][3];
+        qq_gradients((const double (*)[3])T.X, Tm, &det, G);
+        for (int d = 0; d < 10; ++d) v[d] = T.ls[LAT_OF_P2[d]];
+        local_p2_gradient((const double (*)[4][3])G, v, loc);
+        for (int i = 0; i < 10; ++i) {
+            const int64_t g = p2_node_index(m, &T, i);
+            const double n = (cnt[g] += 1.);
+            for (int c = 0; c < 3; ++c) grad[3 * g + c] = ((n - 1.) / n) * grad[3 * g + c] + (1. / n) * loc[i][c];
+        }
+    }
+    free(cnt);
+    return ORC_OK;
+}
+
+static int qq_tet(const orc_mesh_t *m, const double *phi, const double *grad, int64_t t, qqtet_t *Q, tet_t *T)
+{
+    load_tet(m, phi, t, T);
+    memcpy(Q->X, T->X, sizeof Q->X);
+    qq_gradients((const double (*)[3])T->X, Q->T, &Q->det, Q->G);
+    Q->h = cbrt(fabs(Q->det));
+    for (int d = 0; d < 10; ++d) {
+        Q->v[d] = T->ls[LAT_OF_P2[d]];
+        const int64_t g = p2_node_index(m, T, d);
+        for (int c = 0; c < 3; ++c) Q->gh[d][c] = grad[3 * g + c];
+    }
+    Q->w2b.R = Q->w2b.C = 4;
+    for (int v = 0; v < 4; ++v) {
+        for (int i = 0; i < 3; ++i) QA(&Q->w2b, i, v) = T->X[v][i];
+        QA(&Q->w2b, 3, v) = 1.;
+    }
+    return qr_prepare(&Q->w2b) ? ORC_E_ARG : ORC_OK;
+}
+
+/* the state of QuaQuaMapperFunctionCL (levelset/levelsetmapper.cpp:112-290): point (y, s), its barycentric
+ * coordinates in the tet, the recovered gradient there and F = (x0 - y - s gh, -ls) */
+typedef struct {
+    double xcur[4], bxb[4], gh[3], F[4], x0[3];
+    int have_dF;
+    qr_t dFqr;
+    double dF[4][4];
+} qqfun_t;
+
+static void qqfun_compute_F(const qqtet_t *Q, qqfun_t *f)
+{
+    for (int i = 0; i < 3; ++i) f->F[i] = f->x0[i] - f->xcur[i] - f->xcur[3] * f->gh[i];
+    f->F[3] = -p2_eval(Q->v, f->bxb);
+}
+
+/* set_point: the new point's barycentric coordinates in the same tet.  locate_new_point
+ * (levelsetmapper.cpp:463-490) searches a neighbourhood only while its tetra pointer is null, which it
+ * never is here, so points that leave the tet keep it: the level set and the recovered gradient are
+ * extrapolated from the starting tet. */
+static void qqfun_set_point(const qqtet_t *Q, qqfun_t *f, const double x[4])
+{
+    if (f->xcur[0] == x[0] && f->xcur[1] == x[1] && f->xcur[2] == x[2] && f->xcur[3] == x[3]) return;
+    double b[4] = {x[0], x[1], x[2], 1.};
+    qr_solve(&Q->w2b, b);
+    f->have_dF = 0;
+    memcpy(f->xcur, x, sizeof f->xcur);
+    memcpy(f->bxb, b, sizeof b);
+    p2v_eval((const double (*)[3])Q->gh, f->bxb, f->gh);
+    qqfun_compute_F(Q, f);
+}
+
+/* compute_dF and compute_dFinv, levelsetmapper.cpp:222-270 */
+static int qqfun_dF(const qqtet_t *Q, qqfun_t *f)
+{
+    if (f->have_dF) return 0;
+    double g_ls[3] = {0., 0., 0.}, dgh[3][3] = {{0.}};
+    for (int i = 0; i < 10; ++i) {
+        double t[3];
+        p1v_eval((const double (*)[3])Q->G[i], f->bxb, t);
+        for (int c = 0; c < 3; ++c) g_ls[c] += Q->v[i] * t[c];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) dgh[r][c] += Q->gh[i][r] * t[c];
+    }
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) f->dF[i][j] = -f->xcur[3] * dgh[i][j];
+        f->dF[i][i] -= 1.;
+        f->dF[i][3] = -f->gh[i];
+        f->dF[3][i] = -g_ls[i];
+    }
+    f->dF[3][3] = 0.;
+    f->dFqr.R = f->dFqr.C = 4;
+    memcpy(f->dFqr.a, f->dF, sizeof f->dF);
+    if (qr_prepare(&f->dFqr)) return 1;
+    f->have_dF = 1;
+    return 0;
+}
+
+static void mv4(const double A[4][4], const double v[4], double r[4])
+{
+    for (int i = 0; i < 4; ++i) r[i] = A[i][0] * v[0] + A[i][1] * v[1] + A[i][2] * v[2] + A[i][3] * v[3];
+}
+
+static double dot4(const double a[4], const double b[4])
+{
+    double s = 0.0;
+    for (int i = 0; i < 4; ++i) s += a[i] * b[i];
+    return s;
+}
+
+/* the base point of barycentric point xb of the tet: QuaQuaMapperCL::base_point_newton
+ * (levelsetmapper.cpp:551-566) with newton_solve (num/newton.h:64-115).  Returns the base point's
+ * barycentric coordinates (in the tet), dh and the Newton iteration count; 1 on a singular Jacobian. */
+static int qq_base_point(const qqtet_t *Q, const orc_quaqua_t *P, const double xb[4], double bxb[4], double *dh,
+                         int32_t *iters)
+{
+    qqfun_t f;
+    memset(&f, 0, sizeof f);
+    memcpy(f.bxb, xb, sizeof f.bxb);
+    bary_to_world(Q->X, xb, f.x0);
+    f.xcur[0] = f.x0[0]; f.xcur[1] = f.x0[1]; f.xcur[2] = f.x0[2]; f.xcur[3] = 0.;
+    p2v_eval((const double (*)[3])Q->gh, f.bxb, f.gh);
+    f.F[0] = f.F[1] = f.F[2] = 0.;
+    f.F[3] = -p2_eval(Q->v, f.bxb);
+    double x[4];
+    memcpy(x, f.xcur, sizeof x);
+    const double min_step_length = 1e-7;
+    int iter;
+    for (iter = 0;; ++iter) {
+        qqfun_set_point(Q, &f, x);
+        const double normF = sqrt(dot4(f.F, f.F));
+        if (iter >= P->maxiter || normF < P->tol) break;
+        if (qqfun_dF(Q, &f)) return 1;
+        double dx[4], Ddx[4];
+        memcpy(dx, f.F, sizeof dx);
+        qr_solve(&f.dFqr, dx);
+        mv4((const double (*)[4])f.dF, dx, Ddx);
+        const double armijo_slope = P->armijo_c * dot4(f.F, Ddx) / normF;
+        const double dxn = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        double l = 0.5 * Q->h / dxn;
+        if (!(l < 1.)) l = 1.;
+        for (int j = 0; j < P->max_damping && l >= min_step_length; ++j, l *= 0.5) {
+            double xt[4];
+            for (int i = 0; i < 4; ++i) xt[i] = x[i] - l * dx[i];
+            qqfun_set_point(Q, &f, xt);
+            if (sqrt(dot4(f.F, f.F)) < normF + armijo_slope * l) break;
+        }
+        if (l < min_step_length) break;
+        for (int i = 0; i < 4; ++i) x[i] -= l * dx[i];
+    }
+    memcpy(bxb, f.bxb, sizeof f.bxb);
+    *dh = x[3] * norm3(f.gh);
+    if (iters) *iters = iter;
+    return 0;
+}
+
+/* QuaQuaMapperCL::jacobian, levelsetmapper.cpp:590-638: dph = (I + dh Q dn)^-1 Q at the base point */
+static int qq_jacobian(const qqtet_t *Q, const double bxb[4], double dh, double dph[3][3])
+{
+    double gh[3], gp[10][3], n[3] = {0., 0., 0.}, dgh[3][3] = {{0.}};
+    p2v_eval((const double (*)[3])Q->gh, bxb, gh);
+    const double ghn = norm3(gh);
+    double qn[3];
+    for (int c = 0; c < 3; ++c) qn[c] = gh[c] / ghn;
+    for (int i = 0; i < 10; ++i) {
+        p1v_eval((const double (*)[3])Q->G[i], bxb, gp[i]);
+        for (int c = 0; c < 3; ++c) n[c] += Q->v[i] * gp[i][c];
+    }
+    const double nn = norm3(n);
+    for (int c = 0; c < 3; ++c) n[c] /= nn;
+    for (int i = 0; i < 10; ++i)
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) dgh[r][c] += Q->gh[i][r] * gp[i][c];
+    double tq[3];
+    for (int col = 0; col < 3; ++col) {
+        double s = 0.0;
+        for (int i = 0; i < 3; ++i) s += dgh[i][col] * qn[i];
+        tq[col] = s;
+    }
+    double dn[3][3], Qm[3][3];
+    const double ig = 1. / ghn;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) dn[r][c] = ig * (dgh[r][c] - qn[r] * tq[c]);
+    const double qnn = ip3(qn, n);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) Qm[r][c] = (r == c ? 1. : 0.) - (qn[r] / qnn) * n[c];
+    qr_t A;
+    A.R = A.C = 3;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int i = 0; i < 3; ++i) s += (dh * Qm[r][i]) * dn[i][c];
+            QA(&A, r, c) = (r == c ? 1. : 0.) + s;
+        }
+    if (qr_prepare(&A)) return 1;
+    for (int i = 0; i < 3; ++i) {
+        double t[3] = {Qm[0][i], Qm[1][i], Qm[2][i]};
+        qr_solve(&A, t);
+        for (int r = 0; r < 3; ++r) dph[r][i] = t[r];
+    }
+    return 0;
+}
+
+/* W^-1 of gradient_trafo (ifacetransp.cpp:894-918): W = dph^T + nl (n/(nl.n) - dph nl), nl the normal of
+ * the patch's first facet, n the unit level-set gradient at the base point (QuaQuaMapperCL::local_ls_grad) */
+static int qq_winv(const qqtet_t *Q, const double bxb[4], const double dph[3][3], const double nl[3], double Wi[3][3])
+{
+    double n[3] = {0., 0., 0.};
+    for (int i = 0; i < 10; ++i) {
+        double t[3];
+        p1v_eval((const double (*)[3])Q->G[i], bxb, t);
+        for (int c = 0; c < 3; ++c) n[c] += Q->v[i] * t[c];
+    }
+    const double nn = norm3(n);
+    for (int c = 0; c < 3; ++c) n[c] /= nn;
+    double dnl[3], tv[3];
+    mv3(dph, nl, dnl);
+    const double nln = ip3(nl, n);
+    for (int c = 0; c < 3; ++c) tv[c] = n[c] / nln - dnl[c];
+    qr_t W;
+    W.R = W.C = 3;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) QA(&W, r, c) = dph[c][r] + nl[r] * tv[c];
+    if (qr_prepare(&W)) return 1;
+    for (int j = 0; j < 3; ++j) {
+        double t[3] = {0., 0., 0.};
+        t[j] = 1.;
+        qr_solve(&W, t);
+        for (int r = 0; r < 3; ++r) Wi[r][j] = t[r];
+    }
+    return 0;
+}
+
+/* W^-1 at barycentric point xb of the tet for the plane normal nl (row-major) */
+int orc_quaqua_winv(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet, const double xb[4],
+                    const double nl[3], const orc_quaqua_t *P, double winv[9])
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !grad || !xb || !nl || !P || !winv) return ORC_E_ARG;
+    qqtet_t Q;
+    tet_t T;
+    if (qq_tet(m, phi, grad, tet, &Q, &T)) return ORC_E_ARG;
+    double bxb[4], dh, J[3][3], Wi[3][3];
+    if (qq_base_point(&Q, P, xb, bxb, &dh, NULL) || qq_jacobian(&Q, bxb, dh, J) ||
+        qq_winv(&Q, bxb, (const double (*)[3])J, nl, Wi))
+        return ORC_E_NOT_CONVERGED;
+    memcpy(winv, Wi, sizeof Wi);
+    return ORC_OK;
+}
+
+int orc_quaqua_point(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet, const double xb[4],
+                     const orc_quaqua_t *P, double bxb[4], double y[3], double *dh, double dph[9], int32_t *iters)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !grad || !xb || !P || !bxb || !y || !dh || !dph) return ORC_E_ARG;
+    qqtet_t Q;
+    tet_t T;
+    if (qq_tet(m, phi, grad, tet, &Q, &T)) return ORC_E_ARG;
+    if (qq_base_point(&Q, P, xb, bxb, dh, iters)) return ORC_E_NOT_CONVERGED;
+    bary_to_world(Q.X, bxb, y);
+    double J[3][3];
+    if (qq_jacobian(&Q, bxb, *dh, J)) return ORC_E_NOT_CONVERGED;
+    memcpy(dph, J, sizeof J);
+    return ORC_OK;
+}
+
+/* Local P2 matrices of one tet on the mapped surface: the lattice-1 patch and its composite Quad5 rule
+ * (InterfaceCommonDataP2CL::visit, surfactant/ifacetransp.h:258-276), |det| of the map per point
+ * (ProjectedQuadDomain2DCL::assign, ifacetransp.cpp:850-892), W^-1 (gradient_trafo, ifacetransp.cpp:894-918)
+ * and LocalMassP2CL / LocalLaplaceBeltramiP2CL / LocalVectorP2CL (ifacetransp.h:2302-2403; the
+ * Laplace-Beltrami form carries no coefficient there).  out->nfacet = 0: the patch is empty. */
+static int local_p2qq(const qqtet_t *Q, const tet_t *T, const orc_quaqua_t *P, int rhs_kind, const double *rhs_param,
+                      orc_local_p2_t *o)
+{
+    memset(o, 0, sizeof *o);
+    double ls4[4];
+    for (int k = 0; k < 4; ++k) ls4[k] = Q->v[k];
+    patch_t p;
+    int err = make_patch_l1(ls4, &p);
+    if (err) return err;
+    if (p.nf == 0) return ORC_OK;
+    patch_geometry(Q->X, &p);
+    for (int f = 0; f < p.nf; ++f)
+        if (p.deficient[f]) return ORC_E_NOT_CONVERGED;
+    o->nfacet = p.nf;
+    double node[MAXF * 7][4], w[MAXF * 7];
+    const int nq = composite_nodes(&p, 7, Q5_N, Q5_W, node, w);
+    double H[MAXF * 7][10], Gq[MAXF * 7][10][3], ad[MAXF * 7], fq[MAXF * 7];
+    double U[3][2] = {{0.}};
+    for (int k = 0; k < nq; ++k) {
+        const int f = k / 7;
+        if (k % 7 == 0) {
+            qr_t M;
+            M.R = 3; M.C = 2;
+            const double *w0 = p.vw[p.facet[f][0]], *w1 = p.vw[p.facet[f][1]], *w2 = p.vw[p.facet[f][2]];
+            for (int r = 0; r < 3; ++r) {
+                QA(&M, r, 0) = w1[r] - w0[r];
+                QA(&M, r, 1) = w2[r] - w0[r];
+            }
+            if (qr_prepare(&M)) return ORC_E_NOT_CONVERGED;
+            for (int j = 0; j < 2; ++j) {
+                double t[3] = {0., 0., 0.};
+                t[j] = 1.;
+                qr_apply_Q(&M, t);
+                for (int r = 0; r < 3; ++r) U[r][j] = t[r];
+            }
+        }
+        double bxb[4], dh, dph[3][3];
+        if (qq_base_point(Q, P, node[k], bxb, &dh, NULL)) return ORC_E_NOT_CONVERGED;
+        if (qq_jacobian(Q, bxb, dh, dph)) return ORC_E_NOT_CONVERGED;
+        /* |det|: sqrt(det Gram(dph U)), GramMatrix misc/container.h:935-950 */
+        double DU[3][2];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 2; ++c) {
+                double s = 0.0;
+                for (int i = 0; i < 3; ++i) s += dph[r][i] * U[i][c];
+                DU[r][c] = s;
+            }
+        double g00 = 0.0, g10 = 0.0, g11 = 0.0;
+        for (int i = 0; i < 3; ++i) g00 += DU[i][0] * DU[i][0];
+        for (int i = 0; i < 3; ++i) g10 += DU[i][1] * DU[i][0];
+        for (int i = 0; i < 3; ++i) g11 += DU[i][1] * DU[i][1];
+        ad[k] = sqrt(g00 * g11 - g10 * g10);
+        double Wi[3][3];
+        if (qq_winv(Q, bxb, (const double (*)[3])dph, p.normal[0], Wi)) return ORC_E_NOT_CONVERGED;
+        /* the P2 gradients at the planar point, projected on the facet's plane, times W^-1 */
+        const double *nf = p.normal[f];
+        for (int i = 0; i < 10; ++i) {
+            double g[3], t[3];
+            p1v_eval((const double (*)[3])Q->G[i], node[k], g);
+            const double d = ip3(nf, g);
+            for (int c = 0; c < 3; ++c) t[c] = g[c] - d * nf[c];
+            mv3((const double (*)[3])Wi, t, Gq[k][i]);
+        }
+        p2_basis(node[k], H[k]);
+        fq[k] = 0.;
+        if (rhs_kind != ORC_RHS_NONE) {
+            double yw[3];
+            bary_to_world(Q->X, bxb, yw);
+            fq[k] = rhs_fun(rhs_kind, rhs_param, yw);
+        }
+    }
+    for (int i = 0; i < 10; ++i) {
+        for (int j = 0; j <= i; ++j) {
+            double sm = 0.0, sl = 0.0;
+            for (int k = 0; k < nq; ++k) {
+                sm += w[k] * ((ad[k] * H[k][j]) * H[k][i]);
+                sl += w[k] * (ad[k] * ip3(Gq[k][j], Gq[k][i]));
+            }
+            o->mass[i * 10 + j] = o->mass[j * 10 + i] = sm;
+            o->lb[i * 10 + j] = o->lb[j * 10 + i] = sl;
+        }
+        double sr = 0.0;
+        if (rhs_kind != ORC_RHS_NONE)
+            for (int k = 0; k < nq; ++k) sr += w[k] * ((ad[k] * fq[k]) * H[k][i]);
+        o->rhs[i] = sr;
+    }
+    return ORC_OK;
+}
+
+int orc_tet_local_p2qq(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet,
+                       const orc_quaqua_t *P, int rhs_kind, const double *rhs_param, int zero_policy,
+                       orc_local_p2_t *out)
+{
+    (void)zero_policy;
+    init_tables();
+    if (!check_mesh(m) || !phi || !grad || !P || !out) return ORC_E_ARG;
+    qqtet_t Q;
+    tet_t T;
+    if (qq_tet(m, phi, grad, tet, &Q, &T)) return ORC_E_ARG;
+    return local_p2qq(&Q, &T, P, rhs_kind, rhs_param, out);
+}
+
+/* InterfaceMatrixAccuCL<LocalMassP2CL / LocalLaplaceBeltramiP2CL> and InterfaceVectorAccuCL<LocalVectorP2CL>
+ * over InterfaceCommonDataP2CL (surfactant/surfactant.cpp:1600-1640): the tets whose lattice-1 patch is
+ * not empty, in tet order, their couplings added into the P2 numbering's rows (an entry exists where
+ * such a tet couples two DOFs).  grad: the recovered gradient (orc_p2_recovered_gradient). */
+int orc_assemble_p2qq(const orc_mesh_t *m, const double *phi, const double *grad, const int32_t *dof_of_node,
+                      int64_t ndof, const orc_quaqua_t *P, int rhs_kind, const double *rhs_param, int zero_policy,
+                      int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind, double *val_mass,
+                      double *val_lb, double *rhs)
+{
+    init_tables();
+    if (!check_mesh(m) || !phi || !grad || !dof_of_node || !P || !nnz || !rowptr) return ORC_E_ARG;
+    int64_t ncut = 0;
+    int err = orc_classify(m, phi, zero_policy, 0, &ncut, NULL);
+    if (err) return err;
+    int64_t *cut = (int64_t *)malloc(sizeof(int64_t) * (ncut > 0 ? ncut : 1));
+    orc_local_p2_t *loc = (orc_local_p2_t *)malloc(sizeof(orc_local_p2_t) * (ncut > 0 ? ncut : 1));
+    int32_t (*numb)[10] = (int32_t (*)[10])malloc(sizeof(int32_t) * 10 * (ncut > 0 ? ncut : 1));
+    int *st = (int *)calloc(ncut > 0 ? ncut : 1, sizeof(int));
+    brow2_t *rows = (brow2_t *)calloc(ndof > 0 ? ndof : 1, sizeof(brow2_t));
+    if (!cut || !loc || !numb || !st || !rows) {
+        free(cut); free(loc); free(numb); free(st); free(rows);
+        return ORC_E_NOMEM;
+    }
+    orc_classify(m, phi, zero_policy, ncut, &ncut, cut);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t c = 0; c < ncut; ++c) {
+        qqtet_t Q;
+        tet_t T;
+        if (qq_tet(m, phi, grad, cut[c], &Q, &T)) { st[c] = ORC_E_ARG; continue; }
+        const int e = local_p2qq(&Q, &T, P, rhs_kind, rhs_param, &loc[c]);
+        if (e) { st[c] = e; continue; }
+        if (loc[c].nfacet == 0) { st[c] = -1; continue; }
+        for (int i = 0; i < 10; ++i) numb[c][i] = dof_of_node[p2_node_index(m, &T, i)];
+    }
+    if (rhs)
+        for (int64_t i = 0; i < ndof; ++i) rhs[i] = 0.;
+    for (int64_t c = 0; c < ncut && !err; ++c) {
+        if (st[c] > 0) { err = st[c]; break; }
+        if (st[c] < 0) continue;
+        for (int i = 0; i < 10 && !err; ++i) {
+            const int32_t r = numb[c][i];
+            if (r < 0) continue;
+            brow2_t *R = &rows[r];
+            for (int j = 0; j < 10; ++j) {
+                const int32_t cj = numb[c][j];
+                if (cj < 0) continue;
+                int k = 0;
+                for (; k < R->n; ++k)
+                    if (R->col[k] == cj) break;
+                if (k == R->n) {
+                    if (R->n == ROWCAP2) { err = ORC_E_CAPACITY; break; }
+                    R->col[k] = cj;
+                    R->v[0][k] = R->v[1][k] = R->v[2][k] = 0.0;
+                    ++R->n;
+                }
+                R->v[0][k] += loc[c].mass[i * 10 + j];
+                R->v[1][k] += loc[c].lb[i * 10 + j];
+            }
+            if (rhs) rhs[r] += loc[c].rhs[i];
+        }
+    }
+    if (!err) {
+        rowptr[0] = 0;
+        for (int64_t r = 0; r < ndof; ++r) rowptr[r + 1] = rowptr[r] + rows[r].n;
+        *nnz = rowptr[ndof];
+        if (*nnz > cap_nnz) err = ORC_E_CAPACITY;
+    }
+    if (!err && colind) {
+        for (int64_t r = 0; r < ndof; ++r) {
+            brow2_t *R = &rows[r];
+            int32_t ord[ROWCAP2];
+            for (int k = 0; k < R->n; ++k) ord[k] = k;
+            for (int a = 1; a < R->n; ++a) {
+                int32_t x = ord[a];
+                int b = a - 1;
+                while (b >= 0 && R->col[ord[b]] > R->col[x]) { ord[b + 1] = ord[b]; --b; }
+                ord[b + 1] = x;
+            }
+            for (int k = 0; k < R->n; ++k) {
+                const int64_t dst = rowptr[r] + k;
+                colind[dst] = R->col[ord[k]];
+                if (val_mass) val_mass[dst] = R->v[0][ord[k]];
+                if (val_lb) val_lb[dst] = R->v[1][ord[k]];
             }
         }
     }

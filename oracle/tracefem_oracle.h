@@ -177,6 +177,32 @@ int orc_assemble_p2(const orc_mesh_t *m, const double *phi, const int32_t *dof_o
                     int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind,
                     double *val_mass, double *val_lb, double *val_stab, double *rhs);
 
+/* ---- P2 interface space on the QuaQua-mapped surface (InterfaceCommonDataP2CL + QuaQuaMapperCL,
+ *      surfactant/ifacetransp.h:200-276, levelset/levelsetmapper.h:52-140) ---- */
+typedef struct {
+    int32_t maxiter;     /* LevelsetMapper.Iter (20 in the reference's parameter files) */
+    double tol;          /* LevelsetMapper.Tol (1e-7) */
+    double armijo_c;     /* LevelsetMapper.ArmijoConstant (1e-2) */
+    int32_t max_damping; /* the mapper's max_damping_steps (8) */
+} orc_quaqua_t;
+/* averaging_P2_gradient_recovery (num/gradient_recovery.cpp:82-89): 3 doubles per half-grid node */
+int orc_p2_recovered_gradient(const orc_mesh_t *m, const double *phi, double *grad);
+/* base point (barycentric in the tet, and world), quasi-distance dh and Jacobian dph (row-major) of
+ * barycentric point xb of the tet (QuaQuaMapperCL::base_point_newton and ::jacobian) */
+int orc_quaqua_point(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet, const double xb[4],
+                     const orc_quaqua_t *P, double bxb[4], double y[3], double *dh, double dph[9], int32_t *iters);
+/* W^-1 of gradient_trafo (surfactant/ifacetransp.cpp:894-918) at xb for the plane normal nl (row-major) */
+int orc_quaqua_winv(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet, const double xb[4],
+                    const double nl[3], const orc_quaqua_t *P, double winv[9]);
+/* LocalMassP2CL, LocalLaplaceBeltramiP2CL, LocalVectorP2CL of one tet (nfacet 0: empty lattice-1 patch) */
+int orc_tet_local_p2qq(const orc_mesh_t *m, const double *phi, const double *grad, int64_t tet,
+                       const orc_quaqua_t *P, int rhs_kind, const double *rhs_param, int zero_policy,
+                       orc_local_p2_t *out);
+int orc_assemble_p2qq(const orc_mesh_t *m, const double *phi, const double *grad, const int32_t *dof_of_node,
+                      int64_t ndof, const orc_quaqua_t *P, int rhs_kind, const double *rhs_param, int zero_policy,
+                      int64_t cap_nnz, int64_t *nnz, int64_t *rowptr, int32_t *colind, double *val_mass,
+                      double *val_lb, double *rhs);
+
 /* GMRES as above, preconditioned by z = b / d with a given diagonal d (block Jacobi) */
 int orc_gmres_diag(int64_t n, const int64_t *rowptr, const int32_t *colind, const double *val,
                    const double *pcdiag, const double *b, double *x, int32_t restart, int32_t *max_iter,
