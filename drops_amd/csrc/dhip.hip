@@ -4,6 +4,7 @@
 #include "dhip_common.hpp"
 #include "dhip_assembly.hpp"
 #include "dhip_p2.hpp"
+#include "dhip_quaqua.hpp"
 #include "dhip_krylov.hpp"
 #include "dhip_ch.hpp"
 #include "dhip_ch_p2.hpp"
@@ -363,6 +364,9 @@ static int check_err_flag(dhip_ctx *ctx) {
                          "found a full body in the zero level set, grid is too coarse");
     if (f & 1)
         return set_error(ctx, DHIP_E_ZERO_NODE, "levelset is exactly zero at node(s) of a cut tetra");
+    if (f & 4)
+        return set_error(ctx, DHIP_E_NOT_CONVERGED,
+                         "QuaQua mapping: rank-deficient matrix (a degenerate facet or a singular Newton step)");
     return DHIP_OK;
 }
 
@@ -918,6 +922,57 @@ static int build_p2_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
     c->stage = 4;
     if (nnz) *nnz = total;
     return DHIP_OK;
+}
+
+// the mapped-surface forms of every cut tet (after the planar pass has numbered the P2 DOFs): they
+// replace the mass, Laplace-Beltrami and load slots, and flags keep only tets with a lattice-1 patch
+static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t rhs_kind, const double *rhs_param) {
+    if (c->stage < 3 || c->space != 2)
+        return set_error(c, DHIP_E_STATE, "dhip_number_p2if must precede dhip_local_p2_quaqua");
+    if (rhs_kind == DHIP_RHS_VALUES)
+        return set_error(c, DHIP_E_ARG, "values at the planar patch points are not the mapped surface's load");
+    if (rhs_kind < DHIP_RHS_NONE || rhs_kind > DHIP_RHS_CONST)
+        return set_error(c, DHIP_E_ARG, "unknown right-hand-side kind %d", (int)rhs_kind);
+    QQParams P{20, 1e-7, 1e-2, 8};
+    if (p) {
+        if (p->maxiter < 0 || !(p->tol >= 0.) || p->max_damping < 0)
+            return set_error(c, DHIP_E_ARG, "QuaQua parameters: maxiter, max_damping >= 0, tol >= 0");
+        P = QQParams{p->maxiter, p->tol, p->armijo_c, p->max_damping};
+    }
+    const int64_t N = c->ncut;
+    DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+    LocalP2QQArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, rhs_kind, rhs_param ? rhs_param[0] : 0.0, P,
+                     c->loc.as<double>(), c->flags.as<uint8_t>(), c->errflag.as<int32_t>()};
+    if (N > 0) DHIP_LAUNCH(c, "local_p2qq", k_local_p2qq, (unsigned)N, 64, 0, la);
+    int rc = check_err_flag(c);
+    if (rc) return rc;
+    c->stage = 3;
+    return DHIP_OK;
+}
+
+int dhip_local_p2_quaqua(dhip_ctx_t *c, const dhip_quaqua_params_t *params, int32_t rhs_kind,
+                         const double *rhs_param) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    int rc = local_p2qq_impl(c, params, rhs_kind, rhs_param);
+    if (rc) return rc;
+    return flush_profile(c);
+}
+
+int dhip_assemble_p2_quaqua(dhip_ctx_t *c, int32_t zero_policy, const dhip_quaqua_params_t *params,
+                            int32_t rhs_kind, const double *rhs_param, double omit_bound, const double lc[3],
+                            int64_t *ncut, int64_t *ndof, int64_t *nnz) {
+    if (!c) return DHIP_E_ARG;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    static const double one3[3] = {1.0, 1.0, 1.0};
+    int rc = classify_impl(c, zero_policy, ncut, false);
+    if (!rc) rc = local_p2_impl(c, one3, DHIP_RHS_NONE, nullptr);
+    if (!rc) rc = number_p2_impl(c, omit_bound, ndof);
+    if (!rc) rc = local_p2qq_impl(c, params, rhs_kind, rhs_param);
+    if (!rc) rc = build_p2_impl(c, lc, nnz);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
 }
 
 int dhip_local_p2(dhip_ctx_t *c, const double coef[3], int32_t rhs_kind, const double *rhs_param) {

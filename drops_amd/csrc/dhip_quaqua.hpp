@@ -1,0 +1,694 @@
+// dhip_quaqua.hpp -- P2 interface forms on the QuaQua-mapped surface (gfx950).
+//
+// InterfaceCommonDataP2CL with a QuaQuaMapperCL (surfactant/ifacetransp.h:200-276,
+// levelset/levelsetmapper.cpp:551-638): the patch is the tet's lattice-1 patch (the linear
+// interpolant of the level set at its four vertices: one triangle or one planar quadrilateral), its
+// composite Quad5 points x are mapped to base points y on the zero level of the P2 level set by
+// damped Newton on (x - y - s g_h(y), -phi_h(y)) = 0 with the recovered gradient g_h, and the forms
+// carry |det| of the map's Jacobian and, for the Laplace-Beltrami form, its gradient transformation
+// W^-1 (LocalMassP2CL, LocalLaplaceBeltramiP2CL, LocalVectorP2CL, ifacetransp.h:2302-2403).
+//   k_local_p2qq  one wave per cut tet:
+//     lanes 0-9       the recovered gradient at the tet's ten P2 nodes: the running mean over every
+//                     tet of the mesh that holds the node, in tet id order (averaging_P2_gradient_
+//                     recovery through OswaldProjectionP2AccuCL, num/gradient_recovery.cpp:62-89);
+//     lanes 0-39      the tet's P2 gradient fields T DH_iRef at its vertices into LDS;
+//     lanes < 7 nf    one quadrature point each: base point, Jacobian, |det|, W^-1, the ten mapped
+//                     surface gradients and basis values, the load value at the base point;
+//     lanes < 55      pair (i >= j) of the mass and Laplace-Beltrami matrices, lanes 0-9 the load.
+//   Output: the mass, Laplace-Beltrami and load slots of the P2 local array (the stabilisation slot 0),
+//   flags 1 only for tets with a lattice-1 patch, so the P2 row builder adds exactly their couplings.
+// Every expression keeps the oracle's operation order (-ffp-contract=off): bit-identical except where
+// cbrt of the tet's |det| (the Newton damping length) rounds differently.
+#pragma once
+
+#include "dhip_p2.hpp"
+
+namespace dhip {
+
+struct QQParams {
+    int maxiter;
+    double tol, armijo_c;
+    int max_damping;
+};
+
+// Householder QR of a small row-major matrix held in registers (QRDecompCL<R,C>, misc/container.h:
+// 1210-1300; std::pow(x, 2) is the correctly rounded x * x).
+template <int R, int C>
+struct SmallQR {
+    double a[R * C], d[C], beta[C];
+    __device__ __forceinline__ double &at(int i, int j) { return a[i * C + j]; }
+    __device__ __forceinline__ double at(int i, int j) const { return a[i * C + j]; }
+    // false: rank deficient (the reference throws)
+    __device__ inline bool prepare() {
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            double sigma = 0.;
+#pragma unroll
+            for (int i = j; i < R; ++i) sigma += at(i, j) * at(i, j);
+            if (sigma == 0.) return false;
+            d[j] = (at(j, j) < 0 ? 1. : -1.) * sqrt(sigma);
+            beta[j] = 1. / (d[j] * at(j, j) - sigma);
+            at(j, j) -= d[j];
+#pragma unroll
+            for (int k = j + 1; k < C; ++k) {
+                double sp = 0.;
+#pragma unroll
+                for (int i = j; i < R; ++i) sp += at(i, j) * at(i, k);
+                sp *= beta[j];
+#pragma unroll
+                for (int i = j; i < R; ++i) at(i, k) += at(i, j) * sp;
+            }
+        }
+        return true;
+    }
+    __device__ __forceinline__ void reflect(int j, double *b) const {
+        double sp = 0.;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (i >= j) sp += at(i, j) * b[i];
+        sp *= beta[j];
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (i >= j) b[i] += at(i, j) * sp;
+    }
+    __device__ inline void apply_Q(double *b) const {
+#pragma unroll
+        for (int j = C - 1; j >= 0; --j) reflect(j, b);
+    }
+    __device__ inline void solve(double *b) const {
+#pragma unroll
+        for (int j = 0; j < C; ++j) reflect(j, b);
+#pragma unroll
+        for (int i = C - 1; i >= 0; --i) {
+#pragma unroll
+            for (int j = i + 1; j < C; ++j) b[i] -= at(i, j) * b[j];
+            b[i] /= d[i];
+        }
+    }
+};
+
+// GetTrafoTr, geom/multigrid.h:627-647
+__device__ inline double qq_trafo(const double X[4][3], double T[3][3]) {
+    double M[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[j][i] = X[i + 1][j] - X[0][j];
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                       M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+    T[0][0] = (M[1][1] * M[2][2] - M[1][2] * M[2][1]) / det;
+    T[0][1] = (M[2][0] * M[1][2] - M[1][0] * M[2][2]) / det;
+    T[0][2] = (M[1][0] * M[2][1] - M[2][0] * M[1][1]) / det;
+    T[1][0] = (M[2][1] * M[0][2] - M[0][1] * M[2][2]) / det;
+    T[1][1] = (M[0][0] * M[2][2] - M[2][0] * M[0][2]) / det;
+    T[1][2] = (M[2][0] * M[0][1] - M[0][0] * M[2][1]) / det;
+    T[2][0] = (M[0][1] * M[1][2] - M[1][1] * M[0][2]) / det;
+    T[2][1] = (M[1][0] * M[0][2] - M[0][0] * M[1][2]) / det;
+    T[2][2] = (M[0][0] * M[1][1] - M[1][0] * M[0][1]) / det;
+    return det;
+}
+
+// FE_P2CL::DH0Ref .. DH9Ref (num/fe.tpp:122-190) at reference vertex k: (0,0,0), e1, e2, e3
+__device__ inline void qq_dhref(int i, int k, double g[3]) {
+    const double v1 = k == 1 ? 1. : 0., v2 = k == 2 ? 1. : 0., v3 = k == 3 ? 1. : 0.;
+    switch (i) {
+    case 0: g[0] = g[1] = g[2] = 4. * (v1 + v2 + v3) - 3.; break;
+    case 1: g[0] = 4. * v1 - 1.; g[1] = g[2] = 0.; break;
+    case 2: g[0] = g[2] = 0.; g[1] = 4 * v2 - 1.; break;
+    case 3: g[0] = 0.; g[1] = 0.; g[2] = 4. * v3 - 1.; break;
+    case 4: g[0] = 4. * (1. - (2. * v1 + v2 + v3)); g[1] = g[2] = -4. * v1; break;
+    case 5: g[0] = g[2] = -4. * v2; g[1] = 4. * (1. - (2. * v2 + v1 + v3)); break;
+    case 6: g[0] = 4. * v2; g[1] = 4. * v1; g[2] = 0.; break;
+    case 7: g[0] = g[1] = -4. * v3; g[2] = 4. * (1. - (2. * v3 + v1 + v2)); break;
+    case 8: g[0] = 4. * v3; g[1] = 0.; g[2] = 4. * v1; break;
+    default: g[0] = 0.; g[1] = 4. * v3; g[2] = 4. * v2; break;
+    }
+}
+
+__device__ __forceinline__ void qq_mv3(const double A[3][3], const double v[3], double r[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) r[i] = A[i][0] * v[0] + A[i][1] * v[1] + A[i][2] * v[2];
+}
+
+__device__ __forceinline__ double qq_norm3(const double a[3]) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+// T DH_iRef at vertex k of the tet (P2DiscCL::GetGradients, num/discretize.h:1430-1431)
+__device__ __forceinline__ void qq_grad_at_vertex(const double T[3][3], int i, int k, double r[3]) {
+    double g[3];
+    qq_dhref(i, k, g);
+    qq_mv3(T, g, r);
+}
+
+struct QQShared {
+    double X[4][3];
+    double T[3][3];
+    double G[10][4][3];    // T DH_iRef at the vertices
+    double v[10];          // level set, P2 dof order
+    double gh[10][3];      // recovered gradient at the P2 nodes
+    double w2b[16], w2bd[4], w2bb[4];
+    double h;
+    double vb[2][3][4];    // lattice-1 facets: barycentric vertices
+    double af[2];          // (boundary ? 0.5 : 1) |det| of the facet
+    double nrm[2][3];      // facet normals Q e3
+    double U[2][3][2];     // Q e1, Q e2 of the facet QR
+    double w[14], ad[14], f[14], H[14][10], Gq[14][10][3];
+    int nf;
+};
+
+// LocalP1CL<Point3DCL> at b (LinearCombinationCL, num/fe.h:44-53)
+__device__ __forceinline__ void qq_p1v(const QQShared &s, int i, const double b[4], double r[3]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        r[c] = b[0] * s.G[i][0][c] + b[1] * s.G[i][1][c] + b[2] * s.G[i][2][c] + b[3] * s.G[i][3][c];
+}
+
+// LocalP2CL<Point3DCL> of the recovered gradient at b
+__device__ __forceinline__ void qq_p2v(const QQShared &s, const double b[4], double r[3]) {
+    double h[10];
+    p2_basis(b, h);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double t = h[0] * s.gh[0][c];
+#pragma unroll
+        for (int d = 1; d < 10; ++d) t += h[d] * s.gh[d][c];
+        r[c] = t;
+    }
+}
+
+// LocalP2CL<double> of the level set at b
+__device__ __forceinline__ double qq_p2s(const QQShared &s, const double b[4]) {
+    double h[10];
+    p2_basis(b, h);
+    double t = h[0] * s.v[0];
+#pragma unroll
+    for (int d = 1; d < 10; ++d) t += h[d] * s.v[d];
+    return t;
+}
+
+// World2BaryCoordCL (geom/simplex.cpp:479-484): the factorisation of [X; 1] in LDS
+__device__ inline void qq_w2b(const QQShared &s, const double x[3], double b[4]) {
+    SmallQR<4, 4> q;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) q.a[e] = s.w2b[e];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        q.d[e] = s.w2bd[e];
+        q.beta[e] = s.w2bb[e];
+    }
+    b[0] = x[0];
+    b[1] = x[1];
+    b[2] = x[2];
+    b[3] = 1.;
+    q.solve(b);
+}
+
+// QuaQuaMapperFunctionCL::set_point / compute_F: the tet is kept (locate_new_point never leaves it
+// when neighbourhoods are given), the level set and recovered gradient are the tet's polynomials.
+struct QQFun {
+    double xcur[4], bxb[4], gh[3], F[4], x0[3];
+};
+
+__device__ inline void qq_set_point(const QQShared &s, QQFun &f, const double x[4]) {
+    if (f.xcur[0] == x[0] && f.xcur[1] == x[1] && f.xcur[2] == x[2] && f.xcur[3] == x[3]) return;
+    double b[4];
+    qq_w2b(s, x, b);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        f.xcur[i] = x[i];
+        f.bxb[i] = b[i];
+    }
+    qq_p2v(s, f.bxb, f.gh);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) f.F[i] = f.x0[i] - f.xcur[i] - f.xcur[3] * f.gh[i];
+    f.F[3] = -qq_p2s(s, f.bxb);
+}
+
+__device__ __forceinline__ double qq_dot4(const double a[4], const double b[4]) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t += a[i] * b[i];
+    return t;
+}
+
+// base_point_newton (levelsetmapper.cpp:551-566) with newton_solve (num/newton.h:64-115); false on
+// a singular Jacobian
+__device__ inline bool qq_base_point(const QQShared &s, const QQParams &P, const double xb[4], double bxb[4],
+                                     double &dh) {
+    QQFun f;
+    b2w(s.X, xb, f.x0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) f.bxb[i] = xb[i];
+    f.xcur[0] = f.x0[0];
+    f.xcur[1] = f.x0[1];
+    f.xcur[2] = f.x0[2];
+    f.xcur[3] = 0.;
+    qq_p2v(s, f.bxb, f.gh);
+    f.F[0] = f.F[1] = f.F[2] = 0.;
+    f.F[3] = -qq_p2s(s, f.bxb);
+    double x[4] = {f.xcur[0], f.xcur[1], f.xcur[2], f.xcur[3]};
+    const double min_step_length = 1e-7;
+    for (int iter = 0;; ++iter) {
+        qq_set_point(s, f, x);
+        const double normF = sqrt(qq_dot4(f.F, f.F));
+        if (iter >= P.maxiter || normF < P.tol) break;
+        // compute_dF (levelsetmapper.cpp:222-249) and its QR
+        double g_ls[3] = {0., 0., 0.}, dgh[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+#pragma unroll
+        for (int i = 0; i < 10; ++i) {
+            double t[3];
+            qq_p1v(s, i, f.bxb, t);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) g_ls[c] += s.v[i] * t[c];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) dgh[r][c] += s.gh[i][r] * t[c];
+        }
+        SmallQR<4, 4> q;
+        double dF[4][4];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dF[i][j] = -f.xcur[3] * dgh[i][j];
+            dF[i][i] -= 1.;
+            dF[i][3] = -f.gh[i];
+            dF[3][i] = -g_ls[i];
+        }
+        dF[3][3] = 0.;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) q.at(i, j) = dF[i][j];
+        if (!q.prepare()) return false;
+        double dx[4] = {f.F[0], f.F[1], f.F[2], f.F[3]}, Ddx[4];
+        q.solve(dx);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Ddx[i] = dF[i][0] * dx[0] + dF[i][1] * dx[1] + dF[i][2] * dx[2] + dF[i][3] * dx[3];
+        const double armijo_slope = P.armijo_c * qq_dot4(f.F, Ddx) / normF;
+        const double dxn = sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+        double l = 0.5 * s.h / dxn;
+        if (!(l < 1.)) l = 1.;
+        for (int j = 0; j < P.max_damping && l >= min_step_length; ++j, l *= 0.5) {
+            double xt[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xt[i] = x[i] - l * dx[i];
+            qq_set_point(s, f, xt);
+            if (sqrt(qq_dot4(f.F, f.F)) < normF + armijo_slope * l) break;
+        }
+        if (l < min_step_length) break;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] -= l * dx[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) bxb[i] = f.bxb[i];
+    dh = x[3] * qq_norm3(f.gh);
+    return true;
+}
+
+// QuaQuaMapperCL::jacobian (levelsetmapper.cpp:590-638); also the unit level-set gradient n at the
+// base point (local_ls_grad); false on a singular matrix
+__device__ inline bool qq_jacobian(const QQShared &s, const double bxb[4], double dh, double dph[3][3], double n[3]) {
+    double gh[3], dgh[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
+    qq_p2v(s, bxb, gh);
+    const double ghn = qq_norm3(gh);
+    double qn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) qn[c] = gh[c] / ghn;
+    n[0] = n[1] = n[2] = 0.;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        double gp[3];
+        qq_p1v(s, i, bxb, gp);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) n[c] += s.v[i] * gp[c];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dgh[r][c] += s.gh[i][r] * gp[c];
+    }
+    const double nn = qq_norm3(n);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] /= nn;
+    double tq[3];
+#pragma unroll
+    for (int col = 0; col < 3; ++col) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) t += dgh[i][col] * qn[i];
+        tq[col] = t;
+    }
+    double dn[3][3], Qm[3][3];
+    const double ig = 1. / ghn;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dn[r][c] = ig * (dgh[r][c] - qn[r] * tq[c]);
+    const double qnn = qn[0] * n[0] + qn[1] * n[1] + qn[2] * n[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Qm[r][c] = (r == c ? 1. : 0.) - (qn[r] / qnn) * n[c];
+    SmallQR<3, 3> A;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double t = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) t += (dh * Qm[r][i]) * dn[i][c];
+            A.at(r, c) = (r == c ? 1. : 0.) + t;
+        }
+    if (!A.prepare()) return false;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        double t[3] = {Qm[0][i], Qm[1][i], Qm[2][i]};
+        A.solve(t);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) dph[r][i] = t[r];
+    }
+    return true;
+}
+
+struct LocalP2QQArgs {
+    MeshP m;
+    const double *phi;
+    const int64_t *cut;
+    int64_t ncut;
+    int rhs_kind;
+    double rhs_p0;
+    QQParams P;
+    double *loc;      // SoA [LOC2_N][ncut]
+    uint8_t *flags;   // bit0: the tet adds to the mapped matrices
+    int32_t *err;     // bit 1: full body in the zero level set, bit 2: singular matrix in the mapping
+};
+
+// recovered gradient at half-grid node J: running mean over the tets holding J in ascending id
+__device__ inline void qq_recovered(const MeshP &m, const double *phi, const int64_t J[3], double out[3]) {
+    int64_t lo[3], hi[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        lo[c] = J[c] >= 2 ? (J[c] - 1) >> 1 : 0;
+        hi[c] = J[c] >> 1;
+        if (hi[c] > m.n[c] - 1) hi[c] = m.n[c] - 1;
+    }
+    double n = 0., avg[3] = {0., 0., 0.};
+    for (int64_t i3 = lo[2]; i3 <= hi[2]; ++i3)
+        for (int64_t i2 = lo[1]; i2 <= hi[1]; ++i2)
+            for (int64_t i1 = lo[0]; i1 <= hi[0]; ++i1) {
+                const int rel = (int)((J[2] - 2 * i3) * 9 + (J[1] - 2 * i2) * 3 + (J[0] - 2 * i1));
+                const int64_t cell = (i3 * m.n[1] + i2) * m.n[0] + i1;
+                for (int k = 0; k < 6; ++k) {
+                    const int dd = c_tab.p2_of27[k][rel];
+                    if (dd < 0) continue;
+                    double X[4][3], ls[10], T[3][3];
+                    load_tet_geom(m, phi, cell * 6 + k, X, ls);
+                    qq_trafo(X, T);
+                    double vv[10];
+#pragma unroll
+                    for (int d = 0; d < 10; ++d) vv[d] = ls[c_tab.lat_of_p2[d]];
+                    // LocalP2GradientCL::set_tetra: the P1 gradient field at the vertices (summed over the
+                    // ten basis functions), at an edge node the mean of its two vertices
+                    const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+                    const int va = dd < 4 ? dd : ea[dd - 4], vb = dd < 4 ? dd : eb[dd - 4];
+                    double pa[3] = {0., 0., 0.}, pb[3] = {0., 0., 0.};
+#pragma unroll
+                    for (int i = 0; i < 10; ++i) {
+                        double g[3];
+                        qq_grad_at_vertex(T, i, va, g);
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) pa[c] += vv[i] * g[c];
+                        if (dd >= 4) {
+                            qq_grad_at_vertex(T, i, vb, g);
+#pragma unroll
+                            for (int c = 0; c < 3; ++c) pb[c] += vv[i] * g[c];
+                        }
+                    }
+                    double loc[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) loc[c] = dd < 4 ? pa[c] : 0.5 * (pa[c] + pb[c]);
+                    n += 1.;
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) avg[c] = ((n - 1.) / n) * avg[c] + (1. / n) * loc[c];
+                }
+            }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = avg[c];
+}
+
+__global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
+    __shared__ QQShared s;
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (c >= a.ncut) return;
+    const int64_t t = a.cut[c];
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, t, X, ls);
+    double v[10];
+#pragma unroll
+    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+    const int64_t N = a.ncut;
+    double *L = a.loc;
+    // the lattice-1 patch: the four vertex values' sign pattern
+    int sg[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sg[q] = sgn(v[q]);
+    if (!sg[0] && !sg[1] && !sg[2] && !sg[3]) {
+        if (lane == 0) atomicOr(a.err, 2);
+        return;
+    }
+    const int pid = sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40;
+    const int nf = c_tab.rp_size[pid];
+    if (nf == 0) {
+        for (int e = lane; e < LOC2_HAT; e += 64) L[(int64_t)e * N + c] = 0.;
+        if (lane == 0) a.flags[c] = 0;
+        return;
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) s.X[q][cc] = X[q][cc];
+        double T[3][3];
+        const double det = qq_trafo(X, T);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) s.T[r][cc] = T[r][cc];
+        s.h = cbrt(fabs(det));
+        SmallQR<4, 4> q;
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) q.at(i, vv) = X[vv][i];
+            q.at(3, vv) = 1.;
+        }
+        q.prepare();   // a tet of positive volume
+#pragma unroll
+        for (int e = 0; e < 16; ++e) s.w2b[e] = q.a[e];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            s.w2bd[e] = q.d[e];
+            s.w2bb[e] = q.beta[e];
+        }
+        s.nf = nf;
+    }
+    if (lane < 10) {
+        s.v[lane] = v[lane];
+        int64_t J[3];
+        p2_node_of(a.m, t, lane, J);
+        double g[3];
+        qq_recovered(a.m, a.phi, J, g);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) s.gh[lane][cc] = g[cc];
+    }
+    bool bad = false;
+    if (lane < nf) {
+        // facet `lane` of the lattice-1 patch: zero vertices or edge cuts (1 - a) e_a + a e_b
+        const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+        double vb[3][4];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int rep = c_tab.rp_tri[pid][lane][j];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) vb[j][cc] = 0.;
+            if (rep < 4) {
+                vb[j][rep] = 1.;
+            } else {
+                const int ia = ea[rep - 4], ib = eb[rep - 4];
+                const double alpha = v[ia] / (v[ia] - v[ib]);
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc)
+                    vb[j][cc] = (1.0 - alpha) * (cc == ia ? 1. : 0.) + alpha * (cc == ib ? 1. : 0.);
+            }
+        }
+        double w[3][3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) b2w(X, vb[j], w[j]);
+        SmallQR<3, 2> q;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            q.at(r, 0) = w[1][r] - w[0][r];
+            q.at(r, 1) = w[2][r] - w[0][r];
+        }
+        if (!q.prepare()) {
+            bad = true;
+        } else {
+            double dd = 1.0;
+            dd *= q.d[0];
+            dd *= q.d[1];
+            double af = c_tab.rp_bnd[pid] ? 0.5 : 1.;
+            af *= fabs(dd);
+            s.af[lane] = af;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double e3[3] = {0., 0., 0.};
+                e3[j] = 1.;
+                q.apply_Q(e3);
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    if (j < 2) s.U[lane][r][j] = e3[r];
+                    else s.nrm[lane][r] = e3[r];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) s.vb[lane][j][cc] = vb[j][cc];
+        }
+    }
+    if (lane < 40) {
+        const int i = lane / 4, k = lane % 4;
+        double T[3][3];
+        // the lane's own copy of T (lane 0 may still be writing s.T)
+        qq_trafo(X, T);
+        double g[3];
+        qq_grad_at_vertex(T, i, k, g);
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) s.G[i][k][cc] = g[cc];
+    }
+    if (__syncthreads_or(bad)) {
+        if (lane == 0) atomicOr(a.err, 4);
+        return;
+    }
+    // ---- one quadrature point per lane ----
+    const int nq = 7 * nf;
+    if (lane < nq) {
+        const int f = lane / 7, qi = lane - 7 * (lane / 7);
+        double nb[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            double sv = s.vb[f][0][cc] * c_tab.q5n[qi][0];
+            sv += s.vb[f][1][cc] * c_tab.q5n[qi][1];
+            sv += s.vb[f][2][cc] * c_tab.q5n[qi][2];
+            nb[cc] = sv;
+        }
+        s.w[lane] = s.af[f] * c_tab.q5w[qi];
+        double bxb[4], dh, dph[3][3], n[3];
+        if (!qq_base_point(s, a.P, nb, bxb, dh) || !qq_jacobian(s, bxb, dh, dph, n)) {
+            bad = true;
+        } else {
+            // |det| = sqrt(det Gram(dph U)) (ProjectedQuadDomain2DCL::assign, ifacetransp.cpp:850-892)
+            double DU[3][2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    double t2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) t2 += dph[r][i] * s.U[f][i][cc];
+                    DU[r][cc] = t2;
+                }
+            double g00 = 0.0, g10 = 0.0, g11 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g00 += DU[i][0] * DU[i][0];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g10 += DU[i][1] * DU[i][0];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) g11 += DU[i][1] * DU[i][1];
+            s.ad[lane] = sqrt(g00 * g11 - g10 * g10);
+            // W = dph^T + nl (n/(nl.n) - dph nl), nl the first facet's normal (gradient_trafo,
+            // ifacetransp.cpp:894-918)
+            double nl[3] = {s.nrm[0][0], s.nrm[0][1], s.nrm[0][2]};
+            double dnl[3], tv[3];
+            qq_mv3(dph, nl, dnl);
+            const double nln = nl[0] * n[0] + nl[1] * n[1] + nl[2] * n[2];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) tv[cc] = n[cc] / nln - dnl[cc];
+            SmallQR<3, 3> W;
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) W.at(r, cc) = dph[cc][r] + nl[r] * tv[cc];
+            if (!W.prepare()) {
+                bad = true;
+            } else {
+                double Wi[3][3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double e[3] = {0., 0., 0.};
+                    e[j] = 1.;
+                    W.solve(e);
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) Wi[r][j] = e[r];
+                }
+                const double nf3[3] = {s.nrm[f][0], s.nrm[f][1], s.nrm[f][2]};
+#pragma unroll
+                for (int i = 0; i < 10; ++i) {
+                    double g[3], tt[3], r[3];
+                    qq_p1v(s, i, nb, g);
+                    const double d = nf3[0] * g[0] + nf3[1] * g[1] + nf3[2] * g[2];
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) tt[cc] = g[cc] - d * nf3[cc];
+                    qq_mv3(Wi, tt, r);
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) s.Gq[lane][i][cc] = r[cc];
+                }
+                double h[10];
+                p2_basis(nb, h);
+#pragma unroll
+                for (int i = 0; i < 10; ++i) s.H[lane][i] = h[i];
+                double fv = 0.;
+                if (a.rhs_kind != DHIP_RHS_NONE) {
+                    double y[3];
+                    b2w(s.X, bxb, y);
+                    fv = rhs_fun(a.rhs_kind, a.rhs_p0, y);
+                }
+                s.f[lane] = fv;
+            }
+        }
+    }
+    if (__syncthreads_or(bad)) {
+        if (lane == 0) atomicOr(a.err, 4);
+        return;
+    }
+    // ---- contractions in the composite order ----
+    if (lane < 55) {
+        int pi = 0, pj = 0;
+        pair_of(lane, pi, pj);
+        double am = 0.0, al = 0.0;
+        for (int k = 0; k < nq; ++k) {
+            const double w = s.w[k], ad = s.ad[k];
+            am += w * ((ad * s.H[k][pj]) * s.H[k][pi]);
+            const double dq = s.Gq[k][pj][0] * s.Gq[k][pi][0] + s.Gq[k][pj][1] * s.Gq[k][pi][1] +
+                              s.Gq[k][pj][2] * s.Gq[k][pi][2];
+            al += w * (ad * dq);
+        }
+        const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
+        L[(int64_t)(LOC2_MASS + e1) * N + c] = am;
+        L[(int64_t)(LOC2_MASS + e2) * N + c] = am;
+        L[(int64_t)(LOC2_LB + e1) * N + c] = al;
+        L[(int64_t)(LOC2_LB + e2) * N + c] = al;
+        L[(int64_t)(LOC2_STAB + e1) * N + c] = 0.;
+        L[(int64_t)(LOC2_STAB + e2) * N + c] = 0.;
+    }
+    if (lane < 10) {
+        double rh = 0.0;
+        if (a.rhs_kind != DHIP_RHS_NONE)
+            for (int k = 0; k < nq; ++k) rh += s.w[k] * ((s.ad[k] * s.f[k]) * s.H[k][lane]);
+        L[(int64_t)(LOC2_RHS + lane) * N + c] = rh;
+    }
+    if (lane == 0) a.flags[c] = 1;
+}
+
+}  // namespace dhip

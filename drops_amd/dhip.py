@@ -41,7 +41,7 @@ EXPORTS = [
     "dhip_mtx_write", "dhip_mtx_format", "dhip_mtx_read", "dhip_mtx_parse", "dhip_csr_export", "dhip_csr_import",
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
     "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision", "dhip_spmv_t",
-    "dhip_quad_points", "dhip_rhs_values",
+    "dhip_quad_points", "dhip_rhs_values", "dhip_local_p2_quaqua", "dhip_assemble_p2_quaqua",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -51,6 +51,15 @@ class ChParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("epsilon", C.c_double), ("rho", C.c_double), ("sigma", C.c_double),
                 ("S", C.c_double), ("restart", C.c_int32), ("max_iter", C.c_int32), ("tol", C.c_double),
                 ("rel", C.c_int32)]
+
+
+class QuaQuaParams(C.Structure):
+    """dhip_quaqua_params_t: the QuaQuaMapperCL parameters (LevelsetMapper.Iter / Tol / ArmijoConstant,
+    damping steps); the defaults are the reference's parameter files'."""
+    _fields_ = [("maxiter", C.c_int32), ("tol", C.c_double), ("armijo_c", C.c_double), ("max_damping", C.c_int32)]
+
+    def __init__(self, maxiter=20, tol=1e-7, armijo_c=1e-2, max_damping=8):
+        super().__init__(maxiter, tol, armijo_c, max_damping)
 
 
 class BdfParams(C.Structure):
@@ -123,6 +132,9 @@ def load(path=LIB_PATH):
     L.dhip_number_p2if.argtypes = [P, C.c_double, I64]
     L.dhip_build_p2.argtypes = [P, D, I64]
     L.dhip_assemble_p2.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
+    L.dhip_local_p2_quaqua.argtypes = [P, C.POINTER(QuaQuaParams), C.c_int32, D]
+    L.dhip_assemble_p2_quaqua.argtypes = [P, C.c_int32, C.POINTER(QuaQuaParams), C.c_int32, D, C.c_double, D, I64,
+                                          I64, I64]
     L.dhip_assemble_p1_band.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
     L.dhip_get_local_p2.argtypes = [P, D, C.POINTER(C.c_uint8)]
     L.dhip_get_dof_nodes.argtypes = [P, I64]
@@ -737,6 +749,20 @@ class Context:
                                                 C.byref(b), C.byref(c)))
         self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 1
         return a.value, b.value, c.value
+
+    def assemble_p2_quaqua(self, zero_policy=ZERO_THROW, params=None, rhs_kind=RHS_NONE, rhs_param=None,
+                           omit_bound=-1.0, lc=(1.0, 1.0, 0.0)):
+        """P2 mass (slot 0), Laplace-Beltrami (slot 1) and load on the QuaQua-mapped surface, their combination
+        lc[0] M + lc[1] A: (ncut, ndof, nnz); results through csr()."""
+        rp = None if rhs_param is None else np.asarray(rhs_param, np.float64)
+        lcv = np.asarray(lc, np.float64)
+        P = params if params is not None else QuaQuaParams()
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        self.check(self.L.dhip_assemble_p2_quaqua(self.h, zero_policy, C.byref(P), rhs_kind, _p(rp, C.c_double),
+                                                  omit_bound, _p(lcv, C.c_double), C.byref(a), C.byref(b),
+                                                  C.byref(c)))
+        self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 2
+        return self.ncut, self.ndof, self.nnz
 
     def assemble_p2(self, zero_policy=ZERO_THROW, coef=(1.0, 1.0, 1.0), rhs_kind=RHS_NONE, rhs_param=None,
                     omit_bound=-1.0, lc=(1.0, 1.0, 1.0)):

@@ -202,6 +202,37 @@ int dhip_assemble_p2(dhip_ctx_t *ctx, int32_t zero_policy, const double coef[3],
                      const double *rhs_param, double omit_bound, const double lc[3],
                      int64_t *ncut, int64_t *ndof, int64_t *nnz);
 
+/* ---- P2 forms on the QuaQua-mapped surface (higher-order geometry) ----
+ * InterfaceCommonDataP2CL with QuaQuaMapperCL (surfactant/ifacetransp.h:200-276,
+ * levelset/levelsetmapper.h:52-140, levelset/levelsetmapper.cpp:551-638) and
+ * LocalMassP2CL / LocalLaplaceBeltramiP2CL / LocalVectorP2CL (ifacetransp.h:2302-2403), as
+ * surfactant/surfactant.cpp:1596-1640 assembles them: each cut tet's lattice-1 patch (the linear
+ * interpolant of the level set at its vertices), composite Quad5 points mapped by damped Newton to
+ * base points on the zero level of the P2 level set along the recovered gradient
+ * (averaging_P2_gradient_recovery, num/gradient_recovery.cpp:82-89), |det| of the map's Jacobian
+ * and its gradient transformation W^-1 (gradient_trafo, ifacetransp.cpp:894-918).  The load value is
+ * f at the base point.  The Laplace-Beltrami form carries no coefficient (the reference's
+ * LocalLaplaceBeltramiP2CL ignores its D); the stabilisation slot is zero.  As in the reference the
+ * base-point search stays in the starting tetra (locate_new_point, levelsetmapper.cpp:463-490, never
+ * searches the neighbourhood it is given), so the tet's polynomials are extrapolated. */
+typedef struct {
+    int32_t maxiter;     /* LevelsetMapper.Iter: 20 in the reference's parameter files */
+    double tol;          /* LevelsetMapper.Tol: 1e-7 */
+    double armijo_c;     /* LevelsetMapper.ArmijoConstant: 1e-2 */
+    int32_t max_damping; /* damping steps per Newton step: 8 */
+} dhip_quaqua_params_t;
+/* The mapped mass, Laplace-Beltrami and load of every cut tet into the local slots the P2 CSR build
+ * reads (after dhip_number_p2if; dhip_build_p2 follows).  params NULL: the values above.  rhs_kind
+ * DHIP_RHS_NONE, _3XY_MINUS_X3 or _CONST (values at the planar points do not apply). */
+int dhip_local_p2_quaqua(dhip_ctx_t *ctx, const dhip_quaqua_params_t *params, int32_t rhs_kind,
+                         const double *rhs_param);
+/* classify + local_p2 (for the numbering) + number_p2if + local_p2_quaqua + build_p2 in one call:
+ * the CSR holds mass (slot 0), Laplace-Beltrami (slot 1), zero stabilisation (slot 2) and their
+ * combination with lc; a row's entries come from the tets with a lattice-1 patch only. */
+int dhip_assemble_p2_quaqua(dhip_ctx_t *ctx, int32_t zero_policy, const dhip_quaqua_params_t *params,
+                            int32_t rhs_kind, const double *rhs_param, double omit_bound, const double lc[3],
+                            int64_t *ncut, int64_t *ndof, int64_t *nnz);
+
 /* ---- results of the assembly (host copies; any pointer may be NULL) ---- */
 int dhip_get_cut_tets(dhip_ctx_t *ctx, int64_t *tets);
 /* per cut tet: 16 mass, 16 LB, 16 stab, 4 rhs, 4 phi_i^2 integrals, area (57 doubles),

@@ -1,0 +1,109 @@
+"""GPU parity of the P2 forms on the QuaQua-mapped surface (dhip_assemble_p2_quaqua, dhip_local_p2_quaqua)
+against the oracle, through the C ABI.
+
+Bar: the local 10x10 mass and Laplace-Beltrami forms and the load vector, the CSR pattern (only tets with a
+lattice-1 patch add couplings) and every assembled entry bit-exact against the oracle (both sides run the same
+Newton iterations, QR factorisations and sums in the same order without FMA contraction) where the tet's
+cube root of |det| is exact (mesh width a power of two); elsewhere within 1e-12 relative, since cbrt may round
+differently there and it only enters the Newton damping length.  At 128^3 (the surface-solve size): the mapped
+sphere's mass sums to 4 pi (to what the Newton tolerance allows), the Laplace-Beltrami rows to zero, and
+the load of f = 1 is M 1.  Calls out of
+order, planar-point values and bad parameters are refused."""
+import numpy as np
+import pytest
+
+import cases
+import oracle as O
+from drops_amd import dhip as D
+
+pytestmark = pytest.mark.gpu
+
+LC = (1.0, 0.5, 0.0)
+
+
+def run(ctx, mesh, phi, rhs=D.RHS_3XY_MINUS_X3, rhs_param=None, params=None):
+    print(f"[gpu quaqua] mesh n={mesh.n}", flush=True)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    return ctx.assemble_p2_quaqua(params=params, rhs_kind=rhs, rhs_param=rhs_param, lc=LC)
+
+
+@pytest.mark.parametrize("case,n,exact", [(cases.c1_sphere, 16, True), (cases.torus_case, 24, False),
+                                          (cases.c1_sphere, 32, True)])
+def test_mapped_forms_match_oracle(ctx, case, n, exact):
+    mesh, phi = case(n)
+    ncut, ndof, nnz = run(ctx, mesh, phi)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    grad = O.recovered_gradient(om, phi)
+    cut = ctx.cut_tets()
+    loc, flags = ctx.local_p2()
+    worst = 0.0
+    for c in range(0, ncut, max(1, ncut // 400)):
+        r = O.tet_local_p2qq(om, phi, grad, int(cut[c]), rhs_kind=O.RHS_3XY_MINUS_X3)
+        assert bool(flags[c] & 1) == (r.nfacet > 0)
+        ref = np.concatenate([np.ctypeslib.as_array(r.mass), np.ctypeslib.as_array(r.lb),
+                              np.zeros(100), np.ctypeslib.as_array(r.rhs)])
+        got = loc[c, :310]
+        if exact:
+            assert np.array_equal(got, ref), f"tet {cut[c]}"
+        else:
+            worst = max(worst, float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)))
+    assert worst <= 1e-12
+    dof, nd = O.number_p2if(om, phi)
+    assert ndof == nd
+    ref = O.assemble_p2qq(om, phi, grad, dof, nd, rhs_kind=O.RHS_3XY_MINUS_X3)
+    g = ctx.csr()
+    assert nnz == ref["colind"].size
+    assert np.array_equal(g["rowptr"].astype(np.int64), ref["rowptr"])
+    assert np.array_equal(g["colind"], ref["colind"])
+    for k in ("mass", "lb", "rhs"):
+        if exact:
+            assert np.array_equal(g[k], ref[k]), k
+        else:
+            assert np.abs(g[k] - ref[k]).max() <= 1e-12 * np.abs(ref[k]).max(), k
+    assert not g["stab"].any()
+    if exact:
+        assert np.array_equal(g["comb"], 1.0 * (LC[0] * ref["mass"] + LC[1] * ref["lb"]) + LC[2] * 0.0)
+
+
+@pytest.mark.parametrize("tol,bar", [(1e-7, 5e-6), (1e-12, 1e-10)])
+def test_mapped_sphere_at_surface_solve_size(ctx, tol, bar):
+    # The Newton stops at |F| < tol, |F| >= |phi_h(y)|: with the reference's 1e-7 the base points sit up to
+    # ~1e-7 off the sphere, which bounds the area's accuracy on fine meshes (96^3: 1.4e-7, against 2.4e-11
+    # at tol 1e-12 on the oracle).
+    mesh, phi = cases.c1_sphere(128)
+    ncut, ndof, nnz = run(ctx, mesh, phi, rhs=D.RHS_CONST, rhs_param=[1.0], params=D.QuaQuaParams(tol=tol))
+    g = ctx.csr()
+    rp = g["rowptr"].astype(np.int64)
+    rows = np.repeat(np.arange(ndof), np.diff(rp))
+    area = g["mass"].sum()
+    assert abs(area - 4 * np.pi) < bar, area - 4 * np.pi
+    scale = np.abs(g["lb"]).max()
+    lbsum = np.bincount(rows, weights=g["lb"], minlength=ndof)
+    assert np.abs(lbsum).max() <= 1e-12 * scale
+    msum = np.bincount(rows, weights=g["mass"], minlength=ndof)
+    assert np.abs(g["rhs"] - msum).max() <= 1e-15
+    # symmetric pattern and values
+    import scipy.sparse as sp
+    for k in ("mass", "lb"):
+        A = sp.csr_matrix((g[k], g["colind"], rp), shape=(ndof, ndof))
+        assert abs(A - A.T).max() <= 1e-15 * abs(A).max(), k
+
+
+def test_mapped_calls_refused(ctx):
+    mesh, phi = cases.c1_sphere(12)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    ctx.classify()
+    with pytest.raises(D.DhipError) as ei:
+        ctx.check(ctx.L.dhip_local_p2_quaqua(ctx.h, None, D.RHS_NONE, None))
+    assert ei.value.code == D.E_STATE
+    with pytest.raises(D.DhipError) as ei:
+        ctx.assemble_p2_quaqua(rhs_kind=D.RHS_VALUES)
+    assert ei.value.code == D.E_ARG
+    with pytest.raises(D.DhipError) as ei:
+        ctx.assemble_p2_quaqua(params=D.QuaQuaParams(maxiter=-1))
+    assert ei.value.code == D.E_ARG
+    # a fresh, valid call after the refusals
+    ncut, ndof, nnz = ctx.assemble_p2_quaqua()
+    assert ndof > 0 and nnz > 0
