@@ -950,6 +950,25 @@ static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t r
     return DHIP_OK;
 }
 
+int dhip_p2_recovered_gradient(dhip_ctx_t *c, double *grad, int32_t where) {
+    if (!c || !grad) return DHIP_E_ARG;
+    if (!c->has_mesh || !c->phi) return set_error(c, DHIP_E_STATE, "mesh and level set required");
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t nn = (2LL * c->mesh.n[0] + 1) * (2LL * c->mesh.n[1] + 1) * (2LL * c->mesh.n[2] + 1);
+    DevBuf tmp;
+    double *dst = grad;
+    if (where != DHIP_DEVICE) {
+        DHIP_HIPCHK(c, tmp.ensure(sizeof(double) * 3 * nn));
+        dst = tmp.as<double>();
+    }
+    RecGradArgs ra{c->mesh, c->phi, nn, dst};
+    DHIP_LAUNCH(c, "recovered_gradient", k_recovered_gradient, blocks_for(nn, 128), 128, 0, ra);
+    if (where != DHIP_DEVICE)
+        DHIP_HIPCHK(c, hipMemcpyAsync(grad, dst, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost, c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
 int dhip_local_p2_quaqua(dhip_ctx_t *c, const dhip_quaqua_params_t *params, int32_t rhs_kind,
                          const double *rhs_param) {
     if (!c) return DHIP_E_ARG;

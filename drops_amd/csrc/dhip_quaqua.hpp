@@ -436,6 +436,25 @@ __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int
     for (int c = 0; c < 3; ++c) out[c] = avg[c];
 }
 
+// averaging_P2_gradient_recovery over every half-grid node: out[3 j + c]
+struct RecGradArgs {
+    MeshP m;
+    const double *phi;
+    int64_t nnodes;
+    double *out;
+};
+
+__global__ void __launch_bounds__(128) k_recovered_gradient(RecGradArgs a) {
+    const int64_t j = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (j >= a.nnodes) return;
+    const int64_t h1 = 2 * (int64_t)a.m.n[0] + 1, h2 = 2 * (int64_t)a.m.n[1] + 1;
+    const int64_t J[3] = {j % h1, (j / h1) % h2, j / (h1 * h2)};
+    double g[3];
+    qq_recovered(a.m, a.phi, J, g);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.out[3 * j + c] = g[c];
+}
+
 __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
     __shared__ QQShared s;
     const int64_t c = blockIdx.x;

@@ -42,6 +42,7 @@ EXPORTS = [
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
     "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision", "dhip_spmv_t",
     "dhip_quad_points", "dhip_rhs_values", "dhip_local_p2_quaqua", "dhip_assemble_p2_quaqua",
+    "dhip_p2_recovered_gradient",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -133,6 +134,7 @@ def load(path=LIB_PATH):
     L.dhip_build_p2.argtypes = [P, D, I64]
     L.dhip_assemble_p2.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
     L.dhip_local_p2_quaqua.argtypes = [P, C.POINTER(QuaQuaParams), C.c_int32, D]
+    L.dhip_p2_recovered_gradient.argtypes = [P, D, C.c_int32]
     L.dhip_assemble_p2_quaqua.argtypes = [P, C.c_int32, C.POINTER(QuaQuaParams), C.c_int32, D, C.c_double, D, I64,
                                           I64, I64]
     L.dhip_assemble_p1_band.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
@@ -749,6 +751,14 @@ class Context:
                                                 C.byref(b), C.byref(c)))
         self.ncut, self.ndof, self.nnz, self.space = a.value, b.value, c.value, 1
         return a.value, b.value, c.value
+
+    def recovered_gradient(self):
+        """averaging_P2_gradient_recovery of the current level set: shape (half-grid nodes, 3)"""
+        n = self._mesh.n
+        nn = (2 * n[0] + 1) * (2 * n[1] + 1) * (2 * n[2] + 1)
+        g = np.zeros(3 * nn)
+        self.check(self.L.dhip_p2_recovered_gradient(self.h, _p(g, C.c_double), HOST))
+        return g.reshape(nn, 3)
 
     def assemble_p2_quaqua(self, zero_policy=ZERO_THROW, params=None, rhs_kind=RHS_NONE, rhs_param=None,
                            omit_bound=-1.0, lc=(1.0, 1.0, 0.0)):

@@ -40,6 +40,8 @@
 #include <map>
 #include <memory>
 #include <stdexcept>
+#include <unordered_map>
+#include <unordered_set>
 #include <string>
 #include <unordered_map>
 #include <utility>
@@ -799,7 +801,7 @@ inline const ColorClassesCL &MultiGridCL::GetColorClasses(int, const void *) con
     return *colors_;
 }
 
-enum FiniteElementT { P1IF_FE = 1, P2IF_FE = 2 };
+enum FiniteElementT { P1IF_FE = 1, P2IF_FE = 2, vecP2_FE = 3 };
 
 // P2 level set on the half-spaced grid (LevelsetP2CL::Phi)
 class VecDescCL;
@@ -856,6 +858,7 @@ class IdxDescCL {
     // for width > 0 (P1IF_FE), on the GPU
     size_t CreateNumbering(unsigned, const MultiGridCL &mg, const VecDescCL *lsetp, const BndDataCL * = 0,
                            double width = 0.) {
+        if (fe_ == vecP2_FE) return CreateNumbering(0, mg);
         if (!lsetp) throw DROPSErrCL("IdxDescCL::CreateNumbering: no level set function for interface numbering given");
         DeviceCL &dev = DeviceCL::instance();
         const auto &B = mg.brick();
@@ -887,6 +890,16 @@ class IdxDescCL {
         if (num_ && fe_ == P1IF_FE) dev.check(dhip_get_dof_vertices(dev.ctx(), vertex_of_dof_.data()));
         if (num_) dev.check(dhip_get_dof_nodes(dev.ctx(), node_of_dof_.data()));
         return (size_t)ncut;
+    }
+    // vecP2_FE on the whole mesh (the recovered level-set gradient's space): three unknowns per P2 node,
+    // the nodes in the level set's half-grid order
+    size_t CreateNumbering(unsigned, const MultiGridCL &mg) {
+        if (fe_ != vecP2_FE) throw DROPSErrCL("IdxDescCL::CreateNumbering: the interface spaces need the level set");
+        const auto &B = mg.brick();
+        num_ = 3 * (size_t)(2 * B.n[0] + 1) * (size_t)(2 * B.n[1] + 1) * (size_t)(2 * B.n[2] + 1);
+        vertex_of_dof_.clear();
+        node_of_dof_.clear();
+        return 0;
     }
     // DistributeDOFs(level, mg, levelSet, dist) (misc/problem.cpp:283-336) for P1IF_FE: every vertex of
     // every tet with distance() <= dist gets the next number on first encounter in tet order -- the
@@ -974,11 +987,59 @@ class InterfaceCommonDataP1CL;
 class InterfaceCommonDataP2CL;
 class NarrowBandCommonDataP1CL;
 
+// PrincipalLatticeCL::instance (geom/principallattice.h): the lattice a common-data object samples the
+// level set on.  The device passes fix it: order 2 for the planar patches, order 1 for the mapped one.
+class PrincipalLatticeCL {
+    unsigned n_;
+    explicit PrincipalLatticeCL(unsigned n) : n_(n) {}
+
+  public:
+    static const PrincipalLatticeCL &instance(unsigned n) {
+        static const PrincipalLatticeCL l1(1), l2(2);
+        if (n == 1) return l1;
+        if (n == 2) return l2;
+        throw DROPSErrCL("PrincipalLatticeCL: the device passes sample on lattices of order 1 and 2");
+    }
+    unsigned num_intervals() const { return n_; }
+};
+
+// levelset/levelsetmapper.h:38-40
+typedef std::unordered_set<const TetraCL *> TetraSetT;
+typedef std::unordered_map<const TetraCL *, TetraSetT> TetraToTetrasT;
+
+// compute_tetra_neighborhoods (levelset/levelsetmapper.cpp:655-690).  The reference's base-point
+// search never reads these sets: locate_new_point (:463-490) searches them only while the tetra it
+// was given is null, which it never is, so a point that leaves its tetra keeps it.  Nothing to compute.
+inline void compute_tetra_neighborhoods(const MultiGridCL &, const VecDescCL &, const BndDataCL &,
+                                        const PrincipalLatticeCL &, TetraToTetrasT &) {}
+
+// QuaQuaMapperCL (levelset/levelsetmapper.h:52-140): the parameters of the device's base-point Newton
+// (base_point_newton with newton_solve, levelsetmapper.cpp:551-566, num/newton.h:64-115).  Only the
+// Newton method is implemented (LevelsetMapper.Method "Newton" in the reference's parameter files);
+// the fixed-point method with line search is refused.
+class QuaQuaMapperCL {
+  public:
+    dhip_quaqua_params_t params;
+    const VecDescCL *ls, *ls_grad_rec;
+    QuaQuaMapperCL(const MultiGridCL &, VecDescCL &lsarg, const VecDescCL &ls_grad_recarg,
+                   TetraToTetrasT * = nullptr, int maxiter = 100, double tol = 1e-7, bool use_line_search = true,
+                   double armijo_c = 1e-4, unsigned max_damping_steps = 8)
+        : ls(&lsarg), ls_grad_rec(&ls_grad_recarg) {
+        if (use_line_search)
+            throw DROPSErrCL("QuaQuaMapperCL: only the Newton method (LevelsetMapper.Method \"Newton\") is implemented");
+        params.maxiter = maxiter;
+        params.tol = tol;
+        params.armijo_c = armijo_c;
+        params.max_damping = (int32_t)max_damping_steps;
+    }
+};
+
 namespace detail {
 // What the device-assembled accumulators of a tuple ask of one libdhip pass.
 struct DevicePlan {
     InterfaceCommonDataP1CL *common = nullptr;
     InterfaceCommonDataP2CL *common2 = nullptr;
+    const dhip_quaqua_params_t *quaqua = nullptr;   // the P2 pass on the mapped surface
     const NarrowBandCommonDataP1CL *band = nullptr;
     struct MatEntry { MatDescCL *mat; int op; double coef; };
     std::vector<MatEntry> mats;
@@ -1112,11 +1173,22 @@ struct LocalNormalLaplaceBulkP1CL {
     double coef;
     LocalNormalLaplaceBulkP1CL(double D, double = 0., std::nullptr_t = nullptr, double = 0.) : coef(D) {}
 };
-// P2 interface space (surfactant/ifacetransp.h:200-276): the same patch data, P2 basis.  On the
-// host it offers what the P1 class offers; the patch area is the sum of the tet's P2 mass matrix.
+// P2 interface space (surfactant/ifacetransp.h:200-276).  Constructed as the reference does, with a
+// QuaQuaMapperCL and the order-1 lattice: the forms on the mapped surface (each tet's lattice-1 patch
+// mapped to the zero level of the P2 level set; dhip_assemble_p2_quaqua).  Constructed from the level
+// set alone: the forms on the planar order-2 patch of the P1 space (dhip_assemble_p2).  On the host it
+// offers what the P1 class offers; the patch area is the sum of the tet's P2 mass matrix.
 class InterfaceCommonDataP2CL : public InterfaceCommonDataP1CL {
   public:
+    const QuaQuaMapperCL *quaqua = nullptr;
     InterfaceCommonDataP2CL(const VecDescCL &ls_arg, const BndDataCL &b) : InterfaceCommonDataP1CL(ls_arg, b) {}
+    InterfaceCommonDataP2CL(const VecDescCL &ls_arg, const BndDataCL &b, const QuaQuaMapperCL &q,
+                            const PrincipalLatticeCL &lat)
+        : InterfaceCommonDataP1CL(ls_arg, b), quaqua(&q) {
+        if (lat.num_intervals() != 1)
+            throw DROPSErrCL("InterfaceCommonDataP2CL: the mapped surface is built on the order-1 lattice");
+        if (q.ls != &ls_arg) throw DROPSErrCL("InterfaceCommonDataP2CL: the mapper belongs to another level set");
+    }
     TetraAccumulatorCL *clone(int clone_id) override {
         auto *c = new InterfaceCommonDataP2CL(*this);
         c->clones_ = nullptr;
@@ -1125,9 +1197,16 @@ class InterfaceCommonDataP2CL : public InterfaceCommonDataP1CL {
     }
     bool device_plan(detail::DevicePlan &p) override {
         p.common2 = this;
+        p.quaqua = quaqua ? &quaqua->params : nullptr;
         return true;
     }
 };
+// averaging_P2_gradient_recovery (num/gradient_recovery.cpp:82-89) of the P2 level set f into grad, three
+// values per P2 node in the level set's node order (dhip_p2_recovered_gradient)
+inline void averaging_P2_gradient_recovery(const MultiGridCL &mg, const VecDescCL &f, const BndDataCL &fbnd,
+                                           VecDescCL &grad);
+// Mapped-surface forms: LocalMassP2CL and LocalLaplaceBeltramiP2CL (ifacetransp.h:2319-2403) carry no
+// coefficient there (the reference's Laplace-Beltrami class ignores its D); the planar forms scale by it.
 struct LocalMassP2CL {                // surfactant/ifacetransp.h:2379-2403 (LocalAssembler::M_P2P2, :1579)
     static const int device_op = DHIP_OP_MASS;
     double coef;
@@ -1344,6 +1423,8 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
         const double param = plan.rhs_param;
         const int rk = plan.rhs ? plan.rhs_kind : DHIP_RHS_NONE;
         const double lc[3] = {1., 1., 1.};
+        if (p2 && plan.quaqua && rk == DHIP_RHS_VALUES)
+            throw DROPSErrCL("accumulate: the mapped surface's load is f at the base points; give a DeviceFunctionCL");
         if (rk == DHIP_RHS_VALUES) {
             // f at the quadrature points of the patches of the cut tets (the points LocalVectorP1CL /
             // LocalVectorP2CL evaluate it at), evaluated here, summed by the device pass
@@ -1357,7 +1438,10 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
             for (int64_t q = 0; q < np; ++q) fv[(size_t)q] = plan.rhs_fun(&xyz[(size_t)(3 * q)], plan.rhs_time);
             dev.check(dhip_rhs_values(dev.ctx(), fv.data(), np, DHIP_HOST));
         }
-        if (p2)
+        if (p2 && plan.quaqua)
+            dev.check(dhip_assemble_p2_quaqua(dev.ctx(), DHIP_ZERO_THROW, plan.quaqua, rk, &param, -1.0, lc, &ncut,
+                                              &nd, &nnz));
+        else if (p2)
             dev.check(dhip_assemble_p2(dev.ctx(), DHIP_ZERO_THROW, coef, rk, &param, -1.0, lc, &ncut, &nd, &nnz));
         else if (plan.band)
             // SurfactantNarrowBandStblP1CL::Update's tuple: the surface forms on the cut tets, the
@@ -1419,6 +1503,20 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
         t(mg, mg.GetColorClasses());
     else
         t(mg, 0, mg.GetNumTriangTetra());
+}
+
+inline void averaging_P2_gradient_recovery(const MultiGridCL &mg, const VecDescCL &f, const BndDataCL &,
+                                           VecDescCL &grad) {
+    DeviceCL &dev = DeviceCL::instance();
+    const auto &B = mg.brick();
+    const double e[9] = {B.e[0][0], B.e[0][1], B.e[0][2], B.e[1][0], B.e[1][1], B.e[1][2],
+                         B.e[2][0], B.e[2][1], B.e[2][2]};
+    const int32_t n[3] = {B.n[0], B.n[1], B.n[2]};
+    dev.check(dhip_mesh_kuhn(dev.ctx(), B.orig, e, n));
+    dev.check(dhip_levelset_p2(dev.ctx(), f.Data.raw(), (int64_t)f.Data.size(), DHIP_HOST));
+    const size_t nn = (size_t)(2 * n[0] + 1) * (size_t)(2 * n[1] + 1) * (size_t)(2 * n[2] + 1);
+    grad.Data.resize(3 * nn);
+    dev.check(dhip_p2_recovered_gradient(dev.ctx(), grad.Data.raw(), DHIP_HOST));
 }
 
 // ---------------------------------------------------------------------------

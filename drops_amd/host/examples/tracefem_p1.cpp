@@ -183,6 +183,60 @@ static int band(int n, double width, const std::string &out) {
     return 0;
 }
 
+// surfactant/surfactant.cpp:1590-1640: P2 mass, Laplace-Beltrami and load on the QuaQua-mapped sphere
+static int quaqua(int n, const std::string &out) {
+    const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
+    const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
+    const double e1[3] = {4, 0, 0}, e2[3] = {0, 4, 0}, e3[3] = {0, 0, 4};
+    MultiGridCL mg(BrickBuilderCL(o, e1, e2, e3, n, n, n));
+    LevelsetP2CL lset;
+    lset.Init(mg, [](const double *p, double) { return (p[0] * p[0] + p[1] * p[1] + p[2] * p[2]) - 1.0; });
+    BndDataCL lsbnd;
+    IdxDescCL ifacep2idx(P2IF_FE);
+    ifacep2idx.CreateNumbering(0, mg, &lset.Phi, &lsbnd);
+    std::printf("P2-NumUnknowns: %zu\n", ifacep2idx.NumUnknowns());
+    // Recover the gradient of the level set function
+    IdxDescCL vecp2idx(vecP2_FE);
+    vecp2idx.CreateNumbering(0, mg);
+    VecDescCL lsgradrec(&vecp2idx);
+    averaging_P2_gradient_recovery(mg, lset.Phi, lsbnd, lsgradrec);
+    // Compute neighborhoods of the tetras at the interface
+    const PrincipalLatticeCL &lat = PrincipalLatticeCL::instance(1);
+    TetraToTetrasT tetra_neighborhoods;
+    compute_tetra_neighborhoods(mg, lset.Phi, lsbnd, lat, tetra_neighborhoods);
+    QuaQuaMapperCL quaqua(mg, lset.Phi, lsgradrec, &tetra_neighborhoods, 20, 1e-7, false, 1e-2);
+    TetraAccumulatorTupleCL accus;
+    InterfaceCommonDataP2CL cdatap2(lset.Phi, lsbnd, quaqua, lat);
+    accus.push_back(&cdatap2);
+    MatDescCL Mp2(&ifacep2idx, &ifacep2idx), Ap2(&ifacep2idx, &ifacep2idx);
+    InterfaceMatrixAccuCL<LocalMassP2CL, InterfaceCommonDataP2CL> accuMp2(&Mp2, LocalMassP2CL(), cdatap2, "Mp2");
+    accus.push_back(&accuMp2);
+    InterfaceMatrixAccuCL<LocalLaplaceBeltramiP2CL, InterfaceCommonDataP2CL> accuAp2(&Ap2, LocalLaplaceBeltramiP2CL(1.),
+                                                                                    cdatap2, "Ap2");
+    accus.push_back(&accuAp2);
+    VecDescCL bp2(&ifacep2idx);
+    DeviceFunctionCL f;
+    f.kind = DHIP_RHS_3XY_MINUS_X3;
+    InterfaceVectorAccuCL<LocalVectorP2CL, InterfaceCommonDataP2CL> acculoadp2(&bp2, LocalVectorP2CL(f), cdatap2);
+    accus.push_back(&acculoadp2);
+    accumulate(accus, mg, 0, &lsbnd);
+    double area = 0.;
+    for (size_t k = 0; k < Mp2.Data.num_nonzeros(); ++k) area += Mp2.Data.raw_val()[k];
+    std::printf("nnz: %zu\nmapped area - 4 pi: %.6e\n", Mp2.Data.num_nonzeros(), area - 4.0 * std::acos(-1.0));
+    if (!out.empty()) {
+        std::vector<int64_t> rp(Mp2.Data.num_rows() + 1), ci(Mp2.Data.num_nonzeros());
+        for (size_t i = 0; i <= Mp2.Data.num_rows(); ++i) rp[i] = (int64_t)Mp2.Data.raw_row()[i];
+        for (size_t k = 0; k < Mp2.Data.num_nonzeros(); ++k) ci[k] = (int64_t)Mp2.Data.raw_col()[k];
+        write_bin(out + "/rowptr.bin", rp.data(), rp.size() * 8);
+        write_bin(out + "/colind.bin", ci.data(), ci.size() * 8);
+        write_bin(out + "/mass.bin", Mp2.Data.raw_val(), Mp2.Data.num_nonzeros() * 8);
+        write_bin(out + "/lb.bin", Ap2.Data.raw_val(), Ap2.Data.num_nonzeros() * 8);
+        write_bin(out + "/rhs.bin", bp2.Data.raw(), bp2.Data.size() * 8);
+        write_bin(out + "/grad.bin", lsgradrec.Data.raw(), lsgradrec.Data.size() * 8);
+    }
+    return 0;
+}
+
 static int p2(int n, const std::string &out) {
     const double h = 4.0 / n, sn = 1e-3 * h / std::sqrt(14.0);
     const double o[3] = {-2.0 - sn, -2.0 - 2 * sn, -2.0 - 3 * sn};
@@ -559,6 +613,7 @@ int main(int argc, char **argv) {
         if (argc >= 2 && !std::strcmp(argv[1], "sphere20")) return sphere20();
         if (argc >= 3 && !std::strcmp(argv[1], "c1")) return c1(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
         if (argc >= 3 && !std::strcmp(argv[1], "p2")) return p2(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
+        if (argc >= 3 && !std::strcmp(argv[1], "quaqua")) return quaqua(std::atoi(argv[2]), argc >= 4 ? argv[3] : "");
         if (argc >= 4 && !std::strcmp(argv[1], "accus")) {
             if (!std::strcmp(argv[3], "p2"))
                 return accus_case<InterfaceCommonDataP2CL, LocalMassP2CL, LocalLaplaceBeltramiP2CL>(std::atoi(argv[2]),

@@ -221,6 +221,11 @@ typedef struct {
     double armijo_c;     /* LevelsetMapper.ArmijoConstant: 1e-2 */
     int32_t max_damping; /* damping steps per Newton step: 8 */
 } dhip_quaqua_params_t;
+/* averaging_P2_gradient_recovery (num/gradient_recovery.cpp:82-89): the gradient of the P2 level set
+ * at every half-grid node, averaged over the tets that hold the node as OswaldProjectionP2AccuCL does
+ * (num/oswald_projection.h:102-125, a running mean in tet order); grad: 3 doubles per node, nodes in
+ * the level set's order. */
+int dhip_p2_recovered_gradient(dhip_ctx_t *ctx, double *grad, int32_t where);
 /* The mapped mass, Laplace-Beltrami and load of every cut tet into the local slots the P2 CSR build
  * reads (after dhip_number_p2if; dhip_build_p2 follows).  params NULL: the values above.  rhs_kind
  * DHIP_RHS_NONE, _3XY_MINUS_X3 or _CONST (values at the planar points do not apply). */

@@ -107,3 +107,39 @@ def test_mapped_calls_refused(ctx):
     # a fresh, valid call after the refusals
     ncut, ndof, nnz = ctx.assemble_p2_quaqua()
     assert ndof > 0 and nnz > 0
+
+
+@pytest.mark.parametrize("case,n", [(cases.c1_sphere, 12), (cases.torus_case, 16)])
+def test_recovered_gradient_matches_oracle(ctx, case, n):
+    mesh, phi = case(n)
+    print(f"[gpu quaqua] recovered gradient n={n}", flush=True)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    g = ctx.recovered_gradient()
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    assert np.array_equal(g, O.recovered_gradient(om, phi))
+
+
+def test_host_mirror_mapped_forms(tmp_path):
+    """The reference's set-up (surfactant/surfactant.cpp:1596-1640: recovered gradient, neighbourhoods,
+    QuaQuaMapperCL, InterfaceCommonDataP2CL on lattice 1 and the three accumulators) through the C++ mirror
+    gives the oracle's matrices and load bit for bit."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "drops_amd", "lib", "tracefem_p1")
+    print("[gpu quaqua] host mirror", flush=True)
+    r = subprocess.run([exe, "quaqua", "16", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    rd = lambda name, dt: np.fromfile(tmp_path / name, dt)
+    mesh, phi = cases.c1_sphere(16)
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    grad = O.recovered_gradient(om, phi)
+    dof, nd = O.number_p2if(om, phi)
+    ref = O.assemble_p2qq(om, phi, grad, dof, nd, rhs_kind=O.RHS_3XY_MINUS_X3)
+    assert f"P2-NumUnknowns: {nd}" in r.stdout
+    assert np.array_equal(rd("rowptr.bin", np.int64), ref["rowptr"])
+    assert np.array_equal(rd("colind.bin", np.int64), ref["colind"].astype(np.int64))
+    for k in ("mass", "lb", "rhs"):
+        assert np.array_equal(rd(f"{k}.bin", np.float64), ref[k]), k
+    assert np.array_equal(rd("grad.bin", np.float64), grad.reshape(-1))
+    assert abs(float(r.stdout.split("mapped area - 4 pi:")[1].split()[0])) < 3e-6
