@@ -393,6 +393,53 @@ def band_leg(ctx, mesh, phi, coef, lc, width, reps=5, with_cpu=True):
     return out
 
 
+def quaqua_leg(ctx, mesh, phi, reps=5, with_cpu=True):
+    """P2 mass, Laplace-Beltrami and load on the QuaQua-mapped surface (surfactant.cpp:1596-1640's tuple)
+    of the same level set: best wall time of reps (classification, the planar pass for the numbering, the
+    mapped forms, CSR), per-kernel HIP-event averages, the mapped sphere's area error, and the oracle's
+    OpenMP build (recovered gradient, numbering, mapped assembly) timed once beside it."""
+    ts = []
+    for _ in range(reps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ncut, nd, nnz = ctx.assemble_p2_quaqua(rhs_kind=D.RHS_3XY_MINUS_X3)
+        ctx.synchronize()
+        ts.append(time.perf_counter() - t0)
+    ctx.profile_reset()
+    ctx.profile(True)
+    ctx.assemble_p2_quaqua(rhs_kind=D.RHS_3XY_MINUS_X3)
+    kern = {}
+    for k in ("classify", "compact", "hat_p2", "qualify_p2", "first_owner_p2", "dofs_p2", "node_gradient_qq",
+              "local_p2qq", "rowcount_p2", "rowfill_p2"):
+        ms, cnt = ctx.profile_get(k)
+        if cnt:
+            kern[k] = round(1e3 * ms / cnt, 2)
+    ctx.profile(False)
+    g = ctx.csr()
+    out = {"workload": f"P2 mass + Laplace-Beltrami + load (f = 3xy - x^3 at the base points) on the QuaQua-mapped "
+                       f"sphere, {mesh.n[0]}^3 Kuhn cells, Newton tol 1e-7, 20 iterations, Armijo 1e-2",
+           "cut_tets": ncut, "dofs": nd, "nnz": nnz, "s_per_assembly": min(ts), "kernels_us": kern,
+           "mapped_tets_per_s": ncut / (kern["local_p2qq"] * 1e-6) if kern.get("local_p2qq") else None,
+           "area_minus_4pi": float(g["mass"].sum() - 4 * np.pi)}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+        t0 = time.perf_counter()
+        grad = O.recovered_gradient(om, phi)
+        t1 = time.perf_counter()
+        dof, ndc = O.number_p2if(om, phi, kind="omp")
+        r = O.assemble_p2qq(om, phi, grad, dof, ndc, rhs_kind=O.RHS_3XY_MINUS_X3, kind="omp")
+        t2 = time.perf_counter()
+        out["cpu_port"] = {"s": t2 - t0, "recovered_gradient_s": t1 - t0, "numbering_and_mapped_assembly_s": t2 - t1,
+                           "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                           "note": "oracle: serial gradient recovery over every tet, OpenMP numbering and assembly"}
+        out["dofs_match"] = bool(ndc == nd)
+        out["max_rel_diff_vs_oracle"] = float(max(np.abs(g[k] - r[k]).max() / np.abs(r[k]).max()
+                                                  for k in ("mass", "lb", "rhs"))) if ndc == nd else None
+    return out
+
+
 # the FP64 PCG kernel each exchange runs (rocprofv3 names it so in its summaries)
 PCG_KERNEL = {0: "k_pcg64_sr", 1: "k_pcg64_sr", 2: "k_pcg64_tx", 3: "k_pcg64_pipe"}
 
@@ -511,6 +558,7 @@ def main():
                          "scales the FETCH_SIZE/WRITE_SIZE counters)")
     ap.add_argument("--no-ch", action="store_true", help="skip the Cahn-Hilliard trajectory leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems leg")
+    ap.add_argument("--no-quaqua", action="store_true", help="skip the mapped-surface P2 assembly leg")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--ch-cells", type=int, default=96)
     ap.add_argument("--ch-steps", type=int, default=200)
@@ -645,6 +693,8 @@ def main():
     log(f"[bench] formats {fm}")
     bd = band_leg(ctx, mesh, phi, coef, lc, 2.0 * mesh.h, with_cpu=not args.no_cpu)
     log(f"[bench] band {bd}")
+    qq = None if args.no_quaqua else quaqua_leg(ctx, mesh, phi, with_cpu=not args.no_cpu)
+    log(f"[bench] quaqua {qq}")
     ctx.close()
 
     ch = None if args.no_ch else ch_leg(args.ch_cells, args.ch_steps, with_cpu=not args.no_cpu, space=args.ch_space)
@@ -683,6 +733,7 @@ def main():
         "batch": bt,
         "formats": fm,
         "band": bd,
+        "quaqua": qq,
     }
     if cpu is not None:
         line["cpu_baseline"] = {"value": cpu["value"], "unit": "s per solve", "cores": cpu["threads"],

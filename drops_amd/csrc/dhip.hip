@@ -446,7 +446,7 @@ void dhip_destroy(dhip_ctx_t *c) {
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
                       &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
                       &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos,
-                      &c->gm_sell_val32};
+                      &c->gm_sell_val32, &c->qp_off, &c->qp_vals, &c->qq_first, &c->qq_gh};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -854,6 +854,23 @@ static int local_p2_impl(dhip_ctx *c, const double coef[3], int32_t rhs_kind, co
     return DHIP_OK;
 }
 
+// the numbering's integrals alone (k_hat_p2), for a pass whose forms come from the mapped surface
+static int hat_p2_impl(dhip_ctx *c) {
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede the P2 numbering");
+    if (c->band >= 0.) return set_error(c, DHIP_E_STATE, "the narrow band is a P1 assembly");
+    const int64_t N = c->ncut;
+    DHIP_HIPCHK(c, c->loc.ensure(sizeof(double) * LOC2_N * (N + 1)));
+    DHIP_HIPCHK(c, c->flags.ensure(N + 1));
+    LocalP2Args la{c->mesh, c->phi, c->cut.as<int64_t>(), N, 1., 1., 1., DHIP_RHS_NONE, 0.0, c->loc.as<double>(),
+                   c->flags.as<uint8_t>(), c->errflag.as<int32_t>(), nullptr, nullptr};
+    if (N > 0) DHIP_LAUNCH(c, "hat_p2", k_hat_p2, (unsigned)N, 64, 0, la);
+    int rc = check_err_flag(c);
+    if (rc) return rc;
+    c->stage = 2;
+    c->space = 2;
+    return DHIP_OK;
+}
+
 static int number_p2_impl(dhip_ctx *c, double omit_bound, int64_t *ndof) {
     if (c->stage < 2 || c->space != 2)
         return set_error(c, DHIP_E_STATE, "dhip_local_p2 must precede dhip_number_p2if");
@@ -940,10 +957,18 @@ static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t r
         P = QQParams{p->maxiter, p->tol, p->armijo_c, p->max_damping};
     }
     const int64_t N = c->ncut;
+    DHIP_HIPCHK(c, c->qq_first.ensure(sizeof(int32_t) * (10 * N + 1)));
+    DHIP_HIPCHK(c, c->qq_gh.ensure(sizeof(double) * (30 * N + 3)));
     DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+    if (N > 0) {
+        QQGradArgs ga{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->qq_first.as<int32_t>(),
+                      c->qq_gh.as<double>()};
+        DHIP_LAUNCH(c, "node_gradient_qq", k_qq_node_gradient, blocks_for(10 * N, 128), 128, 0, ga);
+    }
     LocalP2QQArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, rhs_kind, rhs_param ? rhs_param[0] : 0.0, P,
-                     c->loc.as<double>(), c->flags.as<uint8_t>(), c->errflag.as<int32_t>()};
-    if (N > 0) DHIP_LAUNCH(c, "local_p2qq", k_local_p2qq, (unsigned)N, 64, 0, la);
+                     c->loc.as<double>(), c->flags.as<uint8_t>(), c->errflag.as<int32_t>(),
+                     c->qq_first.as<int32_t>(), c->qq_gh.as<double>()};
+    if (N > 0) DHIP_LAUNCH(c, "local_p2qq", k_local_p2qq, (unsigned)((N + QQ_TPW - 1) / QQ_TPW), 64, 0, la);
     int rc = check_err_flag(c);
     if (rc) return rc;
     c->stage = 3;
@@ -983,9 +1008,8 @@ int dhip_assemble_p2_quaqua(dhip_ctx_t *c, int32_t zero_policy, const dhip_quaqu
                             int64_t *ncut, int64_t *ndof, int64_t *nnz) {
     if (!c) return DHIP_E_ARG;
     DHIP_HIPCHK(c, hipSetDevice(c->device));
-    static const double one3[3] = {1.0, 1.0, 1.0};
     int rc = classify_impl(c, zero_policy, ncut, false);
-    if (!rc) rc = local_p2_impl(c, one3, DHIP_RHS_NONE, nullptr);
+    if (!rc) rc = hat_p2_impl(c);
     if (!rc) rc = number_p2_impl(c, omit_bound, ndof);
     if (!rc) rc = local_p2qq_impl(c, params, rhs_kind, rhs_param);
     if (!rc) rc = build_p2_impl(c, lc, nnz);

@@ -92,7 +92,8 @@ struct P2Shared {
 // Patch facets of the tet (sub-tet lt, triangle tr, in the order of make_patch) into sh.vb / sh.af,
 // lanes 0..nf-1 one facet each.  Returns the facet count, or -1 when a sub-tet lies entirely in
 // the zero level set (the reference's "full body" error).  Uniform over the wave.
-__device__ inline int p2_patch_facets(P2Shared &sh, const double X[4][3], const double ls[10], int lane) {
+template <class Shared>
+__device__ inline int p2_patch_facets(Shared &sh, const double X[4][3], const double ls[10], int lane) {
     int ntri[8], nf = 0;
     bool zero_body = false;
 #pragma unroll
@@ -261,6 +262,66 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
         L[(LOC2_RHS + lane) * N + c] = rh;
         L[(LOC2_HAT + lane) * N + c] = hat;
     }
+    if (lane == 0) a.flags[c] = 1;
+}
+
+// The numbering's part of k_local_p2 alone: the composite-Quad5 integrals of the squared basis functions
+// (the hat slots) and the non-empty flag, summed as k_local_p2 sums them.  For passes whose forms come
+// from elsewhere (the mapped surface).
+struct HatShared {
+    double vb[P2_MAXF][3][4];
+    double af[P2_MAXF];
+    double h[P2_CHUNK][10];
+    double w[P2_CHUNK];
+};
+
+__global__ void __launch_bounds__(64) k_hat_p2(LocalP2Args a) {
+    __shared__ HatShared sh;
+    const int64_t c = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (c >= a.ncut) return;
+    double X[4][3], ls[10];
+    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+    const int nf = p2_patch_facets(sh, X, ls, lane);
+    if (nf < 0) {
+        if (lane == 0) atomicOr(a.err, 2);
+        return;
+    }
+    const int64_t N = a.ncut;
+    double *L = a.loc;
+    if (nf == 0) {
+        if (lane < 10) L[(LOC2_HAT + lane) * N + c] = 0.;
+        if (lane == 0) a.flags[c] = 0;
+        return;
+    }
+    __syncthreads();
+    double hat = 0.;
+    const int nq = 7 * nf;
+    for (int base = 0; base < nq; base += P2_CHUNK) {
+        const int q = base + lane;
+        if (q < nq) {
+            const int f = q / 7, i = q - 7 * (q / 7);
+            double nb[4];
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                double sv = sh.vb[f][0][cc] * c_tab.q5n[i][0];
+                sv += sh.vb[f][1][cc] * c_tab.q5n[i][1];
+                sv += sh.vb[f][2][cc] * c_tab.q5n[i][2];
+                nb[cc] = sv;
+            }
+            sh.w[lane] = sh.af[f] * c_tab.q5w[i];
+            double h[10];
+            p2_basis(nb, h);
+#pragma unroll
+            for (int d = 0; d < 10; ++d) sh.h[lane][d] = h[d];
+        }
+        __syncthreads();
+        const int cnt = nq - base < P2_CHUNK ? nq - base : P2_CHUNK;
+        if (lane < 10)
+            for (int k = 0; k < cnt; ++k) hat += sh.w[k] * (sh.h[k][lane] * sh.h[k][lane]);
+        __syncthreads();
+    }
+    if (lane < 10) L[(LOC2_HAT + lane) * N + c] = hat;
     if (lane == 0) a.flags[c] = 1;
 }
 

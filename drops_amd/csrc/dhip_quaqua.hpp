@@ -7,14 +7,17 @@
 // damped Newton on (x - y - s g_h(y), -phi_h(y)) = 0 with the recovered gradient g_h, and the forms
 // carry |det| of the map's Jacobian and, for the Laplace-Beltrami form, its gradient transformation
 // W^-1 (LocalMassP2CL, LocalLaplaceBeltramiP2CL, LocalVectorP2CL, ifacetransp.h:2302-2403).
-//   k_local_p2qq  one wave per cut tet:
-//     lanes 0-9       the recovered gradient at the tet's ten P2 nodes: the running mean over every
-//                     tet of the mesh that holds the node, in tet id order (averaging_P2_gradient_
-//                     recovery through OswaldProjectionP2AccuCL, num/gradient_recovery.cpp:62-89);
-//     lanes 0-39      the tet's P2 gradient fields T DH_iRef at its vertices into LDS;
-//     lanes < 7 nf    one quadrature point each: base point, Jacobian, |det|, W^-1, the ten mapped
+//   k_qq_node_gradient  one thread per (cut tet, P2 node): the recovered gradient of each node once, at
+//                     the first cut tet holding it: the running mean over every tet of the mesh that holds
+//                     the node, in tet id order (averaging_P2_gradient_recovery through
+//                     OswaldProjectionP2AccuCL, num/gradient_recovery.cpp:62-89);
+//   k_local_p2qq  four cut tets per wave, 16 lanes each:
+//     lanes j < 10    the recovered gradient at the tet's ten P2 nodes from the node kernel;
+//     all 16          the tet's P2 gradient fields T DH_iRef at its vertices into LDS;
+//     lanes j < 7 nf  one quadrature point each: base point, Jacobian, |det|, W^-1, the ten mapped
 //                     surface gradients and basis values, the load value at the base point;
-//     lanes < 55      pair (i >= j) of the mass and Laplace-Beltrami matrices, lanes 0-9 the load.
+//     all 64          the 4 x (55 pairs (i >= j) of the mass and Laplace-Beltrami matrices + 10 load
+//                     entries), each summed over the points in the composite order.
 //   Output: the mass, Laplace-Beltrami and load slots of the P2 local array (the stabilisation slot 0),
 //   flags 1 only for tets with a lattice-1 patch, so the P2 row builder adds exactly their couplings.
 // Every expression keeps the oracle's operation order (-ffp-contract=off): bit-identical except where
@@ -381,6 +384,8 @@ struct LocalP2QQArgs {
     double *loc;      // SoA [LOC2_N][ncut]
     uint8_t *flags;   // bit0: the tet adds to the mapped matrices
     int32_t *err;     // bit 1: full body in the zero level set, bit 2: singular matrix in the mapping
+    const int32_t *first;   // [10 ncut] the first cut slot holding each slot's P2 node (k_qq_node_gradient)
+    const double *gh;       // [10 ncut][3] the recovered gradient, at first slots
 };
 
 // recovered gradient at half-grid node J: running mean over the tets holding J in ascending id
@@ -436,6 +441,38 @@ __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int
     for (int c = 0; c < 3; ++c) out[c] = avg[c];
 }
 
+// The recovered gradient of the P2 nodes of the cut tets, once per node: slot s = 10 c + d (P2 dof d of
+// cut tet c) finds the first cut tet in id order that holds its node; that slot computes the running
+// mean, every slot records its first slot.
+struct QQGradArgs {
+    MeshP m;
+    const double *phi;
+    const int64_t *cut;
+    int64_t ncut;
+    const uint32_t *cellinfo;
+    int32_t *first;
+    double *gh;
+};
+
+__global__ void __launch_bounds__(128) k_qq_node_gradient(QQGradArgs a) {
+    const int64_t s = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (s >= 10 * a.ncut) return;
+    int64_t J[3];
+    p2_node_of(a.m, a.cut[s / 10], (int)(s % 10), J);
+    int64_t found = s;
+    for_cut_tets_at_node(a.m, a.cellinfo, J, [&](int64_t ci, int, int li, int) {
+        found = ci * 10 + li;
+        return true;
+    });
+    a.first[s] = (int32_t)found;
+    if (found == s) {
+        double g[3];
+        qq_recovered(a.m, a.phi, J, g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) a.gh[3 * s + c] = g[c];
+    }
+}
+
 // averaging_P2_gradient_recovery over every half-grid node: out[3 j + c]
 struct RecGradArgs {
     MeshP m;
@@ -455,35 +492,42 @@ __global__ void __launch_bounds__(128) k_recovered_gradient(RecGradArgs a) {
     for (int c = 0; c < 3; ++c) a.out[3 * j + c] = g[c];
 }
 
+constexpr int QQ_TPW = 4;   // cut tets per wave (16 lanes each: a tet has at most 14 quadrature points)
+
 __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
-    __shared__ QQShared s;
-    const int64_t c = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (c >= a.ncut) return;
-    const int64_t t = a.cut[c];
-    double X[4][3], ls[10];
-    load_tet_geom(a.m, a.phi, t, X, ls);
-    double v[10];
-#pragma unroll
-    for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+    __shared__ QQShared sh[QQ_TPW];
+    const int lane = threadIdx.x, t = lane >> 4, j = lane & 15;
+    const int64_t c = (int64_t)blockIdx.x * QQ_TPW + t;
+    const bool live = c < a.ncut;
+    QQShared &s = sh[t];
     const int64_t N = a.ncut;
     double *L = a.loc;
-    // the lattice-1 patch: the four vertex values' sign pattern
-    int sg[4];
+    // ---- per tet: level set, the lattice-1 patch's sign pattern ----
+    double X[4][3], v[10];
+    int nf = 0, pid = 40;
+    if (live) {
+        double ls[10];
+        load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) sg[q] = sgn(v[q]);
-    if (!sg[0] && !sg[1] && !sg[2] && !sg[3]) {
-        if (lane == 0) atomicOr(a.err, 2);
-        return;
+        for (int d = 0; d < 10; ++d) v[d] = ls[c_tab.lat_of_p2[d]];
+        int sg[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sg[q] = sgn(v[q]);
+        if (!sg[0] && !sg[1] && !sg[2] && !sg[3]) {
+            nf = -1;
+        } else {
+            pid = sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40;
+            nf = c_tab.rp_size[pid];
+        }
     }
-    const int pid = sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40;
-    const int nf = c_tab.rp_size[pid];
-    if (nf == 0) {
-        for (int e = lane; e < LOC2_HAT; e += 64) L[(int64_t)e * N + c] = 0.;
-        if (lane == 0) a.flags[c] = 0;
-        return;
+    if (live && nf < 0 && j == 0) atomicOr(a.err, 2);
+    if (live && nf == 0) {
+        for (int e = j; e < LOC2_HAT; e += 16) L[(int64_t)e * N + c] = 0.;
+        if (j == 0) a.flags[c] = 0;
     }
-    if (lane == 0) {
+    const bool work = live && nf > 0;
+    if (j == 0) s.nf = work ? nf : 0;
+    if (work && j == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q)
 #pragma unroll
@@ -510,40 +554,36 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             s.w2bd[e] = q.d[e];
             s.w2bb[e] = q.beta[e];
         }
-        s.nf = nf;
     }
-    if (lane < 10) {
-        s.v[lane] = v[lane];
-        int64_t J[3];
-        p2_node_of(a.m, t, lane, J);
-        double g[3];
-        qq_recovered(a.m, a.phi, J, g);
+    if (work && j < 10) {
+        s.v[j] = v[j];
+        const int64_t f = a.first[10 * c + j];
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) s.gh[lane][cc] = g[cc];
+        for (int cc = 0; cc < 3; ++cc) s.gh[j][cc] = a.gh[3 * f + cc];
     }
     bool bad = false;
-    if (lane < nf) {
-        // facet `lane` of the lattice-1 patch: zero vertices or edge cuts (1 - a) e_a + a e_b
+    if (work && j < nf) {
+        // facet j of the lattice-1 patch: zero vertices or edge cuts (1 - a) e_a + a e_b
         const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
         double vb[3][4];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const int rep = c_tab.rp_tri[pid][lane][j];
+        for (int jj = 0; jj < 3; ++jj) {
+            const int rep = c_tab.rp_tri[pid][j][jj];
 #pragma unroll
-            for (int cc = 0; cc < 4; ++cc) vb[j][cc] = 0.;
+            for (int cc = 0; cc < 4; ++cc) vb[jj][cc] = 0.;
             if (rep < 4) {
-                vb[j][rep] = 1.;
+                vb[jj][rep] = 1.;
             } else {
                 const int ia = ea[rep - 4], ib = eb[rep - 4];
                 const double alpha = v[ia] / (v[ia] - v[ib]);
 #pragma unroll
                 for (int cc = 0; cc < 4; ++cc)
-                    vb[j][cc] = (1.0 - alpha) * (cc == ia ? 1. : 0.) + alpha * (cc == ib ? 1. : 0.);
+                    vb[jj][cc] = (1.0 - alpha) * (cc == ia ? 1. : 0.) + alpha * (cc == ib ? 1. : 0.);
             }
         }
         double w[3][3];
 #pragma unroll
-        for (int j = 0; j < 3; ++j) b2w(X, vb[j], w[j]);
+        for (int jj = 0; jj < 3; ++jj) b2w(X, vb[jj], w[jj]);
         SmallQR<3, 2> q;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -558,42 +598,43 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             dd *= q.d[1];
             double af = c_tab.rp_bnd[pid] ? 0.5 : 1.;
             af *= fabs(dd);
-            s.af[lane] = af;
+            s.af[j] = af;
 #pragma unroll
-            for (int j = 0; j < 3; ++j) {
+            for (int jj = 0; jj < 3; ++jj) {
                 double e3[3] = {0., 0., 0.};
-                e3[j] = 1.;
+                e3[jj] = 1.;
                 q.apply_Q(e3);
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    if (j < 2) s.U[lane][r][j] = e3[r];
-                    else s.nrm[lane][r] = e3[r];
+                    if (jj < 2) s.U[j][r][jj] = e3[r];
+                    else s.nrm[j][r] = e3[r];
                 }
             }
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
+            for (int jj = 0; jj < 3; ++jj)
 #pragma unroll
-                for (int cc = 0; cc < 4; ++cc) s.vb[lane][j][cc] = vb[j][cc];
+                for (int cc = 0; cc < 4; ++cc) s.vb[j][jj][cc] = vb[jj][cc];
         }
     }
-    if (lane < 40) {
-        const int i = lane / 4, k = lane % 4;
+    if (work) {
         double T[3][3];
-        // the lane's own copy of T (lane 0 may still be writing s.T)
-        qq_trafo(X, T);
-        double g[3];
-        qq_grad_at_vertex(T, i, k, g);
+        qq_trafo(X, T);   // the lane's own copy (lane 0 of the group may still be writing s.T)
+        for (int e = j; e < 40; e += 16) {
+            const int i = e / 4, k = e % 4;
+            double g[3];
+            qq_grad_at_vertex(T, i, k, g);
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) s.G[i][k][cc] = g[cc];
+            for (int cc = 0; cc < 3; ++cc) s.G[i][k][cc] = g[cc];
+        }
     }
     if (__syncthreads_or(bad)) {
         if (lane == 0) atomicOr(a.err, 4);
         return;
     }
-    // ---- one quadrature point per lane ----
-    const int nq = 7 * nf;
-    if (lane < nq) {
-        const int f = lane / 7, qi = lane - 7 * (lane / 7);
+    // ---- one quadrature point per lane of the tet's group ----
+    const int nq = work ? 7 * nf : 0;
+    if (j < nq) {
+        const int f = j / 7, qi = j - 7 * (j / 7);
         double nb[4];
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
@@ -602,7 +643,7 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             sv += s.vb[f][2][cc] * c_tab.q5n[qi][2];
             nb[cc] = sv;
         }
-        s.w[lane] = s.af[f] * c_tab.q5w[qi];
+        s.w[j] = s.af[f] * c_tab.q5w[qi];
         double bxb[4], dh, dph[3][3], n[3];
         if (!qq_base_point(s, a.P, nb, bxb, dh) || !qq_jacobian(s, bxb, dh, dph, n)) {
             bad = true;
@@ -625,7 +666,7 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             for (int i = 0; i < 3; ++i) g10 += DU[i][1] * DU[i][0];
 #pragma unroll
             for (int i = 0; i < 3; ++i) g11 += DU[i][1] * DU[i][1];
-            s.ad[lane] = sqrt(g00 * g11 - g10 * g10);
+            s.ad[j] = sqrt(g00 * g11 - g10 * g10);
             // W = dph^T + nl (n/(nl.n) - dph nl), nl the first facet's normal (gradient_trafo,
             // ifacetransp.cpp:894-918)
             double nl[3] = {s.nrm[0][0], s.nrm[0][1], s.nrm[0][2]};
@@ -644,12 +685,12 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             } else {
                 double Wi[3][3];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) {
+                for (int jj = 0; jj < 3; ++jj) {
                     double e[3] = {0., 0., 0.};
-                    e[j] = 1.;
+                    e[jj] = 1.;
                     W.solve(e);
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) Wi[r][j] = e[r];
+                    for (int r = 0; r < 3; ++r) Wi[r][jj] = e[r];
                 }
                 const double nf3[3] = {s.nrm[f][0], s.nrm[f][1], s.nrm[f][2]};
 #pragma unroll
@@ -661,19 +702,19 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
                     for (int cc = 0; cc < 3; ++cc) tt[cc] = g[cc] - d * nf3[cc];
                     qq_mv3(Wi, tt, r);
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) s.Gq[lane][i][cc] = r[cc];
+                    for (int cc = 0; cc < 3; ++cc) s.Gq[j][i][cc] = r[cc];
                 }
                 double h[10];
                 p2_basis(nb, h);
 #pragma unroll
-                for (int i = 0; i < 10; ++i) s.H[lane][i] = h[i];
+                for (int i = 0; i < 10; ++i) s.H[j][i] = h[i];
                 double fv = 0.;
                 if (a.rhs_kind != DHIP_RHS_NONE) {
                     double y[3];
                     b2w(s.X, bxb, y);
                     fv = rhs_fun(a.rhs_kind, a.rhs_p0, y);
                 }
-                s.f[lane] = fv;
+                s.f[j] = fv;
             }
         }
     }
@@ -681,33 +722,40 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
         if (lane == 0) atomicOr(a.err, 4);
         return;
     }
-    // ---- contractions in the composite order ----
-    if (lane < 55) {
-        int pi = 0, pj = 0;
-        pair_of(lane, pi, pj);
-        double am = 0.0, al = 0.0;
-        for (int k = 0; k < nq; ++k) {
-            const double w = s.w[k], ad = s.ad[k];
-            am += w * ((ad * s.H[k][pj]) * s.H[k][pi]);
-            const double dq = s.Gq[k][pj][0] * s.Gq[k][pi][0] + s.Gq[k][pj][1] * s.Gq[k][pi][1] +
-                              s.Gq[k][pj][2] * s.Gq[k][pi][2];
-            al += w * (ad * dq);
+    // ---- contractions in the composite order: per tet 55 pairs (i >= j) and 10 load entries ----
+    for (int task = lane; task < QQ_TPW * 65; task += 64) {
+        const int tt = task / 65, r = task - 65 * (task / 65);
+        const int64_t ct = (int64_t)blockIdx.x * QQ_TPW + tt;
+        const QQShared &S = sh[tt];
+        if (ct >= N || S.nf <= 0) continue;
+        const int nqt = 7 * S.nf;
+        if (r < 55) {
+            int pi = 0, pj = 0;
+            pair_of(r, pi, pj);
+            double am = 0.0, al = 0.0;
+            for (int k = 0; k < nqt; ++k) {
+                const double w = S.w[k], ad = S.ad[k];
+                am += w * ((ad * S.H[k][pj]) * S.H[k][pi]);
+                const double dq = S.Gq[k][pj][0] * S.Gq[k][pi][0] + S.Gq[k][pj][1] * S.Gq[k][pi][1] +
+                                  S.Gq[k][pj][2] * S.Gq[k][pi][2];
+                al += w * (ad * dq);
+            }
+            const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
+            L[(int64_t)(LOC2_MASS + e1) * N + ct] = am;
+            L[(int64_t)(LOC2_MASS + e2) * N + ct] = am;
+            L[(int64_t)(LOC2_LB + e1) * N + ct] = al;
+            L[(int64_t)(LOC2_LB + e2) * N + ct] = al;
+            L[(int64_t)(LOC2_STAB + e1) * N + ct] = 0.;
+            L[(int64_t)(LOC2_STAB + e2) * N + ct] = 0.;
+        } else {
+            const int i = r - 55;
+            double rh = 0.0;
+            if (a.rhs_kind != DHIP_RHS_NONE)
+                for (int k = 0; k < nqt; ++k) rh += S.w[k] * ((S.ad[k] * S.f[k]) * S.H[k][i]);
+            L[(int64_t)(LOC2_RHS + i) * N + ct] = rh;
         }
-        const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
-        L[(int64_t)(LOC2_MASS + e1) * N + c] = am;
-        L[(int64_t)(LOC2_MASS + e2) * N + c] = am;
-        L[(int64_t)(LOC2_LB + e1) * N + c] = al;
-        L[(int64_t)(LOC2_LB + e2) * N + c] = al;
-        L[(int64_t)(LOC2_STAB + e1) * N + c] = 0.;
-        L[(int64_t)(LOC2_STAB + e2) * N + c] = 0.;
     }
-    if (lane < 10) {
-        double rh = 0.0;
-        if (a.rhs_kind != DHIP_RHS_NONE)
-            for (int k = 0; k < nq; ++k) rh += s.w[k] * ((s.ad[k] * s.f[k]) * s.H[k][lane]);
-        L[(int64_t)(LOC2_RHS + lane) * N + c] = rh;
-    }
-    if (lane == 0) a.flags[c] = 1;
+    if (work && j == 0) a.flags[c] = 1;
 }
 
 }  // namespace dhip
