@@ -409,8 +409,8 @@ def quaqua_leg(ctx, mesh, phi, reps=5, with_cpu=True):
     ctx.profile(True)
     ctx.assemble_p2_quaqua(rhs_kind=D.RHS_3XY_MINUS_X3)
     kern = {}
-    for k in ("classify", "compact", "hat_p2", "qualify_p2", "first_owner_p2", "dofs_p2", "node_gradient_qq",
-              "local_p2qq", "rowcount_p2", "rowfill_p2"):
+    for k in ("classify", "compact", "hat_p2", "qualify_p2", "first_owner_p2", "dofs_p2", "first_qq",
+              "node_gradient_qq", "local_p2qq", "rowcount_p2", "rowfill_p2"):
         ms, cnt = ctx.profile_get(k)
         if cnt:
             kern[k] = round(1e3 * ms / cnt, 2)

@@ -446,7 +446,7 @@ void dhip_destroy(dhip_ctx_t *c) {
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
                       &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
                       &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos,
-                      &c->gm_sell_val32, &c->qp_off, &c->qp_vals, &c->qq_first, &c->qq_gh};
+                      &c->gm_sell_val32, &c->qp_off, &c->qp_vals, &c->qq_first, &c->qq_gh, &c->qq_list};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -959,10 +959,14 @@ static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t r
     const int64_t N = c->ncut;
     DHIP_HIPCHK(c, c->qq_first.ensure(sizeof(int32_t) * (10 * N + 1)));
     DHIP_HIPCHK(c, c->qq_gh.ensure(sizeof(double) * (30 * N + 3)));
+    DHIP_HIPCHK(c, c->qq_list.ensure(sizeof(int32_t) * (10 * N + 2)));
     DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
     if (N > 0) {
+        int32_t *cnt = c->qq_list.as<int32_t>() + 10 * N;
+        DHIP_LAUNCH(c, "zero_count", k_zero_u32, 1, 64, 0, reinterpret_cast<uint32_t *>(cnt), 1);
         QQGradArgs ga{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->qq_first.as<int32_t>(),
-                      c->qq_gh.as<double>()};
+                      c->qq_list.as<int32_t>(), cnt, c->qq_gh.as<double>()};
+        DHIP_LAUNCH(c, "first_qq", k_qq_first, blocks_for(10 * N, 128), 128, 0, ga);
         DHIP_LAUNCH(c, "node_gradient_qq", k_qq_node_gradient, blocks_for(10 * N, 128), 128, 0, ga);
     }
     LocalP2QQArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, rhs_kind, rhs_param ? rhs_param[0] : 0.0, P,
@@ -1095,13 +1099,9 @@ int dhip_get_local_p2(dhip_ctx_t *c, double *local320, uint8_t *flags) {
     if (!c) return DHIP_E_ARG;
     if (c->stage < 2 || c->space != 2) return set_error(c, DHIP_E_STATE, "no P2 local matrices");
     const int64_t N = c->ncut;
-    if (local320 && N > 0) {
-        std::vector<double> soa((size_t)LOC2_N * N);
-        int rc = d2h(c, soa.data(), c->loc.p, sizeof(double) * LOC2_N * N);
+    if (local320 && N > 0) {   // the device array is tet-major already
+        int rc = d2h(c, local320, c->loc.p, sizeof(double) * LOC2_N * N);
         if (rc) return rc;
-        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
-        for (int64_t i = 0; i < N; ++i)
-            for (int e = 0; e < LOC2_N; ++e) local320[i * LOC2_N + e] = soa[(size_t)e * N + i];
     }
     int rc = d2h(c, flags, c->flags.p, N);
     if (rc) return rc;
@@ -2070,6 +2070,16 @@ static int ch_copy(dhip_ctx *c, DevBuf &dst, const void *src, size_t bytes) {
     return DHIP_OK;
 }
 
+// the P2 local mass of every tet, [N][100], out of the tet-major local array [N][LOC2_N]
+static int ch_copy_mass_p2(dhip_ctx *c, DevBuf &dst, const void *loc, int64_t N) {
+    DHIP_HIPCHK(c, dst.ensure(sizeof(double) * 100 * N + 8));
+    if (N)
+        DHIP_HIPCHK(c, hipMemcpy2DAsync(dst.p, sizeof(double) * 100, static_cast<const double *>(loc) + LOC2_MASS,
+                                        sizeof(double) * LOC2_N, sizeof(double) * 100, (size_t)N,
+                                        hipMemcpyDeviceToDevice, c->stream));
+    return DHIP_OK;
+}
+
 static void ch_release(dhip_ch *h) {
     DevBuf *bufs[] = {&h->phi, &h->cut, &h->cellinfo, &h->flags, &h->dofs, &h->rowowner, &h->massloc, &h->rowptr,
                       &h->col, &h->mass, &h->lb, &h->stab, &h->geom, &h->tet, &h->brp, &h->bcol, &h->bval, &h->pcdiag,
@@ -2255,7 +2265,8 @@ static int ch_create(dhip_ctx *c, int32_t zero_policy, const dhip_ch_params_t *p
         (rc = ch_copy(c, h->flags, c->flags.p, N)) ||
         (rc = ch_copy(c, h->dofs, c->dofs.p, sizeof(int32_t) * nloc * N)) ||
         (rc = ch_copy(c, h->rowowner, c->rowowner.p, sizeof(int32_t) * R)) ||
-        (rc = ch_copy(c, h->massloc, c->loc.p, sizeof(double) * nloc * nloc * N)) ||
+        (rc = space == 2 ? ch_copy_mass_p2(c, h->massloc, c->loc.p, N)
+                         : ch_copy(c, h->massloc, c->loc.p, sizeof(double) * nloc * nloc * N)) ||
         (rc = ch_copy(c, h->rowptr, c->rowptr.p, sizeof(int32_t) * (R + 1))) ||
         (rc = ch_copy(c, h->col, c->col.p, sizeof(int32_t) * Z)) ||
         (rc = ch_copy(c, h->mass, c->val[0].p, sizeof(double) * Z)) ||

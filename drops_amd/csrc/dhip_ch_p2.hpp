@@ -160,7 +160,7 @@ struct ChTetP2Args {
     int64_t ncut;
     const uint8_t *flags;
     const int32_t *dofs;     // [ncut][10]
-    const double *massloc;   // P2 local mass, SoA [100][ncut]
+    const double *massloc;   // P2 local mass, [ncut][100]
     const double *T;         // [ncut][10][55]
     const double *conc;      // c on the DOFs (the extrapolated c_e of a BDF2 step)
     const double *conc_r;    // BDF2: the c_r of the time-derivative right-hand side; nullptr otherwise
@@ -200,14 +200,13 @@ __global__ void __launch_bounds__(64 * CHTET2_WAVES) k_ch_tet_p2(ChTetP2Args a) 
     // step adds M c_r
     const int i = lane;
     if (i < 10) {
-        const int64_t N = a.ncut;
         const bool bdf = a.conc_r != nullptr;
         double vt = 0., vw = 0., vr = 0.;
         if (nb[i] >= 0) {
 #pragma unroll
             for (int j = 0; j < 10; ++j) {
                 if (nb[j] < 0) continue;
-                const double mij = a.massloc[(int64_t)(i * 10 + j) * N + c];
+                const double mij = a.massloc[(int64_t)c * 100 + i * 10 + j];
                 vt += mij * xc[j];
                 vw += mij * potential_prime(xc[j]);
                 if (bdf) vr += mij * a.conc_r[nb[j]];

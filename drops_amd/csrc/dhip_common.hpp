@@ -120,7 +120,8 @@ struct dhip_ctx {
     uint64_t geo_gen = 1;   // bumped by every new mesh or level set
     // right-hand side values at the patch quadrature points (DHIP_RHS_VALUES)
     dhip::DevBuf qp_off, qp_vals;
-    dhip::DevBuf qq_first, qq_gh;   // mapped P2 pass: first cut slot of each slot's node, its recovered gradient
+    dhip::DevBuf qq_first, qq_gh, qq_list;   // mapped P2 pass: first cut slot of each slot's node, its recovered
+                                             // gradient, the list of first slots (+ its count)
     int64_t qp_n = -1, qp_ncut = -1;
     uint64_t qp_gen = 0;
     double qp_band = -2.0;

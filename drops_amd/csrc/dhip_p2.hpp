@@ -9,7 +9,7 @@
 //                 lane p < 55 accumulates pair p = (i >= j) of the 10x10 mass and
 //                 Laplace-Beltrami matrices point by point in the composite order;
 //                 the normal-gradient stabilisation uses the 15-point Quad5 volume
-//                 rule.  Output is SoA [LOC2_N][ncut].
+//                 rule.  Output is tet-major [ncut][LOC2_N].
 //   k_qualify_p2 / k_first_p2 / k_dofs
 //                 first-encounter numbering over 10 slots per cut tet
 //                 (CreateNumbOnInterfaceP2, misc/problem.cpp:571-636): a half-grid node
@@ -30,6 +30,9 @@ namespace dhip {
 
 constexpr int LOC2_N = 320;  // 100 mass, 100 lb, 100 stab, 10 rhs, 10 hat_sq
 constexpr int LOC2_MASS = 0, LOC2_LB = 100, LOC2_STAB = 200, LOC2_RHS = 300, LOC2_HAT = 310;
+// element e of cut tet c in the local array (tet-major: a tet's 320 values are contiguous, so the
+// row builder's lanes read a tet's row of one operator as consecutive doubles)
+__device__ __forceinline__ int64_t loc2(int e, int64_t c) { return c * LOC2_N + e; }
 constexpr int P2_MAXF = 16;
 constexpr int P2_CHUNK = 64;
 
@@ -70,7 +73,7 @@ struct LocalP2Args {
     double coef0, coef1, coef2;
     int rhs_kind;
     double rhs_p0;
-    double *loc;      // SoA [LOC2_N][ncut]
+    double *loc;      // tet-major [ncut][LOC2_N]
     uint8_t *flags;   // bit0: non-empty patch
     int32_t *err;
     const double *rhs_vals = nullptr;   // DHIP_RHS_VALUES: f at every patch quadrature point
@@ -188,10 +191,9 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
         if (lane == 0) atomicOr(a.err, 2);
         return;
     }
-    const int64_t N = a.ncut;
     double *L = a.loc;
     if (nf == 0) {
-        for (int e = lane; e < LOC2_N; e += 64) L[e * N + c] = 0.;
+        for (int e = lane; e < LOC2_N; e += 64) L[loc2(e, c)] = 0.;
         if (lane == 0) a.flags[c] = 0;
         return;
     }
@@ -251,16 +253,16 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
         const double st = a.coef2 * val;
         const double mv = a.coef0 * am, lv = a.coef1 * al;
         const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
-        L[(LOC2_MASS + e1) * N + c] = mv;
-        L[(LOC2_MASS + e2) * N + c] = mv;
-        L[(LOC2_LB + e1) * N + c] = lv;
-        L[(LOC2_LB + e2) * N + c] = lv;
-        L[(LOC2_STAB + e1) * N + c] = st;
-        L[(LOC2_STAB + e2) * N + c] = st;
+        L[loc2(LOC2_MASS + e1, c)] = mv;
+        L[loc2(LOC2_MASS + e2, c)] = mv;
+        L[loc2(LOC2_LB + e1, c)] = lv;
+        L[loc2(LOC2_LB + e2, c)] = lv;
+        L[loc2(LOC2_STAB + e1, c)] = st;
+        L[loc2(LOC2_STAB + e2, c)] = st;
     }
     if (lane < 10) {
-        L[(LOC2_RHS + lane) * N + c] = rh;
-        L[(LOC2_HAT + lane) * N + c] = hat;
+        L[loc2(LOC2_RHS + lane, c)] = rh;
+        L[loc2(LOC2_HAT + lane, c)] = hat;
     }
     if (lane == 0) a.flags[c] = 1;
 }
@@ -287,10 +289,9 @@ __global__ void __launch_bounds__(64) k_hat_p2(LocalP2Args a) {
         if (lane == 0) atomicOr(a.err, 2);
         return;
     }
-    const int64_t N = a.ncut;
     double *L = a.loc;
     if (nf == 0) {
-        if (lane < 10) L[(LOC2_HAT + lane) * N + c] = 0.;
+        if (lane < 10) L[loc2(LOC2_HAT + lane, c)] = 0.;
         if (lane == 0) a.flags[c] = 0;
         return;
     }
@@ -321,7 +322,7 @@ __global__ void __launch_bounds__(64) k_hat_p2(LocalP2Args a) {
             for (int k = 0; k < cnt; ++k) hat += sh.w[k] * (sh.h[k][lane] * sh.h[k][lane]);
         __syncthreads();
     }
-    if (lane < 10) L[(LOC2_HAT + lane) * N + c] = hat;
+    if (lane < 10) L[loc2(LOC2_HAT + lane, c)] = hat;
     if (lane == 0) a.flags[c] = 1;
 }
 
@@ -356,7 +357,7 @@ __global__ void k_qualify_p2(QualP2Args a) {
             limit = a.omit_bound * pow(tet_volume(X) * 6, 4. / 3.);
         }
         for (int d = 0; d < 10; ++d)
-            if (a.loc[(int64_t)(LOC2_HAT + d) * a.ncut + c] > limit) q |= 1u << d;
+            if (a.loc[loc2(LOC2_HAT + d, c)] > limit) q |= 1u << d;
     }
     a.qual[c] = (uint16_t)q;
 }
@@ -480,7 +481,6 @@ __device__ inline void p2_row(const RowP2Args &a, RowP2Shared &sh, int64_t r, in
     int64_t J[3];
     p2_node_of(a.m, a.cut[own / 10], own % 10, J);
     double rh = 0.0;
-    const int64_t N = a.ncut;
     for_cut_tets_at_node(a.m, a.cellinfo, J, [&](int64_t ci, int k, int li, int rel) {
         if (!(a.flags[ci] & 1)) return false;
         if (lane < 10) {
@@ -493,13 +493,13 @@ __device__ inline void p2_row(const RowP2Args &a, RowP2Shared &sh, int64_t r, in
                 if (FILL) {
                     const int e = li * 10 + lane;
                     sh.cols[sl] = dj;
-                    sh.am[sl] += a.loc[(int64_t)(LOC2_MASS + e) * N + ci];
-                    sh.al[sl] += a.loc[(int64_t)(LOC2_LB + e) * N + ci];
-                    sh.as[sl] += a.loc[(int64_t)(LOC2_STAB + e) * N + ci];
+                    sh.am[sl] += a.loc[loc2(LOC2_MASS + e, ci)];
+                    sh.al[sl] += a.loc[loc2(LOC2_LB + e, ci)];
+                    sh.as[sl] += a.loc[loc2(LOC2_STAB + e, ci)];
                 }
             }
         }
-        if (FILL) rh += a.loc[(int64_t)(LOC2_RHS + li) * N + ci];
+        if (FILL) rh += a.loc[loc2(LOC2_RHS + li, ci)];
         wave_sync_lds();
         return false;
     });

@@ -7,10 +7,10 @@
 // damped Newton on (x - y - s g_h(y), -phi_h(y)) = 0 with the recovered gradient g_h, and the forms
 // carry |det| of the map's Jacobian and, for the Laplace-Beltrami form, its gradient transformation
 // W^-1 (LocalMassP2CL, LocalLaplaceBeltramiP2CL, LocalVectorP2CL, ifacetransp.h:2302-2403).
-//   k_qq_node_gradient  one thread per (cut tet, P2 node): the recovered gradient of each node once, at
-//                     the first cut tet holding it: the running mean over every tet of the mesh that holds
-//                     the node, in tet id order (averaging_P2_gradient_recovery through
-//                     OswaldProjectionP2AccuCL, num/gradient_recovery.cpp:62-89);
+//   k_qq_first / k_qq_node_gradient  the recovered gradient of each P2 node of the cut tets once, at the
+//                     first cut tet holding it: the running mean over every tet of the mesh that holds the
+//                     node, in tet id order (averaging_P2_gradient_recovery through OswaldProjectionP2AccuCL,
+//                     num/gradient_recovery.cpp:62-89), one thread per distinct node;
 //   k_local_p2qq  four cut tets per wave, 16 lanes each:
 //     lanes j < 10    the recovered gradient at the tet's ten P2 nodes from the node kernel;
 //     all 16          the tet's P2 gradient fields T DH_iRef at its vertices into LDS;
@@ -381,7 +381,7 @@ struct LocalP2QQArgs {
     int rhs_kind;
     double rhs_p0;
     QQParams P;
-    double *loc;      // SoA [LOC2_N][ncut]
+    double *loc;      // tet-major [ncut][LOC2_N]
     uint8_t *flags;   // bit0: the tet adds to the mapped matrices
     int32_t *err;     // bit 1: full body in the zero level set, bit 2: singular matrix in the mapping
     const int32_t *first;   // [10 ncut] the first cut slot holding each slot's P2 node (k_qq_node_gradient)
@@ -441,9 +441,10 @@ __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int
     for (int c = 0; c < 3; ++c) out[c] = avg[c];
 }
 
-// The recovered gradient of the P2 nodes of the cut tets, once per node: slot s = 10 c + d (P2 dof d of
-// cut tet c) finds the first cut tet in id order that holds its node; that slot computes the running
-// mean, every slot records its first slot.
+// The recovered gradient of the P2 nodes of the cut tets, once per node.  k_qq_first: slot s = 10 c + d
+// (P2 dof d of cut tet c) finds the first cut tet in id order that holds its node and records it; the
+// slots that are their node's first append themselves to a list (one atomic per wave).  k_qq_node_gradient:
+// one thread per listed slot computes the node's running mean.
 struct QQGradArgs {
     MeshP m;
     const double *phi;
@@ -451,26 +452,44 @@ struct QQGradArgs {
     int64_t ncut;
     const uint32_t *cellinfo;
     int32_t *first;
+    int32_t *list;    // owner slots, in no particular order
+    int32_t *count;   // their number (zeroed before k_qq_first)
     double *gh;
 };
 
-__global__ void __launch_bounds__(128) k_qq_node_gradient(QQGradArgs a) {
+__global__ void __launch_bounds__(128) k_qq_first(QQGradArgs a) {
     const int64_t s = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (s >= 10 * a.ncut) return;
+    bool own = false;
+    if (s < 10 * a.ncut) {
+        int64_t J[3];
+        p2_node_of(a.m, a.cut[s / 10], (int)(s % 10), J);
+        int64_t found = s;
+        for_cut_tets_at_node(a.m, a.cellinfo, J, [&](int64_t ci, int, int li, int) {
+            found = ci * 10 + li;
+            return true;
+        });
+        a.first[s] = (int32_t)found;
+        own = found == s;
+    }
+    const uint64_t m = __ballot(own);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(a.count, __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (own) a.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)s;
+}
+
+__global__ void __launch_bounds__(128) k_qq_node_gradient(QQGradArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (i >= *a.count) return;
+    const int64_t s = a.list[i];
     int64_t J[3];
     p2_node_of(a.m, a.cut[s / 10], (int)(s % 10), J);
-    int64_t found = s;
-    for_cut_tets_at_node(a.m, a.cellinfo, J, [&](int64_t ci, int, int li, int) {
-        found = ci * 10 + li;
-        return true;
-    });
-    a.first[s] = (int32_t)found;
-    if (found == s) {
-        double g[3];
-        qq_recovered(a.m, a.phi, J, g);
+    double g[3];
+    qq_recovered(a.m, a.phi, J, g);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) a.gh[3 * s + c] = g[c];
-    }
+    for (int c = 0; c < 3; ++c) a.gh[3 * s + c] = g[c];
 }
 
 // averaging_P2_gradient_recovery over every half-grid node: out[3 j + c]
@@ -522,7 +541,7 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
     }
     if (live && nf < 0 && j == 0) atomicOr(a.err, 2);
     if (live && nf == 0) {
-        for (int e = j; e < LOC2_HAT; e += 16) L[(int64_t)e * N + c] = 0.;
+        for (int e = j; e < LOC2_HAT; e += 16) L[loc2(e, c)] = 0.;
         if (j == 0) a.flags[c] = 0;
     }
     const bool work = live && nf > 0;
@@ -741,18 +760,18 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
                 al += w * (ad * dq);
             }
             const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
-            L[(int64_t)(LOC2_MASS + e1) * N + ct] = am;
-            L[(int64_t)(LOC2_MASS + e2) * N + ct] = am;
-            L[(int64_t)(LOC2_LB + e1) * N + ct] = al;
-            L[(int64_t)(LOC2_LB + e2) * N + ct] = al;
-            L[(int64_t)(LOC2_STAB + e1) * N + ct] = 0.;
-            L[(int64_t)(LOC2_STAB + e2) * N + ct] = 0.;
+            L[loc2(LOC2_MASS + e1, ct)] = am;
+            L[loc2(LOC2_MASS + e2, ct)] = am;
+            L[loc2(LOC2_LB + e1, ct)] = al;
+            L[loc2(LOC2_LB + e2, ct)] = al;
+            L[loc2(LOC2_STAB + e1, ct)] = 0.;
+            L[loc2(LOC2_STAB + e2, ct)] = 0.;
         } else {
             const int i = r - 55;
             double rh = 0.0;
             if (a.rhs_kind != DHIP_RHS_NONE)
                 for (int k = 0; k < nqt; ++k) rh += S.w[k] * ((S.ad[k] * S.f[k]) * S.H[k][i]);
-            L[(int64_t)(LOC2_RHS + i) * N + ct] = rh;
+            L[loc2(LOC2_RHS + i, ct)] = rh;
         }
     }
     if (work && j == 0) a.flags[c] = 1;
