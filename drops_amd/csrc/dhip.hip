@@ -446,7 +446,7 @@ void dhip_destroy(dhip_ctx_t *c) {
                       &c->rowlen, &c->rowptr, &c->col, &c->val[0], &c->val[1], &c->val[2], &c->val[3],
                       &c->rhs, &c->diag, &c->scan_tmp, &c->red_tmp, &c->bar, &c->host_stage, &c->qual,
                       &c->gm_sval, &c->probe, &c->gm_sell_off, &c->gm_sell_col, &c->gm_sell_val, &c->gm_sell_pos,
-                      &c->gm_sell_val32, &c->qp_off, &c->qp_vals, &c->qq_first, &c->qq_gh, &c->qq_list};
+                      &c->gm_sell_val32, &c->qp_off, &c->qp_vals, &c->qq_first, &c->qq_gh, &c->qq_list, &c->qqp_off, &c->qqp_vals};
     for (DevBuf *b : bufs) b->release();
     for (auto &b : c->kv) b.release();
     (void)hipStreamDestroy(c->stream);
@@ -941,26 +941,22 @@ static int build_p2_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
     return DHIP_OK;
 }
 
-// the mapped-surface forms of every cut tet (after the planar pass has numbered the P2 DOFs): they
-// replace the mass, Laplace-Beltrami and load slots, and flags keep only tets with a lattice-1 patch
-static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t rhs_kind, const double *rhs_param) {
-    if (c->stage < 3 || c->space != 2)
-        return set_error(c, DHIP_E_STATE, "dhip_number_p2if must precede dhip_local_p2_quaqua");
-    if (rhs_kind == DHIP_RHS_VALUES)
-        return set_error(c, DHIP_E_ARG, "values at the planar patch points are not the mapped surface's load");
-    if (rhs_kind < DHIP_RHS_NONE || rhs_kind > DHIP_RHS_CONST)
-        return set_error(c, DHIP_E_ARG, "unknown right-hand-side kind %d", (int)rhs_kind);
-    QQParams P{20, 1e-7, 1e-2, 8};
+static int qq_params(dhip_ctx *c, const dhip_quaqua_params_t *p, QQParams &P) {
+    P = QQParams{20, 1e-7, 1e-2, 8};
     if (p) {
         if (p->maxiter < 0 || !(p->tol >= 0.) || p->max_damping < 0)
             return set_error(c, DHIP_E_ARG, "QuaQua parameters: maxiter, max_damping >= 0, tol >= 0");
         P = QQParams{p->maxiter, p->tol, p->armijo_c, p->max_damping};
     }
+    return DHIP_OK;
+}
+
+// the recovered gradient at the P2 nodes of the cut tets (k_qq_first, k_qq_node_gradient)
+static int qq_node_gradients(dhip_ctx *c) {
     const int64_t N = c->ncut;
     DHIP_HIPCHK(c, c->qq_first.ensure(sizeof(int32_t) * (10 * N + 1)));
     DHIP_HIPCHK(c, c->qq_gh.ensure(sizeof(double) * (30 * N + 3)));
     DHIP_HIPCHK(c, c->qq_list.ensure(sizeof(int32_t) * (10 * N + 2)));
-    DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
     if (N > 0) {
         int32_t *cnt = c->qq_list.as<int32_t>() + 10 * N;
         DHIP_LAUNCH(c, "zero_count", k_zero_u32, 1, 64, 0, reinterpret_cast<uint32_t *>(cnt), 1);
@@ -969,14 +965,119 @@ static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t r
         DHIP_LAUNCH(c, "first_qq", k_qq_first, blocks_for(10 * N, 128), 128, 0, ga);
         DHIP_LAUNCH(c, "node_gradient_qq", k_qq_node_gradient, blocks_for(10 * N, 128), 128, 0, ga);
     }
+    return DHIP_OK;
+}
+
+// the mapped-surface forms of every cut tet (after the planar pass has numbered the P2 DOFs): they
+// replace the mass, Laplace-Beltrami and load slots, and flags keep only tets with a lattice-1 patch
+static int local_p2qq_impl(dhip_ctx *c, const dhip_quaqua_params_t *p, int32_t rhs_kind, const double *rhs_param) {
+    if (c->stage < 3 || c->space != 2)
+        return set_error(c, DHIP_E_STATE, "dhip_number_p2if must precede dhip_local_p2_quaqua");
+    if (rhs_kind < DHIP_RHS_NONE || rhs_kind > DHIP_RHS_VALUES)
+        return set_error(c, DHIP_E_ARG, "unknown right-hand-side kind %d", (int)rhs_kind);
+    QQParams P;
+    int rc0 = qq_params(c, p, P);
+    if (rc0) return rc0;
+    const double *vals = nullptr;
+    const int32_t *off = nullptr;
+    if (rhs_kind == DHIP_RHS_VALUES) {
+        // the values of f at the base points dhip_quaqua_points handed out for these cut tets and parameters
+        if (!c->qqp_have_vals || c->qqp_gen != c->geo_gen || c->qqp_ncut != c->ncut || c->qqp_maxiter != P.maxiter ||
+            c->qqp_tol != P.tol || c->qqp_armijo != P.armijo_c || c->qqp_maxdamp != P.max_damping)
+            return set_error(c, DHIP_E_STATE,
+                             "DHIP_RHS_VALUES on the mapped surface needs dhip_quaqua_points and dhip_quaqua_values on "
+                             "the current cut tets with the same parameters");
+        vals = c->qqp_vals.as<double>();
+        off = c->qqp_off.as<int32_t>();
+    }
+    const int64_t N = c->ncut;
+    DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+    int rc = qq_node_gradients(c);
+    if (rc) return rc;
     LocalP2QQArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, rhs_kind, rhs_param ? rhs_param[0] : 0.0, P,
                      c->loc.as<double>(), c->flags.as<uint8_t>(), c->errflag.as<int32_t>(),
-                     c->qq_first.as<int32_t>(), c->qq_gh.as<double>()};
+                     c->qq_first.as<int32_t>(), c->qq_gh.as<double>(), nullptr, vals, off};
     if (N > 0) DHIP_LAUNCH(c, "local_p2qq", k_local_p2qq, (unsigned)((N + QQ_TPW - 1) / QQ_TPW), 64, 0, la);
-    int rc = check_err_flag(c);
+    rc = check_err_flag(c);
     if (rc) return rc;
     c->stage = 3;
     return DHIP_OK;
+}
+
+int dhip_quaqua_points(dhip_ctx_t *c, const dhip_quaqua_params_t *params, int64_t *npoints, double *xyz,
+                       int32_t where) {
+    if (!c || !npoints) return DHIP_E_ARG;
+    if (c->stage < 1) return set_error(c, DHIP_E_STATE, "dhip_classify must precede dhip_quaqua_points");
+    if (c->band >= 0.) return set_error(c, DHIP_E_STATE, "the narrow band is a P1 assembly");
+    QQParams P;
+    int rc = qq_params(c, params, P);
+    if (rc) return rc;
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    const int64_t N = c->ncut;
+    DevBuf cnt;
+    DHIP_HIPCHK(c, cnt.ensure(sizeof(int32_t) * (N + 1)));
+    DHIP_HIPCHK(c, c->qqp_off.ensure(sizeof(int32_t) * (N + 1)));
+    int64_t total = 0;
+    if (N > 0) {
+        DHIP_LAUNCH(c, "quaqua_count", k_qq_count, blocks_for(N, 128), 128, 0, c->mesh, c->phi, c->cut.as<int64_t>(), N,
+                    cnt.as<int32_t>());
+        rc = scan_exclusive_i32(c, cnt.as<int32_t>(), c->qqp_off.as<int32_t>(), N, &total);
+        if (rc) return rc;
+        if (total > INT32_MAX)
+            return set_error(c, DHIP_E_ARG, "%lld mapped quadrature points exceed the 32-bit offsets",
+                             (long long)total);
+    }
+    const int32_t t32 = (int32_t)total;
+    DHIP_HIPCHK(c, hipMemcpyAsync(c->qqp_off.as<int32_t>() + N, &t32, sizeof t32, hipMemcpyHostToDevice, c->stream));
+    c->qqp_n = total;
+    c->qqp_ncut = N;
+    c->qqp_gen = c->geo_gen;
+    c->qqp_have_vals = false;
+    c->qqp_maxiter = P.maxiter;
+    c->qqp_tol = P.tol;
+    c->qqp_armijo = P.armijo_c;
+    c->qqp_maxdamp = P.max_damping;
+    *npoints = total;
+    if (xyz && total > 0) {
+        DevBuf tmp;
+        double *dst = xyz;
+        if (where != DHIP_DEVICE) {
+            DHIP_HIPCHK(c, tmp.ensure(sizeof(double) * 3 * total));
+            dst = tmp.as<double>();
+        }
+        DHIP_LAUNCH(c, "zero_flags", k_zero_u32, 1, 64, 0, c->errflag.as<uint32_t>(), 64);
+        rc = qq_node_gradients(c);
+        if (rc) return rc;
+        LocalP2QQArgs la{c->mesh, c->phi, c->cut.as<int64_t>(), N, DHIP_RHS_NONE, 0.0, P, nullptr, nullptr,
+                         c->errflag.as<int32_t>(), c->qq_first.as<int32_t>(), c->qq_gh.as<double>(), dst, nullptr,
+                         c->qqp_off.as<int32_t>()};
+        DHIP_LAUNCH(c, "quaqua_points", k_local_p2qq, (unsigned)((N + QQ_TPW - 1) / QQ_TPW), 64, 0, la);
+        rc = check_err_flag(c);
+        if (rc) return rc;
+        if (where != DHIP_DEVICE)
+            DHIP_HIPCHK(c, hipMemcpyAsync(xyz, dst, sizeof(double) * 3 * total, hipMemcpyDeviceToHost, c->stream));
+        DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    return flush_profile(c);
+}
+
+int dhip_quaqua_values(dhip_ctx_t *c, const double *values, int64_t npoints, int32_t where) {
+    if (!c || (npoints > 0 && !values) || npoints < 0) return DHIP_E_ARG;
+    if (c->qqp_n < 0 || c->qqp_gen != c->geo_gen)
+        return set_error(c, DHIP_E_STATE, "dhip_quaqua_points must precede dhip_quaqua_values on this level set");
+    if (npoints != c->qqp_n)
+        return set_error(c, DHIP_E_ARG, "%lld values for %lld mapped quadrature points", (long long)npoints,
+                         (long long)c->qqp_n);
+    DHIP_HIPCHK(c, hipSetDevice(c->device));
+    DHIP_HIPCHK(c, c->qqp_vals.ensure(sizeof(double) * (npoints + 1)));
+    if (npoints > 0)
+        DHIP_HIPCHK(c, hipMemcpyAsync(c->qqp_vals.p, values, sizeof(double) * npoints,
+                                      where == DHIP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                      c->stream));
+    DHIP_HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->qqp_have_vals = true;
+    return flush_profile(c);
 }
 
 int dhip_p2_recovered_gradient(dhip_ctx_t *c, double *grad, int32_t where) {

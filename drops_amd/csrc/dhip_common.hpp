@@ -126,6 +126,13 @@ struct dhip_ctx {
     uint64_t qp_gen = 0;
     double qp_band = -2.0;
     bool qp_have_vals = false;
+    // the mapped P2 surface's base points handed out for a host load (dhip_quaqua_points / _values)
+    dhip::DevBuf qqp_off, qqp_vals;
+    int64_t qqp_n = -1, qqp_ncut = -1;
+    uint64_t qqp_gen = 0;
+    bool qqp_have_vals = false;
+    int qqp_maxiter = -1, qqp_maxdamp = -1;
+    double qqp_tol = 0., qqp_armijo = 0.;
 
     // assembly state: 0 none, 1 classified, 2 local, 3 numbered, 4 built
     int stage = 0;

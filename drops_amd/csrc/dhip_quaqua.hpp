@@ -386,7 +386,24 @@ struct LocalP2QQArgs {
     int32_t *err;     // bit 1: full body in the zero level set, bit 2: singular matrix in the mapping
     const int32_t *first;   // [10 ncut] the first cut slot holding each slot's P2 node (k_qq_node_gradient)
     const double *gh;       // [10 ncut][3] the recovered gradient, at first slots
+    double *pts = nullptr;          // points mode: the base points (world) out, 3 per point; no forms
+    const double *vals = nullptr;   // DHIP_RHS_VALUES: f at the base points
+    const int32_t *off = nullptr;   //   first point of each cut tet (pts, vals)
 };
+
+// the mapped quadrature points of each cut tet: 7 per facet of its lattice-1 patch
+__global__ void __launch_bounds__(128) k_qq_count(MeshP m, const double *phi, const int64_t *cut, int64_t ncut,
+                                                   int32_t *cnt) {
+    const int64_t c = (int64_t)blockIdx.x * 128 + threadIdx.x;
+    if (c >= ncut) return;
+    double X[4][3], ls[10];
+    load_tet_geom(m, phi, cut[c], X, ls);
+    int sg[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sg[q] = sgn(ls[c_tab.lat_of_p2[q]]);
+    const bool zero = !sg[0] && !sg[1] && !sg[2] && !sg[3];
+    cnt[c] = zero ? 0 : 7 * c_tab.rp_size[sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40];
+}
 
 // recovered gradient at half-grid node J: running mean over the tets holding J in ascending id
 __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int64_t J[3], double out[3]) {
@@ -540,7 +557,7 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
         }
     }
     if (live && nf < 0 && j == 0) atomicOr(a.err, 2);
-    if (live && nf == 0) {
+    if (live && nf == 0 && !a.pts) {
         for (int e = j; e < LOC2_HAT; e += 16) L[loc2(e, c)] = 0.;
         if (j == 0) a.flags[c] = 0;
     }
@@ -728,7 +745,14 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
 #pragma unroll
                 for (int i = 0; i < 10; ++i) s.H[j][i] = h[i];
                 double fv = 0.;
-                if (a.rhs_kind != DHIP_RHS_NONE) {
+                if (a.pts) {
+                    double y[3];
+                    b2w(s.X, bxb, y);
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) a.pts[3 * ((int64_t)a.off[c] + j) + cc] = y[cc];
+                } else if (a.rhs_kind == DHIP_RHS_VALUES) {
+                    fv = a.vals[(int64_t)a.off[c] + j];
+                } else if (a.rhs_kind != DHIP_RHS_NONE) {
                     double y[3];
                     b2w(s.X, bxb, y);
                     fv = rhs_fun(a.rhs_kind, a.rhs_p0, y);
@@ -741,6 +765,7 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
         if (lane == 0) atomicOr(a.err, 4);
         return;
     }
+    if (a.pts) return;   // points mode: the base points only
     // ---- contractions in the composite order: per tet 55 pairs (i >= j) and 10 load entries ----
     for (int task = lane; task < QQ_TPW * 65; task += 64) {
         const int tt = task / 65, r = task - 65 * (task / 65);

@@ -42,7 +42,7 @@ EXPORTS = [
     "dhip_vec_write", "dhip_vec_format", "dhip_vec_read", "dhip_vec_parse", "dhip_vec_count",
     "dhip_assemble_p1_band", "dhip_pcg_noalloc", "dhip_ch_bdf_step", "dhip_ch_set_precision", "dhip_spmv_t",
     "dhip_quad_points", "dhip_rhs_values", "dhip_local_p2_quaqua", "dhip_assemble_p2_quaqua",
-    "dhip_p2_recovered_gradient",
+    "dhip_p2_recovered_gradient", "dhip_quaqua_points", "dhip_quaqua_values",
 ]
 MLP_NPARAM, MLP_NF = 9282, 12
 
@@ -135,6 +135,8 @@ def load(path=LIB_PATH):
     L.dhip_assemble_p2.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
     L.dhip_local_p2_quaqua.argtypes = [P, C.POINTER(QuaQuaParams), C.c_int32, D]
     L.dhip_p2_recovered_gradient.argtypes = [P, D, C.c_int32]
+    L.dhip_quaqua_points.argtypes = [P, C.POINTER(QuaQuaParams), I64, D, C.c_int32]
+    L.dhip_quaqua_values.argtypes = [P, D, C.c_int64, C.c_int32]
     L.dhip_assemble_p2_quaqua.argtypes = [P, C.c_int32, C.POINTER(QuaQuaParams), C.c_int32, D, C.c_double, D, I64,
                                           I64, I64]
     L.dhip_assemble_p1_band.argtypes = [P, C.c_int32, D, C.c_int32, D, C.c_double, D, I64, I64, I64]
@@ -759,6 +761,19 @@ class Context:
         g = np.zeros(3 * nn)
         self.check(self.L.dhip_p2_recovered_gradient(self.h, _p(g, C.c_double), HOST))
         return g.reshape(nn, 3)
+
+    def quaqua_points(self, params=None):
+        """base points (npoints, 3) of every mapped quadrature point of the cut tets (after classify())"""
+        P = params if params is not None else QuaQuaParams()
+        n = C.c_int64()
+        self.check(self.L.dhip_quaqua_points(self.h, C.byref(P), C.byref(n), None, HOST))
+        xyz = np.zeros((max(n.value, 1), 3))
+        self.check(self.L.dhip_quaqua_points(self.h, C.byref(P), C.byref(n), _p(xyz, C.c_double), HOST))
+        return xyz[:n.value]
+
+    def quaqua_values(self, values):
+        v = np.ascontiguousarray(values, np.float64)
+        self.check(self.L.dhip_quaqua_values(self.h, _p(v, C.c_double), v.size, HOST))
 
     def assemble_p2_quaqua(self, zero_policy=ZERO_THROW, params=None, rhs_kind=RHS_NONE, rhs_param=None,
                            omit_bound=-1.0, lc=(1.0, 1.0, 0.0)):

@@ -1423,9 +1423,18 @@ inline void accumulate(TetraAccumulatorTupleCL &t, const MultiGridCL &mg, unsign
         const double param = plan.rhs_param;
         const int rk = plan.rhs ? plan.rhs_kind : DHIP_RHS_NONE;
         const double lc[3] = {1., 1., 1.};
-        if (p2 && plan.quaqua && rk == DHIP_RHS_VALUES)
-            throw DROPSErrCL("accumulate: the mapped surface's load is f at the base points; give a DeviceFunctionCL");
-        if (rk == DHIP_RHS_VALUES) {
+        if (p2 && plan.quaqua && rk == DHIP_RHS_VALUES) {
+            // f at the base points of the mapped quadrature points (LocalVectorP2CL on InterfaceCommonDataP2CL),
+            // evaluated here, summed by the device pass
+            if (!plan.rhs_fun) throw DROPSErrCL("accumulate: no load function");
+            int64_t nc = 0, np = 0;
+            dev.check(dhip_classify(dev.ctx(), DHIP_ZERO_THROW, &nc));
+            dev.check(dhip_quaqua_points(dev.ctx(), plan.quaqua, &np, nullptr, DHIP_HOST));
+            std::vector<double> xyz((size_t)(3 * np) + 3), fv((size_t)np + 1);
+            if (np) dev.check(dhip_quaqua_points(dev.ctx(), plan.quaqua, &np, xyz.data(), DHIP_HOST));
+            for (int64_t q = 0; q < np; ++q) fv[(size_t)q] = plan.rhs_fun(&xyz[(size_t)(3 * q)], plan.rhs_time);
+            dev.check(dhip_quaqua_values(dev.ctx(), fv.data(), np, DHIP_HOST));
+        } else if (rk == DHIP_RHS_VALUES) {
             // f at the quadrature points of the patches of the cut tets (the points LocalVectorP1CL /
             // LocalVectorP2CL evaluate it at), evaluated here, summed by the device pass
             if (plan.band) throw DROPSErrCL("accumulate: a load function on the narrow band is not supported");

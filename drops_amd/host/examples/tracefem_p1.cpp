@@ -223,6 +223,17 @@ static int quaqua(int n, const std::string &out) {
     double area = 0.;
     for (size_t k = 0; k < Mp2.Data.num_nonzeros(); ++k) area += Mp2.Data.raw_val()[k];
     std::printf("nnz: %zu\nmapped area - 4 pi: %.6e\n", Mp2.Data.num_nonzeros(), area - 4.0 * std::acos(-1.0));
+    // the same load from a host function, evaluated at the base points the library lists
+    VecDescCL bu(&ifacep2idx);
+    TetraAccumulatorTupleCL accu;
+    accu.push_back(&cdatap2);
+    InstatScalarFunction fu = [](const double *x, double) { return 3.0 * x[0] * x[1] - x[0] * x[0] * x[0]; };
+    InterfaceVectorAccuCL<LocalVectorP2CL, InterfaceCommonDataP2CL> accuu(&bu, LocalVectorP2CL(fu, 0.), cdatap2);
+    accu.push_back(&accuu);
+    accumulate(accu, mg, 0, &lsbnd);
+    double du = 0.;
+    for (size_t i = 0; i < bu.Data.size(); ++i) du = std::max(du, std::fabs(bu.Data[i] - bp2.Data[i]));
+    std::printf("user rhs vs device rhs: %.3e\n", du);
     if (!out.empty()) {
         std::vector<int64_t> rp(Mp2.Data.num_rows() + 1), ci(Mp2.Data.num_nonzeros());
         for (size_t i = 0; i <= Mp2.Data.num_rows(); ++i) rp[i] = (int64_t)Mp2.Data.raw_row()[i];

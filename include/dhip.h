@@ -226,9 +226,18 @@ typedef struct {
  * (num/oswald_projection.h:102-125, a running mean in tet order); grad: 3 doubles per node, nodes in
  * the level set's order. */
 int dhip_p2_recovered_gradient(dhip_ctx_t *ctx, double *grad, int32_t where);
+/* A load function evaluated by the caller (LocalVectorP2CL with an InstatScalarFunction,
+ * ifacetransp.h:2302-2311: f at the base points): dhip_quaqua_points (after dhip_classify) lists the
+ * base points of every mapped quadrature point, the cut tets in order and each tet's points in the
+ * composite order (7 per facet of its lattice-1 patch); *npoints = their number, xyz (may be NULL)
+ * receives 3 coordinates each.  dhip_quaqua_values hands f at those points back (npoints values);
+ * an assembly with DHIP_RHS_VALUES on the same level set and parameters then uses them. */
+int dhip_quaqua_points(dhip_ctx_t *ctx, const dhip_quaqua_params_t *params, int64_t *npoints, double *xyz,
+                       int32_t where);
+int dhip_quaqua_values(dhip_ctx_t *ctx, const double *values, int64_t npoints, int32_t where);
 /* The mapped mass, Laplace-Beltrami and load of every cut tet into the local slots the P2 CSR build
  * reads (after dhip_number_p2if; dhip_build_p2 follows).  params NULL: the values above.  rhs_kind
- * DHIP_RHS_NONE, _3XY_MINUS_X3 or _CONST (values at the planar points do not apply). */
+ * DHIP_RHS_NONE, _3XY_MINUS_X3, _CONST or _VALUES (given by dhip_quaqua_values). */
 int dhip_local_p2_quaqua(dhip_ctx_t *ctx, const dhip_quaqua_params_t *params, int32_t rhs_kind,
                          const double *rhs_param);
 /* classify + local_p2 (for the numbering) + number_p2if + local_p2_quaqua + build_p2 in one call:

@@ -7,8 +7,9 @@ Newton iterations, QR factorisations and sums in the same order without FMA cont
 cube root of |det| is exact (mesh width a power of two); elsewhere within 1e-12 relative, since cbrt may round
 differently there and it only enters the Newton damping length.  At 128^3 (the surface-solve size): the mapped
 sphere's mass sums to 4 pi (to what the Newton tolerance allows), the Laplace-Beltrami rows to zero, and
-the load of f = 1 is M 1.  Calls out of
-order, planar-point values and bad parameters are refused."""
+the load of f = 1 is M 1.  A load given as values at the base points the library lists
+(dhip_quaqua_points / dhip_quaqua_values) equals the device kind's bit for bit, is linear in f, and is
+refused when stale.  Calls out of order and bad parameters are refused."""
 import numpy as np
 import pytest
 
@@ -98,8 +99,11 @@ def test_mapped_calls_refused(ctx):
     with pytest.raises(D.DhipError) as ei:
         ctx.check(ctx.L.dhip_local_p2_quaqua(ctx.h, None, D.RHS_NONE, None))
     assert ei.value.code == D.E_STATE
-    with pytest.raises(D.DhipError) as ei:
+    with pytest.raises(D.DhipError) as ei:   # values before any base points
         ctx.assemble_p2_quaqua(rhs_kind=D.RHS_VALUES)
+    assert ei.value.code == D.E_STATE
+    with pytest.raises(D.DhipError) as ei:
+        ctx.assemble_p2_quaqua(rhs_kind=99)
     assert ei.value.code == D.E_ARG
     with pytest.raises(D.DhipError) as ei:
         ctx.assemble_p2_quaqua(params=D.QuaQuaParams(maxiter=-1))
@@ -107,6 +111,69 @@ def test_mapped_calls_refused(ctx):
     # a fresh, valid call after the refusals
     ncut, ndof, nnz = ctx.assemble_p2_quaqua()
     assert ndof > 0 and nnz > 0
+
+
+def f_kind(xyz):
+    """f = 3xy - x^3 with the device's evaluation order: (3 x) y - (x x) x"""
+    x, y = xyz[:, 0], xyz[:, 1]
+    return (3.0 * x) * y - (x * x) * x
+
+
+@pytest.mark.parametrize("case,n", [(cases.c1_sphere, 24), (cases.torus_case, 16)])
+def test_mapped_load_from_values(ctx, case, n):
+    mesh, phi = case(n)
+    print(f"[gpu quaqua] values n={n}", flush=True)
+    P = D.QuaQuaParams()
+    run(ctx, mesh, phi, params=P)
+    ref = ctx.csr()["rhs"].copy()
+    run(ctx, mesh, phi, rhs=D.RHS_CONST, rhs_param=[2.5], params=P)
+    cref = ctx.csr()["rhs"].copy()
+    flags = ctx.local_p2()[1]
+    ctx.classify()
+    xyz = ctx.quaqua_points(P)
+    # 7 points per facet of each cut tet's lattice-1 patch, in the cut tets' order
+    assert xyz.shape[0] > 0 and xyz.shape[0] % 7 == 0
+    assert (flags & 1).sum() <= xyz.shape[0] // 7 <= 2 * (flags & 1).sum()
+    if case is cases.c1_sphere:   # the base points lie on the sphere, to the Newton tolerance
+        assert np.abs(np.linalg.norm(xyz, axis=1) - 1.0).max() <= 1e-7
+    # the points do not depend on the call: a second listing is identical
+    assert np.array_equal(ctx.quaqua_points(P), xyz)
+    ctx.quaqua_values(f_kind(xyz))
+    ctx.assemble_p2_quaqua(params=P, rhs_kind=D.RHS_VALUES, lc=LC)
+    assert np.array_equal(ctx.csr()["rhs"], ref)
+    ctx.quaqua_values(np.full(xyz.shape[0], 2.5))
+    ctx.assemble_p2_quaqua(params=P, rhs_kind=D.RHS_VALUES, lc=LC)
+    assert np.array_equal(ctx.csr()["rhs"], cref)
+    g1, g2 = np.sin(xyz[:, 0]) * xyz[:, 2], np.cos(3.0 * xyz[:, 1])
+    b = []
+    for g in (g1, g2, g1 + g2):
+        ctx.quaqua_values(g)
+        ctx.assemble_p2_quaqua(params=P, rhs_kind=D.RHS_VALUES, lc=LC)
+        b.append(ctx.csr()["rhs"].copy())
+    assert np.abs(b[2] - (b[0] + b[1])).max() <= 1e-14 * (np.abs(b[0]).max() + np.abs(b[1]).max())
+
+
+def test_mapped_values_refused_when_stale(ctx):
+    mesh, phi = cases.c1_sphere(16)
+    print("[gpu quaqua] values state checks", flush=True)
+    ctx.mesh(mesh)
+    ctx.levelset(phi)
+    ctx.classify()
+    xyz = ctx.quaqua_points()
+    with pytest.raises(D.DhipError) as ei:
+        ctx.quaqua_values(np.zeros(xyz.shape[0] + 1))
+    assert ei.value.code == D.E_ARG
+    ctx.quaqua_values(f_kind(xyz))
+    with pytest.raises(D.DhipError) as ei:   # other Newton parameters: other base points
+        ctx.assemble_p2_quaqua(params=D.QuaQuaParams(tol=1e-12), rhs_kind=D.RHS_VALUES)
+    assert ei.value.code == D.E_STATE
+    ctx.levelset(phi)          # a new level set: the values no longer belong to the cut tets
+    with pytest.raises(D.DhipError) as ei:
+        ctx.assemble_p2_quaqua(rhs_kind=D.RHS_VALUES)
+    assert ei.value.code == D.E_STATE
+    with pytest.raises(D.DhipError) as ei:
+        ctx.quaqua_values(f_kind(xyz))
+    assert ei.value.code == D.E_STATE
 
 
 @pytest.mark.parametrize("case,n", [(cases.c1_sphere, 12), (cases.torus_case, 16)])
@@ -143,3 +210,5 @@ def test_host_mirror_mapped_forms(tmp_path):
         assert np.array_equal(rd(f"{k}.bin", np.float64), ref[k]), k
     assert np.array_equal(rd("grad.bin", np.float64), grad.reshape(-1))
     assert abs(float(r.stdout.split("mapped area - 4 pi:")[1].split()[0])) < 3e-6
+    # LocalVectorP2CL with a host function: f at the listed base points, the device kind's load bit for bit
+    assert float(r.stdout.split("user rhs vs device rhs:")[1].split()[0]) == 0.0
