@@ -16,11 +16,12 @@
 //                 belongs to the first (tet, local node) in mesh order whose squared
 //                 basis function has a large enough interface integral.
 //   k_rowcount_p2 / k_rowfill_p2
-//                 one wave per DOF row: walks the cut tets that contain the row node
-//                 in ascending tet id; lanes 0-9 scatter that tet's row into 125
-//                 neighbour slots in LDS (the 5x5x5 half-grid block around the node),
-//                 so each entry is summed in the order of the reference's row map;
-//                 then the slots are ranked by column and written out.
+//                 one wave per DOF row: lanes list the cut tets that contain the row
+//                 node (ascending tet id) at once, load six tets' rows together and
+//                 sum them tet by tet into 125 neighbour slots in LDS (the 5x5x5
+//                 half-grid block around the node), so each entry is summed in the
+//                 order of the reference's row map; then the present slots are
+//                 compacted, ranked by column and written out.
 // Compiled with -ffp-contract=off like the P1 path: bit-identical to the oracle.
 #pragma once
 
@@ -462,6 +463,9 @@ struct RowP2Shared {
     double am[128], al[128], as[128];
     int32_t cols[128];
     int32_t pres[128];
+    int32_t ccol[128];   // the present columns, compacted in slot order
+    int32_t tci[48];     // the row node's flagged cut tets, in for_cut_tets_at_node's order
+    int32_t tinf[48];    //   k | li << 3 | rel << 8
 };
 
 __device__ __forceinline__ void wave_sync_lds() {
@@ -469,6 +473,10 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// One row.  Lanes 0-47 look at the (cell, Kuhn tet) candidates around the node at once, lane order being
+// for_cut_tets_at_node's walk order (cells z, y, x ascending, then k), and list the flagged cut tets holding
+// the node.  Six tets at a time, lanes 10 t + j load node j's DOF and local entries of tet t together; the
+// sums into the 125 neighbour slots then go tet by tet, so each slot is summed in walk order.
 template <bool FILL>
 __device__ inline void p2_row(const RowP2Args &a, RowP2Shared &sh, int64_t r, int lane) {
     for (int s = lane; s < 128; s += 64) {
@@ -476,46 +484,111 @@ __device__ inline void p2_row(const RowP2Args &a, RowP2Shared &sh, int64_t r, in
         sh.cols[s] = 0;
         sh.am[s] = sh.al[s] = sh.as[s] = 0.0;
     }
-    wave_sync_lds();
     const int32_t own = a.rowowner[r];
     int64_t J[3];
     p2_node_of(a.m, a.cut[own / 10], own % 10, J);
-    double rh = 0.0;
-    for_cut_tets_at_node(a.m, a.cellinfo, J, [&](int64_t ci, int k, int li, int rel) {
-        if (!(a.flags[ci] & 1)) return false;
-        if (lane < 10) {
-            const int32_t dj = a.dofs[ci * 10 + lane];
-            if (dj >= 0) {
-                const int nj = c_tab.p2_node27[k][lane];
-                const int sl = (nj % 3 - rel % 3 + 2) + 5 * ((nj / 3) % 3 - (rel / 3) % 3 + 2) +
-                               25 * (nj / 9 - rel / 9 + 2);
-                sh.pres[sl] = 1;
-                if (FILL) {
-                    const int e = li * 10 + lane;
-                    sh.cols[sl] = dj;
-                    sh.am[sl] += a.loc[loc2(LOC2_MASS + e, ci)];
-                    sh.al[sl] += a.loc[loc2(LOC2_LB + e, ci)];
-                    sh.as[sl] += a.loc[loc2(LOC2_STAB + e, ci)];
+    int32_t tci = -1;
+    int tinf = 0;
+    if (lane < 48) {
+        const int q = lane / 6, k = lane - 6 * q;
+        int64_t cc[3];
+        bool ok = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int64_t lo = (J[d] & 1) ? (J[d] >> 1) : (J[d] >> 1) - 1, hi = J[d] >> 1;
+            if (lo < 0) lo = 0;
+            if (hi > a.m.n[d] - 1) hi = a.m.n[d] - 1;
+            cc[d] = lo + ((q >> d) & 1);
+            ok = ok && cc[d] <= hi;
+        }
+        if (ok) {
+            const uint32_t info = a.cellinfo[(cc[2] * a.m.n[1] + cc[1]) * a.m.n[0] + cc[0]];
+            const uint32_t mask = info & 63u;
+            if ((mask >> k) & 1u) {
+                const int rel = (int)((J[0] - 2 * cc[0]) + 3 * (J[1] - 2 * cc[1]) + 9 * (J[2] - 2 * cc[2]));
+                const int li = c_tab.p2_of27[k][rel];
+                if (li >= 0) {
+                    const int32_t t = (int32_t)((info >> 6) + __popc(mask & ((1u << k) - 1u)));
+                    if (a.flags[t] & 1) {
+                        tci = t;
+                        tinf = k | (li << 3) | (rel << 8);
+                    }
                 }
             }
         }
-        if (FILL) rh += a.loc[loc2(LOC2_RHS + li, ci)];
-        wave_sync_lds();
-        return false;
-    });
+    }
+    const uint64_t bal = __ballot(tci >= 0);
+    const int ntet = __popcll(bal);
+    if (tci >= 0) {
+        const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+        sh.tci[pos] = tci;
+        sh.tinf[pos] = tinf;
+    }
     wave_sync_lds();
+    double rv = 0.0;
+    if (FILL && lane < ntet) rv = a.loc[loc2(LOC2_RHS + ((sh.tinf[lane] >> 3) & 15), sh.tci[lane])];
+    const int tl = lane / 10, j = lane - 10 * tl;
+    for (int b = 0; b < ntet; b += 6) {
+        const int t = b + tl;
+        const bool act = lane < 60 && t < ntet;
+        int32_t dj = -1;
+        int sl = 0;
+        double vm = 0.0, vl = 0.0, vs = 0.0;
+        if (act) {
+            const int32_t c = sh.tci[t];
+            const int inf = sh.tinf[t];
+            const int k = inf & 7, li = (inf >> 3) & 15, rel = inf >> 8;
+            dj = a.dofs[(int64_t)c * 10 + j];
+            if (FILL) {
+                const int e = li * 10 + j;
+                vm = a.loc[loc2(LOC2_MASS + e, c)];
+                vl = a.loc[loc2(LOC2_LB + e, c)];
+                vs = a.loc[loc2(LOC2_STAB + e, c)];
+            }
+            const int nj = c_tab.p2_node27[k][j];
+            sl = (nj % 3 - rel % 3 + 2) + 5 * ((nj / 3) % 3 - (rel / 3) % 3 + 2) + 25 * (nj / 9 - rel / 9 + 2);
+        }
+        const int nb = ntet - b < 6 ? ntet - b : 6;
+        for (int u = 0; u < nb; ++u) {
+            if (act && tl == u && dj >= 0) {
+                sh.pres[sl] = 1;
+                if (FILL) {
+                    sh.cols[sl] = dj;
+                    sh.am[sl] += vm;
+                    sh.al[sl] += vl;
+                    sh.as[sl] += vs;
+                }
+            }
+            wave_sync_lds();
+        }
+    }
+    const bool p0 = sh.pres[lane] != 0, p1 = sh.pres[lane + 64] != 0;
+    const uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+    const int cnt = __popcll(m0) + __popcll(m1);
     if (!FILL) {
-        const int cnt = __popcll(__ballot(sh.pres[lane] != 0)) + __popcll(__ballot(sh.pres[lane + 64] != 0));
         if (lane == 0) a.rowlen[r] = cnt;
         return;
     }
+    double rh = 0.0;
+    for (int t = 0; t < ntet; ++t) rh += __shfl(rv, t);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const int32_t c0 = sh.cols[lane], c1 = sh.cols[lane + 64];
+    if (p0) sh.ccol[__popcll(m0 & lt)] = c0;
+    if (p1) sh.ccol[__popcll(m0) + __popcll(m1 & lt)] = c1;
+    wave_sync_lds();
+    int rk0 = 0, rk1 = 0;
+    for (int u = 0; u < cnt; ++u) {
+        const int32_t cu = sh.ccol[u];
+        rk0 += cu < c0 ? 1 : 0;
+        rk1 += cu < c1 ? 1 : 0;
+    }
     const int64_t base = a.rowptr[r];
-    for (int s = lane; s < 125; s += 64) {
-        if (!sh.pres[s]) continue;
-        const int32_t cs = sh.cols[s];
-        int rank = 0;
-        for (int u = 0; u < 125; ++u) rank += (sh.pres[u] && sh.cols[u] < cs) ? 1 : 0;
-        const int64_t dst = base + rank;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (!(h ? p1 : p0)) continue;
+        const int s = lane + 64 * h;
+        const int32_t cs = h ? c1 : c0;
+        const int64_t dst = base + (h ? rk1 : rk0);
         a.col[dst] = cs;
         a.vm[dst] = sh.am[s];
         a.vl[dst] = sh.al[s];
