@@ -863,7 +863,7 @@ static int hat_p2_impl(dhip_ctx *c) {
     DHIP_HIPCHK(c, c->flags.ensure(N + 1));
     LocalP2Args la{c->mesh, c->phi, c->cut.as<int64_t>(), N, 1., 1., 1., DHIP_RHS_NONE, 0.0, c->loc.as<double>(),
                    c->flags.as<uint8_t>(), c->errflag.as<int32_t>(), nullptr, nullptr};
-    if (N > 0) DHIP_LAUNCH(c, "hat_p2", k_hat_p2, (unsigned)N, 64, 0, la);
+    if (N > 0) DHIP_LAUNCH(c, "hat_p2", k_hat_p2, (unsigned)((N + HAT_TPW - 1) / HAT_TPW), 64, 0, la);
     int rc = check_err_flag(c);
     if (rc) return rc;
     c->stage = 2;
@@ -962,8 +962,8 @@ static int qq_node_gradients(dhip_ctx *c) {
         DHIP_LAUNCH(c, "zero_count", k_zero_u32, 1, 64, 0, reinterpret_cast<uint32_t *>(cnt), 1);
         QQGradArgs ga{c->mesh, c->phi, c->cut.as<int64_t>(), N, c->cellinfo.as<uint32_t>(), c->qq_first.as<int32_t>(),
                       c->qq_list.as<int32_t>(), cnt, c->qq_gh.as<double>()};
-        DHIP_LAUNCH(c, "first_qq", k_qq_first, blocks_for(10 * N, 128), 128, 0, ga);
-        DHIP_LAUNCH(c, "node_gradient_qq", k_qq_node_gradient, blocks_for(10 * N, 128), 128, 0, ga);
+        DHIP_LAUNCH(c, "first_qq", k_qq_first, blocks_for(10 * N, QQ_FIRST_THREADS), QQ_FIRST_THREADS, 0, ga);
+        DHIP_LAUNCH(c, "node_gradient_qq", k_qq_node_gradient, QQN_BLOCKS, 64, 0, ga);
     }
     return DHIP_OK;
 }

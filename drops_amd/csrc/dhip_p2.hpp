@@ -270,61 +270,69 @@ __global__ void __launch_bounds__(64) k_local_p2(LocalP2Args a) {
 
 // The numbering's part of k_local_p2 alone: the composite-Quad5 integrals of the squared basis functions
 // (the hat slots) and the non-empty flag, summed as k_local_p2 sums them.  For passes whose forms come
-// from elsewhere (the mapped surface).
+// from elsewhere (the mapped surface).  Four tets per wave, 16 lanes each: lane j < nf builds facet j, the
+// points go through LDS sixteen at a time and lanes j < 10 sum them in the composite order.
+constexpr int HAT_TPW = 4;
+
 struct HatShared {
     double vb[P2_MAXF][3][4];
     double af[P2_MAXF];
-    double h[P2_CHUNK][10];
-    double w[P2_CHUNK];
+    double h[16][10];
+    double w[16];
+    int nq;
 };
 
 __global__ void __launch_bounds__(64) k_hat_p2(LocalP2Args a) {
-    __shared__ HatShared sh;
-    const int64_t c = blockIdx.x;
-    const int lane = threadIdx.x;
-    if (c >= a.ncut) return;
-    double X[4][3], ls[10];
-    load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
-    const int nf = p2_patch_facets(sh, X, ls, lane);
-    if (nf < 0) {
-        if (lane == 0) atomicOr(a.err, 2);
-        return;
-    }
+    __shared__ HatShared sh[HAT_TPW];
+    const int lane = threadIdx.x, t = lane >> 4, j = lane & 15;
+    const int64_t c = (int64_t)blockIdx.x * HAT_TPW + t;
+    HatShared &s = sh[t];
     double *L = a.loc;
-    if (nf == 0) {
-        if (lane < 10) L[loc2(LOC2_HAT + lane, c)] = 0.;
-        if (lane == 0) a.flags[c] = 0;
-        return;
+    int nf = 0;
+    if (c < a.ncut) {
+        double X[4][3], ls[10];
+        load_tet_geom(a.m, a.phi, a.cut[c], X, ls);
+        nf = p2_patch_facets(s, X, ls, j);
+        if (nf < 0) {
+            if (j == 0) atomicOr(a.err, 2);
+        } else if (nf == 0) {
+            if (j < 10) L[loc2(LOC2_HAT + j, c)] = 0.;
+            if (j == 0) a.flags[c] = 0;
+        }
     }
+    const int nq = nf > 0 ? 7 * nf : 0;
+    if (j == 0) s.nq = nq;
     __syncthreads();
+    const int mq = max(max(sh[0].nq, sh[1].nq), max(sh[2].nq, sh[3].nq));
     double hat = 0.;
-    const int nq = 7 * nf;
-    for (int base = 0; base < nq; base += P2_CHUNK) {
-        const int q = base + lane;
+    for (int base = 0; base < mq; base += 16) {
+        const int q = base + j;
         if (q < nq) {
             const int f = q / 7, i = q - 7 * (q / 7);
             double nb[4];
 #pragma unroll
             for (int cc = 0; cc < 4; ++cc) {
-                double sv = sh.vb[f][0][cc] * c_tab.q5n[i][0];
-                sv += sh.vb[f][1][cc] * c_tab.q5n[i][1];
-                sv += sh.vb[f][2][cc] * c_tab.q5n[i][2];
+                double sv = s.vb[f][0][cc] * c_tab.q5n[i][0];
+                sv += s.vb[f][1][cc] * c_tab.q5n[i][1];
+                sv += s.vb[f][2][cc] * c_tab.q5n[i][2];
                 nb[cc] = sv;
             }
-            sh.w[lane] = sh.af[f] * c_tab.q5w[i];
+            s.w[j] = s.af[f] * c_tab.q5w[i];
             double h[10];
             p2_basis(nb, h);
 #pragma unroll
-            for (int d = 0; d < 10; ++d) sh.h[lane][d] = h[d];
+            for (int d = 0; d < 10; ++d) s.h[j][d] = h[d];
         }
         __syncthreads();
-        const int cnt = nq - base < P2_CHUNK ? nq - base : P2_CHUNK;
-        if (lane < 10)
-            for (int k = 0; k < cnt; ++k) hat += sh.w[k] * (sh.h[k][lane] * sh.h[k][lane]);
+        const int cnt = nq - base < 16 ? nq - base : 16;
+        if (j < 10)
+            for (int k = 0; k < cnt; ++k) hat += s.w[k] * (s.h[k][j] * s.h[k][j]);
         __syncthreads();
     }
-    if (lane < 10) L[loc2(LOC2_HAT + lane, c)] = hat;
-    if (lane == 0) a.flags[c] = 1;
+    if (nq > 0) {
+        if (j < 10) L[loc2(LOC2_HAT + j, c)] = hat;
+        if (j == 0) a.flags[c] = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------

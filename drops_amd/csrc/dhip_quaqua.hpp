@@ -405,16 +405,77 @@ __global__ void __launch_bounds__(128) k_qq_count(MeshP m, const double *phi, co
     cnt[c] = zero ? 0 : 7 * c_tab.rp_size[sg[0] + 3 * sg[1] + 9 * sg[2] + 27 * sg[3] + 40];
 }
 
-// recovered gradient at half-grid node J: running mean over the tets holding J in ascending id
-__device__ inline void qq_recovered(const MeshP &m, const double *phi, const int64_t J[3], double out[3]) {
-    int64_t lo[3], hi[3];
+// LocalP2GradientCL::set_tetra (num/fe.h): the P1 gradient field of the P2 level set at the vertices (summed
+// over the ten basis functions), at an edge node the mean of its two vertices; tet `cell * 6 + k`, P2 node dd
+__device__ inline void qq_tet_node_gradient(const MeshP &m, const double *phi, int64_t cell, int k, int dd,
+                                            double loc[3]) {
+    double X[4][3], ls[10], T[3][3];
+    load_tet_geom(m, phi, cell * 6 + k, X, ls);
+    qq_trafo(X, T);
+    double vv[10];
+#pragma unroll
+    for (int d = 0; d < 10; ++d) vv[d] = ls[c_tab.lat_of_p2[d]];
+    const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
+    const int va = dd < 4 ? dd : ea[dd - 4], vb = dd < 4 ? dd : eb[dd - 4];
+    double pa[3] = {0., 0., 0.}, pb[3] = {0., 0., 0.};
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+        double g[3];
+        qq_grad_at_vertex(T, i, va, g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pa[c] += vv[i] * g[c];
+        if (dd >= 4) {
+            qq_grad_at_vertex(T, i, vb, g);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) pb[c] += vv[i] * g[c];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) loc[c] = dd < 4 ? pa[c] : 0.5 * (pa[c] + pb[c]);
+}
+
+// the cells around half-grid node J (those whose closure holds it)
+__device__ __forceinline__ void qq_cell_range(const MeshP &m, const int64_t J[3], int64_t lo[3], int64_t hi[3]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         lo[c] = J[c] >= 2 ? (J[c] - 1) >> 1 : 0;
         hi[c] = J[c] >> 1;
         if (hi[c] > m.n[c] - 1) hi[c] = m.n[c] - 1;
     }
-    double n = 0., avg[3] = {0., 0., 0.};
+}
+
+// the running mean of averaging_P2_gradient_recovery: ((n-1)/n) avg + (1/n) val.  A half-grid node lies in at
+// most 24 tets; for those n the two quotients come from tables the compiler divides out (IEEE division is
+// correctly rounded, so they are the run-time quotients bit for bit)
+#define DHIP_QQ_W(n) ((n - 1.) / n)
+#define DHIP_QQ_R(n) (1. / n)
+__constant__ const double qq_mean_a[25] = {0., DHIP_QQ_W(1.), DHIP_QQ_W(2.), DHIP_QQ_W(3.), DHIP_QQ_W(4.),
+    DHIP_QQ_W(5.), DHIP_QQ_W(6.), DHIP_QQ_W(7.), DHIP_QQ_W(8.), DHIP_QQ_W(9.), DHIP_QQ_W(10.), DHIP_QQ_W(11.),
+    DHIP_QQ_W(12.), DHIP_QQ_W(13.), DHIP_QQ_W(14.), DHIP_QQ_W(15.), DHIP_QQ_W(16.), DHIP_QQ_W(17.),
+    DHIP_QQ_W(18.), DHIP_QQ_W(19.), DHIP_QQ_W(20.), DHIP_QQ_W(21.), DHIP_QQ_W(22.), DHIP_QQ_W(23.),
+    DHIP_QQ_W(24.)};
+__constant__ const double qq_mean_b[25] = {0., DHIP_QQ_R(1.), DHIP_QQ_R(2.), DHIP_QQ_R(3.), DHIP_QQ_R(4.),
+    DHIP_QQ_R(5.), DHIP_QQ_R(6.), DHIP_QQ_R(7.), DHIP_QQ_R(8.), DHIP_QQ_R(9.), DHIP_QQ_R(10.), DHIP_QQ_R(11.),
+    DHIP_QQ_R(12.), DHIP_QQ_R(13.), DHIP_QQ_R(14.), DHIP_QQ_R(15.), DHIP_QQ_R(16.), DHIP_QQ_R(17.),
+    DHIP_QQ_R(18.), DHIP_QQ_R(19.), DHIP_QQ_R(20.), DHIP_QQ_R(21.), DHIP_QQ_R(22.), DHIP_QQ_R(23.),
+    DHIP_QQ_R(24.)};
+#undef DHIP_QQ_W
+#undef DHIP_QQ_R
+
+__device__ __forceinline__ void qq_mean_step(int &n, double avg[3], const double val[3]) {
+    ++n;
+    const double dn = (double)n;
+    const double wa = n <= 24 ? qq_mean_a[n] : (dn - 1.) / dn, wb = n <= 24 ? qq_mean_b[n] : 1. / dn;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) avg[c] = wa * avg[c] + wb * val[c];
+}
+
+// recovered gradient at half-grid node J: running mean over the tets holding J in ascending id
+__device__ inline void qq_recovered(const MeshP &m, const double *phi, const int64_t J[3], double out[3]) {
+    int64_t lo[3], hi[3];
+    qq_cell_range(m, J, lo, hi);
+    int n = 0;
+    double avg[3] = {0., 0., 0.};
     for (int64_t i3 = lo[2]; i3 <= hi[2]; ++i3)
         for (int64_t i2 = lo[1]; i2 <= hi[1]; ++i2)
             for (int64_t i1 = lo[0]; i1 <= hi[0]; ++i1) {
@@ -423,35 +484,9 @@ __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int
                 for (int k = 0; k < 6; ++k) {
                     const int dd = c_tab.p2_of27[k][rel];
                     if (dd < 0) continue;
-                    double X[4][3], ls[10], T[3][3];
-                    load_tet_geom(m, phi, cell * 6 + k, X, ls);
-                    qq_trafo(X, T);
-                    double vv[10];
-#pragma unroll
-                    for (int d = 0; d < 10; ++d) vv[d] = ls[c_tab.lat_of_p2[d]];
-                    // LocalP2GradientCL::set_tetra: the P1 gradient field at the vertices (summed over the
-                    // ten basis functions), at an edge node the mean of its two vertices
-                    const int ea[6] = {0, 0, 1, 0, 1, 2}, eb[6] = {1, 2, 2, 3, 3, 3};
-                    const int va = dd < 4 ? dd : ea[dd - 4], vb = dd < 4 ? dd : eb[dd - 4];
-                    double pa[3] = {0., 0., 0.}, pb[3] = {0., 0., 0.};
-#pragma unroll
-                    for (int i = 0; i < 10; ++i) {
-                        double g[3];
-                        qq_grad_at_vertex(T, i, va, g);
-#pragma unroll
-                        for (int c = 0; c < 3; ++c) pa[c] += vv[i] * g[c];
-                        if (dd >= 4) {
-                            qq_grad_at_vertex(T, i, vb, g);
-#pragma unroll
-                            for (int c = 0; c < 3; ++c) pb[c] += vv[i] * g[c];
-                        }
-                    }
                     double loc[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) loc[c] = dd < 4 ? pa[c] : 0.5 * (pa[c] + pb[c]);
-                    n += 1.;
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) avg[c] = ((n - 1.) / n) * avg[c] + (1. / n) * loc[c];
+                    qq_tet_node_gradient(m, phi, cell, k, dd, loc);
+                    qq_mean_step(n, avg, loc);
                 }
             }
 #pragma unroll
@@ -460,8 +495,7 @@ __device__ inline void qq_recovered(const MeshP &m, const double *phi, const int
 
 // The recovered gradient of the P2 nodes of the cut tets, once per node.  k_qq_first: slot s = 10 c + d
 // (P2 dof d of cut tet c) finds the first cut tet in id order that holds its node and records it; the
-// slots that are their node's first append themselves to a list (one atomic per wave).  k_qq_node_gradient:
-// one thread per listed slot computes the node's running mean.
+// slots that are their node's first append themselves to a list (one atomic per block).
 struct QQGradArgs {
     MeshP m;
     const double *phi;
@@ -474,8 +508,12 @@ struct QQGradArgs {
     double *gh;
 };
 
-__global__ void __launch_bounds__(128) k_qq_first(QQGradArgs a) {
-    const int64_t s = (int64_t)blockIdx.x * 128 + threadIdx.x;
+constexpr int QQ_FIRST_THREADS = 1024;
+
+__global__ void __launch_bounds__(QQ_FIRST_THREADS) k_qq_first(QQGradArgs a) {
+    __shared__ int wbase[QQ_FIRST_THREADS / 64];
+    __shared__ int bbase;
+    const int64_t s = (int64_t)blockIdx.x * QQ_FIRST_THREADS + threadIdx.x;
     bool own = false;
     if (s < 10 * a.ncut) {
         int64_t J[3];
@@ -488,25 +526,119 @@ __global__ void __launch_bounds__(128) k_qq_first(QQGradArgs a) {
         a.first[s] = (int32_t)found;
         own = found == s;
     }
+    // one atomic per block: the owners of a block take consecutive list entries, wave by wave
     const uint64_t m = __ballot(own);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63, leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(a.count, __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (own) a.list[base + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)s;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wbase[w] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = 0;
+        for (int i = 0; i < QQ_FIRST_THREADS / 64; ++i) {
+            const int n = wbase[i];
+            wbase[i] = t;
+            t += n;
+        }
+        bbase = t ? atomicAdd(a.count, t) : 0;
+    }
+    __syncthreads();
+    if (own) a.list[bbase + wbase[w] + __popcll(m & ((1ull << lane) - 1ull))] = (int32_t)s;
 }
 
-__global__ void __launch_bounds__(128) k_qq_node_gradient(QQGradArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * 128 + threadIdx.x;
-    if (i >= *a.count) return;
-    const int64_t s = a.list[i];
-    int64_t J[3];
-    p2_node_of(a.m, a.cut[s / 10], (int)(s % 10), J);
-    double g[3];
-    qq_recovered(a.m, a.phi, J, g);
+// k_qq_node_gradient: a wave takes QQN_NODES listed nodes at a time and lists their (node, tet) pairs -- node
+// by node, each node's candidates (cell, Kuhn tet) in qq_recovered's walk order, cells z, y, x ascending, then
+// k -- compacted in LDS; lanes evaluate the pairs' tet values 64 at a time, then lane b forms node b's running
+// mean over its pairs in order.  A half-grid node lies in at most 24 tets (a mesh vertex: 6 in each of the two
+// cells whose diagonal it ends, 2 in each of the other six), so QQN_NODES * 24 pairs bound a batch.
+constexpr int QQN_NODES = 8;
+constexpr int QQN_MAXP = QQN_NODES * 24;
+constexpr int QQN_BLOCKS = 8192;   // one wave each, striding over the batches (the count is on the device)
+
+struct QQNodeShared {
+    int64_t J[QQN_NODES][3];
+    int64_t cell[QQN_MAXP];
+    int32_t kd[QQN_MAXP];   // k | dd << 3
+    double val[QQN_MAXP][3];
+};
+
+__global__ void __launch_bounds__(64) k_qq_node_gradient(QQGradArgs a) {
+    __shared__ QQNodeShared sh;
+    const int lane = threadIdx.x;
+    const int64_t cnt = *a.count;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int64_t i0 = (int64_t)blockIdx.x * QQN_NODES; i0 < cnt; i0 += (int64_t)gridDim.x * QQN_NODES) {
+        const int nn = cnt - i0 < QQN_NODES ? (int)(cnt - i0) : QQN_NODES;
+        int64_t s = 0;
+        if (lane < nn) {
+            s = a.list[i0 + lane];
+            int64_t J[3];
+            p2_node_of(a.m, a.cut[s / 10], (int)(s % 10), J);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) a.gh[3 * s + c] = g[c];
+            for (int c = 0; c < 3; ++c) sh.J[lane][c] = J[c];
+        }
+        wave_sync_lds();
+        int np = 0, start = 0;   // lane b <= nn: the first pair of node b
+#pragma unroll 1
+        for (int r = 0; r < QQN_NODES * 48 / 64; ++r) {
+            const int cand = r * 64 + lane, b = cand / 48, q6 = cand - 48 * b;
+            bool valid = false;
+            int64_t cell = 0;
+            int kd = 0;
+            if (b < nn) {
+                const int q = q6 / 6, k = q6 - 6 * q;
+                const int64_t J[3] = {sh.J[b][0], sh.J[b][1], sh.J[b][2]};
+                int64_t lo[3], hi[3], cc[3];
+                qq_cell_range(a.m, J, lo, hi);
+                bool ok = true;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    cc[d] = lo[d] + ((q >> d) & 1);
+                    ok = ok && cc[d] <= hi[d];
+                }
+                if (ok) {
+                    const int rel = (int)((J[2] - 2 * cc[2]) * 9 + (J[1] - 2 * cc[1]) * 3 + (J[0] - 2 * cc[0]));
+                    const int dd = c_tab.p2_of27[k][rel];
+                    if (dd >= 0) {
+                        valid = true;
+                        cell = (cc[2] * a.m.n[1] + cc[1]) * a.m.n[0] + cc[0];
+                        kd = k | (dd << 3);
+                    }
+                }
+            }
+            const uint64_t bal = __ballot(valid);
+            const int p = np + __popcll(bal & lt);
+            if (valid && p < QQN_MAXP) {
+                sh.cell[p] = cell;
+                sh.kd[p] = kd;
+            }
+            np += __popcll(bal);
+            const int lim = 48 * lane - 64 * r;   // this round's candidates of the nodes before node `lane`
+            const uint64_t low = lim <= 0 ? 0ull : lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
+            start += __popcll(bal & low);
+        }
+        if (np > QQN_MAXP) np = QQN_MAXP;
+        wave_sync_lds();
+#pragma unroll 1
+        for (int p = lane; p < np; p += 64) {
+            const int kd = sh.kd[p];
+            double loc[3];
+            qq_tet_node_gradient(a.m, a.phi, sh.cell[p], kd & 7, kd >> 3, loc);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) sh.val[p][c] = loc[c];
+        }
+        wave_sync_lds();
+        const int end = __shfl(start, (lane + 1) & 63, 64);
+        if (lane < nn) {
+            int n = 0;
+            double avg[3] = {0., 0., 0.};
+            for (int p = start; p < end && p < np; ++p) {
+                const double v[3] = {sh.val[p][0], sh.val[p][1], sh.val[p][2]};
+                qq_mean_step(n, avg, v);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) a.gh[3 * s + c] = avg[c];
+        }
+        wave_sync_lds();
+    }
 }
 
 // averaging_P2_gradient_recovery over every half-grid node: out[3 j + c]
