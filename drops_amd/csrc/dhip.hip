@@ -771,7 +771,7 @@ static int build_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
                c->dofs.as<int32_t>(), c->rowowner.as<int32_t>(), R, c->loc.as<double>(), c->rowlen.as<int32_t>(),
                c->rowptr.as<int32_t>(), nullptr, nullptr, nullptr, nullptr, nullptr, c->rhs.as<double>(),
                c->diag.as<double>(), lc ? lc[0] : 1.0, lc ? lc[1] : 1.0, lc ? lc[2] : 1.0};
-    if (R > 0) DHIP_LAUNCH(c, "rowcount", k_rowcount, blocks_for(R, 128), 128, 0, ra);
+    if (R > 0) DHIP_LAUNCH(c, "rowcount", k_rows_w<false>, blocks_for(R, ROWW_WAVES), 64 * ROWW_WAVES, 0, ra);
     int64_t total = 0;
     int rc = scan_exclusive_i32(c, c->rowlen.as<int32_t>(), c->rowptr.as<int32_t>(), R, &total);
     if (rc) return rc;
@@ -784,7 +784,7 @@ static int build_impl(dhip_ctx *c, const double lc[3], int64_t *nnz) {
     ra.vl = c->val[1].as<double>();
     ra.vs = c->val[2].as<double>();
     ra.vc = c->val[3].as<double>();
-    if (R > 0) DHIP_LAUNCH(c, "rowfill", k_rowfill, blocks_for(R, 128), 128, 0, ra);
+    if (R > 0) DHIP_LAUNCH(c, "rowfill", k_rows_w<true>, blocks_for(R, ROWW_WAVES), 64 * ROWW_WAVES, 0, ra);
     c->nnz = total;
     c->stage = 4;
     if (nnz) *nnz = total;

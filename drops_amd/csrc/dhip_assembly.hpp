@@ -14,11 +14,13 @@
 //                first-encounter interface numbering without atomics: a vertex
 //                belongs to the first (tet, local vertex) in mesh order whose
 //                phi_i^2 integral passes the limit; owners are ranked by a scan.
-//   k_rowcount / k_rowfill
-//                one lane per DOF row: walks the <= 24 incident tets of the row
-//                vertex in ascending tet id and sums local entries in that
-//                order (the order of the reference's hash-map +=), sorts the
-//                <= 15 columns and writes CSR rows, comb values and rhs.
+//   k_rows_w<false> / k_rows_w<true> (rowcount / rowfill)
+//                one wave per DOF row: lists the <= 24 incident tets of the row
+//                vertex in ascending tet id, loads their local entries together
+//                and sums them in that order (the order of the reference's
+//                hash-map +=), ranks the <= 15 columns and writes CSR rows, comb
+//                values and rhs.  rowcount_one / rowfill_one are the same rows
+//                for one lane each (the batched kernel's workgroup loops).
 // Compiled with -ffp-contract=off: every product and sum is rounded as in the
 // reference's evaluation order.
 #pragma once
@@ -795,12 +797,6 @@ __device__ inline void rowcount_one(const RowArgs &a, int64_t r) {
     a.rowlen[r] = __popc(present);
 }
 
-__global__ void __launch_bounds__(128) k_rowcount(RowArgs a) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.ndof) return;
-    rowcount_one(a, r);
-}
-
 __device__ inline void rowfill_one(const RowArgs &a, int64_t r) {
     int64_t x1, x2, x3;
     slot_vertex(a.m, a.cut, a.rowowner[r], x1, x2, x3);
@@ -857,12 +853,127 @@ __device__ inline void rowfill_one(const RowArgs &a, int64_t r) {
     a.rhs[r] = rh;
 }
 
-constexpr int ROW_BLOCK = 128;
+// One row per wave (rowcount_one / rowfill_one's rows with the tets' loads side by side): lanes 0-47 look at
+// the (cell, Kuhn tet) candidates of the row vertex at once, lane order being for_incident_cut_tets' walk
+// order (d = 7 .. 0, then k), and list the flagged cut tets; sixteen tets at a time, lane 4 t + j loads
+// column vertex j of tet t, and the sums into the 15 neighbour slots go tet by tet, so every slot is summed
+// in walk order as rowfill_one sums it.
+constexpr int ROWW_WAVES = 4;
 
-__global__ void __launch_bounds__(ROW_BLOCK) k_rowfill(RowArgs a) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+struct RowWShared {
+    double am[16], al[16], as[16];
+    int32_t cols[16];
+    int32_t pres[16];
+    int32_t tci[48];
+    int32_t tinf[48];   // k | li << 3
+};
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(64 * ROWW_WAVES) k_rows_w(RowArgs a) {
+    __shared__ RowWShared shm[ROWW_WAVES];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * ROWW_WAVES + w;
     if (r >= a.ndof) return;
-    rowfill_one(a, r);
+    RowWShared &sh = shm[w];
+    if (lane < 16) {
+        sh.pres[lane] = 0;
+        sh.cols[lane] = 0;
+        sh.am[lane] = sh.al[lane] = sh.as[lane] = 0.0;
+    }
+    int64_t x1, x2, x3;
+    slot_vertex(a.m, a.cut, a.rowowner[r], x1, x2, x3);
+    int32_t tci = -1;
+    int tinf = 0;
+    if (lane < 48) {
+        const int d = 7 - lane / 6, k = lane % 6;
+        const int64_t c1 = x1 - (d & 1), c2 = x2 - ((d >> 1) & 1), c3 = x3 - ((d >> 2) & 1);
+        const int li = c_tab.inc[d][k];
+        if (li >= 0 && c1 >= 0 && c2 >= 0 && c3 >= 0 && c1 < a.m.n[0] && c2 < a.m.n[1] && c3 < a.m.n[2]) {
+            const uint32_t info = a.cellinfo[(c3 * a.m.n[1] + c2) * a.m.n[0] + c1];
+            const uint32_t mask = info & 63u;
+            if ((mask >> k) & 1u) {
+                const int32_t t = (int32_t)((info >> 6) + __popc(mask & ((1u << k) - 1u)));
+                if (a.flags[t] & 1) {
+                    tci = t;
+                    tinf = k | (li << 3);
+                }
+            }
+        }
+    }
+    const uint64_t bal = __ballot(tci >= 0);
+    const int ntet = __popcll(bal);
+    if (tci >= 0) {
+        const int pos = __popcll(bal & ((1ull << lane) - 1ull));
+        sh.tci[pos] = tci;
+        sh.tinf[pos] = tinf;
+    }
+    wave_sync_lds();
+    const int64_t N = a.ncut;
+    double rv = 0.0;
+    if (FILL && lane < ntet) rv = a.loc[(int64_t)(LOC_RHS + (sh.tinf[lane] >> 3)) * N + sh.tci[lane]];
+    const int tl = lane >> 2, j = lane & 3;
+    for (int b = 0; b < ntet; b += 16) {
+        const int t = b + tl;
+        const bool act = t < ntet;
+        int32_t dj = -1;
+        int sl = 0;
+        double vm = 0.0, vl = 0.0, vs = 0.0;
+        if (act) {
+            const int32_t ci = sh.tci[t];
+            const int inf = sh.tinf[t], k = inf & 7, li = inf >> 3;
+            dj = a.dofs[(int64_t)ci * 4 + j];
+            sl = c_tab.slot[k][li][j];
+            if (FILL) {
+                const int e = li * 4 + j;
+                vm = a.loc[(int64_t)(LOC_MASS + e) * N + ci];
+                vl = a.loc[(int64_t)(LOC_LB + e) * N + ci];
+                vs = a.loc[(int64_t)(LOC_STAB + e) * N + ci];
+            }
+        }
+        if (!FILL) {
+            if (act && dj >= 0) sh.pres[sl] = 1;
+            continue;
+        }
+        const int nb = ntet - b < 16 ? ntet - b : 16;
+        for (int u = 0; u < nb; ++u) {
+            if (act && tl == u && dj >= 0) {
+                sh.pres[sl] = 1;
+                sh.cols[sl] = dj;
+                sh.am[sl] += vm;
+                sh.al[sl] += vl;
+                sh.as[sl] += vs;
+            }
+            wave_sync_lds();
+        }
+    }
+    wave_sync_lds();
+    const bool pr = lane < 15 && sh.pres[lane] != 0;
+    const uint64_t pm = __ballot(pr);
+    if (!FILL) {
+        if (lane == 0) a.rowlen[r] = __popcll(pm);
+        return;
+    }
+    double rh = 0.0;
+    for (int t = 0; t < ntet; ++t) rh += __shfl(rv, t, 64);
+    if (pr) {
+        const int32_t cs = sh.cols[lane];
+        int rank = 0;
+        for (int u = 0; u < 15; ++u) rank += (((pm >> u) & 1ull) && sh.cols[u] < cs) ? 1 : 0;
+        const int64_t dst = a.rowptr[r] + rank;
+        a.col[dst] = cs;
+        a.vm[dst] = sh.am[lane];
+        a.vl[dst] = sh.al[lane];
+        a.vs[dst] = sh.as[lane];
+        const double cv = 1.0 * (a.lc0 * sh.am[lane] + a.lc1 * sh.al[lane]) + a.lc2 * sh.as[lane];
+        a.vc[dst] = cv;
+        if (cs == r) a.diag[r] = cv;
+    }
+    if (lane == 0) a.rhs[r] = rh;
 }
 
 }  // namespace dhip

@@ -476,11 +476,6 @@ struct RowP2Shared {
     int32_t tinf[48];    //   k | li << 3 | rel << 8
 };
 
-__device__ __forceinline__ void wave_sync_lds() {
-    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // One row.  Lanes 0-47 look at the (cell, Kuhn tet) candidates around the node at once, lane order being
 // for_cut_tets_at_node's walk order (cells z, y, x ascending, then k), and list the flagged cut tets holding
 // the node.  Six tets at a time, lanes 10 t + j load node j's DOF and local entries of tet t together; the
