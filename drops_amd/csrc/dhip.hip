@@ -2153,6 +2153,7 @@ struct dhip_ch {
     DevBuf hist[3];
     int nhist = 0;
     DevBuf feat, pred, xstart;
+    DevBuf r3;   // the residuals of the network's and the extrapolation's candidates
     // P2 space: contribution lists of the entries and rows, row of each entry
     int space = 1;
     DevBuf eptr, econt, rptr, rcont, erow;
@@ -2934,8 +2935,16 @@ static int ch_learned_start(dhip_ch *h, dhip_mlp *q, double *x0, int32_t **flag_
     const double *h1 = hist[std::min(1, levels - 1)], *h2 = hist[std::min(2, levels - 1)];
     CombineArgs ca{xn, h1, h2, h->pred.as<float>(), q->scale.as<double>(), n, q->norm, x0, xq};
     DHIP_LAUNCH(c, "mlp_combine", k_mlp_combine, blocks_for(n, 256), 256, 0, ca);
-    DHIP_LAUNCH(c, "ch_three_resid", k_ch_three_resid, SCALE_BLOCKS, RES3_B, 0, n2, h->K.rowptr, h->K.col, h->K.val,
-                h->rhs.as<double>(), xn, x0, xq, part);
+    // the candidates' residuals: x_n's is `res`; the network's and the extrapolation's through the same
+    // tiled SpMV (row products summed in column order)
+    DHIP_HIPCHK(c, h->r3.ensure(sizeof(double) * (2 * n2 + 1)));
+    double *r1 = h->r3.as<double>(), *r2 = r1 + n2;
+    DHIP_LAUNCH(c, "ch_residual", k_spmv<true>, blocks_for(n2, rb), SPMV_T, 0, n2, rb, h->K.rowptr, h->K.col, h->K.val,
+                x0, h->rhs.as<double>(), r1);
+    DHIP_LAUNCH(c, "ch_residual", k_spmv<true>, blocks_for(n2, rb), SPMV_T, 0, n2, rb, h->K.rowptr, h->K.col, h->K.val,
+                xq, h->rhs.as<double>(), r2);
+    DHIP_LAUNCH(c, "ch_three_norms", k_ch_three_norms, SCALE_BLOCKS, RES3_B, 0, n2, h->rhs.as<double>(), res, r1, r2,
+                part);
     // the network's candidate is trusted while the extrapolation's residual exceeds 100 x tolerance
     const double gate = 100.0 * h->p.tol;
     DHIP_LAUNCH(c, "ch_pick", k_ch_pick, 1, 64, 0, part, SCALE_BLOCKS, gate, flag);

@@ -343,46 +343,19 @@ __global__ void __launch_bounds__(KB) k_ch_mnorm(int64_t n, const int32_t *rp, c
     }
 }
 
-// r = b - K x (one lane per row of the block system, row gathers batched)
-// |b - K x_k|^2 of three candidate starts and |b|^2 per block (block-order sums later)
+// |r_k|^2 of three residuals r_k = b - K x_k of candidate starts (each from k_spmv<true>: a row's
+// products summed in column order) and |b|^2, per block: thread t of block g takes rows
+// g * RES3_B + t + k * gridDim.x * RES3_B (block-order sums later)
 constexpr int RES3_B = 256;
-__global__ void __launch_bounds__(RES3_B) k_ch_three_resid(int64_t n2, const int32_t *rp, const int32_t *col,
-                                                           const double *val, const double *b, const double *x0,
-                                                           const double *x1, const double *x2, double *part) {
+__global__ void __launch_bounds__(RES3_B) k_ch_three_norms(int64_t n2, const double *b, const double *r0,
+                                                           const double *r1, const double *r2, double *part) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, sb = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * RES3_B + threadIdx.x; i < n2; i += (int64_t)gridDim.x * RES3_B) {
-        double p0 = 0.0, p1 = 0.0, p2 = 0.0;
-        const int32_t e0 = rp[i], e1 = rp[i + 1];
-        for (int32_t q0 = e0; q0 < e1; q0 += 8) {
-            const int len = e1 - q0 < 8 ? e1 - q0 : 8;
-            int32_t cc[8];
-            double vv[8], a0[8], a1[8], a2[8];
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < len) {
-                    cc[j] = col[q0 + j];
-                    vv[j] = val[q0 + j];
-                }
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < len) {
-                    a0[j] = x0[cc[j]];
-                    a1[j] = x1[cc[j]];
-                    a2[j] = x2[cc[j]];
-                }
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-                if (j < len) {
-                    p0 += vv[j] * a0[j];
-                    p1 += vv[j] * a1[j];
-                    p2 += vv[j] * a2[j];
-                }
-        }
-        const double r0 = b[i] - p0, r1 = b[i] - p1, r2 = b[i] - p2;
-        s0 += r0 * r0;
-        s1 += r1 * r1;
-        s2 += r2 * r2;
-        sb += b[i] * b[i];
+        const double v0 = r0[i], v1 = r1[i], v2 = r2[i], bi = b[i];
+        s0 += v0 * v0;
+        s1 += v1 * v1;
+        s2 += v2 * v2;
+        sb += bi * bi;
     }
     s0 = block_sum<RES3_B>(s0);
     s1 = block_sum<RES3_B>(s1);
