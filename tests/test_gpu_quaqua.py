@@ -9,7 +9,9 @@ differently there and it only enters the Newton damping length.  At 128^3 (the s
 sphere's mass sums to 4 pi (to what the Newton tolerance allows), the Laplace-Beltrami rows to zero, and
 the load of f = 1 is M 1.  A load given as values at the base points the library lists
 (dhip_quaqua_points / dhip_quaqua_values) equals the device kind's bit for bit, is linear in f, and is
-refused when stale.  Calls out of order and bad parameters are refused."""
+refused when stale.  Calls out of order, bad parameters and exact zeros of the level set at nodes of cut
+tets are refused.  Ragged, sheared and column bricks, a plane (whose mapped surface is itself) and a single
+cell match the oracle; an interface-free level set maps and assembles nothing."""
 import numpy as np
 import pytest
 
@@ -91,6 +93,58 @@ def test_mapped_sphere_at_surface_solve_size(ctx, tol, bar):
         assert abs(A - A.T).max() <= 1e-15 * abs(A).max(), k
 
 
+@pytest.mark.parametrize("name", ["ellipsoid_ragged", "skew_brick", "plane_x041", "sphere20"])
+def test_mapped_forms_other_geometries(ctx, name):
+    """Ragged brick (17x13x11), sheared parallelepiped brick, a plane through a 10x1x1 column, the unit-cube
+    sphere of tests/interfaceP1FE.cpp: the CSR pattern bit-exact against the oracle, every value within
+    1e-12 relative; the plane's mapped surface is the plane itself (area 1)."""
+    if name == "plane_x041":
+        mesh, phi, _ = cases.planes()
+    else:
+        mesh, phi = cases.PARITY_CASES[name]()
+    ncut, ndof, nnz = run(ctx, mesh, phi, rhs=D.RHS_CONST, rhs_param=[1.0])
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    grad = O.recovered_gradient(om, phi)
+    dof, nd = O.number_p2if(om, phi)
+    ref = O.assemble_p2qq(om, phi, grad, dof, nd, rhs_kind=O.RHS_CONST, rhs_param=[1.0])
+    g = ctx.csr()
+    assert ndof == nd and nnz == ref["colind"].size
+    assert np.array_equal(g["rowptr"].astype(np.int64), ref["rowptr"])
+    assert np.array_equal(g["colind"], ref["colind"])
+    for k in ("mass", "lb", "rhs"):
+        assert np.abs(g[k] - ref[k]).max() <= 1e-12 * np.abs(ref[k]).max(), k
+    if name == "plane_x041":
+        assert abs(g["mass"].sum() - 1.0) <= 1e-14
+        gr = ctx.recovered_gradient()   # the gradient of x - 0.41, up to rounding
+        assert np.abs(gr - np.array([1.0, 0.0, 0.0])).max() <= 1e-12
+
+
+def test_mapped_single_cell_and_empty(ctx):
+    """One brick cell with one negative corner (the oracle's numbering and matrices), and an interface-free
+    level set: nothing to map, nothing assembled, no base points."""
+    from drops_amd.mesh import KuhnMesh
+    mesh = KuhnMesh.box(0.0, 1.0, 1)
+    phi = np.ones(mesh.halfgrid_shape)
+    phi[0, 0, 2] = -1.0
+    ncut, ndof, nnz = run(ctx, mesh, phi, rhs=D.RHS_CONST, rhs_param=[1.0])
+    om = O.make_mesh(mesh.orig, mesh.e, mesh.n)
+    grad = O.recovered_gradient(om, phi)
+    dof, nd = O.number_p2if(om, phi)
+    ref = O.assemble_p2qq(om, phi, grad, dof, nd, rhs_kind=O.RHS_CONST, rhs_param=[1.0])
+    g = ctx.csr()
+    assert ncut == 2 and ndof == nd and nnz == ref["colind"].size
+    assert np.array_equal(g["colind"], ref["colind"])
+    for k in ("mass", "lb", "rhs"):
+        assert np.abs(g[k] - ref[k]).max() <= 1e-12 * max(np.abs(ref[k]).max(), 1e-300), k
+    phi[:] = 1.0
+    assert run(ctx, mesh, phi) == (0, 0, 0)
+    assert ctx.csr()["rowptr"].tolist() == [0]
+    ctx.classify()
+    assert ctx.quaqua_points().shape == (0, 3)
+    ctx.quaqua_values(np.zeros(0))
+    assert ctx.assemble_p2_quaqua(rhs_kind=D.RHS_VALUES, lc=LC) == (0, 0, 0)
+
+
 def test_mapped_calls_refused(ctx):
     mesh, phi = cases.c1_sphere(12)
     ctx.mesh(mesh)
@@ -111,6 +165,14 @@ def test_mapped_calls_refused(ctx):
     # a fresh, valid call after the refusals
     ncut, ndof, nnz = ctx.assemble_p2_quaqua()
     assert ndof > 0 and nnz > 0
+    # a level set exactly zero at nodes of cut tets (the plane x = 0.4 through a layer of vertices): the
+    # reference's numbering throws, and so does the mapped pass under DHIP_ZERO_THROW
+    pm, _, p04 = cases.planes()
+    ctx.mesh(pm)
+    ctx.levelset(p04)
+    with pytest.raises(D.DhipError) as ei:
+        ctx.assemble_p2_quaqua()
+    assert ei.value.code == D.E_ZERO_NODE
 
 
 def f_kind(xyz):
