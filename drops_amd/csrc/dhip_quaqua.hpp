@@ -660,6 +660,29 @@ __global__ void __launch_bounds__(128) k_recovered_gradient(RecGradArgs a) {
     for (int c = 0; c < 3; ++c) a.out[3 * j + c] = g[c];
 }
 
+// mass and Laplace-Beltrami sums of pair (pi, pj) over points k0 .. k0 + 6, added to am, al in point order
+__device__ __forceinline__ void qq_pair7(const QQShared &S, int k0, int pi, int pj, double &am, double &al) {
+    double w[7], ad[7], hj[7], hi[7], gj[7][3], gi[7][3];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        w[k] = S.w[k0 + k];
+        ad[k] = S.ad[k0 + k];
+        hj[k] = S.H[k0 + k][pj];
+        hi[k] = S.H[k0 + k][pi];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            gj[k][c] = S.Gq[k0 + k][pj][c];
+            gi[k][c] = S.Gq[k0 + k][pi][c];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        am += w[k] * ((ad[k] * hj[k]) * hi[k]);
+        const double dq = gj[k][0] * gi[k][0] + gj[k][1] * gi[k][1] + gj[k][2] * gi[k][2];
+        al += w[k] * (ad[k] * dq);
+    }
+}
+
 constexpr int QQ_TPW = 4;   // cut tets per wave (16 lanes each: a tet has at most 14 quadrature points)
 
 __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
@@ -909,12 +932,18 @@ __global__ void __launch_bounds__(64) k_local_p2qq(LocalP2QQArgs a) {
             int pi = 0, pj = 0;
             pair_of(r, pi, pj);
             double am = 0.0, al = 0.0;
-            for (int k = 0; k < nqt; ++k) {
-                const double w = S.w[k], ad = S.ad[k];
-                am += w * ((ad * S.H[k][pj]) * S.H[k][pi]);
-                const double dq = S.Gq[k][pj][0] * S.Gq[k][pi][0] + S.Gq[k][pj][1] * S.Gq[k][pi][1] +
-                                  S.Gq[k][pj][2] * S.Gq[k][pi][2];
-                al += w * (ad * dq);
+            if (nqt % 7 == 0) {
+                // a facet's seven points at a time: every LDS read of the chunk is issued before the
+                // first sum, which still runs point by point in the composite order
+                for (int k0 = 0; k0 < nqt; k0 += 7) qq_pair7(S, k0, pi, pj, am, al);
+            } else {
+                for (int k = 0; k < nqt; ++k) {
+                    const double w = S.w[k], ad = S.ad[k];
+                    am += w * ((ad * S.H[k][pj]) * S.H[k][pi]);
+                    const double dq = S.Gq[k][pj][0] * S.Gq[k][pi][0] + S.Gq[k][pj][1] * S.Gq[k][pi][1] +
+                                      S.Gq[k][pj][2] * S.Gq[k][pi][2];
+                    al += w * (ad * dq);
+                }
             }
             const int e1 = pi * 10 + pj, e2 = pj * 10 + pi;
             L[loc2(LOC2_MASS + e1, ct)] = am;
