@@ -877,7 +877,8 @@ template <bool FILL>
 __global__ void __launch_bounds__(64 * ROWW_WAVES) k_rows_w(RowArgs a) {
     __shared__ RowWShared shm[ROWW_WAVES];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * ROWW_WAVES + w;
+    // each XCD takes a contiguous run of rows, whose tets its L2 then shares
+    const int64_t r = (int64_t)xcd_position(blockIdx.x, gridDim.x) * ROWW_WAVES + w;
     if (r >= a.ndof) return;
     RowWShared &sh = shm[w];
     if (lane < 16) {

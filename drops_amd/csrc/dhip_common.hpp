@@ -17,6 +17,14 @@
 
 namespace dhip {
 
+// Blocks of a launch are dispatched to the 8 XCDs round robin (block b to XCD b % 8).  Position of
+// block b when each XCD's blocks are to own consecutive row ranges: the blocks of the XCDs before
+// its own, then b / 8.  A bijection on [0, nb) whatever the dispatch does.
+__device__ inline unsigned xcd_position(unsigned b, unsigned nb) {
+    const unsigned x = b % 8, q = nb / 8, rem = nb % 8;
+    return x * q + (x < rem ? x : rem) + b / 8;
+}
+
 // ---------------------------------------------------------------------------
 // Device tables (filled once per process from host-side derivations)
 // ---------------------------------------------------------------------------

@@ -342,13 +342,6 @@ __device__ inline bool sync_reduce(SyncSlot *slots, uint32_t g, double a0, doubl
 // two grid synchronisations: after q = A p (lambda) and after the vector
 // update (|r|^2, z.r).
 // ---------------------------------------------------------------------------
-// Blocks of a launch are dispatched to the 8 XCDs round robin (block b to XCD b % 8).  Position of
-// block b when each XCD's blocks are to own consecutive row ranges: the blocks of the XCDs before
-// its own, then b / 8.  A bijection on [0, nb) whatever the dispatch does.
-__device__ inline unsigned xcd_position(unsigned b, unsigned nb) {
-    const unsigned x = b % 8, q = nb / 8, rem = nb % 8;
-    return x * q + (x < rem ? x : rem) + b / 8;
-}
 
 // Row range of this block among gridDim.x equal ranges: range index blockIdx.x, or with xcd_map the
 // block's position above, so that the rows a block gathers from its neighbours were mostly written

@@ -607,7 +607,7 @@ template <bool FILL>
 __global__ void __launch_bounds__(64 * ROWP2_WAVES) k_rows_p2(RowP2Args a) {
     __shared__ RowP2Shared sh[ROWP2_WAVES];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * ROWP2_WAVES + w;
+    const int64_t r = (int64_t)xcd_position(blockIdx.x, gridDim.x) * ROWP2_WAVES + w;   // a run of rows per XCD
     if (r >= a.ndof) return;
     p2_row<FILL>(a, sh[w], r, lane);
 }
